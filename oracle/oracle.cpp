@@ -1455,8 +1455,7 @@ int oracle_solve(const pdehip_problem* pp, const pdehip_options* oo, double* u_o
         for (int j = 0; j < g.ny; ++j)
           for (int i = 0; i < g.nx; ++i)
             if (IsBoundary(g, i, j, k)) b[g.idx(i, j, k)] = x[g.idx(i, j, k)];
-      KrylovResult kr = MgPcg(H.L[0].op, H, b, x, std::max(1, o.max_iter), o.tol, o.tol_mode,
-                              o.precond != PDEHIP_PC_NONE);
+      KrylovResult kr = MgPcg(H.L[0].op, H, b, x, std::max(1, o.max_iter), o.tol, o.tol_mode, true);
       rep->iterations = kr.iters;
       rep->final_metric = kr.metric;
       rep->converged = kr.ok;

@@ -1,0 +1,191 @@
+// common.hpp — device-side definitions shared by every HIP translation unit of libpdehip.
+//
+// Grid layout (all levels, all precisions): full vertex-centred grid including boundary
+// nodes, x fastest, idx = (k*ny + j)*nx + i (reference cpu_utils.h:13-19).  Boundary rows are
+// identity; interior rows are a 5-point (2D) or 7-point (3D) stencil.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pdehip.h"
+
+namespace pdehip {
+
+constexpr int kWave = 64;          // CDNA wavefront
+constexpr int kBlock = 256;        // default workgroup (4 waves)
+constexpr int kMaxPartials = 4096; // stage-1 reduction partials per dot product
+
+// ------------------------------------------------------------------ operator descriptor
+// One level's operator, passed to kernels by value.
+struct DevOp {
+  int nx, ny, nz;        // nz == 1: 2D
+  int kind;              // pdehip_op_kind
+  long long sxy;         // nx*ny
+  long long n;           // nx*ny*nz
+  double w[7];           // CONST weights: C, W, E, S, N, B(k-1), F(k+1)
+  const double* pw;      // POINTWISE weights, structure of arrays [7][n]
+  const void* kappa;     // FLUX conductivity (element type = kernel's TK)
+  double fx, fy, fz;     // FLUX: a/h^2 per axis
+  double e;              // FLUX: constant reaction term
+  const double* e_field; // FLUX: per-node reaction term (or null)
+  double theta[6];       // folded closure factors (0 = not folded / Dirichlet)
+};
+
+template <typename T>
+struct W7 {
+  T C, W, E, S, N, B, F;
+};
+
+template <typename T> __device__ __forceinline__ T ld(const T* p, long long i) { return p[i]; }
+__device__ __forceinline__ float ldf(const __half* p, long long i) { return __half2float(p[i]); }
+__device__ __forceinline__ float ldf(const float* p, long long i) { return p[i]; }
+__device__ __forceinline__ double ldf(const double* p, long long i) { return p[i]; }
+
+template <typename T> struct Conv;
+template <> struct Conv<double> {
+  __device__ static double from(double v) { return v; }
+  __device__ static double from(float v) { return static_cast<double>(v); }
+  __device__ static double from(__half v) { return static_cast<double>(__half2float(v)); }
+};
+template <> struct Conv<float> {
+  __device__ static float from(double v) { return static_cast<float>(v); }
+  __device__ static float from(float v) { return v; }
+  __device__ static float from(__half v) { return __half2float(v); }
+};
+
+template <typename S> __device__ __forceinline__ S store_as(double v);
+template <> __device__ __forceinline__ double store_as<double>(double v) { return v; }
+template <> __device__ __forceinline__ float store_as<float>(double v) { return static_cast<float>(v); }
+template <> __device__ __forceinline__ __half store_as<__half>(double v) { return __float2half(static_cast<float>(v)); }
+template <typename S> __device__ __forceinline__ S store_f(float v);
+template <> __device__ __forceinline__ double store_f<double>(float v) { return v; }
+template <> __device__ __forceinline__ float store_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ __half store_f<__half>(float v) { return __float2half(v); }
+
+// Harmonic mean of two node conductivities, times the axis factor (face coefficient).
+template <typename T>
+__device__ __forceinline__ T face_coef(T kc, T kn, T f) {
+  const T s = kc + kn;
+  return s > T(0) ? f * (T(2) * kc * kn) / s : T(0);
+}
+
+// Weights of interior row (i,j,k) before folding.
+template <int DIM, int KIND, typename T, typename TK>
+__device__ __forceinline__ W7<T> row_weights(const DevOp& op, long long id) {
+  W7<T> w;
+  if constexpr (KIND == PDEHIP_OP_CONST) {
+    w.C = T(op.w[0]); w.W = T(op.w[1]); w.E = T(op.w[2]); w.S = T(op.w[3]); w.N = T(op.w[4]);
+    w.B = T(op.w[5]); w.F = T(op.w[6]);
+  } else if constexpr (KIND == PDEHIP_OP_POINTWISE) {
+    const double* p = op.pw;
+    const long long n = op.n;
+    w.C = T(p[id]); w.W = T(p[n + id]); w.E = T(p[2 * n + id]); w.S = T(p[3 * n + id]);
+    w.N = T(p[4 * n + id]);
+    if constexpr (DIM == 3) { w.B = T(p[5 * n + id]); w.F = T(p[6 * n + id]); }
+    else { w.B = T(0); w.F = T(0); }
+  } else {
+    const TK* k = static_cast<const TK*>(op.kappa);
+    const T kc = Conv<T>::from(k[id]);
+    const T fx = T(op.fx), fy = T(op.fy);
+    w.W = face_coef<T>(kc, Conv<T>::from(k[id - 1]), fx);
+    w.E = face_coef<T>(kc, Conv<T>::from(k[id + 1]), fx);
+    w.S = face_coef<T>(kc, Conv<T>::from(k[id - op.nx]), fy);
+    w.N = face_coef<T>(kc, Conv<T>::from(k[id + op.nx]), fy);
+    T sum = w.W + w.E + w.S + w.N;
+    if constexpr (DIM == 3) {
+      const T fz = T(op.fz);
+      w.B = face_coef<T>(kc, Conv<T>::from(k[id - op.sxy]), fz);
+      w.F = face_coef<T>(kc, Conv<T>::from(k[id + op.sxy]), fz);
+      sum += w.B + w.F;
+    } else {
+      w.B = T(0); w.F = T(0);
+    }
+    const T e = op.e_field ? T(op.e_field[id]) : T(op.e);
+    w.C = e - sum;
+  }
+  return w;
+}
+
+// Folding: an interior row next to a folded face adds w_nb*theta_f to its centre.
+template <int DIM, typename T>
+__device__ __forceinline__ void fold_center(const DevOp& op, int i, int j, int k, W7<T>& w) {
+  if (i == 1) w.C += w.W * T(op.theta[0]);
+  if (i == op.nx - 2) w.C += w.E * T(op.theta[1]);
+  if (j == 1) w.C += w.S * T(op.theta[2]);
+  if (j == op.ny - 2) w.C += w.N * T(op.theta[3]);
+  if constexpr (DIM == 3) {
+    if (k == 1) w.C += w.B * T(op.theta[4]);
+    if (k == op.nz - 2) w.C += w.F * T(op.theta[5]);
+  }
+}
+
+__device__ __forceinline__ bool is_boundary2(const DevOp& op, int i, int j) {
+  return i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1;
+}
+
+// ------------------------------------------------------------------ reductions
+// Sum over the 64 lanes of a wavefront.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+// Block sum: wave shuffle, then one LDS slot per wave, then the first wave.  Result valid in
+// thread 0.  `lds` must hold blockDim.x/64 elements.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  const int nw = (blockDim.x + kWave - 1) / kWave;
+  T r = (threadIdx.x < nw) ? lds[threadIdx.x] : T(0);
+  if (wid == 0) r = wave_sum(r);
+  __syncthreads();
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T* lds) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  const int nw = (blockDim.x + kWave - 1) / kWave;
+  T r = (threadIdx.x < nw) ? lds[threadIdx.x] : T(0);
+  if (wid == 0) r = wave_max(r);
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------------ device scalars
+// Scalars of one Krylov solve, resident in device memory so iterations never wait on the host.
+struct KScalars {
+  double rz, rz_old, pq, rr, alpha, beta, omega, rho, rho_old, tt, ts;
+  double r0, target, metric;
+  double eps;        // breakdown threshold on denominators (reference 1e-30)
+  double scale;      // mixed precision: normalisation of the preconditioner input
+  double inv_scale;
+  double snorm;
+  int done;          // 1 once the stop rule holds; later kernels return immediately
+  int breakdown;     // 1 when a denominator fell below eps
+  int iter;          // completed iterations
+  int flag_s;        // BiCGStab: ||s|| met the target, x += alpha*p_hat pending
+  int cycle_done;    // GMRES: the current restart cycle is finished
+  int steps;         // GMRES: Arnoldi steps of the current cycle
+  int pad0, pad1;
+};
+
+}  // namespace pdehip

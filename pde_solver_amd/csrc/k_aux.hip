@@ -1,0 +1,521 @@
+// k_aux.hip — boundary closures, multigrid transfers, coarsest-level solve, vector kernels and
+// the deterministic two-stage reductions whose second stage updates device-resident scalars.
+#include "kernels.hpp"
+
+namespace pdehip {
+namespace {
+
+template <Prec P> struct PT;
+template <> struct PT<P64> { using T = double; using S = double; };
+template <> struct PT<P32> { using T = float; using S = float; };
+template <> struct PT<P16> { using T = float; using S = __half; };
+
+__device__ __forceinline__ double ldd(const double* p, long long i) { return p[i]; }
+__device__ __forceinline__ double ldd(const float* p, long long i) { return p[i]; }
+__device__ __forceinline__ double ldd(const __half* p, long long i) { return __half2float(p[i]); }
+
+__device__ __forceinline__ bool skip(const KScalars* sc) { return sc && (sc->done | sc->breakdown); }
+
+long long grid1(long long n, long long cap = 8192) {
+  long long b = (n + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  return b > cap ? cap : b;
+}
+
+// Face node q of axis `axis` on the low (side 0) or high (side 1) face: node index and the
+// index of its inward neighbour.  Face node order: x faces k*ny+j, y faces k*nx+i, z faces j*nx+i.
+__device__ __forceinline__ void face_node(const DevOp& op, int axis, int side, long long q,
+                                          long long* node, long long* inner) {
+  if (axis == 0) {
+    const long long j = q % op.ny, k = q / op.ny;
+    const long long i = side ? op.nx - 1 : 0, ii = side ? op.nx - 2 : 1;
+    *node = (k * op.ny + j) * op.nx + i;
+    *inner = (k * op.ny + j) * op.nx + ii;
+  } else if (axis == 1) {
+    const long long i = q % op.nx, k = q / op.nx;
+    const long long j = side ? op.ny - 1 : 0, jj = side ? op.ny - 2 : 1;
+    *node = (k * op.ny + j) * op.nx + i;
+    *inner = (k * op.ny + jj) * op.nx + i;
+  } else {
+    const long long i = q % op.nx, j = q / op.nx;
+    const long long k = side ? op.nz - 1 : 0, kk = side ? op.nz - 2 : 1;
+    *node = (k * op.ny + j) * op.nx + i;
+    *inner = (kk * op.ny + j) * op.nx + i;
+  }
+}
+
+__host__ __device__ inline long long face_size(const DevOp& op, int axis) {
+  return axis == 0 ? (long long)op.ny * op.nz : (axis == 1 ? (long long)op.nx * op.nz : op.sxy);
+}
+
+// ----------------------------------------------------------------------------- closures
+// Neumann/Robin closure of one axis, low face then high face per thread (reference
+// cpu_utils.cpp:630-720 / 774-901 for AXIS; OUTWARD: alpha*u + beta*du/dn = gamma).
+__global__ void k_closure_nr(DevOp op, FaceData fd, int axis, double* u, const KScalars* sc) {
+  if (skip(sc)) return;
+  const long long nq = face_size(op, axis);
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nq;
+       q += (long long)gridDim.x * blockDim.x) {
+    for (int side = 0; side < 2; ++side) {
+      const int f = 2 * axis + side;
+      const int kind = fd.kind[f];
+      if (kind == PDEHIP_BC_DIRICHLET) continue;
+      long long node, inner;
+      face_node(op, axis, side, q, &node, &inner);
+      const double h = fd.h[f];
+      const double in = u[inner];
+      if (kind == PDEHIP_BC_NEUMANN) {
+        const double g = fd.value[f][q];
+        if (fd.convention == PDEHIP_BCCONV_AXIS) u[node] = side ? in + h * g : in - h * g;
+        else u[node] = in + h * g;
+      } else {
+        const double al = fd.alpha[f][q], be = fd.beta[f][q], ga = fd.gamma[f][q];
+        if (fd.convention == PDEHIP_BCCONV_AXIS && side) {
+          const double den = al - be / h;
+          if (fabs(den) > 1e-12) u[node] = (ga - (be / h) * in) / den;
+        } else {
+          const double den = al + be / h;
+          if (fabs(den) > 1e-12) u[node] = (ga + (be / h) * in) / den;
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_closure_d(DevOp op, FaceData fd, int axis, double* u, const KScalars* sc) {
+  if (skip(sc)) return;
+  const long long nq = face_size(op, axis);
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nq;
+       q += (long long)gridDim.x * blockDim.x) {
+    for (int side = 0; side < 2; ++side) {
+      const int f = 2 * axis + side;
+      if (fd.kind[f] != PDEHIP_BC_DIRICHLET) continue;
+      long long node, inner;
+      face_node(op, axis, side, q, &node, &inner);
+      u[node] = fd.value[f][q];
+    }
+  }
+}
+
+// Homogeneous extension u_face = theta * u_inner, low then high face.
+template <Prec P>
+__global__ void k_extend(DevOp op, int axis, void* uv, const KScalars* sc) {
+  using S = typename PT<P>::S;
+  if (skip(sc)) return;
+  S* u = static_cast<S*>(uv);
+  const long long nq = face_size(op, axis);
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nq;
+       q += (long long)gridDim.x * blockDim.x) {
+    for (int side = 0; side < 2; ++side) {
+      long long node, inner;
+      face_node(op, axis, side, q, &node, &inner);
+      u[node] = store_as<S>(op.theta[2 * axis + side] * ldd(u, inner));
+    }
+  }
+}
+
+// Transpose of k_extend: high face then low face, fold into the inner node and clear.
+template <Prec P>
+__global__ void k_fold_t(DevOp op, int axis, void* rv, const KScalars* sc) {
+  using S = typename PT<P>::S;
+  if (skip(sc)) return;
+  S* r = static_cast<S*>(rv);
+  const long long nq = face_size(op, axis);
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nq;
+       q += (long long)gridDim.x * blockDim.x) {
+    for (int side = 1; side >= 0; --side) {
+      long long node, inner;
+      face_node(op, axis, side, q, &node, &inner);
+      const double th = op.theta[2 * axis + side];
+      r[inner] = store_as<S>(ldd(r, inner) + th * ldd(r, node));
+      r[node] = store_as<S>(0.0);
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------- transfers
+template <Prec PF, Prec PC>
+__global__ void __launch_bounds__(kBlock) k_restrict(DevOp f, const void* rfv, DevOp c, void* bcv,
+                                                    double scale, const double* dscale,
+                                                    const KScalars* sc) {
+  using SF = typename PT<PF>::S;
+  using SC = typename PT<PC>::S;
+  if (skip(sc)) return;
+  const SF* rf = static_cast<const SF*>(rfv);
+  SC* bc = static_cast<SC*>(bcv);
+  const double s = dscale ? scale * (*dscale) : scale;
+  const bool d3 = c.nz > 1;
+  for (long long cid = blockIdx.x * (long long)blockDim.x + threadIdx.x; cid < c.n;
+       cid += (long long)gridDim.x * blockDim.x) {
+    const int ic = static_cast<int>(cid % c.nx);
+    const int jc = static_cast<int>((cid / c.nx) % c.ny);
+    const int kc = static_cast<int>(cid / c.sxy);
+    const int fi = 2 * ic, fj = 2 * jc, fk = d3 ? 2 * kc : 0;
+    double acc = 0.0;
+    const double w1[3] = {0.25, 0.5, 0.25};
+    const int dk0 = (d3 && fk > 0) ? -1 : 0, dk1 = (d3 && fk < f.nz - 1) ? 1 : 0;
+    const int dj0 = fj > 0 ? -1 : 0, dj1 = fj < f.ny - 1 ? 1 : 0;
+    const int di0 = fi > 0 ? -1 : 0, di1 = fi < f.nx - 1 ? 1 : 0;
+    for (int dk = dk0; dk <= dk1; ++dk) {
+      const double wk = d3 ? w1[dk + 1] : 1.0;
+      for (int dj = dj0; dj <= dj1; ++dj) {
+        const double wjk = wk * w1[dj + 1];
+        const long long row = ((long long)(fk + dk) * f.ny + (fj + dj)) * f.nx + fi;
+        for (int di = di0; di <= di1; ++di) acc += wjk * w1[di + 1] * ldd(rf, row + di);
+      }
+    }
+    bc[cid] = store_as<SC>(acc * s);
+  }
+}
+
+template <Prec PC, Prec PF>
+__global__ void __launch_bounds__(kBlock) k_prolong(DevOp c, const void* ucv, DevOp f, void* ufv,
+                                                   double factor, const double* dscale,
+                                                   const KScalars* sc) {
+  using SC = typename PT<PC>::S;
+  using SF = typename PT<PF>::S;
+  if (skip(sc)) return;
+  const SC* uc = static_cast<const SC*>(ucv);
+  SF* uf = static_cast<SF*>(ufv);
+  const double s = dscale ? factor * (*dscale) : factor;
+  const bool d3 = f.nz > 1;
+  for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < f.n;
+       id += (long long)gridDim.x * blockDim.x) {
+    const int i = static_cast<int>(id % f.nx);
+    const int j = static_cast<int>((id / f.nx) % f.ny);
+    const int k = static_cast<int>(id / f.sxy);
+    if (is_boundary2(f, i, j) || (d3 && (k == 0 || k == f.nz - 1))) continue;
+    const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
+    const int oi = i & 1, oj = j & 1, ok = d3 ? (k & 1) : 0;
+    double acc = 0.0;
+    for (int a = 0; a <= ok; ++a)
+      for (int bb = 0; bb <= oj; ++bb) {
+        const long long row = ((long long)(kc + a) * c.ny + (jc + bb)) * c.nx + ic;
+        for (int cc = 0; cc <= oi; ++cc) acc += ldd(uc, row + cc);
+      }
+    const int cnt = (1 + oi) * (1 + oj) * (1 + ok);
+    uf[id] = store_as<SF>(ldd(uf, id) + s * acc / cnt);
+  }
+}
+
+// One wavefront per row of the dense inverse: x[map[r]] = sum_q Ainv[r][q] * b[map[q]].
+template <Prec P>
+__global__ void __launch_bounds__(kBlock) k_coarse_dense(const double* __restrict__ ainv,
+                                                        const int* __restrict__ map, int m,
+                                                        const void* bv, void* xv,
+                                                        const KScalars* sc) {
+  using S = typename PT<P>::S;
+  if (skip(sc)) return;
+  const S* b = static_cast<const S*>(bv);
+  S* x = static_cast<S*>(xv);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (row >= m) return;
+  double acc = 0.0;
+  for (int q = lane; q < m; q += kWave) acc += ainv[(long long)row * m + q] * ldd(b, map[q]);
+  acc = wave_sum(acc);
+  if (lane == 0) x[map[row]] = store_as<S>(acc);
+}
+
+// kappa restriction for the hierarchy (full weighting inside, injection on the boundary).
+__global__ void k_restrict_kappa(DevOp f, const double* kf, DevOp c, double* kc) {
+  const bool d3 = c.nz > 1;
+  for (long long cid = blockIdx.x * (long long)blockDim.x + threadIdx.x; cid < c.n;
+       cid += (long long)gridDim.x * blockDim.x) {
+    const int ic = static_cast<int>(cid % c.nx);
+    const int jc = static_cast<int>((cid / c.nx) % c.ny);
+    const int kcc = static_cast<int>(cid / c.sxy);
+    const int fi = 2 * ic, fj = 2 * jc, fk = d3 ? 2 * kcc : 0;
+    const long long fid = ((long long)fk * f.ny + fj) * f.nx + fi;
+    if (is_boundary2(c, ic, jc) || (d3 && (kcc == 0 || kcc == c.nz - 1))) { kc[cid] = kf[fid]; continue; }
+    const double w1[3] = {0.25, 0.5, 0.25};
+    double acc = 0.0;
+    for (int dk = (d3 ? -1 : 0); dk <= (d3 ? 1 : 0); ++dk)
+      for (int dj = -1; dj <= 1; ++dj)
+        for (int di = -1; di <= 1; ++di)
+          acc += w1[di + 1] * w1[dj + 1] * (d3 ? w1[dk + 1] : 1.0) *
+                 kf[((long long)(fk + dk) * f.ny + (fj + dj)) * f.nx + fi + di];
+    kc[cid] = acc;
+  }
+}
+
+// ----------------------------------------------------------------------------- vectors
+template <Prec P>
+__global__ void k_fill(void* xv, long long n, double v) {
+  using S = typename PT<P>::S;
+  S* x = static_cast<S*>(xv);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    x[i] = store_as<S>(v);
+}
+
+template <Prec PI, Prec PO>
+__global__ void k_convert(const void* inv, void* outv, long long n, double scale) {
+  using SI = typename PT<PI>::S;
+  using SO = typename PT<PO>::S;
+  const SI* in = static_cast<const SI*>(inv);
+  SO* out = static_cast<SO*>(outv);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = store_as<SO>(ldd(in, i) * scale);
+}
+
+template <Prec P>
+__global__ void __launch_bounds__(kBlock) k_dot(const void* av, const void* bv, long long n,
+                                               double* partials, const KScalars* sc) {
+  using S = typename PT<P>::S;
+  __shared__ double lds[kBlock / kWave];
+  if (skip(sc)) return;
+  const S* a = static_cast<const S*>(av);
+  const S* b = static_cast<const S*>(bv);
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    acc += ldd(a, i) * ldd(b, i);
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <Prec P>
+__global__ void __launch_bounds__(kBlock) k_cg_update(void* xv, void* rv, const void* pvv,
+                                                     const void* qv, long long n, double* partials,
+                                                     double* maxpart, const KScalars* sc) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  __shared__ double lds[kBlock / kWave];
+  if (skip(sc)) return;
+  S* x = static_cast<S*>(xv);
+  S* r = static_cast<S*>(rv);
+  const S* p = static_cast<const S*>(pvv);
+  const S* q = static_cast<const S*>(qv);
+  const T alpha = T(sc->alpha);
+  double acc = 0.0, mx = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const T xi = T(ldd(x, i)) + alpha * T(ldd(p, i));
+    const T ri = T(ldd(r, i)) - alpha * T(ldd(q, i));
+    x[i] = store_as<S>(xi);
+    r[i] = store_as<S>(ri);
+    acc += static_cast<double>(ri) * static_cast<double>(ri);
+    mx = fmax(mx, fabs(static_cast<double>(ri)));
+  }
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  if (maxpart) {
+    const double m = block_max(mx, lds);
+    if (threadIdx.x == 0) maxpart[blockIdx.x] = m;
+  }
+}
+
+template <Prec P>
+__global__ void k_xpay(const void* zv, void* pvv, long long n, const KScalars* sc) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  if (skip(sc)) return;
+  const S* z = static_cast<const S*>(zv);
+  S* p = static_cast<S*>(pvv);
+  const T beta = T(sc->beta);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = store_as<S>(T(ldd(z, i)) + beta * T(ldd(p, i)));
+}
+
+template <Prec P>
+__global__ void k_axpy_dev(void* yv, const void* xv, long long n, const double* a, double sign,
+                           const KScalars* sc) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  if (skip(sc)) return;
+  S* y = static_cast<S*>(yv);
+  const S* x = static_cast<const S*>(xv);
+  const T av = T(sign * (a ? *a : 1.0));
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = store_as<S>(T(ldd(y, i)) + av * T(ldd(x, i)));
+}
+
+// Stage 2 of every reduction: one block sums the partials in a fixed order (deterministic
+// for a fixed launch shape) and updates the solve's scalars.
+__global__ void __launch_bounds__(1024) k_finalize(KScalars* sc, const double* partials, int np,
+                                                  int op, double tol, int tol_mode, double* slot) {
+  __shared__ double lds[1024 / kWave];
+  if (sc && (sc->done | sc->breakdown) && op != FIN_STORE && op != FIN_MAX) return;
+  const bool is_max = (op == FIN_MAX || op == FIN_RELAX);
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x)
+    acc = is_max ? fmax(acc, partials[i]) : acc + partials[i];
+  const double s = is_max ? block_max(acc, lds) : block_sum(acc, lds);
+  if (threadIdx.x != 0) return;
+  switch (op) {
+    case FIN_CG_ALPHA:
+      sc->pq = s;
+      if (fabs(s) < sc->eps) { sc->breakdown = 1; break; }
+      sc->alpha = sc->rz / s;
+      break;
+    case FIN_CG_RR:
+      sc->rr = s;
+      sc->metric = sqrt(fmax(0.0, s));
+      sc->iter += 1;
+      if (sc->metric < sc->target) sc->done = 1;
+      break;
+    case FIN_CG_BETA:
+      sc->rz_old = sc->rz;
+      sc->beta = s / sc->rz;
+      sc->rz = s;
+      break;
+    case FIN_CG_INIT:
+      sc->rz = s;
+      break;
+    case FIN_R0: {
+      const double r0 = sqrt(fmax(0.0, s));
+      sc->r0 = r0;
+      sc->metric = r0;
+      sc->target = tol_mode == PDEHIP_TOL_REL ? tol * r0 : tol;
+      if (r0 < sc->target || r0 == 0.0) sc->done = 1;
+      break;
+    }
+    case FIN_STORE:
+      *slot = s;
+      break;
+    case FIN_MAX:
+      *slot = s;
+      break;
+    case FIN_RELAX:
+      if (slot) *slot = s;
+      sc->metric = s;
+      sc->iter += 1;
+      if (s < tol) sc->done = 1;
+      break;
+  }
+}
+
+}  // namespace
+
+// ============================================================================== launchers
+void closures_nr(const Launch& L, const DevOp& op, const FaceData& fd, double* u) {
+  const int axes = op.nz > 1 ? 3 : 2;
+  for (int a = 0; a < axes; ++a) {
+    if (fd.kind[2 * a] == PDEHIP_BC_DIRICHLET && fd.kind[2 * a + 1] == PDEHIP_BC_DIRICHLET) continue;
+    hipLaunchKernelGGL(k_closure_nr, dim3(grid1(face_size(op, a), 1024)), dim3(kBlock), 0,
+                       L.stream, op, fd, a, u, L.sc);
+  }
+}
+
+void closures_dirichlet(const Launch& L, const DevOp& op, const FaceData& fd, double* u) {
+  const int axes = op.nz > 1 ? 3 : 2;
+  for (int a = 0; a < axes; ++a) {
+    if (fd.kind[2 * a] != PDEHIP_BC_DIRICHLET && fd.kind[2 * a + 1] != PDEHIP_BC_DIRICHLET) continue;
+    hipLaunchKernelGGL(k_closure_d, dim3(grid1(face_size(op, a), 1024)), dim3(kBlock), 0,
+                       L.stream, op, fd, a, u, L.sc);
+  }
+}
+
+#define PDEHIP_BY_PREC(p, KER, ...)                                                   \
+  switch (p) {                                                                        \
+    case P64: hipLaunchKernelGGL(KER<P64>, __VA_ARGS__); break;                       \
+    case P32: hipLaunchKernelGGL(KER<P32>, __VA_ARGS__); break;                       \
+    default: hipLaunchKernelGGL(KER<P16>, __VA_ARGS__); break;                        \
+  }
+
+void extend_closure(const Launch& L, const DevOp& op, Prec p, void* u) {
+  const int axes = op.nz > 1 ? 3 : 2;
+  for (int a = 0; a < axes; ++a) {
+    const dim3 g(grid1(face_size(op, a), 1024));
+    PDEHIP_BY_PREC(p, k_extend, g, dim3(kBlock), 0, L.stream, op, a, u, L.sc);
+  }
+}
+
+void fold_transpose(const Launch& L, const DevOp& op, Prec p, void* r) {
+  const int axes = op.nz > 1 ? 3 : 2;
+  for (int a = axes - 1; a >= 0; --a) {
+    const dim3 g(grid1(face_size(op, a), 1024));
+    PDEHIP_BY_PREC(p, k_fold_t, g, dim3(kBlock), 0, L.stream, op, a, r, L.sc);
+  }
+}
+
+void restrict_fw(const Launch& L, const DevOp& fine, Prec pf, const void* rf, const DevOp& coarse,
+                 Prec pc, void* bc, double scale, const double* dev_scale) {
+  const dim3 g(grid1(coarse.n));
+#define R_(A, B) hipLaunchKernelGGL((k_restrict<A, B>), g, dim3(kBlock), 0, L.stream, fine, rf, coarse, bc, scale, dev_scale, L.sc)
+  if (pf == P64 && pc == P64) R_(P64, P64);
+  else if (pf == P32 && pc == P32) R_(P32, P32);
+  else if (pf == P32 && pc == P16) R_(P32, P16);
+  else if (pf == P16 && pc == P16) R_(P16, P16);
+  else if (pf == P64 && pc == P32) R_(P64, P32);
+  else R_(P16, P32);
+#undef R_
+}
+
+void prolong_add(const Launch& L, const DevOp& coarse, Prec pc, const void* uc, const DevOp& fine,
+                 Prec pf, void* uf, double factor, const double* dev_scale) {
+  const dim3 g(grid1(fine.n));
+#define P_(A, B) hipLaunchKernelGGL((k_prolong<A, B>), g, dim3(kBlock), 0, L.stream, coarse, uc, fine, uf, factor, dev_scale, L.sc)
+  if (pc == P64 && pf == P64) P_(P64, P64);
+  else if (pc == P32 && pf == P32) P_(P32, P32);
+  else if (pc == P16 && pf == P32) P_(P16, P32);
+  else if (pc == P16 && pf == P16) P_(P16, P16);
+  else if (pc == P32 && pf == P64) P_(P32, P64);
+  else P_(P32, P16);
+#undef P_
+}
+
+void coarse_dense(const Launch& L, Prec p, const double* ainv, const int* map, int m, const void* b,
+                  void* x) {
+  const int rows_per_block = kBlock / kWave;
+  const dim3 g((m + rows_per_block - 1) / rows_per_block);
+  PDEHIP_BY_PREC(p, k_coarse_dense, g, dim3(kBlock), 0, L.stream, ainv, map, m, b, x, L.sc);
+}
+
+void restrict_kappa(hipStream_t s, const DevOp& fine, const double* kf, const DevOp& coarse,
+                    double* kc) {
+  hipLaunchKernelGGL(k_restrict_kappa, dim3(grid1(coarse.n)), dim3(kBlock), 0, s, fine, kf, coarse, kc);
+}
+
+int vec_blocks(long long n) { return static_cast<int>(grid1(n, 2048)); }
+
+void fill(hipStream_t s, Prec p, void* x, long long n, double v) {
+  const dim3 g(grid1(n));
+  PDEHIP_BY_PREC(p, k_fill, g, dim3(kBlock), 0, s, x, n, v);
+}
+
+void convert(hipStream_t s, Prec pin, const void* in, Prec pout, void* out, long long n, double scale) {
+  const dim3 g(grid1(n));
+#define C_(A, B) hipLaunchKernelGGL((k_convert<A, B>), g, dim3(kBlock), 0, s, in, out, n, scale)
+  if (pin == P64 && pout == P32) C_(P64, P32);
+  else if (pin == P32 && pout == P64) C_(P32, P64);
+  else if (pin == P64 && pout == P64) C_(P64, P64);
+  else if (pin == P32 && pout == P32) C_(P32, P32);
+  else if (pin == P64 && pout == P16) C_(P64, P16);
+  else C_(P16, P64);
+#undef C_
+}
+
+void dot(const Launch& L, Prec p, const void* a, const void* b, long long n, double* partials) {
+  const dim3 g(vec_blocks(n));
+  PDEHIP_BY_PREC(p, k_dot, g, dim3(kBlock), 0, L.stream, a, b, n, partials, L.sc);
+}
+
+void finalize(const Launch& L, KScalars* sc, const double* partials, int nparts, int op, double tol,
+              int tol_mode, double* out_slot) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, L.stream, sc, partials, nparts, op, tol,
+                     tol_mode, out_slot);
+}
+
+void cg_update(const Launch& L, Prec p, void* x, void* r, const void* pv, const void* q, long long n,
+               double* partials, double* maxpart) {
+  const dim3 g(vec_blocks(n));
+  PDEHIP_BY_PREC(p, k_cg_update, g, dim3(kBlock), 0, L.stream, x, r, pv, q, n, partials, maxpart, L.sc);
+}
+
+void cg_xpay(const Launch& L, Prec p, const void* z, void* pv, long long n) {
+  const dim3 g(vec_blocks(n));
+  PDEHIP_BY_PREC(p, k_xpay, g, dim3(kBlock), 0, L.stream, z, pv, n, L.sc);
+}
+
+void axpy_dev(const Launch& L, Prec p, void* y, const void* x, long long n, const double* a,
+              double sign) {
+  const dim3 g(vec_blocks(n));
+  PDEHIP_BY_PREC(p, k_axpy_dev, g, dim3(kBlock), 0, L.stream, y, x, n, a, sign, L.sc);
+}
+
+}  // namespace pdehip

@@ -1,0 +1,362 @@
+// k_krylov.hip — BiCGStab and GMRES vector kernels with device-resident scalars.
+//
+// BiCGStab follows the reference CPU algorithm (src/backends/cpu/solvers/krylov.cpp:82-185),
+// including its recomputation r = b - A x every iteration, with an optional right
+// preconditioner.  GMRES follows krylov.cpp:187-333 (restarted, Givens rotations with the CPU
+// sign convention) but orthogonalises by classical Gram-Schmidt applied twice (CGS2): each
+// pass is one multi-dot and one multi-axpy launch instead of 2(j+1) launches of modified
+// Gram-Schmidt; the resulting basis agrees with MGS to rounding.
+#include "kernels_krylov.hpp"
+
+namespace pdehip {
+namespace {
+
+__device__ __forceinline__ bool skip(const KScalars* sc) { return sc->done | sc->breakdown; }
+
+int blocks_for(long long n) {
+  long long b = (n + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  return static_cast<int>(b > 1024 ? 1024 : b);
+}
+
+// ----------------------------------------------------------------------------- BiCGStab
+__global__ void k_bicg_p(const double* __restrict__ r, double* __restrict__ p,
+                         const double* __restrict__ v, long long n, const KScalars* sc) {
+  if (skip(sc)) return;
+  const bool first = sc->iter == 0;
+  const double beta = sc->beta, omega = sc->omega;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = first ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_bicg_s(const double* __restrict__ r,
+                                                  const double* __restrict__ v,
+                                                  double* __restrict__ s, long long n,
+                                                  double* partials, const KScalars* sc) {
+  __shared__ double lds[kBlock / kWave];
+  if (skip(sc)) return;
+  const double alpha = sc->alpha;
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double si = r[i] - alpha * v[i];
+    s[i] = si;
+    acc += si * si;
+  }
+  const double t = block_sum(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// x += alpha*p_hat when ||s|| already met the target (krylov.cpp:143-148).
+__global__ void k_bicg_xs(double* __restrict__ x, const double* __restrict__ ph, long long n,
+                          const KScalars* sc) {
+  if (skip(sc) || !sc->flag_s) return;
+  const double alpha = sc->alpha;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    x[i] += alpha * ph[i];
+}
+
+__global__ void __launch_bounds__(kBlock) k_dot2(const double* __restrict__ a,
+                                                const double* __restrict__ b,
+                                                const double* __restrict__ c, long long n,
+                                                double* p1, double* p2, const KScalars* sc) {
+  __shared__ double lds[kBlock / kWave];
+  if (skip(sc)) return;
+  double ab = 0.0, ac = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    ab += a[i] * b[i];
+    ac += a[i] * c[i];
+  }
+  const double s1 = block_sum(ab, lds);
+  const double s2 = block_sum(ac, lds);
+  if (threadIdx.x == 0) { p1[blockIdx.x] = s1; p2[blockIdx.x] = s2; }
+}
+
+__global__ void k_bicg_x(double* __restrict__ x, const double* __restrict__ ph,
+                         const double* __restrict__ sh, long long n, const KScalars* sc) {
+  if (skip(sc)) return;
+  const double alpha = sc->alpha, omega = sc->omega;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    x[i] += alpha * ph[i] + omega * sh[i];
+}
+
+// Jacobi preconditioner: out = in / diag on interior rows (folded diagonal), identity on
+// boundary rows.
+template <int DIM, int KIND>
+__global__ void k_jacobi_pc(DevOp op, const double* __restrict__ in, double* __restrict__ out,
+                            const KScalars* sc) {
+  if (sc && skip(sc)) return;
+  for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < op.n;
+       id += (long long)gridDim.x * blockDim.x) {
+    const int i = static_cast<int>(id % op.nx);
+    const int j = static_cast<int>((id / op.nx) % op.ny);
+    const int k = static_cast<int>(id / op.sxy);
+    if (is_boundary2(op, i, j) || (DIM == 3 && (k == 0 || k == op.nz - 1))) { out[id] = in[id]; continue; }
+    W7<double> w = row_weights<DIM, KIND, double, double>(op, id);
+    fold_center<DIM, double>(op, i, j, k, w);
+    out[id] = in[id] / w.C;
+  }
+}
+
+// ----------------------------------------------------------------------------- GMRES
+constexpr int kMaxBasis = 64;
+
+// partials[i*nb + block] = V_i . w for i < cnt (V rows of length ld).
+__global__ void __launch_bounds__(kBlock) k_multidot(const double* __restrict__ V, long long ld,
+                                                    int cnt, const double* __restrict__ w,
+                                                    long long n, double* partials,
+                                                    const KScalars* sc) {
+  __shared__ double lds[kBlock / kWave];
+  if (skip(sc) || sc->cycle_done) return;
+  const int nb = gridDim.x;
+  for (int i0 = 0; i0 < cnt; i0 += 8) {
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int m = min(8, cnt - i0);
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
+         q += (long long)gridDim.x * blockDim.x) {
+      const double wv = w[q];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (t < m) acc[t] += V[(long long)(i0 + t) * ld + q] * wv;
+    }
+    for (int t = 0; t < m; ++t) {
+      const double s = block_sum(acc[t], lds);
+      if (threadIdx.x == 0) partials[(long long)(i0 + t) * nb + blockIdx.x] = s;
+    }
+  }
+}
+
+// w -= sum_i V_i h_i (h from device memory); also used with sign +1 for x updates.
+__global__ void k_multiaxpy(const double* __restrict__ V, long long ld, int cnt,
+                            const double* __restrict__ h, double* __restrict__ w, long long n,
+                            double sign, int check_cycle, const KScalars* sc) {
+  if (skip(sc) || (check_cycle && sc->cycle_done)) return;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int i = 0; i < cnt; ++i) acc += h[i] * V[(long long)i * ld + q];
+    w[q] += sign * acc;
+  }
+}
+
+// Sum the multi-dot partials into hcol[i] (accumulate when add != 0).
+__global__ void __launch_bounds__(256) k_fin_multi(const double* partials, int nb, int cnt,
+                                                  double* hcol, int add, const KScalars* sc) {
+  __shared__ double lds[256 / kWave];
+  if (skip(sc) || sc->cycle_done) return;
+  for (int i = 0; i < cnt; ++i) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[(long long)i * nb + b];
+    const double s = block_sum(acc, lds);
+    if (threadIdx.x == 0) hcol[i] = add ? hcol[i] + s : s;
+  }
+}
+
+// H[j+1][j] = ||w||, V_{j+1} = w / H[j+1][j]
+__global__ void k_gm_normalize(double* __restrict__ vnext, const double* __restrict__ w,
+                               long long n, const double* hval, const KScalars* sc) {
+  if (skip(sc) || sc->cycle_done) return;
+  const double h = *hval;
+  if (!(h > 0.0)) return;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x)
+    vnext[q] = w[q] / h;
+}
+
+__global__ void __launch_bounds__(256) k_fin_norm(const double* partials, int nb, double* out,
+                                                 const KScalars* sc) {
+  __shared__ double lds[256 / kWave];
+  if (skip(sc) || sc->cycle_done) return;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[b];
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) *out = sqrt(fmax(0.0, s));
+}
+
+// Givens step j (krylov.cpp:262-295): H is (m+1) x m row-major with leading dimension m.
+__global__ void k_givens(double* H, double* cs, double* sn, double* g, int j, int m,
+                         KScalars* sc) {
+  if (skip(sc) || sc->cycle_done) return;
+  for (int i = 0; i < j; ++i) {
+    const double v1 = H[i * m + j], v2 = H[(i + 1) * m + j];
+    H[i * m + j] = cs[i] * v1 - sn[i] * v2;
+    H[(i + 1) * m + j] = sn[i] * v1 + cs[i] * v2;
+  }
+  const double h1 = H[(j + 1) * m + j], h0 = H[j * m + j];
+  const double denom = sqrt(h0 * h0 + h1 * h1);
+  if (denom < 1e-30) { sc->breakdown = 1; return; }
+  cs[j] = h0 / denom;
+  sn[j] = -h1 / denom;
+  H[j * m + j] = cs[j] * h0 - sn[j] * h1;
+  H[(j + 1) * m + j] = 0.0;
+  const double g0 = g[j], g1 = g[j + 1];
+  g[j] = cs[j] * g0 - sn[j] * g1;
+  g[j + 1] = sn[j] * g0 + cs[j] * g1;
+  sc->steps = j + 1;
+  sc->iter += 1;
+  sc->metric = fabs(g[j + 1]);
+  if (fabs(g[j + 1]) < sc->target) sc->cycle_done = 1;
+}
+
+// Back substitution over the steps of the current cycle.
+__global__ void k_backsub(const double* H, const double* g, double* y, int m, KScalars* sc) {
+  if (skip(sc)) return;
+  const int dim = sc->steps;
+  for (int i = dim - 1; i >= 0; --i) {
+    double sum = g[i];
+    for (int q = i + 1; q < dim; ++q) sum -= H[i * m + q] * y[q];
+    if (fabs(H[i * m + i]) < 1e-30) { sc->breakdown = 1; return; }
+    y[i] = sum / H[i * m + i];
+  }
+  for (int i = dim; i < m; ++i) y[i] = 0.0;
+}
+
+// New cycle: V_0 = r / beta, g = [beta, 0, ...]; clears the cycle flags.
+__global__ void k_gm_start(double* __restrict__ v0, const double* __restrict__ r, long long n,
+                           double* g, int m, KScalars* sc) {
+  if (skip(sc)) return;
+  const double beta = sc->metric;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x)
+    v0[q] = r[q] / beta;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // cycle flags are reset by k_gm_reset before this kernel's first use
+    g[0] = beta;
+    for (int i = 1; i <= m; ++i) g[i] = 0.0;
+  }
+}
+
+__global__ void k_gm_setcol(double* H, const double* hc, int j, int m, int cnt,
+                            const KScalars* sc) {
+  if (skip(sc) || sc->cycle_done) return;
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) H[i * m + j] = hc[i] + hc[64 + i];
+}
+
+__global__ void k_gm_reset(KScalars* sc) {
+  if (threadIdx.x == 0) { sc->cycle_done = 0; sc->steps = 0; }
+}
+
+// Stage-2 reductions for BiCGStab and the true-residual checks.
+__global__ void __launch_bounds__(1024) k_fin_k(KScalars* sc, const double* p1, const double* p2,
+                                               int np, int op) {
+  __shared__ double lds[1024 / kWave];
+  if (skip(sc)) return;
+  double a1 = 0.0, a2 = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    a1 += p1[i];
+    if (p2) a2 += p2[i];
+  }
+  const double s1 = block_sum(a1, lds);
+  const double s2 = p2 ? block_sum(a2, lds) : 0.0;
+  if (threadIdx.x != 0) return;
+  switch (op) {
+    case KFIN_BI_RHO:
+      sc->rho = s1;
+      if (fabs(s1) < sc->eps) { sc->breakdown = 1; break; }
+      if (sc->iter > 0) sc->beta = (s1 / sc->rho_old) * (sc->alpha / sc->omega);
+      break;
+    case KFIN_BI_ALPHA:
+      if (fabs(s1) < sc->eps) { sc->breakdown = 1; break; }
+      sc->alpha = sc->rho / s1;
+      break;
+    case KFIN_BI_S:
+      sc->snorm = sqrt(fmax(0.0, s1));
+      if (sc->snorm < sc->target) sc->flag_s = 1;
+      break;
+    case KFIN_BI_OMEGA:
+      sc->tt = s1;
+      sc->ts = s2;
+      if (fabs(s1) < sc->eps) { sc->breakdown = 1; break; }
+      sc->omega = s2 / s1;
+      break;
+    case KFIN_BI_R:
+      sc->metric = sqrt(fmax(0.0, s1));
+      sc->iter += 1;
+      if (sc->metric < sc->target) { sc->done = 1; break; }
+      if (fabs(sc->omega) < sc->eps) { sc->breakdown = 1; break; }
+      sc->rho_old = sc->rho;
+      break;
+    case KFIN_GM_R:
+      sc->metric = sqrt(fmax(0.0, s1));
+      if (sc->metric < sc->target) sc->done = 1;
+      break;
+  }
+}
+
+__global__ void k_bi_sdone(KScalars* sc) {
+  if (threadIdx.x == 0 && !(sc->done | sc->breakdown) && sc->flag_s) {
+    sc->metric = sc->snorm;
+    sc->iter += 1;
+    sc->done = 1;
+  }
+}
+
+}  // namespace
+
+int krylov_blocks(long long n) { return blocks_for(n); }
+
+void bicg_p(hipStream_t s, const double* r, double* p, const double* v, long long n, const KScalars* sc) {
+  hipLaunchKernelGGL(k_bicg_p, dim3(blocks_for(n)), dim3(kBlock), 0, s, r, p, v, n, sc);
+}
+void bicg_s(hipStream_t s, const double* r, const double* v, double* sv, long long n, double* partials, const KScalars* sc) {
+  hipLaunchKernelGGL(k_bicg_s, dim3(blocks_for(n)), dim3(kBlock), 0, s, r, v, sv, n, partials, sc);
+}
+void bicg_xs(hipStream_t s, double* x, const double* ph, long long n, const KScalars* sc) {
+  hipLaunchKernelGGL(k_bicg_xs, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, ph, n, sc);
+}
+void dot2(hipStream_t s, const double* a, const double* b, const double* c, long long n, double* p1, double* p2, const KScalars* sc) {
+  hipLaunchKernelGGL(k_dot2, dim3(blocks_for(n)), dim3(kBlock), 0, s, a, b, c, n, p1, p2, sc);
+}
+void bicg_x(hipStream_t s, double* x, const double* ph, const double* sh, long long n, const KScalars* sc) {
+  hipLaunchKernelGGL(k_bicg_x, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, ph, sh, n, sc);
+}
+void bi_sdone(hipStream_t s, KScalars* sc) { hipLaunchKernelGGL(k_bi_sdone, dim3(1), dim3(64), 0, s, sc); }
+void fin_k(hipStream_t s, KScalars* sc, const double* p1, const double* p2, int np, int op) {
+  hipLaunchKernelGGL(k_fin_k, dim3(1), dim3(1024), 0, s, sc, p1, p2, np, op);
+}
+
+void jacobi_pc(hipStream_t s, const DevOp& op, const double* in, double* out, const KScalars* sc) {
+  long long b = (op.n + kBlock - 1) / kBlock;
+  if (b > 4096) b = 4096;
+  const bool d3 = op.nz > 1;
+#define J_(D, K) hipLaunchKernelGGL((k_jacobi_pc<D, K>), dim3(b), dim3(kBlock), 0, s, op, in, out, sc)
+  if (op.kind == PDEHIP_OP_POINTWISE) { if (d3) J_(3, PDEHIP_OP_POINTWISE); else J_(2, PDEHIP_OP_POINTWISE); }
+  else if (op.kind == PDEHIP_OP_FLUX) { if (d3) J_(3, PDEHIP_OP_FLUX); else J_(2, PDEHIP_OP_FLUX); }
+  else { if (d3) J_(3, PDEHIP_OP_CONST); else J_(2, PDEHIP_OP_CONST); }
+#undef J_
+}
+
+void multidot(hipStream_t s, const double* V, long long ld, int cnt, const double* w, long long n, double* partials, int nb, const KScalars* sc) {
+  hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, cnt, w, n, partials, sc);
+}
+void multiaxpy(hipStream_t s, const double* V, long long ld, int cnt, const double* h, double* w, long long n, double sign, int check_cycle, const KScalars* sc) {
+  hipLaunchKernelGGL(k_multiaxpy, dim3(blocks_for(n)), dim3(kBlock), 0, s, V, ld, cnt, h, w, n, sign, check_cycle, sc);
+}
+void fin_multi(hipStream_t s, const double* partials, int nb, int cnt, double* hcol, int add, const KScalars* sc) {
+  hipLaunchKernelGGL(k_fin_multi, dim3(1), dim3(256), 0, s, partials, nb, cnt, hcol, add, sc);
+}
+void gm_normalize(hipStream_t s, double* vnext, const double* w, long long n, const double* hval, const KScalars* sc) {
+  hipLaunchKernelGGL(k_gm_normalize, dim3(blocks_for(n)), dim3(kBlock), 0, s, vnext, w, n, hval, sc);
+}
+void fin_norm(hipStream_t s, const double* partials, int nb, double* out, const KScalars* sc) {
+  hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(256), 0, s, partials, nb, out, sc);
+}
+void givens(hipStream_t s, double* H, double* cs, double* sn, double* g, int j, int m, KScalars* sc) {
+  hipLaunchKernelGGL(k_givens, dim3(1), dim3(1), 0, s, H, cs, sn, g, j, m, sc);
+}
+void backsub(hipStream_t s, const double* H, const double* g, double* y, int m, KScalars* sc) {
+  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(1), 0, s, H, g, y, m, sc);
+}
+void gm_setcol(hipStream_t s, double* H, const double* hc, int j, int m, int cnt, const KScalars* sc) {
+  hipLaunchKernelGGL(k_gm_setcol, dim3(1), dim3(64), 0, s, H, hc, j, m, cnt, sc);
+}
+void gm_start(hipStream_t s, double* v0, const double* r, long long n, double* g, int m, KScalars* sc) {
+  hipLaunchKernelGGL(k_gm_reset, dim3(1), dim3(64), 0, s, sc);
+  hipLaunchKernelGGL(k_gm_start, dim3(blocks_for(n)), dim3(kBlock), 0, s, v0, r, n, g, m, sc);
+}
+
+}  // namespace pdehip

@@ -1,0 +1,109 @@
+// kernels.hpp — host-side launchers of the device kernels (defined in k_*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace pdehip {
+
+// Storage/compute precision of one level's vectors.
+//  P64 : double vectors, double kappa, double arithmetic
+//  P32 : float vectors, float kappa, float arithmetic
+//  P16 : __half vectors (scaled), float kappa, float arithmetic
+enum Prec { P64 = 0, P32 = 1, P16 = 2 };
+
+inline size_t prec_bytes(Prec p) { return p == P64 ? 8 : (p == P32 ? 4 : 2); }
+inline size_t kappa_bytes(Prec p) { return p == P64 ? 8 : 4; }
+
+struct Launch {
+  hipStream_t stream;
+  const KScalars* sc;   // device scalars; kernels return early once sc->done (may be null)
+};
+
+// ---- stencils (k_stencil.hip)
+// y = A x (identity boundary rows).  If dot_partials != null, also block partials of x.y.
+void apply(const Launch& L, const DevOp& op, Prec p, const void* x, void* y, double* dot_partials);
+// r = b - A x on every row (boundary rows: b - x).  If partials != null, block partials of r.r.
+void residual(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x, void* r,
+              double* partials);
+// One colour of red-black Gauss-Seidel for A x = b (interior rows of colour (i+j+k)%2 == color).
+// zero_guess: x is known to be zero on entry of this colour's first sweep (skips its loads).
+void rbgs(const Launch& L, const DevOp& op, Prec p, const void* b, void* x, int color);
+// Number of blocks `partials`-producing kernels use for this op (<= kMaxPartials).
+int partial_blocks(const DevOp& op);
+
+// Reference-semantics relaxation (FP64): interior update (-(f + N(u)) - sum w u_nb)/C.
+struct NonlinearTerms {
+  int n;
+  pdehip_nonlinear t[PDEHIP_MAX_NONLINEAR];
+};
+void jacobi_ref(const Launch& L, const DevOp& op, const double* f, double f_const,
+                const NonlinearTerms& nl, const double* x, double* next, double* maxdelta_partials,
+                int* degenerate);
+void sor_rb_ref(const Launch& L, const DevOp& op, const double* f, double f_const,
+                const NonlinearTerms& nl, double* x, double omega, int color,
+                double* maxdelta_partials, int* degenerate);
+int relax_blocks(const DevOp& op);
+
+// ---- boundary closures (k_aux.hip)
+struct FaceData {
+  int kind[6];
+  const double* value[6];   // per face node
+  const double* alpha[6];
+  const double* beta[6];
+  const double* gamma[6];
+  double h[6];
+  int convention;
+};
+// Reference closure order: axis by axis (x faces, y faces, z faces).
+void closures_nr(const Launch& L, const DevOp& op, const FaceData& fd, double* u);
+void closures_dirichlet(const Launch& L, const DevOp& op, const FaceData& fd, double* u);
+// Homogeneous closure extension (corrections) and its transpose, on a level's vector.
+void extend_closure(const Launch& L, const DevOp& op, Prec p, void* u);
+void fold_transpose(const Launch& L, const DevOp& op, Prec p, void* r);
+
+// ---- multigrid transfers
+// coarse = FW(fine) * scale on every coarse node that has fine neighbours (boundary too).
+void restrict_fw(const Launch& L, const DevOp& fine, Prec pf, const void* rf, const DevOp& coarse,
+                 Prec pc, void* bc, double scale, const double* dev_scale);
+// fine interior += factor * interpolate(coarse).
+void prolong_add(const Launch& L, const DevOp& coarse, Prec pc, const void* uc, const DevOp& fine,
+                 Prec pf, void* uf, double factor, const double* dev_scale);
+// Coarsest level: x[map] = Ainv * b[map] in FP64.
+void coarse_dense(const Launch& L, Prec p, const double* ainv, const int* map, int m,
+                  const void* b, void* x);
+// kappa restriction (FLUX hierarchy set-up), FP64 in/out.
+void restrict_kappa(hipStream_t s, const DevOp& fine, const double* kf, const DevOp& coarse,
+                    double* kc);
+
+// ---- vectors and scalars
+void fill(hipStream_t s, Prec p, void* x, long long n, double v);
+void convert(hipStream_t s, Prec pin, const void* in, Prec pout, void* out, long long n, double scale);
+// partials of a.b (or a.a when b == a)
+void dot(const Launch& L, Prec p, const void* a, const void* b, long long n, double* partials);
+int vec_blocks(long long n);
+
+// Finalize step codes: sum the partials and update the scalars (single block).
+enum FinOp {
+  FIN_CG_ALPHA = 0,    // pq = sum; alpha = rz/pq
+  FIN_CG_RR = 1,       // rr = sum; metric = sqrt(rr); done = metric < target
+  FIN_CG_BETA = 2,     // rz_new = sum; beta = rz_new/rz; rz = rz_new
+  FIN_CG_INIT = 3,     // rz = sum (first r.z)
+  FIN_R0 = 4,          // r0 = sqrt(sum); target from tol; metric = r0; done if r0 < target
+  FIN_STORE = 5,       // out[slot] = sum
+  FIN_MAX = 6,         // out[slot] = max
+  FIN_RELAX = 7,       // metric = max; iter++; done = metric < tol (relaxation stop rule)
+};
+void finalize(const Launch& L, KScalars* sc, const double* partials, int nparts, int op,
+              double tol, int tol_mode, double* out_slot);
+
+// CG updates: x += alpha p, r -= alpha q, partials of r.r (and max|r| into maxpart if non-null).
+void cg_update(const Launch& L, Prec p, void* x, void* r, const void* pv, const void* q,
+               long long n, double* partials, double* maxpart);
+// p = z + beta p
+void cg_xpay(const Launch& L, Prec p, const void* z, void* pv, long long n);
+// y += a x with a from device scalar (mode) or constant
+void axpy_dev(const Launch& L, Prec p, void* y, const void* x, long long n, const double* a, double sign);
+
+}  // namespace pdehip

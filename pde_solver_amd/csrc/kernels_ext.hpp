@@ -1,0 +1,32 @@
+// kernels_ext.hpp — launchers of k_time.hip (time stages, residual norms, lexicographic GS).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace pdehip {
+
+void time_stage(hipStream_t s, const DevOp& op, const double* v, const double* un, double* acc,
+                double* out, const double* f, double f_const, const NonlinearTerms& nl, double ut,
+                double dt, double c1, int mode);
+void time_stage32(hipStream_t s, const DevOp& op, const float* v, const float* un, float* acc,
+                  float* out, const double* f, double f_const, const NonlinearTerms& nl, double ut,
+                  double dt, double c1, int mode);
+void time_utt(hipStream_t s, const DevOp& op, const double* u, double* vel, double* next,
+              const double* f, double f_const, const NonlinearTerms& nl, double ut, double utt,
+              double dt);
+int resnorm_blocks(const DevOp& op);
+void residual_norms(hipStream_t s, const DevOp& op, const FaceData& fd, const double* u,
+                    const double* f, double f_const, const NonlinearTerms& nl, double* sums,
+                    double* maxs, int* degenerate);
+void lex_gs(hipStream_t s, int nx, int ny, double ax, double by, double center, const double* b,
+            double* u, int iters);
+
+void build_rhs(hipStream_t s, const DevOp& op, const double* f, double f_const, const double* bnd,
+               double* b);
+void add_f32(hipStream_t s, double* x, const float* e, long long n, const double* scale, double hs);
+void to_f32_scaled(hipStream_t s, const double* in, float* out, long long n, const double* norm);
+int maxabs_blocks(long long n);
+void maxabs32(hipStream_t s, const float* a, long long n, double* partials);
+void fin_scale(hipStream_t s, const double* partials, int np, KScalars* sc);
+
+}  // namespace pdehip

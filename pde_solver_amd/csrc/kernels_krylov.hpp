@@ -1,0 +1,30 @@
+// kernels_krylov.hpp — launchers of k_krylov.hip (BiCGStab, GMRES, Jacobi preconditioner).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace pdehip {
+
+enum KFinOp { KFIN_BI_RHO = 0, KFIN_BI_ALPHA = 1, KFIN_BI_S = 2, KFIN_BI_OMEGA = 3, KFIN_BI_R = 4,
+              KFIN_GM_R = 5 };
+
+int krylov_blocks(long long n);
+void bicg_p(hipStream_t s, const double* r, double* p, const double* v, long long n, const KScalars* sc);
+void bicg_s(hipStream_t s, const double* r, const double* v, double* sv, long long n, double* partials, const KScalars* sc);
+void bicg_xs(hipStream_t s, double* x, const double* ph, long long n, const KScalars* sc);
+void dot2(hipStream_t s, const double* a, const double* b, const double* c, long long n, double* p1, double* p2, const KScalars* sc);
+void bicg_x(hipStream_t s, double* x, const double* ph, const double* sh, long long n, const KScalars* sc);
+void bi_sdone(hipStream_t s, KScalars* sc);
+void fin_k(hipStream_t s, KScalars* sc, const double* p1, const double* p2, int np, int op);
+void jacobi_pc(hipStream_t s, const DevOp& op, const double* in, double* out, const KScalars* sc);
+void multidot(hipStream_t s, const double* V, long long ld, int cnt, const double* w, long long n, double* partials, int nb, const KScalars* sc);
+void multiaxpy(hipStream_t s, const double* V, long long ld, int cnt, const double* h, double* w, long long n, double sign, int check_cycle, const KScalars* sc);
+void fin_multi(hipStream_t s, const double* partials, int nb, int cnt, double* hcol, int add, const KScalars* sc);
+void gm_normalize(hipStream_t s, double* vnext, const double* w, long long n, const double* hval, const KScalars* sc);
+void fin_norm(hipStream_t s, const double* partials, int nb, double* out, const KScalars* sc);
+void givens(hipStream_t s, double* H, double* cs, double* sn, double* g, int j, int m, KScalars* sc);
+void backsub(hipStream_t s, const double* H, const double* g, double* y, int m, KScalars* sc);
+void gm_setcol(hipStream_t s, double* H, const double* hc, int j, int m, int cnt, const KScalars* sc);
+void gm_start(hipStream_t s, double* v0, const double* r, long long n, double* g, int m, KScalars* sc);
+
+}  // namespace pdehip

@@ -1,0 +1,1316 @@
+// runtime.cpp — host runtime of libpdehip (see runtime.hpp).
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace pdehip {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  if (e == hipErrorOutOfMemory) throw Error(PDEHIP_ERR_OOM, std::string(what) + ": out of device memory");
+  throw Error(PDEHIP_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ buffers
+void DBuf::alloc(Ctx* c, size_t bytes) {
+  release();
+  if (bytes == 0) return;
+  const size_t now = c->used.fetch_add(bytes) + bytes;
+  if (c->budget && now > c->budget) {
+    c->used.fetch_sub(bytes);
+    throw Error(PDEHIP_ERR_OOM, "allocation of " + std::to_string(bytes) +
+                                    " bytes exceeds the context's HBM budget");
+  }
+  size_t pk = c->peak.load();
+  while (now > pk && !c->peak.compare_exchange_weak(pk, now)) {}
+  void* p = nullptr;
+  const hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    c->used.fetch_sub(bytes);
+    hip_check(e, "hipMalloc");
+  }
+  ctx_ = c;
+  p_ = p;
+  bytes_ = bytes;
+}
+
+void DBuf::release() {
+  if (p_) {
+    (void)hipFree(p_);
+    ctx_->used.fetch_sub(bytes_);
+  }
+  p_ = nullptr;
+  bytes_ = 0;
+}
+
+// ------------------------------------------------------------------ geometry
+Geo make_geo(const pdehip_problem& p) {
+  Geo g;
+  g.nx = p.nx;
+  g.ny = p.ny;
+  g.nz = std::max(1, p.nz);
+  g.is3d = g.nz > 1;
+  g.xmin = p.xmin; g.xmax = p.xmax; g.ymin = p.ymin; g.ymax = p.ymax;
+  g.zmin = p.zmin; g.zmax = p.zmax;
+  g.dx = (p.xmax - p.xmin) / static_cast<double>(g.nx - 1);
+  g.dy = (p.ymax - p.ymin) / static_cast<double>(g.ny - 1);
+  g.dz = g.is3d ? (p.zmax - p.zmin) / static_cast<double>(g.nz - 1) : 1.0;
+  g.n = (long long)g.nx * g.ny * g.nz;
+  return g;
+}
+
+Geo coarsen(const Geo& f) {
+  Geo c = f;
+  c.nx = (f.nx + 1) / 2;
+  c.ny = (f.ny + 1) / 2;
+  c.nz = f.is3d ? (f.nz + 1) / 2 : 1;
+  c.dx = (c.xmax - c.xmin) / static_cast<double>(c.nx - 1);
+  c.dy = (c.ymax - c.ymin) / static_cast<double>(c.ny - 1);
+  c.dz = c.is3d ? (c.zmax - c.zmin) / static_cast<double>(c.nz - 1) : 1.0;
+  c.n = (long long)c.nx * c.ny * c.nz;
+  return c;
+}
+
+static bool can_coarsen(int n) { return n >= 5 && (n % 2 == 1); }  // multigrid.cpp:10-12
+
+// Level rule of the reference V-cycle (multigrid.cpp:150-170), applied to every axis.
+int level_count(const Geo& g0, int max_levels) {
+  int levels = 1;
+  Geo g = g0;
+  while (levels < max_levels) {
+    if (!can_coarsen(g.nx) || !can_coarsen(g.ny) || (g.is3d && !can_coarsen(g.nz))) break;
+    const Geo c = coarsen(g);
+    if (c.nx < 3 || c.ny < 3 || (g.is3d && c.nz < 3)) break;
+    ++levels;
+    g = c;
+    if (c.nx <= 5 || c.ny <= 5 || (g.is3d && c.nz <= 5)) break;
+  }
+  return levels;
+}
+
+static size_t face_count(const Geo& g, int f) {
+  if (f < 2) return (size_t)g.ny * g.nz;
+  if (f < 4) return (size_t)g.nx * g.nz;
+  return (size_t)g.nx * g.ny;
+}
+
+static double eval_expr(const pdehip_expr& e, size_t q, double x, double y, double z) {
+  if (e.samples) return e.samples[q];
+  return e.c + e.cx * x + e.cy * y + e.cz * z;
+}
+
+HostFaces sample_faces(const pdehip_problem& p, const Geo& g) {
+  HostFaces h;
+  h.convention = p.bc_convention;
+  const int nf = g.is3d ? 6 : 4;
+  for (int f = 0; f < 6; ++f) {
+    h.kind[f] = f < nf ? p.bc[f].kind : PDEHIP_BC_DIRICHLET;
+    h.h[f] = f < 2 ? g.dx : (f < 4 ? g.dy : g.dz);
+    if (f >= nf) continue;
+    const size_t m = face_count(g, f);
+    h.value[f].resize(m); h.alpha[f].resize(m); h.beta[f].resize(m); h.gamma[f].resize(m);
+    for (size_t q = 0; q < m; ++q) {
+      double x, y, z;
+      if (f < 2) {
+        const int j = static_cast<int>(q % g.ny), k = static_cast<int>(q / g.ny);
+        x = f == 0 ? g.xmin : g.xmax; y = g.ymin + j * g.dy; z = g.is3d ? g.zmin + k * g.dz : 0.0;
+      } else if (f < 4) {
+        const int i = static_cast<int>(q % g.nx), k = static_cast<int>(q / g.nx);
+        x = g.xmin + i * g.dx; y = f == 2 ? g.ymin : g.ymax; z = g.is3d ? g.zmin + k * g.dz : 0.0;
+      } else {
+        const int i = static_cast<int>(q % g.nx), j = static_cast<int>(q / g.nx);
+        x = g.xmin + i * g.dx; y = g.ymin + j * g.dy; z = f == 4 ? g.zmin : g.zmax;
+      }
+      const pdehip_face_bc& bc = p.bc[f];
+      h.value[f][q] = eval_expr(bc.value, q, x, y, z);
+      h.alpha[f][q] = eval_expr(bc.alpha, q, x, y, z);
+      h.beta[f][q] = eval_expr(bc.beta, q, x, y, z);
+      h.gamma[f][q] = eval_expr(bc.gamma, q, x, y, z);
+    }
+  }
+  return h;
+}
+
+void fold_thetas(const pdehip_problem& p, const Geo& g, bool folded, double theta[6]) {
+  for (int f = 0; f < 6; ++f) theta[f] = 0.0;
+  if (!folded) return;
+  const int nf = g.is3d ? 6 : 4;
+  for (int f = 0; f < nf; ++f) {
+    const pdehip_face_bc& bc = p.bc[f];
+    const double h = f < 2 ? g.dx : (f < 4 ? g.dy : g.dz);
+    if (bc.kind == PDEHIP_BC_NEUMANN) {
+      theta[f] = 1.0;
+    } else if (bc.kind == PDEHIP_BC_ROBIN) {
+      const double al = bc.alpha.c, be = bc.beta.c;
+      const bool high = (f & 1) != 0;
+      if (p.bc_convention == PDEHIP_BCCONV_AXIS && high) theta[f] = -(be / h) / (al - be / h);
+      else theta[f] = (be / h) / (al + be / h);
+    }
+  }
+}
+
+// Reference coefficient formulas (cpu_utils.cpp:293-297; solver.cpp:723-729); upwind
+// first-order convection is an extension.
+static void weights_from(double a, double b, double az, double c, double d, double dz, double e,
+                         const Geo& g, int convection, double w[7]) {
+  const double ax = a / (g.dx * g.dx);
+  const double by = b / (g.dy * g.dy);
+  const double cz = g.is3d ? az / (g.dz * g.dz) : 0.0;
+  double C = g.is3d ? (-2.0 * ax - 2.0 * by - 2.0 * cz + e) : (-2.0 * ax - 2.0 * by + e);
+  double W, E, S, N, B = 0.0, F = 0.0;
+  if (convection == PDEHIP_CONV_UPWIND) {
+    E = ax + std::min(c, 0.0) / g.dx;
+    W = ax - std::max(c, 0.0) / g.dx;
+    N = by + std::min(d, 0.0) / g.dy;
+    S = by - std::max(d, 0.0) / g.dy;
+    C += std::abs(c) / g.dx + std::abs(d) / g.dy;
+    if (g.is3d) {
+      F = cz + std::min(dz, 0.0) / g.dz;
+      B = cz - std::max(dz, 0.0) / g.dz;
+      C += std::abs(dz) / g.dz;
+    }
+  } else {
+    const double cx = c / (2.0 * g.dx), dyc = d / (2.0 * g.dy);
+    E = ax + cx; W = ax - cx; N = by + dyc; S = by - dyc;
+    if (g.is3d) {
+      const double dzc = dz / (2.0 * g.dz);
+      F = cz + dzc; B = cz - dzc;
+    }
+  }
+  w[0] = C; w[1] = W; w[2] = E; w[3] = S; w[4] = N; w[5] = B; w[6] = F;
+}
+
+void const_weights(const pdehip_problem& p, const Geo& g, double w[7]) {
+  weights_from(p.a, p.b, p.az, p.c, p.d, p.dz, p.e, g, p.convection, w);
+}
+
+void DevFaces::upload(Ctx* c, const HostFaces& h, const Geo& g, hipStream_t s) {
+  size_t total = 0;
+  for (int f = 0; f < 6; ++f) total += 4 * h.value[f].size();
+  std::vector<double> host(std::max<size_t>(total, 1));
+  buf.alloc(c, host.size() * sizeof(double));
+  double* base = buf.as<double>();
+  size_t off = 0;
+  for (int f = 0; f < 6; ++f) {
+    fd.kind[f] = h.kind[f];
+    fd.h[f] = h.h[f];
+    const size_t m = h.value[f].size();
+    const std::vector<double>* src[4] = {&h.value[f], &h.alpha[f], &h.beta[f], &h.gamma[f]};
+    const double** dst[4] = {&fd.value[f], &fd.alpha[f], &fd.beta[f], &fd.gamma[f]};
+    for (int t = 0; t < 4; ++t) {
+      std::copy(src[t]->begin(), src[t]->end(), host.begin() + off);
+      *dst[t] = m ? base + off : nullptr;
+      off += m;
+    }
+  }
+  fd.convention = h.convention;
+  (void)g;
+  HIPCHK(hipMemcpyAsync(base, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------------ operators
+static DevOp make_devop(const Geo& g, int kind) {
+  DevOp op{};
+  op.nx = g.nx; op.ny = g.ny; op.nz = g.nz;
+  op.kind = kind;
+  op.sxy = g.sxy();
+  op.n = g.n;
+  return op;
+}
+
+// Fill a level's FP64 operator (and FP32 operator when `mixed`, scaled by sigma).
+static void build_level_ops(Plan& P, Level& lv, const double* kappa64_dev, bool folded) {
+  const pdehip_problem& p = P.prob;
+  lv.op64 = make_devop(lv.g, p.op_kind);
+  fold_thetas(p, lv.g, folded, lv.op64.theta);
+  if (p.op_kind == PDEHIP_OP_CONST) {
+    const_weights(p, lv.g, lv.op64.w);
+  } else if (p.op_kind == PDEHIP_OP_FLUX) {
+    lv.op64.kappa = kappa64_dev;
+    lv.op64.fx = p.a / (lv.g.dx * lv.g.dx);
+    lv.op64.fy = p.a / (lv.g.dy * lv.g.dy);
+    lv.op64.fz = lv.g.is3d ? p.a / (lv.g.dz * lv.g.dz) : 0.0;
+    lv.op64.e = p.e;
+  }
+  lv.op32 = lv.op64;
+  if (P.mixed) {
+    const double s = lv.sigma;
+    for (double& wv : lv.op32.w) wv *= s;
+    lv.op32.fx *= s; lv.op32.fy *= s; lv.op32.fz *= s; lv.op32.e *= s;
+    if (p.op_kind == PDEHIP_OP_FLUX) {
+      lv.kap32.alloc(P.ctx, lv.g.n * sizeof(float));
+      convert(P.stream(), P64, kappa64_dev, P32, lv.kap32.get(), lv.g.n, 1.0);
+      lv.op32.kappa = lv.kap32.get();
+    }
+  }
+}
+
+// Host-side weights of every interior row of a (small) level, for the dense coarse solve.
+static void host_level_weights(Plan& P, const Level& lv, std::vector<double>& W) {
+  const Geo& g = lv.g;
+  const pdehip_problem& p = P.prob;
+  W.assign((size_t)g.n * 7, 0.0);
+  std::vector<double> kap;
+  if (p.op_kind == PDEHIP_OP_FLUX) {
+    kap.resize(g.n);
+    HIPCHK(hipMemcpy(kap.data(), lv.op64.kappa, g.n * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  double wc[7];
+  const_weights(p, g, wc);
+  const int k0 = g.is3d ? 1 : 0, k1 = g.is3d ? g.nz - 1 : 1;
+  for (int k = k0; k < k1; ++k)
+    for (int j = 1; j < g.ny - 1; ++j)
+      for (int i = 1; i < g.nx - 1; ++i) {
+        const long long id = ((long long)k * g.ny + j) * g.nx + i;
+        double w[7];
+        if (p.op_kind == PDEHIP_OP_FLUX) {
+          auto face = [&](long long nb, double fct) {
+            const double kc = kap[id], kn = kap[nb], s = kc + kn;
+            return s > 0.0 ? fct * (2.0 * kc * kn) / s : 0.0;
+          };
+          const double fx = lv.op64.fx, fy = lv.op64.fy, fz = lv.op64.fz;
+          w[1] = face(id - 1, fx); w[2] = face(id + 1, fx);
+          w[3] = face(id - g.nx, fy); w[4] = face(id + g.nx, fy);
+          double sum = w[1] + w[2] + w[3] + w[4];
+          w[5] = w[6] = 0.0;
+          if (g.is3d) { w[5] = face(id - g.sxy(), fz); w[6] = face(id + g.sxy(), fz); sum += w[5] + w[6]; }
+          w[0] = lv.op64.e - sum;
+        } else {
+          std::copy(wc, wc + 7, w);
+        }
+        const double* th = lv.op64.theta;
+        if (i == 1) w[0] += w[1] * th[0];
+        if (i == g.nx - 2) w[0] += w[2] * th[1];
+        if (j == 1) w[0] += w[3] * th[2];
+        if (j == g.ny - 2) w[0] += w[4] * th[3];
+        if (g.is3d) {
+          if (k == 1) w[0] += w[5] * th[4];
+          if (k == g.nz - 2) w[0] += w[6] * th[5];
+        }
+        for (int t = 0; t < 7; ++t) W[(size_t)id * 7 + t] = w[t];
+      }
+}
+
+// Dense inverse of the coarsest level's interior operator (scaled by sigma in mixed mode).
+static bool build_dense(Plan& P, Level& lv) {
+  const Geo& g = lv.g;
+  std::vector<long> pos(g.n, -1);
+  std::vector<int> map;
+  const int k0 = g.is3d ? 1 : 0, k1 = g.is3d ? g.nz - 1 : 1;
+  for (int k = k0; k < k1; ++k)
+    for (int j = 1; j < g.ny - 1; ++j)
+      for (int i = 1; i < g.nx - 1; ++i) {
+        const long long id = ((long long)k * g.ny + j) * g.nx + i;
+        pos[id] = static_cast<long>(map.size());
+        map.push_back(static_cast<int>(id));
+      }
+  const size_t m = map.size();
+  if (m == 0 || m > 1500) return false;
+  std::vector<double> W;
+  host_level_weights(P, lv, W);
+  const double s = P.mixed ? lv.sigma : 1.0;
+  std::vector<double> A(m * m, 0.0), inv(m * m, 0.0);
+  for (size_t r = 0; r < m; ++r) {
+    const long long id = map[r];
+    const double* w = &W[(size_t)id * 7];
+    A[r * m + r] = s * w[0];
+    auto put = [&](long long nb, double wv) { if (pos[nb] >= 0) A[r * m + pos[nb]] += s * wv; };
+    put(id - 1, w[1]); put(id + 1, w[2]); put(id - g.nx, w[3]); put(id + g.nx, w[4]);
+    if (g.is3d) { put(id - g.sxy(), w[5]); put(id + g.sxy(), w[6]); }
+    inv[r * m + r] = 1.0;
+  }
+  for (size_t c = 0; c < m; ++c) {  // Gauss-Jordan with partial pivoting
+    size_t piv = c;
+    for (size_t r = c + 1; r < m; ++r) if (std::abs(A[r * m + c]) > std::abs(A[piv * m + c])) piv = r;
+    if (std::abs(A[piv * m + c]) < 1e-300) return false;
+    if (piv != c)
+      for (size_t q = 0; q < m; ++q) { std::swap(A[c * m + q], A[piv * m + q]); std::swap(inv[c * m + q], inv[piv * m + q]); }
+    const double d = 1.0 / A[c * m + c];
+    for (size_t q = 0; q < m; ++q) { A[c * m + q] *= d; inv[c * m + q] *= d; }
+    for (size_t r = 0; r < m; ++r) {
+      if (r == c) continue;
+      const double fct = A[r * m + c];
+      if (fct == 0.0) continue;
+      for (size_t q = 0; q < m; ++q) { A[r * m + q] -= fct * A[c * m + q]; inv[r * m + q] -= fct * inv[c * m + q]; }
+    }
+  }
+  lv.m = static_cast<int>(m);
+  lv.ainv.alloc(P.ctx, m * m * sizeof(double));
+  lv.map.alloc(P.ctx, m * sizeof(int));
+  HIPCHK(hipMemcpy(lv.ainv.get(), inv.data(), m * m * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(lv.map.get(), map.data(), m * sizeof(int), hipMemcpyHostToDevice));
+  return true;
+}
+
+// ------------------------------------------------------------------ plan creation
+Plan::~Plan() {
+  if (hsc) (void)hipHostFree(hsc);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+}
+
+static bool all_dirichlet(const pdehip_problem& p, const Geo& g) {
+  for (int f = 0; f < (g.is3d ? 6 : 4); ++f)
+    if (p.bc[f].kind != PDEHIP_BC_DIRICHLET) return false;
+  return true;
+}
+
+static void validate(const pdehip_problem& p, const pdehip_options& o) {
+  if (p.nx < 3 || p.ny < 3 || (p.nz > 1 && p.nz < 3))
+    throw Error(PDEHIP_ERR_INVALID, "grid must be at least 3 nodes per axis");
+  if (p.xmax <= p.xmin || p.ymax <= p.ymin || (p.nz > 1 && p.zmax <= p.zmin))
+    throw Error(PDEHIP_ERR_INVALID, "domain bounds are invalid");
+  if (p.op_kind == PDEHIP_OP_FLUX && !p.kappa)
+    throw Error(PDEHIP_ERR_INVALID, "FLUX operator needs a kappa field");
+  if (o.method < PDEHIP_JACOBI || o.method > PDEHIP_MGPCG)
+    throw Error(PDEHIP_ERR_INVALID, "unknown method");
+  if (o.method == PDEHIP_SOR && (o.sor_omega <= 0.0 || o.sor_omega >= 2.0))
+    throw Error(PDEHIP_ERR_INVALID, "SOR omega must be in (0,2)");
+}
+
+static void upload(Ctx* c, DBuf& b, const double* host, size_t n, hipStream_t s) {
+  b.alloc(c, n * sizeof(double));
+  HIPCHK(hipMemcpyAsync(b.get(), host, n * sizeof(double), hipMemcpyHostToDevice, s));
+}
+
+std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehip_options& o) {
+  validate(p, o);
+  auto P = std::make_unique<Plan>();
+  P->ctx = ctx;
+  P->prob = p;
+  P->opt = o;
+  P->g = make_geo(p);
+  const Geo& g = P->g;
+  const hipStream_t s = ctx->stream;
+  const int m = o.method;
+  const bool mg = (m == PDEHIP_MG && !o.mg_single_cycle) || m == PDEHIP_MGPCG;
+  const bool krylov = (m == PDEHIP_CG || m == PDEHIP_BICGSTAB || m == PDEHIP_GMRES);
+  P->folded = mg || (krylov && o.bc_rows == PDEHIP_ROWS_FOLDED);
+  if (krylov && !P->folded && !all_dirichlet(p, g))
+    throw Error(PDEHIP_ERR_UNSUPPORTED, "Krylov methods require Dirichlet boundaries on all sides");
+  P->mixed = (m == PDEHIP_MGPCG) && o.precision == PDEHIP_MIXED;
+  if (mg && p.op_kind == PDEHIP_OP_POINTWISE)
+    throw Error(PDEHIP_ERR_UNSUPPORTED, "multigrid needs constant or FLUX coefficients");
+  if (m == PDEHIP_MG && o.mg_single_cycle &&
+      (g.is3d || !all_dirichlet(p, g) || p.op_kind != PDEHIP_OP_CONST || p.c != 0.0 || p.d != 0.0))
+    throw Error(PDEHIP_ERR_UNSUPPORTED, "Multigrid requires Dirichlet boundaries on all sides");
+  if (mg || (m == PDEHIP_MG)) {
+    if (!can_coarsen(g.nx) || !can_coarsen(g.ny) || (g.is3d && !can_coarsen(g.nz)))
+      throw Error(PDEHIP_ERR_UNSUPPORTED, "multigrid requires odd grid dimensions of at least 5");
+  }
+  for (int f = 0; f < 6 && P->folded; ++f) {
+    const pdehip_face_bc& bc = p.bc[f];
+    if (bc.kind == PDEHIP_BC_ROBIN && (bc.alpha.samples || bc.beta.samples || bc.alpha.cx != 0 ||
+                                       bc.alpha.cy != 0 || bc.alpha.cz != 0 || bc.beta.cx != 0 ||
+                                       bc.beta.cy != 0 || bc.beta.cz != 0))
+      throw Error(PDEHIP_ERR_UNSUPPORTED, "folded Robin rows need constant alpha and beta per face");
+  }
+  P->nl.n = std::min(p.n_nonlinear, PDEHIP_MAX_NONLINEAR);
+  for (int t = 0; t < P->nl.n; ++t) P->nl.t[t] = p.nonlinear[t];
+  if (P->nl.n && (krylov || mg || m == PDEHIP_MG))
+    throw Error(PDEHIP_ERR_UNSUPPORTED, "linear PDE required for this method");
+
+  P->hfaces = sample_faces(p, g);
+  P->faces.upload(ctx, P->hfaces, g, s);
+
+  // ---- fine level operator
+  const int nl = mg ? level_count(g, std::max(1, o.mg_max_levels))
+                    : (m == PDEHIP_MG ? level_count(g, std::max(1, o.mg_max_levels)) : 1);
+  P->L.resize(nl);
+  P->L[0].g = g;
+  if (p.op_kind == PDEHIP_OP_FLUX) upload(ctx, P->L[0].kap64, p.kappa, g.n, s);
+  P->L[0].sigma = 1.0;
+  build_level_ops(*P, P->L[0], P->L[0].kap64.as<double>(), P->folded);
+  if (p.op_kind == PDEHIP_OP_POINTWISE) {
+    // per-node weights (C, W, E, S, N, B, F) from the coefficient fields (relaxation.cpp:187-201)
+    std::vector<double> W((size_t)g.n * 7, 0.0);
+    for (int k = 0; k < g.nz; ++k)
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) {
+          const long long id = ((long long)k * g.ny + j) * g.nx + i;
+          auto cf = [&](int c, double v) { return p.coef[c] ? p.coef[c][id] : v; };
+          double w[7];
+          weights_from(cf(PDEHIP_COEF_A, p.a), cf(PDEHIP_COEF_B, p.b), cf(PDEHIP_COEF_AZ, p.az),
+                       cf(PDEHIP_COEF_C, p.c), cf(PDEHIP_COEF_D, p.d), cf(PDEHIP_COEF_DZ, p.dz),
+                       cf(PDEHIP_COEF_E, p.e), g, p.convection, w);
+          for (int t = 0; t < 7; ++t) W[(size_t)t * g.n + id] = w[t];
+        }
+    upload(ctx, P->L[0].pw, W.data(), W.size(), s);
+    P->L[0].op64.pw = P->L[0].pw.as<double>();
+    P->L[0].op32 = P->L[0].op64;
+  }
+
+  // ---- source, boundary data, right-hand side
+  P->f_const = p.f;
+  if (p.f_field) {
+    upload(ctx, P->f, p.f_field, g.n, s);
+    P->hf.assign(p.f_field, p.f_field + g.n);
+  }
+  const Launch L0{s, nullptr};
+  P->x0.alloc(ctx, g.n * sizeof(double));
+  fill(s, P64, P->x0.get(), g.n, 0.0);
+  if (P->folded) closures_nr(L0, P->op(), P->faces.fd, P->x0.as<double>());
+  closures_dirichlet(L0, P->op(), P->faces.fd, P->x0.as<double>());
+  P->bvec.alloc(ctx, g.n * sizeof(double));
+  build_rhs(s, P->op(), P->f.as<double>(), P->f_const, P->x0.as<double>(), P->bvec.as<double>());
+  // Krylov on reference rows starts from x = 0 (krylov.cpp:25); folded solves start from the
+  // boundary data so boundary rows of the residual vanish.
+  if (!P->folded) fill(s, P64, P->x0.get(), g.n, 0.0);
+  P->x.alloc(ctx, g.n * sizeof(double));
+
+  // ---- hierarchy
+  if (nl > 1 || mg) {
+    for (int l = 1; l < nl; ++l) {
+      Level& f = P->L[l - 1];
+      Level& c = P->L[l];
+      c.g = coarsen(f.g);
+      c.sigma = P->mixed ? c.g.dx * c.g.dx : 1.0;
+      if (p.op_kind == PDEHIP_OP_FLUX) {
+        c.kap64.alloc(ctx, c.g.n * sizeof(double));
+        const DevOp fo = make_devop(f.g, p.op_kind), co = make_devop(c.g, p.op_kind);
+        restrict_kappa(s, fo, f.kap64.as<double>(), co, c.kap64.as<double>());
+      }
+      build_level_ops(*P, c, c.kap64.as<double>(), P->folded);
+    }
+    for (int l = 0; l < nl; ++l) {
+      Level& lv = P->L[l];
+      lv.prec = P->mixed ? (l == 0 ? P32 : P16) : P64;
+      const size_t eb = prec_bytes(lv.prec);
+      lv.r.alloc(ctx, lv.g.n * eb);
+      fill(s, lv.prec, lv.r.get(), lv.g.n, 0.0);
+      if (l > 0) {
+        lv.u.alloc(ctx, lv.g.n * eb);
+        lv.b.alloc(ctx, lv.g.n * eb);
+        fill(s, lv.prec, lv.u.get(), lv.g.n, 0.0);
+        fill(s, lv.prec, lv.b.get(), lv.g.n, 0.0);
+      }
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    if (mg) build_dense(*P, P->L[nl - 1]);
+  }
+
+  // ---- work vectors
+  const size_t n8 = g.n * sizeof(double), n4 = g.n * sizeof(float);
+  auto z8 = [&](DBuf& b) { b.alloc(ctx, n8); fill(s, P64, b.get(), g.n, 0.0); };
+  auto z4 = [&](DBuf& b) { b.alloc(ctx, n4); fill(s, P32, b.get(), g.n, 0.0); };
+  if (m == PDEHIP_CG || m == PDEHIP_MGPCG) {
+    z8(P->r); z8(P->p); z8(P->q);
+    if (m == PDEHIP_MGPCG && !P->mixed) z8(P->z);
+    if (P->mixed) { z4(P->r32); z4(P->x32); z4(P->p32); z4(P->q32); z4(P->z32); }
+  } else if (m == PDEHIP_BICGSTAB) {
+    z8(P->r); z8(P->p); z8(P->q); z8(P->z); z8(P->w1); z8(P->w2); z8(P->w3);
+    if (o.precond == PDEHIP_PC_JACOBI) z8(P->w4);
+  } else if (m == PDEHIP_GMRES) {
+    const int mm = std::max(1, std::min(o.gmres_restart, std::max(1, o.max_iter)));
+    z8(P->r); z8(P->w1); z8(P->w2); z8(P->w3);
+    P->V.alloc(ctx, (size_t)(mm + 1) * n8);
+    P->H.alloc(ctx, (size_t)(mm + 1) * mm * sizeof(double));
+    P->cs.alloc(ctx, mm * sizeof(double));
+    P->sn.alloc(ctx, mm * sizeof(double));
+    P->gv.alloc(ctx, (mm + 1) * sizeof(double));
+    P->yv.alloc(ctx, mm * sizeof(double));
+    P->hcb.alloc(ctx, 2 * 64 * sizeof(double));
+    HIPCHK(hipMemsetAsync(P->H.get(), 0, P->H.bytes(), s));
+  } else if (m == PDEHIP_MG) {
+    z8(P->r); z8(P->q);
+  } else if (m == PDEHIP_JACOBI) {
+    z8(P->r);
+  }
+
+  // ---- reductions and scalars
+  int np = std::max(partial_blocks(P->op()), vec_blocks(g.n));
+  np = std::max(np, relax_blocks(P->op()));
+  np = std::max(np, krylov_blocks(g.n));
+  np = std::max(np, resnorm_blocks(P->op()));
+  np = std::max(np, maxabs_blocks(g.n));
+  P->npart = np;
+  const int gm = (m == PDEHIP_GMRES) ? std::max(1, std::min(o.gmres_restart, std::max(1, o.max_iter))) + 1 : 1;
+  P->part.alloc(ctx, (size_t)np * gm * sizeof(double));
+  P->part2.alloc(ctx, (size_t)np * sizeof(double));
+  P->part3.alloc(ctx, (size_t)np * sizeof(double) + 64);
+  P->scal.alloc(ctx, 2 * sizeof(KScalars));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&P->hsc), 2 * sizeof(KScalars)));
+  HIPCHK(hipEventCreate(&P->ev0));
+  HIPCHK(hipEventCreate(&P->ev1));
+  HIPCHK(hipStreamSynchronize(s));
+  P->bytes_held = ctx->used.load();
+  return P;
+}
+
+void plan_set_source(Plan& P, const double* f_host) {
+  const hipStream_t s = P.stream();
+  if (f_host) {
+    if (!P.f) P.f.alloc(P.ctx, P.g.n * sizeof(double));
+    HIPCHK(hipMemcpyAsync(P.f.get(), f_host, P.g.n * sizeof(double), hipMemcpyHostToDevice, s));
+  } else {
+    P.f.release();
+  }
+  build_rhs(s, P.op(), P.f.as<double>(), P.f_const, P.x0.as<double>(), P.bvec.as<double>());
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------------ scalars helpers
+static void init_scalars(Plan& P, int i, double eps) {
+  KScalars k{};
+  k.eps = eps;
+  k.scale = 1.0;
+  k.inv_scale = 1.0;
+  P.hsc[i] = k;
+  HIPCHK(hipMemcpyAsync(P.sc(i), &P.hsc[i], sizeof(KScalars), hipMemcpyHostToDevice, P.stream()));
+}
+
+static const KScalars& pull(Plan& P, int i) {
+  HIPCHK(hipMemcpyAsync(&P.hsc[i], P.sc(i), sizeof(KScalars), hipMemcpyDeviceToHost, P.stream()));
+  HIPCHK(hipStreamSynchronize(P.stream()));
+  return P.hsc[i];
+}
+
+static bool cancelled(const Plan& P) { return P.opt.cancel && *P.opt.cancel; }
+
+static int check_every(const Plan& P) {
+  if (P.opt.check_every > 0) return P.opt.check_every;
+  if (P.g.n >= 2000000) return 1;
+  if (P.g.n >= 200000) return 4;
+  return 16;
+}
+
+// ------------------------------------------------------------------ multigrid cycle
+// One V-cycle on b0 (level-0 right-hand side) into u0 (level-0 correction, zero on entry).
+static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0) {
+  const int nl = static_cast<int>(P.L.size());
+  const int pre = std::max(0, P.opt.mg_pre), post = std::max(0, P.opt.mg_post);
+  KScalars* sc = const_cast<KScalars*>(L.sc);
+  for (int l = 0; l < nl - 1; ++l) {
+    Level& f = P.L[l];
+    Level& c = P.L[l + 1];
+    const DevOp& fo = P.mixed ? f.op32 : f.op64;
+    const DevOp& co = P.mixed ? c.op32 : c.op64;
+    const void* bl = l == 0 ? b0 : f.b.get();
+    void* ul = l == 0 ? u0 : f.u.get();
+    for (int it = 0; it < pre; ++it) { rbgs(L, fo, f.prec, bl, ul, 0); rbgs(L, fo, f.prec, bl, ul, 1); }
+    residual(L, fo, f.prec, bl, ul, f.r.get(), nullptr);
+    restrict_fw(L, fo, f.prec, f.r.get(), co, c.prec, c.b.get(), 1.0,
+                (P.mixed && l == 0) ? &sc->scale : nullptr);
+    fold_transpose(L, co, c.prec, c.b.get());
+    fill(L.stream, c.prec, c.u.get(), c.g.n, 0.0);
+  }
+  Level& cl = P.L[nl - 1];
+  const DevOp& cop = P.mixed ? cl.op32 : cl.op64;
+  void* cu = nl == 1 ? u0 : cl.u.get();
+  const void* cb = nl == 1 ? b0 : cl.b.get();
+  if (cl.m > 0) {
+    coarse_dense(L, cl.prec, cl.ainv.as<double>(), cl.map.as<int>(), cl.m, cb, cu);
+  } else {
+    const int half = std::max(1, P.opt.mg_coarse_iters / 2);
+    for (int it = 0; it < half; ++it) { rbgs(L, cop, cl.prec, cb, cu, 0); rbgs(L, cop, cl.prec, cb, cu, 1); }
+    for (int it = 0; it < half; ++it) { rbgs(L, cop, cl.prec, cb, cu, 1); rbgs(L, cop, cl.prec, cb, cu, 0); }
+  }
+  for (int l = nl - 2; l >= 0; --l) {
+    Level& f = P.L[l];
+    Level& c = P.L[l + 1];
+    const DevOp& fo = P.mixed ? f.op32 : f.op64;
+    const DevOp& co = P.mixed ? c.op32 : c.op64;
+    const void* bl = l == 0 ? b0 : f.b.get();
+    void* ul = l == 0 ? u0 : f.u.get();
+    extend_closure(L, co, c.prec, c.u.get());
+    prolong_add(L, co, c.prec, c.u.get(), fo, f.prec, ul, c.sigma / f.sigma,
+                (P.mixed && l == 0) ? &sc->inv_scale : nullptr);
+    for (int it = 0; it < post; ++it) { rbgs(L, fo, f.prec, bl, ul, 1); rbgs(L, fo, f.prec, bl, ul, 0); }
+  }
+}
+
+// ------------------------------------------------------------------ solvers
+static void report_from(const KScalars& k, pdehip_report* rep) {
+  rep->iterations = k.iter;
+  rep->final_metric = k.metric;
+  rep->converged = k.done && !k.breakdown;
+}
+
+// PCG / CG on level-0 FP64 vectors.  use_mg: MG preconditioner; else plain CG
+// (reference CgSolve, krylov.cpp:14-80).
+static void solve_cg(Plan& P, bool use_mg, pdehip_report* rep) {
+  const hipStream_t s = P.stream();
+  const Geo& g = P.g;
+  const DevOp& op = P.op();
+  const long long n = g.n;
+  init_scalars(P, 0, use_mg ? 1e-300 : 1e-30);
+  KScalars* sc = P.sc(0);
+  const Launch L{s, sc};
+  double* part = P.part.as<double>();
+  HIPCHK(hipMemcpyAsync(P.x.get(), P.x0.get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  const Launch L0{s, nullptr};
+  residual(L0, op, P64, P.bvec.get(), P.x.get(), P.r.get(), part);
+  const int np0 = partial_blocks(op);
+  finalize(L0, sc, part, np0, FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
+  void* z = use_mg ? P.z.get() : P.r.get();
+  if (use_mg) {
+    fill(s, P64, z, n, 0.0);
+    vcycle(P, L, P.r.get(), z);
+    dot(L, P64, P.r.get(), z, n, part);
+    finalize(L, sc, part, vec_blocks(n), FIN_CG_INIT, 0, 0, nullptr);
+  } else {
+    finalize(L, sc, part, np0, FIN_CG_INIT, 0, 0, nullptr);
+  }
+  HIPCHK(hipMemcpyAsync(P.p.get(), z, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  const int every = check_every(P);
+  const int max_iter = std::max(1, P.opt.max_iter);
+  int launched = 0;
+  while (launched < max_iter) {
+    const int chunk = std::min(every, max_iter - launched);
+    for (int c = 0; c < chunk; ++c) {
+      apply(L, op, P64, P.p.get(), P.q.get(), part);
+      finalize(L, sc, part, partial_blocks(op), FIN_CG_ALPHA, 0, 0, nullptr);
+      cg_update(L, P64, P.x.get(), P.r.get(), P.p.get(), P.q.get(), n, part, nullptr);
+      finalize(L, sc, part, vec_blocks(n), FIN_CG_RR, 0, 0, nullptr);
+      if (use_mg) {
+        fill(s, P64, z, n, 0.0);
+        vcycle(P, L, P.r.get(), z);
+        dot(L, P64, P.r.get(), z, n, part);
+      }
+      finalize(L, sc, part, vec_blocks(n), FIN_CG_BETA, 0, 0, nullptr);
+      cg_xpay(L, P64, z, P.p.get(), n);
+    }
+    launched += chunk;
+    const KScalars& k = pull(P, 0);
+    if (k.done || k.breakdown || cancelled(P)) break;
+  }
+  const KScalars& k = pull(P, 0);
+  report_from(k, rep);
+  if (cancelled(P) && !k.done) throw Error(PDEHIP_ERR_CANCELLED, "solve cancelled");
+  if (k.breakdown) rep->status = PDEHIP_ERR_BREAKDOWN;
+  else if (!k.done) rep->status = PDEHIP_ERR_NOT_CONVERGED;
+}
+
+// Mixed precision MG-PCG: FP64 iterative refinement around FP32 MG-PCG inner solves whose
+// V-cycles keep the coarse levels in FP16 (per-level sigma = h^2 scaling plus a per-cycle
+// input normalisation).  The outer residual b - A x, the solution accumulator, the outer
+// norms and the coarsest-level solve stay FP64.
+static void solve_mixed(Plan& P, pdehip_report* rep) {
+  const hipStream_t s = P.stream();
+  const Geo& g = P.g;
+  const long long n = g.n;
+  const DevOp& op64 = P.L[0].op64;
+  const DevOp& op32 = P.L[0].op32;
+  double* part = P.part.as<double>();
+  double* part2 = P.part2.as<double>();
+  init_scalars(P, 0, 1e-300);
+  init_scalars(P, 1, 1e-300);
+  KScalars* so = P.sc(0);
+  KScalars* si = P.sc(1);
+  const Launch L0{s, nullptr};
+  const Launch Li{s, si};
+  HIPCHK(hipMemcpyAsync(P.x.get(), P.x0.get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  residual(L0, op64, P64, P.bvec.get(), P.x.get(), P.r.get(), part);
+  finalize(L0, so, part, partial_blocks(op64), FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
+  double inner_tol = P.opt.inner_tol > 0 ? P.opt.inner_tol : 1e-4;
+  const int max_ref = 30;
+  int inner_total = 0, refinements = 0;
+  const int every = check_every(P);
+  const int max_iter = std::max(1, P.opt.max_iter);
+  for (int ref = 0; ref < max_ref; ++ref) {
+    const KScalars& ko = pull(P, 0);
+    if (ko.done || cancelled(P)) break;
+    const double rnorm = ko.metric;
+    ++refinements;
+    // inner: A e = r / ||r||, e0 = 0
+    to_f32_scaled(s, P.r.as<double>(), P.r32.as<float>(), n, &so->metric);
+    fill(s, P32, P.x32.get(), n, 0.0);
+    KScalars ki{};
+    ki.eps = 1e-300; ki.scale = 1.0; ki.inv_scale = 1.0;
+    P.hsc[1] = ki;
+    HIPCHK(hipMemcpyAsync(si, &P.hsc[1], sizeof(KScalars), hipMemcpyHostToDevice, s));
+    dot(L0, P32, P.r32.get(), P.r32.get(), n, part);
+    finalize(L0, si, part, vec_blocks(n), FIN_R0, inner_tol, PDEHIP_TOL_REL, nullptr);
+    maxabs32(s, P.r32.as<float>(), n, part2);
+    fin_scale(s, part2, maxabs_blocks(n), si);
+    fill(s, P32, P.z32.get(), n, 0.0);
+    vcycle(P, Li, P.r32.get(), P.z32.get());
+    dot(Li, P32, P.r32.get(), P.z32.get(), n, part);
+    finalize(Li, si, part, vec_blocks(n), FIN_CG_INIT, 0, 0, nullptr);
+    HIPCHK(hipMemcpyAsync(P.p32.get(), P.z32.get(), n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    int launched = 0;
+    while (launched < max_iter) {
+      const int chunk = std::min(every, max_iter - launched);
+      for (int c = 0; c < chunk; ++c) {
+        apply(Li, op32, P32, P.p32.get(), P.q32.get(), part);
+        finalize(Li, si, part, partial_blocks(op32), FIN_CG_ALPHA, 0, 0, nullptr);
+        cg_update(Li, P32, P.x32.get(), P.r32.get(), P.p32.get(), P.q32.get(), n, part, part2);
+        finalize(Li, si, part, vec_blocks(n), FIN_CG_RR, 0, 0, nullptr);
+        fin_scale(s, part2, vec_blocks(n), si);
+        fill(s, P32, P.z32.get(), n, 0.0);
+        vcycle(P, Li, P.r32.get(), P.z32.get());
+        dot(Li, P32, P.r32.get(), P.z32.get(), n, part);
+        finalize(Li, si, part, vec_blocks(n), FIN_CG_BETA, 0, 0, nullptr);
+        cg_xpay(Li, P32, P.z32.get(), P.p32.get(), n);
+      }
+      launched += chunk;
+      const KScalars& k = pull(P, 1);
+      if (k.done || k.breakdown || cancelled(P)) break;
+    }
+    const KScalars& k = pull(P, 1);
+    inner_total += k.iter;
+    add_f32(s, P.x.as<double>(), P.x32.as<float>(), n, nullptr, rnorm);
+    residual(L0, op64, P64, P.bvec.get(), P.x.get(), P.r.get(), part);
+    finalize(L0, so, part, partial_blocks(op64), FIN_CG_RR, 0, 0, nullptr);
+    if (k.breakdown) break;
+  }
+  const KScalars& ko = pull(P, 0);
+  rep->iterations = refinements;
+  rep->refinements = refinements;
+  rep->inner_iterations = inner_total;
+  rep->final_metric = ko.metric;
+  rep->converged = ko.done;
+  if (cancelled(P) && !ko.done) throw Error(PDEHIP_ERR_CANCELLED, "solve cancelled");
+  if (!ko.done) rep->status = PDEHIP_ERR_NOT_CONVERGED;
+}
+
+// Iterated V-cycles (extension) until ||b - A u||_2 meets the target.
+static void solve_mg_iter(Plan& P, pdehip_report* rep) {
+  const hipStream_t s = P.stream();
+  const long long n = P.g.n;
+  const DevOp& op = P.op();
+  init_scalars(P, 0, 1e-300);
+  KScalars* sc = P.sc(0);
+  const Launch L0{s, nullptr};
+  const Launch L{s, sc};
+  double* part = P.part.as<double>();
+  HIPCHK(hipMemcpyAsync(P.x.get(), P.x0.get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  residual(L0, op, P64, P.bvec.get(), P.x.get(), P.r.get(), part);
+  finalize(L0, sc, part, partial_blocks(op), FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
+  const int max_iter = std::max(1, P.opt.max_iter);
+  const int every = check_every(P);
+  int launched = 0;
+  while (launched < max_iter) {
+    const int chunk = std::min(every, max_iter - launched);
+    for (int c = 0; c < chunk; ++c) {
+      // correction form: e = V(r) from a zero guess, x += e
+      fill(s, P64, P.q.get(), n, 0.0);
+      vcycle(P, L, P.r.get(), P.q.get());
+      axpy_dev(L, P64, P.x.get(), P.q.get(), n, nullptr, 1.0);
+      residual(L, op, P64, P.bvec.get(), P.x.get(), P.r.get(), part);
+      finalize(L, sc, part, partial_blocks(op), FIN_CG_RR, 0, 0, nullptr);
+    }
+    launched += chunk;
+    const KScalars& k = pull(P, 0);
+    if (k.done || cancelled(P)) break;
+  }
+  const KScalars& k = pull(P, 0);
+  report_from(k, rep);
+  if (!k.done) rep->status = PDEHIP_ERR_NOT_CONVERGED;
+}
+
+// The reference's single V-cycle (MultigridVcyclePoisson, multigrid.cpp:121-222): 2D, constant
+// a/b/e, lexicographic Gauss-Seidel (wavefront order on the device), one cycle.
+static void solve_mg_reference(Plan& P, pdehip_report* rep) {
+  const hipStream_t s = P.stream();
+  const pdehip_problem& p = P.prob;
+  const Launch L0{s, nullptr};
+  const int nl = static_cast<int>(P.L.size());
+  // levels' constant coefficients at their spacing
+  std::vector<Geo> G(nl);
+  G[0] = P.g;
+  for (int l = 1; l < nl; ++l) G[l] = coarsen(G[l - 1]);
+  std::vector<DBuf> U(nl), B(nl), R(nl);
+  for (int l = 0; l < nl; ++l) {
+    const size_t bytes = G[l].n * sizeof(double);
+    if (l > 0) { U[l].alloc(P.ctx, bytes); B[l].alloc(P.ctx, bytes); fill(s, P64, U[l].get(), G[l].n, 0.0); fill(s, P64, B[l].get(), G[l].n, 0.0); }
+    R[l].alloc(P.ctx, bytes);
+  }
+  // xvec = Dirichlet values on a zero grid (solver.cpp:573-580)
+  fill(s, P64, P.x.get(), P.g.n, 0.0);
+  closures_dirichlet(L0, P.op(), P.faces.fd, P.x.as<double>());
+  auto uptr = [&](int l) { return l == 0 ? P.x.as<double>() : U[l].as<double>(); };
+  auto bptr = [&](int l) { return l == 0 ? P.bvec.as<double>() : B[l].as<double>(); };
+  auto coef = [&](int l, double* ax, double* by, double* ct) {
+    *ax = p.a / (G[l].dx * G[l].dx);
+    *by = p.b / (G[l].dy * G[l].dy);
+    *ct = -2.0 * (*ax) - 2.0 * (*by) + p.e;
+  };
+  for (int l = 0; l < nl; ++l) {
+    double ax, by, ct;
+    coef(l, &ax, &by, &ct);
+    if (std::abs(ct) < 1e-12) throw Error(PDEHIP_ERR_DEGENERATE, "CPU multigrid solver failed");
+  }
+  for (int l = 0; l < nl - 1; ++l) {
+    double ax, by, ct;
+    coef(l, &ax, &by, &ct);
+    lex_gs(s, G[l].nx, G[l].ny, ax, by, ct, bptr(l), uptr(l), std::max(0, P.opt.mg_pre));
+    DevOp fo = make_devop(G[l], PDEHIP_OP_CONST);
+    const_weights(p, G[l], fo.w);
+    DevOp co = make_devop(G[l + 1], PDEHIP_OP_CONST);
+    residual(L0, fo, P64, bptr(l), uptr(l), R[l].get(), nullptr);
+    restrict_fw(L0, fo, P64, R[l].get(), co, P64, B[l + 1].get(), 1.0, nullptr);
+    fold_transpose(L0, co, P64, B[l + 1].get());   // theta = 0: clears the coarse boundary
+    fill(s, P64, U[l + 1].get(), G[l + 1].n, 0.0);
+  }
+  {
+    double ax, by, ct;
+    coef(nl - 1, &ax, &by, &ct);
+    lex_gs(s, G[nl - 1].nx, G[nl - 1].ny, ax, by, ct, bptr(nl - 1), uptr(nl - 1),
+           std::max(1, P.opt.mg_coarse_iters));
+  }
+  for (int l = nl - 2; l >= 0; --l) {
+    DevOp fo = make_devop(G[l], PDEHIP_OP_CONST);
+    DevOp co = make_devop(G[l + 1], PDEHIP_OP_CONST);
+    prolong_add(L0, co, P64, U[l + 1].get(), fo, P64, uptr(l), 1.0, nullptr);
+    double ax, by, ct;
+    coef(l, &ax, &by, &ct);
+    lex_gs(s, G[l].nx, G[l].ny, ax, by, ct, bptr(l), uptr(l), std::max(0, P.opt.mg_post));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  rep->iterations = 1;
+  rep->converged = 1;
+}
+
+// Jacobi / Gauss-Seidel / SOR with the reference's closure order (relaxation.cpp:310-574,
+// 52-308; solver.cpp:751-981).  Gauss-Seidel and SOR sweep red then black on the device.
+static void solve_relax(Plan& P, pdehip_report* rep) {
+  const hipStream_t s = P.stream();
+  const Geo& g = P.g;
+  const long long n = g.n;
+  const DevOp& op = P.op();
+  init_scalars(P, 0, 0.0);
+  KScalars* sc = P.sc(0);
+  const Launch L{s, sc};
+  double* part = P.part.as<double>();
+  double* maxslot = P.part3.as<double>() + P.npart;
+  int* degenerate = reinterpret_cast<int*>(P.part3.as<double>() + P.npart + 1);
+  HIPCHK(hipMemsetAsync(degenerate, 0, sizeof(int), s));
+  const FaceData& fd = P.faces.fd;
+  double* grid = P.x.as<double>();
+  double* next = P.r.get() ? P.r.as<double>() : nullptr;
+  fill(s, P64, grid, n, 0.0);
+  const Launch L0{s, nullptr};
+  closures_dirichlet(L0, op, fd, grid);
+  if (g.is3d) { closures_nr(L0, op, fd, grid); closures_dirichlet(L0, op, fd, grid); }
+  const bool jac = P.opt.method == PDEHIP_JACOBI;
+  if (jac) HIPCHK(hipMemcpyAsync(next, grid, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  const double omega = P.opt.method == PDEHIP_SOR ? P.opt.sor_omega : 1.0;
+  const int max_iter = std::max(1, P.opt.max_iter);
+  const int every = std::max(check_every(P), 32);
+  const double tol = P.opt.tol;
+  // relaxation stop rule: max |du| < tol, evaluated on the device each sweep
+  int launched = 0;
+  std::vector<double*> bufs = {grid, next};
+  int cur = 0;
+  while (launched < max_iter) {
+    const int chunk = std::min(every, max_iter - launched);
+    for (int c = 0; c < chunk; ++c) {
+      double* u = bufs[cur];
+      closures_nr(L, op, fd, u);
+      if (jac) {
+        double* v = bufs[cur ^ 1];
+        jacobi_ref(L, op, P.f.as<double>(), P.f_const, P.nl, u, v, part, degenerate);
+        closures_nr(L, op, fd, v);
+        closures_dirichlet(L, op, fd, v);
+      } else {
+        sor_rb_ref(L, op, P.f.as<double>(), P.f_const, P.nl, u, omega, 0, part, degenerate);
+        sor_rb_ref(L, op, P.f.as<double>(), P.f_const, P.nl, u, omega, 1, part, degenerate);
+        if (g.is3d) { closures_nr(L, op, fd, u); closures_dirichlet(L, op, fd, u); }
+      }
+      finalize(L, sc, part, relax_blocks(op), FIN_RELAX, tol, 0, maxslot);
+      if (jac) cur ^= 1;
+    }
+    launched += chunk;
+    const KScalars& k = pull(P, 0);
+    int deg = 0;
+    HIPCHK(hipMemcpy(&deg, degenerate, sizeof(int), hipMemcpyDeviceToHost));
+    if (deg) throw Error(PDEHIP_ERR_DEGENERATE, "degenerate PDE center coefficient");
+    if (k.done || cancelled(P)) break;
+  }
+  const KScalars& k = pull(P, 0);
+  if (cancelled(P) && !k.done) throw Error(PDEHIP_ERR_CANCELLED, "solve cancelled");
+  // the sweep that set `done` is the last one that changed the grid
+  const int sweeps = k.iter;
+  const int final_buf = jac ? (sweeps % 2) : 0;
+  if (final_buf != 0) HIPCHK(hipMemcpyAsync(grid, bufs[1], n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  rep->iterations = sweeps;
+  rep->final_metric = k.metric;
+  rep->converged = k.done;
+}
+
+// BiCGStab (reference krylov.cpp:82-185) with optional Jacobi right preconditioner.
+static void solve_bicgstab(Plan& P, pdehip_report* rep) {
+  const hipStream_t s = P.stream();
+  const long long n = P.g.n;
+  const DevOp& op = P.op();
+  init_scalars(P, 0, 1e-30);
+  KScalars* sc = P.sc(0);
+  const Launch L{s, sc};
+  const Launch L0{s, nullptr};
+  double* part = P.part.as<double>();
+  double* part2 = P.part2.as<double>();
+  double *x = P.x.as<double>(), *r = P.r.as<double>(), *pv = P.p.as<double>(), *v = P.q.as<double>();
+  double *rhat = P.z.as<double>(), *sv = P.w1.as<double>(), *t = P.w2.as<double>(), *sh = P.w3.as<double>();
+  const bool pc = P.opt.precond == PDEHIP_PC_JACOBI;
+  double* ph = pc ? P.w4.as<double>() : pv;
+  double* shv = pc ? sh : sv;
+  HIPCHK(hipMemcpyAsync(x, P.x0.get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  residual(L0, op, P64, P.bvec.get(), x, r, part);
+  finalize(L0, sc, part, partial_blocks(op), FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
+  HIPCHK(hipMemcpyAsync(rhat, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  const int kb = krylov_blocks(n);
+  const int max_iter = std::max(1, P.opt.max_iter);
+  const int every = check_every(P);
+  int launched = 0;
+  while (launched < max_iter) {
+    const int chunk = std::min(every, max_iter - launched);
+    for (int c = 0; c < chunk; ++c) {
+      dot(L, P64, rhat, r, n, part);
+      fin_k(s, sc, part, nullptr, vec_blocks(n), KFIN_BI_RHO);
+      bicg_p(s, r, pv, v, n, sc);
+      if (pc) jacobi_pc(s, op, pv, ph, sc);
+      apply(L, op, P64, ph, v, nullptr);
+      dot(L, P64, rhat, v, n, part);
+      fin_k(s, sc, part, nullptr, vec_blocks(n), KFIN_BI_ALPHA);
+      bicg_s(s, r, v, sv, n, part, sc);
+      fin_k(s, sc, part, nullptr, kb, KFIN_BI_S);
+      bicg_xs(s, x, ph, n, sc);
+      bi_sdone(s, sc);
+      if (pc) jacobi_pc(s, op, sv, shv, sc);
+      apply(L, op, P64, shv, t, nullptr);
+      dot2(s, t, t, sv, n, part, part2, sc);
+      fin_k(s, sc, part, part2, kb, KFIN_BI_OMEGA);
+      bicg_x(s, x, ph, shv, n, sc);
+      residual(L, op, P64, P.bvec.get(), x, r, part);
+      fin_k(s, sc, part, nullptr, partial_blocks(op), KFIN_BI_R);
+    }
+    launched += chunk;
+    const KScalars& k = pull(P, 0);
+    if (k.done || k.breakdown || cancelled(P)) break;
+  }
+  const KScalars& k = pull(P, 0);
+  report_from(k, rep);
+  if (cancelled(P) && !k.done) throw Error(PDEHIP_ERR_CANCELLED, "solve cancelled");
+  if (k.breakdown && !k.done) rep->status = PDEHIP_ERR_BREAKDOWN;
+  else if (!k.done) rep->status = PDEHIP_ERR_NOT_CONVERGED;
+}
+
+// Restarted GMRES(m) (reference krylov.cpp:187-333) with CGS2 and optional Jacobi right
+// preconditioner; the back-substitution uses the current cycle's step count.
+static void solve_gmres(Plan& P, pdehip_report* rep) {
+  const hipStream_t s = P.stream();
+  const long long n = P.g.n;
+  const DevOp& op = P.op();
+  const int max_iter = std::max(1, P.opt.max_iter);
+  const int mm = std::max(1, std::min(P.opt.gmres_restart, max_iter));
+  init_scalars(P, 0, 1e-30);
+  KScalars* sc = P.sc(0);
+  const Launch L{s, sc};
+  const Launch L0{s, nullptr};
+  double* part = P.part.as<double>();
+  double *x = P.x.as<double>(), *r = P.r.as<double>(), *w = P.w1.as<double>();
+  double *zt = P.w2.as<double>(), *tmp = P.w3.as<double>();
+  double* V = P.V.as<double>();
+  double *H = P.H.as<double>(), *cs = P.cs.as<double>(), *sn = P.sn.as<double>();
+  double *gv = P.gv.as<double>(), *yv = P.yv.as<double>();
+  const bool pc = P.opt.precond == PDEHIP_PC_JACOBI;
+  HIPCHK(hipMemcpyAsync(x, P.x0.get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  residual(L0, op, P64, P.bvec.get(), x, r, part);
+  finalize(L0, sc, part, partial_blocks(op), FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
+  const int kb = krylov_blocks(n);
+  int iter_total = 0;
+  while (iter_total < max_iter) {
+    const KScalars& k0 = pull(P, 0);
+    if (k0.done || k0.breakdown || cancelled(P)) break;
+    const int m = std::min(mm, max_iter - iter_total);
+    gm_start(s, V, r, n, gv, m, sc);
+    for (int j = 0; j < m; ++j) {
+      const double* vj = V + (long long)j * n;
+      if (pc) { jacobi_pc(s, op, vj, zt, sc); apply(L, op, P64, zt, w, nullptr); }
+      else apply(L, op, P64, vj, w, nullptr);
+      double* hcol = P.hcb.as<double>();  // two passes of (j+1) coefficients
+      for (int pass = 0; pass < 2; ++pass) {
+        multidot(s, V, n, j + 1, w, n, part, kb, sc);
+        fin_multi(s, part, kb, j + 1, hcol + (pass ? 64 : 0), 0, sc);
+        multiaxpy(s, V, n, j + 1, hcol + (pass ? 64 : 0), w, n, -1.0, 1, sc);
+      }
+      gm_setcol(s, H, hcol, j, m, j + 1, sc);   // H[i][j] = h1[i] + h2[i]
+      dot(L, P64, w, w, n, part);
+      fin_norm(s, part, vec_blocks(n), H + (long long)(j + 1) * m + j, sc);
+      gm_normalize(s, V + (long long)(j + 1) * n, w, n, H + (long long)(j + 1) * m + j, sc);
+      givens(s, H, cs, sn, gv, j, m, sc);
+    }
+    backsub(s, H, gv, yv, m, sc);
+    // x += M (V y)
+    fill(s, P64, zt, n, 0.0);
+    const KScalars& kc = pull(P, 0);
+    const int steps = kc.steps;
+    if (kc.breakdown) break;
+    multiaxpy(s, V, n, steps, yv, zt, n, 1.0, 0, sc);
+    if (pc) { jacobi_pc(s, op, zt, w, sc); axpy_dev(L, P64, x, w, n, nullptr, 1.0); }
+    else axpy_dev(L, P64, x, zt, n, nullptr, 1.0);
+    residual(L, op, P64, P.bvec.get(), x, r, part);
+    fin_k(s, sc, part, nullptr, partial_blocks(op), KFIN_GM_R);
+    iter_total += steps;
+    HIPCHK(hipMemsetAsync(H, 0, P.H.bytes(), s));
+  }
+  const KScalars& k = pull(P, 0);
+  rep->iterations = iter_total;
+  rep->final_metric = k.metric;
+  rep->converged = k.done;
+  if (cancelled(P) && !k.done) throw Error(PDEHIP_ERR_CANCELLED, "solve cancelled");
+  if (k.breakdown && !k.done) rep->status = PDEHIP_ERR_BREAKDOWN;
+  else if (!k.done) rep->status = PDEHIP_ERR_NOT_CONVERGED;
+}
+
+void plan_residual_norms(Plan& P, const double* u_dev, double* l2, double* linf) {
+  const hipStream_t s = P.stream();
+  double* sums = P.part.as<double>();
+  double* maxs = P.part2.as<double>();
+  int* degenerate = reinterpret_cast<int*>(P.part3.as<double>() + P.npart + 1);
+  HIPCHK(hipMemsetAsync(degenerate, 0, sizeof(int), s));
+  DevOp op = P.op();
+  for (double& t : op.theta) t = 0.0;
+  residual_norms(s, op, P.faces.fd, u_dev, P.f.as<double>(), P.f_const, P.nl, sums, maxs, degenerate);
+  const int nb = resnorm_blocks(op);
+  std::vector<double> hs(nb), hm(nb);
+  HIPCHK(hipMemcpyAsync(hs.data(), sums, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(hm.data(), maxs, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double a = 0.0, m = 0.0;
+  for (int i = 0; i < nb; ++i) { a += hs[i]; m = std::max(m, hm[i]); }
+  *l2 = std::sqrt(std::max(0.0, a));
+  *linf = m;
+}
+
+void plan_solve(Plan& P, pdehip_report* rep) {
+  std::memset(rep, 0, sizeof(*rep));
+  const hipStream_t s = P.stream();
+  HIPCHK(hipEventRecord(P.ev0, s));
+  switch (P.opt.method) {
+    case PDEHIP_JACOBI:
+    case PDEHIP_GAUSS_SEIDEL:
+    case PDEHIP_SOR: solve_relax(P, rep); break;
+    case PDEHIP_CG: solve_cg(P, false, rep); break;
+    case PDEHIP_BICGSTAB: solve_bicgstab(P, rep); break;
+    case PDEHIP_GMRES: solve_gmres(P, rep); break;
+    case PDEHIP_MG:
+      if (P.opt.mg_single_cycle) solve_mg_reference(P, rep);
+      else solve_mg_iter(P, rep);
+      break;
+    case PDEHIP_MGPCG:
+      if (P.mixed) solve_mixed(P, rep);
+      else solve_cg(P, true, rep);
+      break;
+  }
+  // boundary reconstruction of folded solves (closures then Dirichlet, reference order)
+  const Launch L0{s, nullptr};
+  if (P.folded) {
+    closures_nr(L0, P.op(), P.faces.fd, P.x.as<double>());
+    closures_dirichlet(L0, P.op(), P.faces.fd, P.x.as<double>());
+  }
+  HIPCHK(hipEventRecord(P.ev1, s));
+  HIPCHK(hipEventSynchronize(P.ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, P.ev0, P.ev1));
+  rep->solve_ms = ms;
+  // relative residual of the final iterate and the reference residual norms
+  double* part = P.part.as<double>();
+  const int mth = P.opt.method;
+  if (P.r && mth != PDEHIP_JACOBI && mth != PDEHIP_GAUSS_SEIDEL && mth != PDEHIP_SOR) {
+    residual(L0, P.op(), P64, P.bvec.get(), P.x.get(), P.r.get(), part);
+    const int np = partial_blocks(P.op());
+    std::vector<double> h(np);
+    HIPCHK(hipMemcpyAsync(h.data(), part, np * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double rr = 0.0;
+    for (double v : h) rr += v;
+    dot(L0, P64, P.bvec.get(), P.bvec.get(), P.g.n, part);
+    const int nv = vec_blocks(P.g.n);
+    h.resize(nv);
+    HIPCHK(hipMemcpyAsync(h.data(), part, nv * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double bb = 0.0;
+    for (double v : h) bb += v;
+    rep->rel_residual = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+  }
+  plan_residual_norms(P, P.x.as<double>(), &rep->residual_l2, &rep->residual_linf);
+}
+
+void plan_download(Plan& P, double* u_out) {
+  HIPCHK(hipMemcpyAsync(u_out, P.x.get(), P.g.n * sizeof(double), hipMemcpyDeviceToHost, P.stream()));
+  HIPCHK(hipStreamSynchronize(P.stream()));
+}
+
+// ------------------------------------------------------------------ apply (reference rows)
+void op_apply(Ctx* ctx, const pdehip_problem& p, const double* x, double* y) {
+  pdehip_options o{};
+  o.method = PDEHIP_JACOBI;
+  o.max_iter = 1;
+  o.sor_omega = 1.5;
+  auto P = plan_create(ctx, p, o);
+  const hipStream_t s = ctx->stream;
+  DBuf dx(ctx, P->g.n * sizeof(double)), dy(ctx, P->g.n * sizeof(double));
+  HIPCHK(hipMemcpyAsync(dx.get(), x, P->g.n * sizeof(double), hipMemcpyHostToDevice, s));
+  apply(Launch{s, nullptr}, P->op(), P64, dx.get(), dy.get(), nullptr);
+  HIPCHK(hipMemcpyAsync(y, dy.get(), P->g.n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------------ time series
+// SolvePDETimeSeries (solver.cpp:989-1329): 2D explicit steps, closures after each step.
+void time_series(Ctx* ctx, const pdehip_problem& p, const pdehip_time_config& tc,
+                 pdehip_frame_fn cb, void* user, double* u_out, pdehip_report* rep) {
+  if (p.nz > 1) throw Error(PDEHIP_ERR_UNSUPPORTED, "3D Time Series Solver not yet refactored.");
+  if (tc.dt <= 0.0) throw Error(PDEHIP_ERR_INVALID, "time step dt must be positive");
+  const bool has_ut = std::abs(p.ut) > 1e-12, has_utt = std::abs(p.utt) > 1e-12;
+  if (!has_ut && !has_utt) throw Error(PDEHIP_ERR_INVALID, "time-dependent solve requires u_t or u_tt term");
+  if (has_utt && p.initial && !p.initial_velocity)
+    throw Error(PDEHIP_ERR_INVALID, "checkpoint restart for u_tt requires velocity data");
+  pdehip_options o{};
+  o.method = PDEHIP_JACOBI;
+  o.max_iter = 1;
+  o.sor_omega = 1.5;
+  auto P = plan_create(ctx, p, o);
+  const Geo& g = P->g;
+  const hipStream_t s = ctx->stream;
+  const long long n = g.n;
+  const DevOp& op = P->op();
+  const FaceData& fd = P->faces.fd;
+  const Launch L0{s, nullptr};
+  DBuf u(ctx, n * 8), un(ctx, n * 8), acc(ctx, n * 8), ta(ctx, n * 8), tb(ctx, n * 8), vel;
+  if (p.initial) HIPCHK(hipMemcpyAsync(u.get(), p.initial, n * 8, hipMemcpyHostToDevice, s));
+  else fill(s, P64, u.get(), n, 0.0);
+  if (has_utt) {
+    vel.alloc(ctx, n * 8);
+    if (p.initial_velocity) HIPCHK(hipMemcpyAsync(vel.get(), p.initial_velocity, n * 8, hipMemcpyHostToDevice, s));
+    else fill(s, P64, vel.get(), n, 0.0);
+  }
+  closures_dirichlet(L0, op, fd, u.as<double>());
+  closures_nr(L0, op, fd, u.as<double>());
+  closures_dirichlet(L0, op, fd, u.as<double>());
+  const int frames = std::max(1, tc.frames);
+  const bool integ = !has_utt && tc.integrator != PDEHIP_TI_FORWARD_EULER;
+  const double dt = tc.dt;
+  const double* f = P->f.as<double>();
+  const double fc = P->f_const, ut = p.ut;
+  const NonlinearTerms& nl = P->nl;
+  // frame buffering: up to buffer_frames frames stay on the device before one batched copy
+  int buffer = tc.buffer_frames > 0 ? tc.buffer_frames : 1;
+  buffer = std::max(1, std::min(buffer, frames));
+  DBuf fbuf(ctx, (size_t)buffer * n * 8);
+  std::vector<double> host((size_t)buffer * n), vhost;
+  int in_buf = 0, buf_first = 0;
+  bool stop = false;
+  auto flush = [&]() {
+    if (in_buf == 0) return;
+    HIPCHK(hipMemcpyAsync(host.data(), fbuf.get(), (size_t)in_buf * n * 8, hipMemcpyDeviceToHost, s));
+    if (has_utt) { vhost.resize(n); HIPCHK(hipMemcpyAsync(vhost.data(), vel.get(), n * 8, hipMemcpyDeviceToHost, s)); }
+    HIPCHK(hipStreamSynchronize(s));
+    for (int q = 0; q < in_buf && !stop; ++q) {
+      const int fr = buf_first + q;
+      if (cb && !cb(user, fr, tc.t_start + fr * dt, host.data() + (size_t)q * n,
+                    has_utt ? vhost.data() : nullptr))
+        stop = true;
+    }
+    in_buf = 0;
+  };
+  auto t0 = std::chrono::steady_clock::now();
+  int steps = 0;
+  for (int frame = 0; frame < frames && !stop; ++frame) {
+    if (cb) {
+      if (in_buf == 0) buf_first = frame;
+      HIPCHK(hipMemcpyAsync(fbuf.as<double>() + (size_t)in_buf * n, u.get(), n * 8, hipMemcpyDeviceToDevice, s));
+      ++in_buf;
+      // velocity is reported at the flush only; flush every frame for u_tt to keep it exact
+      if (in_buf == buffer || has_utt) flush();
+    }
+    if (frame + 1 >= frames) break;
+    double* U = u.as<double>();
+    if (integ) {
+      switch (tc.integrator) {
+        case PDEHIP_TI_RK2:
+          time_stage(s, op, U, U, acc.as<double>(), ta.as<double>(), f, fc, nl, ut, dt, dt, 0);
+          time_stage(s, op, ta.as<double>(), U, acc.as<double>(), U, f, fc, nl, ut, dt, 0.5 * dt, 3);
+          break;
+        case PDEHIP_TI_RK4:
+          time_stage(s, op, U, U, acc.as<double>(), ta.as<double>(), f, fc, nl, ut, dt, 0.5 * dt, 0);
+          time_stage(s, op, ta.as<double>(), U, acc.as<double>(), tb.as<double>(), f, fc, nl, ut, dt, 0.5 * dt, 1);
+          time_stage(s, op, tb.as<double>(), U, acc.as<double>(), ta.as<double>(), f, fc, nl, ut, dt, dt, 1);
+          time_stage(s, op, ta.as<double>(), U, acc.as<double>(), U, f, fc, nl, ut, dt, dt / 6.0, 2);
+          break;
+        case PDEHIP_TI_SSPRK2:
+          HIPCHK(hipMemcpyAsync(un.get(), U, n * 8, hipMemcpyDeviceToDevice, s));
+          time_stage(s, op, U, U, nullptr, ta.as<double>(), f, fc, nl, ut, dt, dt, 0);
+          time_stage(s, op, ta.as<double>(), un.as<double>(), nullptr, U, f, fc, nl, ut, dt, 0.0, 4);
+          break;
+        case PDEHIP_TI_SSPRK3:
+          HIPCHK(hipMemcpyAsync(un.get(), U, n * 8, hipMemcpyDeviceToDevice, s));
+          time_stage(s, op, U, U, nullptr, ta.as<double>(), f, fc, nl, ut, dt, dt, 0);
+          time_stage(s, op, ta.as<double>(), un.as<double>(), nullptr, tb.as<double>(), f, fc, nl, ut, dt, 0.0, 5);
+          time_stage(s, op, tb.as<double>(), un.as<double>(), nullptr, U, f, fc, nl, ut, dt, 0.0, 6);
+          break;
+        default:
+          time_stage(s, op, U, U, nullptr, ta.as<double>(), f, fc, nl, ut, dt, dt, 0);
+          HIPCHK(hipMemcpyAsync(U, ta.get(), n * 8, hipMemcpyDeviceToDevice, s));
+      }
+    } else if (has_utt) {
+      time_utt(s, op, U, vel.as<double>(), ta.as<double>(), f, fc, nl, ut, p.utt, dt);
+      std::swap(u, ta);
+    } else {
+      time_stage(s, op, U, U, nullptr, ta.as<double>(), f, fc, nl, ut, dt, dt, 0);
+      std::swap(u, ta);
+    }
+    closures_nr(L0, op, fd, u.as<double>());
+    closures_dirichlet(L0, op, fd, u.as<double>());
+    ++steps;
+  }
+  flush();
+  HIPCHK(hipStreamSynchronize(s));
+  auto t1 = std::chrono::steady_clock::now();
+  if (u_out) HIPCHK(hipMemcpy(u_out, u.get(), n * 8, hipMemcpyDeviceToHost));
+  if (rep) {
+    std::memset(rep, 0, sizeof(*rep));
+    rep->iterations = steps;
+    rep->converged = 1;
+    rep->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+}
+
+// ------------------------------------------------------------------ kernel timing
+double time_kernel(Plan& P, int kernel, int precision, int reps, double* bytes) {
+  const hipStream_t s = P.stream();
+  const long long n = P.g.n;
+  const Launch L{s, nullptr};
+  const bool p32 = precision == PDEHIP_MIXED;
+  const Prec pr = p32 ? P32 : P64;
+  const size_t es = prec_bytes(pr);
+  const DevOp& op = p32 ? P.L[0].op32 : P.L[0].op64;
+  if (p32 && P.prob.op_kind == PDEHIP_OP_FLUX && !P.L[0].kap32)
+    throw Error(PDEHIP_ERR_INVALID, "plan holds no FP32 operator");
+  const size_t kb = P.prob.op_kind == PDEHIP_OP_FLUX ? kappa_bytes(pr) : 0;
+  DBuf a(P.ctx, n * es), b(P.ctx, n * es), c(P.ctx, n * es);
+  fill(s, pr, a.get(), n, 1.0);
+  fill(s, pr, b.get(), n, 0.5);
+  fill(s, pr, c.get(), n, 0.0);
+  auto once = [&]() {
+    switch (kernel) {
+      case 0: apply(L, op, pr, a.get(), b.get(), nullptr); break;
+      case 1: rbgs(L, op, pr, c.get(), a.get(), 0); rbgs(L, op, pr, c.get(), a.get(), 1); break;
+      case 2: residual(L, op, pr, c.get(), a.get(), b.get(), nullptr); break;
+      default: convert(s, pr, a.get(), pr, b.get(), n, 1.0); break;
+    }
+  };
+  switch (kernel) {
+    case 0: *bytes = (double)n * (2 * es + kb); break;          // read x (+kappa), write y
+    case 1: *bytes = (double)n * (3 * es + kb); break;          // read u, b (+kappa), write u
+    case 2: *bytes = (double)n * (3 * es + kb); break;          // read b, x (+kappa), write r
+    default: *bytes = (double)n * 2 * es; break;                // read + write
+  }
+  once();
+  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipEventRecord(P.ev0, s));
+  for (int i = 0; i < reps; ++i) once();
+  HIPCHK(hipEventRecord(P.ev1, s));
+  HIPCHK(hipEventSynchronize(P.ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, P.ev0, P.ev1));
+  return ms / std::max(1, reps);
+}
+
+}  // namespace pdehip

@@ -1,0 +1,166 @@
+// runtime.hpp — host runtime of libpdehip: device context, HBM-budgeted buffers, problem
+// assembly, multigrid hierarchy and the solver drivers behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "kernels_ext.hpp"
+#include "kernels_krylov.hpp"
+
+namespace pdehip {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+#define HIPCHK(x) ::pdehip::hip_check((x), #x)
+
+// ------------------------------------------------------------------ context
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0;                 // bytes this context may allocate
+  std::atomic<size_t> used{0};
+  std::atomic<size_t> peak{0};
+  int cus = 0;
+  std::string arch;
+  std::mutex mu;                     // serialises one-shot solves on the context stream
+  std::string last_error;
+  std::mutex err_mu;
+  void set_error(const std::string& e) {
+    std::lock_guard<std::mutex> g(err_mu);
+    last_error = e;
+  }
+};
+
+// Device buffer charged against the context budget.
+class DBuf {
+ public:
+  DBuf() = default;
+  DBuf(Ctx* c, size_t bytes) { alloc(c, bytes); }
+  ~DBuf() { release(); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept { *this = std::move(o); }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      ctx_ = o.ctx_; p_ = o.p_; bytes_ = o.bytes_;
+      o.p_ = nullptr; o.bytes_ = 0; o.ctx_ = nullptr;
+    }
+    return *this;
+  }
+  void alloc(Ctx* c, size_t bytes);
+  void release();
+  void* get() const { return p_; }
+  template <typename T> T* as() const { return static_cast<T*>(p_); }
+  size_t bytes() const { return bytes_; }
+  explicit operator bool() const { return p_ != nullptr; }
+
+ private:
+  Ctx* ctx_ = nullptr;
+  void* p_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+// ------------------------------------------------------------------ geometry
+struct Geo {
+  int nx = 0, ny = 0, nz = 1;
+  bool is3d = false;
+  double xmin = 0, xmax = 1, ymin = 0, ymax = 1, zmin = 0, zmax = 1;
+  double dx = 0, dy = 0, dz = 1;
+  long long n = 0;
+  long long sxy() const { return (long long)nx * ny; }
+};
+Geo make_geo(const pdehip_problem& p);
+Geo coarsen(const Geo& f);
+int level_count(const Geo& g, int max_levels);
+
+// Host copy of a face's boundary data sampled per face node.
+struct HostFaces {
+  int kind[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<double> value[6], alpha[6], beta[6], gamma[6];
+  double h[6] = {1, 1, 1, 1, 1, 1};
+  int convention = 0;
+};
+HostFaces sample_faces(const pdehip_problem& p, const Geo& g);
+// Folded closure factor theta of each face at the level's spacing.
+void fold_thetas(const pdehip_problem& p, const Geo& g, bool folded, double theta[6]);
+// CONST weights (C, W, E, S, N, B, F) at a level's spacing.
+void const_weights(const pdehip_problem& p, const Geo& g, double w[7]);
+
+// Device copy of HostFaces.
+struct DevFaces {
+  DBuf buf;
+  FaceData fd{};
+  void upload(Ctx* c, const HostFaces& h, const Geo& g, hipStream_t s);
+};
+
+// ------------------------------------------------------------------ multigrid level
+struct Level {
+  Geo g;
+  DevOp op64{}, op32{};
+  DBuf kap64, kap32, pw;
+  DBuf u, b, r;
+  Prec prec = P64;
+  double sigma = 1.0;     // stored correction = u / sigma (mixed, P16 levels)
+  DBuf ainv, map;         // coarsest level dense inverse
+  int m = 0;
+};
+
+// ------------------------------------------------------------------ plan
+struct Plan {
+  Ctx* ctx = nullptr;
+  pdehip_problem prob{};   // host pointers valid only during creation
+  pdehip_options opt{};
+  Geo g;
+  bool folded = false;
+  bool mixed = false;
+  HostFaces hfaces;
+  DevFaces faces;
+  NonlinearTerms nl{};
+  DBuf f;                  // per-node source (or empty: constant prob.f)
+  double f_const = 0.0;
+  DBuf bvec, x, x0;        // FP64 right-hand side, solution, boundary data
+  std::vector<Level> L;    // L[0] is the fine level (operator for every method)
+  // Krylov work vectors
+  DBuf r, p, q, z, w1, w2, w3, w4;
+  DBuf r32, x32, p32, q32, z32;
+  DBuf V, H, cs, sn, gv, yv, hcb;   // GMRES (hcb: two staged Hessenberg columns)
+  DBuf part, part2, part3;     // reduction partials
+  int npart = 0;
+  DBuf scal;                   // KScalars x 2 (outer/inner)
+  KScalars* hsc = nullptr;     // pinned host mirror
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  size_t bytes_held = 0;
+  std::vector<double> hf;      // host copy of the source (set_source)
+
+  ~Plan();
+  KScalars* sc(int i = 0) const { return scal.as<KScalars>() + i; }
+  hipStream_t stream() const { return ctx->stream; }
+  const DevOp& op() const { return L[0].op64; }
+};
+
+std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehip_options& o);
+void plan_set_source(Plan& P, const double* f_host);
+void plan_solve(Plan& P, pdehip_report* rep);
+void plan_download(Plan& P, double* u_out);
+void plan_residual_norms(Plan& P, const double* u_dev, double* l2, double* linf);
+
+// Standalone entry points.
+void op_apply(Ctx* ctx, const pdehip_problem& p, const double* x, double* y);
+void time_series(Ctx* ctx, const pdehip_problem& p, const pdehip_time_config& tc,
+                 pdehip_frame_fn cb, void* user, double* u_out, pdehip_report* rep);
+double time_kernel(Plan& P, int kernel, int precision, int reps, double* bytes);
+
+}  // namespace pdehip

@@ -210,7 +210,7 @@ CASES: List[Case] = [
          _case_relax(capi.JACOBI, 17, "dirichlet", nl=[(capi.NL_POWER, 0.5, 3), (capi.NL_SIN, 0.2, 0)])),
     Case("jacobi3d_mixed_9", "solve", _case_relax3d(capi.JACOBI, 9)),
     Case("sor3d_var_9", "solve", _case_relax3d(capi.SOR, 9, var=True)),
-    Case("mg_single_cycle_33", "solve", _case_mg_single(), tol_rel=0.0, hip=True),
+    Case("mg_single_cycle_33", "solve", _case_mg_single()),
     Case("apply_const_19x15", "apply", _case_apply(False)),
     Case("apply_var_19x15", "apply", _case_apply(True)),
     Case("residual_2d_mixed_21", "residual", _case_residual(2)),

@@ -1,0 +1,173 @@
+"""GPU parity of the extension solvers (no reference counterpart) against the CPU oracle, and
+the resident-plan / serving / batch entry points against one-shot solves."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pde_solver_amd import capi
+from pde_solver_amd.capi import BC
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from pde_solver_amd import hip
+    c = hip.Context()
+    yield c
+    c.close()
+
+
+def varcoef3d(n, seed=0):
+    """-div(k grad u) = f, k = exp(0.5 g) (g smooth Gaussian field), Dirichlet 0 on x/y faces,
+    Robin (1, 1, 0) outward on z faces: the structure of the 257^3 workload at small n."""
+    s = capi.Spec(n, n, n, a=-1.0, op_kind=capi.OP_FLUX)
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal(s.shape)
+    for ax in range(3):   # cheap smoothing
+        g = (np.roll(g, 1, ax) + 2 * g + np.roll(g, -1, ax)) / 4
+    g = (g - g.mean()) / g.std()
+    s.kappa = np.exp(0.5 * g)
+    f = rng.standard_normal(s.shape)
+    s.f_field = -(f - f.mean())
+    s.bcs = [BC.dirichlet(0.0)] * 4 + [BC.robin(1.0, 1.0, 0.0)] * 2
+    s.bc_convention = capi.BCCONV_OUTWARD
+    return s
+
+
+def mgpcg(tol=1e-12, precision=capi.FP64, levels=10, **kw):
+    return capi.options(capi.MGPCG, tol=tol, tol_mode=capi.TOL_REL, precond=capi.PC_MG,
+                        bc_rows=capi.ROWS_FOLDED, precision=precision, mg_max_levels=levels,
+                        max_iter=200, **kw)
+
+
+@pytest.mark.parametrize("n", [17, 33])
+def test_mgpcg_fp64_varcoef3d_matches_oracle(ctx, n):
+    print(f"mgpcg fp64 varcoef3d n={n}", flush=True)
+    s = varcoef3d(n)
+    st_o, u_o, rep_o = O.solve(s, mgpcg())
+    assert st_o == 0
+    st, u, rep = ctx.solve(s, mgpcg())
+    print(f"  iterations hip {rep.iterations} oracle {rep_o.iterations}  rel {rel(u, u_o):.2e}", flush=True)
+    assert rel(u, u_o) < 1e-10
+    assert abs(rep.iterations - rep_o.iterations) <= 1
+
+
+@pytest.mark.parametrize("n", [17, 33])
+def test_mgpcg_mixed_varcoef3d_matches_fp64(ctx, n):
+    print(f"mgpcg mixed varcoef3d n={n}", flush=True)
+    s = varcoef3d(n, seed=1)
+    st_o, u_o, _ = O.solve(s, mgpcg(tol=1e-13))
+    st, u, rep = ctx.solve(s, mgpcg(tol=1e-10, precision=capi.MIXED))
+    print(f"  refinements {rep.refinements} inner {rep.inner_iterations} rel {rel(u, u_o):.2e} "
+          f"relres {rep.rel_residual:.2e}", flush=True)
+    assert st == 0
+    assert rel(u, u_o) < 1e-8
+    assert rep.refinements <= 6
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+def test_mgpcg_2d_mixed_bcs(ctx, precision):
+    s = capi.Spec(129, 129, a=-1.0, b=-1.0, e=5.0)
+    s.f_field = np.random.default_rng(4).standard_normal(s.shape)
+    s.bcs = [BC.neumann(0.0), BC.robin(1.0, 1.0, 0.2), BC.dirichlet(0.1, 0.5), BC.neumann(0.3), BC(), BC()]
+    s.bc_convention = capi.BCCONV_OUTWARD
+    st_o, u_o, _ = O.solve(s, mgpcg(tol=1e-13))
+    st, u, rep = ctx.solve(s, mgpcg(tol=1e-11, precision=precision))
+    assert st == 0
+    assert rel(u, u_o) < (1e-10 if precision == capi.FP64 else 1e-9)
+
+
+def test_mg_iterated_matches_oracle(ctx):
+    s = capi.Spec(65, 65, a=-1.0, b=-1.0)
+    s.f_field = np.random.default_rng(2).standard_normal(s.shape)
+    s.bcs = [BC.dirichlet(0.2)] * 3 + [BC.robin(1, 0.5, 0)] + [BC(), BC()]
+    s.bc_convention = capi.BCCONV_OUTWARD
+    opt = capi.options(capi.MG, tol=1e-11, tol_mode=capi.TOL_REL, max_iter=100)
+    st_o, u_o, rep_o = O.solve(s, opt)
+    st, u, rep = ctx.solve(s, opt)
+    assert st == 0 and st_o == 0
+    assert rel(u, u_o) < 1e-10
+    assert rep.iterations == rep_o.iterations
+
+
+@pytest.mark.parametrize("method", [capi.BICGSTAB, capi.GMRES])
+def test_jacobi_pc_krylov_upwind_matches_oracle(ctx, method):
+    s = capi.Spec(65, 65, a=-0.01, b=-0.01, c=0.8, d=-0.6, convection=capi.CONV_UPWIND)
+    s.f_field = np.random.default_rng(7).standard_normal(s.shape)
+    opt = capi.options(method, tol=1e-11, tol_mode=capi.TOL_REL, precond=capi.PC_JACOBI,
+                       bc_rows=capi.ROWS_FOLDED, gmres_restart=30, max_iter=5000)
+    st_o, u_o, rep_o = O.solve(s, opt)
+    st, u, rep = ctx.solve(s, opt)
+    assert st == 0 and st_o == 0, ctx.error()
+    assert rel(u, u_o) < 1e-9
+
+
+def test_cg_folded_robin_matches_oracle(ctx):
+    s = capi.Spec(65, 65, a=-1.0, b=-1.0)
+    s.f_field = np.random.default_rng(3).standard_normal(s.shape)
+    s.bcs = [BC.robin(1, 1, 0.1)] * 4 + [BC(), BC()]
+    s.bc_convention = capi.BCCONV_OUTWARD
+    opt = capi.options(capi.CG, tol=1e-12, tol_mode=capi.TOL_REL, bc_rows=capi.ROWS_FOLDED, max_iter=5000)
+    st_o, u_o, _ = O.solve(s, opt)
+    st, u, _ = ctx.solve(s, opt)
+    assert st == 0 and rel(u, u_o) < 1e-10
+
+
+def test_plan_reuse_matches_one_shot(ctx):
+    s = varcoef3d(17, seed=5)
+    opt = mgpcg(tol=1e-10)
+    plan = ctx.plan(s, opt)
+    rng = np.random.default_rng(9)
+    for _ in range(2):
+        f = rng.standard_normal(s.shape)
+        plan.set_source(f)
+        plan.solve()
+        u_plan = plan.download()
+        s.f_field = f
+        _, u_one, _ = ctx.solve(s, opt)
+        assert np.array_equal(u_plan, u_one)
+    plan.close()
+
+
+def test_server_results_identical_alone_and_in_mix(ctx):
+    from pde_solver_amd import hip
+    specs = []
+    rng = np.random.default_rng(11)
+    for n in (33, 65, 129, 33, 65):
+        s = capi.Spec(n, n, a=-1.0, b=-1.0, e=float(rng.uniform(0, 10)))
+        s.f_field = rng.standard_normal(s.shape)
+        specs.append(s)
+    opt = mgpcg(tol=1e-10)
+    alone = [ctx.solve(s, opt)[1] for s in specs]
+    cfg = capi.ServerConfig()
+    cfg.max_inflight, cfg.streams, cfg.cache_operators = 8, 4, 1
+    srv = ctx.server(cfg)
+    tickets = [srv.submit(s, opt) for s in specs]
+    outs = [srv.wait(t) for t in tickets]
+    for (st, u, rep), a in zip(outs, alone):
+        assert st == 0
+        assert np.array_equal(u, a)
+    srv.close()
+
+
+def test_batch_matches_individual(ctx):
+    specs = []
+    rng = np.random.default_rng(12)
+    for n in (33, 65, 33, 129):
+        s = capi.Spec(n, n, a=-1.0, b=-1.0, e=float(rng.uniform(0, 50)))
+        s.f_field = rng.standard_normal(s.shape)
+        s.bcs = [BC.dirichlet(0.0), BC.neumann(0.0), BC.robin(1, 1, 0), BC.dirichlet(0.0), BC(), BC()]
+        s.bc_convention = capi.BCCONV_OUTWARD
+        specs.append(s)
+    opt = mgpcg(tol=1e-10)
+    st, outs, reps = ctx.solve_batch(specs, opt)
+    assert st == 0
+    for s, u in zip(specs, outs):
+        _, u1, _ = O.solve(s, mgpcg(tol=1e-12))
+        assert rel(u, u1) < 1e-8
