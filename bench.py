@@ -1,0 +1,228 @@
+"""Headline benchmark: solves/s and p50/p99 latency at relative residual 1e-8 for the
+single large solve, mixed precision vs plain FP64 on one MI355X, plus the dominant kernel's
+HBM roofline fraction and the CPU oracle timed on the host's cores.
+
+Workload: 3D variable-coefficient diffusion -div(k grad u) = f on [0,1]^3, 257^3 nodes, 7-point
+flux stencil, k = exp(0.5 g) with g a seeded Gaussian random field smoothed by a Gaussian filter
+of sigma = 4 cells and normalised to unit variance, f seeded N(0,1) with zero mean, Dirichlet 0
+on the x/y faces, Robin (alpha=1, beta=1, gamma=0, outward normal) on the z faces.  MG-preconditioned
+CG, V(2,2) red-black Gauss-Seidel, 6 levels (257..9, dense FP64 coarsest solve).  The plain
+path is FP64 throughout; the mixed path runs FP32 MG-PCG inner solves with FP16 coarse-level
+storage inside FP64 iterative refinement.  A step is one complete solve from a zero initial
+guess with the operator, hierarchy and right-hand side resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 257] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/s + p99 latency @ rel-resid 1e-8, mixed vs FP64 GPU; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_problem(n: int):
+    from scipy.ndimage import gaussian_filter
+
+    from pde_solver_amd import capi
+    from pde_solver_amd.capi import BC
+
+    g = np.random.default_rng(2).standard_normal((n, n, n))
+    g = gaussian_filter(g, sigma=4.0, mode="reflect", truncate=4.0)
+    g = (g - g.mean()) / g.std()
+    f = np.random.default_rng(20).standard_normal((n, n, n))
+    f -= f.mean()
+    s = capi.Spec(n, n, n, a=-1.0, op_kind=capi.OP_FLUX)
+    s.kappa = np.exp(0.5 * g)
+    s.f_field = -f                        # reference sign: L u + f = 0
+    s.bcs = [BC.dirichlet(0.0)] * 4 + [BC.robin(1.0, 1.0, 0.0)] * 2
+    s.bc_convention = capi.BCCONV_OUTWARD
+    return s
+
+
+def options(precision: int, tol: float):
+    from pde_solver_amd import capi
+    return capi.options(capi.MGPCG, tol=tol, tol_mode=capi.TOL_REL, precond=capi.PC_MG,
+                        bc_rows=capi.ROWS_FOLDED, precision=precision, mg_max_levels=6,
+                        mg_pre=2, mg_post=2, max_iter=200, inner_tol=1e-4)
+
+
+def timed_solves(plan, steps: int, warmup: int):
+    for _ in range(warmup):
+        plan.solve()
+    lat, dev, reps = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t = time.perf_counter()
+        _, rep = plan.solve()
+        lat.append((time.perf_counter() - t) * 1e3)
+        dev.append(rep.solve_ms)
+        reps.append(rep)
+    wall = time.perf_counter() - t0
+    return wall, lat, dev, reps
+
+
+def pct(v, q):
+    return float(np.percentile(np.asarray(v), q))
+
+
+def cpu_baseline(spec, n_threads: int):
+    from oracle import oracle as O
+    from pde_solver_amd import capi
+    O.set_threads(n_threads)
+    t = time.perf_counter()
+    st, u, rep = O.solve(spec, options(capi.FP64, 1e-8))
+    total = time.perf_counter() - t
+    return {"status": st, "solve_s": rep.solve_ms / 1e3, "total_s": total, "iterations": rep.iterations}, u
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=257)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path "
+              "(--gpus must be 1; sharding is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from pde_solver_amd import capi, hip
+
+    ok, note = hip.is_available()
+    log(f"device: {note}")
+    if not ok:
+        raise SystemExit("no usable gfx9 device")
+    n = args.n
+    t = time.perf_counter()
+    spec = make_problem(n)
+    log(f"inputs {n}^3 generated in {time.perf_counter() - t:.1f}s")
+    ctx = hip.Context()
+
+    # FP64 oracle solution on the GPU (plain path driven to rel-resid 1e-12)
+    t = time.perf_counter()
+    ref_plan = ctx.plan(spec, options(capi.FP64, 1e-12))
+    _, rep_ref = ref_plan.solve()
+    u_ref = ref_plan.download()
+    ref_plan.close()
+    log(f"FP64 reference solve (rel 1e-12): {rep_ref.iterations} iterations, "
+        f"{time.perf_counter() - t:.1f}s incl. set-up")
+
+    plan64 = ctx.plan(spec, options(capi.FP64, 1e-8))
+    planmx = ctx.plan(spec, options(capi.MIXED, 1e-8))
+    log(f"plans resident: fp64 {plan64.bytes() / 2**30:.2f} GiB, mixed {planmx.bytes() / 2**30:.2f} GiB")
+
+    # ---- mixed (the measured path)
+    wall_mx, lat_mx, dev_mx, reps_mx = timed_solves(planmx, args.steps, args.warmup)
+    u_mx = planmx.download()
+    log(f"mixed: {args.steps} solves in {wall_mx:.3f}s, refinements {reps_mx[-1].refinements}, "
+        f"inner iterations {reps_mx[-1].inner_iterations}, rel_resid {reps_mx[-1].rel_residual:.2e}")
+
+    # ---- plain FP64
+    steps64 = max(3, args.steps // 2)
+    wall_64, lat_64, dev_64, reps_64 = timed_solves(plan64, steps64, 1)
+    u_64 = plan64.download()
+    log(f"fp64: {steps64} solves in {wall_64:.3f}s, iterations {reps_64[-1].iterations}, "
+        f"rel_resid {reps_64[-1].rel_residual:.2e}")
+
+    nref = float(np.linalg.norm(u_ref))
+    err_mx = float(np.linalg.norm(u_mx - u_ref) / nref)
+    err_64 = float(np.linalg.norm(u_64 - u_ref) / nref)
+
+    # ---- roofline of the dominant kernel (fine-level red-black GS sweep) with HIP events
+    ms_mx, bytes_mx = planmx.time_kernel(1, capi.MIXED, reps=20)
+    ms_64, bytes_64 = plan64.time_kernel(1, capi.FP64, reps=20)
+    ms_ap, bytes_ap = plan64.time_kernel(0, capi.FP64, reps=20)
+    ms_cp, bytes_cp = plan64.time_kernel(3, capi.FP64, reps=20)
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9  # noqa: E731
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "roofline_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("rbgs_sweep_fp32_bytes_per_launch")
+        except Exception:
+            traffic = None
+    roof = {"bound": "hbm", "kernel": "rbgs sweep (red+black), FP32, fine level 257^3",
+            "achieved": round(gbs(bytes_mx, ms_mx), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs(bytes_mx, ms_mx) / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes": bytes_mx, "avg_ms": round(ms_mx, 4)}
+    kernels = {
+        "rbgs_sweep_fp64": {"avg_ms": round(ms_64, 4), "GB/s": round(gbs(bytes_64, ms_64), 1)},
+        "apply_fp64": {"avg_ms": round(ms_ap, 4), "GB/s": round(gbs(bytes_ap, ms_ap), 1)},
+        "stream_copy_fp64": {"avg_ms": round(ms_cp, 4), "GB/s": round(gbs(bytes_cp, ms_cp), 1)},
+    }
+    log(f"roofline: rbgs fp32 {roof['achieved']} GB/s, rbgs fp64 {kernels['rbgs_sweep_fp64']['GB/s']} GB/s, "
+        f"apply fp64 {kernels['apply_fp64']['GB/s']} GB/s, copy {kernels['stream_copy_fp64']['GB/s']} GB/s")
+
+    # ---- CPU baseline: the oracle's FP64 MG-PCG (OpenMP) on the same problem
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        threads = max(1, min(threads, 64))
+        log(f"cpu baseline: oracle FP64 MG-PCG on {threads} threads ...")
+        res, u_cpu = cpu_baseline(spec, threads)
+        cpu = {"value": round(1.0 / res["solve_s"], 4), "unit": "solves/s", "cores": threads,
+               "kind": "port",
+               "sample": f"one full {n}^3 FP64 MG-PCG solve to rel-resid 1e-8 "
+                         f"({res['iterations']} iterations, solve phase {res['solve_s']:.2f}s, "
+                         f"set-up excluded; {res['total_s']:.2f}s with set-up)",
+               "rel_l2_vs_gpu_fp64": float(np.linalg.norm(u_cpu - u_64) / np.linalg.norm(u_64))}
+        log(f"cpu baseline: {cpu['value']} solves/s")
+
+    value = args.steps / wall_mx
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_mx / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 MG-PCG + f16 coarse storage inside f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"3D var-coef diffusion -div(k grad u)=f, {n}^3, 7-pt flux stencil, "
+                               "k=exp(0.5 g) (Gaussian field sigma=4 cells, seed 2), f~N(0,1) seed 20, "
+                               "Dirichlet 0 x/y faces, Robin(1,1,0) z faces; MG-PCG V(2,2) RB-GS "
+                               "6 levels, rel-resid 1e-8",
+                   "n": n, "levels": 6, "tol": 1e-8},
+        "latency_ms": {"p50": round(pct(lat_mx, 50), 3), "p99": round(pct(lat_mx, 99), 3),
+                       "device_p50": round(pct(dev_mx, 50), 3)},
+        "fp64": {"value": round(steps64 / wall_64, 3), "unit": "solves/s",
+                 "p50_ms": round(pct(lat_64, 50), 3), "p99_ms": round(pct(lat_64, 99), 3),
+                 "iterations": reps_64[-1].iterations, "rel_residual": reps_64[-1].rel_residual,
+                 "err_vs_fp64_oracle": err_64},
+        "mixed": {"refinements": reps_mx[-1].refinements, "inner_iterations": reps_mx[-1].inner_iterations,
+                  "rel_residual": reps_mx[-1].rel_residual, "err_vs_fp64_oracle": err_mx},
+        "speedup_mixed_vs_fp64": round(value / (steps64 / wall_64), 3),
+        "roofline": roof,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    plan64.close()
+    planmx.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -18,6 +18,7 @@
 #include "pdehip.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1455,7 +1456,9 @@ int oracle_solve(const pdehip_problem* pp, const pdehip_options* oo, double* u_o
         for (int j = 0; j < g.ny; ++j)
           for (int i = 0; i < g.nx; ++i)
             if (IsBoundary(g, i, j, k)) b[g.idx(i, j, k)] = x[g.idx(i, j, k)];
+      const auto t0 = std::chrono::steady_clock::now();
       KrylovResult kr = MgPcg(H.L[0].op, H, b, x, std::max(1, o.max_iter), o.tol, o.tol_mode, true);
+      rep->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
       rep->iterations = kr.iters;
       rep->final_metric = kr.metric;
       rep->converged = kr.ok;

@@ -531,7 +531,7 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
   const int gm = (m == PDEHIP_GMRES) ? std::max(1, std::min(o.gmres_restart, std::max(1, o.max_iter))) + 1 : 1;
   P->part.alloc(ctx, (size_t)np * gm * sizeof(double));
   P->part2.alloc(ctx, (size_t)np * sizeof(double));
-  P->part3.alloc(ctx, (size_t)np * sizeof(double) + 64);
+  P->part3.alloc(ctx, (size_t)2 * np * sizeof(double) + 64);
   P->scal.alloc(ctx, 2 * sizeof(KScalars));
   HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&P->hsc), 2 * sizeof(KScalars)));
   HIPCHK(hipEventCreate(&P->ev0));
@@ -1081,6 +1081,8 @@ void plan_residual_norms(Plan& P, const double* u_dev, double* l2, double* linf)
 void plan_solve(Plan& P, pdehip_report* rep) {
   std::memset(rep, 0, sizeof(*rep));
   const hipStream_t s = P.stream();
+  init_scalars(P, 0, 0.0);
+  init_scalars(P, 1, 0.0);
   HIPCHK(hipEventRecord(P.ev0, s));
   switch (P.opt.method) {
     case PDEHIP_JACOBI:
@@ -1098,38 +1100,48 @@ void plan_solve(Plan& P, pdehip_report* rep) {
       else solve_cg(P, true, rep);
       break;
   }
-  // boundary reconstruction of folded solves (closures then Dirichlet, reference order)
+  HIPCHK(hipEventRecord(P.ev1, s));
   const Launch L0{s, nullptr};
+  // true residual of the solver's own system, before folded boundaries are reconstructed
+  double* part = P.part.as<double>();
+  const int mth = P.opt.method;
+  const bool resid = P.r && mth != PDEHIP_JACOBI && mth != PDEHIP_GAUSS_SEIDEL && mth != PDEHIP_SOR;
+  if (resid) residual(L0, P.op(), P64, P.bvec.get(), P.x.get(), P.r.get(), part);
+  // boundary reconstruction of folded solves (closures then Dirichlet, reference order)
   if (P.folded) {
     closures_nr(L0, P.op(), P.faces.fd, P.x.as<double>());
     closures_dirichlet(L0, P.op(), P.faces.fd, P.x.as<double>());
   }
-  HIPCHK(hipEventRecord(P.ev1, s));
-  HIPCHK(hipEventSynchronize(P.ev1));
+  // ComputeResidualNorms on the device, one synchronisation for both reductions
+  double* sums = P.part3.as<double>();
+  double* maxs = sums + P.npart;
+  int* degenerate = reinterpret_cast<int*>(sums + 2 * P.npart + 1);
+  HIPCHK(hipMemsetAsync(degenerate, 0, sizeof(int), s));
+  DevOp rop = P.op();
+  for (double& t : rop.theta) t = 0.0;
+  residual_norms(s, rop, P.faces.fd, P.x.as<double>(), P.f.as<double>(), P.f_const, P.nl, sums,
+                 maxs, degenerate);
+  const int np = partial_blocks(P.op()), nb = resnorm_blocks(rop);
+  std::vector<double> h(np), hs(nb), hm(nb);
+  if (resid) HIPCHK(hipMemcpyAsync(h.data(), part, np * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(hs.data(), sums, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(hm.data(), maxs, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&P.hsc[0], P.sc(0), sizeof(KScalars), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, P.ev0, P.ev1));
   rep->solve_ms = ms;
-  // relative residual of the final iterate and the reference residual norms
-  double* part = P.part.as<double>();
-  const int mth = P.opt.method;
-  if (P.r && mth != PDEHIP_JACOBI && mth != PDEHIP_GAUSS_SEIDEL && mth != PDEHIP_SOR) {
-    residual(L0, P.op(), P64, P.bvec.get(), P.x.get(), P.r.get(), part);
-    const int np = partial_blocks(P.op());
-    std::vector<double> h(np);
-    HIPCHK(hipMemcpyAsync(h.data(), part, np * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+  if (resid) {
     double rr = 0.0;
     for (double v : h) rr += v;
-    dot(L0, P64, P.bvec.get(), P.bvec.get(), P.g.n, part);
-    const int nv = vec_blocks(P.g.n);
-    h.resize(nv);
-    HIPCHK(hipMemcpyAsync(h.data(), part, nv * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    double bb = 0.0;
-    for (double v : h) bb += v;
-    rep->rel_residual = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    // relative to the solve's initial residual (||b|| for reference rows, which start at x = 0)
+    const double r0 = P.hsc[0].r0;
+    rep->rel_residual = r0 > 0 ? std::sqrt(rr) / r0 : std::sqrt(rr);
   }
-  plan_residual_norms(P, P.x.as<double>(), &rep->residual_l2, &rep->residual_linf);
+  double a = 0.0, m = 0.0;
+  for (int i = 0; i < nb; ++i) { a += hs[i]; m = std::max(m, hm[i]); }
+  rep->residual_l2 = std::sqrt(std::max(0.0, a));
+  rep->residual_linf = m;
 }
 
 void plan_download(Plan& P, double* u_out) {
