@@ -47,6 +47,35 @@ def _compile(src: str) -> str:
     return obj
 
 
+HOST_SOURCES = ["host/hip_solver.cpp", "host/backend_hip.cpp", "host/latex_expr.cpp"]
+HOST_LIB = os.path.join(HERE, "libpdehip_host.so")
+SELF_TEST = os.path.join(HERE, "bin", "hip_self_test")
+
+
+def build_host(verbose: bool = False) -> None:
+    """The C++ plug-in layer (hip_solver.h + backend registry) and its self-test driver, with
+    g++ against the compatible type headers in host/compat."""
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "host", "compat"),
+           "-I" + os.path.join(HERE, "host")]
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
+    srcs = [os.path.join(HERE, s) for s in HOST_SOURCES]
+    link = ["-L" + HERE, "-lpdehip", "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    r = subprocess.run(["g++", *flags, *inc, "-shared", *srcs, "-o", HOST_LIB + ".tmp", *link],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host layer build failed\n{r.stderr[-6000:]}")
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    os.makedirs(os.path.dirname(SELF_TEST), exist_ok=True)
+    r = subprocess.run(["g++", *flags, *inc, os.path.join(ROOT, "tests", "cpp", "hip_self_test.cpp"),
+                        "-o", SELF_TEST + ".tmp", "-L" + HERE, "-lpdehip_host", "-lpdehip",
+                        "-Wl,-rpath,$ORIGIN/..", "-lpthread"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"self-test build failed\n{r.stderr[-6000:]}")
+    os.replace(SELF_TEST + ".tmp", SELF_TEST)
+    if verbose:
+        print(f"built {HOST_LIB} and {SELF_TEST}")
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -60,6 +89,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT}")
+    build_host(verbose)
     return OUT
 
 

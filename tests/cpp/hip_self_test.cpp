@@ -1,0 +1,251 @@
+// hip_self_test.cpp — the reference's self-test cases (src/self_test.cpp:168-411) run through
+// SolveWithBackend(..., BackendKind::HIP), plus HIP-only cases (3D Krylov/multigrid with Robin
+// faces, the time-series entry).  Prints one [OK]/[FAIL] line per case; exit code 0 when all pass.
+#include <cmath>
+#include <functional>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "backend.h"
+#include "backend_capability_matrix.h"
+#include "hip_solver.h"
+#include "solver.h"
+
+namespace {
+
+BoundaryCondition::Expression Linear(double c, double x, double y, double z = 0.0) {
+  BoundaryCondition::Expression e;
+  e.constant = c; e.x = x; e.y = y; e.z = z;
+  return e;
+}
+BoundaryCondition::Expression Latex(const std::string& s) {
+  BoundaryCondition::Expression e;
+  e.latex = s;
+  return e;
+}
+BoundarySet DirichletAll(const BoundaryCondition::Expression& e) {
+  BoundarySet bc;
+  for (BoundaryCondition* f : {&bc.left, &bc.right, &bc.bottom, &bc.top, &bc.front, &bc.back}) {
+    f->kind = BCKind::Dirichlet;
+    f->value = e;
+  }
+  return bc;
+}
+Domain Unit(int n, int nz = 1) {
+  Domain d;
+  d.nx = n; d.ny = n; d.nz = nz;
+  return d;
+}
+void Configure(SolveInput* in, SolveMethod m, int max_iter, double tol) {
+  in->solver = {};
+  in->solver.method = m;
+  in->solver.max_iter = max_iter;
+  in->solver.tol = tol;
+  in->solver.sor_omega = 1.7;      // self_test.cpp:121
+  in->solver.gmres_restart = 30;
+}
+
+// ComputeErrorNorms2D/3D-like max norm against the exact solution.
+double ErrInf(const Domain& d, const std::vector<double>& u,
+              const std::function<double(double, double, double)>& exact) {
+  const double dx = (d.xmax - d.xmin) / (d.nx - 1), dy = (d.ymax - d.ymin) / (d.ny - 1);
+  const double dz = d.nz > 1 ? (d.zmax - d.zmin) / (d.nz - 1) : 0.0;
+  double m = 0.0;
+  for (int k = 0; k < std::max(1, d.nz); ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        const size_t id = ((size_t)k * d.ny + j) * d.nx + i;
+        m = std::max(m, std::abs(u[id] - exact(d.xmin + i * dx, d.ymin + j * dy, d.zmin + k * dz)));
+      }
+  return m;
+}
+
+int g_fail = 0, g_total = 0;
+
+void Run(const std::string& name, const SolveInput& in,
+         const std::function<double(double, double, double)>& exact, double max_err) {
+  ++g_total;
+  BackendKind sel = BackendKind::Auto;
+  std::string note;
+  SolveOutput out = SolveWithBackend(in, BackendKind::HIP, &sel, &note);
+  if (!out.error.empty()) {
+    std::cout << "[FAIL] " << name << ": " << out.error << "\n";
+    ++g_fail;
+    return;
+  }
+  const double e = ErrInf(in.domain, out.grid, exact);
+  const bool pass = e <= max_err && sel == BackendKind::HIP;
+  std::cout << (pass ? "[OK] " : "[FAIL] ") << name << " backend=" << BackendKindName(sel)
+            << " errinf=" << e << " tol=" << max_err << " resid2=" << out.residual_l2
+            << " residinf=" << out.residual_linf << "\n";
+  if (!pass) ++g_fail;
+}
+
+}  // namespace
+
+int main() {
+  std::string note;
+  if (!HipIsAvailable(&note)) {
+    std::cout << "[FAIL] HIP unavailable: " << note << "\n";
+    return 1;
+  }
+  std::cout << "HIP: " << note << "\n";
+  const auto harmonic = [](double x, double y, double) { return x + 2.0 * y; };
+  const auto zero = [](double, double, double) { return 0.0; };
+
+  // Solver coverage (self_test.cpp:241-277): 33^2, tol 1e-8.
+  SolveInput h;
+  h.domain = Unit(33);
+  h.pde.a = -1.0; h.pde.b = -1.0;
+  h.bc = DirichletAll(Linear(0.0, 1.0, 2.0));
+  SolveInput z = h;
+  z.bc = DirichletAll(Linear(0.0, 0.0, 0.0));
+  struct M { SolveMethod m; const char* name; } methods[] = {
+      {SolveMethod::SOR, "SOR"}, {SolveMethod::CG, "CG"}, {SolveMethod::BiCGStab, "BiCGStab"},
+      {SolveMethod::GMRES, "GMRES"}, {SolveMethod::MultigridVcycle, "MG"}};
+  for (const M& m : methods) {
+    const bool krylov = m.m == SolveMethod::CG || m.m == SolveMethod::BiCGStab || m.m == SolveMethod::GMRES;
+    SolveInput in = krylov ? z : h;
+    Configure(&in, m.m, m.m == SolveMethod::MultigridVcycle ? 120 : 4000, 1e-8);
+    const double thr = krylov ? 1e-8 : (m.m == SolveMethod::MultigridVcycle ? 5e-6 : 1e-6);
+    Run(std::string(krylov ? "Poisson zero (" : "Poisson harmonic (") + m.name + ")", in,
+        krylov ? zero : harmonic, thr);
+  }
+  // CG on non-zero Dirichlet data (the reference's CG diverges on these rows)
+  {
+    SolveInput in = h;
+    Configure(&in, SolveMethod::CG, 4000, 1e-10);
+    Run("Poisson harmonic (CG, folded Dirichlet rows)", in, harmonic, 1e-9);
+  }
+  // Regression suite (self_test.cpp:279-340): 21^2, SOR, tol 1e-10.
+  {
+    SolveInput in;
+    in.domain = Unit(21);
+    in.pde.a = -1.0; in.pde.b = -1.0; in.pde.e = 1.0; in.pde.f = -1.0;
+    in.bc = DirichletAll(Linear(1.0, 0.0, 0.0));
+    Configure(&in, SolveMethod::SOR, 6000, 1e-10);
+    Run("Helmholtz constant (SOR)", in, [](double, double, double) { return 1.0; }, 1e-7);
+  }
+  {
+    SolveInput in;
+    in.domain = Unit(21);
+    in.pde.a = -2.0; in.pde.b = -3.0; in.pde.f = 10.0;
+    in.bc = DirichletAll(Latex("x^2 + y^2"));
+    Configure(&in, SolveMethod::SOR, 6000, 1e-10);
+    Run("Anisotropic diffusion (SOR)", in, [](double x, double y, double) { return x * x + y * y; }, 1e-7);
+  }
+  // MMS convergence (self_test.cpp:168-221): CG, 17^2 vs 33^2, ratio > 3.
+  {
+    ++g_total;
+    SolveInput in;
+    in.pde.a = -1.0; in.pde.b = -1.0;
+    in.bc = DirichletAll(Latex("\\sin(\\pi*x)*\\sin(\\pi*y)"));
+    in.pde.rhs_latex = "-(-(-\\pi^2*\\sin(\\pi*x)*\\sin(\\pi*y))-(-\\pi^2*\\sin(\\pi*x)*\\sin(\\pi*y)))";
+    Configure(&in, SolveMethod::CG, 8000, 1e-10);
+    const auto ex = [](double x, double y, double) { return std::sin(M_PI * x) * std::sin(M_PI * y); };
+    in.domain = Unit(17);
+    SolveOutput c = SolveWithBackend(in, BackendKind::HIP, nullptr, nullptr);
+    in.domain = Unit(33);
+    SolveOutput f = SolveWithBackend(in, BackendKind::HIP, nullptr, nullptr);
+    if (!c.error.empty() || !f.error.empty()) {
+      std::cout << "[FAIL] MMS convergence (CG): " << c.error << f.error << "\n";
+      ++g_fail;
+    } else {
+      const double ec = ErrInf(Unit(17), c.grid, ex), ef = ErrInf(Unit(33), f.grid, ex);
+      const double ratio = ec / std::max(1e-14, ef);
+      std::cout << (ratio > 3.0 ? "[OK] " : "[FAIL] ") << "MMS convergence (CG) errinf_coarse=" << ec
+                << " errinf_fine=" << ef << " ratio=" << ratio << "\n";
+      if (!(ratio > 3.0)) ++g_fail;
+    }
+  }
+  // HIP-only: 3D Krylov and multigrid with Robin faces (outward), manufactured solution.
+  {
+    SolveInput in;
+    in.domain = Unit(33, 33);
+    in.pde.a = in.pde.b = in.pde.az = -1.0;
+    in.pde.rhs_latex = "-3*\\pi^2*\\sin(\\pi*x)*\\sin(\\pi*y)*\\sin(\\pi*z)";
+    in.bc = DirichletAll(Linear(0.0, 0.0, 0.0));
+    const auto ex = [](double x, double y, double zz) {
+      return std::sin(M_PI * x) * std::sin(M_PI * y) * std::sin(M_PI * zz);
+    };
+    Configure(&in, SolveMethod::CG, 4000, 1e-10);
+    Run("3D Poisson (CG)", in, ex, 2e-3);
+    Configure(&in, SolveMethod::MultigridVcycle, 100, 1e-10);
+    in.solver.hip_mg_pcg = true;
+    Run("3D Poisson (MG-PCG)", in, ex, 2e-3);
+    in.solver.hip_precision = 1;
+    Run("3D Poisson (MG-PCG mixed precision)", in, ex, 2e-3);
+  }
+  {
+    // Robin (outward) faces: u = cos(pi x/2)... use a closed form u = 1 + x with
+    // du/dn on x faces handled by Robin alpha*u + beta*du/dn = gamma
+    SolveInput in;
+    in.domain = Unit(33);
+    in.pde.a = -1.0; in.pde.b = -1.0;
+    in.bc = DirichletAll(Linear(1.0, 1.0, 0.0));
+    in.bc.left.kind = BCKind::Robin;     // at x=0: u=1, du/dn = -1 -> 1*u + 1*(-1) = 0
+    in.bc.left.alpha = Linear(1.0, 0, 0); in.bc.left.beta = Linear(1.0, 0, 0); in.bc.left.gamma = Linear(0.0, 0, 0);
+    in.bc.right.kind = BCKind::Neumann;  // du/dx = 1 (axis convention = outward on the high face)
+    in.bc.right.value = Linear(1.0, 0, 0);
+    Configure(&in, SolveMethod::CG, 4000, 1e-11);
+    in.solver.hip_outward_robin = true;
+    Run("Linear solution, Robin + Neumann faces (CG folded rows)", in,
+        [](double x, double, double) { return 1.0 + x; }, 1e-8);
+    Configure(&in, SolveMethod::MultigridVcycle, 200, 1e-11);
+    in.solver.hip_outward_robin = true;
+    Run("Linear solution, Robin + Neumann faces (MG)", in, [](double x, double, double) { return 1.0 + x; }, 1e-8);
+  }
+  // Capability matrix: HIP lifts the 2D/Dirichlet-only gates, keeps the shape gate.
+  {
+    ++g_total;
+    SolveInput in;
+    in.domain = Unit(9, 9);
+    in.bc.front.kind = BCKind::Robin;
+    std::string r;
+    const bool hip_ok = BackendSupportsMethodForInput(BackendKind::HIP, SolveMethod::CG, in, &r);
+    const bool cuda_ok = BackendSupportsMethodForInput(BackendKind::CUDA, SolveMethod::CG, in, &r);
+    in.domain_shape = "x^2+y^2-0.25";
+    const bool shape_ok = BackendSupportsMethodForInput(BackendKind::HIP, SolveMethod::Jacobi, in, &r);
+    const bool pass = hip_ok && !cuda_ok && !shape_ok && ParseBackendKind("hip") == BackendKind::HIP;
+    std::cout << (pass ? "[OK] " : "[FAIL] ") << "capability matrix HIP row\n";
+    if (!pass) ++g_fail;
+  }
+  // Time series: heat equation decay of sin(pi x) sin(pi y): u(t) ~ exp(-2 pi^2 t) (FE, RK4).
+  for (TimeIntegrator ti : {TimeIntegrator::ForwardEuler, TimeIntegrator::RK4}) {
+    ++g_total;
+    SolveInput in;
+    in.domain = Unit(33);
+    in.pde.a = -1.0; in.pde.b = -1.0; in.pde.ut = 1.0;
+    in.bc = DirichletAll(Linear(0.0, 0.0, 0.0));
+    const int n = 33;
+    in.initial_grid.resize(n * n);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i)
+        in.initial_grid[j * n + i] = std::sin(M_PI * i / 32.0) * std::sin(M_PI * j / 32.0);
+    in.time.enabled = true;
+    in.time.dt = 0.2 / (32.0 * 32.0);
+    in.time.frames = 201;
+    in.time.integrator = ti;
+    int frames_seen = 0;
+    std::vector<double> last;
+    SolveOutput out = SolvePDETimeSeriesHip(in, [&](int, double, const std::vector<double>& g,
+                                                    const std::vector<double>*) {
+      ++frames_seen;
+      last = g;
+      return true;
+    });
+    const double t = in.time.dt * 200;
+    const double h = 1.0 / 32.0;
+    const double lam = 8.0 / (h * h) * std::pow(std::sin(M_PI * h / 2.0), 2);  // discrete eigenvalue
+    const double expect = std::exp(-lam * t);
+    const double mid = last.empty() ? 0.0 : last[16 * n + 16];
+    const bool pass = out.error.empty() && frames_seen == 201 && std::abs(mid - expect) < 5e-3 * expect;
+    std::cout << (pass ? "[OK] " : "[FAIL] ") << "heat decay time series ("
+              << (ti == TimeIntegrator::RK4 ? "RK4" : "FE") << ") u_mid=" << mid << " expect=" << expect
+              << " frames=" << frames_seen << " " << out.error << "\n";
+    if (!pass) ++g_fail;
+  }
+  std::cout << (g_total - g_fail) << "/" << g_total << " passed\n";
+  return g_fail == 0 ? 0 : 1;
+}
