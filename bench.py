@@ -145,10 +145,13 @@ def main() -> int:
     err_mx = float(np.linalg.norm(u_mx - u_ref) / nref)
     err_64 = float(np.linalg.norm(u_64 - u_ref) / nref)
 
-    # ---- roofline of the dominant kernel (fine-level red-black GS sweep) with HIP events
-    ms_mx, bytes_mx = planmx.time_kernel(1, capi.MIXED, reps=20)
-    ms_64, bytes_64 = plan64.time_kernel(1, capi.FP64, reps=20)
+    # ---- roofline of the dominant kernel (fine-level fused red-black GS sweep) with HIP events
+    # on the plan's stream (the stream the solver launches it on)
+    ms_mx, bytes_mx = planmx.time_kernel(4, capi.MIXED, reps=20)
+    ms_64, bytes_64 = plan64.time_kernel(4, capi.FP64, reps=20)
+    ms_ip, bytes_ip = planmx.time_kernel(1, capi.MIXED, reps=20)
     ms_ap, bytes_ap = plan64.time_kernel(0, capi.FP64, reps=20)
+    ms_rs, bytes_rs = plan64.time_kernel(2, capi.FP64, reps=20)
     ms_cp, bytes_cp = plan64.time_kernel(3, capi.FP64, reps=20)
     gbs = lambda b, ms: b / (ms * 1e-3) / 1e9  # noqa: E731
     traffic = None
@@ -158,13 +161,15 @@ def main() -> int:
             traffic = json.load(open(tpath)).get("rbgs_sweep_fp32_bytes_per_launch")
         except Exception:
             traffic = None
-    roof = {"bound": "hbm", "kernel": "rbgs sweep (red+black), FP32, fine level 257^3",
+    roof = {"bound": "hbm", "kernel": f"k_rbgs_fused3 (red+black sweep, one launch), FP32, fine level {n}^3",
             "achieved": round(gbs(bytes_mx, ms_mx), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs(bytes_mx, ms_mx) / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes": bytes_mx, "avg_ms": round(ms_mx, 4)}
     kernels = {
         "rbgs_sweep_fp64": {"avg_ms": round(ms_64, 4), "GB/s": round(gbs(bytes_64, ms_64), 1)},
+        "rbgs_inplace_2launch_fp32": {"avg_ms": round(ms_ip, 4), "GB/s": round(gbs(bytes_ip, ms_ip), 1)},
         "apply_fp64": {"avg_ms": round(ms_ap, 4), "GB/s": round(gbs(bytes_ap, ms_ap), 1)},
+        "residual_fp64": {"avg_ms": round(ms_rs, 4), "GB/s": round(gbs(bytes_rs, ms_rs), 1)},
         "stream_copy_fp64": {"avg_ms": round(ms_cp, 4), "GB/s": round(gbs(bytes_cp, ms_cp), 1)},
     }
     log(f"roofline: rbgs fp32 {roof['achieved']} GB/s, rbgs fp64 {kernels['rbgs_sweep_fp64']['GB/s']} GB/s, "

@@ -213,12 +213,22 @@ int pdehip_plan_download(pdehip_plan* plan, double* u_out);
 /* Bytes of device memory the plan holds. */
 size_t pdehip_plan_bytes(const pdehip_plan* plan);
 /* Time `reps` back-to-back launches of one kernel of the plan with HIP events on the plan's
- * stream.  kernel: 0 = fine-level operator apply, 1 = fine-level smoother sweep (both colours),
- * 2 = fine-level residual+restriction, 3 = stream copy of one fine vector (bandwidth probe).
- * Returns the mean milliseconds per launch and the algorithmic bytes per launch. */
+ * stream.  kernel: 0 = fine-level operator apply, 1 = fine-level red-black sweep as two in-place
+ * colour launches, 2 = fine-level residual, 3 = stream copy of one fine vector (bandwidth
+ * probe), 4 = fine-level red-black sweep as one fused out-of-place launch.
+ * Returns the mean milliseconds per sweep/launch and the algorithmic bytes per sweep/launch. */
 int pdehip_plan_time_kernel(pdehip_plan* plan, int kernel, int precision, int reps,
                             double* avg_ms, double* algorithmic_bytes);
 void pdehip_plan_destroy(pdehip_plan* plan);
+
+/* ---------------------------------------------------------------- launch tuning */
+/* Process-wide launch-shape knobs (no counterpart in the reference; used by the autotuner and
+ * for A/B timing).  Keys: "smooth.fused" (0/1: multigrid smooths with the fused out-of-place
+ * sweep), "smooth.rows" (2 or 4: rows per thread of the 3D sweep tile), "smooth.zchunk" (z
+ * planes per 3D workgroup, 4..512), "smooth.ychunk" (2D rows per wavefront, 16..4096).
+ * Results do not depend on any of them.  Not thread-safe against running solves. */
+int pdehip_tune_set(const char* key, int value);
+int pdehip_tune_get(const char* key, int* value);
 
 /* ---------------------------------------------------------------- time series */
 /* TimeIntegrator (include/pde_types.h:310-319); explicit ones only. */

@@ -241,6 +241,31 @@ int pdehip_plan_time_kernel(pdehip_plan* plan, int kernel, int precision, int re
   });
 }
 
+int pdehip_tune_set(const char* key, int value) {
+  if (!key) return PDEHIP_ERR_INVALID;
+  SmoothTuning t = smooth_tuning();
+  const std::string k(key);
+  if (k == "smooth.fused" && (value == 0 || value == 1)) t.fused = value;
+  else if (k == "smooth.rows" && (value == 2 || value == 4)) t.rows = value;
+  else if (k == "smooth.zchunk" && value >= 4 && value <= 512) t.zchunk = value;
+  else if (k == "smooth.ychunk" && value >= 16 && value <= 4096) t.ychunk = value;
+  else return PDEHIP_ERR_INVALID;
+  set_smooth_tuning(t);
+  return PDEHIP_OK;
+}
+
+int pdehip_tune_get(const char* key, int* value) {
+  if (!key || !value) return PDEHIP_ERR_INVALID;
+  const SmoothTuning t = smooth_tuning();
+  const std::string k(key);
+  if (k == "smooth.fused") *value = t.fused;
+  else if (k == "smooth.rows") *value = t.rows;
+  else if (k == "smooth.zchunk") *value = t.zchunk;
+  else if (k == "smooth.ychunk") *value = t.ychunk;
+  else return PDEHIP_ERR_INVALID;
+  return PDEHIP_OK;
+}
+
 void pdehip_plan_destroy(pdehip_plan* plan) {
   if (!plan) return;
   {

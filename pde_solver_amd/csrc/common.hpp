@@ -30,7 +30,6 @@ struct DevOp {
   const void* kappa;     // FLUX conductivity (element type = kernel's TK)
   double fx, fy, fz;     // FLUX: a/h^2 per axis
   double e;              // FLUX: constant reaction term
-  const double* e_field; // FLUX: per-node reaction term (or null)
   double theta[6];       // folded closure factors (0 = not folded / Dirichlet)
 };
 
@@ -72,8 +71,29 @@ __device__ __forceinline__ T face_coef(T kc, T kn, T f) {
   return s > T(0) ? f * (T(2) * kc * kn) / s : T(0);
 }
 
-// Weights of interior row (i,j,k) before folding.
-template <int DIM, int KIND, typename T, typename TK>
+// Reciprocal for the smoothers: the hardware estimate (float, 1 ulp) or the estimate refined by
+// two Newton steps (double).  The operator itself (apply, residual) divides exactly; a
+// smoother only has to be the same in every kernel that applies it.
+template <typename T> __device__ __forceinline__ T recip(T x);
+template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double recip<double>(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
+// Smoother face coefficient: face_coef with the division replaced by recip().  Symmetric bit
+// for bit in (kc, kn), so a face computed from either side agrees.
+template <typename T>
+__device__ __forceinline__ T face_smooth(T kc, T kn, T f) {
+  const T s = kc + kn;
+  return s > T(0) ? f * (T(2) * kc * kn) * recip<T>(s) : T(0);
+}
+
+// Weights of interior row (i,j,k) before folding.  SMOOTH: faces by face_smooth (the
+// smoothers' operator), else by face_coef (the exact operator).
+template <int DIM, int KIND, typename T, typename TK, bool SMOOTH = false>
 __device__ __forceinline__ W7<T> row_weights(const DevOp& op, long long id) {
   W7<T> w;
   if constexpr (KIND == PDEHIP_OP_CONST) {
@@ -90,36 +110,53 @@ __device__ __forceinline__ W7<T> row_weights(const DevOp& op, long long id) {
     const TK* k = static_cast<const TK*>(op.kappa);
     const T kc = Conv<T>::from(k[id]);
     const T fx = T(op.fx), fy = T(op.fy);
-    w.W = face_coef<T>(kc, Conv<T>::from(k[id - 1]), fx);
-    w.E = face_coef<T>(kc, Conv<T>::from(k[id + 1]), fx);
-    w.S = face_coef<T>(kc, Conv<T>::from(k[id - op.nx]), fy);
-    w.N = face_coef<T>(kc, Conv<T>::from(k[id + op.nx]), fy);
+    auto face = [](T a, T b, T f) { return SMOOTH ? face_smooth<T>(a, b, f) : face_coef<T>(a, b, f); };
+    w.W = face(kc, Conv<T>::from(k[id - 1]), fx);
+    w.E = face(kc, Conv<T>::from(k[id + 1]), fx);
+    w.S = face(kc, Conv<T>::from(k[id - op.nx]), fy);
+    w.N = face(kc, Conv<T>::from(k[id + op.nx]), fy);
     T sum = w.W + w.E + w.S + w.N;
     if constexpr (DIM == 3) {
       const T fz = T(op.fz);
-      w.B = face_coef<T>(kc, Conv<T>::from(k[id - op.sxy]), fz);
-      w.F = face_coef<T>(kc, Conv<T>::from(k[id + op.sxy]), fz);
+      w.B = face(kc, Conv<T>::from(k[id - op.sxy]), fz);
+      w.F = face(kc, Conv<T>::from(k[id + op.sxy]), fz);
       sum += w.B + w.F;
     } else {
       w.B = T(0); w.F = T(0);
     }
-    const T e = op.e_field ? T(op.e_field[id]) : T(op.e);
-    w.C = e - sum;
+    w.C = T(op.e) - sum;
   }
   return w;
 }
 
-// Folding: an interior row next to a folded face adds w_nb*theta_f to its centre.
+// Folding: an interior row next to a folded face adds w_nb*theta_f to its centre (explicit FMA,
+// so every kernel that folds a row rounds it the same way).
 template <int DIM, typename T>
 __device__ __forceinline__ void fold_center(const DevOp& op, int i, int j, int k, W7<T>& w) {
-  if (i == 1) w.C += w.W * T(op.theta[0]);
-  if (i == op.nx - 2) w.C += w.E * T(op.theta[1]);
-  if (j == 1) w.C += w.S * T(op.theta[2]);
-  if (j == op.ny - 2) w.C += w.N * T(op.theta[3]);
+  if (i == 1) w.C = fma(w.W, T(op.theta[0]), w.C);
+  if (i == op.nx - 2) w.C = fma(w.E, T(op.theta[1]), w.C);
+  if (j == 1) w.C = fma(w.S, T(op.theta[2]), w.C);
+  if (j == op.ny - 2) w.C = fma(w.N, T(op.theta[3]), w.C);
   if constexpr (DIM == 3) {
-    if (k == 1) w.C += w.B * T(op.theta[4]);
-    if (k == op.nz - 2) w.C += w.F * T(op.theta[5]);
+    if (k == 1) w.C = fma(w.B, T(op.theta[4]), w.C);
+    if (k == op.nz - 2) w.C = fma(w.F, T(op.theta[5]), w.C);
   }
+}
+
+// Gauss-Seidel value of one interior row: (b - sum_nb w_nb u_nb) * recip(C), the neighbour sum
+// as a fixed FMA chain (E, W, N, S, F, B).  Every smoother kernel uses this, so the fused and
+// the in-place sweeps agree bit for bit whatever the compiler's contraction choices.
+template <int DIM, typename T>
+__device__ __forceinline__ T gs_value(const W7<T>& w, T b, T ue, T uw, T un, T us, T uf, T ub) {
+  T s = w.E * ue;
+  s = fma(w.W, uw, s);
+  s = fma(w.N, un, s);
+  s = fma(w.S, us, s);
+  if constexpr (DIM == 3) {
+    s = fma(w.F, uf, s);
+    s = fma(w.B, ub, s);
+  }
+  return (b - s) * recip<T>(w.C);
 }
 
 __device__ __forceinline__ bool is_boundary2(const DevOp& op, int i, int j) {

@@ -1,0 +1,482 @@
+// k_smooth.hip — fused red-black Gauss-Seidel sweep for gfx950: both colours of one sweep in
+// a single pass over HBM, out of place (x_in -> x_out).
+//
+// The two-launch in-place sweep (k_rbgs) touches every cache line of x, b and kappa once per
+// colour, so one sweep moves about twice the compulsory bytes.  Here one workgroup owns an
+// (i,j) tile and marches it through a chunk of z planes:
+//
+//   * extended tile: 64 columns (one wavefront, lane = x) by R*8 rows (8 waves x R rows per
+//     thread); the output tile is the extended tile minus a 2-node ring (60 x (8R-4));
+//   * for plane p the first colour is updated on the tile plus a 1-node ring from old values
+//     of the second colour (planes p-1, p, p+1), then the second colour of plane p-1 is
+//     updated on the tile from the new first-colour values (planes p-2, p-1, p) — the ring
+//     makes the tile self-contained, so no workgroup waits for another;
+//   * z neighbours of a node live in the owning thread's registers (4-plane queue of x and of
+//     kappa, 2 planes of b); in-plane neighbours come from two LDS plane slots;
+//   * x_out receives every node of the output tile (boundary nodes keep x_in).
+//
+// Every node's update uses the same weights, the same FMA chain (gs_value) and the same rounding
+// to the storage type as the in-place two-colour sweep, so x_out equals bit for bit what
+// rbgs(c1); rbgs(1-c1) would leave in x.  HBM traffic per sweep is the compulsory
+// (3 s + s_k) N; the ring and the two extra planes per z chunk are re-reads served by L2 /
+// Infinity Cache.  Workgroups are numbered so that each XCD (round-robin dispatch) receives a
+// contiguous run of tiles, which keeps ring re-reads inside one XCD's L2.
+#include "kernels.hpp"
+
+namespace pdehip {
+namespace {
+
+constexpr int kLanes = 64;
+constexpr int kHalo = 2;
+constexpr int kTileX = kLanes - 2 * kHalo;
+constexpr int kWaves = 8;
+constexpr int kThreads = kLanes * kWaves;
+
+template <Prec P> struct PT;
+template <> struct PT<P64> { using T = double; using S = double; };
+template <> struct PT<P32> { using T = float; using S = float; };
+template <> struct PT<P16> { using T = float; using S = __half; };
+
+template <typename S> __device__ __forceinline__ auto ldS(const S* p, long long i);
+template <> __device__ __forceinline__ auto ldS<double>(const double* p, long long i) { return p[i]; }
+template <> __device__ __forceinline__ auto ldS<float>(const float* p, long long i) { return p[i]; }
+template <> __device__ __forceinline__ auto ldS<__half>(const __half* p, long long i) { return __half2float(p[i]); }
+
+// Value as the in-place sweep would read it back: rounded to the storage type.
+template <typename S, typename T> __device__ __forceinline__ T rounded(T v);
+template <> __device__ __forceinline__ double rounded<double, double>(double v) { return v; }
+template <> __device__ __forceinline__ float rounded<float, float>(float v) { return v; }
+template <> __device__ __forceinline__ float rounded<__half, float>(float v) { return __half2float(__float2half(v)); }
+
+template <typename S, typename T> __device__ __forceinline__ S to_store(T v);
+template <> __device__ __forceinline__ double to_store<double, double>(double v) { return v; }
+template <> __device__ __forceinline__ float to_store<float, float>(float v) { return v; }
+template <> __device__ __forceinline__ __half to_store<__half, float>(float v) { return __float2half(v); }
+
+__device__ __forceinline__ bool skip(const KScalars* sc) { return sc && (sc->done | sc->breakdown); }
+
+// Smoother weights of an interior row from face coefficients already at hand (FLUX: W, E, S,
+// N, B, F faces computed by face_smooth), or from the operator (CONST, POINTWISE).  The sum and
+// the centre follow row_weights<..., SMOOTH=true> operation for operation.
+template <int DIM, int KIND, typename T>
+__device__ __forceinline__ W7<T> weights_at(const DevOp& op, long long id, T fw, T fe, T fs,
+                                            T fn, T fb, T ff) {
+  W7<T> w;
+  if constexpr (KIND == PDEHIP_OP_CONST) {
+    w.C = T(op.w[0]); w.W = T(op.w[1]); w.E = T(op.w[2]); w.S = T(op.w[3]); w.N = T(op.w[4]);
+    w.B = T(op.w[5]); w.F = T(op.w[6]);
+  } else if constexpr (KIND == PDEHIP_OP_POINTWISE) {
+    const double* p = op.pw;
+    const long long n = op.n;
+    w.C = T(p[id]); w.W = T(p[n + id]); w.E = T(p[2 * n + id]); w.S = T(p[3 * n + id]);
+    w.N = T(p[4 * n + id]);
+    if constexpr (DIM == 3) { w.B = T(p[5 * n + id]); w.F = T(p[6 * n + id]); }
+    else { w.B = T(0); w.F = T(0); }
+  } else {
+    w.W = fw; w.E = fe; w.S = fs; w.N = fn;
+    T sum = w.W + w.E + w.S + w.N;
+    if constexpr (DIM == 3) {
+      w.B = fb; w.F = ff;
+      sum += w.B + w.F;
+    } else {
+      w.B = T(0); w.F = T(0);
+    }
+    w.C = T(op.e) - sum;
+  }
+  return w;
+}
+
+// Tile index with XCD-contiguous numbering: dispatch slot b goes to XCD b % 8; tile
+// (b % 8) * per + b / 8 gives XCD x the tiles [x*per, (x+1)*per).  gridDim.x = 8 * per.
+__device__ __forceinline__ int xcd_tile() {
+  const int per = static_cast<int>(gridDim.x >> 3);
+  return static_cast<int>(blockIdx.x & 7) * per + static_cast<int>(blockIdx.x >> 3);
+}
+
+__device__ __forceinline__ int mod3(int p) { return ((p % 3) + 3) % 3; }
+
+// ------------------------------------------------------------------------------- 3D
+// Plane step p (p runs from the prologue start to k1):
+//   1. queues advance; loads for x(p+1), b(p+1), kappa(p+3) are issued (consumed one or more
+//      steps later, so their latency overlaps the step's arithmetic);
+//   2. x(p) and kappa(p+1) go to LDS; barrier;
+//   3. FLUX: every node of the extended tile computes its east, north and front faces of plane
+//      p+1 (face ring slot (p+1) mod 3 in LDS; front faces in registers) — each face once;
+//   4. first colour on plane p (tile + ring): faces of slot p mod 3, z faces from registers;
+//   5. second colour on plane p-1 (tile): faces of slot (p-1) mod 3; plane p-1 of the tile is
+//      stored; barrier.
+// Loads are issued unconditionally (out-of-grid lanes read element 0 and discard it), so the
+// compiler can count outstanding loads instead of draining them at every branch.
+template <int KIND, Prec P, int R, bool ZERO>
+__global__ void __launch_bounds__(kThreads)
+k_rbgs_fused3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v,
+              void* __restrict__ xout_v, int c1, int tiles_x, int tiles_y, int ntiles,
+              int kchunk, const KScalars* sc) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  constexpr int TYE = R * kWaves;
+  constexpr int TY = TYE - 2 * kHalo;
+  constexpr bool HK = KIND == PDEHIP_OP_FLUX;
+  constexpr int KR = HK ? TYE : 1;
+  __shared__ T lu[2][TYE][kLanes];
+  __shared__ T lk[KR][kLanes];
+  __shared__ T lfe[HK ? 3 : 1][KR][kLanes];
+  __shared__ T lfn[HK ? 3 : 1][KR][kLanes];
+  if (skip(sc)) return;
+  const int t = xcd_tile();
+  if (t >= ntiles) return;
+  const S* __restrict__ b = static_cast<const S*>(bv);
+  const S* __restrict__ xin = static_cast<const S*>(xin_v);
+  const T* __restrict__ kap = static_cast<const T*>(op.kappa);
+  S* __restrict__ xout = static_cast<S*>(xout_v);
+  const int tx = t % tiles_x;
+  const int rest = t / tiles_x;
+  const int ty = rest % tiles_y;
+  const int cz = rest / tiles_y;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wv = threadIdx.x / kLanes;
+  const int i = tx * kTileX - kHalo + lane;
+  const int k0 = cz * kchunk;
+  const int k1 = min(k0 + kchunk, op.nz);
+  const int nz = op.nz;
+  const int c2 = 1 - c1;
+  const long long sxy = op.sxy;
+  const T fx = T(op.fx), fy = T(op.fy), fz = T(op.fz);
+
+  const bool in_x = i >= 0 && i < op.nx;
+  const bool int_x = i >= 1 && i <= op.nx - 2;
+  const bool ring_x = lane >= 1 && lane <= kLanes - 2;
+  const bool tile_x = lane >= kHalo && lane < kLanes - kHalo;
+
+  // per-row flags (bit r): in grid, first-colour candidate (ring & interior), second-colour
+  // candidate (tile & interior), output (tile & in grid)
+  unsigned m_in = 0, m_r1 = 0, m_r2 = 0, m_out = 0;
+  long long col[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int lr = wv * R + r;
+    const int j = ty * TY - kHalo + lr;
+    const bool in_y = j >= 0 && j < op.ny;
+    const bool int_y = j >= 1 && j <= op.ny - 2;
+    const bool ring_y = lr >= 1 && lr <= TYE - 2;
+    const bool tile_y = lr >= kHalo && lr < TYE - kHalo;
+    col[r] = in_x && in_y ? (long long)j * op.nx + i : 0;
+    if (in_x && in_y) m_in |= 1u << r;
+    if (ring_x && ring_y && int_x && int_y) m_r1 |= 1u << r;
+    if (tile_x && tile_y && int_x && int_y) m_r2 |= 1u << r;
+    if (tile_x && tile_y && in_x && in_y) m_out |= 1u << r;
+  }
+  const int jrow0 = ty * TY - kHalo + wv * R;
+
+  auto ld_u = [&](int r, int k) -> T {
+    if constexpr (ZERO) {
+      return T(0);
+    } else {
+      const bool ok = ((m_in >> r) & 1u) && k >= 0 && k < nz;
+      const T v = T(ldS(xin, ok ? k * sxy + col[r] : 0));
+      return ok ? v : T(0);
+    }
+  };
+  auto ld_k = [&](int r, int k) -> T {
+    if constexpr (HK) {
+      const bool ok = ((m_in >> r) & 1u) && k >= 0 && k < nz;
+      const T v = kap[ok ? k * sxy + col[r] : 0];
+      return ok ? v : T(0);
+    } else {
+      return T(0);
+    }
+  };
+  auto ld_b = [&](int r, int k) -> T {
+    const bool ok = ((m_r1 >> r) & 1u) && k >= 1 && k < nz - 1;
+    const T v = T(ldS(b, ok ? k * sxy + col[r] : 0));
+    return ok ? v : T(0);
+  };
+
+  // queues (after the shift of step p): U = x(p-2..p+1), B = b(p-1..p+1), K = kappa(p+1..p+3),
+  // ZF = front faces zf(p-2..p+1) with zf(k) = face(kappa(k), kappa(k+1))
+  const int pstart = HK ? k0 - 3 : k0 - 1;
+  T U[R][4], B[R][3], K[R][3], ZF[R][4];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    U[r][0] = U[r][1] = T(0);
+    U[r][2] = ld_u(r, pstart - 1);
+    U[r][3] = ld_u(r, pstart);
+    B[r][0] = B[r][1] = T(0);
+    B[r][2] = ld_b(r, pstart);
+    K[r][0] = T(0);
+    K[r][1] = ld_k(r, pstart + 1);
+    K[r][2] = ld_k(r, pstart + 2);
+    ZF[r][0] = ZF[r][1] = ZF[r][2] = ZF[r][3] = T(0);
+  }
+
+  for (int p = pstart; p <= k1; ++p) {
+    const int sl = p & 1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int lr = wv * R + r;
+      U[r][0] = U[r][1]; U[r][1] = U[r][2]; U[r][2] = U[r][3]; U[r][3] = ld_u(r, p + 1);
+      B[r][0] = B[r][1]; B[r][1] = B[r][2]; B[r][2] = ld_b(r, p + 1);
+      if constexpr (HK) {
+        K[r][0] = K[r][1]; K[r][1] = K[r][2]; K[r][2] = ld_k(r, p + 3);
+        ZF[r][0] = ZF[r][1]; ZF[r][1] = ZF[r][2]; ZF[r][2] = ZF[r][3];
+        lk[lr][lane] = K[r][0];
+      }
+      lu[sl][lr][lane] = U[r][2];
+    }
+    __syncthreads();
+    if constexpr (HK) {
+      const int fs = mod3(p + 1);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int lr = wv * R + r;
+        const T kc = K[r][0];
+        ZF[r][3] = face_smooth<T>(kc, K[r][1], fz);
+        lfe[fs][lr][lane] = lane < kLanes - 1 ? face_smooth<T>(kc, lk[lr][lane + 1], fx) : T(0);
+        lfn[fs][lr][lane] = lr < TYE - 1 ? face_smooth<T>(kc, lk[lr + 1][lane], fy) : T(0);
+      }
+    }
+    // first colour on plane p (tile + ring)
+    if (p >= k0 - 1 && p >= 1 && p <= nz - 2) {
+      const int f0 = HK ? mod3(p) : 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int j = jrow0 + r;
+        if (((m_r1 >> r) & 1u) && ((i + j + p) & 1) == c1) {
+          const int lr = wv * R + r;
+          const long long id = p * sxy + col[r];
+          T aw = T(0), ae = T(0), as = T(0), an = T(0);
+          if constexpr (HK) {
+            aw = lfe[f0][lr][lane - 1]; ae = lfe[f0][lr][lane];
+            as = lfn[f0][lr - 1][lane]; an = lfn[f0][lr][lane];
+          }
+          W7<T> w = weights_at<3, KIND, T>(op, id, aw, ae, as, an, ZF[r][1], ZF[r][2]);
+          fold_center<3, T>(op, i, j, p, w);
+          const T v = rounded<S, T>(gs_value<3, T>(w, B[r][1], lu[sl][lr][lane + 1],
+                                                   lu[sl][lr][lane - 1], lu[sl][lr + 1][lane],
+                                                   lu[sl][lr - 1][lane], U[r][3], U[r][1]));
+          U[r][2] = v;
+          lu[sl][lr][lane] = v;
+        }
+      }
+    }
+    // second colour on plane q = p - 1 (tile), then the tile's plane q goes out
+    const int q = p - 1;
+    if (q >= k0 && q < k1) {
+      const int sq = sl ^ 1;
+      if (q >= 1 && q <= nz - 2) {
+        const int f1 = HK ? mod3(q) : 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int j = jrow0 + r;
+          if (((m_r2 >> r) & 1u) && ((i + j + q) & 1) == c2) {
+            const int lr = wv * R + r;
+            const long long id = q * sxy + col[r];
+            T aw = T(0), ae = T(0), as = T(0), an = T(0);
+            if constexpr (HK) {
+              aw = lfe[f1][lr][lane - 1]; ae = lfe[f1][lr][lane];
+              as = lfn[f1][lr - 1][lane]; an = lfn[f1][lr][lane];
+            }
+            W7<T> w = weights_at<3, KIND, T>(op, id, aw, ae, as, an, ZF[r][0], ZF[r][1]);
+            fold_center<3, T>(op, i, j, q, w);
+            U[r][1] = rounded<S, T>(gs_value<3, T>(w, B[r][0], lu[sq][lr][lane + 1],
+                                                   lu[sq][lr][lane - 1], lu[sq][lr + 1][lane],
+                                                   lu[sq][lr - 1][lane], U[r][2], U[r][0]));
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if ((m_out >> r) & 1u) xout[q * sxy + col[r]] = to_store<S, T>(U[r][1]);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------- 2D
+// One wavefront per 60-column strip; rows are the marching axis, x neighbours come from the
+// adjacent lanes (ds_bpermute), y neighbours from the thread's row queue.  No LDS, no barrier.
+// Faces: east from the lane to the right, north from the next row (both by face_smooth).
+template <int KIND, Prec P, bool ZERO>
+__global__ void __launch_bounds__(kThreads)
+k_rbgs_fused2(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v,
+              void* __restrict__ xout_v, int c1, int strips, int ntiles, int jchunk,
+              const KScalars* sc) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  constexpr bool HK = KIND == PDEHIP_OP_FLUX;
+  if (skip(sc)) return;
+  const int t = xcd_tile() * kWaves + static_cast<int>(threadIdx.x / kLanes);
+  if (t >= ntiles) return;   // whole wavefronts leave together
+  const S* __restrict__ b = static_cast<const S*>(bv);
+  const S* __restrict__ xin = static_cast<const S*>(xin_v);
+  const T* __restrict__ kap = static_cast<const T*>(op.kappa);
+  S* __restrict__ xout = static_cast<S*>(xout_v);
+  const int sx = t % strips;
+  const int cy = t / strips;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int i = sx * kTileX - kHalo + lane;
+  const int j0 = cy * jchunk;
+  const int j1 = min(j0 + jchunk, op.ny);
+  const int ny = op.ny;
+  const int c2 = 1 - c1;
+  const bool in_x = i >= 0 && i < op.nx;
+  const bool r1 = lane >= 1 && lane <= kLanes - 2 && i >= 1 && i <= op.nx - 2;
+  const bool r2 = lane >= kHalo && lane < kLanes - kHalo && i >= 1 && i <= op.nx - 2;
+  const bool out = lane >= kHalo && lane < kLanes - kHalo && in_x;
+  const long long nx = op.nx;
+  const T fx = T(op.fx), fy = T(op.fy);
+
+  auto ld_u = [&](int j) -> T {
+    if constexpr (ZERO) {
+      return T(0);
+    } else {
+      const bool ok = in_x && j >= 0 && j < ny;
+      const T v = T(ldS(xin, ok ? j * nx + i : 0));
+      return ok ? v : T(0);
+    }
+  };
+  auto ld_k = [&](int j) -> T {
+    if constexpr (HK) {
+      const bool ok = in_x && j >= 0 && j < ny;
+      const T v = kap[ok ? j * nx + i : 0];
+      return ok ? v : T(0);
+    } else {
+      return T(0);
+    }
+  };
+  auto ld_b = [&](int j) -> T {
+    const bool ok = r1 && j >= 1 && j < ny - 1;
+    const T v = T(ldS(b, ok ? j * nx + i : 0));
+    return ok ? v : T(0);
+  };
+  // queues after the shift of step p: U = x(p-2..p+1), B = b(p-1..p+1), K = kappa(p-2..p+2)
+  T U[4], B[3], K[5];
+  U[0] = U[1] = T(0); U[2] = ld_u(j0 - 2); U[3] = ld_u(j0 - 1);
+  B[0] = B[1] = T(0); B[2] = ld_b(j0 - 1);
+  K[0] = T(0); K[1] = ld_k(j0 - 3); K[2] = ld_k(j0 - 2); K[3] = ld_k(j0 - 1); K[4] = ld_k(j0);
+  for (int p = j0 - 1; p <= j1; ++p) {
+    U[0] = U[1]; U[1] = U[2]; U[2] = U[3]; U[3] = ld_u(p + 1);
+    B[0] = B[1]; B[1] = B[2]; B[2] = ld_b(p + 1);
+    if constexpr (HK) { K[0] = K[1]; K[1] = K[2]; K[2] = K[3]; K[3] = K[4]; K[4] = ld_k(p + 2); }
+    // first colour on row p
+    {
+      const T uw = __shfl_up(U[2], 1, kLanes), ue = __shfl_down(U[2], 1, kLanes);
+      T kw = T(0), ke = T(0);
+      if constexpr (HK) { kw = __shfl_up(K[2], 1, kLanes); ke = __shfl_down(K[2], 1, kLanes); }
+      if (r1 && p >= 1 && p <= ny - 2 && ((i + p) & 1) == c1) {
+        const long long id = p * nx + i;
+        T aw = T(0), ae = T(0), as = T(0), an = T(0);
+        if constexpr (HK) {
+          aw = face_smooth<T>(K[2], kw, fx); ae = face_smooth<T>(K[2], ke, fx);
+          as = face_smooth<T>(K[2], K[1], fy); an = face_smooth<T>(K[2], K[3], fy);
+        }
+        W7<T> w = weights_at<2, KIND, T>(op, id, aw, ae, as, an, T(0), T(0));
+        fold_center<2, T>(op, i, p, 0, w);
+        U[2] = rounded<S, T>(gs_value<2, T>(w, B[1], ue, uw, U[3], U[1], T(0), T(0)));
+      }
+    }
+    // second colour on row q = p - 1
+    const int q = p - 1;
+    if (q >= j0 && q < j1) {
+      const T uw = __shfl_up(U[1], 1, kLanes), ue = __shfl_down(U[1], 1, kLanes);
+      T kw = T(0), ke = T(0);
+      if constexpr (HK) { kw = __shfl_up(K[1], 1, kLanes); ke = __shfl_down(K[1], 1, kLanes); }
+      if (r2 && q >= 1 && q <= ny - 2 && ((i + q) & 1) == c2) {
+        const long long id = q * nx + i;
+        T aw = T(0), ae = T(0), as = T(0), an = T(0);
+        if constexpr (HK) {
+          aw = face_smooth<T>(K[1], kw, fx); ae = face_smooth<T>(K[1], ke, fx);
+          as = face_smooth<T>(K[1], K[0], fy); an = face_smooth<T>(K[1], K[2], fy);
+        }
+        W7<T> w = weights_at<2, KIND, T>(op, id, aw, ae, as, an, T(0), T(0));
+        fold_center<2, T>(op, i, q, 0, w);
+        U[1] = rounded<S, T>(gs_value<2, T>(w, B[0], ue, uw, U[2], U[0], T(0), T(0)));
+      }
+      if (out) xout[q * nx + i] = to_store<S, T>(U[1]);
+    }
+  }
+}
+
+SmoothTuning g_tune{};
+
+template <int KIND, Prec P>
+void launch3(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout, int c1,
+             bool zero_in) {
+  const int rows = g_tune.rows == 4 ? 4 : 2;
+  const int ty = rows * kWaves - 2 * kHalo;
+  const int tiles_x = (op.nx + kTileX - 1) / kTileX;
+  const int tiles_y = (op.ny + ty - 1) / ty;
+  // z chunk: the configured length, shortened on small levels so the launch still has about
+  // two workgroups per CU
+  int kchunk = g_tune.zchunk > 0 ? g_tune.zchunk : 32;
+  while (kchunk > 4 && (long long)tiles_x * tiles_y * ((op.nz + kchunk - 1) / kchunk) < 512)
+    kchunk /= 2;
+  const int chunks = (op.nz + kchunk - 1) / kchunk;
+  const int ntiles = tiles_x * tiles_y * chunks;
+  const dim3 g(static_cast<unsigned>(8 * ((ntiles + 7) / 8)));
+#define PDEHIP_F3(RR, ZZ)                                                                        \
+  hipLaunchKernelGGL((k_rbgs_fused3<KIND, P, RR, ZZ>), g, dim3(kThreads), 0, L.stream, op, b,     \
+                     xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc)
+  if (rows == 4) {
+    if (zero_in) PDEHIP_F3(4, true); else PDEHIP_F3(4, false);
+  } else {
+    if (zero_in) PDEHIP_F3(2, true); else PDEHIP_F3(2, false);
+  }
+#undef PDEHIP_F3
+}
+
+template <int KIND, Prec P>
+void launch2(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout, int c1,
+             bool zero_in) {
+  const int strips = (op.nx + kTileX - 1) / kTileX;
+  // rows per wavefront: enough wavefronts to fill the chip, at least 16 rows each
+  int jchunk = g_tune.ychunk > 0 ? g_tune.ychunk : 64;
+  while (jchunk > 16 && (long long)strips * ((op.ny + jchunk - 1) / jchunk) < 4096) jchunk /= 2;
+  const int chunks = (op.ny + jchunk - 1) / jchunk;
+  const int ntiles = strips * chunks;
+  const int groups = (ntiles + kWaves - 1) / kWaves;
+  const dim3 g(static_cast<unsigned>(8 * ((groups + 7) / 8)));
+  if (zero_in)
+    hipLaunchKernelGGL((k_rbgs_fused2<KIND, P, true>), g, dim3(kThreads), 0, L.stream, op, b, xin,
+                       xout, c1, strips, ntiles, jchunk, L.sc);
+  else
+    hipLaunchKernelGGL((k_rbgs_fused2<KIND, P, false>), g, dim3(kThreads), 0, L.stream, op, b, xin,
+                       xout, c1, strips, ntiles, jchunk, L.sc);
+}
+
+template <int KIND>
+void launch_kind(const Launch& L, const DevOp& op, Prec p, const void* b, const void* xin,
+                 void* xout, int c1, bool zero_in) {
+  if (op.nz > 1) {
+    switch (p) {
+      case P64: launch3<KIND, P64>(L, op, b, xin, xout, c1, zero_in); break;
+      case P32: launch3<KIND, P32>(L, op, b, xin, xout, c1, zero_in); break;
+      default: launch3<KIND, P16>(L, op, b, xin, xout, c1, zero_in); break;
+    }
+  } else {
+    switch (p) {
+      case P64: launch2<KIND, P64>(L, op, b, xin, xout, c1, zero_in); break;
+      case P32: launch2<KIND, P32>(L, op, b, xin, xout, c1, zero_in); break;
+      default: launch2<KIND, P16>(L, op, b, xin, xout, c1, zero_in); break;
+    }
+  }
+}
+
+}  // namespace
+
+void set_smooth_tuning(const SmoothTuning& t) { g_tune = t; }
+SmoothTuning smooth_tuning() { return g_tune; }
+
+void rbgs_sweep(const Launch& L, const DevOp& op, Prec p, const void* b, const void* xin,
+                void* xout, int first_color, bool zero_in) {
+  switch (op.kind) {
+    case PDEHIP_OP_CONST: launch_kind<PDEHIP_OP_CONST>(L, op, p, b, xin, xout, first_color, zero_in); break;
+    case PDEHIP_OP_POINTWISE:
+      if (op.nz > 1) launch3<PDEHIP_OP_POINTWISE, P64>(L, op, b, xin, xout, first_color, zero_in);
+      else launch2<PDEHIP_OP_POINTWISE, P64>(L, op, b, xin, xout, first_color, zero_in);
+      break;
+    default: launch_kind<PDEHIP_OP_FLUX>(L, op, p, b, xin, xout, first_color, zero_in); break;
+  }
+}
+
+}  // namespace pdehip

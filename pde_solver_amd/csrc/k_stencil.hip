@@ -122,11 +122,11 @@ __global__ void __launch_bounds__(kBlock) k_rbgs(DevOp op, const void* __restric
       const int i = 1 + ((1 + j + color) & 1) + 2 * q;
       if (i > op.nx - 2) continue;
       const long long id = (long long)j * op.nx + i;
-      W7<T> w = row_weights<2, KIND, T, K>(op, id);
+      W7<T> w = row_weights<2, KIND, T, K, true>(op, id);
       fold_center<2, T>(op, i, j, 0, w);
-      const T s = w.E * ldT(x, id + 1) + w.W * ldT(x, id - 1) + w.N * ldT(x, id + op.nx) +
-                  w.S * ldT(x, id - op.nx);
-      x[id] = store_as<S>(static_cast<double>((ldT(b, id) - s) / w.C));
+      const T v = gs_value<2, T>(w, ldT(b, id), ldT(x, id + 1), ldT(x, id - 1), ldT(x, id + op.nx),
+                                 ldT(x, id - op.nx), T(0), T(0));
+      x[id] = store_as<S>(static_cast<double>(v));
     }
   } else {
     const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -140,11 +140,11 @@ __global__ void __launch_bounds__(kBlock) k_rbgs(DevOp op, const void* __restric
     int k = k0 + ((i + j + k0 + color) & 1);
     for (; k < k1; k += 2) {
       const long long id = k * op.sxy + p;
-      W7<T> w = row_weights<3, KIND, T, K>(op, id);
+      W7<T> w = row_weights<3, KIND, T, K, true>(op, id);
       fold_center<3, T>(op, i, j, k, w);
-      const T s = w.E * ldT(x, id + 1) + w.W * ldT(x, id - 1) + w.N * ldT(x, id + op.nx) +
-                  w.S * ldT(x, id - op.nx) + w.F * ldT(x, id + op.sxy) + w.B * ldT(x, id - op.sxy);
-      x[id] = store_as<S>(static_cast<double>((ldT(b, id) - s) / w.C));
+      const T v = gs_value<3, T>(w, ldT(b, id), ldT(x, id + 1), ldT(x, id - 1), ldT(x, id + op.nx),
+                                 ldT(x, id - op.nx), ldT(x, id + op.sxy), ldT(x, id - op.sxy));
+      x[id] = store_as<S>(static_cast<double>(v));
     }
   }
 }

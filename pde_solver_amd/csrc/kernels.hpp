@@ -28,8 +28,21 @@ void apply(const Launch& L, const DevOp& op, Prec p, const void* x, void* y, dou
 void residual(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x, void* r,
               double* partials);
 // One colour of red-black Gauss-Seidel for A x = b (interior rows of colour (i+j+k)%2 == color).
-// zero_guess: x is known to be zero on entry of this colour's first sweep (skips its loads).
 void rbgs(const Launch& L, const DevOp& op, Prec p, const void* b, void* x, int color);
+// One full red-black sweep (colour first_color, then the other) out of place: x_out = the x an
+// in-place rbgs(first_color); rbgs(1 - first_color) would leave, read from x_in.  zero_in: x_in
+// is known to be zero everywhere (its loads are skipped; x_in may be null).  (k_smooth.hip)
+void rbgs_sweep(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x_in,
+                void* x_out, int first_color, bool zero_in);
+// Launch shape of the fused sweep (process-wide).
+struct SmoothTuning {
+  int fused = 1;     // 0: multigrid smooths with the two-launch in-place sweep
+  int rows = 2;      // 3D: rows per thread (tile 60 x (8*rows - 4)), 2 or 4
+  int zchunk = 32;   // 3D: z planes per workgroup
+  int ychunk = 64;   // 2D: rows per wavefront (upper bound)
+};
+void set_smooth_tuning(const SmoothTuning& t);
+SmoothTuning smooth_tuning();
 // Number of blocks `partials`-producing kernels use for this op (<= kMaxPartials).
 int partial_blocks(const DevOp& op);
 

@@ -488,6 +488,10 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
         fill(s, lv.prec, lv.u.get(), lv.g.n, 0.0);
         fill(s, lv.prec, lv.b.get(), lv.g.n, 0.0);
       }
+      if (l < nl - 1) {
+        lv.alt.alloc(ctx, lv.g.n * eb);
+        fill(s, lv.prec, lv.alt.get(), lv.g.n, 0.0);
+      }
     }
     HIPCHK(hipStreamSynchronize(s));
     if (mg) build_dense(*P, P->L[nl - 1]);
@@ -579,7 +583,28 @@ static int check_every(const Plan& P) {
 }
 
 // ------------------------------------------------------------------ multigrid cycle
-// One V-cycle on b0 (level-0 right-hand side) into u0 (level-0 correction, zero on entry).
+// `sweeps` red-black sweeps on level lv (colour c1 first in each sweep), result in u.  Pairs of
+// sweeps run as fused out-of-place sweeps u -> alt -> u; an odd last sweep runs in place.
+// zero: u is known to be zero on entry (it need not hold zeros when sweeps >= 2).
+static void smooth(const Launch& L, Level& lv, const DevOp& op, const void* b, void* u,
+                   int sweeps, int c1, bool zero) {
+  int done = 0;
+  if (lv.alt && smooth_tuning().fused) {
+    for (; done + 2 <= sweeps; done += 2) {
+      rbgs_sweep(L, op, lv.prec, b, u, lv.alt.get(), c1, zero);
+      rbgs_sweep(L, op, lv.prec, b, lv.alt.get(), u, c1, false);
+      zero = false;
+    }
+  }
+  if (zero && done < sweeps) fill(L.stream, lv.prec, u, lv.g.n, 0.0);
+  for (; done < sweeps; ++done) {
+    rbgs(L, op, lv.prec, b, u, c1);
+    rbgs(L, op, lv.prec, b, u, 1 - c1);
+  }
+}
+
+// One V-cycle on b0 (level-0 right-hand side) into u0 (level-0 correction from a zero guess;
+// u0's contents on entry are ignored).
 static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0) {
   const int nl = static_cast<int>(P.L.size());
   const int pre = std::max(0, P.opt.mg_pre), post = std::max(0, P.opt.mg_post);
@@ -591,17 +616,19 @@ static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0) {
     const DevOp& co = P.mixed ? c.op32 : c.op64;
     const void* bl = l == 0 ? b0 : f.b.get();
     void* ul = l == 0 ? u0 : f.u.get();
-    for (int it = 0; it < pre; ++it) { rbgs(L, fo, f.prec, bl, ul, 0); rbgs(L, fo, f.prec, bl, ul, 1); }
+    // every level's correction starts from zero; smooth() never reads it
+    smooth(L, f, fo, bl, ul, pre, 0, true);
+    if (pre == 0) fill(L.stream, f.prec, ul, f.g.n, 0.0);
     residual(L, fo, f.prec, bl, ul, f.r.get(), nullptr);
     restrict_fw(L, fo, f.prec, f.r.get(), co, c.prec, c.b.get(), 1.0,
                 (P.mixed && l == 0) ? &sc->scale : nullptr);
     fold_transpose(L, co, c.prec, c.b.get());
-    fill(L.stream, c.prec, c.u.get(), c.g.n, 0.0);
   }
   Level& cl = P.L[nl - 1];
   const DevOp& cop = P.mixed ? cl.op32 : cl.op64;
   void* cu = nl == 1 ? u0 : cl.u.get();
   const void* cb = nl == 1 ? b0 : cl.b.get();
+  fill(L.stream, cl.prec, cu, cl.g.n, 0.0);
   if (cl.m > 0) {
     coarse_dense(L, cl.prec, cl.ainv.as<double>(), cl.map.as<int>(), cl.m, cb, cu);
   } else {
@@ -619,7 +646,7 @@ static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0) {
     extend_closure(L, co, c.prec, c.u.get());
     prolong_add(L, co, c.prec, c.u.get(), fo, f.prec, ul, c.sigma / f.sigma,
                 (P.mixed && l == 0) ? &sc->inv_scale : nullptr);
-    for (int it = 0; it < post; ++it) { rbgs(L, fo, f.prec, bl, ul, 1); rbgs(L, fo, f.prec, bl, ul, 0); }
+    smooth(L, f, fo, bl, ul, post, 1, false);
   }
 }
 
@@ -648,7 +675,6 @@ static void solve_cg(Plan& P, bool use_mg, pdehip_report* rep) {
   finalize(L0, sc, part, np0, FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
   void* z = use_mg ? P.z.get() : P.r.get();
   if (use_mg) {
-    fill(s, P64, z, n, 0.0);
     vcycle(P, L, P.r.get(), z);
     dot(L, P64, P.r.get(), z, n, part);
     finalize(L, sc, part, vec_blocks(n), FIN_CG_INIT, 0, 0, nullptr);
@@ -667,7 +693,6 @@ static void solve_cg(Plan& P, bool use_mg, pdehip_report* rep) {
       cg_update(L, P64, P.x.get(), P.r.get(), P.p.get(), P.q.get(), n, part, nullptr);
       finalize(L, sc, part, vec_blocks(n), FIN_CG_RR, 0, 0, nullptr);
       if (use_mg) {
-        fill(s, P64, z, n, 0.0);
         vcycle(P, L, P.r.get(), z);
         dot(L, P64, P.r.get(), z, n, part);
       }
@@ -727,7 +752,6 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
     finalize(L0, si, part, vec_blocks(n), FIN_R0, inner_tol, PDEHIP_TOL_REL, nullptr);
     maxabs32(s, P.r32.as<float>(), n, part2);
     fin_scale(s, part2, maxabs_blocks(n), si);
-    fill(s, P32, P.z32.get(), n, 0.0);
     vcycle(P, Li, P.r32.get(), P.z32.get());
     dot(Li, P32, P.r32.get(), P.z32.get(), n, part);
     finalize(Li, si, part, vec_blocks(n), FIN_CG_INIT, 0, 0, nullptr);
@@ -741,7 +765,6 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
         cg_update(Li, P32, P.x32.get(), P.r32.get(), P.p32.get(), P.q32.get(), n, part, part2);
         finalize(Li, si, part, vec_blocks(n), FIN_CG_RR, 0, 0, nullptr);
         fin_scale(s, part2, vec_blocks(n), si);
-        fill(s, P32, P.z32.get(), n, 0.0);
         vcycle(P, Li, P.r32.get(), P.z32.get());
         dot(Li, P32, P.r32.get(), P.z32.get(), n, part);
         finalize(Li, si, part, vec_blocks(n), FIN_CG_BETA, 0, 0, nullptr);
@@ -788,7 +811,6 @@ static void solve_mg_iter(Plan& P, pdehip_report* rep) {
     const int chunk = std::min(every, max_iter - launched);
     for (int c = 0; c < chunk; ++c) {
       // correction form: e = V(r) from a zero guess, x += e
-      fill(s, P64, P.q.get(), n, 0.0);
       vcycle(P, L, P.r.get(), P.q.get());
       axpy_dev(L, P64, P.x.get(), P.q.get(), n, nullptr, 1.0);
       residual(L, op, P64, P.bvec.get(), P.x.get(), P.r.get(), part);
@@ -1305,12 +1327,14 @@ double time_kernel(Plan& P, int kernel, int precision, int reps, double* bytes) 
       case 0: apply(L, op, pr, a.get(), b.get(), nullptr); break;
       case 1: rbgs(L, op, pr, c.get(), a.get(), 0); rbgs(L, op, pr, c.get(), a.get(), 1); break;
       case 2: residual(L, op, pr, c.get(), a.get(), b.get(), nullptr); break;
+      case 4: rbgs_sweep(L, op, pr, c.get(), a.get(), b.get(), 0, false); break;
       default: convert(s, pr, a.get(), pr, b.get(), n, 1.0); break;
     }
   };
   switch (kernel) {
     case 0: *bytes = (double)n * (2 * es + kb); break;          // read x (+kappa), write y
-    case 1: *bytes = (double)n * (3 * es + kb); break;          // read u, b (+kappa), write u
+    case 1:
+    case 4: *bytes = (double)n * (3 * es + kb); break;          // read u, b (+kappa), write u
     case 2: *bytes = (double)n * (3 * es + kb); break;          // read b, x (+kappa), write r
     default: *bytes = (double)n * 2 * es; break;                // read + write
   }

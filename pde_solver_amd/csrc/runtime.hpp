@@ -112,6 +112,7 @@ struct Level {
   DevOp op64{}, op32{};
   DBuf kap64, kap32, pw;
   DBuf u, b, r;
+  DBuf alt;               // second smoother buffer (out-of-place fused sweeps)
   Prec prec = P64;
   double sigma = 1.0;     // stored correction = u / sigma (mixed, P16 levels)
   DBuf ainv, map;         // coarsest level dense inverse

@@ -57,8 +57,25 @@ def lib():
         L.pdehip_server_wait.argtypes = [VP, C.c_uint64, R]
         L.pdehip_server_stats_get.argtypes = [VP, C.POINTER(capi.ServerStats)]
         L.pdehip_server_destroy.argtypes = [VP]
+        L.pdehip_tune_set.argtypes = [C.c_char_p, C.c_int]
+        L.pdehip_tune_get.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
         _lib = L
     return _lib
+
+
+def tune_set(key: str, value: int) -> None:
+    """Set a process-wide launch-shape knob (pdehip_tune_set)."""
+    st = lib().pdehip_tune_set(key.encode(), int(value))
+    if st != 0:
+        raise HipError(st, f"invalid tuning {key}={value}")
+
+
+def tune_get(key: str) -> int:
+    v = C.c_int()
+    st = lib().pdehip_tune_get(key.encode(), C.byref(v))
+    if st != 0:
+        raise HipError(st, f"unknown tuning key {key}")
+    return v.value
 
 
 def is_available():

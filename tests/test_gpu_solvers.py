@@ -171,3 +171,22 @@ def test_batch_matches_individual(ctx):
     for s, u in zip(specs, outs):
         _, u1, _ = O.solve(s, mgpcg(tol=1e-12))
         assert rel(u, u1) < 1e-8
+
+
+def test_mg_linear_solution_robin_neumann(ctx):
+    """u = 1 + x solves -u_xx - u_yy = 0 with Robin (u + du/dn = 0) at x=0, du/dx = 1 at x=1,
+    Dirichlet 1 + x on y faces: the iterated V-cycle recovers it to rounding and tracks the
+    oracle's iteration count."""
+    s = capi.Spec(33, 33, a=-1.0, b=-1.0)
+    s.bcs = [BC.robin(1, 1, 0), BC.neumann(1.0), BC.dirichlet(1.0, 1.0), BC.dirichlet(1.0, 1.0),
+             BC(), BC()]
+    s.bc_convention = capi.BCCONV_OUTWARD
+    opt = capi.options(capi.MG, tol=1e-10, max_iter=200, bc_rows=capi.ROWS_FOLDED,
+                       precond=capi.PC_MG)
+    st_o, u_o, rep_o = O.solve(s, opt)
+    st, u, rep = ctx.solve(s, opt)
+    print("hip", st, rep.iterations, rep.final_metric, "oracle", rep_o.iterations, rep_o.final_metric)
+    assert st == 0 and st_o == 0, ctx.error()
+    x = np.linspace(0.0, 1.0, 33)
+    assert np.abs(u - (1.0 + x)[None, :]).max() < 1e-11
+    assert abs(rep.iterations - rep_o.iterations) <= 1
