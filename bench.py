@@ -12,6 +12,7 @@ storage inside FP64 iterative refinement.  A step is one complete solve from a z
 guess with the operator, hierarchy and right-hand side resident in HBM.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 257] [--no-cpu-baseline]
+    python bench.py --only mixed|fp64 [--steps K] [--warmup W]   (one path only, for profiling)
 """
 from __future__ import annotations
 
@@ -97,10 +98,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=257)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
+                    help="run W + K solves of one path and print their times (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
-              "(--gpus must be 1; sharding is described in DESIGN.md)", file=sys.stderr)
+              "(--gpus must be 1)", file=sys.stderr)
         return 2
 
     from pde_solver_amd import capi, hip
@@ -114,6 +117,18 @@ def main() -> int:
     spec = make_problem(n)
     log(f"inputs {n}^3 generated in {time.perf_counter() - t:.1f}s")
     ctx = hip.Context()
+
+    if args.only:
+        plan = ctx.plan(spec, options(capi.MIXED if args.only == "mixed" else capi.FP64, 1e-8))
+        wall, lat, dev, reps = timed_solves(plan, args.steps, args.warmup)
+        print(json.dumps({"path": args.only, "n": n, "steps": args.steps, "warmup": args.warmup,
+                          "solves_per_s": round(args.steps / wall, 3),
+                          "device_ms": [round(v, 3) for v in dev],
+                          "iterations": reps[-1].iterations,
+                          "inner_iterations": reps[-1].inner_iterations}), flush=True)
+        plan.close()
+        ctx.close()
+        return 0
 
     # FP64 oracle solution on the GPU (plain path driven to rel-resid 1e-12)
     t = time.perf_counter()

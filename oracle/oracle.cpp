@@ -12,8 +12,8 @@
 // 3D Krylov) are marked "extension"; they are pinned by dense direct solves, closed forms
 // and manufactured solutions in tests/ rather than by the reference.
 //
-// Parity status of each function is listed in DESIGN.md (pinned by golden vectors from
-// the reference build in oracle/_ref, by closed forms, or both).
+// Each function is pinned by golden vectors from the reference build (oracle/_ref), by closed
+// forms and dense direct solves in tests/, or both.
 
 #include "pdehip.h"
 

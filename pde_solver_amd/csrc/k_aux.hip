@@ -97,43 +97,33 @@ __global__ void k_closure_d(DevOp op, FaceData fd, int axis, double* u, const KS
   }
 }
 
-// Homogeneous extension u_face = theta * u_inner, low then high face.
-template <Prec P>
-__global__ void k_extend(DevOp op, int axis, void* uv, const KScalars* sc) {
-  using S = typename PT<P>::S;
-  if (skip(sc)) return;
-  S* u = static_cast<S*>(uv);
-  const long long nq = face_size(op, axis);
-  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nq;
-       q += (long long)gridDim.x * blockDim.x) {
-    for (int side = 0; side < 2; ++side) {
-      long long node, inner;
-      face_node(op, axis, side, q, &node, &inner);
-      u[node] = store_as<S>(op.theta[2 * axis + side] * ldd(u, inner));
-    }
-  }
-}
-
-// Transpose of k_extend: high face then low face, fold into the inner node and clear.
-template <Prec P>
-__global__ void k_fold_t(DevOp op, int axis, void* rv, const KScalars* sc) {
-  using S = typename PT<P>::S;
-  if (skip(sc)) return;
-  S* r = static_cast<S*>(rv);
-  const long long nq = face_size(op, axis);
-  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nq;
-       q += (long long)gridDim.x * blockDim.x) {
-    for (int side = 1; side >= 0; --side) {
-      long long node, inner;
-      face_node(op, axis, side, q, &node, &inner);
-      const double th = op.theta[2 * axis + side];
-      r[inner] = store_as<S>(ldd(r, inner) + th * ldd(r, node));
-      r[node] = store_as<S>(0.0);
-    }
-  }
-}
-
 // ----------------------------------------------------------------------------- transfers
+// Both transfers run one wavefront per grid row (lanes along x, 64-bit-free index math); the
+// per-row quantities (which coarse / fine rows, their weights) are uniform across the wave.
+
+// Per-axis fold list of coarse coordinate c on an axis of n nodes: the node itself (weight 1)
+// plus the face node it absorbs (weight theta) when it is the first/last interior node.
+__device__ __forceinline__ int fold_list(int c, int n, double th_lo, double th_hi, int* idx, double* w) {
+  idx[0] = c; w[0] = 1.0;
+  int m = 1;
+  if (c == 1) { idx[m] = 0; w[m] = th_lo; ++m; }
+  if (c == n - 2) { idx[m] = n - 1; w[m] = th_hi; ++m; }
+  return m;
+}
+
+// Full weighting along x of fine row `row` around fine column fi (clipped at the row ends).
+template <typename SF>
+__device__ __forceinline__ double fw_x(const SF* row, int fi, int nx) {
+  double a = 0.5 * ldd(row, fi);
+  if (fi > 0) a += 0.25 * ldd(row, fi - 1);
+  if (fi < nx - 1) a += 0.25 * ldd(row, fi + 1);
+  return a;
+}
+
+// Coarse right-hand side = FoldTranspose(FW(r_fine)) * scale in one pass: interior coarse
+// nodes absorb theta times the restricted value of the face nodes next to them (tensor product
+// over the axes, which is what folding x, then y, then z faces composes to); coarse boundary
+// nodes are zero.  Wave = one coarse row (jc, kc).
 template <Prec PF, Prec PC>
 __global__ void __launch_bounds__(kBlock) k_restrict(DevOp f, const void* rfv, DevOp c, void* bcv,
                                                     double scale, const double* dscale,
@@ -141,33 +131,66 @@ __global__ void __launch_bounds__(kBlock) k_restrict(DevOp f, const void* rfv, D
   using SF = typename PT<PF>::S;
   using SC = typename PT<PC>::S;
   if (skip(sc)) return;
+  const int rows = c.ny * c.nz;
+  const int crow = blockIdx.x * (kBlock / kWave) + static_cast<int>(threadIdx.x / kWave);
+  if (crow >= rows) return;
+  const int lane = threadIdx.x & (kWave - 1);
   const SF* rf = static_cast<const SF*>(rfv);
-  SC* bc = static_cast<SC*>(bcv);
-  const double s = dscale ? scale * (*dscale) : scale;
+  SC* bc = static_cast<SC*>(bcv) + (long long)crow * c.nx;
   const bool d3 = c.nz > 1;
-  for (long long cid = blockIdx.x * (long long)blockDim.x + threadIdx.x; cid < c.n;
-       cid += (long long)gridDim.x * blockDim.x) {
-    const int ic = static_cast<int>(cid % c.nx);
-    const int jc = static_cast<int>((cid / c.nx) % c.ny);
-    const int kc = static_cast<int>(cid / c.sxy);
-    const int fi = 2 * ic, fj = 2 * jc, fk = d3 ? 2 * kc : 0;
+  const int jc = crow % c.ny, kc = crow / c.ny;
+  const bool brow = jc == 0 || jc == c.ny - 1 || (d3 && (kc == 0 || kc == c.nz - 1));
+  if (brow) {
+    for (int ic = lane; ic < c.nx; ic += kWave) bc[ic] = store_as<SC>(0.0);
+    return;
+  }
+  const double s = dscale ? scale * (*dscale) : scale;
+  int yi[3], zi[3];
+  double yw[3], zw[3];
+  const int my = fold_list(jc, c.ny, c.theta[2], c.theta[3], yi, yw);
+  int mz = 1;
+  zi[0] = 0; zw[0] = 1.0;
+  if (d3) mz = fold_list(kc, c.nz, c.theta[4], c.theta[5], zi, zw);
+  const double w1[3] = {0.25, 0.5, 0.25};
+  const long long fsxy = (long long)f.nx * f.ny;
+  for (int ic = lane; ic < c.nx; ic += kWave) {
+    if (ic == 0 || ic == c.nx - 1) { bc[ic] = store_as<SC>(0.0); continue; }
+    int xi[3];
+    double xw[3];
+    const int mx = fold_list(ic, c.nx, c.theta[0], c.theta[1], xi, xw);
     double acc = 0.0;
-    const double w1[3] = {0.25, 0.5, 0.25};
-    const int dk0 = (d3 && fk > 0) ? -1 : 0, dk1 = (d3 && fk < f.nz - 1) ? 1 : 0;
-    const int dj0 = fj > 0 ? -1 : 0, dj1 = fj < f.ny - 1 ? 1 : 0;
-    const int di0 = fi > 0 ? -1 : 0, di1 = fi < f.nx - 1 ? 1 : 0;
-    for (int dk = dk0; dk <= dk1; ++dk) {
-      const double wk = d3 ? w1[dk + 1] : 1.0;
-      for (int dj = dj0; dj <= dj1; ++dj) {
-        const double wjk = wk * w1[dj + 1];
-        const long long row = ((long long)(fk + dk) * f.ny + (fj + dj)) * f.nx + fi;
-        for (int di = di0; di <= di1; ++di) acc += wjk * w1[di + 1] * ldd(rf, row + di);
+    for (int a = 0; a < mz; ++a) {
+      const int fk = d3 ? 2 * zi[a] : 0;
+      for (int bb = 0; bb < my; ++bb) {
+        const double wyz = zw[a] * yw[bb];
+        if (wyz == 0.0) continue;
+        const int fj = 2 * yi[bb];
+        // FW of fine (., fj, fk): rows fj-1..fj+1 x planes fk-1..fk+1, clipped
+        double sub[3] = {0.0, 0.0, 0.0};
+        for (int dk = -1; dk <= 1; ++dk) {
+          if (!d3 && dk != 0) continue;
+          const int k = fk + dk;
+          if (k < 0 || k >= f.nz) continue;
+          const double wk = d3 ? w1[dk + 1] : 1.0;
+          for (int dj = -1; dj <= 1; ++dj) {
+            const int j = fj + dj;
+            if (j < 0 || j >= f.ny) continue;
+            const SF* row = rf + k * fsxy + (long long)j * f.nx;
+            const double wjk = wk * w1[dj + 1];
+            for (int q = 0; q < mx; ++q) sub[q] += wjk * fw_x(row, 2 * xi[q], f.nx);
+          }
+        }
+        for (int q = 0; q < mx; ++q) acc += wyz * xw[q] * sub[q];
       }
     }
-    bc[cid] = store_as<SC>(acc * s);
+    bc[ic] = store_as<SC>(acc * s);
   }
 }
 
+// fine interior += factor * trilinear interpolation of the coarse correction, whose face nodes
+// are taken as the homogeneous closure extension: theta times the inward neighbour per axis
+// (edges and corners the product), which is what extending x, then y, then z faces composes
+// to.  Wave = one fine row (j, k).
 template <Prec PC, Prec PF>
 __global__ void __launch_bounds__(kBlock) k_prolong(DevOp c, const void* ucv, DevOp f, void* ufv,
                                                    double factor, const double* dscale,
@@ -175,26 +198,50 @@ __global__ void __launch_bounds__(kBlock) k_prolong(DevOp c, const void* ucv, De
   using SC = typename PT<PC>::S;
   using SF = typename PT<PF>::S;
   if (skip(sc)) return;
-  const SC* uc = static_cast<const SC*>(ucv);
-  SF* uf = static_cast<SF*>(ufv);
-  const double s = dscale ? factor * (*dscale) : factor;
+  const int rows = f.ny * f.nz;
+  const int frow = blockIdx.x * (kBlock / kWave) + static_cast<int>(threadIdx.x / kWave);
+  if (frow >= rows) return;
   const bool d3 = f.nz > 1;
-  for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < f.n;
-       id += (long long)gridDim.x * blockDim.x) {
-    const int i = static_cast<int>(id % f.nx);
-    const int j = static_cast<int>((id / f.nx) % f.ny);
-    const int k = static_cast<int>(id / f.sxy);
-    if (is_boundary2(f, i, j) || (d3 && (k == 0 || k == f.nz - 1))) continue;
-    const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
-    const int oi = i & 1, oj = j & 1, ok = d3 ? (k & 1) : 0;
-    double acc = 0.0;
-    for (int a = 0; a <= ok; ++a)
-      for (int bb = 0; bb <= oj; ++bb) {
-        const long long row = ((long long)(kc + a) * c.ny + (jc + bb)) * c.nx + ic;
-        for (int cc = 0; cc <= oi; ++cc) acc += ldd(uc, row + cc);
+  const int j = frow % f.ny, k = frow / f.ny;
+  if (j == 0 || j == f.ny - 1 || (d3 && (k == 0 || k == f.nz - 1))) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const SC* uc = static_cast<const SC*>(ucv);
+  SF* uf = static_cast<SF*>(ufv) + (long long)frow * f.nx;
+  const double s = dscale ? factor * (*dscale) : factor;
+  // coarse rows touched by this fine row, mapped through the closure extension
+  const int jc = j >> 1, kc = k >> 1;
+  const int oj = j & 1, ok = d3 ? (k & 1) : 0;
+  const SC* crow[4];
+  double cw[4];
+  int nr = 0;
+  for (int a = 0; a <= ok; ++a)
+    for (int bb = 0; bb <= oj; ++bb) {
+      int y = jc + bb, z = kc + a;
+      double w = 1.0;
+      if (y == 0) { w *= c.theta[2]; y = 1; } else if (y == c.ny - 1) { w *= c.theta[3]; y = c.ny - 2; }
+      if (d3) {
+        if (z == 0) { w *= c.theta[4]; z = 1; } else if (z == c.nz - 1) { w *= c.theta[5]; z = c.nz - 2; }
       }
-    const int cnt = (1 + oi) * (1 + oj) * (1 + ok);
-    uf[id] = store_as<SF>(ldd(uf, id) + s * acc / cnt);
+      crow[nr] = uc + ((long long)z * c.ny + y) * c.nx;
+      cw[nr] = w;
+      ++nr;
+    }
+  const double inv_rows = 1.0 / nr;
+  for (int i = 1 + lane; i < f.nx - 1; i += kWave) {
+    const int ic = i >> 1, oi = i & 1;
+    // columns ic and (oi) ic+1, mapped through the closure extension
+    int x0 = ic, x1 = ic + 1;
+    double w0 = 1.0, w1 = 1.0;
+    if (x0 == 0) { w0 = c.theta[0]; x0 = 1; }
+    if (x1 == c.nx - 1) { w1 = c.theta[1]; x1 = c.nx - 2; }
+    double acc = 0.0;
+    for (int r = 0; r < nr; ++r) {
+      if (cw[r] == 0.0) continue;
+      double v = w0 == 0.0 ? 0.0 : w0 * ldd(crow[r], x0);
+      if (oi) v = 0.5 * (v + (w1 == 0.0 ? 0.0 : w1 * ldd(crow[r], x1)));
+      acc += cw[r] * v;
+    }
+    uf[i] = store_as<SF>(ldd(uf, i) + s * (acc * inv_rows));
   }
 }
 
@@ -417,25 +464,10 @@ void closures_dirichlet(const Launch& L, const DevOp& op, const FaceData& fd, do
     default: hipLaunchKernelGGL(KER<P16>, __VA_ARGS__); break;                        \
   }
 
-void extend_closure(const Launch& L, const DevOp& op, Prec p, void* u) {
-  const int axes = op.nz > 1 ? 3 : 2;
-  for (int a = 0; a < axes; ++a) {
-    const dim3 g(grid1(face_size(op, a), 1024));
-    PDEHIP_BY_PREC(p, k_extend, g, dim3(kBlock), 0, L.stream, op, a, u, L.sc);
-  }
-}
-
-void fold_transpose(const Launch& L, const DevOp& op, Prec p, void* r) {
-  const int axes = op.nz > 1 ? 3 : 2;
-  for (int a = axes - 1; a >= 0; --a) {
-    const dim3 g(grid1(face_size(op, a), 1024));
-    PDEHIP_BY_PREC(p, k_fold_t, g, dim3(kBlock), 0, L.stream, op, a, r, L.sc);
-  }
-}
-
 void restrict_fw(const Launch& L, const DevOp& fine, Prec pf, const void* rf, const DevOp& coarse,
                  Prec pc, void* bc, double scale, const double* dev_scale) {
-  const dim3 g(grid1(coarse.n));
+  const int waves = kBlock / kWave;
+  const dim3 g(static_cast<unsigned>((coarse.ny * coarse.nz + waves - 1) / waves));
 #define R_(A, B) hipLaunchKernelGGL((k_restrict<A, B>), g, dim3(kBlock), 0, L.stream, fine, rf, coarse, bc, scale, dev_scale, L.sc)
   if (pf == P64 && pc == P64) R_(P64, P64);
   else if (pf == P32 && pc == P32) R_(P32, P32);
@@ -448,7 +480,8 @@ void restrict_fw(const Launch& L, const DevOp& fine, Prec pf, const void* rf, co
 
 void prolong_add(const Launch& L, const DevOp& coarse, Prec pc, const void* uc, const DevOp& fine,
                  Prec pf, void* uf, double factor, const double* dev_scale) {
-  const dim3 g(grid1(fine.n));
+  const int waves = kBlock / kWave;
+  const dim3 g(static_cast<unsigned>((fine.ny * fine.nz + waves - 1) / waves));
 #define P_(A, B) hipLaunchKernelGGL((k_prolong<A, B>), g, dim3(kBlock), 0, L.stream, coarse, uc, fine, uf, factor, dev_scale, L.sc)
   if (pc == P64 && pf == P64) P_(P64, P64);
   else if (pc == P32 && pf == P32) P_(P32, P32);

@@ -72,15 +72,14 @@ struct FaceData {
 // Reference closure order: axis by axis (x faces, y faces, z faces).
 void closures_nr(const Launch& L, const DevOp& op, const FaceData& fd, double* u);
 void closures_dirichlet(const Launch& L, const DevOp& op, const FaceData& fd, double* u);
-// Homogeneous closure extension (corrections) and its transpose, on a level's vector.
-void extend_closure(const Launch& L, const DevOp& op, Prec p, void* u);
-void fold_transpose(const Launch& L, const DevOp& op, Prec p, void* r);
-
 // ---- multigrid transfers
-// coarse = FW(fine) * scale on every coarse node that has fine neighbours (boundary too).
+// Coarse right-hand side: full weighting of the fine residual on every coarse node, then the
+// transpose of the homogeneous closure extension (coarse.theta) folded in, times scale
+// (times *dev_scale when given).  Coarse boundary nodes come out zero.  One pass.
 void restrict_fw(const Launch& L, const DevOp& fine, Prec pf, const void* rf, const DevOp& coarse,
                  Prec pc, void* bc, double scale, const double* dev_scale);
-// fine interior += factor * interpolate(coarse).
+// fine interior += factor * trilinear interpolation of the coarse correction, whose boundary
+// values are taken as the homogeneous closure extension (theta times the inward neighbour).
 void prolong_add(const Launch& L, const DevOp& coarse, Prec pc, const void* uc, const DevOp& fine,
                  Prec pf, void* uf, double factor, const double* dev_scale);
 // Coarsest level: x[map] = Ainv * b[map] in FP64.

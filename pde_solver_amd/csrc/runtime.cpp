@@ -622,7 +622,6 @@ static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0) {
     residual(L, fo, f.prec, bl, ul, f.r.get(), nullptr);
     restrict_fw(L, fo, f.prec, f.r.get(), co, c.prec, c.b.get(), 1.0,
                 (P.mixed && l == 0) ? &sc->scale : nullptr);
-    fold_transpose(L, co, c.prec, c.b.get());
   }
   Level& cl = P.L[nl - 1];
   const DevOp& cop = P.mixed ? cl.op32 : cl.op64;
@@ -643,7 +642,6 @@ static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0) {
     const DevOp& co = P.mixed ? c.op32 : c.op64;
     const void* bl = l == 0 ? b0 : f.b.get();
     void* ul = l == 0 ? u0 : f.u.get();
-    extend_closure(L, co, c.prec, c.u.get());
     prolong_add(L, co, c.prec, c.u.get(), fo, f.prec, ul, c.sigma / f.sigma,
                 (P.mixed && l == 0) ? &sc->inv_scale : nullptr);
     smooth(L, f, fo, bl, ul, post, 1, false);
@@ -866,7 +864,6 @@ static void solve_mg_reference(Plan& P, pdehip_report* rep) {
     DevOp co = make_devop(G[l + 1], PDEHIP_OP_CONST);
     residual(L0, fo, P64, bptr(l), uptr(l), R[l].get(), nullptr);
     restrict_fw(L0, fo, P64, R[l].get(), co, P64, B[l + 1].get(), 1.0, nullptr);
-    fold_transpose(L0, co, P64, B[l + 1].get());   // theta = 0: clears the coarse boundary
     fill(s, P64, U[l + 1].get(), G[l + 1].n, 0.0);
   }
   {

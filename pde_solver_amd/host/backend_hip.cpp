@@ -2,8 +2,8 @@
 //
 // Restates the reference's selection logic (src/backend_detect.cpp:25-90,
 // src/backend_solve.cpp:53-211, src/solver_tokens.cpp:16-35) and capability matrix
-// (src/backend_capability_matrix.cpp:68-336) with a HIP entry; the hunks a maintainer applies
-// to the reference files are listed in INTEGRATION.md.  In a standalone build the CPU backend
+// (src/backend_capability_matrix.cpp:68-336) with a HIP entry, so that the same functions,
+// patched with a HIP case, slot into the reference tree.  In a standalone build the CPU backend
 // (SolvePDE) is a weak reference: when it is not linked, CPU is reported unavailable and no
 // request is routed to it.
 #include <algorithm>
