@@ -98,8 +98,18 @@ __global__ void k_closure_d(DevOp op, FaceData fd, int axis, double* u, const KS
 }
 
 // ----------------------------------------------------------------------------- transfers
-// Both transfers run one wavefront per grid row (lanes along x, 64-bit-free index math); the
-// per-row quantities (which coarse / fine rows, their weights) are uniform across the wave.
+// Both transfers are column marches: a 64 x 4 workgroup owns 64 x 4 (i, j) columns and walks a
+// chunk of z planes, carrying the z interpolation (prolongation) or z weighting (restriction)
+// in registers, so each in-plane sum is formed once.  Index math is 32-bit.
+constexpr int kTX = 64, kTY = 4;
+
+// z planes per workgroup: 16, shortened until the launch has ~4096 workgroups (latency hiding
+// on the small levels).
+int zchunk_for(int gx, int gy, int nz) {
+  int zc = 16;
+  while (zc > 1 && (long long)gx * gy * ((nz + zc - 1) / zc) < 4096) zc /= 2;
+  return zc;
+}
 
 // Per-axis fold list of coarse coordinate c on an axis of n nodes: the node itself (weight 1)
 // plus the face node it absorbs (weight theta) when it is the first/last interior node.
@@ -111,137 +121,157 @@ __device__ __forceinline__ int fold_list(int c, int n, double th_lo, double th_h
   return m;
 }
 
-// Full weighting along x of fine row `row` around fine column fi (clipped at the row ends).
+// 3 x 3 in-plane full weighting of fine plane `pl` at (fi, fj), clipped at the plane edges.
 template <typename SF>
-__device__ __forceinline__ double fw_x(const SF* row, int fi, int nx) {
-  double a = 0.5 * ldd(row, fi);
-  if (fi > 0) a += 0.25 * ldd(row, fi - 1);
-  if (fi < nx - 1) a += 0.25 * ldd(row, fi + 1);
-  return a;
+__device__ __forceinline__ double fw2(const DevOp& f, const SF* pl, int fi, int fj) {
+  const double w1[3] = {0.25, 0.5, 0.25};
+  double acc = 0.0;
+  for (int dj = -1; dj <= 1; ++dj) {
+    const int j = fj + dj;
+    if (j < 0 || j >= f.ny) continue;
+    const SF* row = pl + j * f.nx;
+    double a = 0.5 * ldd(row, fi);
+    if (fi > 0) a += 0.25 * ldd(row, fi - 1);
+    if (fi < f.nx - 1) a += 0.25 * ldd(row, fi + 1);
+    acc += w1[dj + 1] * a;
+  }
+  return acc;
 }
 
 // Coarse right-hand side = FoldTranspose(FW(r_fine)) * scale in one pass: interior coarse
 // nodes absorb theta times the restricted value of the face nodes next to them (tensor product
 // over the axes, which is what folding x, then y, then z faces composes to); coarse boundary
-// nodes are zero.  Wave = one coarse row (jc, kc).
+// nodes are zero.
 template <Prec PF, Prec PC>
-__global__ void __launch_bounds__(kBlock) k_restrict(DevOp f, const void* rfv, DevOp c, void* bcv,
-                                                    double scale, const double* dscale,
-                                                    const KScalars* sc) {
+__global__ void __launch_bounds__(kTX * kTY) k_restrict(DevOp f, const void* rfv, DevOp c,
+                                                       void* bcv, double scale,
+                                                       const double* dscale, int kZC,
+                                                       const KScalars* sc) {
   using SF = typename PT<PF>::S;
   using SC = typename PT<PC>::S;
   if (skip(sc)) return;
-  const int rows = c.ny * c.nz;
-  const int crow = blockIdx.x * (kBlock / kWave) + static_cast<int>(threadIdx.x / kWave);
-  if (crow >= rows) return;
-  const int lane = threadIdx.x & (kWave - 1);
-  const SF* rf = static_cast<const SF*>(rfv);
-  SC* bc = static_cast<SC*>(bcv) + (long long)crow * c.nx;
+  const int ic = blockIdx.x * kTX + threadIdx.x;
+  const int jc = blockIdx.y * kTY + threadIdx.y;
+  if (ic >= c.nx || jc >= c.ny) return;
   const bool d3 = c.nz > 1;
-  const int jc = crow % c.ny, kc = crow / c.ny;
-  const bool brow = jc == 0 || jc == c.ny - 1 || (d3 && (kc == 0 || kc == c.nz - 1));
-  if (brow) {
-    for (int ic = lane; ic < c.nx; ic += kWave) bc[ic] = store_as<SC>(0.0);
+  const int kc0 = d3 ? blockIdx.z * kZC : 0;
+  const int kc1 = d3 ? min(kc0 + kZC, c.nz) : 1;
+  const SF* rf = static_cast<const SF*>(rfv);
+  SC* bc = static_cast<SC*>(bcv);
+  const long long csxy = (long long)c.nx * c.ny;
+  const long long fsxy = (long long)f.nx * f.ny;
+  const long long ccol = (long long)jc * c.nx + ic;
+  if (ic == 0 || jc == 0 || ic == c.nx - 1 || jc == c.ny - 1) {
+    for (int kc = kc0; kc < kc1; ++kc) bc[kc * csxy + ccol] = store_as<SC>(0.0);
     return;
   }
   const double s = dscale ? scale * (*dscale) : scale;
-  int yi[3], zi[3];
-  double yw[3], zw[3];
+  int xi[3], yi[3];
+  double xw[3], yw[3];
+  const int mx = fold_list(ic, c.nx, c.theta[0], c.theta[1], xi, xw);
   const int my = fold_list(jc, c.ny, c.theta[2], c.theta[3], yi, yw);
-  int mz = 1;
-  zi[0] = 0; zw[0] = 1.0;
-  if (d3) mz = fold_list(kc, c.nz, c.theta[4], c.theta[5], zi, zw);
-  const double w1[3] = {0.25, 0.5, 0.25};
-  const long long fsxy = (long long)f.nx * f.ny;
-  for (int ic = lane; ic < c.nx; ic += kWave) {
-    if (ic == 0 || ic == c.nx - 1) { bc[ic] = store_as<SC>(0.0); continue; }
-    int xi[3];
-    double xw[3];
-    const int mx = fold_list(ic, c.nx, c.theta[0], c.theta[1], xi, xw);
+  // in-plane folded full weighting of fine plane fk at this coarse column
+  auto Q = [&](int fk) -> double {
+    if (fk < 0 || fk >= f.nz) return 0.0;
+    const SF* pl = rf + fk * fsxy;
     double acc = 0.0;
-    for (int a = 0; a < mz; ++a) {
-      const int fk = d3 ? 2 * zi[a] : 0;
-      for (int bb = 0; bb < my; ++bb) {
-        const double wyz = zw[a] * yw[bb];
-        if (wyz == 0.0) continue;
-        const int fj = 2 * yi[bb];
-        // FW of fine (., fj, fk): rows fj-1..fj+1 x planes fk-1..fk+1, clipped
-        double sub[3] = {0.0, 0.0, 0.0};
-        for (int dk = -1; dk <= 1; ++dk) {
-          if (!d3 && dk != 0) continue;
-          const int k = fk + dk;
-          if (k < 0 || k >= f.nz) continue;
-          const double wk = d3 ? w1[dk + 1] : 1.0;
-          for (int dj = -1; dj <= 1; ++dj) {
-            const int j = fj + dj;
-            if (j < 0 || j >= f.ny) continue;
-            const SF* row = rf + k * fsxy + (long long)j * f.nx;
-            const double wjk = wk * w1[dj + 1];
-            for (int q = 0; q < mx; ++q) sub[q] += wjk * fw_x(row, 2 * xi[q], f.nx);
-          }
-        }
-        for (int q = 0; q < mx; ++q) acc += wyz * xw[q] * sub[q];
+    for (int b = 0; b < my; ++b)
+      for (int q = 0; q < mx; ++q) {
+        const double w = yw[b] * xw[q];
+        if (w != 0.0) acc += w * fw2(f, pl, 2 * xi[q], 2 * yi[b]);
       }
+    return acc;
+  };
+  if (!d3) {
+    bc[ccol] = store_as<SC>(Q(0) * s);
+    return;
+  }
+  double q_lo = (kc0 >= 1) ? Q(2 * kc0 - 1) : 0.0;   // Q(2kc - 1) of the current kc
+  for (int kc = kc0; kc < kc1; ++kc) {
+    if (kc == 0 || kc == c.nz - 1) {
+      bc[kc * csxy + ccol] = store_as<SC>(0.0);
+      q_lo = (kc == 0) ? Q(1) : q_lo;
+      continue;
     }
-    bc[ic] = store_as<SC>(acc * s);
+    const double q_mid = Q(2 * kc), q_hi = Q(2 * kc + 1);
+    double v = 0.25 * q_lo + 0.5 * q_mid + 0.25 * q_hi;
+    if (kc == 1 && c.theta[4] != 0.0) v += c.theta[4] * (0.5 * Q(0) + 0.25 * q_lo);
+    if (kc == c.nz - 2 && c.theta[5] != 0.0) v += c.theta[5] * (0.25 * q_hi + 0.5 * Q(2 * kc + 2));
+    bc[kc * csxy + ccol] = store_as<SC>(v * s);
+    q_lo = q_hi;
   }
 }
 
 // fine interior += factor * trilinear interpolation of the coarse correction, whose face nodes
 // are taken as the homogeneous closure extension: theta times the inward neighbour per axis
 // (edges and corners the product), which is what extending x, then y, then z faces composes
-// to.  Wave = one fine row (j, k).
+// to.
 template <Prec PC, Prec PF>
-__global__ void __launch_bounds__(kBlock) k_prolong(DevOp c, const void* ucv, DevOp f, void* ufv,
-                                                   double factor, const double* dscale,
-                                                   const KScalars* sc) {
+__global__ void __launch_bounds__(kTX * kTY) k_prolong(DevOp c, const void* ucv, DevOp f,
+                                                      void* ufv, double factor,
+                                                      const double* dscale, int kZC,
+                                                      const KScalars* sc) {
   using SC = typename PT<PC>::S;
   using SF = typename PT<PF>::S;
   if (skip(sc)) return;
-  const int rows = f.ny * f.nz;
-  const int frow = blockIdx.x * (kBlock / kWave) + static_cast<int>(threadIdx.x / kWave);
-  if (frow >= rows) return;
+  const int i = blockIdx.x * kTX + threadIdx.x;
+  const int j = blockIdx.y * kTY + threadIdx.y;
+  if (i < 1 || j < 1 || i > f.nx - 2 || j > f.ny - 2) return;
   const bool d3 = f.nz > 1;
-  const int j = frow % f.ny, k = frow / f.ny;
-  if (j == 0 || j == f.ny - 1 || (d3 && (k == 0 || k == f.nz - 1))) return;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int k0 = d3 ? max(1, (int)blockIdx.z * kZC) : 0;
+  const int k1 = d3 ? min((int)blockIdx.z * kZC + kZC, f.nz - 1) : 1;
+  if (k0 >= k1) return;
   const SC* uc = static_cast<const SC*>(ucv);
-  SF* uf = static_cast<SF*>(ufv) + (long long)frow * f.nx;
+  SF* uf = static_cast<SF*>(ufv);
   const double s = dscale ? factor * (*dscale) : factor;
-  // coarse rows touched by this fine row, mapped through the closure extension
-  const int jc = j >> 1, kc = k >> 1;
-  const int oj = j & 1, ok = d3 ? (k & 1) : 0;
-  const SC* crow[4];
-  double cw[4];
-  int nr = 0;
-  for (int a = 0; a <= ok; ++a)
-    for (int bb = 0; bb <= oj; ++bb) {
-      int y = jc + bb, z = kc + a;
-      double w = 1.0;
-      if (y == 0) { w *= c.theta[2]; y = 1; } else if (y == c.ny - 1) { w *= c.theta[3]; y = c.ny - 2; }
-      if (d3) {
-        if (z == 0) { w *= c.theta[4]; z = 1; } else if (z == c.nz - 1) { w *= c.theta[5]; z = c.nz - 2; }
-      }
-      crow[nr] = uc + ((long long)z * c.ny + y) * c.nx;
-      cw[nr] = w;
-      ++nr;
+  const long long csxy = (long long)c.nx * c.ny;
+  const long long fsxy = (long long)f.nx * f.ny;
+  // in-plane bilinear stencil (columns x0/x1, rows y0/y1) through the closure extension
+  const int ic = i >> 1, oi = i & 1, jc = j >> 1, oj = j & 1;
+  int x0 = ic, x1 = ic + 1, y0 = jc, y1 = jc + 1;
+  double wx0 = 1.0, wx1 = 1.0, wy0 = 1.0, wy1 = 1.0;
+  if (x0 == 0) { wx0 = c.theta[0]; x0 = 1; }
+  if (x1 == c.nx - 1) { wx1 = c.theta[1]; x1 = c.nx - 2; }
+  if (y0 == 0) { wy0 = c.theta[2]; y0 = 1; }
+  if (y1 == c.ny - 1) { wy1 = c.theta[3]; y1 = c.ny - 2; }
+  const double hx = oi ? 0.5 : 1.0, hy = oj ? 0.5 : 1.0;
+  if (!oi) wx1 = 0.0;
+  if (!oj) wy1 = 0.0;
+  // bilinear value of coarse plane kc (mapped through the z extension)
+  auto P = [&](int kc) -> double {
+    double wz = 1.0;
+    if (d3) {
+      if (kc == 0) { wz = c.theta[4]; kc = 1; } else if (kc == c.nz - 1) { wz = c.theta[5]; kc = c.nz - 2; }
     }
-  const double inv_rows = 1.0 / nr;
-  for (int i = 1 + lane; i < f.nx - 1; i += kWave) {
-    const int ic = i >> 1, oi = i & 1;
-    // columns ic and (oi) ic+1, mapped through the closure extension
-    int x0 = ic, x1 = ic + 1;
-    double w0 = 1.0, w1 = 1.0;
-    if (x0 == 0) { w0 = c.theta[0]; x0 = 1; }
-    if (x1 == c.nx - 1) { w1 = c.theta[1]; x1 = c.nx - 2; }
-    double acc = 0.0;
-    for (int r = 0; r < nr; ++r) {
-      if (cw[r] == 0.0) continue;
-      double v = w0 == 0.0 ? 0.0 : w0 * ldd(crow[r], x0);
-      if (oi) v = 0.5 * (v + (w1 == 0.0 ? 0.0 : w1 * ldd(crow[r], x1)));
-      acc += cw[r] * v;
+    if (wz == 0.0) return 0.0;
+    const SC* pl = uc + kc * csxy;
+    double a = 0.0;
+    if (wy0 != 0.0) {
+      const SC* r = pl + y0 * c.nx;
+      double v = wx0 != 0.0 ? wx0 * ldd(r, x0) : 0.0;
+      if (wx1 != 0.0) v += wx1 * ldd(r, x1);
+      a += wy0 * (v * hx);
     }
-    uf[i] = store_as<SF>(ldd(uf, i) + s * (acc * inv_rows));
+    if (wy1 != 0.0) {
+      const SC* r = pl + y1 * c.nx;
+      double v = wx0 != 0.0 ? wx0 * ldd(r, x0) : 0.0;
+      if (wx1 != 0.0) v += wx1 * ldd(r, x1);
+      a += wy1 * (v * hx);
+    }
+    return wz * (a * hy);
+  };
+  int kc_have = -1;
+  double p_lo = 0.0, p_hi = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const int kc = k >> 1, ok = d3 ? (k & 1) : 0;
+    if (kc != kc_have) {
+      p_lo = (kc == kc_have + 1 && kc_have >= 0) ? p_hi : P(kc);
+      p_hi = d3 ? P(kc + 1) : 0.0;
+      kc_have = kc;
+    }
+    const double v = ok ? 0.5 * (p_lo + p_hi) : p_lo;
+    const long long id = k * fsxy + (long long)j * f.nx + i;
+    uf[id] = store_as<SF>(ldd(uf, id) + s * v);
   }
 }
 
@@ -466,9 +496,11 @@ void closures_dirichlet(const Launch& L, const DevOp& op, const FaceData& fd, do
 
 void restrict_fw(const Launch& L, const DevOp& fine, Prec pf, const void* rf, const DevOp& coarse,
                  Prec pc, void* bc, double scale, const double* dev_scale) {
-  const int waves = kBlock / kWave;
-  const dim3 g(static_cast<unsigned>((coarse.ny * coarse.nz + waves - 1) / waves));
-#define R_(A, B) hipLaunchKernelGGL((k_restrict<A, B>), g, dim3(kBlock), 0, L.stream, fine, rf, coarse, bc, scale, dev_scale, L.sc)
+  const int gx = (coarse.nx + kTX - 1) / kTX, gy = (coarse.ny + kTY - 1) / kTY;
+  const int zc = coarse.nz > 1 ? zchunk_for(gx, gy, coarse.nz) : 1;
+  const dim3 g(gx, gy, coarse.nz > 1 ? (coarse.nz + zc - 1) / zc : 1);
+  const dim3 blk(kTX, kTY);
+#define R_(A, B) hipLaunchKernelGGL((k_restrict<A, B>), g, blk, 0, L.stream, fine, rf, coarse, bc, scale, dev_scale, zc, L.sc)
   if (pf == P64 && pc == P64) R_(P64, P64);
   else if (pf == P32 && pc == P32) R_(P32, P32);
   else if (pf == P32 && pc == P16) R_(P32, P16);
@@ -480,9 +512,11 @@ void restrict_fw(const Launch& L, const DevOp& fine, Prec pf, const void* rf, co
 
 void prolong_add(const Launch& L, const DevOp& coarse, Prec pc, const void* uc, const DevOp& fine,
                  Prec pf, void* uf, double factor, const double* dev_scale) {
-  const int waves = kBlock / kWave;
-  const dim3 g(static_cast<unsigned>((fine.ny * fine.nz + waves - 1) / waves));
-#define P_(A, B) hipLaunchKernelGGL((k_prolong<A, B>), g, dim3(kBlock), 0, L.stream, coarse, uc, fine, uf, factor, dev_scale, L.sc)
+  const int gx = (fine.nx + kTX - 1) / kTX, gy = (fine.ny + kTY - 1) / kTY;
+  const int zc = fine.nz > 1 ? zchunk_for(gx, gy, fine.nz) : 1;
+  const dim3 g(gx, gy, fine.nz > 1 ? (fine.nz + zc - 1) / zc : 1);
+  const dim3 blk(kTX, kTY);
+#define P_(A, B) hipLaunchKernelGGL((k_prolong<A, B>), g, blk, 0, L.stream, coarse, uc, fine, uf, factor, dev_scale, zc, L.sc)
   if (pc == P64 && pf == P64) P_(P64, P64);
   else if (pc == P32 && pf == P32) P_(P32, P32);
   else if (pc == P16 && pf == P32) P_(P16, P32);

@@ -1,0 +1,278 @@
+// k_plane.hip — 3D operator apply and residual as a plane march for gfx950.
+//
+// A workgroup (8 waves) owns a 62 x (8R-2) (i,j) tile (extended by a 1-node ring to 64 lanes
+// x 8R rows) and walks a chunk of z planes.  Per plane p:
+//   * x(p) and kappa(p+1) go to LDS (two slots each);
+//   * FLUX: every node of the extended tile forms its east and north faces of plane p+1 once
+//     (two LDS slots) and its front face in registers;
+//   * the tile's plane p is produced from x neighbours in LDS / registers and the faces of
+//     plane p formed one step earlier: y = A x (partials of x.y) or r = b - A x (partials of
+//     r.r).  Boundary rows are identity.
+// Double-buffered slots make one barrier per plane enough: a slot is rewritten two planes
+// after it was read, and a thread can only be there once every thread has passed the barrier
+// in between.  The FP64 operator divides exactly (face_coef); the FP32/FP16 operators use the
+// hardware reciprocal (face_smooth), which is within their own rounding.
+#include "kernels.hpp"
+
+namespace pdehip {
+namespace {
+
+constexpr int kLanes = 64;
+constexpr int kWaves = 8;
+constexpr int kThreads = kLanes * kWaves;
+
+template <Prec P> struct PT;
+template <> struct PT<P64> { using T = double; using S = double; };
+template <> struct PT<P32> { using T = float; using S = float; };
+template <> struct PT<P16> { using T = float; using S = __half; };
+
+template <typename S> __device__ __forceinline__ auto ldS(const S* p, long long i);
+template <> __device__ __forceinline__ auto ldS<double>(const double* p, long long i) { return p[i]; }
+template <> __device__ __forceinline__ auto ldS<float>(const float* p, long long i) { return p[i]; }
+template <> __device__ __forceinline__ auto ldS<__half>(const __half* p, long long i) { return __half2float(p[i]); }
+
+__device__ __forceinline__ bool skip(const KScalars* sc) { return sc && (sc->done | sc->breakdown); }
+
+template <Prec P, typename T>
+__device__ __forceinline__ T face_op(T kc, T kn, T f) {
+  if constexpr (P == P64) return face_coef<T>(kc, kn, f);
+  else return face_smooth<T>(kc, kn, f);
+}
+
+// MODE 0: y = A x, partials of x.y.  MODE 1: r = b - A x, partials of r.r.
+template <int KIND, Prec P, int MODE, int R>
+__global__ void __launch_bounds__(kThreads)
+k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, void* __restrict__ yv,
+         double* partials, int tiles_x, int tiles_y, int ntiles, int kchunk, const KScalars* sc) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  constexpr int TYE = R * kWaves;
+  constexpr int TY = TYE - 2;
+  constexpr int TX = kLanes - 2;
+  constexpr bool HK = KIND == PDEHIP_OP_FLUX;
+  constexpr int KR = HK ? TYE : 1;
+  __shared__ T lx[2][TYE][kLanes];
+  __shared__ T lk[HK ? 2 : 1][KR][kLanes];
+  __shared__ T lfe[HK ? 2 : 1][KR][kLanes];
+  __shared__ T lfn[HK ? 2 : 1][KR][kLanes];
+  __shared__ double lred[kWaves];
+  if (skip(sc)) return;
+  const int per = static_cast<int>(gridDim.x >> 3);
+  const int t = static_cast<int>(blockIdx.x & 7) * per + static_cast<int>(blockIdx.x >> 3);
+  if (t >= ntiles) {
+    if (partials && threadIdx.x == 0) partials[blockIdx.x] = 0.0;
+    return;
+  }
+  const S* __restrict__ x = static_cast<const S*>(xv);
+  const S* __restrict__ b = static_cast<const S*>(bv);
+  const T* __restrict__ kap = static_cast<const T*>(op.kappa);
+  S* __restrict__ y = static_cast<S*>(yv);
+  const int tx = t % tiles_x;
+  const int rest = t / tiles_x;
+  const int ty = rest % tiles_y;
+  const int cz = rest / tiles_y;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wv = threadIdx.x / kLanes;
+  const int i = tx * TX - 1 + lane;
+  const int k0 = cz * kchunk;
+  const int k1 = min(k0 + kchunk, op.nz);
+  const int nz = op.nz;
+  const long long sxy = op.sxy;
+  const T fx = T(op.fx), fy = T(op.fy), fz = T(op.fz);
+  const bool in_x = i >= 0 && i < op.nx;
+  const bool bnd_x = i == 0 || i == op.nx - 1;
+  const bool tile_x = lane >= 1 && lane <= kLanes - 2;
+
+  unsigned m_in = 0, m_out = 0, m_bxy = 0;
+  long long col[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int lr = wv * R + r;
+    const int j = ty * TY - 1 + lr;
+    const bool in_y = j >= 0 && j < op.ny;
+    col[r] = in_x && in_y ? (long long)j * op.nx + i : 0;
+    if (in_x && in_y) m_in |= 1u << r;
+    if (in_x && in_y && tile_x && lr >= 1 && lr <= TYE - 2) m_out |= 1u << r;
+    if (bnd_x || j == 0 || j == op.ny - 1) m_bxy |= 1u << r;
+  }
+  const int jrow0 = ty * TY - 1 + wv * R;
+
+  auto ld_x = [&](int r, int k) -> T {
+    const bool ok = ((m_in >> r) & 1u) && k >= 0 && k < nz;
+    const T v = T(ldS(x, ok ? k * sxy + col[r] : 0));
+    return ok ? v : T(0);
+  };
+  auto ld_k = [&](int r, int k) -> T {
+    if constexpr (HK) {
+      const bool ok = ((m_in >> r) & 1u) && k >= 0 && k < nz;
+      const T v = kap[ok ? k * sxy + col[r] : 0];
+      return ok ? v : T(0);
+    } else {
+      return T(0);
+    }
+  };
+  auto ld_b = [&](int r, int k) -> T {
+    if constexpr (MODE == 1) {
+      const bool ok = ((m_out >> r) & 1u) && k >= 0 && k < nz;
+      const T v = T(ldS(b, ok ? k * sxy + col[r] : 0));
+      return ok ? v : T(0);
+    } else {
+      return T(0);
+    }
+  };
+
+  // queues after the shift of step p: X = x(p-1..p+1), XN = x(p+2) (prefetch),
+  // K = kappa(p+1..p+2), ZF = front faces zf(p-1..p+1) with zf(k) = face(kappa(k), kappa(k+1)),
+  // B = b(p..p+1).  FLUX starts two planes early so zf(k0-1) and the faces of plane k0 exist.
+  const int pstart = HK ? k0 - 2 : k0;
+  T X[R][3], XN[R], K[R][2], ZF[R][3], B[R][2];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    X[r][0] = T(0);
+    X[r][1] = ld_x(r, pstart - 1);
+    X[r][2] = ld_x(r, pstart);
+    XN[r] = ld_x(r, pstart + 1);
+    K[r][0] = T(0);
+    K[r][1] = ld_k(r, pstart + 1);
+    ZF[r][0] = ZF[r][1] = ZF[r][2] = T(0);
+    B[r][0] = T(0);
+    B[r][1] = ld_b(r, pstart);
+  }
+  double acc = 0.0;
+  for (int p = pstart; p < k1; ++p) {
+    const int sl = p & 1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int lr = wv * R + r;
+      X[r][0] = X[r][1]; X[r][1] = X[r][2]; X[r][2] = XN[r]; XN[r] = ld_x(r, p + 2);
+      B[r][0] = B[r][1]; B[r][1] = ld_b(r, p + 1);
+      if constexpr (HK) {
+        K[r][0] = K[r][1]; K[r][1] = ld_k(r, p + 2);
+        ZF[r][0] = ZF[r][1]; ZF[r][1] = ZF[r][2];
+        lk[sl][lr][lane] = K[r][0];
+      }
+      lx[sl][lr][lane] = X[r][1];
+    }
+    __syncthreads();
+    if constexpr (HK) {
+      // faces of plane p+1 into slot (p+1) & 1
+      const int fs = sl ^ 1;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int lr = wv * R + r;
+        const T kc = K[r][0];
+        ZF[r][2] = face_op<P, T>(kc, K[r][1], fz);
+        lfe[fs][lr][lane] = lane < kLanes - 1 ? face_op<P, T>(kc, lk[sl][lr][lane + 1], fx) : T(0);
+        lfn[fs][lr][lane] = lr < TYE - 1 ? face_op<P, T>(kc, lk[sl][lr + 1][lane], fy) : T(0);
+      }
+    }
+    if (p >= k0) {
+      // plane p of the tile; its faces were formed one step earlier (slot p & 1)
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (!((m_out >> r) & 1u)) continue;
+        const int lr = wv * R + r;
+        const int j = jrow0 + r;
+        const T xc = X[r][1];
+        T out;
+        if (((m_bxy >> r) & 1u) || p == 0 || p == nz - 1) {
+          out = MODE == 0 ? xc : T(B[r][0] - xc);
+        } else {
+          W7<T> w;
+          if constexpr (KIND == PDEHIP_OP_CONST) {
+            w.C = T(op.w[0]); w.W = T(op.w[1]); w.E = T(op.w[2]); w.S = T(op.w[3]);
+            w.N = T(op.w[4]); w.B = T(op.w[5]); w.F = T(op.w[6]);
+          } else if constexpr (KIND == PDEHIP_OP_POINTWISE) {
+            const long long id = p * sxy + col[r];
+            const double* pw = op.pw;
+            const long long n = op.n;
+            w.C = T(pw[id]); w.W = T(pw[n + id]); w.E = T(pw[2 * n + id]); w.S = T(pw[3 * n + id]);
+            w.N = T(pw[4 * n + id]); w.B = T(pw[5 * n + id]); w.F = T(pw[6 * n + id]);
+          } else {
+            w.W = lfe[sl][lr][lane - 1]; w.E = lfe[sl][lr][lane];
+            w.S = lfn[sl][lr - 1][lane]; w.N = lfn[sl][lr][lane];
+            T sum = w.W + w.E + w.S + w.N;
+            w.B = ZF[r][0]; w.F = ZF[r][1];
+            sum += w.B + w.F;
+            w.C = T(op.e) - sum;
+          }
+          fold_center<3, T>(op, i, j, p, w);
+          const T ax = w.C * xc + w.E * lx[sl][lr][lane + 1] + w.W * lx[sl][lr][lane - 1] +
+                       w.N * lx[sl][lr + 1][lane] + w.S * lx[sl][lr - 1][lane] + w.F * X[r][2] +
+                       w.B * X[r][0];
+          out = MODE == 0 ? ax : T(B[r][0] - ax);
+        }
+        y[p * sxy + col[r]] = store_as<S>(static_cast<double>(out));
+        if (partials)
+          acc += MODE == 0 ? static_cast<double>(xc) * static_cast<double>(out)
+                           : static_cast<double>(out) * static_cast<double>(out);
+      }
+    }
+  }
+  if (partials) {
+    const double s = block_sum(acc, lred);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  }
+}
+
+struct Shape { int tiles_x, tiles_y, chunks, ntiles, kchunk, blocks; };
+
+Shape shape_of(const DevOp& op) {
+  constexpr int R = 2;
+  const int ty = R * kWaves - 2;
+  Shape s;
+  s.tiles_x = (op.nx + kLanes - 3) / (kLanes - 2);
+  s.tiles_y = (op.ny + ty - 1) / ty;
+  // z chunk: 32 planes, shortened on small levels (about two workgroups per CU), lengthened on
+  // huge ones (at most kMaxPartials workgroups, the reduction partials' capacity)
+  auto tiles = [&](int kc) { return (long long)s.tiles_x * s.tiles_y * ((op.nz + kc - 1) / kc); };
+  s.kchunk = 32;
+  while (s.kchunk > 4 && tiles(s.kchunk) < 512) s.kchunk /= 2;
+  while (s.kchunk < op.nz && tiles(s.kchunk) > kMaxPartials - 8) s.kchunk *= 2;
+  s.chunks = (op.nz + s.kchunk - 1) / s.kchunk;
+  s.ntiles = s.tiles_x * s.tiles_y * s.chunks;
+  s.blocks = 8 * ((s.ntiles + 7) / 8);
+  return s;
+}
+
+template <int KIND, Prec P, int MODE>
+void launch(const Launch& L, const DevOp& op, const void* b, const void* x, void* y,
+            double* partials) {
+  const Shape s = shape_of(op);
+  hipLaunchKernelGGL((k_plane3<KIND, P, MODE, 2>), dim3(s.blocks), dim3(kThreads), 0, L.stream,
+                     op, b, x, y, partials, s.tiles_x, s.tiles_y, s.ntiles, s.kchunk, L.sc);
+}
+
+template <int MODE>
+void launch_mode(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x, void* y,
+                 double* partials) {
+  switch (op.kind) {
+    case PDEHIP_OP_CONST:
+      if (p == P64) launch<PDEHIP_OP_CONST, P64, MODE>(L, op, b, x, y, partials);
+      else if (p == P32) launch<PDEHIP_OP_CONST, P32, MODE>(L, op, b, x, y, partials);
+      else launch<PDEHIP_OP_CONST, P16, MODE>(L, op, b, x, y, partials);
+      break;
+    case PDEHIP_OP_POINTWISE:
+      launch<PDEHIP_OP_POINTWISE, P64, MODE>(L, op, b, x, y, partials);
+      break;
+    default:
+      if (p == P64) launch<PDEHIP_OP_FLUX, P64, MODE>(L, op, b, x, y, partials);
+      else if (p == P32) launch<PDEHIP_OP_FLUX, P32, MODE>(L, op, b, x, y, partials);
+      else launch<PDEHIP_OP_FLUX, P16, MODE>(L, op, b, x, y, partials);
+  }
+}
+
+}  // namespace
+
+int plane_blocks(const DevOp& op) { return shape_of(op).blocks; }
+
+void apply_plane(const Launch& L, const DevOp& op, Prec p, const void* x, void* y, double* partials) {
+  launch_mode<0>(L, op, p, nullptr, x, y, partials);
+}
+
+void residual_plane(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x, void* r,
+                    double* partials) {
+  launch_mode<1>(L, op, p, b, x, r, partials);
+}
+
+}  // namespace pdehip

@@ -7,12 +7,15 @@
 // neighbours of the running node stay in registers (u[k-1], u[k], u[k+1]) instead of being
 // re-read.  x/y neighbours come through L1/L2 (the same rows are read by the adjacent
 // threads of the block).  Each launch has >= 8 x #CU workgroups for the 257^3 level.
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace pdehip {
 namespace {
 
 constexpr int kZChunk = 16;
+bool g_plane = true;
 
 template <Prec P> struct PT;
 template <> struct PT<P64> { using T = double; using S = double; using K = double; };
@@ -294,19 +297,32 @@ void launch_rbgs_p(const Launch& L, const DevOp& op, Prec p, const void* b, void
 
 }  // namespace
 
+// 3D levels use the plane-march kernels (k_plane.hip); 2D levels the grid-stride ones here.
 int partial_blocks(const DevOp& op) {
+  if (op.nz > 1 && g_plane) return plane_blocks(op);
   const dim3 g = grid_for(op);
   return static_cast<int>(g.x * g.y);
 }
 
 void apply(const Launch& L, const DevOp& op, Prec p, const void* x, void* y, double* partials) {
-  launch_stencil<0>(L, op, p, nullptr, x, y, partials);
+  if (op.nz > 1 && g_plane) apply_plane(L, op, p, x, y, partials);
+  else launch_stencil<0>(L, op, p, nullptr, x, y, partials);
 }
 
 void residual(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x, void* r,
               double* partials) {
-  launch_stencil<1>(L, op, p, b, x, r, partials);
+  if (op.nz > 1 && g_plane) residual_plane(L, op, p, b, x, r, partials);
+  else launch_stencil<1>(L, op, p, b, x, r, partials);
 }
+
+int partial_blocks_max(const DevOp& op) {
+  const dim3 g = grid_for(op);
+  const int legacy = static_cast<int>(g.x * g.y);
+  return op.nz > 1 ? std::max(legacy, plane_blocks(op)) : legacy;
+}
+
+void set_plane_stencils(bool on) { g_plane = on; }
+bool plane_stencils() { return g_plane; }
 
 void rbgs(const Launch& L, const DevOp& op, Prec p, const void* b, void* x, int color) {
   const bool d3 = op.nz > 1;

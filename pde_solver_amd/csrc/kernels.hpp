@@ -45,6 +45,16 @@ void set_smooth_tuning(const SmoothTuning& t);
 SmoothTuning smooth_tuning();
 // Number of blocks `partials`-producing kernels use for this op (<= kMaxPartials).
 int partial_blocks(const DevOp& op);
+// 3D apply / residual as plane marches (k_plane.hip); apply()/residual() route 3D levels there
+// unless switched off (A/B timing).
+void apply_plane(const Launch& L, const DevOp& op, Prec p, const void* x, void* y, double* partials);
+void residual_plane(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x, void* r,
+                    double* partials);
+int plane_blocks(const DevOp& op);
+// Partials capacity that covers either routing of this op.
+int partial_blocks_max(const DevOp& op);
+void set_plane_stencils(bool on);
+bool plane_stencils();
 
 // Reference-semantics relaxation (FP64): interior update (-(f + N(u)) - sum w u_nb)/C.
 struct NonlinearTerms {

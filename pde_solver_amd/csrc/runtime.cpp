@@ -526,7 +526,7 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
   }
 
   // ---- reductions and scalars
-  int np = std::max(partial_blocks(P->op()), vec_blocks(g.n));
+  int np = std::max(partial_blocks_max(P->op()), vec_blocks(g.n));
   np = std::max(np, relax_blocks(P->op()));
   np = std::max(np, krylov_blocks(g.n));
   np = std::max(np, resnorm_blocks(P->op()));
