@@ -225,8 +225,10 @@ void pdehip_plan_destroy(pdehip_plan* plan);
 /* Process-wide launch-shape knobs (no counterpart in the reference; used by the autotuner and
  * for A/B timing).  Keys: "smooth.fused" (0/1: multigrid smooths with the fused out-of-place
  * sweep), "smooth.rows" (2 or 4: rows per thread of the 3D sweep tile), "smooth.zchunk" (z
- * planes per 3D workgroup, 4..512), "smooth.ychunk" (2D rows per wavefront, 16..4096),
- * "stencil.plane" (0/1: 3D apply and residual as plane-march kernels).
+ * planes per 3D workgroup, 4..512, or 0 to choose from occupancy), "smooth.ychunk" (2D rows
+ * per wavefront, 16..4096), "stencil.plane" (0/1: 3D apply and residual as plane-march
+ * kernels), "smooth.pair" (0/1: 3D FP32/FP16 sweeps with column-pair tiles, no colour
+ * divergence).
  * Results do not depend on any of them.  Not thread-safe against running solves. */
 int pdehip_tune_set(const char* key, int value);
 int pdehip_tune_get(const char* key, int* value);

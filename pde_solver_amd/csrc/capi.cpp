@@ -247,8 +247,9 @@ int pdehip_tune_set(const char* key, int value) {
   const std::string k(key);
   if (k == "smooth.fused" && (value == 0 || value == 1)) t.fused = value;
   else if (k == "smooth.rows" && (value == 2 || value == 4)) t.rows = value;
-  else if (k == "smooth.zchunk" && value >= 4 && value <= 512) t.zchunk = value;
+  else if (k == "smooth.zchunk" && (value == 0 || (value >= 4 && value <= 512))) t.zchunk = value;
   else if (k == "smooth.ychunk" && value >= 16 && value <= 4096) t.ychunk = value;
+  else if (k == "smooth.pair" && (value == 0 || value == 1)) t.pair = value;
   else if (k == "stencil.plane" && (value == 0 || value == 1)) { set_plane_stencils(value == 1); return PDEHIP_OK; }
   else return PDEHIP_ERR_INVALID;
   set_smooth_tuning(t);
@@ -263,6 +264,7 @@ int pdehip_tune_get(const char* key, int* value) {
   else if (k == "smooth.rows") *value = t.rows;
   else if (k == "smooth.zchunk") *value = t.zchunk;
   else if (k == "smooth.ychunk") *value = t.ychunk;
+  else if (k == "smooth.pair") *value = t.pair;
   else if (k == "stencil.plane") *value = plane_stencils() ? 1 : 0;
   else return PDEHIP_ERR_INVALID;
   return PDEHIP_OK;

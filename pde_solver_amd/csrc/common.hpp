@@ -163,6 +163,24 @@ __device__ __forceinline__ bool is_boundary2(const DevOp& op, int i, int j) {
   return i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1;
 }
 
+// ------------------------------------------------------------------ launch shaping (host)
+// z-chunk count for a plane-march launch: `tiles` (i,j) tiles each split into chunks of
+// ceil(nz / chunks) planes plus `prologue` warm-up planes; `capacity` workgroups run at once.
+// Minimises rounds x planes-per-workgroup, so the last round is not a thin tail.
+inline int plane_chunk(long long tiles, int nz, int capacity, int prologue, int min_planes) {
+  int best_kc = nz;
+  long long best = -1;
+  for (int chunks = 1; chunks <= nz; ++chunks) {
+    const int kc = (nz + chunks - 1) / chunks;
+    if (kc < min_planes && chunks > 1) break;
+    const long long n = tiles * ((nz + kc - 1) / kc);
+    const long long rounds = (n + capacity - 1) / capacity;
+    const long long cost = rounds * (kc + prologue);
+    if (best < 0 || cost < best) { best = cost; best_kc = kc; }
+  }
+  return best_kc;
+}
+
 // ------------------------------------------------------------------ reductions
 // Sum over the 64 lanes of a wavefront.
 template <typename T>

@@ -217,18 +217,19 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
 
 struct Shape { int tiles_x, tiles_y, chunks, ntiles, kchunk, blocks; };
 
+// Launch shape shared by every instantiation (the partials count must not depend on the
+// precision): occupancy of the FP64 FLUX residual, the heaviest of them.
 Shape shape_of(const DevOp& op) {
   constexpr int R = 2;
   const int ty = R * kWaves - 2;
   Shape s;
   s.tiles_x = (op.nx + kLanes - 3) / (kLanes - 2);
   s.tiles_y = (op.ny + ty - 1) / ty;
-  // z chunk: 32 planes, shortened on small levels (about two workgroups per CU), lengthened on
-  // huge ones (at most kMaxPartials workgroups, the reduction partials' capacity)
-  auto tiles = [&](int kc) { return (long long)s.tiles_x * s.tiles_y * ((op.nz + kc - 1) / kc); };
-  s.kchunk = 32;
-  while (s.kchunk > 4 && tiles(s.kchunk) < 512) s.kchunk /= 2;
-  while (s.kchunk < op.nz && tiles(s.kchunk) > kMaxPartials - 8) s.kchunk *= 2;
+  static const int occ = blocks_per_cu(k_plane3<PDEHIP_OP_FLUX, P64, 1, R>, kThreads);
+  const long long txy = (long long)s.tiles_x * s.tiles_y;
+  s.kchunk = plane_chunk(txy, op.nz, device_cus() * occ, 2, 4);
+  // at most kMaxPartials workgroups (the reduction partials' capacity)
+  while (s.kchunk < op.nz && txy * ((op.nz + s.kchunk - 1) / s.kchunk) > kMaxPartials - 8) s.kchunk *= 2;
   s.chunks = (op.nz + s.kchunk - 1) / s.kchunk;
   s.ntiles = s.tiles_x * s.tiles_y * s.chunks;
   s.blocks = 8 * ((s.ntiles + 7) / 8);

@@ -21,6 +21,8 @@
 // (3 s + s_k) N; the ring and the two extra planes per z chunk are re-reads served by L2 /
 // Infinity Cache.  Workgroups are numbered so that each XCD (round-robin dispatch) receives a
 // contiguous run of tiles, which keeps ring re-reads inside one XCD's L2.
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace pdehip {
@@ -397,6 +399,226 @@ k_rbgs_fused2(DevOp op, const void* __restrict__ bv, const void* __restrict__ xi
   }
 }
 
+// ------------------------------------------------------------------------------- 3D, paired
+// Divergence-free variant: each lane owns a column pair (2l, 2l+1) of a 128-wide extended tile
+// (output 124 wide).  In any row and plane the first colour sits at the same pair offset s in
+// every lane (s depends on row, plane and tile origin only), and the second colour of plane
+// p-1 at the same offset, so both colour phases keep every lane busy.  Same plane pipeline,
+// same face ring and the same arithmetic per node as k_rbgs_fused3.
+constexpr int kPairCols = 2 * kLanes;
+constexpr int kPairTileX = kPairCols - 2 * kHalo;
+
+template <int KIND, Prec P, int R, bool ZERO>
+__global__ void __launch_bounds__(kThreads)
+k_rbgs_pair3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v,
+             void* __restrict__ xout_v, int c1, int tiles_x, int tiles_y, int ntiles, int kchunk,
+             const KScalars* sc) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  constexpr int TYE = R * kWaves;
+  constexpr int TY = TYE - 2 * kHalo;
+  constexpr bool HK = KIND == PDEHIP_OP_FLUX;
+  constexpr int KR = HK ? TYE : 1;
+  constexpr int W = kPairCols;
+  __shared__ T lu[2][TYE][W];
+  __shared__ T lk[KR][W];
+  __shared__ T lfe[HK ? 3 : 1][KR][W];
+  __shared__ T lfn[HK ? 3 : 1][KR][W];
+  if (skip(sc)) return;
+  const int t = xcd_tile();
+  if (t >= ntiles) return;
+  const S* __restrict__ b = static_cast<const S*>(bv);
+  const S* __restrict__ xin = static_cast<const S*>(xin_v);
+  const T* __restrict__ kap = static_cast<const T*>(op.kappa);
+  S* __restrict__ xout = static_cast<S*>(xout_v);
+  const int tx = t % tiles_x;
+  const int rest = t / tiles_x;
+  const int ty = rest % tiles_y;
+  const int cz = rest / tiles_y;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wv = threadIdx.x / kLanes;
+  const int i0 = tx * kPairTileX - kHalo;      // global x of extended column 0 (even)
+  const int k0 = cz * kchunk;
+  const int k1 = min(k0 + kchunk, op.nz);
+  const int nz = op.nz;
+  const long long sxy = op.sxy;
+  const T fx = T(op.fx), fy = T(op.fy), fz = T(op.fz);
+  const int jrow0 = ty * TY - kHalo + wv * R;
+
+  // masks, bit (2r + s): in grid, first-colour candidate, second-colour candidate, output
+  unsigned m_in = 0, m_r1 = 0, m_r2 = 0, m_out = 0;
+  long long col[R][2];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int lr = wv * R + r;
+    const int j = jrow0 + r;
+    const bool in_y = j >= 0 && j < op.ny;
+    const bool int_y = j >= 1 && j <= op.ny - 2;
+    const bool ring_y = lr >= 1 && lr <= TYE - 2;
+    const bool tile_y = lr >= kHalo && lr < TYE - kHalo;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int c = 2 * lane + q;
+      const int i = i0 + c;
+      const bool in_x = i >= 0 && i < op.nx;
+      const bool int_x = i >= 1 && i <= op.nx - 2;
+      const bool ring_x = c >= 1 && c <= W - 2;
+      const bool tile_x = c >= kHalo && c < W - kHalo;
+      const unsigned bit = 1u << (2 * r + q);
+      col[r][q] = in_x && in_y ? (long long)j * op.nx + i : 0;
+      if (in_x && in_y) m_in |= bit;
+      if (ring_x && ring_y && int_x && int_y) m_r1 |= bit;
+      if (tile_x && tile_y && int_x && int_y) m_r2 |= bit;
+      if (tile_x && tile_y && in_x && in_y) m_out |= bit;
+    }
+  }
+  auto ld_u = [&](int r, int q, int k) -> T {
+    if constexpr (ZERO) {
+      return T(0);
+    } else {
+      const bool ok = ((m_in >> (2 * r + q)) & 1u) && k >= 0 && k < nz;
+      const T v = T(ldS(xin, ok ? k * sxy + col[r][q] : 0));
+      return ok ? v : T(0);
+    }
+  };
+  auto ld_k = [&](int r, int q, int k) -> T {
+    if constexpr (HK) {
+      const bool ok = ((m_in >> (2 * r + q)) & 1u) && k >= 0 && k < nz;
+      const T v = kap[ok ? k * sxy + col[r][q] : 0];
+      return ok ? v : T(0);
+    } else {
+      return T(0);
+    }
+  };
+  auto ld_b = [&](int r, int q, int k) -> T {
+    const bool ok = ((m_r1 >> (2 * r + q)) & 1u) && k >= 1 && k < nz - 1;
+    const T v = T(ldS(b, ok ? k * sxy + col[r][q] : 0));
+    return ok ? v : T(0);
+  };
+
+  const int pstart = HK ? k0 - 3 : k0 - 1;
+  T U[R][2][4], B[R][2][3], K[R][2][3], ZF[R][2][4];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      U[r][q][0] = U[r][q][1] = T(0);
+      U[r][q][2] = ld_u(r, q, pstart - 1);
+      U[r][q][3] = ld_u(r, q, pstart);
+      B[r][q][0] = B[r][q][1] = T(0);
+      B[r][q][2] = ld_b(r, q, pstart);
+      K[r][q][0] = T(0);
+      K[r][q][1] = ld_k(r, q, pstart + 1);
+      K[r][q][2] = ld_k(r, q, pstart + 2);
+      ZF[r][q][0] = ZF[r][q][1] = ZF[r][q][2] = ZF[r][q][3] = T(0);
+    }
+
+  for (int p = pstart; p <= k1; ++p) {
+    const int sl = p & 1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int lr = wv * R + r;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        U[r][q][0] = U[r][q][1]; U[r][q][1] = U[r][q][2]; U[r][q][2] = U[r][q][3];
+        U[r][q][3] = ld_u(r, q, p + 1);
+        B[r][q][0] = B[r][q][1]; B[r][q][1] = B[r][q][2]; B[r][q][2] = ld_b(r, q, p + 1);
+        if constexpr (HK) {
+          K[r][q][0] = K[r][q][1]; K[r][q][1] = K[r][q][2]; K[r][q][2] = ld_k(r, q, p + 3);
+          ZF[r][q][0] = ZF[r][q][1]; ZF[r][q][1] = ZF[r][q][2]; ZF[r][q][2] = ZF[r][q][3];
+          lk[lr][2 * lane + q] = K[r][q][0];
+        }
+        lu[sl][lr][2 * lane + q] = U[r][q][2];
+      }
+    }
+    __syncthreads();
+    if constexpr (HK) {
+      const int fs = mod3(p + 1);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int lr = wv * R + r;
+        const T ke = lane < kLanes - 1 ? lk[lr][2 * lane + 2] : T(0);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int c = 2 * lane + q;
+          const T kc = K[r][q][0];
+          ZF[r][q][3] = face_smooth<T>(kc, K[r][q][1], fz);
+          lfe[fs][lr][c] = face_smooth<T>(kc, q == 0 ? K[r][1][0] : ke, fx);
+          lfn[fs][lr][c] = lr < TYE - 1 ? face_smooth<T>(kc, lk[lr + 1][c], fy) : T(0);
+        }
+      }
+    }
+    // first colour on plane p (tile + ring).  The pair offset s of a row is wave-uniform: branch
+    // on it into two compile-time bodies (a runtime index into the register arrays would put
+    // them in scratch memory).
+    if (p >= k0 - 1 && p >= 1 && p <= nz - 2) {
+      const int f0 = HK ? mod3(p) : 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int j = jrow0 + r;
+        const int lr = wv * R + r;
+        auto body = [&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          if (!((m_r1 >> (2 * r + q)) & 1u)) return;
+          const int c = 2 * lane + q;
+          const long long id = p * sxy + col[r][q];
+          T aw = T(0), ae = T(0), as = T(0), an = T(0);
+          if constexpr (HK) {
+            aw = lfe[f0][lr][c - 1]; ae = lfe[f0][lr][c];
+            as = lfn[f0][lr - 1][c]; an = lfn[f0][lr][c];
+          }
+          W7<T> w = weights_at<3, KIND, T>(op, id, aw, ae, as, an, ZF[r][q][1], ZF[r][q][2]);
+          fold_center<3, T>(op, i0 + c, j, p, w);
+          const T v = rounded<S, T>(gs_value<3, T>(w, B[r][q][1], lu[sl][lr][c + 1],
+                                                   lu[sl][lr][c - 1], lu[sl][lr + 1][c],
+                                                   lu[sl][lr - 1][c], U[r][q][3], U[r][q][1]));
+          U[r][q][2] = v;
+          lu[sl][lr][c] = v;
+        };
+        if ((c1 + i0 + j + p) & 1) body(std::integral_constant<int, 1>{});
+        else body(std::integral_constant<int, 0>{});
+      }
+    }
+    // second colour on plane q = p - 1 (tile): the same pair offset as plane p's first colour
+    const int qp = p - 1;
+    if (qp >= k0 && qp < k1) {
+      const int sq = sl ^ 1;
+      if (qp >= 1 && qp <= nz - 2) {
+        const int f1 = HK ? mod3(qp) : 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int j = jrow0 + r;
+          const int lr = wv * R + r;
+          auto body = [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if (!((m_r2 >> (2 * r + q)) & 1u)) return;
+            const int c = 2 * lane + q;
+            const long long id = qp * sxy + col[r][q];
+            T aw = T(0), ae = T(0), as = T(0), an = T(0);
+            if constexpr (HK) {
+              aw = lfe[f1][lr][c - 1]; ae = lfe[f1][lr][c];
+              as = lfn[f1][lr - 1][c]; an = lfn[f1][lr][c];
+            }
+            W7<T> w = weights_at<3, KIND, T>(op, id, aw, ae, as, an, ZF[r][q][0], ZF[r][q][1]);
+            fold_center<3, T>(op, i0 + c, j, qp, w);
+            U[r][q][1] = rounded<S, T>(gs_value<3, T>(w, B[r][q][0], lu[sq][lr][c + 1],
+                                                      lu[sq][lr][c - 1], lu[sq][lr + 1][c],
+                                                      lu[sq][lr - 1][c], U[r][q][2], U[r][q][0]));
+          };
+          if ((c1 + i0 + j + p) & 1) body(std::integral_constant<int, 1>{});
+          else body(std::integral_constant<int, 0>{});
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+          if ((m_out >> (2 * r + q)) & 1u) xout[qp * sxy + col[r][q]] = to_store<S, T>(U[r][q][1]);
+    }
+    __syncthreads();
+  }
+}
+
 SmoothTuning g_tune{};
 
 template <int KIND, Prec P>
@@ -406,11 +628,14 @@ void launch3(const Launch& L, const DevOp& op, const void* b, const void* xin, v
   const int ty = rows * kWaves - 2 * kHalo;
   const int tiles_x = (op.nx + kTileX - 1) / kTileX;
   const int tiles_y = (op.ny + ty - 1) / ty;
-  // z chunk: the configured length, shortened on small levels so the launch still has about
-  // two workgroups per CU
-  int kchunk = g_tune.zchunk > 0 ? g_tune.zchunk : 32;
-  while (kchunk > 4 && (long long)tiles_x * tiles_y * ((op.nz + kchunk - 1) / kchunk) < 512)
-    kchunk /= 2;
+  // z chunk: configured, or chosen so the launch fills the resident workgroup slots in whole
+  // rounds (occupancy of the exact kernel instantiation)
+  static const int occ4f = blocks_per_cu(k_rbgs_fused3<KIND, P, 4, false>, kThreads);
+  static const int occ2f = blocks_per_cu(k_rbgs_fused3<KIND, P, 2, false>, kThreads);
+  const int capacity = device_cus() * (rows == 4 ? occ4f : occ2f);
+  const int prologue = KIND == PDEHIP_OP_FLUX ? 3 : 1;
+  const int kchunk = g_tune.zchunk > 0 ? g_tune.zchunk
+                                       : plane_chunk((long long)tiles_x * tiles_y, op.nz, capacity, prologue, 4);
   const int chunks = (op.nz + kchunk - 1) / kchunk;
   const int ntiles = tiles_x * tiles_y * chunks;
   const dim3 g(static_cast<unsigned>(8 * ((ntiles + 7) / 8)));
@@ -423,6 +648,29 @@ void launch3(const Launch& L, const DevOp& op, const void* b, const void* xin, v
     if (zero_in) PDEHIP_F3(2, true); else PDEHIP_F3(2, false);
   }
 #undef PDEHIP_F3
+}
+
+template <int KIND, Prec P>
+void launch3_pair(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout,
+                  int c1, bool zero_in) {
+  constexpr int rows = 2;
+  const int ty = rows * kWaves - 2 * kHalo;
+  const int tiles_x = (op.nx + kPairTileX - 1) / kPairTileX;
+  const int tiles_y = (op.ny + ty - 1) / ty;
+  static const int occ = blocks_per_cu(k_rbgs_pair3<KIND, P, rows, false>, kThreads);
+  const int prologue = KIND == PDEHIP_OP_FLUX ? 3 : 1;
+  const int kchunk = g_tune.zchunk > 0 ? g_tune.zchunk
+                                       : plane_chunk((long long)tiles_x * tiles_y, op.nz,
+                                                     device_cus() * occ, prologue, 4);
+  const int chunks = (op.nz + kchunk - 1) / kchunk;
+  const int ntiles = tiles_x * tiles_y * chunks;
+  const dim3 g(static_cast<unsigned>(8 * ((ntiles + 7) / 8)));
+  if (zero_in)
+    hipLaunchKernelGGL((k_rbgs_pair3<KIND, P, rows, true>), g, dim3(kThreads), 0, L.stream, op, b,
+                       xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc);
+  else
+    hipLaunchKernelGGL((k_rbgs_pair3<KIND, P, rows, false>), g, dim3(kThreads), 0, L.stream, op, b,
+                       xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc);
 }
 
 template <int KIND, Prec P>
@@ -448,10 +696,17 @@ template <int KIND>
 void launch_kind(const Launch& L, const DevOp& op, Prec p, const void* b, const void* xin,
                  void* xout, int c1, bool zero_in) {
   if (op.nz > 1) {
+    const bool pair = g_tune.pair != 0;
     switch (p) {
       case P64: launch3<KIND, P64>(L, op, b, xin, xout, c1, zero_in); break;
-      case P32: launch3<KIND, P32>(L, op, b, xin, xout, c1, zero_in); break;
-      default: launch3<KIND, P16>(L, op, b, xin, xout, c1, zero_in); break;
+      case P32:
+        if (pair) launch3_pair<KIND, P32>(L, op, b, xin, xout, c1, zero_in);
+        else launch3<KIND, P32>(L, op, b, xin, xout, c1, zero_in);
+        break;
+      default:
+        if (pair) launch3_pair<KIND, P16>(L, op, b, xin, xout, c1, zero_in);
+        else launch3<KIND, P16>(L, op, b, xin, xout, c1, zero_in);
+        break;
     }
   } else {
     switch (p) {

@@ -16,6 +16,16 @@ enum Prec { P64 = 0, P32 = 1, P16 = 2 };
 inline size_t prec_bytes(Prec p) { return p == P64 ? 8 : (p == P32 ? 4 : 2); }
 inline size_t kappa_bytes(Prec p) { return p == P64 ? 8 : 4; }
 
+// Compute units of the current device and resident workgroups per CU of a kernel (cached by
+// the callers per kernel instantiation).
+int device_cus();
+template <typename K>
+int blocks_per_cu(K kernel, int threads) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1) n = 1;
+  return n;
+}
+
 struct Launch {
   hipStream_t stream;
   const KScalars* sc;   // device scalars; kernels return early once sc->done (may be null)
@@ -38,8 +48,9 @@ void rbgs_sweep(const Launch& L, const DevOp& op, Prec p, const void* b, const v
 struct SmoothTuning {
   int fused = 1;     // 0: multigrid smooths with the two-launch in-place sweep
   int rows = 2;      // 3D: rows per thread (tile 60 x (8*rows - 4)), 2 or 4
-  int zchunk = 32;   // 3D: z planes per workgroup
+  int zchunk = 0;    // 3D: z planes per workgroup (0: chosen from occupancy and grid size)
   int ychunk = 64;   // 2D: rows per wavefront (upper bound)
+  int pair = 0;      // 3D FP32/FP16: column-pair (divergence-free) tiles
 };
 void set_smooth_tuning(const SmoothTuning& t);
 SmoothTuning smooth_tuning();

@@ -8,6 +8,16 @@
 
 namespace pdehip {
 
+int device_cus() {
+  static const int n = [] {
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
+    return prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  }();
+  return n;
+}
+
 void hip_check(hipError_t e, const char* what) {
   if (e == hipSuccess) return;
   if (e == hipErrorOutOfMemory) throw Error(PDEHIP_ERR_OOM, std::string(what) + ": out of device memory");

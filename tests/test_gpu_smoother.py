@@ -10,7 +10,8 @@ from pde_solver_amd.capi import BC
 
 pytestmark = pytest.mark.gpu
 
-DEFAULTS = {"smooth.fused": 1, "smooth.rows": 2, "smooth.zchunk": 32, "smooth.ychunk": 64}
+DEFAULTS = {"smooth.fused": 1, "smooth.rows": 2, "smooth.zchunk": 0, "smooth.ychunk": 64,
+            "smooth.pair": 0}
 
 
 @pytest.fixture(scope="module")
@@ -61,6 +62,9 @@ CASES_3D = [
     ("flux", (33, 65, 129), {"rows": 4, "zchunk": 4}),
     ("flux", (65, 33, 17), {"rows": 2, "zchunk": 8}),
     ("const", (17, 129, 65), {"rows": 4, "zchunk": 16}),
+    ("flux", (33, 65, 129), {"rows": 2, "zchunk": 0, "pair": 1}),
+    ("flux", (65, 33, 257), {"rows": 2, "zchunk": 8, "pair": 1}),
+    ("const", (17, 129, 65), {"rows": 2, "zchunk": 0, "pair": 1}),
 ]
 
 
