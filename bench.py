@@ -13,6 +13,9 @@ guess with the operator, hierarchy and right-hand side resident in HBM.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 257] [--no-cpu-baseline]
     python bench.py --only mixed|fp64 [--steps K] [--warmup W]   (one path only, for profiling)
+    python bench.py --kernels 4,3 [--precision mixed|fp64] [--steps K]
+        (K launches of plan kernels only: 0 apply, 1 in-place sweep, 2 residual, 3 copy,
+         4 fused sweep; for hardware-counter runs)
 """
 from __future__ import annotations
 
@@ -100,6 +103,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="run W + K solves of one path and print their times (profiling runs)")
+    ap.add_argument("--kernels", default=None,
+                    help="comma list of plan kernels to launch K times each (counter runs)")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
@@ -117,6 +123,20 @@ def main() -> int:
     spec = make_problem(n)
     log(f"inputs {n}^3 generated in {time.perf_counter() - t:.1f}s")
     ctx = hip.Context()
+
+    if args.kernels:
+        prec = capi.MIXED if args.precision == "mixed" else capi.FP64
+        plan = ctx.plan(spec, options(prec, 1e-8))
+        res = {}
+        for k in [int(v) for v in args.kernels.split(",")]:
+            ms, nbytes = plan.time_kernel(k, prec, reps=args.steps)
+            res[str(k)] = {"avg_ms": round(ms, 5), "algorithmic_bytes": nbytes,
+                           "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+        print(json.dumps({"kernels": res, "precision": args.precision, "n": n,
+                          "launches_each": args.steps + 1}), flush=True)
+        plan.close()
+        ctx.close()
+        return 0
 
     if args.only:
         plan = ctx.plan(spec, options(capi.MIXED if args.only == "mixed" else capi.FP64, 1e-8))

@@ -27,7 +27,7 @@ struct DevOp {
   long long n;           // nx*ny*nz
   double w[7];           // CONST weights: C, W, E, S, N, B(k-1), F(k+1)
   const double* pw;      // POINTWISE weights, structure of arrays [7][n]
-  const void* kappa;     // FLUX conductivity (element type = kernel's TK)
+  const void* kappa;     // FLUX: kappa per node (FP64 levels) or 1/kappa (FP32/FP16 levels)
   double fx, fy, fz;     // FLUX: a/h^2 per axis
   double e;              // FLUX: constant reaction term
   double theta[6];       // folded closure factors (0 = not folded / Dirichlet)
@@ -64,15 +64,8 @@ template <> __device__ __forceinline__ double store_f<double>(float v) { return 
 template <> __device__ __forceinline__ float store_f<float>(float v) { return v; }
 template <> __device__ __forceinline__ __half store_f<__half>(float v) { return __float2half(v); }
 
-// Harmonic mean of two node conductivities, times the axis factor (face coefficient).
-template <typename T>
-__device__ __forceinline__ T face_coef(T kc, T kn, T f) {
-  const T s = kc + kn;
-  return s > T(0) ? f * (T(2) * kc * kn) / s : T(0);
-}
-
 // Reciprocal for the smoothers: the hardware estimate (float, 1 ulp) or the estimate refined by
-// two Newton steps (double).  The operator itself (apply, residual) divides exactly; a
+// two Newton steps (double).  The FP64 operator itself (apply, residual) divides exactly; a
 // smoother only has to be the same in every kernel that applies it.
 template <typename T> __device__ __forceinline__ T recip(T x);
 template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -83,12 +76,28 @@ template <> __device__ __forceinline__ double recip<double>(double x) {
   return r;
 }
 
-// Smoother face coefficient: face_coef with the division replaced by recip().  Symmetric bit
-// for bit in (kc, kn), so a face computed from either side agrees.
-template <typename T>
-__device__ __forceinline__ T face_smooth(T kc, T kn, T f) {
-  const T s = kc + kn;
-  return s > T(0) ? f * (T(2) * kc * kn) * recip<T>(s) : T(0);
+// Face coefficient: the axis factor a/h^2 times the harmonic mean of the two node
+// conductivities.  FP64 levels store kappa and divide exactly.  FP32/FP16 levels store
+// rho = 1/kappa, where the harmonic mean is 2/(rho_c + rho_n): one add and one hardware
+// reciprocal (kappa = 0 gives rho = inf and a zero face).  Both forms are symmetric bit for bit
+// in their two arguments, so a face computed from either side agrees.
+template <typename T> __device__ __forceinline__ T face_coef(T kc, T kn, T f);
+template <> __device__ __forceinline__ double face_coef<double>(double kc, double kn, double f) {
+  const double s = kc + kn;
+  return s > 0.0 ? f * (2.0 * kc * kn) / s : 0.0;
+}
+template <> __device__ __forceinline__ float face_coef<float>(float rc, float rn, float f) {
+  return (f + f) * __builtin_amdgcn_rcpf(rc + rn);
+}
+
+// Smoother face coefficient: FP64 with the division replaced by recip(); FP32 as face_coef.
+template <typename T> __device__ __forceinline__ T face_smooth(T kc, T kn, T f);
+template <> __device__ __forceinline__ double face_smooth<double>(double kc, double kn, double f) {
+  const double s = kc + kn;
+  return s > 0.0 ? f * (2.0 * kc * kn) * recip<double>(s) : 0.0;
+}
+template <> __device__ __forceinline__ float face_smooth<float>(float rc, float rn, float f) {
+  return face_coef<float>(rc, rn, f);
 }
 
 // Weights of interior row (i,j,k) before folding.  SMOOTH: faces by face_smooth (the

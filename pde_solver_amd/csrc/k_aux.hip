@@ -337,6 +337,13 @@ __global__ void k_convert(const void* inv, void* outv, long long n, double scale
     out[i] = store_as<SO>(ldd(in, i) * scale);
 }
 
+// rho = 1/kappa in FP32 (the conductivity form the FP32/FP16 operators store).
+__global__ void k_recip_f32(const double* in, float* out, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = static_cast<float>(1.0 / in[i]);
+}
+
 template <Prec P>
 __global__ void __launch_bounds__(kBlock) k_dot(const void* av, const void* bv, long long n,
                                                double* partials, const KScalars* sc) {
@@ -555,6 +562,10 @@ void convert(hipStream_t s, Prec pin, const void* in, Prec pout, void* out, long
   else if (pin == P64 && pout == P16) C_(P64, P16);
   else C_(P16, P64);
 #undef C_
+}
+
+void reciprocal_f32(hipStream_t s, const double* in, float* out, long long n) {
+  hipLaunchKernelGGL(k_recip_f32, dim3(grid1(n)), dim3(kBlock), 0, s, in, out, n);
 }
 
 void dot(const Launch& L, Prec p, const void* a, const void* b, long long n, double* partials) {

@@ -10,8 +10,8 @@
 //     r.r).  Boundary rows are identity.
 // Double-buffered slots make one barrier per plane enough: a slot is rewritten two planes
 // after it was read, and a thread can only be there once every thread has passed the barrier
-// in between.  The FP64 operator divides exactly (face_coef); the FP32/FP16 operators use the
-// hardware reciprocal (face_smooth), which is within their own rounding.
+// in between.  The FP64 operator divides exactly (face_coef); the FP32/FP16 operators keep
+// rho = 1/kappa and form each face with one hardware reciprocal, within their own rounding.
 #include "kernels.hpp"
 
 namespace pdehip {
@@ -35,8 +35,7 @@ __device__ __forceinline__ bool skip(const KScalars* sc) { return sc && (sc->don
 
 template <Prec P, typename T>
 __device__ __forceinline__ T face_op(T kc, T kn, T f) {
-  if constexpr (P == P64) return face_coef<T>(kc, kn, f);
-  else return face_smooth<T>(kc, kn, f);
+  return face_coef<T>(kc, kn, f);
 }
 
 // MODE 0: y = A x, partials of x.y.  MODE 1: r = b - A x, partials of r.r.
@@ -83,42 +82,31 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
   const bool bnd_x = i == 0 || i == op.nx - 1;
   const bool tile_x = lane >= 1 && lane <= kLanes - 2;
 
-  unsigned m_in = 0, m_out = 0, m_bxy = 0;
+  // Lanes and rows outside the grid load a clamped in-grid node: no output reads their values.
+  unsigned m_out = 0, m_bxy = 0, m_fxy = 0;
   long long col[R];
+  const int ci = min(max(i, 0), op.nx - 1);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int lr = wv * R + r;
     const int j = ty * TY - 1 + lr;
     const bool in_y = j >= 0 && j < op.ny;
-    col[r] = in_x && in_y ? (long long)j * op.nx + i : 0;
-    if (in_x && in_y) m_in |= 1u << r;
+    col[r] = (long long)min(max(j, 0), op.ny - 1) * op.nx + ci;
     if (in_x && in_y && tile_x && lr >= 1 && lr <= TYE - 2) m_out |= 1u << r;
     if (bnd_x || j == 0 || j == op.ny - 1) m_bxy |= 1u << r;
+    if (i == 1 || i == op.nx - 2 || j == 1 || j == op.ny - 2) m_fxy |= 1u << r;
   }
   const int jrow0 = ty * TY - 1 + wv * R;
 
-  auto ld_x = [&](int r, int k) -> T {
-    const bool ok = ((m_in >> r) & 1u) && k >= 0 && k < nz;
-    const T v = T(ldS(x, ok ? k * sxy + col[r] : 0));
-    return ok ? v : T(0);
-  };
+  auto kcl = [&](int k) -> long long { return (long long)min(max(k, 0), nz - 1) * sxy; };
+  auto ld_x = [&](int r, int k) -> T { return T(ldS(x, kcl(k) + col[r])); };
   auto ld_k = [&](int r, int k) -> T {
-    if constexpr (HK) {
-      const bool ok = ((m_in >> r) & 1u) && k >= 0 && k < nz;
-      const T v = kap[ok ? k * sxy + col[r] : 0];
-      return ok ? v : T(0);
-    } else {
-      return T(0);
-    }
+    if constexpr (HK) return kap[kcl(k) + col[r]];
+    else return T(0);
   };
   auto ld_b = [&](int r, int k) -> T {
-    if constexpr (MODE == 1) {
-      const bool ok = ((m_out >> r) & 1u) && k >= 0 && k < nz;
-      const T v = T(ldS(b, ok ? k * sxy + col[r] : 0));
-      return ok ? v : T(0);
-    } else {
-      return T(0);
-    }
+    if constexpr (MODE == 1) return T(ldS(b, kcl(k) + col[r]));
+    else return T(0);
   };
 
   // queues after the shift of step p: X = x(p-1..p+1), XN = x(p+2) (prefetch),
@@ -196,7 +184,7 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
             sum += w.B + w.F;
             w.C = T(op.e) - sum;
           }
-          fold_center<3, T>(op, i, j, p, w);
+          if (((m_fxy >> r) & 1u) || p == 1 || p == nz - 2) fold_center<3, T>(op, i, j, p, w);
           const T ax = w.C * xc + w.E * lx[sl][lr][lane + 1] + w.W * lx[sl][lr][lane - 1] +
                        w.N * lx[sl][lr + 1][lane] + w.S * lx[sl][lr - 1][lane] + w.F * X[r][2] +
                        w.B * X[r][0];

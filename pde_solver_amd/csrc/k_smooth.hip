@@ -99,16 +99,21 @@ __device__ __forceinline__ int mod3(int p) { return ((p % 3) + 3) % 3; }
 
 // ------------------------------------------------------------------------------- 3D
 // Plane step p (p runs from the prologue start to k1):
-//   1. queues advance; loads for x(p+1), b(p+1), kappa(p+3) are issued (consumed one or more
-//      steps later, so their latency overlaps the step's arithmetic);
+//   1. queues advance; loads for x(p+2), b(p+2), kappa(p+4) are issued (consumed two steps
+//      later, so their latency overlaps two steps of arithmetic);
 //   2. x(p) and kappa(p+1) go to LDS; barrier;
 //   3. FLUX: every node of the extended tile computes its east, north and front faces of plane
-//      p+1 (face ring slot (p+1) mod 3 in LDS; front faces in registers) — each face once;
-//   4. first colour on plane p (tile + ring): faces of slot p mod 3, z faces from registers;
-//   5. second colour on plane p-1 (tile): faces of slot (p-1) mod 3; plane p-1 of the tile is
-//      stored; barrier.
-// Loads are issued unconditionally (out-of-grid lanes read element 0 and discard it), so the
-// compiler can count outstanding loads instead of draining them at every branch.
+//      p+1 (LDS face ring; front faces in registers) — each face once;
+//   4. first colour on plane p (tile + ring): faces of plane p, z faces from registers;
+//   5. second colour on plane p-1 (tile): faces of plane p-1; plane p-1 of the tile is stored.
+// ONEBAR (4-byte types): LDS slots x3 for x, x2 for kappa, x4 for faces, so every slot is
+// rewritten only after a barrier that all of its readers have passed — one barrier per plane.
+// Otherwise (FP64, where those slots would not fit next to enough workgroups): x2 / x1 / x3
+// slots and a second barrier at the end of the step.
+// Loads are issued unconditionally (out-of-grid lanes and planes read a clamped in-grid node),
+// so the compiler counts outstanding loads instead of draining them at every branch.
+__device__ __forceinline__ int modn(int p, int n) { return ((p % n) + n) % n; }
+
 template <int KIND, Prec P, int R, bool ZERO>
 __global__ void __launch_bounds__(kThreads)
 k_rbgs_fused3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v,
@@ -119,11 +124,13 @@ k_rbgs_fused3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xi
   constexpr int TYE = R * kWaves;
   constexpr int TY = TYE - 2 * kHalo;
   constexpr bool HK = KIND == PDEHIP_OP_FLUX;
+  constexpr bool ONEBAR = sizeof(T) == 4;
+  constexpr int NU = ONEBAR ? 3 : 2, NK = ONEBAR ? 2 : 1, NF = ONEBAR ? 4 : 3;
   constexpr int KR = HK ? TYE : 1;
-  __shared__ T lu[2][TYE][kLanes];
-  __shared__ T lk[KR][kLanes];
-  __shared__ T lfe[HK ? 3 : 1][KR][kLanes];
-  __shared__ T lfn[HK ? 3 : 1][KR][kLanes];
+  __shared__ T lu[NU][TYE][kLanes];
+  __shared__ T lk[HK ? NK : 1][KR][kLanes];
+  __shared__ T lfe[HK ? NF : 1][KR][kLanes];
+  __shared__ T lfn[HK ? NF : 1][KR][kLanes];
   if (skip(sc)) return;
   const int t = xcd_tile();
   if (t >= ntiles) return;
@@ -150,10 +157,12 @@ k_rbgs_fused3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xi
   const bool ring_x = lane >= 1 && lane <= kLanes - 2;
   const bool tile_x = lane >= kHalo && lane < kLanes - kHalo;
 
-  // per-row flags (bit r): in grid, first-colour candidate (ring & interior), second-colour
-  // candidate (tile & interior), output (tile & in grid)
-  unsigned m_in = 0, m_r1 = 0, m_r2 = 0, m_out = 0;
+  // per-row flags (bit r): first-colour candidate (ring & interior), second-colour candidate
+  // (tile & interior), output (tile & in grid), next to a face (rows that may fold).  Lanes and
+  // rows outside the grid load a clamped in-grid node: no update ever reads their values.
+  unsigned m_r1 = 0, m_r2 = 0, m_out = 0, m_fxy = 0;
   long long col[R];
+  const int ci = min(max(i, 0), op.nx - 1);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int lr = wv * R + r;
@@ -162,84 +171,79 @@ k_rbgs_fused3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xi
     const bool int_y = j >= 1 && j <= op.ny - 2;
     const bool ring_y = lr >= 1 && lr <= TYE - 2;
     const bool tile_y = lr >= kHalo && lr < TYE - kHalo;
-    col[r] = in_x && in_y ? (long long)j * op.nx + i : 0;
-    if (in_x && in_y) m_in |= 1u << r;
+    col[r] = (long long)min(max(j, 0), op.ny - 1) * op.nx + ci;
     if (ring_x && ring_y && int_x && int_y) m_r1 |= 1u << r;
     if (tile_x && tile_y && int_x && int_y) m_r2 |= 1u << r;
     if (tile_x && tile_y && in_x && in_y) m_out |= 1u << r;
+    if (i == 1 || i == op.nx - 2 || j == 1 || j == op.ny - 2) m_fxy |= 1u << r;
   }
   const int jrow0 = ty * TY - kHalo + wv * R;
 
+  // plane index clamped into the grid (uniform across the workgroup)
+  auto kcl = [&](int k) -> long long { return (long long)min(max(k, 0), nz - 1) * sxy; };
   auto ld_u = [&](int r, int k) -> T {
-    if constexpr (ZERO) {
-      return T(0);
-    } else {
-      const bool ok = ((m_in >> r) & 1u) && k >= 0 && k < nz;
-      const T v = T(ldS(xin, ok ? k * sxy + col[r] : 0));
-      return ok ? v : T(0);
-    }
+    if constexpr (ZERO) return T(0);
+    else return T(ldS(xin, kcl(k) + col[r]));
   };
   auto ld_k = [&](int r, int k) -> T {
-    if constexpr (HK) {
-      const bool ok = ((m_in >> r) & 1u) && k >= 0 && k < nz;
-      const T v = kap[ok ? k * sxy + col[r] : 0];
-      return ok ? v : T(0);
-    } else {
-      return T(0);
-    }
+    if constexpr (HK) return kap[kcl(k) + col[r]];
+    else return T(0);
   };
-  auto ld_b = [&](int r, int k) -> T {
-    const bool ok = ((m_r1 >> r) & 1u) && k >= 1 && k < nz - 1;
-    const T v = T(ldS(b, ok ? k * sxy + col[r] : 0));
-    return ok ? v : T(0);
-  };
+  auto ld_b = [&](int r, int k) -> T { return T(ldS(b, kcl(k) + col[r])); };
 
-  // queues (after the shift of step p): U = x(p-2..p+1), B = b(p-1..p+1), K = kappa(p+1..p+3),
+  // queues after the shift of step p: U = x(p-2..p+2), B = b(p-1..p+2), K = kappa(p+1..p+4),
   // ZF = front faces zf(p-2..p+1) with zf(k) = face(kappa(k), kappa(k+1))
   const int pstart = HK ? k0 - 3 : k0 - 1;
-  T U[R][4], B[R][3], K[R][3], ZF[R][4];
+  T U[R][5], B[R][4], K[R][4], ZF[R][4];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    U[r][0] = U[r][1] = T(0);
+    U[r][0] = T(0);
+    U[r][1] = ld_u(r, pstart - 2);
     U[r][2] = ld_u(r, pstart - 1);
     U[r][3] = ld_u(r, pstart);
-    B[r][0] = B[r][1] = T(0);
+    U[r][4] = ld_u(r, pstart + 1);
+    B[r][0] = T(0);
+    B[r][1] = ld_b(r, pstart - 1);
     B[r][2] = ld_b(r, pstart);
+    B[r][3] = ld_b(r, pstart + 1);
     K[r][0] = T(0);
     K[r][1] = ld_k(r, pstart + 1);
     K[r][2] = ld_k(r, pstart + 2);
+    K[r][3] = ld_k(r, pstart + 3);
     ZF[r][0] = ZF[r][1] = ZF[r][2] = ZF[r][3] = T(0);
   }
 
   for (int p = pstart; p <= k1; ++p) {
-    const int sl = p & 1;
+    const int su = modn(p, NU);
+    const int sk = modn(p, NK);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int lr = wv * R + r;
-      U[r][0] = U[r][1]; U[r][1] = U[r][2]; U[r][2] = U[r][3]; U[r][3] = ld_u(r, p + 1);
-      B[r][0] = B[r][1]; B[r][1] = B[r][2]; B[r][2] = ld_b(r, p + 1);
+      U[r][0] = U[r][1]; U[r][1] = U[r][2]; U[r][2] = U[r][3]; U[r][3] = U[r][4];
+      U[r][4] = ld_u(r, p + 2);
+      B[r][0] = B[r][1]; B[r][1] = B[r][2]; B[r][2] = B[r][3]; B[r][3] = ld_b(r, p + 2);
       if constexpr (HK) {
-        K[r][0] = K[r][1]; K[r][1] = K[r][2]; K[r][2] = ld_k(r, p + 3);
+        K[r][0] = K[r][1]; K[r][1] = K[r][2]; K[r][2] = K[r][3]; K[r][3] = ld_k(r, p + 4);
         ZF[r][0] = ZF[r][1]; ZF[r][1] = ZF[r][2]; ZF[r][2] = ZF[r][3];
-        lk[lr][lane] = K[r][0];
+        lk[sk][lr][lane] = K[r][0];
       }
-      lu[sl][lr][lane] = U[r][2];
+      lu[su][lr][lane] = U[r][2];
     }
     __syncthreads();
     if constexpr (HK) {
-      const int fs = mod3(p + 1);
+      const int fs = modn(p + 1, NF);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int lr = wv * R + r;
         const T kc = K[r][0];
         ZF[r][3] = face_smooth<T>(kc, K[r][1], fz);
-        lfe[fs][lr][lane] = lane < kLanes - 1 ? face_smooth<T>(kc, lk[lr][lane + 1], fx) : T(0);
-        lfn[fs][lr][lane] = lr < TYE - 1 ? face_smooth<T>(kc, lk[lr + 1][lane], fy) : T(0);
+        lfe[fs][lr][lane] = lane < kLanes - 1 ? face_smooth<T>(kc, lk[sk][lr][lane + 1], fx) : T(0);
+        lfn[fs][lr][lane] = lr < TYE - 1 ? face_smooth<T>(kc, lk[sk][lr + 1][lane], fy) : T(0);
       }
     }
     // first colour on plane p (tile + ring)
     if (p >= k0 - 1 && p >= 1 && p <= nz - 2) {
-      const int f0 = HK ? mod3(p) : 0;
+      const int f0 = HK ? modn(p, NF) : 0;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int j = jrow0 + r;
@@ -252,21 +256,21 @@ k_rbgs_fused3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xi
             as = lfn[f0][lr - 1][lane]; an = lfn[f0][lr][lane];
           }
           W7<T> w = weights_at<3, KIND, T>(op, id, aw, ae, as, an, ZF[r][1], ZF[r][2]);
-          fold_center<3, T>(op, i, j, p, w);
-          const T v = rounded<S, T>(gs_value<3, T>(w, B[r][1], lu[sl][lr][lane + 1],
-                                                   lu[sl][lr][lane - 1], lu[sl][lr + 1][lane],
-                                                   lu[sl][lr - 1][lane], U[r][3], U[r][1]));
+          if (((m_fxy >> r) & 1u) || p == 1 || p == nz - 2) fold_center<3, T>(op, i, j, p, w);
+          const T v = rounded<S, T>(gs_value<3, T>(w, B[r][1], lu[su][lr][lane + 1],
+                                                   lu[su][lr][lane - 1], lu[su][lr + 1][lane],
+                                                   lu[su][lr - 1][lane], U[r][3], U[r][1]));
           U[r][2] = v;
-          lu[sl][lr][lane] = v;
+          lu[su][lr][lane] = v;
         }
       }
     }
     // second colour on plane q = p - 1 (tile), then the tile's plane q goes out
     const int q = p - 1;
     if (q >= k0 && q < k1) {
-      const int sq = sl ^ 1;
+      const int sq = modn(q, NU);
       if (q >= 1 && q <= nz - 2) {
-        const int f1 = HK ? mod3(q) : 0;
+        const int f1 = HK ? modn(q, NF) : 0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int j = jrow0 + r;
@@ -279,7 +283,7 @@ k_rbgs_fused3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xi
               as = lfn[f1][lr - 1][lane]; an = lfn[f1][lr][lane];
             }
             W7<T> w = weights_at<3, KIND, T>(op, id, aw, ae, as, an, ZF[r][0], ZF[r][1]);
-            fold_center<3, T>(op, i, j, q, w);
+            if (((m_fxy >> r) & 1u) || q == 1 || q == nz - 2) fold_center<3, T>(op, i, j, q, w);
             U[r][1] = rounded<S, T>(gs_value<3, T>(w, B[r][0], lu[sq][lr][lane + 1],
                                                    lu[sq][lr][lane - 1], lu[sq][lr + 1][lane],
                                                    lu[sq][lr - 1][lane], U[r][2], U[r][0]));
@@ -290,7 +294,7 @@ k_rbgs_fused3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xi
       for (int r = 0; r < R; ++r)
         if ((m_out >> r) & 1u) xout[q * sxy + col[r]] = to_store<S, T>(U[r][1]);
     }
-    __syncthreads();
+    if constexpr (!ONEBAR) __syncthreads();
   }
 }
 

@@ -113,6 +113,8 @@ void restrict_kappa(hipStream_t s, const DevOp& fine, const double* kf, const De
 // ---- vectors and scalars
 void fill(hipStream_t s, Prec p, void* x, long long n, double v);
 void convert(hipStream_t s, Prec pin, const void* in, Prec pout, void* out, long long n, double scale);
+// out = 1/in in FP32 (FP32/FP16 operators keep FLUX conductivities as rho = 1/kappa).
+void reciprocal_f32(hipStream_t s, const double* in, float* out, long long n);
 // partials of a.b (or a.a when b == a)
 void dot(const Launch& L, Prec p, const void* a, const void* b, long long n, double* partials);
 int vec_blocks(long long n);

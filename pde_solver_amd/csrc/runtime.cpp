@@ -252,8 +252,8 @@ static void build_level_ops(Plan& P, Level& lv, const double* kappa64_dev, bool 
     for (double& wv : lv.op32.w) wv *= s;
     lv.op32.fx *= s; lv.op32.fy *= s; lv.op32.fz *= s; lv.op32.e *= s;
     if (p.op_kind == PDEHIP_OP_FLUX) {
-      lv.kap32.alloc(P.ctx, lv.g.n * sizeof(float));
-      convert(P.stream(), P64, kappa64_dev, P32, lv.kap32.get(), lv.g.n, 1.0);
+      lv.kap32.alloc(P.ctx, lv.g.n * sizeof(float));   // rho = 1/kappa (see face_coef)
+      reciprocal_f32(P.stream(), kappa64_dev, lv.kap32.as<float>(), lv.g.n);
       lv.op32.kappa = lv.kap32.get();
     }
   }
