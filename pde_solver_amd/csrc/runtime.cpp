@@ -1325,17 +1325,31 @@ double time_kernel(Plan& P, int kernel, int precision, int reps, double* bytes) 
   if (p32 && P.prob.op_kind == PDEHIP_OP_FLUX && !P.L[0].kap32)
     throw Error(PDEHIP_ERR_INVALID, "plan holds no FP32 operator");
   const size_t kb = P.prob.op_kind == PDEHIP_OP_FLUX ? kappa_bytes(pr) : 0;
-  DBuf a(P.ctx, n * es), b(P.ctx, n * es), c(P.ctx, n * es);
-  fill(s, pr, a.get(), n, 1.0);
-  fill(s, pr, b.get(), n, 0.5);
-  fill(s, pr, c.get(), n, 0.0);
+  // The plan's own level-0 work vectors when it has them (MG-PCG plans): the timed launches
+  // then touch the same buffers, and the same cache footprint, as the solver's launches.
+  // Their contents are scratch between solves.  Otherwise three temporary vectors.
+  DBuf ta, tb, tc;
+  void *a = nullptr, *b = nullptr, *c = nullptr, *alt = nullptr;
+  if (P.opt.method == PDEHIP_MGPCG && (p32 == P.mixed)) {
+    a = p32 ? P.z32.get() : P.z.get();      // V-cycle correction (smoother x)
+    c = p32 ? P.r32.get() : P.r.get();      // V-cycle right-hand side
+    b = P.L[0].r.get();                     // level-0 residual
+    alt = P.L[0].alt.get();
+  }
+  if (!a || !b || !c) {
+    ta.alloc(P.ctx, n * es); tb.alloc(P.ctx, n * es); tc.alloc(P.ctx, n * es);
+    a = ta.get(); b = tb.get(); c = tc.get();
+    fill(s, pr, c, n, 0.0);
+  }
+  if (!alt) alt = b;
+  fill(s, pr, a, n, 1.0);
   auto once = [&]() {
     switch (kernel) {
-      case 0: apply(L, op, pr, a.get(), b.get(), nullptr); break;
-      case 1: rbgs(L, op, pr, c.get(), a.get(), 0); rbgs(L, op, pr, c.get(), a.get(), 1); break;
-      case 2: residual(L, op, pr, c.get(), a.get(), b.get(), nullptr); break;
-      case 4: rbgs_sweep(L, op, pr, c.get(), a.get(), b.get(), 0, false); break;
-      default: convert(s, pr, a.get(), pr, b.get(), n, 1.0); break;
+      case 0: apply(L, op, pr, a, b, nullptr); break;
+      case 1: rbgs(L, op, pr, c, a, 0); rbgs(L, op, pr, c, a, 1); break;
+      case 2: residual(L, op, pr, c, a, b, nullptr); break;
+      case 4: rbgs_sweep(L, op, pr, c, a, alt, 0, false); break;
+      default: convert(s, pr, a, pr, b, n, 1.0); break;
     }
   };
   switch (kernel) {
