@@ -265,6 +265,19 @@ typedef struct pdehip_batch_config {
 int pdehip_solve_batch(pdehip_ctx* ctx, int count, const pdehip_problem* probs,
                        const pdehip_options* opt, const pdehip_batch_config* cfg,
                        double* const* u_out, pdehip_report* reports);
+/* Resident batch: operators, hierarchies and Krylov state of `count` problems kept in HBM and
+ * solved together in lock step (2D constant-coefficient MG-PCG, FP64 or mixed precision): each
+ * operation of an iteration is one launch over every unfinished problem.  No reference
+ * counterpart (the reference runs a batch as sequential solves, src/main.cpp:2071-2097).
+ * pdehip_batch_solve fills count reports; download copies problem `index`'s solution. */
+typedef struct pdehip_batch pdehip_batch;
+int pdehip_batch_create(pdehip_ctx* ctx, int count, const pdehip_problem* probs,
+                        const pdehip_options* opt, const pdehip_batch_config* cfg,
+                        pdehip_batch** out);
+int pdehip_batch_solve(pdehip_batch* batch, pdehip_report* reports);
+int pdehip_batch_download(pdehip_batch* batch, int index, double* u_out);
+size_t pdehip_batch_bytes(const pdehip_batch* batch);
+void pdehip_batch_destroy(pdehip_batch* batch);
 
 /* ---------------------------------------------------------------- serving */
 typedef struct pdehip_server pdehip_server;

@@ -1,0 +1,366 @@
+// batch.cpp — host side of the batched MG-PCG engine (see batch.hpp).
+//
+// Set-up builds one resident Plan per problem (the same operators, folded boundary rows,
+// hierarchy and dense coarsest inverse a single solve would use), then the device tables and
+// the work-item lists:
+//   * per level l: smoother strips (problem, 60-column strip, row range), residual/prolongation
+//     row blocks, restriction coarse-row blocks (only problems whose hierarchy continues below
+//     l) and dense-solve rows (problems whose coarsest level is l);
+//   * level 0: element blocks for the vector kernels and row blocks for the operator apply,
+//     each with per-problem [first, count) ranges for the segmented reductions.
+// The solve runs the PCG (FP64) or the FP64 refinement around FP32/FP16 MG-PCG (mixed) of
+// every problem in lock step; converged problems drop out item by item on the device.  The
+// host reads one counter of unfinished problems every few iterations.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace pdehip {
+
+namespace {
+
+struct List {
+  DBuf items;
+  int n = 0;
+  DBuf first, cnt;   // per problem (reduction lists only)
+  const BItem* d() const { return items.as<BItem>(); }
+};
+
+void upload_list(Ctx* ctx, hipStream_t s, List& L, const std::vector<BItem>& v,
+                 const std::vector<int>* first = nullptr, const std::vector<int>* cnt = nullptr) {
+  L.n = static_cast<int>(v.size());
+  if (!v.empty()) {
+    L.items.alloc(ctx, v.size() * sizeof(BItem));
+    HIPCHK(hipMemcpyAsync(L.items.get(), v.data(), v.size() * sizeof(BItem), hipMemcpyHostToDevice, s));
+  }
+  if (first) {
+    L.first.alloc(ctx, first->size() * sizeof(int));
+    L.cnt.alloc(ctx, cnt->size() * sizeof(int));
+    HIPCHK(hipMemcpyAsync(L.first.get(), first->data(), first->size() * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(L.cnt.get(), cnt->data(), cnt->size() * sizeof(int), hipMemcpyHostToDevice, s));
+  }
+}
+
+}  // namespace
+
+struct BatchPlan {
+  Ctx* ctx = nullptr;
+  int n = 0;
+  pdehip_options opt{};
+  pdehip_batch_config cfg{};
+  bool mixed = false;
+  int L = 0;                               // deepest hierarchy
+  std::vector<std::unique_ptr<Plan>> plans;
+  std::vector<int> levels;                 // per problem
+  DBuf tab;                                // BEnt [L][n]
+  DBuf vecs;                               // BVec [n]
+  DBuf dense;                              // BDense [n]
+  std::vector<List> sweep, rows, crows, drows;   // per level
+  List el0, rows0;                         // level-0 element / row items (with ranges)
+  DBuf part, maxp;
+  DBuf active;
+  int* hactive = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  size_t bytes = 0;                        // device memory held by the batch and its plans
+  ~BatchPlan() {
+    if (hactive) (void)hipHostFree(hactive);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+  const BEnt* T(int l) const { return tab.as<BEnt>() + (size_t)l * n; }
+  const BVec* V() const { return vecs.as<BVec>(); }
+  Prec prec(int l) const { return mixed ? (l == 0 ? P32 : P16) : P64; }
+};
+
+bool batch_eligible(int count, const pdehip_problem* probs, const pdehip_options& opt,
+                    std::string* why) {
+  auto no = [&](const char* r) { if (why) *why = r; return false; };
+  if (count < 1) return no("empty batch");
+  if (opt.method != PDEHIP_MGPCG) return no("batched engine runs MG-PCG");
+  if (opt.mg_pre < 2 || opt.mg_pre % 2 || opt.mg_post < 2 || opt.mg_post % 2)
+    return no("batched engine needs even pre/post smoothing counts");
+  for (int i = 0; i < count; ++i) {
+    const pdehip_problem& p = probs[i];
+    if (p.nz > 1) return no("batched engine is 2D");
+    if (p.op_kind != PDEHIP_OP_CONST) return no("batched engine needs constant coefficients");
+    if (p.c != 0.0 || p.d != 0.0) return no("batched engine needs no convection");
+    if (p.n_nonlinear > 0) return no("batched engine needs linear problems");
+  }
+  return true;
+}
+
+BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pdehip_options& opt,
+                      const pdehip_batch_config* cfg) {
+  std::string why;
+  if (!batch_eligible(count, probs, opt, &why)) throw Error(PDEHIP_ERR_UNSUPPORTED, why);
+  BatchPtr B(new BatchPlan());
+  B->ctx = ctx;
+  B->n = count;
+  B->opt = opt;
+  if (cfg) B->cfg = *cfg;
+  B->mixed = opt.precision == PDEHIP_MIXED;
+  const hipStream_t s = ctx->stream;
+  const size_t used0 = ctx->used.load();
+  B->plans.reserve(count);
+  for (int i = 0; i < count; ++i) {
+    B->plans.push_back(plan_create(ctx, probs[i], opt));
+    Plan& P = *B->plans.back();
+    const int nl = static_cast<int>(P.L.size());
+    if (nl < 2 || P.L[nl - 1].m <= 0)
+      throw Error(PDEHIP_ERR_UNSUPPORTED, "batched engine needs a dense coarsest level");
+    B->levels.push_back(nl);
+    B->L = std::max(B->L, nl);
+  }
+  const int n = count, L = B->L;
+
+  // ---- device tables
+  std::vector<BEnt> tab((size_t)L * n);
+  std::vector<BVec> vecs(n);
+  std::vector<bk::BDense> dense(n);
+  for (int p = 0; p < n; ++p) {
+    Plan& P = *B->plans[p];
+    const int nl = B->levels[p];
+    KScalars* sc_inner = B->mixed ? P.sc(1) : P.sc(0);
+    for (int l = 0; l < nl; ++l) {
+      Level& lv = P.L[l];
+      BEnt& e = tab[(size_t)l * n + p];
+      std::memset(&e, 0, sizeof(e));
+      e.op = B->mixed ? lv.op32 : lv.op64;
+      if (l == 0) {
+        e.u = B->mixed ? P.z32.get() : P.z.get();
+        e.b = B->mixed ? P.r32.get() : P.r.get();
+      } else {
+        e.u = lv.u.get();
+        e.b = lv.b.get();
+      }
+      e.r = lv.r.get();
+      e.alt = lv.alt.get();
+      e.sc = sc_inner;
+      e.pfac = (l + 1 < nl) ? P.L[l + 1].sigma / lv.sigma : 1.0;
+    }
+    BVec& v = vecs[p];
+    std::memset(&v, 0, sizeof(v));
+    v.x = B->mixed ? P.x32.get() : P.x.get();
+    v.r = B->mixed ? P.r32.get() : P.r.get();
+    v.p = B->mixed ? P.p32.get() : P.p.get();
+    v.q = B->mixed ? P.q32.get() : P.q.get();
+    v.z = B->mixed ? P.z32.get() : P.z.get();
+    v.x64 = P.x.as<double>();
+    v.r64 = P.r.as<double>();
+    v.b64 = P.bvec.as<double>();
+    v.x0 = P.x0.as<double>();
+    v.so = P.sc(0);
+    v.si = P.sc(1);
+    v.n = P.g.n;
+    v.nx = P.g.nx;
+    v.ny = P.g.ny;
+    v.op64 = P.L[0].op64;
+    Level& cl = P.L[nl - 1];
+    dense[p] = bk::BDense{cl.ainv.as<double>(), cl.map.as<int>(), cl.m, 0};
+  }
+  B->tab.alloc(ctx, tab.size() * sizeof(BEnt));
+  B->vecs.alloc(ctx, vecs.size() * sizeof(BVec));
+  B->dense.alloc(ctx, dense.size() * sizeof(bk::BDense));
+  HIPCHK(hipMemcpyAsync(B->tab.get(), tab.data(), tab.size() * sizeof(BEnt), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(B->vecs.get(), vecs.data(), vecs.size() * sizeof(BVec), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(B->dense.get(), dense.data(), dense.size() * sizeof(bk::BDense), hipMemcpyHostToDevice, s));
+
+  // ---- work items
+  const int strip = 60;
+  const int ychunk = B->cfg.tile_y > 0 ? B->cfg.tile_y : 64;
+  B->sweep.resize(L); B->rows.resize(L); B->crows.resize(L); B->drows.resize(L);
+  for (int l = 0; l < L; ++l) {
+    std::vector<BItem> sw, rw, cw, dw;
+    for (int p = 0; p < n; ++p) {
+      const int nl = B->levels[p];
+      if (l >= nl) continue;
+      const Geo& g = B->plans[p]->L[l].g;
+      if (l == nl - 1) {
+        for (int r = 0; r < dense[p].m; ++r) dw.push_back(BItem{p, r, 0, 0});
+        continue;
+      }
+      for (int sx = 0; sx * strip < g.nx; ++sx)
+        for (int j0 = 0; j0 < g.ny; j0 += ychunk) sw.push_back(BItem{p, sx, j0, std::min(g.ny, j0 + ychunk)});
+      const int rc = std::max(1, 4096 / g.nx);
+      for (int j0 = 0; j0 < g.ny; j0 += rc) rw.push_back(BItem{p, j0, std::min(g.ny, j0 + rc), 0});
+      const Geo& gc = B->plans[p]->L[l + 1].g;
+      const int rcc = std::max(1, 4096 / gc.nx);
+      for (int j0 = 0; j0 < gc.ny; j0 += rcc) cw.push_back(BItem{p, j0, std::min(gc.ny, j0 + rcc), 0});
+    }
+    upload_list(ctx, s, B->sweep[l], sw);
+    upload_list(ctx, s, B->rows[l], rw);
+    if (l + 1 < L) upload_list(ctx, s, B->crows[l + 1], cw);
+    upload_list(ctx, s, B->drows[l], dw);
+  }
+  {
+    std::vector<BItem> el, rw;
+    std::vector<int> ef(n), ec(n), rf(n), rcn(n);
+    for (int p = 0; p < n; ++p) {
+      const Geo& g = B->plans[p]->g;
+      ef[p] = static_cast<int>(el.size());
+      for (long long e0 = 0; e0 < g.n; e0 += 4096)
+        el.push_back(BItem{p, static_cast<int>(e0), static_cast<int>(std::min(g.n, e0 + 4096)), 0});
+      ec[p] = static_cast<int>(el.size()) - ef[p];
+      rf[p] = static_cast<int>(rw.size());
+      const int rc = std::max(1, 4096 / g.nx);
+      for (int j0 = 0; j0 < g.ny; j0 += rc) rw.push_back(BItem{p, j0, std::min(g.ny, j0 + rc), 0});
+      rcn[p] = static_cast<int>(rw.size()) - rf[p];
+    }
+    upload_list(ctx, s, B->el0, el, &ef, &ec);
+    upload_list(ctx, s, B->rows0, rw, &rf, &rcn);
+  }
+  const int np = std::max(B->el0.n, B->rows0.n);
+  B->part.alloc(ctx, (size_t)np * sizeof(double));
+  B->maxp.alloc(ctx, (size_t)np * sizeof(double));
+  B->active.alloc(ctx, sizeof(int));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&B->hactive), sizeof(int)));
+  HIPCHK(hipEventCreate(&B->ev0));
+  HIPCHK(hipEventCreate(&B->ev1));
+  HIPCHK(hipStreamSynchronize(s));
+  B->bytes = ctx->used.load() - used0;
+  return B;
+}
+
+namespace {
+
+// One V-cycle of every unfinished problem: level-0 input tab[0].b, output tab[0].u.
+void vcycle(BatchPlan& B) {
+  const hipStream_t s = B.ctx->stream;
+  const int L = B.L;
+  const int pre = std::max(2, B.opt.mg_pre), post = std::max(2, B.opt.mg_post);
+  for (int l = 0; l < L; ++l) {
+    bk::coarse_dense(s, B.prec(l), B.T(l), B.dense.as<bk::BDense>(), B.drows[l].d(), B.drows[l].n);
+    const List& sw = B.sweep[l];
+    if (sw.n == 0) continue;
+    for (int it = 0; it < pre; it += 2) {
+      bk::sweep(s, B.prec(l), B.T(l), sw.d(), sw.n, 0, 0, it == 0);
+      bk::sweep(s, B.prec(l), B.T(l), sw.d(), sw.n, 0, 1, false);
+    }
+    bk::residual(s, B.prec(l), B.T(l), B.rows[l].d(), B.rows[l].n);
+    bk::restrict_(s, B.prec(l), B.prec(l + 1), B.T(l), B.T(l + 1), B.crows[l + 1].d(),
+                  B.crows[l + 1].n, B.mixed && l == 0);
+  }
+  for (int l = L - 2; l >= 0; --l) {
+    const List& sw = B.sweep[l];
+    if (sw.n == 0) continue;
+    bk::prolong(s, B.prec(l + 1), B.prec(l), B.T(l + 1), B.T(l), B.rows[l].d(), B.rows[l].n,
+                B.mixed && l == 0);
+    for (int it = 0; it < post; it += 2) {
+      bk::sweep(s, B.prec(l), B.T(l), sw.d(), sw.n, 1, 0, false);
+      bk::sweep(s, B.prec(l), B.T(l), sw.d(), sw.n, 1, 1, false);
+    }
+  }
+}
+
+int active(BatchPlan& B, int inner) {
+  const hipStream_t s = B.ctx->stream;
+  bk::active_count(s, B.V(), B.n, inner, B.active.as<int>());
+  HIPCHK(hipMemcpyAsync(B.hactive, B.active.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return *B.hactive;
+}
+
+// PCG on every problem in lock step.  Krylov precision kp (P64: the FP64 path on the outer
+// scalars; P32: the inner solve of a refinement on the inner scalars).
+void pcg(BatchPlan& B, Prec kp, int max_iter) {
+  const hipStream_t s = B.ctx->stream;
+  const int inner = kp == P32 ? 1 : 0;
+  double* part = B.part.as<double>();
+  double* maxp = B.mixed && inner ? B.maxp.as<double>() : nullptr;
+  const List& el = B.el0;
+  const List& rw = B.rows0;
+  auto fin = [&](const List& Lst, int op, double* mp) {
+    bk::finalize(s, B.V(), Lst.first.as<int>(), Lst.cnt.as<int>(), B.n, part, mp, op, inner, 0.0, 0);
+  };
+  vcycle(B);
+  bk::vec(s, kp, bk::V_DOT_RZ, B.V(), el.d(), el.n, part, nullptr);
+  fin(el, FIN_CG_INIT, nullptr);
+  bk::vec(s, kp, bk::V_COPY_ZP, B.V(), el.d(), el.n, nullptr, nullptr);
+  const int every = 2;
+  for (int it = 0; it < max_iter; ++it) {
+    bk::apply_dot_tab(s, kp, B.V(), B.T(0), rw.d(), rw.n, part, false);
+    fin(rw, FIN_CG_ALPHA, nullptr);
+    bk::vec(s, kp, bk::V_CG_UPDATE, B.V(), el.d(), el.n, part, maxp);
+    fin(el, FIN_CG_RR, maxp);
+    vcycle(B);
+    bk::vec(s, kp, bk::V_DOT_RZ, B.V(), el.d(), el.n, part, nullptr);
+    fin(el, FIN_CG_BETA, nullptr);
+    bk::vec(s, kp, bk::V_XPAY, B.V(), el.d(), el.n, nullptr, nullptr);
+    if ((it + 1) % every == 0 && active(B, inner) == 0) break;
+  }
+}
+
+}  // namespace
+
+void batch_solve(BatchPlan& B, pdehip_report* reports) {
+  const hipStream_t s = B.ctx->stream;
+  const int n = B.n;
+  double* part = B.part.as<double>();
+  const List& rw = B.rows0;
+  const List& el = B.el0;
+  const int max_iter = std::max(1, B.opt.max_iter);
+  bk::init_scalars(s, B.V(), n, 1e-300);
+  HIPCHK(hipEventRecord(B.ev0, s));
+  bk::vec(s, P64, bk::V_COPY_X0, B.V(), el.d(), el.n, nullptr, nullptr);
+  bk::apply_dot_tab(s, P64, B.V(), B.T(0), rw.d(), rw.n, part, true);
+  bk::finalize(s, B.V(), rw.first.as<int>(), rw.cnt.as<int>(), n, part, nullptr, FIN_R0, 0,
+               B.opt.tol, B.opt.tol_mode);
+  if (!B.mixed) {
+    pcg(B, P64, max_iter);
+  } else {
+    const double inner_tol = B.opt.inner_tol > 0 ? B.opt.inner_tol : 1e-4;
+    for (int ref = 0; ref < 30; ++ref) {
+      if (active(B, 0) == 0) break;
+      bk::reset_inner(s, B.V(), n);
+      bk::vec(s, P32, bk::V_TO32, B.V(), el.d(), el.n, nullptr, nullptr);
+      bk::vec(s, P32, bk::V_RR, B.V(), el.d(), el.n, part, B.maxp.as<double>());
+      bk::finalize(s, B.V(), el.first.as<int>(), el.cnt.as<int>(), n, part, B.maxp.as<double>(),
+                   FIN_R0, 1, inner_tol, PDEHIP_TOL_REL);
+      pcg(B, P32, max_iter);
+      bk::vec(s, P32, bk::V_ADD32, B.V(), el.d(), el.n, nullptr, nullptr);
+      bk::apply_dot_tab(s, P64, B.V(), B.T(0), rw.d(), rw.n, part, true);
+      bk::finalize(s, B.V(), rw.first.as<int>(), rw.cnt.as<int>(), n, part, nullptr, FIN_CG_RR,
+                   0, 0.0, 0);
+    }
+    bk::reset_inner(s, B.V(), n);   // folds the last refinement's inner iterations in
+  }
+  HIPCHK(hipEventRecord(B.ev1, s));
+  for (auto& P : B.plans) plan_finish_enqueue(*P);
+  HIPCHK(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, B.ev0, B.ev1));
+  for (int p = 0; p < n; ++p) {
+    Plan& P = *B.plans[p];
+    pdehip_report rep{};
+    plan_finish_collect(P, &rep);
+    const KScalars& k = P.hsc[0];
+    rep.solve_ms = ms;
+    rep.converged = k.done && !k.breakdown;
+    if (B.mixed) {
+      rep.refinements = k.steps;
+      rep.iterations = k.steps;
+      rep.inner_iterations = k.pad0;
+    } else {
+      rep.iterations = k.iter;
+    }
+    rep.final_metric = k.metric;
+    rep.status = k.breakdown ? PDEHIP_ERR_BREAKDOWN : (k.done ? PDEHIP_OK : PDEHIP_ERR_NOT_CONVERGED);
+    if (reports) reports[p] = rep;
+  }
+}
+
+void batch_download(BatchPlan& B, int i, double* u_out) { plan_download(*B.plans.at(i), u_out); }
+
+size_t batch_bytes(const BatchPlan& B) { return B.bytes; }
+
+void BatchDeleter::operator()(BatchPlan* b) const { delete b; }
+
+void batch_destroy(BatchPtr& B) {
+  if (B) (void)hipStreamSynchronize(B->ctx->stream);
+  B.reset();
+}
+
+}  // namespace pdehip

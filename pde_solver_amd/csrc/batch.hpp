@@ -1,0 +1,100 @@
+// batch.hpp — batched multigrid-preconditioned CG for many independent 2D problems.
+//
+// Each problem keeps its own resident Plan (operators, hierarchy, boundary data, dense coarse
+// inverse, work vectors); the batch engine drives all of them in lock step with batched
+// launches: every operation of a PCG iteration (operator apply, vector updates, each level's
+// smoother sweeps, residual, restriction, prolongation, coarse solve, reductions) is ONE
+// launch whose work items cover every unfinished problem.  Work items are wavefront-sized
+// (a strip or a block of rows of one problem), so small and large grids share a launch and
+// the chip fills even when every single problem is small.  Each problem has its own device
+// scalars; kernels skip the items of problems that have converged, and the reductions are
+// segmented per problem (items of a problem are contiguous) and deterministic.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace pdehip {
+
+// One (problem, level) entry of the device table.
+struct BEnt {
+  DevOp op;          // the level's operator in the path's precision
+  void* u;           // level correction (level 0: the preconditioner output z)
+  void* b;           // level right-hand side (level 0: the preconditioner input r)
+  void* r;           // level residual
+  void* alt;         // second smoother buffer
+  KScalars* sc;      // the problem's scalars of the current Krylov loop (inner or outer)
+  double pfac;       // prolongation factor into this level: sigma(coarse) / sigma(this)
+  int pad;
+};
+
+// Level-0 vectors of one problem.
+struct BVec {
+  void *x, *r, *p, *q, *z;       // Krylov vectors in the inner precision
+  double *x64, *r64, *b64, *x0;  // FP64 outer vectors (mixed) / solution (FP64)
+  KScalars* so;                  // outer scalars (FP64 refinement; FP64 path: the CG scalars)
+  KScalars* si;                  // inner scalars (mixed)
+  long long n;
+  int nx, ny;
+  DevOp op64;                    // level-0 FP64 operator (outer residual)
+};
+
+// Work item: problem p and a range (meaning depends on the list).
+struct BItem { int p, a, b, c; };
+
+struct BatchPlan;
+struct BatchDeleter { void operator()(BatchPlan* b) const; };
+using BatchPtr = std::unique_ptr<BatchPlan, BatchDeleter>;
+BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pdehip_options& opt,
+                      const pdehip_batch_config* cfg);
+void batch_solve(BatchPlan& B, pdehip_report* reports);
+void batch_download(BatchPlan& B, int i, double* u_out);
+size_t batch_bytes(const BatchPlan& B);
+void batch_destroy(BatchPtr& B);
+// True when the batched engine handles this set (2D, constant-coefficient operators, MG-PCG).
+bool batch_eligible(int count, const pdehip_problem* probs, const pdehip_options& opt,
+                    std::string* why);
+
+// ---- batched kernels (k_batch.hip)
+namespace bk {
+// Smoother: one red-black sweep of every listed (problem, strip, row range), out of place
+// between u and alt (dir 0: u -> alt, 1: alt -> u).  zero: u is zero (skip its loads).
+void sweep(hipStream_t s, Prec p, const BEnt* tab, const BItem* items, int n, int c1, int dir,
+           bool zero);
+// r = b - A u over listed (problem, row range) items; MODE apply: y = A x with x.y partials.
+void residual(hipStream_t s, Prec p, const BEnt* tab, const BItem* items, int n);
+// Coarse right-hand side (FW + folded closure transpose) for listed coarse (problem, row range).
+void restrict_(hipStream_t s, Prec pf, Prec pc, const BEnt* fine, const BEnt* coarse,
+               const BItem* items, int n, bool scale0);
+// Fine += pfac * interpolate(closure-extended coarse) for listed fine (problem, row range).
+void prolong(hipStream_t s, Prec pc, Prec pf, const BEnt* coarse, const BEnt* fine,
+             const BItem* items, int n, bool scale0);
+// Coarsest-level dense solve: x[map] = Ainv b[map], one wavefront per (problem, row) item.
+struct BDense { const double* ainv; const int* map; int m; int pad; };
+void coarse_dense(hipStream_t s, Prec p, const BEnt* tab, const BDense* dense, const BItem* items,
+                  int n);
+// Level-0 Krylov vector kernels over (problem, element range) items; partials per item.
+enum VecOp { V_APPLY_DOT = 0, V_CG_UPDATE = 1, V_DOT_RZ = 2, V_XPAY = 3, V_RESID64 = 4,
+             V_TO32 = 5, V_ADD32 = 6, V_COPY_ZP = 7, V_ZERO_X = 8, V_COPY_X0 = 9, V_RR = 10 };
+// (V_APPLY_DOT and V_RESID64 are row kernels: apply_dot_tab.)
+void vec(hipStream_t s, Prec p, int op, const BVec* vecs, const BItem* items, int n,
+         double* partials, double* maxpart);
+// Level-0 row items: inner q = A p with partials p.q (operator and scalars of tab0), or the
+// outer FP64 residual r64 = b64 - A64 x64 with partials r.r.
+void apply_dot_tab(hipStream_t s, Prec p, const BVec* vecs, const BEnt* tab0, const BItem* rows,
+                   int n, double* partials, bool outer);
+// Inner scalars reset for a refinement (outer metric kept as the correction scale).
+void reset_inner(hipStream_t s, const BVec* vecs, int nprob);
+void init_scalars(hipStream_t s, const BVec* vecs, int nprob, double eps);
+// Segmented finalize: problem q sums partials[first[q] .. first[q]+cnt[q]) and applies FinOp
+// `op` to its scalars (inner when inner != 0).
+void finalize(hipStream_t s, const BVec* vecs, const int* first, const int* cnt, int nprob,
+              const double* partials, const double* maxpart, int op, int inner, double tol,
+              int tol_mode);
+// Count of problems whose selected scalars are not done (written to *active).
+void active_count(hipStream_t s, const BVec* vecs, int nprob, int inner, int* active);
+}  // namespace bk
+
+}  // namespace pdehip

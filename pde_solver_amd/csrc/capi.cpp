@@ -6,6 +6,7 @@
 #include <string>
 
 #include "runtime.hpp"
+#include "batch.hpp"
 #include "serving.hpp"
 
 using namespace pdehip;
@@ -300,6 +301,57 @@ int pdehip_solve_batch(pdehip_ctx* ctx, int count, const pdehip_problem* probs,
     HIPCHK(hipSetDevice(ctx->c.device));
     return solve_batch(&ctx->c, count, probs, *opt, cfg, u_out, reports);
   });
+}
+
+struct pdehip_batch {
+  pdehip_ctx* owner;
+  pdehip::BatchPtr b;
+};
+
+int pdehip_batch_create(pdehip_ctx* ctx, int count, const pdehip_problem* probs,
+                        const pdehip_options* opt, const pdehip_batch_config* cfg,
+                        pdehip_batch** out) {
+  if (!ctx || !out || count < 1 || !probs || !opt) return PDEHIP_ERR_INVALID;
+  *out = nullptr;
+  return guarded(ctx, [&]() {
+    std::lock_guard<std::mutex> g(ctx->c.mu);
+    HIPCHK(hipSetDevice(ctx->c.device));
+    auto h = std::make_unique<pdehip_batch>();
+    h->owner = ctx;
+    h->b = batch_create(&ctx->c, count, probs, *opt, cfg);
+    *out = h.release();
+    return PDEHIP_OK;
+  });
+}
+
+int pdehip_batch_solve(pdehip_batch* b, pdehip_report* reports) {
+  if (!b) return PDEHIP_ERR_INVALID;
+  return guarded(b->owner, [&]() {
+    std::lock_guard<std::mutex> g(b->owner->c.mu);
+    HIPCHK(hipSetDevice(b->owner->c.device));
+    batch_solve(*b->b, reports);
+    return PDEHIP_OK;
+  });
+}
+
+int pdehip_batch_download(pdehip_batch* b, int index, double* u_out) {
+  if (!b || !u_out) return PDEHIP_ERR_INVALID;
+  return guarded(b->owner, [&]() {
+    std::lock_guard<std::mutex> g(b->owner->c.mu);
+    batch_download(*b->b, index, u_out);
+    return PDEHIP_OK;
+  });
+}
+
+size_t pdehip_batch_bytes(const pdehip_batch* b) { return b ? batch_bytes(*b->b) : 0; }
+
+void pdehip_batch_destroy(pdehip_batch* b) {
+  if (!b) return;
+  {
+    std::lock_guard<std::mutex> g(b->owner->c.mu);
+    batch_destroy(b->b);
+  }
+  delete b;
 }
 
 int pdehip_server_create(pdehip_ctx* ctx, const pdehip_server_config* cfg, pdehip_server** out) {

@@ -1,0 +1,626 @@
+// k_batch.hip — batched kernels for many independent 2D constant-coefficient problems.
+//
+// Every launch walks a list of work items; an item belongs to one problem (BItem.p) and is
+// processed by one wavefront (stencil, transfer and smoother items) or one workgroup (vector
+// items).  Per-problem data comes from a device table (BEnt per problem and level, BVec per
+// problem), so problems of any size share a launch.  Items of a problem whose scalars say
+// "done" return at once.  Reductions write one partial per item; items of a problem are
+// contiguous, and the segmented finalize sums each problem's range in a fixed order.
+#include "batch.hpp"
+
+namespace pdehip {
+namespace bk {
+namespace {
+
+constexpr int kLanes = 64;
+constexpr int kTileX = kLanes - 4;   // smoother strip: 60 output columns, 2-node ring
+constexpr int kWavesPB = 4;          // wavefront items per 256-thread workgroup
+
+template <Prec P> struct PT;
+template <> struct PT<P64> { using T = double; using S = double; };
+template <> struct PT<P32> { using T = float; using S = float; };
+template <> struct PT<P16> { using T = float; using S = __half; };
+
+template <typename S> __device__ __forceinline__ auto ldS(const S* p, long long i);
+template <> __device__ __forceinline__ auto ldS<double>(const double* p, long long i) { return p[i]; }
+template <> __device__ __forceinline__ auto ldS<float>(const float* p, long long i) { return p[i]; }
+template <> __device__ __forceinline__ auto ldS<__half>(const __half* p, long long i) { return __half2float(p[i]); }
+
+template <typename S, typename T> __device__ __forceinline__ T rounded(T v);
+template <> __device__ __forceinline__ double rounded<double, double>(double v) { return v; }
+template <> __device__ __forceinline__ float rounded<float, float>(float v) { return v; }
+template <> __device__ __forceinline__ float rounded<__half, float>(float v) { return __half2float(__float2half(v)); }
+
+__device__ __forceinline__ bool done(const KScalars* sc) { return sc && (sc->done | sc->breakdown); }
+
+__device__ __forceinline__ int wave_item() {
+  return static_cast<int>(blockIdx.x) * kWavesPB + static_cast<int>(threadIdx.x / kLanes);
+}
+
+template <typename T>
+__device__ __forceinline__ W7<T> const_w(const DevOp& op) {
+  W7<T> w;
+  w.C = T(op.w[0]); w.W = T(op.w[1]); w.E = T(op.w[2]); w.S = T(op.w[3]); w.N = T(op.w[4]);
+  w.B = T(0); w.F = T(0);
+  return w;
+}
+
+// ------------------------------------------------------------------------------- smoother
+// One red-black sweep of item (p, strip a, rows [b, c)): the single-problem 2D fused sweep
+// (k_smooth.hip) with the operator and buffers of problem p.
+template <Prec P, bool ZERO>
+__global__ void __launch_bounds__(kLanes * kWavesPB)
+kb_sweep(const BEnt* __restrict__ tab, const BItem* __restrict__ items, int n, int c1, int dir) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  const int w = wave_item();
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BEnt& e = tab[it.p];
+  if (done(e.sc)) return;
+  const DevOp& op = e.op;
+  const S* __restrict__ b = static_cast<const S*>(e.b);
+  const S* __restrict__ xin = static_cast<const S*>(dir ? e.alt : e.u);
+  S* __restrict__ xout = static_cast<S*>(dir ? e.u : e.alt);
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int i = it.a * kTileX - 2 + lane;
+  const int j0 = it.b, j1 = it.c;
+  const int ny = op.ny;
+  const long long nx = op.nx;
+  const int c2 = 1 - c1;
+  const bool in_x = i >= 0 && i < op.nx;
+  const bool r1 = lane >= 1 && lane <= kLanes - 2 && i >= 1 && i <= op.nx - 2;
+  const bool r2 = lane >= 2 && lane < kLanes - 2 && i >= 1 && i <= op.nx - 2;
+  const bool out = lane >= 2 && lane < kLanes - 2 && in_x;
+  const int ci = min(max(i, 0), op.nx - 1);
+  auto ld_u = [&](int j) -> T {
+    if constexpr (ZERO) return T(0);
+    else return T(ldS(xin, (long long)min(max(j, 0), ny - 1) * nx + ci));
+  };
+  auto ld_b = [&](int j) -> T { return T(ldS(b, (long long)min(max(j, 0), ny - 1) * nx + ci)); };
+  const W7<T> w0 = const_w<T>(op);
+  T U[4], B[3];
+  U[0] = U[1] = T(0); U[2] = ld_u(j0 - 2); U[3] = ld_u(j0 - 1);
+  B[0] = B[1] = T(0); B[2] = ld_b(j0 - 1);
+  for (int p = j0 - 1; p <= j1; ++p) {
+    U[0] = U[1]; U[1] = U[2]; U[2] = U[3]; U[3] = ld_u(p + 1);
+    B[0] = B[1]; B[1] = B[2]; B[2] = ld_b(p + 1);
+    {
+      const T uw = __shfl_up(U[2], 1, kLanes), ue = __shfl_down(U[2], 1, kLanes);
+      if (r1 && p >= 1 && p <= ny - 2 && ((i + p) & 1) == c1) {
+        W7<T> wt = w0;
+        fold_center<2, T>(op, i, p, 0, wt);
+        U[2] = rounded<S, T>(gs_value<2, T>(wt, B[1], ue, uw, U[3], U[1], T(0), T(0)));
+      }
+    }
+    const int q = p - 1;
+    if (q >= j0 && q < j1) {
+      const T uw = __shfl_up(U[1], 1, kLanes), ue = __shfl_down(U[1], 1, kLanes);
+      if (r2 && q >= 1 && q <= ny - 2 && ((i + q) & 1) == c2) {
+        W7<T> wt = w0;
+        fold_center<2, T>(op, i, q, 0, wt);
+        U[1] = rounded<S, T>(gs_value<2, T>(wt, B[0], ue, uw, U[2], U[0], T(0), T(0)));
+      }
+      if (out) xout[q * nx + i] = store_as<S>(static_cast<double>(U[1]));
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------- residual
+// r = b - A u on rows [a, b) of problem p (boundary rows b - u).
+template <Prec P>
+__global__ void __launch_bounds__(kLanes * kWavesPB)
+kb_residual(const BEnt* __restrict__ tab, const BItem* __restrict__ items, int n) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  const int w = wave_item();
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BEnt& e = tab[it.p];
+  if (done(e.sc)) return;
+  const DevOp& op = e.op;
+  const S* __restrict__ u = static_cast<const S*>(e.u);
+  const S* __restrict__ b = static_cast<const S*>(e.b);
+  S* __restrict__ r = static_cast<S*>(e.r);
+  const int lane = threadIdx.x & (kLanes - 1);
+  const long long nx = op.nx;
+  const W7<T> w0 = const_w<T>(op);
+  for (int j = it.a; j < it.b; ++j) {
+    for (int i = lane; i < op.nx; i += kLanes) {
+      const long long id = j * nx + i;
+      const T uc = T(ldS(u, id));
+      T out;
+      if (i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1) {
+        out = T(ldS(b, id)) - uc;
+      } else {
+        W7<T> wt = w0;
+        fold_center<2, T>(op, i, j, 0, wt);
+        const T ax = wt.C * uc + wt.E * T(ldS(u, id + 1)) + wt.W * T(ldS(u, id - 1)) +
+                     wt.N * T(ldS(u, id + nx)) + wt.S * T(ldS(u, id - nx));
+        out = T(ldS(b, id)) - ax;
+      }
+      r[id] = store_as<S>(static_cast<double>(out));
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------- transfers
+__device__ __forceinline__ int fold_list(int c, int n, double th_lo, double th_hi, int* idx, double* w) {
+  idx[0] = c; w[0] = 1.0;
+  int m = 1;
+  if (c == 1) { idx[m] = 0; w[m] = th_lo; ++m; }
+  if (c == n - 2) { idx[m] = n - 1; w[m] = th_hi; ++m; }
+  return m;
+}
+
+template <typename SF>
+__device__ __forceinline__ double fw2(const DevOp& f, const SF* rf, int fi, int fj) {
+  const double w1[3] = {0.25, 0.5, 0.25};
+  double acc = 0.0;
+  for (int dj = -1; dj <= 1; ++dj) {
+    const int j = fj + dj;
+    if (j < 0 || j >= f.ny) continue;
+    const SF* row = rf + (long long)j * f.nx;
+    double a = 0.5 * static_cast<double>(ldS(row, fi));
+    if (fi > 0) a += 0.25 * static_cast<double>(ldS(row, fi - 1));
+    if (fi < f.nx - 1) a += 0.25 * static_cast<double>(ldS(row, fi + 1));
+    acc += w1[dj + 1] * a;
+  }
+  return acc;
+}
+
+// Coarse rows [a, b) of problem p: FW of the fine residual with the folded closure transpose
+// (coarse boundary nodes zero), times the problem's normalisation scale when scale0.
+template <Prec PF, Prec PC>
+__global__ void __launch_bounds__(kLanes * kWavesPB)
+kb_restrict(const BEnt* __restrict__ ftab, const BEnt* __restrict__ ctab,
+            const BItem* __restrict__ items, int n, int scale0) {
+  using SF = typename PT<PF>::S;
+  using SC = typename PT<PC>::S;
+  const int w = wave_item();
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BEnt& fe = ftab[it.p];
+  const BEnt& ce = ctab[it.p];
+  if (done(fe.sc)) return;
+  const DevOp& f = fe.op;
+  const DevOp& c = ce.op;
+  const SF* rf = static_cast<const SF*>(fe.r);
+  SC* bc = static_cast<SC*>(ce.b);
+  const double s = scale0 ? fe.sc->scale : 1.0;
+  const int lane = threadIdx.x & (kLanes - 1);
+  for (int jc = it.a; jc < it.b; ++jc) {
+    SC* row = bc + (long long)jc * c.nx;
+    if (jc == 0 || jc == c.ny - 1) {
+      for (int ic = lane; ic < c.nx; ic += kLanes) row[ic] = store_as<SC>(0.0);
+      continue;
+    }
+    int yi[3];
+    double yw[3];
+    const int my = fold_list(jc, c.ny, c.theta[2], c.theta[3], yi, yw);
+    for (int ic = lane; ic < c.nx; ic += kLanes) {
+      if (ic == 0 || ic == c.nx - 1) { row[ic] = store_as<SC>(0.0); continue; }
+      int xi[3];
+      double xw[3];
+      const int mx = fold_list(ic, c.nx, c.theta[0], c.theta[1], xi, xw);
+      double acc = 0.0;
+      for (int bb = 0; bb < my; ++bb)
+        for (int q = 0; q < mx; ++q) {
+          const double wt = yw[bb] * xw[q];
+          if (wt != 0.0) acc += wt * fw2(f, rf, 2 * xi[q], 2 * yi[bb]);
+        }
+      row[ic] = store_as<SC>(acc * s);
+    }
+  }
+}
+
+// Fine rows [a, b) of problem p += pfac * (inv_scale when scale0) * bilinear interpolation of
+// the closure-extended coarse correction.
+template <Prec PC, Prec PF>
+__global__ void __launch_bounds__(kLanes * kWavesPB)
+kb_prolong(const BEnt* __restrict__ ctab, const BEnt* __restrict__ ftab,
+           const BItem* __restrict__ items, int n, int scale0) {
+  using SC = typename PT<PC>::S;
+  using SF = typename PT<PF>::S;
+  const int w = wave_item();
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BEnt& ce = ctab[it.p];
+  const BEnt& fe = ftab[it.p];
+  if (done(fe.sc)) return;
+  const DevOp& c = ce.op;
+  const DevOp& f = fe.op;
+  const SC* uc = static_cast<const SC*>(ce.u);
+  SF* uf = static_cast<SF*>(fe.u);
+  const double s = fe.pfac * (scale0 ? fe.sc->inv_scale : 1.0);
+  const int lane = threadIdx.x & (kLanes - 1);
+  for (int j = max(it.a, 1); j < min(it.b, f.ny - 1); ++j) {
+    const int jc = j >> 1, oj = j & 1;
+    int y0 = jc, y1 = jc + 1;
+    double wy0 = 1.0, wy1 = oj ? 1.0 : 0.0;
+    if (y0 == 0) { wy0 = c.theta[2]; y0 = 1; }
+    if (y1 == c.ny - 1) { wy1 *= c.theta[3]; y1 = c.ny - 2; }
+    const double hy = oj ? 0.5 : 1.0;
+    const SC* r0 = uc + (long long)y0 * c.nx;
+    const SC* r1 = uc + (long long)y1 * c.nx;
+    for (int i = 1 + lane; i < f.nx - 1; i += kLanes) {
+      const int ic = i >> 1, oi = i & 1;
+      int x0 = ic, x1 = ic + 1;
+      double wx0 = 1.0, wx1 = oi ? 1.0 : 0.0;
+      if (x0 == 0) { wx0 = c.theta[0]; x0 = 1; }
+      if (x1 == c.nx - 1) { wx1 *= c.theta[1]; x1 = c.nx - 2; }
+      const double hx = oi ? 0.5 : 1.0;
+      double a = 0.0;
+      if (wy0 != 0.0) {
+        double v = wx0 != 0.0 ? wx0 * static_cast<double>(ldS(r0, x0)) : 0.0;
+        if (wx1 != 0.0) v += wx1 * static_cast<double>(ldS(r0, x1));
+        a += wy0 * (v * hx);
+      }
+      if (wy1 != 0.0) {
+        double v = wx0 != 0.0 ? wx0 * static_cast<double>(ldS(r1, x0)) : 0.0;
+        if (wx1 != 0.0) v += wx1 * static_cast<double>(ldS(r1, x1));
+        a += wy1 * (v * hx);
+      }
+      const long long id = (long long)j * f.nx + i;
+      uf[id] = store_as<SF>(static_cast<double>(ldS(uf, id)) + s * (a * hy));
+    }
+  }
+}
+
+// Coarsest level: item (p, row r): x[map[r]] = sum_q Ainv[r][q] b[map[q]].
+template <Prec P>
+__global__ void __launch_bounds__(kLanes * kWavesPB)
+kb_dense(const BEnt* __restrict__ tab, const BDense* __restrict__ dense,
+         const BItem* __restrict__ items, int n) {
+  using S = typename PT<P>::S;
+  const int w = wave_item();
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BEnt& e = tab[it.p];
+  if (done(e.sc)) return;
+  const BDense d = dense[it.p];
+  const S* b = static_cast<const S*>(e.b);
+  S* x = static_cast<S*>(e.u);
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int row = it.a;
+  double acc = 0.0;
+  for (int q = lane; q < d.m; q += kLanes)
+    acc += d.ainv[(long long)row * d.m + q] * static_cast<double>(ldS(b, d.map[q]));
+  acc = wave_sum(acc);
+  if (lane == 0) x[d.map[row]] = store_as<S>(acc);
+}
+
+// ------------------------------------------------------------------------------- level-0 Krylov
+// Item (p, e0, e1) over the problem's level-0 vectors, one workgroup per item.
+template <Prec P>
+__global__ void __launch_bounds__(256)
+kb_vec(const BVec* __restrict__ vecs, const BItem* __restrict__ items, int n, int op,
+       int inner, double* partials, double* maxpart) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  __shared__ double lds[256 / kWave];
+  const int w = blockIdx.x;
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BVec& v = vecs[it.p];
+  const KScalars* sc = inner ? v.si : v.so;
+  double acc = 0.0, mx = 0.0;
+  const bool skip_all = done(sc) && op != V_ADD32;
+  if (!skip_all) {
+    S* x = static_cast<S*>(v.x);
+    S* r = static_cast<S*>(v.r);
+    S* pp = static_cast<S*>(v.p);
+    const S* q = static_cast<const S*>(v.q);
+    const S* z = static_cast<const S*>(v.z);
+    for (long long i = it.a + threadIdx.x; i < it.b; i += blockDim.x) {
+      switch (op) {
+        case V_CG_UPDATE: {
+          const T a = T(sc->alpha);
+          const T xi = T(ldS(x, i)) + a * T(ldS(pp, i));
+          const T ri = T(ldS(r, i)) - a * T(ldS(q, i));
+          x[i] = store_as<S>(static_cast<double>(xi));
+          r[i] = store_as<S>(static_cast<double>(ri));
+          acc += static_cast<double>(ri) * static_cast<double>(ri);
+          mx = fmax(mx, fabs(static_cast<double>(ri)));
+          break;
+        }
+        case V_DOT_RZ:
+          acc += static_cast<double>(ldS(r, i)) * static_cast<double>(ldS(z, i));
+          break;
+        case V_XPAY: {
+          const T bt = T(sc->beta);
+          pp[i] = store_as<S>(static_cast<double>(T(ldS(z, i)) + bt * T(ldS(pp, i))));
+          break;
+        }
+        case V_COPY_ZP:
+          pp[i] = z[i];
+          break;
+        case V_RR: {
+          const double ri = static_cast<double>(ldS(r, i));
+          acc += ri * ri;
+          mx = fmax(mx, fabs(ri));
+          break;
+        }
+        case V_ZERO_X:
+          x[i] = store_as<S>(0.0);
+          break;
+        case V_TO32: {   // inner rhs = r64 / ||r64|| (outer metric), x32 = 0
+          const double nr = v.so->metric;
+          const double s = nr > 0.0 ? 1.0 / nr : 1.0;
+          r[i] = store_as<S>(v.r64[i] * s);
+          x[i] = store_as<S>(0.0);
+          break;
+        }
+        case V_ADD32:    // x64 += ||r|| * x32 (problems already converged outside skip)
+          if (!done(v.so)) v.x64[i] += v.so->snorm * static_cast<double>(ldS(x, i));
+          break;
+        case V_COPY_X0:
+          v.x64[i] = v.x0[i];
+          break;
+        default:
+          break;
+      }
+    }
+  }
+  if (partials) {
+    const double s = block_sum(acc, lds);
+    if (threadIdx.x == 0) partials[w] = s;
+  }
+  if (maxpart) {
+    const double m = block_max(mx, lds);
+    if (threadIdx.x == 0) maxpart[w] = m;
+  }
+}
+
+// Level-0 rows [a, b) of problem p.  Inner (OUTER false): q = A p with the level-0 operator of
+// the Krylov precision (tab0) and partials of p.q.  Outer: r64 = b64 - A64 x64 with partials
+// of r.r (the FP64 refinement residual).
+template <Prec P, bool OUTER>
+__global__ void __launch_bounds__(kLanes * kWavesPB)
+kb_apply_dot(const BVec* __restrict__ vecs, const BEnt* __restrict__ tab0,
+             const BItem* __restrict__ items, int n, double* partials) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  const int w = wave_item();
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BVec& v = vecs[it.p];
+  const int lane = threadIdx.x & (kLanes - 1);
+  double acc = 0.0;
+  const KScalars* sc = OUTER ? v.so : tab0[it.p].sc;
+  if (!done(sc)) {
+    const DevOp& op = OUTER ? v.op64 : tab0[it.p].op;
+    const S* x = OUTER ? reinterpret_cast<const S*>(v.x64) : static_cast<const S*>(v.p);
+    const S* b = OUTER ? reinterpret_cast<const S*>(v.b64) : nullptr;
+    S* y = OUTER ? reinterpret_cast<S*>(v.r64) : static_cast<S*>(v.q);
+    const long long nx = op.nx;
+    const W7<T> w0 = const_w<T>(op);
+    for (int j = it.a; j < it.b; ++j) {
+      for (int i = lane; i < op.nx; i += kLanes) {
+        const long long id = j * nx + i;
+        const T xc = T(ldS(x, id));
+        T ax;
+        if (i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1) {
+          ax = xc;
+        } else {
+          W7<T> wt = w0;
+          fold_center<2, T>(op, i, j, 0, wt);
+          ax = wt.C * xc + wt.E * T(ldS(x, id + 1)) + wt.W * T(ldS(x, id - 1)) +
+               wt.N * T(ldS(x, id + nx)) + wt.S * T(ldS(x, id - nx));
+        }
+        if constexpr (OUTER) {
+          const T rv = T(ldS(b, id)) - ax;
+          y[id] = store_as<S>(static_cast<double>(rv));
+          acc += static_cast<double>(rv) * static_cast<double>(rv);
+        } else {
+          y[id] = store_as<S>(static_cast<double>(ax));
+          acc += static_cast<double>(xc) * static_cast<double>(ax);
+        }
+      }
+    }
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) partials[w] = acc;
+}
+
+// Segmented finalize: workgroup q sums problem q's partial range in a fixed order.
+__global__ void __launch_bounds__(256)
+kb_finalize(const BVec* __restrict__ vecs, const int* __restrict__ first,
+            const int* __restrict__ cnt, int nprob, const double* __restrict__ partials,
+            const double* __restrict__ maxpart, int op, int inner, double tol, int tol_mode) {
+  __shared__ double lds[256 / kWave];
+  const int q = blockIdx.x;
+  if (q >= nprob) return;
+  const BVec& v = vecs[q];
+  KScalars* sc = inner ? v.si : v.so;
+  if (done(sc)) return;
+  double acc = 0.0, mx = 0.0;
+  for (int t = threadIdx.x; t < cnt[q]; t += blockDim.x) {
+    acc += partials[first[q] + t];
+    if (maxpart) mx = fmax(mx, maxpart[first[q] + t]);
+  }
+  const double s = block_sum(acc, lds);
+  const double m = maxpart ? block_max(mx, lds) : 0.0;
+  if (threadIdx.x != 0) return;
+  switch (op) {
+    case FIN_CG_ALPHA:
+      sc->pq = s;
+      if (fabs(s) < sc->eps) { sc->breakdown = 1; break; }
+      sc->alpha = sc->rz / s;
+      break;
+    case FIN_CG_RR:
+      sc->rr = s;
+      sc->metric = sqrt(fmax(0.0, s));
+      sc->iter += 1;
+      if (sc->metric < sc->target) sc->done = 1;
+      break;
+    case FIN_CG_BETA:
+      sc->rz_old = sc->rz;
+      sc->beta = s / sc->rz;
+      sc->rz = s;
+      break;
+    case FIN_CG_INIT:
+      sc->rz = s;
+      break;
+    case FIN_R0: {
+      const double r0 = sqrt(fmax(0.0, s));
+      sc->r0 = r0;
+      sc->metric = r0;
+      sc->target = tol_mode == PDEHIP_TOL_REL ? tol * r0 : tol;
+      if (r0 < sc->target || r0 == 0.0) sc->done = 1;
+      break;
+    }
+    default:
+      break;
+  }
+  if (maxpart && !(sc->done | sc->breakdown)) {
+    sc->inv_scale = m > 0.0 ? m : 1.0;
+    sc->scale = m > 0.0 ? 1.0 / m : 1.0;
+  }
+}
+
+// Inner scalars of every unfinished problem reset for a new refinement; the outer metric is
+// kept as the correction's scale (snorm).
+__global__ void kb_reset_inner(const BVec* __restrict__ vecs, int nprob, int outer_iter_inc) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nprob) return;
+  const BVec& v = vecs[q];
+  v.so->pad0 += v.si->iter;          // inner iterations of the finished refinement
+  KScalars k{};
+  k.eps = 1e-300;
+  k.scale = 1.0;
+  k.inv_scale = 1.0;
+  k.done = (v.so->done | v.so->breakdown);
+  *v.si = k;
+  if (!k.done) {
+    v.so->snorm = v.so->metric;
+    v.so->steps += outer_iter_inc;   // refinements
+  }
+}
+
+// Fresh scalars for a solve (outer and inner): breakdown threshold, unit scales, counters 0.
+__global__ void kb_init_scalars(const BVec* __restrict__ vecs, int nprob, double eps) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nprob) return;
+  KScalars k{};
+  k.eps = eps;
+  k.scale = 1.0;
+  k.inv_scale = 1.0;
+  *vecs[q].so = k;
+  *vecs[q].si = k;
+}
+
+__global__ void kb_active(const BVec* __restrict__ vecs, int nprob, int inner, int* active) {
+  __shared__ double lds[1024 / kWave];
+  double a = 0.0;
+  for (int q = threadIdx.x; q < nprob; q += blockDim.x) {
+    const KScalars* sc = inner ? vecs[q].si : vecs[q].so;
+    a += (sc->done | sc->breakdown) ? 0.0 : 1.0;
+  }
+  const double s = block_sum(a, lds);
+  if (threadIdx.x == 0) *active = static_cast<int>(s);
+}
+
+unsigned waves_grid(int n) { return static_cast<unsigned>((n + kWavesPB - 1) / kWavesPB); }
+
+}  // namespace
+
+void sweep(hipStream_t s, Prec p, const BEnt* tab, const BItem* items, int n, int c1, int dir,
+           bool zero) {
+  if (n <= 0) return;
+  const dim3 g(waves_grid(n)), b(kLanes * kWavesPB);
+#define SW_(PP, ZZ) hipLaunchKernelGGL((kb_sweep<PP, ZZ>), g, b, 0, s, tab, items, n, c1, dir)
+  switch (p) {
+    case P64: if (zero) SW_(P64, true); else SW_(P64, false); break;
+    case P32: if (zero) SW_(P32, true); else SW_(P32, false); break;
+    default: if (zero) SW_(P16, true); else SW_(P16, false); break;
+  }
+#undef SW_
+}
+
+void residual(hipStream_t s, Prec p, const BEnt* tab, const BItem* items, int n) {
+  if (n <= 0) return;
+  const dim3 g(waves_grid(n)), b(kLanes * kWavesPB);
+  switch (p) {
+    case P64: hipLaunchKernelGGL((kb_residual<P64>), g, b, 0, s, tab, items, n); break;
+    case P32: hipLaunchKernelGGL((kb_residual<P32>), g, b, 0, s, tab, items, n); break;
+    default: hipLaunchKernelGGL((kb_residual<P16>), g, b, 0, s, tab, items, n); break;
+  }
+}
+
+void restrict_(hipStream_t s, Prec pf, Prec pc, const BEnt* fine, const BEnt* coarse,
+               const BItem* items, int n, bool scale0) {
+  if (n <= 0) return;
+  const dim3 g(waves_grid(n)), b(kLanes * kWavesPB);
+  const int sc = scale0 ? 1 : 0;
+  if (pf == P64) hipLaunchKernelGGL((kb_restrict<P64, P64>), g, b, 0, s, fine, coarse, items, n, sc);
+  else if (pf == P32) hipLaunchKernelGGL((kb_restrict<P32, P16>), g, b, 0, s, fine, coarse, items, n, sc);
+  else hipLaunchKernelGGL((kb_restrict<P16, P16>), g, b, 0, s, fine, coarse, items, n, sc);
+  (void)pc;
+}
+
+void prolong(hipStream_t s, Prec pc, Prec pf, const BEnt* coarse, const BEnt* fine,
+             const BItem* items, int n, bool scale0) {
+  if (n <= 0) return;
+  const dim3 g(waves_grid(n)), b(kLanes * kWavesPB);
+  const int sc = scale0 ? 1 : 0;
+  if (pf == P64) hipLaunchKernelGGL((kb_prolong<P64, P64>), g, b, 0, s, coarse, fine, items, n, sc);
+  else if (pf == P32) hipLaunchKernelGGL((kb_prolong<P16, P32>), g, b, 0, s, coarse, fine, items, n, sc);
+  else hipLaunchKernelGGL((kb_prolong<P16, P16>), g, b, 0, s, coarse, fine, items, n, sc);
+  (void)pc;
+}
+
+void coarse_dense(hipStream_t s, Prec p, const BEnt* tab, const BDense* dense, const BItem* items,
+                  int n) {
+  if (n <= 0) return;
+  const dim3 g(waves_grid(n)), b(kLanes * kWavesPB);
+  switch (p) {
+    case P64: hipLaunchKernelGGL((kb_dense<P64>), g, b, 0, s, tab, dense, items, n); break;
+    case P32: hipLaunchKernelGGL((kb_dense<P32>), g, b, 0, s, tab, dense, items, n); break;
+    default: hipLaunchKernelGGL((kb_dense<P16>), g, b, 0, s, tab, dense, items, n); break;
+  }
+}
+
+void vec(hipStream_t s, Prec p, int op, const BVec* vecs, const BItem* items, int n,
+         double* partials, double* maxpart) {
+  if (n <= 0) return;
+  const int inner = (p == P32) ? 1 : 0;
+  if (p == P64)
+    hipLaunchKernelGGL((kb_vec<P64>), dim3(n), dim3(256), 0, s, vecs, items, n, op, inner, partials, maxpart);
+  else
+    hipLaunchKernelGGL((kb_vec<P32>), dim3(n), dim3(256), 0, s, vecs, items, n, op, inner, partials, maxpart);
+}
+
+void apply_dot_tab(hipStream_t s, Prec p, const BVec* vecs, const BEnt* tab0, const BItem* rows,
+                   int n, double* partials, bool outer) {
+  if (n <= 0) return;
+  const dim3 g(waves_grid(n)), b(kLanes * kWavesPB);
+  if (outer)
+    hipLaunchKernelGGL((kb_apply_dot<P64, true>), g, b, 0, s, vecs, tab0, rows, n, partials);
+  else if (p == P64)
+    hipLaunchKernelGGL((kb_apply_dot<P64, false>), g, b, 0, s, vecs, tab0, rows, n, partials);
+  else
+    hipLaunchKernelGGL((kb_apply_dot<P32, false>), g, b, 0, s, vecs, tab0, rows, n, partials);
+}
+
+void finalize(hipStream_t s, const BVec* vecs, const int* first, const int* cnt, int nprob,
+              const double* partials, const double* maxpart, int op, int inner, double tol,
+              int tol_mode) {
+  hipLaunchKernelGGL(kb_finalize, dim3(nprob), dim3(256), 0, s, vecs, first, cnt, nprob, partials,
+                     maxpart, op, inner, tol, tol_mode);
+}
+
+void reset_inner(hipStream_t s, const BVec* vecs, int nprob) {
+  hipLaunchKernelGGL(kb_reset_inner, dim3((nprob + 255) / 256), dim3(256), 0, s, vecs, nprob, 1);
+}
+
+void init_scalars(hipStream_t s, const BVec* vecs, int nprob, double eps) {
+  hipLaunchKernelGGL(kb_init_scalars, dim3((nprob + 255) / 256), dim3(256), 0, s, vecs, nprob, eps);
+}
+
+void active_count(hipStream_t s, const BVec* vecs, int nprob, int inner, int* active) {
+  hipLaunchKernelGGL(kb_active, dim3(1), dim3(1024), 0, s, vecs, nprob, inner, active);
+}
+
+}  // namespace bk
+}  // namespace pdehip

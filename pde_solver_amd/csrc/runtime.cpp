@@ -359,6 +359,7 @@ static bool build_dense(Plan& P, Level& lv) {
 // ------------------------------------------------------------------ plan creation
 Plan::~Plan() {
   if (hsc) (void)hipHostFree(hsc);
+  if (hstage) (void)hipHostFree(hstage);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
 }
@@ -548,6 +549,11 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
   P->part3.alloc(ctx, (size_t)2 * np * sizeof(double) + 64);
   P->scal.alloc(ctx, 2 * sizeof(KScalars));
   HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&P->hsc), 2 * sizeof(KScalars)));
+  {
+    DevOp rop = P->op();
+    const size_t nst = (size_t)partial_blocks_max(P->op()) + 2 * (size_t)resnorm_blocks(rop) + 8;
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&P->hstage), nst * sizeof(double)));
+  }
   HIPCHK(hipEventCreate(&P->ev0));
   HIPCHK(hipEventCreate(&P->ev1));
   HIPCHK(hipStreamSynchronize(s));
@@ -576,6 +582,8 @@ static void init_scalars(Plan& P, int i, double eps) {
   P.hsc[i] = k;
   HIPCHK(hipMemcpyAsync(P.sc(i), &P.hsc[i], sizeof(KScalars), hipMemcpyHostToDevice, P.stream()));
 }
+
+void plan_init_scalars(Plan& P, int i, double eps) { init_scalars(P, i, eps); }
 
 static const KScalars& pull(Plan& P, int i) {
   HIPCHK(hipMemcpyAsync(&P.hsc[i], P.sc(i), sizeof(KScalars), hipMemcpyDeviceToHost, P.stream()));
@@ -1130,18 +1138,25 @@ void plan_solve(Plan& P, pdehip_report* rep) {
       break;
   }
   HIPCHK(hipEventRecord(P.ev1, s));
+  plan_finish_enqueue(P);
+  HIPCHK(hipStreamSynchronize(s));
+  plan_finish_collect(P, rep);
+}
+
+// Post-solve work on the plan's stream, no host wait: the solver system's true residual,
+// boundary reconstruction of folded solves (closures then Dirichlet, reference order), the
+// reference residual norms, and the copies the report needs into the plan's pinned staging.
+void plan_finish_enqueue(Plan& P) {
+  const hipStream_t s = P.stream();
   const Launch L0{s, nullptr};
-  // true residual of the solver's own system, before folded boundaries are reconstructed
   double* part = P.part.as<double>();
   const int mth = P.opt.method;
-  const bool resid = P.r && mth != PDEHIP_JACOBI && mth != PDEHIP_GAUSS_SEIDEL && mth != PDEHIP_SOR;
-  if (resid) residual(L0, P.op(), P64, P.bvec.get(), P.x.get(), P.r.get(), part);
-  // boundary reconstruction of folded solves (closures then Dirichlet, reference order)
+  P.fin_resid = P.r && mth != PDEHIP_JACOBI && mth != PDEHIP_GAUSS_SEIDEL && mth != PDEHIP_SOR;
+  if (P.fin_resid) residual(L0, P.op(), P64, P.bvec.get(), P.x.get(), P.r.get(), part);
   if (P.folded) {
     closures_nr(L0, P.op(), P.faces.fd, P.x.as<double>());
     closures_dirichlet(L0, P.op(), P.faces.fd, P.x.as<double>());
   }
-  // ComputeResidualNorms on the device, one synchronisation for both reductions
   double* sums = P.part3.as<double>();
   double* maxs = sums + P.npart;
   int* degenerate = reinterpret_cast<int*>(sums + 2 * P.npart + 1);
@@ -1151,24 +1166,29 @@ void plan_solve(Plan& P, pdehip_report* rep) {
   residual_norms(s, rop, P.faces.fd, P.x.as<double>(), P.f.as<double>(), P.f_const, P.nl, sums,
                  maxs, degenerate);
   const int np = partial_blocks(P.op()), nb = resnorm_blocks(rop);
-  std::vector<double> h(np), hs(nb), hm(nb);
-  if (resid) HIPCHK(hipMemcpyAsync(h.data(), part, np * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(hs.data(), sums, nb * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(hm.data(), maxs, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  double* h = P.hstage;
+  if (P.fin_resid) HIPCHK(hipMemcpyAsync(h, part, np * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(h + np, sums, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(h + np + nb, maxs, nb * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(&P.hsc[0], P.sc(0), sizeof(KScalars), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+}
+
+// Report fields from the staged copies (after the stream has been synchronised).
+void plan_finish_collect(Plan& P, pdehip_report* rep) {
+  DevOp rop = P.op();
+  const int np = partial_blocks(P.op()), nb = resnorm_blocks(rop);
+  const double* h = P.hstage;
   float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, P.ev0, P.ev1));
-  rep->solve_ms = ms;
-  if (resid) {
+  if (hipEventElapsedTime(&ms, P.ev0, P.ev1) == hipSuccess) rep->solve_ms = ms;
+  if (P.fin_resid) {
     double rr = 0.0;
-    for (double v : h) rr += v;
+    for (int i = 0; i < np; ++i) rr += h[i];
     // relative to the solve's initial residual (||b|| for reference rows, which start at x = 0)
     const double r0 = P.hsc[0].r0;
     rep->rel_residual = r0 > 0 ? std::sqrt(rr) / r0 : std::sqrt(rr);
   }
   double a = 0.0, m = 0.0;
-  for (int i = 0; i < nb; ++i) { a += hs[i]; m = std::max(m, hm[i]); }
+  for (int i = 0; i < nb; ++i) { a += h[np + i]; m = std::max(m, h[np + nb + i]); }
   rep->residual_l2 = std::sqrt(std::max(0.0, a));
   rep->residual_linf = m;
 }

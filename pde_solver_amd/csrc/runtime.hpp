@@ -142,6 +142,8 @@ struct Plan {
   int npart = 0;
   DBuf scal;                   // KScalars x 2 (outer/inner)
   KScalars* hsc = nullptr;     // pinned host mirror
+  double* hstage = nullptr;    // pinned staging of the report's reductions
+  bool fin_resid = false;      // plan_finish_enqueue staged the true-residual partials
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   size_t bytes_held = 0;
   std::vector<double> hf;      // host copy of the source (set_source)
@@ -155,6 +157,13 @@ struct Plan {
 std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehip_options& o);
 void plan_set_source(Plan& P, const double* f_host);
 void plan_solve(Plan& P, pdehip_report* rep);
+// The tail of plan_solve split in two: enqueue the post-solve work (true residual, boundary
+// reconstruction, residual norms, staging copies) without waiting, then fill the report after
+// the stream is synchronised.  Lets a batch finish many plans with one synchronisation.
+void plan_finish_enqueue(Plan& P);
+void plan_finish_collect(Plan& P, pdehip_report* rep);
+// Scalars of a plan (device) and the plan's KScalars initialisation, for batched drivers.
+void plan_init_scalars(Plan& P, int i, double eps);
 void plan_download(Plan& P, double* u_out);
 void plan_residual_norms(Plan& P, const double* u_dev, double* l2, double* linf);
 

@@ -10,6 +10,7 @@
 // runs the same kernels with the same launch shapes whatever else is in flight, so its
 // result is bitwise the same alone and in the mix.
 #include "serving.hpp"
+#include "batch.hpp"
 
 #include <chrono>
 #include <condition_variable>
@@ -304,8 +305,21 @@ void server_destroy(pdehip_server* s) {
 // Batched solve, first form: every problem through a resident plan on the context stream.
 int solve_batch(Ctx* ctx, int count, const pdehip_problem* probs, const pdehip_options& opt,
                 const pdehip_batch_config* cfg, double* const* u_out, pdehip_report* reports) {
-  (void)cfg;
   int worst = PDEHIP_OK;
+  if (batch_eligible(count, probs, opt, nullptr)) {
+    // one lock-step batched solve over resident per-problem plans
+    auto B = batch_create(ctx, count, probs, opt, cfg);
+    std::vector<pdehip_report> reps(count);
+    batch_solve(*B, reps.data());
+    for (int i = 0; i < count; ++i) {
+      if (u_out[i]) batch_download(*B, i, u_out[i]);
+      if (reports) reports[i] = reps[i];
+      if (reps[i].status != PDEHIP_OK) worst = reps[i].status;
+    }
+    batch_destroy(B);
+    return worst;
+  }
+  // other problem classes: one resident plan after the other
   for (int i = 0; i < count; ++i) {
     pdehip_report rep{};
     auto P = plan_create(ctx, probs[i], opt);

@@ -57,6 +57,12 @@ def lib():
         L.pdehip_server_wait.argtypes = [VP, C.c_uint64, R]
         L.pdehip_server_stats_get.argtypes = [VP, C.POINTER(capi.ServerStats)]
         L.pdehip_server_destroy.argtypes = [VP]
+        L.pdehip_batch_create.argtypes = [VP, C.c_int, P, O, C.POINTER(capi.BatchConfig), C.POINTER(VP)]
+        L.pdehip_batch_solve.argtypes = [VP, R]
+        L.pdehip_batch_download.argtypes = [VP, C.c_int, DP]
+        L.pdehip_batch_bytes.argtypes = [VP]
+        L.pdehip_batch_bytes.restype = C.c_size_t
+        L.pdehip_batch_destroy.argtypes = [VP]
         L.pdehip_tune_set.argtypes = [C.c_char_p, C.c_int]
         L.pdehip_tune_get.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
         _lib = L
@@ -174,6 +180,51 @@ class Context:
 
     def server(self, cfg: Optional[capi.ServerConfig] = None) -> "Server":
         return Server(self, cfg)
+
+    def batch(self, specs: Sequence[capi.Spec], opt: capi.Options,
+              cfg: Optional[capi.BatchConfig] = None) -> "Batch":
+        return Batch(self, specs, opt, cfg)
+
+
+class Batch:
+    """A resident batch: many problems solved together in lock step (pdehip_batch_*)."""
+
+    def __init__(self, ctx: Context, specs: Sequence[capi.Spec], opt: capi.Options,
+                 cfg: Optional[capi.BatchConfig] = None):
+        self.ctx = ctx
+        self.specs = list(specs)
+        self._probs = (capi.Problem * len(specs))(*[s.to_c() for s in specs])
+        self.h = C.c_void_p()
+        ctx._check(lib().pdehip_batch_create(ctx.h, len(specs), self._probs, C.byref(opt),
+                                             C.byref(cfg) if cfg is not None else None,
+                                             C.byref(self.h)))
+
+    def solve(self, check: bool = True):
+        reps = (capi.Report * len(self.specs))()
+        st = lib().pdehip_batch_solve(self.h, reps)
+        if check:
+            self.ctx._check(st)
+        return st, list(reps)
+
+    def download(self, i: int) -> np.ndarray:
+        s = self.specs[i]
+        u = np.zeros(s.n)
+        self.ctx._check(lib().pdehip_batch_download(self.h, i, capi.dptr(u)))
+        return u.reshape(s.shape)
+
+    def bytes(self) -> int:
+        return int(lib().pdehip_batch_bytes(self.h))
+
+    def close(self):
+        if self.h:
+            lib().pdehip_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Plan:
