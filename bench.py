@@ -13,6 +13,7 @@ guess with the operator, hierarchy and right-hand side resident in HBM.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 257] [--no-cpu-baseline]
     python bench.py --only mixed|fp64 [--steps K] [--warmup W]   (one path only, for profiling)
+    python bench.py --workload batch [--steps K] [--count 512]  (512 uneven 2D problems)
     python bench.py --kernels 4,3 [--precision mixed|fp64] [--steps K]
         (K launches of plan kernels only: 0 apply, 1 in-place sweep, 2 residual, 3 copy,
          4 fused sweep; for hardware-counter runs)
@@ -56,6 +57,207 @@ def make_problem(n: int):
     s.bcs = [BC.dirichlet(0.0)] * 4 + [BC.robin(1.0, 1.0, 0.0)] * 2
     s.bc_convention = capi.BCCONV_OUTWARD
     return s
+
+
+def make_batch_problems(count: int = 512, seed: int = 3):
+    """512 independent 2D problems: n = 2^(5+K)+1 with K ~ U{0..5} (log-uniform over
+    33..1025); the first half Poisson -lap u = f, the rest Helmholtz -lap u + c u = f with
+    c ~ U(0, 50); f a sum of 8 Fourier modes sin(pi p x + phi) sin(pi q y + psi) with p, q ~
+    U{1..8}, amplitudes N(0, 1), phases U(0, 2 pi); each face Dirichlet 0 / Neumann 0 /
+    Robin(1, 1, 0) (outward normal) drawn uniformly.  A Poisson problem that drew Neumann on
+    all four faces is singular; it gets the Helmholtz shift c = 1."""
+    from pde_solver_amd import capi
+    from pde_solver_amd.capi import BC
+
+    rng = np.random.default_rng(seed)
+    kinds = [BC.dirichlet(0.0), BC.neumann(0.0), BC.robin(1.0, 1.0, 0.0)]
+    specs = []
+    for t in range(count):
+        n = 2 ** (5 + int(rng.integers(0, 6))) + 1
+        c = 0.0 if t < count // 2 else float(rng.uniform(0.0, 50.0))
+        x = np.linspace(0.0, 1.0, n)
+        f = np.zeros((n, n))
+        for _ in range(8):
+            p, q = rng.integers(1, 9, size=2)
+            amp = rng.standard_normal()
+            phi, psi = rng.uniform(0.0, 2 * np.pi, size=2)
+            f += amp * np.outer(np.sin(np.pi * q * x + psi), np.sin(np.pi * p * x + phi))
+        faces = [kinds[k] for k in rng.integers(0, 3, size=4)]
+        if c == 0.0 and all(fc.kind == capi.NEUMANN for fc in faces):
+            c = 1.0
+        s = capi.Spec(n, n, a=-1.0, b=-1.0, e=c)
+        s.f_field = -f                     # reference sign: L u + f = 0
+        s.bcs = faces + [BC(), BC()]
+        s.bc_convention = capi.BCCONV_OUTWARD
+        specs.append(s)
+    return specs
+
+
+def batch_options(precision: int, tol: float):
+    from pde_solver_amd import capi
+    return capi.options(capi.MGPCG, tol=tol, tol_mode=capi.TOL_REL, precond=capi.PC_MG,
+                        bc_rows=capi.ROWS_FOLDED, precision=precision, mg_pre=2, mg_post=2,
+                        max_iter=200, inner_tol=1e-4)
+
+
+def bench_batch(args) -> int:
+    """Batched uneven sizes: 512 2D problems per step.  Mixed batched (FP32 MG-PCG with FP64
+    residual replacement) vs FP64 batched vs the plain FP64 path (one resident plan solved after
+    the other), errors against an FP64 oracle solve (FP64 batch to rel-resid 1e-12)."""
+    from pde_solver_amd import capi, hip
+
+    ok, note = hip.is_available()
+    log(f"device: {note}")
+    if not ok:
+        raise SystemExit("no usable gfx9 device")
+    t = time.perf_counter()
+    specs = make_batch_problems(args.count)
+    nodes = sum(s.n for s in specs)
+    sizes = {}
+    for s in specs:
+        sizes[s.nx] = sizes.get(s.nx, 0) + 1
+    log(f"{len(specs)} problems, {nodes / 1e6:.1f}M nodes, sizes {dict(sorted(sizes.items()))}, "
+        f"generated in {time.perf_counter() - t:.1f}s")
+    ctx = hip.Context()
+    tuned = capi.BatchConfig(autotune=1, use_graph=1)
+
+    t = time.perf_counter()
+    ref = ctx.batch(specs, batch_options(capi.FP64, 1e-12), tuned)
+    ref.solve()
+    u_ref = [ref.download(i) for i in range(len(specs))]
+    ref.close()
+    log(f"FP64 oracle batch (rel-resid 1e-12) in {time.perf_counter() - t:.1f}s incl. set-up")
+
+    def errs(us):
+        return max(float(np.linalg.norm(u - r) / max(np.linalg.norm(r), 1e-300)) for u, r in zip(us, u_ref))
+
+    def run(precision, tol, fp16, steps, roofline=False):
+        hip.tune_set("mixed.fp16", fp16)
+        try:
+            t0 = time.perf_counter()
+            b = ctx.batch(specs, batch_options(precision, tol), tuned)
+            setup = time.perf_counter() - t0
+        finally:
+            hip.tune_set("mixed.fp16", 1)
+        for _ in range(args.warmup):
+            b.solve()
+        lat, dev = [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            _, reps = b.solve()
+            lat.append((time.perf_counter() - t1) * 1e3)
+            dev.append(reps[0].solve_ms)
+        wall = time.perf_counter() - t0
+        us = [b.download(i) for i in range(len(specs))]
+        out = {"solves_per_s": round(len(specs) * steps / wall, 1),
+               "batch_ms_p50": round(pct(lat, 50), 3), "batch_ms_p99": round(pct(lat, 99), 3),
+               "device_ms_p50": round(pct(dev, 50), 3), "setup_s": round(setup, 2),
+               "max_err_vs_fp64_oracle": errs(us),
+               "max_rel_residual": max(r.rel_residual for r in reps),
+               "all_converged": all(r.status == 0 for r in reps),
+               "resident_GiB": round(b.bytes() / 2**30, 2),
+               "iterations_max": max(r.iterations for r in reps),
+               "inner_iterations_max": max(r.inner_iterations for r in reps)}
+        if roofline:
+            kern = {}
+            for k, name in ((0, "sweep"), (1, "residual"), (2, "apply_dot"), (3, "residual_fp64")):
+                ms, nb = b.time_kernel(k, 10)
+                kern[name] = {"avg_ms": round(ms, 4), "GB/s": round(nb / (ms * 1e-3) / 1e9, 1),
+                              "algorithmic_bytes": nb}
+            out["level0_kernels"] = kern
+        b.close()
+        return out
+
+    mixed = run(capi.MIXED, 1e-8, 0, args.steps, roofline=True)
+    log(f"mixed batched (FP32 coarse): {mixed['solves_per_s']} solves/s, batch p50 "
+        f"{mixed['batch_ms_p50']} ms, max err {mixed['max_err_vs_fp64_oracle']:.2e}")
+    mixed16 = run(capi.MIXED, 1e-8, 1, args.steps)
+    log(f"mixed batched (FP16 coarse): {mixed16['solves_per_s']} solves/s")
+    fp64 = run(capi.FP64, 1e-8, 1, args.steps, roofline=True)
+    log(f"fp64 batched: {fp64['solves_per_s']} solves/s, batch p50 {fp64['batch_ms_p50']} ms")
+    tight_mixed = run(capi.MIXED, 1e-10, 0, max(1, args.steps // 2))
+    tight_fp64 = run(capi.FP64, 1e-10, 1, max(1, args.steps // 2))
+    log(f"at rel-resid 1e-10: mixed {tight_mixed['solves_per_s']} (err "
+        f"{tight_mixed['max_err_vs_fp64_oracle']:.1e}), fp64 {tight_fp64['solves_per_s']} solves/s")
+
+    # plain FP64 path: one resident plan per problem, solved one after the other
+    plans = [ctx.plan(s, batch_options(capi.FP64, 1e-8)) for s in specs]
+    for pl in plans[:8]:
+        pl.solve()
+    lat = []
+    t0 = time.perf_counter()
+    for pl in plans:
+        t1 = time.perf_counter()
+        pl.solve()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    wall_plain = time.perf_counter() - t0
+    err_plain = errs([pl.download() for pl in plans])
+    for pl in plans:
+        pl.close()
+    plain = {"solves_per_s": round(len(specs) / wall_plain, 1),
+             "latency_ms_p50": round(pct(lat, 50), 3), "latency_ms_p99": round(pct(lat, 99), 3),
+             "max_err_vs_fp64_oracle": err_plain}
+    log(f"plain fp64 (sequential resident plans): {plain['solves_per_s']} solves/s")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        threads = max(1, min(threads, 64))
+        O.set_threads(threads)
+        # problems in batch order (the order is random, so the size mix is the batch's) until
+        # about 15 s of CPU work
+        t0 = time.perf_counter()
+        done = 0
+        while done < len(specs) and (time.perf_counter() - t0 < 15.0 or done < 8):
+            O.solve(specs[done], batch_options(capi.FP64, 1e-8))
+            done += 1
+        total = time.perf_counter() - t0
+        cpu = {"value": round(done / total, 2), "unit": "solves/s", "cores": threads,
+               "kind": "port",
+               "sample": f"the first {done} problems of the batch "
+                         f"({sum(s.n for s in specs[:done]) / 1e6:.1f}M nodes), FP64 MG-PCG "
+                         f"to rel 1e-8, one after the other, {total:.1f}s wall"}
+        log(f"cpu baseline: {cpu['value']} solves/s on {threads} threads (sample)")
+
+    sw = mixed["level0_kernels"]["sweep"]
+    out = {
+        "metric": METRIC,
+        "value": mixed["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["batch_ms_p50"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 MG-PCG + f64 residual replacement",
+        "data": "synthetic",
+        "config": {"workload": f"batched uneven sizes: {len(specs)} 2D problems per step, n "
+                               "log-uniform in {33..1025}, half Poisson / half Helmholtz "
+                               "c~U(0,50), 8-mode Fourier RHS, D/N/R per face, MG-PCG V(2,2) "
+                               "to rel-resid 1e-8",
+                   "problems": len(specs), "nodes": nodes},
+        "latency_ms": {"p50": mixed["batch_ms_p50"], "p99": mixed["batch_ms_p99"]},
+        "roofline": {"bound": "hbm", "kernel": "kb_sweep<P32> (red+black, one launch) over every "
+                                               "problem's level 0",
+                     "achieved": sw["GB/s"], "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(sw["GB/s"] / 8000.0, 3), "traffic": None,
+                     "algorithmic_bytes": sw["algorithmic_bytes"], "avg_ms": sw["avg_ms"]},
+        "mixed_batched": mixed,
+        "mixed_batched_fp16_coarse": mixed16,
+        "fp64_batched": fp64,
+        "plain_fp64": plain,
+        "at_rel_resid_1e-10": {"mixed_batched": tight_mixed, "fp64_batched": tight_fp64},
+        "speedup_mixed_batched_vs_plain_fp64": round(mixed["solves_per_s"] / plain["solves_per_s"], 2),
+        "speedup_mixed_batched_vs_fp64_batched": round(mixed["solves_per_s"] / fp64["solves_per_s"], 2),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
 
 
 def options(precision: int, tol: float):
@@ -106,11 +308,17 @@ def main() -> int:
     ap.add_argument("--kernels", default=None,
                     help="comma list of plan kernels to launch K times each (counter runs)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--workload", choices=["single", "batch"], default="single",
+                    help="single: the 257^3 solve (default); batch: 512 uneven 2D problems")
+    ap.add_argument("--count", type=int, default=512, help="batch workload: problems per step")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
               "(--gpus must be 1)", file=sys.stderr)
         return 2
+
+    if args.workload == "batch":
+        return bench_batch(args)
 
     from pde_solver_amd import capi, hip
 

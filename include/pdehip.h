@@ -190,6 +190,7 @@ int pdehip_abi_version(void);
 int pdehip_is_available(char* note, int note_len);
 /* hbm_budget_bytes: admission/allocation cap for this context (0 = 80% of free device memory). */
 int pdehip_create(int device, size_t hbm_budget_bytes, pdehip_ctx** out);
+/* Plans, batches and servers created on ctx must be destroyed before it. */
 void pdehip_destroy(pdehip_ctx* ctx);
 const char* pdehip_last_error(const pdehip_ctx* ctx);
 
@@ -228,8 +229,10 @@ void pdehip_plan_destroy(pdehip_plan* plan);
  * planes per 3D workgroup, 4..512, or 0 to choose from occupancy), "smooth.ychunk" (2D rows
  * per wavefront, 16..4096), "stencil.plane" (0/1: 3D apply and residual as plane-march
  * kernels), "smooth.pair" (0/1: 3D FP32/FP16 sweeps with column-pair tiles, no colour
- * divergence).
- * Results do not depend on any of them.  Not thread-safe against running solves. */
+ * divergence).  Results do not depend on any of these.
+ * "mixed.fp16" (0/1, default 1): mixed-precision plans created afterwards store their coarse
+ * multigrid levels in FP16 (1) or FP32 (0); this one changes the preconditioner's rounding.
+ * Not thread-safe against running solves. */
 int pdehip_tune_set(const char* key, int value);
 int pdehip_tune_get(const char* key, int* value);
 
@@ -256,11 +259,16 @@ int pdehip_time_series(pdehip_ctx* ctx, const pdehip_problem* prob, const pdehip
 /* Solve `count` independent problems in one persistent launch (MG-CG, 2D CONST/FLUX,
  * FOLDED rows).  u_out[i] receives problem i.  reports may be NULL. */
 typedef struct pdehip_batch_config {
-  int precision;          /* PDEHIP_FP64 or PDEHIP_MIXED */
-  int use_graph;          /* capture the launch sequence in a hipGraph */
-  int autotune;           /* 1: pick tile/team shape by timing candidates (cached per context) */
-  int tile_x, tile_y;     /* explicit tile shape when autotune == 0 (0 = default) */
-  int waves_per_cu;       /* persistent grid size = CUs * waves_per_cu (0 = default) */
+  int precision;          /* PDEHIP_FP64 or PDEHIP_MIXED (pdehip_solve_batch; the resident
+                             batch takes the precision from its options) */
+  int use_graph;          /* 1: the launch sequence of two PCG iterations is captured once in a
+                             hipGraph and replayed */
+  int autotune;           /* 1: per-level work-item shapes picked by timing candidates on the
+                             batch's own buffers (cached per context by the batch's sizes) */
+  int tile_x;             /* nodes per row block of the residual/prolongation/restriction
+                             kernels (whole rows; 0 = default 256/128) */
+  int tile_y;             /* smoother strip height in rows (0 = by level size) */
+  int waves_per_cu;       /* reserved, 0 */
 } pdehip_batch_config;
 int pdehip_solve_batch(pdehip_ctx* ctx, int count, const pdehip_problem* probs,
                        const pdehip_options* opt, const pdehip_batch_config* cfg,
@@ -269,7 +277,8 @@ int pdehip_solve_batch(pdehip_ctx* ctx, int count, const pdehip_problem* probs,
  * solved together in lock step (2D constant-coefficient MG-PCG, FP64 or mixed precision): each
  * operation of an iteration is one launch over every unfinished problem.  No reference
  * counterpart (the reference runs a batch as sequential solves, src/main.cpp:2071-2097).
- * pdehip_batch_solve fills count reports; download copies problem `index`'s solution. */
+ * pdehip_batch_solve fills count reports and returns PDEHIP_OK when every problem converged,
+ * else the status of the first one that did not; download copies problem `index`'s solution. */
 typedef struct pdehip_batch pdehip_batch;
 int pdehip_batch_create(pdehip_ctx* ctx, int count, const pdehip_problem* probs,
                         const pdehip_options* opt, const pdehip_batch_config* cfg,
@@ -277,6 +286,12 @@ int pdehip_batch_create(pdehip_ctx* ctx, int count, const pdehip_problem* probs,
 int pdehip_batch_solve(pdehip_batch* batch, pdehip_report* reports);
 int pdehip_batch_download(pdehip_batch* batch, int index, double* u_out);
 size_t pdehip_batch_bytes(const pdehip_batch* batch);
+/* Measurement: mean duration of one batched level-0 launch over every problem, timed with HIP
+ * events on the batch stream.  kernel: 0 smoother red+black sweep, 1 residual, 2 Krylov
+ * apply + dot, 3 FP64 residual.  algorithmic_bytes (may be NULL): compulsory bytes per launch.
+ * Overwrites the batch's work vectors and scalars; the next pdehip_batch_solve starts afresh. */
+int pdehip_batch_time_kernel(pdehip_batch* batch, int kernel, int reps, double* avg_ms,
+                             double* algorithmic_bytes);
 void pdehip_batch_destroy(pdehip_batch* batch);
 
 /* ---------------------------------------------------------------- serving */

@@ -17,6 +17,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <string>
 
 namespace pdehip {
 
@@ -42,6 +44,7 @@ void upload_list(Ctx* ctx, hipStream_t s, List& L, const std::vector<BItem>& v,
     HIPCHK(hipMemcpyAsync(L.first.get(), first->data(), first->size() * sizeof(int), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(L.cnt.get(), cnt->data(), cnt->size() * sizeof(int), hipMemcpyHostToDevice, s));
   }
+  HIPCHK(hipStreamSynchronize(s));   // the host vectors may be temporaries
 }
 
 }  // namespace
@@ -60,20 +63,192 @@ struct BatchPlan {
   DBuf dense;                              // BDense [n]
   std::vector<List> sweep, rows, crows, drows;   // per level
   List el0, rows0;                         // level-0 element / row items (with ranges)
+  std::vector<LevelShape> shape;           // per level work-item shapes
   DBuf part, maxp;
   DBuf active;
   int* hactive = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};   // captured PCG iterations (FP64 / inner)
   size_t bytes = 0;                        // device memory held by the batch and its plans
   ~BatchPlan() {
+    for (hipGraphExec_t g : graph)
+      if (g) (void)hipGraphExecDestroy(g);
     if (hactive) (void)hipHostFree(hactive);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
   }
   const BEnt* T(int l) const { return tab.as<BEnt>() + (size_t)l * n; }
   const BVec* V() const { return vecs.as<BVec>(); }
-  Prec prec(int l) const { return mixed ? (l == 0 ? P32 : P16) : P64; }
+  Prec coarse = P16;                       // mixed: storage of levels >= 1 (from the plans)
+  Prec prec(int l) const { return mixed ? (l == 0 ? P32 : coarse) : P64; }
 };
+
+namespace {
+
+// Work-item shapes.  Smoother items are 60-column strips of `ychunk` rows (a wavefront marches
+// the rows, so short chunks trade ring re-loads for more items in flight); row items cover
+// whole rows, about `row_nodes` nodes each (at least one row); restriction items cover about
+// `crow_nodes` coarse nodes.
+LevelShape default_shape(const BatchPlan& B, int l) {
+  int maxny = 0;
+  for (int p = 0; p < B.n; ++p)
+    if (l < B.levels[p]) maxny = std::max(maxny, B.plans[p]->L[l].g.ny);
+  LevelShape sh;
+  sh.ychunk = maxny <= 65 ? 8 : (maxny <= 257 ? 16 : 32);
+  sh.row_nodes = 256;
+  sh.crow_nodes = 128;
+  return sh;
+}
+
+std::vector<BItem> sweep_items(const BatchPlan& B, int l, int ychunk) {
+  std::vector<BItem> v;
+  const int strip = 60;
+  for (int p = 0; p < B.n; ++p) {
+    if (l >= B.levels[p] - 1) continue;            // coarsest level: dense solve
+    const Geo& g = B.plans[p]->L[l].g;
+    for (int sx = 0; sx * strip < g.nx; ++sx)
+      for (int j0 = 0; j0 < g.ny; j0 += ychunk) v.push_back(BItem{p, sx, j0, std::min(g.ny, j0 + ychunk)});
+  }
+  return v;
+}
+
+// Row blocks of level l.  coarse_side: the level is the coarse side of a restriction (only
+// problems whose hierarchy reaches it from above); otherwise problems that smooth on level l.
+// first/cnt: per-problem item ranges for the segmented reductions.
+std::vector<BItem> row_items(const BatchPlan& B, int l, int nodes, bool coarse_side,
+                             std::vector<int>* first = nullptr, std::vector<int>* cnt = nullptr) {
+  std::vector<BItem> v;
+  if (first) { first->assign(B.n, 0); cnt->assign(B.n, 0); }
+  for (int p = 0; p < B.n; ++p) {
+    if (first) (*first)[p] = static_cast<int>(v.size());
+    const bool has = coarse_side ? (l >= 1 && l < B.levels[p]) : (l < B.levels[p] - 1 || (first && l == 0));
+    if (has) {
+      const Geo& g = B.plans[p]->L[l].g;
+      const int rc = std::max(1, nodes / g.nx);
+      for (int j0 = 0; j0 < g.ny; j0 += rc) v.push_back(BItem{p, j0, std::min(g.ny, j0 + rc), 0});
+    }
+    if (first) (*cnt)[p] = static_cast<int>(v.size()) - (*first)[p];
+  }
+  return v;
+}
+
+std::vector<BItem> dense_items(const BatchPlan& B, int l, const std::vector<bk::BDense>& dense) {
+  std::vector<BItem> v;
+  for (int p = 0; p < B.n; ++p)
+    if (l == B.levels[p] - 1)
+      for (int r = 0; r < dense[p].m; ++r) v.push_back(BItem{p, r, 0, 0});
+  return v;
+}
+
+// Mean time of `reps` launches of f (after one warm-up) on the batch stream.
+template <typename F>
+float time_ms(BatchPlan& B, F&& f, int reps) {
+  const hipStream_t s = B.ctx->stream;
+  f();
+  HIPCHK(hipEventRecord(B.ev0, s));
+  for (int r = 0; r < reps; ++r) f();
+  HIPCHK(hipEventRecord(B.ev1, s));
+  HIPCHK(hipEventSynchronize(B.ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, B.ev0, B.ev1));
+  return ms / reps;
+}
+
+std::string shape_key(const BatchPlan& B) {
+  std::map<std::pair<int, int>, int> hist;
+  for (const auto& P : B.plans) hist[{P->g.nx, P->g.ny}] += 1;
+  std::string key = B.mixed ? "mixed" : "fp64";
+  for (const auto& kv : hist)
+    key += ":" + std::to_string(kv.first.first) + "x" + std::to_string(kv.first.second) + "*" +
+           std::to_string(kv.second);
+  return key;
+}
+
+void upload_shapes(BatchPlan& B, const std::vector<bk::BDense>* dense) {
+  Ctx* ctx = B.ctx;
+  const hipStream_t s = ctx->stream;
+  for (int l = 0; l < B.L; ++l) {
+    upload_list(ctx, s, B.sweep[l], sweep_items(B, l, B.shape[l].ychunk));
+    upload_list(ctx, s, B.rows[l], row_items(B, l, B.shape[l].row_nodes, false));
+    if (l + 1 < B.L)
+      upload_list(ctx, s, B.crows[l + 1], row_items(B, l + 1, B.shape[l + 1].crow_nodes, true));
+    if (dense) upload_list(ctx, s, B.drows[l], dense_items(B, l, *dense));
+  }
+}
+
+// Autotuner: for every level, time the smoother sweep pair over each candidate strip height,
+// the residual + prolongation over each row-block size and the restriction over each coarse
+// block size, on this batch's own operators and buffers with every problem active, and keep
+// the fastest.  Results are cached in the context under the batch's size histogram.  The
+// level buffers hold zeros here, and every timed kernel maps zeros to zeros.
+void autotune(BatchPlan& B) {
+  Ctx* ctx = B.ctx;
+  const hipStream_t s = ctx->stream;
+  const std::string key = shape_key(B);
+  {
+    std::lock_guard<std::mutex> g(ctx->tune_mu);
+    auto it = ctx->batch_shapes.find(key);
+    if (it != ctx->batch_shapes.end() && static_cast<int>(it->second.size()) == B.L * 3) {
+      for (int l = 0; l < B.L; ++l)
+        B.shape[l] = LevelShape{it->second[3 * l], it->second[3 * l + 1], it->second[3 * l + 2]};
+      upload_shapes(B, nullptr);
+      return;
+    }
+  }
+  bk::init_scalars(s, B.V(), B.n, 1e-300);
+  const int reps = 3;
+  for (int l = 0; l < B.L; ++l) {
+    const Prec pl = B.prec(l);
+    if (B.sweep[l].n > 0) {
+      float best = 1e30f;
+      int pick = B.shape[l].ychunk;
+      for (int yc : {8, 16, 32, 64}) {
+        List tmp;
+        upload_list(ctx, s, tmp, sweep_items(B, l, yc));
+        const float t = time_ms(B, [&] {
+          bk::sweep(s, pl, B.T(l), tmp.d(), tmp.n, 0, 0, false);
+          bk::sweep(s, pl, B.T(l), tmp.d(), tmp.n, 0, 1, false);
+        }, reps);
+        if (t < best) { best = t; pick = yc; }
+      }
+      B.shape[l].ychunk = pick;
+    }
+    if (l + 1 < B.L && B.rows[l].n > 0) {
+      const Prec pc = B.prec(l + 1);
+      const bool s0 = B.mixed && l == 0;
+      float best = 1e30f;
+      int pick = B.shape[l].row_nodes;
+      for (int rn : {64, 256, 1024, 4096}) {
+        List tmp;
+        upload_list(ctx, s, tmp, row_items(B, l, rn, false));
+        const float t = time_ms(B, [&] {
+          bk::residual(s, pl, B.T(l), tmp.d(), tmp.n);
+          bk::prolong(s, pc, pl, B.T(l + 1), B.T(l), tmp.d(), tmp.n, s0);
+        }, reps);
+        if (t < best) { best = t; pick = rn; }
+      }
+      B.shape[l].row_nodes = pick;
+      best = 1e30f;
+      pick = B.shape[l + 1].crow_nodes;
+      for (int cn : {32, 128, 512, 2048}) {
+        List tmp;
+        upload_list(ctx, s, tmp, row_items(B, l + 1, cn, true));
+        const float t = time_ms(B, [&] {
+          bk::restrict_(s, pl, pc, B.T(l), B.T(l + 1), tmp.d(), tmp.n, s0);
+        }, reps);
+        if (t < best) { best = t; pick = cn; }
+      }
+      B.shape[l + 1].crow_nodes = pick;
+    }
+  }
+  upload_shapes(B, nullptr);
+  std::vector<int> flat;
+  for (const LevelShape& sh : B.shape) { flat.push_back(sh.ychunk); flat.push_back(sh.row_nodes); flat.push_back(sh.crow_nodes); }
+  std::lock_guard<std::mutex> g(ctx->tune_mu);
+  ctx->batch_shapes[key] = flat;
+}
+
+}  // namespace
 
 bool batch_eligible(int count, const pdehip_problem* probs, const pdehip_options& opt,
                     std::string* why) {
@@ -113,6 +288,7 @@ BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pd
       throw Error(PDEHIP_ERR_UNSUPPORTED, "batched engine needs a dense coarsest level");
     B->levels.push_back(nl);
     B->L = std::max(B->L, nl);
+    B->coarse = P.L[1].prec;
   }
   const int n = count, L = B->L;
 
@@ -168,48 +344,32 @@ BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pd
   HIPCHK(hipMemcpyAsync(B->vecs.get(), vecs.data(), vecs.size() * sizeof(BVec), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(B->dense.get(), dense.data(), dense.size() * sizeof(bk::BDense), hipMemcpyHostToDevice, s));
 
-  // ---- work items
-  const int strip = 60;
-  const int ychunk = B->cfg.tile_y > 0 ? B->cfg.tile_y : 64;
-  B->sweep.resize(L); B->rows.resize(L); B->crows.resize(L); B->drows.resize(L);
-  for (int l = 0; l < L; ++l) {
-    std::vector<BItem> sw, rw, cw, dw;
-    for (int p = 0; p < n; ++p) {
-      const int nl = B->levels[p];
-      if (l >= nl) continue;
-      const Geo& g = B->plans[p]->L[l].g;
-      if (l == nl - 1) {
-        for (int r = 0; r < dense[p].m; ++r) dw.push_back(BItem{p, r, 0, 0});
-        continue;
-      }
-      for (int sx = 0; sx * strip < g.nx; ++sx)
-        for (int j0 = 0; j0 < g.ny; j0 += ychunk) sw.push_back(BItem{p, sx, j0, std::min(g.ny, j0 + ychunk)});
-      const int rc = std::max(1, 4096 / g.nx);
-      for (int j0 = 0; j0 < g.ny; j0 += rc) rw.push_back(BItem{p, j0, std::min(g.ny, j0 + rc), 0});
-      const Geo& gc = B->plans[p]->L[l + 1].g;
-      const int rcc = std::max(1, 4096 / gc.nx);
-      for (int j0 = 0; j0 < gc.ny; j0 += rcc) cw.push_back(BItem{p, j0, std::min(gc.ny, j0 + rcc), 0});
+  // ---- work items (shapes per level: defaults, explicit tile, or timed candidates)
+  B->shape.assign(L, LevelShape{});
+  for (int l = 0; l < L; ++l) B->shape[l] = default_shape(*B, l);
+  if (B->cfg.tile_y > 0 || B->cfg.tile_x > 0)
+    for (LevelShape& sh : B->shape) {
+      if (B->cfg.tile_y > 0) sh.ychunk = B->cfg.tile_y;
+      if (B->cfg.tile_x > 0) sh.row_nodes = sh.crow_nodes = B->cfg.tile_x;
     }
-    upload_list(ctx, s, B->sweep[l], sw);
-    upload_list(ctx, s, B->rows[l], rw);
-    if (l + 1 < L) upload_list(ctx, s, B->crows[l + 1], cw);
-    upload_list(ctx, s, B->drows[l], dw);
-  }
+  B->sweep.resize(L); B->rows.resize(L); B->crows.resize(L); B->drows.resize(L);
+  HIPCHK(hipEventCreate(&B->ev0));
+  HIPCHK(hipEventCreate(&B->ev1));
+  upload_shapes(*B, &dense);
+  if (B->cfg.autotune) autotune(*B);
   {
-    std::vector<BItem> el, rw;
-    std::vector<int> ef(n), ec(n), rf(n), rcn(n);
+    std::vector<BItem> el;
+    std::vector<int> ef(n), ec(n);
     for (int p = 0; p < n; ++p) {
       const Geo& g = B->plans[p]->g;
       ef[p] = static_cast<int>(el.size());
       for (long long e0 = 0; e0 < g.n; e0 += 4096)
         el.push_back(BItem{p, static_cast<int>(e0), static_cast<int>(std::min(g.n, e0 + 4096)), 0});
       ec[p] = static_cast<int>(el.size()) - ef[p];
-      rf[p] = static_cast<int>(rw.size());
-      const int rc = std::max(1, 4096 / g.nx);
-      for (int j0 = 0; j0 < g.ny; j0 += rc) rw.push_back(BItem{p, j0, std::min(g.ny, j0 + rc), 0});
-      rcn[p] = static_cast<int>(rw.size()) - rf[p];
     }
     upload_list(ctx, s, B->el0, el, &ef, &ec);
+    std::vector<int> rf, rcn;
+    const std::vector<BItem> rw = row_items(*B, 0, B->shape[0].row_nodes, false, &rf, &rcn);
     upload_list(ctx, s, B->rows0, rw, &rf, &rcn);
   }
   const int np = std::max(B->el0.n, B->rows0.n);
@@ -217,8 +377,6 @@ BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pd
   B->maxp.alloc(ctx, (size_t)np * sizeof(double));
   B->active.alloc(ctx, sizeof(int));
   HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&B->hactive), sizeof(int)));
-  HIPCHK(hipEventCreate(&B->ev0));
-  HIPCHK(hipEventCreate(&B->ev1));
   HIPCHK(hipStreamSynchronize(s));
   B->bytes = ctx->used.load() - used0;
   return B;
@@ -264,8 +422,11 @@ int active(BatchPlan& B, int inner) {
 }
 
 // PCG on every problem in lock step.  Krylov precision kp (P64: the FP64 path on the outer
-// scalars; P32: the inner solve of a refinement on the inner scalars).
-void pcg(BatchPlan& B, Prec kp, int max_iter) {
+// scalars; P32: mixed precision on the inner scalars, with the correction accumulated in the
+// FP64 solution and the FP32 residual replaced by the FP64 true residual whenever it has
+// dropped by `delta` since the last replacement or claims convergence; only a true residual
+// stops a problem).
+void pcg(BatchPlan& B, Prec kp, int max_iter, double delta) {
   const hipStream_t s = B.ctx->stream;
   const int inner = kp == P32 ? 1 : 0;
   double* part = B.part.as<double>();
@@ -280,22 +441,56 @@ void pcg(BatchPlan& B, Prec kp, int max_iter) {
   fin(el, FIN_CG_INIT, nullptr);
   bk::vec(s, kp, bk::V_COPY_ZP, B.V(), el.d(), el.n, nullptr, nullptr);
   const int every = 2;
-  for (int it = 0; it < max_iter; ++it) {
-    bk::apply_dot_tab(s, kp, B.V(), B.T(0), rw.d(), rw.n, part, false);
-    fin(rw, FIN_CG_ALPHA, nullptr);
-    bk::vec(s, kp, bk::V_CG_UPDATE, B.V(), el.d(), el.n, part, maxp);
-    fin(el, FIN_CG_RR, maxp);
-    vcycle(B);
-    bk::vec(s, kp, bk::V_DOT_RZ, B.V(), el.d(), el.n, part, nullptr);
-    fin(el, FIN_CG_BETA, nullptr);
-    bk::vec(s, kp, bk::V_XPAY, B.V(), el.d(), el.n, nullptr, nullptr);
-    if ((it + 1) % every == 0 && active(B, inner) == 0) break;
+  auto iterations = [&] {
+    for (int c = 0; c < every; ++c) {
+      bk::apply_dot_tab(s, kp, B.V(), B.T(0), rw.d(), rw.n, part, false);
+      fin(rw, FIN_CG_ALPHA, nullptr);
+      bk::vec(s, kp, bk::V_CG_UPDATE, B.V(), el.d(), el.n, part, maxp);
+      if (inner) {
+        bk::finalize(s, B.V(), el.first.as<int>(), el.cnt.as<int>(), B.n, part, maxp,
+                     FIN_RU_CHECK, 1, delta, 0);
+        bk::replace_residual(s, B.V(), rw.d(), rw.n, part);
+        fin(rw, FIN_RU_APPLY, nullptr);
+      } else {
+        fin(el, FIN_CG_RR, maxp);
+      }
+      vcycle(B);
+      bk::vec(s, kp, bk::V_DOT_RZ, B.V(), el.d(), el.n, part, nullptr);
+      fin(el, FIN_CG_BETA, nullptr);
+      bk::vec(s, kp, bk::V_XPAY, B.V(), el.d(), el.n, nullptr, nullptr);
+    }
+  };
+  for (int it = 0; it < max_iter; it += every) {
+    if (B.cfg.use_graph) {
+      // the launch sequence of `every` iterations, captured once per Krylov precision (every
+      // pointer and item list is resident, so the graph stays valid for the batch's life)
+      hipGraphExec_t& ge = B.graph[inner];
+      if (!ge) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          iterations();
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIPCHK(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(e);
+      }
+      HIPCHK(hipGraphLaunch(ge, s));
+    } else {
+      iterations();
+    }
+    if (active(B, inner) == 0) break;
   }
 }
 
 }  // namespace
 
-void batch_solve(BatchPlan& B, pdehip_report* reports) {
+int batch_solve(BatchPlan& B, pdehip_report* reports) {
   const hipStream_t s = B.ctx->stream;
   const int n = B.n;
   double* part = B.part.as<double>();
@@ -309,29 +504,23 @@ void batch_solve(BatchPlan& B, pdehip_report* reports) {
   bk::finalize(s, B.V(), rw.first.as<int>(), rw.cnt.as<int>(), n, part, nullptr, FIN_R0, 0,
                B.opt.tol, B.opt.tol_mode);
   if (!B.mixed) {
-    pcg(B, P64, max_iter);
+    pcg(B, P64, max_iter, 0.0);
   } else {
-    const double inner_tol = B.opt.inner_tol > 0 ? B.opt.inner_tol : 1e-4;
-    for (int ref = 0; ref < 30; ++ref) {
-      if (active(B, 0) == 0) break;
-      bk::reset_inner(s, B.V(), n);
-      bk::vec(s, P32, bk::V_TO32, B.V(), el.d(), el.n, nullptr, nullptr);
-      bk::vec(s, P32, bk::V_RR, B.V(), el.d(), el.n, part, B.maxp.as<double>());
-      bk::finalize(s, B.V(), el.first.as<int>(), el.cnt.as<int>(), n, part, B.maxp.as<double>(),
-                   FIN_R0, 1, inner_tol, PDEHIP_TOL_REL);
-      pcg(B, P32, max_iter);
-      bk::vec(s, P32, bk::V_ADD32, B.V(), el.d(), el.n, nullptr, nullptr);
-      bk::apply_dot_tab(s, P64, B.V(), B.T(0), rw.d(), rw.n, part, true);
-      bk::finalize(s, B.V(), rw.first.as<int>(), rw.cnt.as<int>(), n, part, nullptr, FIN_CG_RR,
-                   0, 0.0, 0);
-    }
-    bk::reset_inner(s, B.V(), n);   // folds the last refinement's inner iterations in
+    // inner scale fixed at |r0|; inner stop only through replacements (target 0 here)
+    const double delta = B.opt.inner_tol > 0 ? B.opt.inner_tol : 1e-2;
+    bk::ru_start(s, B.V(), n);
+    bk::vec(s, P32, bk::V_TO32, B.V(), el.d(), el.n, nullptr, nullptr);
+    bk::vec(s, P32, bk::V_RR, B.V(), el.d(), el.n, part, B.maxp.as<double>());
+    bk::finalize(s, B.V(), el.first.as<int>(), el.cnt.as<int>(), n, part, B.maxp.as<double>(),
+                 FIN_R0, 1, 0.0, PDEHIP_TOL_ABS);
+    pcg(B, P32, max_iter, delta);
   }
   HIPCHK(hipEventRecord(B.ev1, s));
   for (auto& P : B.plans) plan_finish_enqueue(*P);
   HIPCHK(hipStreamSynchronize(s));
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, B.ev0, B.ev1));
+  int status = PDEHIP_OK;
   for (int p = 0; p < n; ++p) {
     Plan& P = *B.plans[p];
     pdehip_report rep{};
@@ -348,8 +537,49 @@ void batch_solve(BatchPlan& B, pdehip_report* reports) {
     }
     rep.final_metric = k.metric;
     rep.status = k.breakdown ? PDEHIP_ERR_BREAKDOWN : (k.done ? PDEHIP_OK : PDEHIP_ERR_NOT_CONVERGED);
+    if (rep.status == PDEHIP_OK && !std::isfinite(rep.rel_residual)) rep.status = PDEHIP_ERR_BREAKDOWN;
+    if (rep.status != PDEHIP_OK && status == PDEHIP_OK) status = rep.status;
     if (reports) reports[p] = rep;
   }
+  return status;
+}
+
+double batch_time_kernel(BatchPlan& B, int kernel, int reps, double* algorithmic_bytes) {
+  const hipStream_t s = B.ctx->stream;
+  const Prec p0 = B.prec(0);
+  const Prec kp = B.mixed ? P32 : P64;
+  const double es = p0 == P64 ? 8.0 : 4.0;
+  long long n0 = 0;
+  for (const auto& P : B.plans) n0 += P->g.n;
+  bk::init_scalars(s, B.V(), B.n, 1e-300);   // every problem active (the next solve resets them)
+  double bytes = 0.0;
+  float ms = 0.f;
+  double* part = B.part.as<double>();
+  switch (kernel) {
+    case 0:   // level-0 smoother: one red + black sweep, out of place (read u, b; write alt)
+      bytes = 3.0 * es * static_cast<double>(n0);
+      ms = time_ms(B, [&] { bk::sweep(s, p0, B.T(0), B.sweep[0].d(), B.sweep[0].n, 0, 0, false); },
+                   std::max(1, reps));
+      break;
+    case 1:   // level-0 residual r = b - A u
+      bytes = 3.0 * es * static_cast<double>(n0);
+      ms = time_ms(B, [&] { bk::residual(s, p0, B.T(0), B.rows[0].d(), B.rows[0].n); }, std::max(1, reps));
+      break;
+    case 2:   // Krylov apply q = A p with p.q partials
+      bytes = 2.0 * (kp == P64 ? 8.0 : 4.0) * static_cast<double>(n0);
+      ms = time_ms(B, [&] { bk::apply_dot_tab(s, kp, B.V(), B.T(0), B.rows0.d(), B.rows0.n, part, false); },
+                   std::max(1, reps));
+      break;
+    case 3:   // FP64 outer residual r64 = b64 - A64 x64
+      bytes = 24.0 * static_cast<double>(n0);
+      ms = time_ms(B, [&] { bk::apply_dot_tab(s, P64, B.V(), B.T(0), B.rows0.d(), B.rows0.n, part, true); },
+                   std::max(1, reps));
+      break;
+    default:
+      throw Error(PDEHIP_ERR_INVALID, "batch_time_kernel: kernel 0..3");
+  }
+  if (algorithmic_bytes) *algorithmic_bytes = bytes;
+  return ms;
 }
 
 void batch_download(BatchPlan& B, int i, double* u_out) { plan_download(*B.plans.at(i), u_out); }

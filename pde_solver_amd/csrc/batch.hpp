@@ -44,13 +44,22 @@ struct BVec {
 // Work item: problem p and a range (meaning depends on the list).
 struct BItem { int p, a, b, c; };
 
+// Work-item shape of one level: smoother strip height (rows), nodes per residual/prolongation
+// row block, coarse nodes per restriction block (whole rows, at least one).
+struct LevelShape { int ychunk = 16; int row_nodes = 256; int crow_nodes = 128; };
+
 struct BatchPlan;
 struct BatchDeleter { void operator()(BatchPlan* b) const; };
 using BatchPtr = std::unique_ptr<BatchPlan, BatchDeleter>;
 BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pdehip_options& opt,
                       const pdehip_batch_config* cfg);
-void batch_solve(BatchPlan& B, pdehip_report* reports);
+// Returns PDEHIP_OK, or the status of the first problem that did not converge.
+int batch_solve(BatchPlan& B, pdehip_report* reports);
 void batch_download(BatchPlan& B, int i, double* u_out);
+// Mean duration (ms) of one launch of a batched level-0 kernel over every problem (HIP events
+// on the batch stream): 0 smoother sweep pair, 1 residual, 2 Krylov apply + dot, 3 FP64
+// residual.  algorithmic_bytes: compulsory bytes of one launch.
+double batch_time_kernel(BatchPlan& B, int kernel, int reps, double* algorithmic_bytes);
 size_t batch_bytes(const BatchPlan& B);
 void batch_destroy(BatchPtr& B);
 // True when the batched engine handles this set (2D, constant-coefficient operators, MG-PCG).
@@ -76,8 +85,9 @@ struct BDense { const double* ainv; const int* map; int m; int pad; };
 void coarse_dense(hipStream_t s, Prec p, const BEnt* tab, const BDense* dense, const BItem* items,
                   int n);
 // Level-0 Krylov vector kernels over (problem, element range) items; partials per item.
+// V_CG_UPDATE of an inner (mixed) solve adds |r64| alpha p to the FP64 solution directly.
 enum VecOp { V_APPLY_DOT = 0, V_CG_UPDATE = 1, V_DOT_RZ = 2, V_XPAY = 3, V_RESID64 = 4,
-             V_TO32 = 5, V_ADD32 = 6, V_COPY_ZP = 7, V_ZERO_X = 8, V_COPY_X0 = 9, V_RR = 10 };
+             V_TO32 = 5, V_COPY_ZP = 7, V_ZERO_X = 8, V_COPY_X0 = 9, V_RR = 10 };
 // (V_APPLY_DOT and V_RESID64 are row kernels: apply_dot_tab.)
 void vec(hipStream_t s, Prec p, int op, const BVec* vecs, const BItem* items, int n,
          double* partials, double* maxpart);
@@ -87,6 +97,11 @@ void apply_dot_tab(hipStream_t s, Prec p, const BVec* vecs, const BEnt* tab0, co
                    int n, double* partials, bool outer);
 // Inner scalars reset for a refinement (outer metric kept as the correction scale).
 void reset_inner(hipStream_t s, const BVec* vecs, int nprob);
+// Mixed solve with residual replacement: inner scalars fresh, inner scale = initial residual.
+void ru_start(hipStream_t s, const BVec* vecs, int nprob);
+// Residual replacement over level-0 row items for problems whose inner scalars flag it:
+// r32 = (b64 - A64 x64) / snorm, partials |b - A x|^2.
+void replace_residual(hipStream_t s, const BVec* vecs, const BItem* rows, int n, double* partials);
 void init_scalars(hipStream_t s, const BVec* vecs, int nprob, double eps);
 // Segmented finalize: problem q sums partials[first[q] .. first[q]+cnt[q]) and applies FinOp
 // `op` to its scalars (inner when inner != 0).

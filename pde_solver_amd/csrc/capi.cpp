@@ -252,6 +252,7 @@ int pdehip_tune_set(const char* key, int value) {
   else if (k == "smooth.ychunk" && value >= 16 && value <= 4096) t.ychunk = value;
   else if (k == "smooth.pair" && (value == 0 || value == 1)) t.pair = value;
   else if (k == "stencil.plane" && (value == 0 || value == 1)) { set_plane_stencils(value == 1); return PDEHIP_OK; }
+  else if (k == "mixed.fp16" && (value == 0 || value == 1)) { set_mixed_fp16(value == 1); return PDEHIP_OK; }
   else return PDEHIP_ERR_INVALID;
   set_smooth_tuning(t);
   return PDEHIP_OK;
@@ -267,6 +268,7 @@ int pdehip_tune_get(const char* key, int* value) {
   else if (k == "smooth.ychunk") *value = t.ychunk;
   else if (k == "smooth.pair") *value = t.pair;
   else if (k == "stencil.plane") *value = plane_stencils() ? 1 : 0;
+  else if (k == "mixed.fp16") *value = mixed_fp16() ? 1 : 0;
   else return PDEHIP_ERR_INVALID;
   return PDEHIP_OK;
 }
@@ -329,8 +331,7 @@ int pdehip_batch_solve(pdehip_batch* b, pdehip_report* reports) {
   return guarded(b->owner, [&]() {
     std::lock_guard<std::mutex> g(b->owner->c.mu);
     HIPCHK(hipSetDevice(b->owner->c.device));
-    batch_solve(*b->b, reports);
-    return PDEHIP_OK;
+    return batch_solve(*b->b, reports);
   });
 }
 
@@ -344,6 +345,17 @@ int pdehip_batch_download(pdehip_batch* b, int index, double* u_out) {
 }
 
 size_t pdehip_batch_bytes(const pdehip_batch* b) { return b ? batch_bytes(*b->b) : 0; }
+
+int pdehip_batch_time_kernel(pdehip_batch* b, int kernel, int reps, double* avg_ms,
+                             double* algorithmic_bytes) {
+  if (!b || !avg_ms) return PDEHIP_ERR_INVALID;
+  return guarded(b->owner, [&]() {
+    std::lock_guard<std::mutex> g(b->owner->c.mu);
+    HIPCHK(hipSetDevice(b->owner->c.device));
+    *avg_ms = batch_time_kernel(*b->b, kernel, reps, algorithmic_bytes);
+    return PDEHIP_OK;
+  });
+}
 
 void pdehip_batch_destroy(pdehip_batch* b) {
   if (!b) return;

@@ -31,6 +31,7 @@ struct DevOp {
   double fx, fy, fz;     // FLUX: a/h^2 per axis
   double e;              // FLUX: constant reaction term
   double theta[6];       // folded closure factors (0 = not folded / Dirichlet)
+  double phi[6];         // 1 - theta, formed in FP64 (1 = Dirichlet / not folded)
 };
 
 template <typename T>
@@ -152,6 +153,36 @@ __device__ __forceinline__ void fold_center(const DevOp& op, int i, int j, int k
   }
 }
 
+// A x on one interior row in difference form, for operators whose rows sum to the reaction
+// term e (constant coefficients without variable reaction, flux form):
+//   A x = e x_c - sum_nb w_nb (x_c - x_nb),
+// where a neighbour on a folded face contributes x_c - x_nb -> phi x_c - x_nb (phi = 1 - theta;
+// x_nb is that boundary node's stored value).  The same operator as C' x_c + sum w x with the
+// folded centre C', but for smooth x every term is small, so the FP32 product keeps its relative
+// accuracy where the sum form cancels ~ (8/h^2) / lambda of it away.  `w` holds the unfolded
+// neighbour weights.
+template <int DIM, typename T>
+__device__ __forceinline__ T apply_diff(const DevOp& op, const W7<T>& w, T e, int i, int j, int k,
+                                        T xc, T xw, T xe, T xs, T xn, T xb, T xf) {
+  T cw = xc, ce = xc, cs = xc, cn = xc, cb = xc, cf = xc;
+  if (i == 1) cw = T(op.phi[0]) * xc;
+  if (i == op.nx - 2) ce = T(op.phi[1]) * xc;
+  if (j == 1) cs = T(op.phi[2]) * xc;
+  if (j == op.ny - 2) cn = T(op.phi[3]) * xc;
+  T a = e * xc;
+  a = fma(-w.E, ce - xe, a);
+  a = fma(-w.W, cw - xw, a);
+  a = fma(-w.N, cn - xn, a);
+  a = fma(-w.S, cs - xs, a);
+  if constexpr (DIM == 3) {
+    if (k == 1) cb = T(op.phi[4]) * xc;
+    if (k == op.nz - 2) cf = T(op.phi[5]) * xc;
+    a = fma(-w.F, cf - xf, a);
+    a = fma(-w.B, cb - xb, a);
+  }
+  return a;
+}
+
 // Gauss-Seidel value of one interior row: (b - sum_nb w_nb u_nb) * recip(C), the neighbour sum
 // as a fixed FMA chain (E, W, N, S, F, B).  Every smoother kernel uses this, so the fused and
 // the in-place sweeps agree bit for bit whatever the compiler's contraction choices.
@@ -191,6 +222,14 @@ inline int plane_chunk(long long tiles, int nz, int capacity, int prologue, int 
 }
 
 // ------------------------------------------------------------------ reductions
+// Maximum that propagates NaN (fmax drops it), so a non-finite value in a max-reduction
+// reaches the stop rule instead of reading as a small norm.
+template <typename T>
+__device__ __forceinline__ T nmax(T a, T b) { return (a != a || a > b) ? a : b; }
+
+// Non-finite reduction result (NaN or +-inf).
+__device__ __forceinline__ bool nonfinite(double v) { return !(fabs(v) <= 1.7976931348623157e308); }
+
 // Sum over the 64 lanes of a wavefront.
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -201,7 +240,7 @@ __device__ __forceinline__ T wave_sum(T v) {
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
 #pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+  for (int off = kWave / 2; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off, kWave));
   return v;
 }
 

@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(void* xv, void* rv, const 
     x[i] = store_as<S>(xi);
     r[i] = store_as<S>(ri);
     acc += static_cast<double>(ri) * static_cast<double>(ri);
-    mx = fmax(mx, fabs(static_cast<double>(ri)));
+    mx = nmax(mx, fabs(static_cast<double>(ri)));
   }
   const double s = block_sum(acc, lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(1024) k_finalize(KScalars* sc, const double* p
   const bool is_max = (op == FIN_MAX || op == FIN_RELAX);
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x)
-    acc = is_max ? fmax(acc, partials[i]) : acc + partials[i];
+    acc = is_max ? nmax(acc, partials[i]) : acc + partials[i];
   const double s = is_max ? block_max(acc, lds) : block_sum(acc, lds);
   if (threadIdx.x != 0) return;
   switch (op) {
@@ -438,8 +438,9 @@ __global__ void __launch_bounds__(1024) k_finalize(KScalars* sc, const double* p
       break;
     case FIN_CG_RR:
       sc->rr = s;
-      sc->metric = sqrt(fmax(0.0, s));
+      sc->metric = sqrt(s);
       sc->iter += 1;
+      if (nonfinite(s)) { sc->breakdown = 1; break; }
       if (sc->metric < sc->target) sc->done = 1;
       break;
     case FIN_CG_BETA:
@@ -451,7 +452,8 @@ __global__ void __launch_bounds__(1024) k_finalize(KScalars* sc, const double* p
       sc->rz = s;
       break;
     case FIN_R0: {
-      const double r0 = sqrt(fmax(0.0, s));
+      const double r0 = sqrt(s);
+      if (nonfinite(s)) { sc->breakdown = 1; break; }
       sc->r0 = r0;
       sc->metric = r0;
       sc->target = tol_mode == PDEHIP_TOL_REL ? tol * r0 : tol;

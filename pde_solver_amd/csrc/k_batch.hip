@@ -305,7 +305,7 @@ kb_vec(const BVec* __restrict__ vecs, const BItem* __restrict__ items, int n, in
   const BVec& v = vecs[it.p];
   const KScalars* sc = inner ? v.si : v.so;
   double acc = 0.0, mx = 0.0;
-  const bool skip_all = done(sc) && op != V_ADD32;
+  const bool skip_all = done(sc);
   if (!skip_all) {
     S* x = static_cast<S*>(v.x);
     S* r = static_cast<S*>(v.r);
@@ -316,12 +316,16 @@ kb_vec(const BVec* __restrict__ vecs, const BItem* __restrict__ items, int n, in
       switch (op) {
         case V_CG_UPDATE: {
           const T a = T(sc->alpha);
-          const T xi = T(ldS(x, i)) + a * T(ldS(pp, i));
+          if (inner) {
+            // the inner correction goes straight into the FP64 solution: x64 += |r64| alpha p
+            v.x64[i] += (v.so->snorm * sc->alpha) * static_cast<double>(ldS(pp, i));
+          } else {
+            x[i] = store_as<S>(static_cast<double>(T(ldS(x, i)) + a * T(ldS(pp, i))));
+          }
           const T ri = T(ldS(r, i)) - a * T(ldS(q, i));
-          x[i] = store_as<S>(static_cast<double>(xi));
           r[i] = store_as<S>(static_cast<double>(ri));
           acc += static_cast<double>(ri) * static_cast<double>(ri);
-          mx = fmax(mx, fabs(static_cast<double>(ri)));
+          mx = nmax(mx, fabs(static_cast<double>(ri)));
           break;
         }
         case V_DOT_RZ:
@@ -338,22 +342,18 @@ kb_vec(const BVec* __restrict__ vecs, const BItem* __restrict__ items, int n, in
         case V_RR: {
           const double ri = static_cast<double>(ldS(r, i));
           acc += ri * ri;
-          mx = fmax(mx, fabs(ri));
+          mx = nmax(mx, fabs(ri));
           break;
         }
         case V_ZERO_X:
           x[i] = store_as<S>(0.0);
           break;
-        case V_TO32: {   // inner rhs = r64 / ||r64|| (outer metric), x32 = 0
-          const double nr = v.so->metric;
+        case V_TO32: {   // inner rhs = r64 / snorm (the fixed inner scale)
+          const double nr = v.so->snorm;
           const double s = nr > 0.0 ? 1.0 / nr : 1.0;
           r[i] = store_as<S>(v.r64[i] * s);
-          x[i] = store_as<S>(0.0);
           break;
         }
-        case V_ADD32:    // x64 += ||r|| * x32 (problems already converged outside skip)
-          if (!done(v.so)) v.x64[i] += v.so->snorm * static_cast<double>(ldS(x, i));
-          break;
         case V_COPY_X0:
           v.x64[i] = v.x0[i];
           break;
@@ -395,6 +395,7 @@ kb_apply_dot(const BVec* __restrict__ vecs, const BEnt* __restrict__ tab0,
     S* y = OUTER ? reinterpret_cast<S*>(v.r64) : static_cast<S*>(v.q);
     const long long nx = op.nx;
     const W7<T> w0 = const_w<T>(op);
+    const T e = T(op.e);
     for (int j = it.a; j < it.b; ++j) {
       for (int i = lane; i < op.nx; i += kLanes) {
         const long long id = j * nx + i;
@@ -403,10 +404,8 @@ kb_apply_dot(const BVec* __restrict__ vecs, const BEnt* __restrict__ tab0,
         if (i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1) {
           ax = xc;
         } else {
-          W7<T> wt = w0;
-          fold_center<2, T>(op, i, j, 0, wt);
-          ax = wt.C * xc + wt.E * T(ldS(x, id + 1)) + wt.W * T(ldS(x, id - 1)) +
-               wt.N * T(ldS(x, id + nx)) + wt.S * T(ldS(x, id - nx));
+          ax = apply_diff<2, T>(op, w0, e, i, j, 0, xc, T(ldS(x, id - 1)), T(ldS(x, id + 1)),
+                                T(ldS(x, id - nx)), T(ldS(x, id + nx)), T(0), T(0));
         }
         if constexpr (OUTER) {
           const T rv = T(ldS(b, id)) - ax;
@@ -416,6 +415,49 @@ kb_apply_dot(const BVec* __restrict__ vecs, const BEnt* __restrict__ tab0,
           y[id] = store_as<S>(static_cast<double>(ax));
           acc += static_cast<double>(xc) * static_cast<double>(ax);
         }
+      }
+    }
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) partials[w] = acc;
+}
+
+// Residual replacement (mixed precision): for problems whose inner scalars flag it, the FP32
+// Krylov residual is replaced by the true residual (b64 - A64 x64) / snorm, computed in FP64
+// from the FP64 solution; partials are |b - A x|^2 in FP64 (zero for other problems).
+__global__ void __launch_bounds__(kLanes * kWavesPB)
+kb_replace(const BVec* __restrict__ vecs, const BItem* __restrict__ items, int n,
+           double* partials) {
+  const int w = wave_item();
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BVec& v = vecs[it.p];
+  const int lane = threadIdx.x & (kLanes - 1);
+  double acc = 0.0;
+  const KScalars* si = v.si;
+  if (!done(si) && si->flag_s) {
+    const DevOp& op = v.op64;
+    const double* x = v.x64;
+    const double* b = v.b64;
+    float* r = static_cast<float*>(v.r);
+    const double inv = 1.0 / v.so->snorm;
+    const long long nx = op.nx;
+    const W7<double> w0 = const_w<double>(op);
+    const double e = op.e;
+    for (int j = it.a; j < it.b; ++j) {
+      for (int i = lane; i < op.nx; i += kLanes) {
+        const long long id = j * nx + i;
+        const double xc = x[id];
+        double ax;
+        if (i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1) {
+          ax = xc;
+        } else {
+          ax = apply_diff<2, double>(op, w0, e, i, j, 0, xc, x[id - 1], x[id + 1], x[id - nx],
+                                     x[id + nx], 0.0, 0.0);
+        }
+        const double rv = b[id] - ax;
+        r[id] = static_cast<float>(rv * inv);
+        acc += rv * rv;
       }
     }
   }
@@ -437,7 +479,7 @@ kb_finalize(const BVec* __restrict__ vecs, const int* __restrict__ first,
   double acc = 0.0, mx = 0.0;
   for (int t = threadIdx.x; t < cnt[q]; t += blockDim.x) {
     acc += partials[first[q] + t];
-    if (maxpart) mx = fmax(mx, maxpart[first[q] + t]);
+    if (maxpart) mx = nmax(mx, maxpart[first[q] + t]);
   }
   const double s = block_sum(acc, lds);
   const double m = maxpart ? block_max(mx, lds) : 0.0;
@@ -450,8 +492,9 @@ kb_finalize(const BVec* __restrict__ vecs, const int* __restrict__ first,
       break;
     case FIN_CG_RR:
       sc->rr = s;
-      sc->metric = sqrt(fmax(0.0, s));
+      sc->metric = sqrt(s);
       sc->iter += 1;
+      if (nonfinite(s)) { sc->breakdown = 1; break; }
       if (sc->metric < sc->target) sc->done = 1;
       break;
     case FIN_CG_BETA:
@@ -463,11 +506,34 @@ kb_finalize(const BVec* __restrict__ vecs, const int* __restrict__ first,
       sc->rz = s;
       break;
     case FIN_R0: {
-      const double r0 = sqrt(fmax(0.0, s));
+      const double r0 = sqrt(s);
+      if (nonfinite(s)) { sc->breakdown = 1; break; }
       sc->r0 = r0;
       sc->metric = r0;
       sc->target = tol_mode == PDEHIP_TOL_REL ? tol * r0 : tol;
       if (r0 < sc->target || r0 == 0.0) sc->done = 1;
+      break;
+    }
+    case FIN_RU_CHECK: {   // inner scalars; tol = the replacement threshold delta
+      sc->rr = s;
+      sc->metric = sqrt(s);
+      sc->iter += 1;
+      if (nonfinite(s)) { sc->breakdown = 1; v.so->breakdown = 1; break; }
+      sc->rho = fmax(sc->rho, sc->metric);   // largest residual since the last replacement
+      v.so->pad0 = sc->iter;
+      sc->flag_s = (sc->metric < tol * sc->rho || sc->metric * v.so->snorm < v.so->target) ? 1 : 0;
+      break;
+    }
+    case FIN_RU_APPLY: {   // inner scalars; the outer ones take the true residual
+      if (!sc->flag_s) break;
+      KScalars* so = v.so;
+      so->metric = sqrt(s);
+      so->steps += 1;
+      if (nonfinite(s)) { so->breakdown = 1; sc->breakdown = 1; break; }
+      sc->metric = so->metric / so->snorm;
+      sc->rho = sc->metric;
+      sc->flag_s = 0;
+      if (so->metric < so->target) { so->done = 1; sc->done = 1; }
       break;
     }
     default:
@@ -499,6 +565,23 @@ __global__ void kb_reset_inner(const BVec* __restrict__ vecs, int nprob, int out
 }
 
 // Fresh scalars for a solve (outer and inner): breakdown threshold, unit scales, counters 0.
+// Start of a mixed solve with residual replacement: fresh inner scalars (done when the outer
+// ones are), the inner scale fixed at the initial residual norm.
+__global__ void kb_ru_start(const BVec* __restrict__ vecs, int nprob) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nprob) return;
+  const BVec& v = vecs[q];
+  KScalars k{};
+  k.eps = 1e-300;
+  k.scale = 1.0;
+  k.inv_scale = 1.0;
+  k.done = (v.so->done | v.so->breakdown);
+  *v.si = k;
+  v.so->snorm = v.so->metric > 0.0 ? v.so->metric : 1.0;
+  v.so->steps = 0;
+  v.so->pad0 = 0;
+}
+
 __global__ void kb_init_scalars(const BVec* __restrict__ vecs, int nprob, double eps) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nprob) return;
@@ -553,10 +636,13 @@ void restrict_(hipStream_t s, Prec pf, Prec pc, const BEnt* fine, const BEnt* co
   if (n <= 0) return;
   const dim3 g(waves_grid(n)), b(kLanes * kWavesPB);
   const int sc = scale0 ? 1 : 0;
-  if (pf == P64) hipLaunchKernelGGL((kb_restrict<P64, P64>), g, b, 0, s, fine, coarse, items, n, sc);
-  else if (pf == P32) hipLaunchKernelGGL((kb_restrict<P32, P16>), g, b, 0, s, fine, coarse, items, n, sc);
-  else hipLaunchKernelGGL((kb_restrict<P16, P16>), g, b, 0, s, fine, coarse, items, n, sc);
-  (void)pc;
+#define RB_(A, B) hipLaunchKernelGGL((kb_restrict<A, B>), g, b, 0, s, fine, coarse, items, n, sc)
+  if (pf == P64 && pc == P64) RB_(P64, P64);
+  else if (pf == P32 && pc == P16) RB_(P32, P16);
+  else if (pf == P32 && pc == P32) RB_(P32, P32);
+  else if (pf == P16 && pc == P16) RB_(P16, P16);
+  else throw Error(PDEHIP_ERR_UNSUPPORTED, "batched restriction: unsupported precision pair");
+#undef RB_
 }
 
 void prolong(hipStream_t s, Prec pc, Prec pf, const BEnt* coarse, const BEnt* fine,
@@ -564,10 +650,13 @@ void prolong(hipStream_t s, Prec pc, Prec pf, const BEnt* coarse, const BEnt* fi
   if (n <= 0) return;
   const dim3 g(waves_grid(n)), b(kLanes * kWavesPB);
   const int sc = scale0 ? 1 : 0;
-  if (pf == P64) hipLaunchKernelGGL((kb_prolong<P64, P64>), g, b, 0, s, coarse, fine, items, n, sc);
-  else if (pf == P32) hipLaunchKernelGGL((kb_prolong<P16, P32>), g, b, 0, s, coarse, fine, items, n, sc);
-  else hipLaunchKernelGGL((kb_prolong<P16, P16>), g, b, 0, s, coarse, fine, items, n, sc);
-  (void)pc;
+#define PB_(A, B) hipLaunchKernelGGL((kb_prolong<A, B>), g, b, 0, s, coarse, fine, items, n, sc)
+  if (pc == P64 && pf == P64) PB_(P64, P64);
+  else if (pc == P16 && pf == P32) PB_(P16, P32);
+  else if (pc == P32 && pf == P32) PB_(P32, P32);
+  else if (pc == P16 && pf == P16) PB_(P16, P16);
+  else throw Error(PDEHIP_ERR_UNSUPPORTED, "batched prolongation: unsupported precision pair");
+#undef PB_
 }
 
 void coarse_dense(hipStream_t s, Prec p, const BEnt* tab, const BDense* dense, const BItem* items,
@@ -612,6 +701,16 @@ void finalize(hipStream_t s, const BVec* vecs, const int* first, const int* cnt,
 
 void reset_inner(hipStream_t s, const BVec* vecs, int nprob) {
   hipLaunchKernelGGL(kb_reset_inner, dim3((nprob + 255) / 256), dim3(256), 0, s, vecs, nprob, 1);
+}
+
+void replace_residual(hipStream_t s, const BVec* vecs, const BItem* rows, int n, double* partials) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(kb_replace, dim3(waves_grid(n)), dim3(kLanes * kWavesPB), 0, s, vecs, rows, n,
+                     partials);
+}
+
+void ru_start(hipStream_t s, const BVec* vecs, int nprob) {
+  hipLaunchKernelGGL(kb_ru_start, dim3((nprob + 255) / 256), dim3(256), 0, s, vecs, nprob);
 }
 
 void init_scalars(hipStream_t s, const BVec* vecs, int nprob, double eps) {

@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(256) k_fin_norm(const double* partials, int nb
   double acc = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[b];
   const double s = block_sum(acc, lds);
-  if (threadIdx.x == 0) *out = sqrt(fmax(0.0, s));
+  if (threadIdx.x == 0) *out = sqrt(s);
 }
 
 // Givens step j (krylov.cpp:262-295): H is (m+1) x m row-major with leading dimension m.
@@ -264,7 +264,8 @@ __global__ void __launch_bounds__(1024) k_fin_k(KScalars* sc, const double* p1, 
       sc->alpha = sc->rho / s1;
       break;
     case KFIN_BI_S:
-      sc->snorm = sqrt(fmax(0.0, s1));
+      sc->snorm = sqrt(s1);
+      if (nonfinite(s1)) { sc->breakdown = 1; break; }
       if (sc->snorm < sc->target) sc->flag_s = 1;
       break;
     case KFIN_BI_OMEGA:
@@ -274,14 +275,16 @@ __global__ void __launch_bounds__(1024) k_fin_k(KScalars* sc, const double* p1, 
       sc->omega = s2 / s1;
       break;
     case KFIN_BI_R:
-      sc->metric = sqrt(fmax(0.0, s1));
+      sc->metric = sqrt(s1);
       sc->iter += 1;
+      if (nonfinite(s1)) { sc->breakdown = 1; break; }
       if (sc->metric < sc->target) { sc->done = 1; break; }
       if (fabs(sc->omega) < sc->eps) { sc->breakdown = 1; break; }
       sc->rho_old = sc->rho;
       break;
     case KFIN_GM_R:
-      sc->metric = sqrt(fmax(0.0, s1));
+      sc->metric = sqrt(s1);
+      if (nonfinite(s1)) { sc->breakdown = 1; break; }
       if (sc->metric < sc->target) sc->done = 1;
       break;
   }

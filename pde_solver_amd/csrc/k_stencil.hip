@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(kBlock) k_jacobi_ref(DevOp op, const double* _
     const double nlv = nl.n ? eval_nonlinear(nl, x[id]) : 0.0;
     const double v = (-(fv + nlv) - s) / w.C;
     next[id] = v;
-    md = fmax(md, fabs(v - x[id]));
+    md = nmax(md, fabs(v - x[id]));
   }
   const double m = block_max(md, lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = m;
@@ -228,10 +228,10 @@ __global__ void __launch_bounds__(kBlock) k_sor_rb_ref(DevOp op, const double* _
     const double jac = (-(fv + nlv) - s) / w.C;
     const double upd = (1.0 - omega) * old + omega * jac;
     x[id] = upd;
-    md = fmax(md, fabs(upd - old));
+    md = nmax(md, fabs(upd - old));
   }
   const double m = block_max(md, lds);
-  if (threadIdx.x == 0) partials[blockIdx.x] = fmax(m, color ? partials[blockIdx.x] : 0.0);
+  if (threadIdx.x == 0) partials[blockIdx.x] = nmax(m, color ? partials[blockIdx.x] : 0.0);
 }
 
 // ----------------------------------------------------------------------------- dispatch

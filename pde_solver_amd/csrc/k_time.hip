@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(kBlock) k_resnorm(DevOp op, FaceData fd, const
       r = w.C * u[id] + s + ((f ? f[id] : f_const) + nlv);
     }
     acc += r * r;
-    mx = fmax(mx, fabs(r));
+    mx = nmax(mx, fabs(r));
   }
   const double s = block_sum(acc, lds);
   const double m = block_max(mx, lds);
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(kBlock) k_maxabs(const T* __restrict__ a, long
   double m = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
-    m = fmax(m, fabs(static_cast<double>(a[i])));
+    m = nmax(m, fabs(static_cast<double>(a[i])));
   const double r = block_max(m, lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(1024) k_fin_scale(const double* partials, int 
   __shared__ double lds[1024 / kWave];
   if (sc->done | sc->breakdown) return;
   double m = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) m = fmax(m, partials[i]);
+  for (int i = threadIdx.x; i < np; i += blockDim.x) m = nmax(m, partials[i]);
   const double r = block_max(m, lds);
   if (threadIdx.x == 0) {
     sc->inv_scale = r > 0.0 ? r : 1.0;

@@ -25,6 +25,12 @@ void hip_check(hipError_t e, const char* what) {
 }
 
 // ------------------------------------------------------------------ buffers
+namespace {
+std::atomic<bool> g_mixed_fp16{true};
+}  // namespace
+void set_mixed_fp16(bool on) { g_mixed_fp16.store(on); }
+bool mixed_fp16() { return g_mixed_fp16.load(); }
+
 void DBuf::alloc(Ctx* c, size_t bytes) {
   release();
   if (bytes == 0) return;
@@ -229,6 +235,7 @@ static DevOp make_devop(const Geo& g, int kind) {
   op.kind = kind;
   op.sxy = g.sxy();
   op.n = g.n;
+  for (double& v : op.phi) v = 1.0;
   return op;
 }
 
@@ -237,6 +244,8 @@ static void build_level_ops(Plan& P, Level& lv, const double* kappa64_dev, bool 
   const pdehip_problem& p = P.prob;
   lv.op64 = make_devop(lv.g, p.op_kind);
   fold_thetas(p, lv.g, folded, lv.op64.theta);
+  for (int f = 0; f < 6; ++f) lv.op64.phi[f] = 1.0 - lv.op64.theta[f];
+  lv.op64.e = p.e;
   if (p.op_kind == PDEHIP_OP_CONST) {
     const_weights(p, lv.g, lv.op64.w);
   } else if (p.op_kind == PDEHIP_OP_FLUX) {
@@ -489,7 +498,7 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
     }
     for (int l = 0; l < nl; ++l) {
       Level& lv = P->L[l];
-      lv.prec = P->mixed ? (l == 0 ? P32 : P16) : P64;
+      lv.prec = P->mixed ? (l == 0 || !mixed_fp16() ? P32 : P16) : P64;
       const size_t eb = prec_bytes(lv.prec);
       lv.r.alloc(ctx, lv.g.n * eb);
       fill(s, lv.prec, lv.r.get(), lv.g.n, 0.0);

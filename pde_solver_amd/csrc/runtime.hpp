@@ -6,6 +6,7 @@
 
 #include <atomic>
 #include <memory>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -26,6 +27,11 @@ void hip_check(hipError_t e, const char* what);
 #define HIPCHK(x) ::pdehip::hip_check((x), #x)
 
 // ------------------------------------------------------------------ context
+// Mixed precision: coarse multigrid levels stored in FP16 (default) or FP32; read when a plan
+// is created.
+void set_mixed_fp16(bool on);
+bool mixed_fp16();
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -37,6 +43,8 @@ struct Ctx {
   std::mutex mu;                     // serialises one-shot solves on the context stream
   std::string last_error;
   std::mutex err_mu;
+  std::mutex tune_mu;                // batch work-item shapes picked by the autotuner
+  std::map<std::string, std::vector<int>> batch_shapes;
   void set_error(const std::string& e) {
     std::lock_guard<std::mutex> g(err_mu);
     last_error = e;

@@ -4,6 +4,7 @@ the library is missing — there is no CPU fallback."""
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 from typing import List, Optional, Sequence
 
@@ -62,6 +63,8 @@ def lib():
         L.pdehip_batch_download.argtypes = [VP, C.c_int, DP]
         L.pdehip_batch_bytes.argtypes = [VP]
         L.pdehip_batch_bytes.restype = C.c_size_t
+        L.pdehip_batch_time_kernel.argtypes = [VP, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double)]
         L.pdehip_batch_destroy.argtypes = [VP]
         L.pdehip_tune_set.argtypes = [C.c_char_p, C.c_int]
         L.pdehip_tune_get.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
@@ -95,12 +98,17 @@ class Context:
 
     def __init__(self, device: int = 0, hbm_budget: int = 0):
         self.h = C.c_void_p()
+        self._children = weakref.WeakSet()   # plans, batches and servers made from this context
         st = lib().pdehip_create(device, hbm_budget, C.byref(self.h))
         if st != 0:
             raise HipError(st, lib().pdehip_last_error(None).decode())
 
     def close(self):
+        """Destroy the context; plans, batches and servers still open on it are closed first
+        (the C API requires them to go before their context)."""
         if self.h:
+            for child in list(self._children):
+                child.close()
             lib().pdehip_destroy(self.h)
             self.h = C.c_void_p()
 
@@ -192,6 +200,7 @@ class Batch:
     def __init__(self, ctx: Context, specs: Sequence[capi.Spec], opt: capi.Options,
                  cfg: Optional[capi.BatchConfig] = None):
         self.ctx = ctx
+        ctx._children.add(self)
         self.specs = list(specs)
         self._probs = (capi.Problem * len(specs))(*[s.to_c() for s in specs])
         self.h = C.c_void_p()
@@ -215,6 +224,12 @@ class Batch:
     def bytes(self) -> int:
         return int(lib().pdehip_batch_bytes(self.h))
 
+    def time_kernel(self, kernel: int, reps: int = 10):
+        """(avg ms, algorithmic bytes) of one batched level-0 launch (pdehip_batch_time_kernel)."""
+        ms, nb = C.c_double(), C.c_double()
+        self.ctx._check(lib().pdehip_batch_time_kernel(self.h, kernel, reps, C.byref(ms), C.byref(nb)))
+        return ms.value, nb.value
+
     def close(self):
         if self.h:
             lib().pdehip_batch_destroy(self.h)
@@ -232,6 +247,7 @@ class Plan:
 
     def __init__(self, ctx: Context, spec: capi.Spec, opt: capi.Options):
         self.ctx = ctx
+        ctx._children.add(self)
         self.spec = spec
         self.h = C.c_void_p()
         p = spec.to_c()
@@ -278,6 +294,7 @@ class Server:
 
     def __init__(self, ctx: Context, cfg: Optional[capi.ServerConfig] = None):
         self.ctx = ctx
+        ctx._children.add(self)
         self.h = C.c_void_p()
         self._keep = {}
         ctx._check(lib().pdehip_server_create(ctx.h, C.byref(cfg) if cfg is not None else None,

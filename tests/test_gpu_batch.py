@@ -91,3 +91,97 @@ def test_batch_repeat_solves_identical(ctx):
     for i in range(len(specs)):
         assert np.array_equal(first[i], b.download(i))
     b.close()
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+def test_batch_autotuned_graph_matches_default(ctx, precision):
+    """Autotuned work-item shapes and the captured iteration graph change only the order of
+    the partial sums: same answers to the solve tolerance, and graph replays are repeatable."""
+    specs = problems([129, 33, 513, 65, 257, 17, 1025, 33], seed=5)
+    print(f"batch autotune+graph precision={precision}", flush=True)
+    opt = opts(precision, 1e-10)
+    b0 = ctx.batch(specs, opt)
+    _, r0 = b0.solve()
+    u0 = [b0.download(i) for i in range(len(specs))]
+    b0.close()
+    b1 = ctx.batch(specs, opt, capi.BatchConfig(autotune=1, use_graph=1))
+    _, r1 = b1.solve()
+    u1 = [b1.download(i) for i in range(len(specs))]
+    _, r2 = b1.solve()
+    for i in range(len(specs)):
+        assert r1[i].status == 0 and r1[i].rel_residual < 2e-10
+        assert abs(r1[i].inner_iterations - r0[i].inner_iterations) <= 1
+        assert rel(u1[i], u0[i]) < 2e-8
+        assert np.array_equal(u1[i], b1.download(i))
+        assert r2[i].iterations == r1[i].iterations
+    b1.close()
+
+
+def test_batch_mixed_fp32_coarse_levels(ctx):
+    """mixed.fp16 = 0: coarse levels in FP32; same answers against the oracle."""
+    from pde_solver_amd import hip
+    specs = problems([65, 129, 33], seed=6)
+    hip.tune_set("mixed.fp16", 0)
+    try:
+        b = ctx.batch(specs, opts(capi.MIXED, 1e-10))
+        _, reps = b.solve()
+        for i, s in enumerate(specs):
+            _, uo, _ = O.solve(s, opts(capi.FP64, 1e-13))
+            assert reps[i].status == 0 and reps[i].rel_residual < 2e-10
+            assert rel(b.download(i), uo) < 5e-7
+        b.close()
+    finally:
+        hip.tune_set("mixed.fp16", 1)
+
+
+def test_context_close_with_open_batch_and_plan():
+    """Closing a context closes the plans and batches still open on it first (regression: the
+    batch's later destructor used the freed context)."""
+    import gc
+    from pde_solver_amd import hip
+    c = hip.Context()
+    specs = problems([33, 65], seed=9)
+    b = c.batch(specs, opts(capi.FP64, 1e-8))
+    p = c.plan(specs[0], opts(capi.FP64, 1e-8))
+    b.solve()
+    c.close()
+    assert not b.h and not p.h
+    del b, p
+    gc.collect()
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+def test_nonfinite_source_is_an_error_not_convergence(ctx, precision):
+    """A NaN in the data must surface as an error: reductions propagate NaN (a NaN-dropping max
+    or sqrt(max(0, nan)) would read it as a zero residual and report convergence)."""
+    specs = problems([33, 65], seed=10)
+    specs[1].f_field[10, 10] = np.nan
+    opt = opts(precision, 1e-8)
+    b = ctx.batch(specs, opt)
+    st, reps = b.solve(check=False)
+    assert st != 0
+    assert reps[0].status == 0
+    assert reps[1].status != 0
+    b.close()
+    st, u, rep = ctx.solve(specs[1], opt, check=False)
+    assert st != 0
+
+
+def test_batch_time_kernel_then_solve(ctx):
+    """pdehip_batch_time_kernel times the level-0 launches over every problem; a solve after it
+    starts afresh and gives the same answer as before."""
+    specs = problems([65, 33, 129], seed=11)
+    b = ctx.batch(specs, opts(capi.MIXED, 1e-10))
+    b.solve()
+    before = [b.download(i) for i in range(len(specs))]
+    n0 = sum(s.n for s in specs)
+    for k, per_node in ((0, 12.0), (1, 12.0), (2, 8.0), (3, 24.0)):
+        ms, nb = b.time_kernel(k, 3)
+        assert ms > 0.0
+        assert nb == per_node * n0
+    with pytest.raises(Exception):
+        b.time_kernel(7, 1)
+    b.solve()
+    for i in range(len(specs)):
+        assert np.array_equal(before[i], b.download(i))
+    b.close()
