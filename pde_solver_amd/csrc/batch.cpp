@@ -66,6 +66,8 @@ struct BatchPlan {
   std::vector<LevelShape> shape;           // per level work-item shapes
   DBuf part, maxp;
   DBuf active;
+  DBuf faces, psum, pmax, rep;             // finish: per-problem boundary data, partials, results
+  BReport* hrep = nullptr;                 // pinned copy of rep
   int* hactive = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph[2] = {nullptr, nullptr};   // captured PCG iterations (FP64 / inner)
@@ -74,6 +76,7 @@ struct BatchPlan {
     for (hipGraphExec_t g : graph)
       if (g) (void)hipGraphExecDestroy(g);
     if (hactive) (void)hipHostFree(hactive);
+    if (hrep) (void)hipHostFree(hrep);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
   }
@@ -377,6 +380,19 @@ BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pd
   B->maxp.alloc(ctx, (size_t)np * sizeof(double));
   B->active.alloc(ctx, sizeof(int));
   HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&B->hactive), sizeof(int)));
+  {
+    std::vector<BFace> fc(n);
+    for (int p = 0; p < n; ++p) {
+      const Plan& P = *B->plans[p];
+      fc[p] = BFace{P.faces.fd, P.f ? P.f.as<double>() : nullptr, P.f_const};
+    }
+    B->faces.alloc(ctx, n * sizeof(BFace));
+    HIPCHK(hipMemcpyAsync(B->faces.get(), fc.data(), n * sizeof(BFace), hipMemcpyHostToDevice, s));
+    B->psum.alloc(ctx, (size_t)std::max(1, B->el0.n) * sizeof(double));
+    B->pmax.alloc(ctx, (size_t)std::max(1, B->el0.n) * sizeof(double));
+    B->rep.alloc(ctx, n * sizeof(BReport));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&B->hrep), n * sizeof(BReport)));
+  }
   HIPCHK(hipStreamSynchronize(s));
   B->bytes = ctx->used.load() - used0;
   return B;
@@ -516,22 +532,32 @@ int batch_solve(BatchPlan& B, pdehip_report* reports) {
     pcg(B, P32, max_iter, delta);
   }
   HIPCHK(hipEventRecord(B.ev1, s));
-  for (auto& P : B.plans) plan_finish_enqueue(*P);
+  // finish, batched: residual of the solved system, boundary closures, reference residual
+  // norms, one gather and one copy of every problem's results
+  const BVec* V = B.V();
+  bk::apply_dot_tab(s, P64, V, B.T(0), rw.d(), rw.n, part, true, true);
+  for (int phase = 0; phase < 4; ++phase) bk::closures(s, V, B.faces.as<BFace>(), n, phase);
+  bk::resnorm(s, V, B.faces.as<BFace>(), el.d(), el.n, B.psum.as<double>(), B.pmax.as<double>());
+  bk::finish_reduce(s, V, rw.first.as<int>(), rw.cnt.as<int>(), part, el.first.as<int>(),
+                    el.cnt.as<int>(), B.psum.as<double>(), B.pmax.as<double>(), n,
+                    B.rep.as<BReport>());
+  HIPCHK(hipMemcpyAsync(B.hrep, B.rep.get(), n * sizeof(BReport), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, B.ev0, B.ev1));
   int status = PDEHIP_OK;
   for (int p = 0; p < n; ++p) {
-    Plan& P = *B.plans[p];
+    const BReport& k = B.hrep[p];
     pdehip_report rep{};
-    plan_finish_collect(P, &rep);
-    const KScalars& k = P.hsc[0];
     rep.solve_ms = ms;
+    rep.rel_residual = k.r0 > 0 ? std::sqrt(k.rr) / k.r0 : std::sqrt(k.rr);
+    rep.residual_l2 = std::sqrt(k.l2sq);
+    rep.residual_linf = k.linf;
     rep.converged = k.done && !k.breakdown;
     if (B.mixed) {
       rep.refinements = k.steps;
       rep.iterations = k.steps;
-      rep.inner_iterations = k.pad0;
+      rep.inner_iterations = k.inner;
     } else {
       rep.iterations = k.iter;
     }

@@ -41,6 +41,17 @@ struct BVec {
   DevOp op64;                    // level-0 FP64 operator (outer residual)
 };
 
+// Finish data of one problem: boundary data, source.
+struct BFace { FaceData fd; const double* f; double f_const; };
+
+// Per-problem results of a batched solve, gathered on the device and copied once.
+struct BReport {
+  double rr;          // |b - A x|^2 of the solved (folded) system
+  double l2sq, linf;  // reference residual norms (squared L2, max)
+  double metric, r0;  // outer scalars
+  int done, breakdown, iter, steps, inner, pad;
+};
+
 // Work item: problem p and a range (meaning depends on the list).
 struct BItem { int p, a, b, c; };
 
@@ -92,9 +103,9 @@ enum VecOp { V_APPLY_DOT = 0, V_CG_UPDATE = 1, V_DOT_RZ = 2, V_XPAY = 3, V_RESID
 void vec(hipStream_t s, Prec p, int op, const BVec* vecs, const BItem* items, int n,
          double* partials, double* maxpart);
 // Level-0 row items: inner q = A p with partials p.q (operator and scalars of tab0), or the
-// outer FP64 residual r64 = b64 - A64 x64 with partials r.r.
+// outer FP64 residual r64 = b64 - A64 x64 with partials r.r (all: also for finished problems).
 void apply_dot_tab(hipStream_t s, Prec p, const BVec* vecs, const BEnt* tab0, const BItem* rows,
-                   int n, double* partials, bool outer);
+                   int n, double* partials, bool outer, bool all = false);
 // Inner scalars reset for a refinement (outer metric kept as the correction scale).
 void reset_inner(hipStream_t s, const BVec* vecs, int nprob);
 // Mixed solve with residual replacement: inner scalars fresh, inner scale = initial residual.
@@ -110,6 +121,15 @@ void finalize(hipStream_t s, const BVec* vecs, const int* first, const int* cnt,
               int tol_mode);
 // Count of problems whose selected scalars are not done (written to *active).
 void active_count(hipStream_t s, const BVec* vecs, int nprob, int inner, int* active);
+// Finish: boundary closures of every problem's FP64 solution (phase 0/1: Neumann/Robin x/y
+// faces, 2/3: Dirichlet x/y faces, launched in that order), the reference residual norms over
+// element items (partials per item), and the per-problem gather into BReport.
+void closures(hipStream_t s, const BVec* vecs, const BFace* faces, int nprob, int phase);
+void resnorm(hipStream_t s, const BVec* vecs, const BFace* faces, const BItem* items, int n,
+             double* sums, double* maxs);
+void finish_reduce(hipStream_t s, const BVec* vecs, const int* rfirst, const int* rcnt,
+                   const double* rr_part, const int* efirst, const int* ecnt, const double* sums,
+                   const double* maxs, int nprob, BReport* out);
 }  // namespace bk
 
 }  // namespace pdehip

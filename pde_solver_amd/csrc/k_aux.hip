@@ -1,6 +1,6 @@
 // k_aux.hip — boundary closures, multigrid transfers, coarsest-level solve, vector kernels and
 // the deterministic two-stage reductions whose second stage updates device-resident scalars.
-#include "kernels.hpp"
+#include "faces_dev.hpp"
 
 namespace pdehip {
 namespace {
@@ -62,22 +62,8 @@ __global__ void k_closure_nr(DevOp op, FaceData fd, int axis, double* u, const K
       if (kind == PDEHIP_BC_DIRICHLET) continue;
       long long node, inner;
       face_node(op, axis, side, q, &node, &inner);
-      const double h = fd.h[f];
-      const double in = u[inner];
-      if (kind == PDEHIP_BC_NEUMANN) {
-        const double g = fd.value[f][q];
-        if (fd.convention == PDEHIP_BCCONV_AXIS) u[node] = side ? in + h * g : in - h * g;
-        else u[node] = in + h * g;
-      } else {
-        const double al = fd.alpha[f][q], be = fd.beta[f][q], ga = fd.gamma[f][q];
-        if (fd.convention == PDEHIP_BCCONV_AXIS && side) {
-          const double den = al - be / h;
-          if (fabs(den) > 1e-12) u[node] = (ga - (be / h) * in) / den;
-        } else {
-          const double den = al + be / h;
-          if (fabs(den) > 1e-12) u[node] = (ga + (be / h) * in) / den;
-        }
-      }
+      double v;
+      if (closure_nr_value(fd, f, side, q, u[inner], &v)) u[node] = v;
     }
   }
 }

@@ -7,6 +7,7 @@
 // "done" return at once.  Reductions write one partial per item; items of a problem are
 // contiguous, and the segmented finalize sums each problem's range in a fixed order.
 #include "batch.hpp"
+#include "faces_dev.hpp"
 
 namespace pdehip {
 namespace bk {
@@ -375,7 +376,7 @@ kb_vec(const BVec* __restrict__ vecs, const BItem* __restrict__ items, int n, in
 // Level-0 rows [a, b) of problem p.  Inner (OUTER false): q = A p with the level-0 operator of
 // the Krylov precision (tab0) and partials of p.q.  Outer: r64 = b64 - A64 x64 with partials
 // of r.r (the FP64 refinement residual).
-template <Prec P, bool OUTER>
+template <Prec P, bool OUTER, bool ALL = false>
 __global__ void __launch_bounds__(kLanes * kWavesPB)
 kb_apply_dot(const BVec* __restrict__ vecs, const BEnt* __restrict__ tab0,
              const BItem* __restrict__ items, int n, double* partials) {
@@ -388,7 +389,7 @@ kb_apply_dot(const BVec* __restrict__ vecs, const BEnt* __restrict__ tab0,
   const int lane = threadIdx.x & (kLanes - 1);
   double acc = 0.0;
   const KScalars* sc = OUTER ? v.so : tab0[it.p].sc;
-  if (!done(sc)) {
+  if (ALL || !done(sc)) {
     const DevOp& op = OUTER ? v.op64 : tab0[it.p].op;
     const S* x = OUTER ? reinterpret_cast<const S*>(v.x64) : static_cast<const S*>(v.p);
     const S* b = OUTER ? reinterpret_cast<const S*>(v.b64) : nullptr;
@@ -604,6 +605,111 @@ __global__ void kb_active(const BVec* __restrict__ vecs, int nprob, int inner, i
   if (threadIdx.x == 0) *active = static_cast<int>(s);
 }
 
+// ------------------------------------------------------------------------------- finish
+// Boundary closures of problem blockIdx.x's FP64 solution, one face pair per launch (phase).
+__global__ void __launch_bounds__(256)
+kb_closure(const BVec* __restrict__ vecs, const BFace* __restrict__ faces, int nprob, int phase) {
+  const int q0 = blockIdx.x;
+  if (q0 >= nprob) return;
+  const BVec& v = vecs[q0];
+  const FaceData& fd = faces[q0].fd;
+  const int axis = phase & 1;
+  const bool dir = phase >= 2;
+  const int nx = v.nx, ny = v.ny;
+  const int nq = axis == 0 ? ny : nx;
+  double* u = v.x64;
+  for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+    for (int side = 0; side < 2; ++side) {
+      const int f = 2 * axis + side;
+      const bool is_d = fd.kind[f] == PDEHIP_BC_DIRICHLET;
+      if (is_d != dir) continue;
+      long long node, inner;
+      if (axis == 0) {
+        node = (long long)q * nx + (side ? nx - 1 : 0);
+        inner = (long long)q * nx + (side ? nx - 2 : 1);
+      } else {
+        node = (long long)(side ? ny - 1 : 0) * nx + q;
+        inner = (long long)(side ? ny - 2 : 1) * nx + q;
+      }
+      if (is_d) {
+        u[node] = fd.value[f][q];
+      } else {
+        double val;
+        if (closure_nr_value(fd, f, side, q, u[inner], &val)) u[node] = val;
+      }
+    }
+  }
+}
+
+// Reference residual norms of the FP64 solution over element items (one workgroup each).
+__global__ void __launch_bounds__(256)
+kb_resnorm(const BVec* __restrict__ vecs, const BFace* __restrict__ faces,
+           const BItem* __restrict__ items, int n, double* sums, double* maxs) {
+  __shared__ double lds[256 / kWave];
+  const int w = blockIdx.x;
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BVec& v = vecs[it.p];
+  const BFace& bf = faces[it.p];
+  const DevOp& op = v.op64;
+  const double* __restrict__ u = v.x64;
+  const W7<double> wt = const_w<double>(op);
+  double acc = 0.0, mx = 0.0;
+  for (long long id = it.a + threadIdx.x; id < it.b; id += blockDim.x) {
+    const int i = static_cast<int>(id % op.nx);
+    const int j = static_cast<int>(id / op.nx);
+    double r;
+    if (is_boundary2(op, i, j)) {
+      if (!resnorm_boundary2(op, bf.fd, u, id, i, j, &r)) continue;
+    } else {
+      const double sn = wt.E * u[id + 1] + wt.W * u[id - 1] + wt.N * u[id + op.nx] + wt.S * u[id - op.nx];
+      r = wt.C * u[id] + sn + (bf.f ? bf.f[id] : bf.f_const);
+    }
+    acc += r * r;
+    mx = nmax(mx, fabs(r));
+  }
+  const double s = block_sum(acc, lds);
+  const double m = block_max(mx, lds);
+  if (threadIdx.x == 0) { sums[w] = s; maxs[w] = m; }
+}
+
+// Per-problem gather: sums of the solved-system residual partials (row items) and of the
+// reference-norm partials (element items) in a fixed order, and the outer scalars.
+__global__ void __launch_bounds__(64)
+kb_finish_reduce(const BVec* __restrict__ vecs, const int* __restrict__ rfirst,
+                 const int* __restrict__ rcnt, const double* __restrict__ rr_part,
+                 const int* __restrict__ efirst, const int* __restrict__ ecnt,
+                 const double* __restrict__ sums, const double* __restrict__ maxs, int nprob,
+                 BReport* __restrict__ out) {
+  const int q = blockIdx.x;
+  if (q >= nprob) return;
+  const int lane = threadIdx.x;
+  double rr = 0.0, l2 = 0.0, mx = 0.0;
+  for (int t = lane; t < rcnt[q]; t += 64) rr += rr_part[rfirst[q] + t];
+  for (int t = lane; t < ecnt[q]; t += 64) {
+    l2 += sums[efirst[q] + t];
+    mx = nmax(mx, maxs[efirst[q] + t]);
+  }
+  rr = wave_sum(rr);
+  l2 = wave_sum(l2);
+  mx = wave_max(mx);
+  if (lane != 0) return;
+  const KScalars* so = vecs[q].so;
+  const KScalars* si = vecs[q].si;
+  BReport r{};
+  r.rr = rr;
+  r.l2sq = l2;
+  r.linf = mx;
+  r.metric = so->metric;
+  r.r0 = so->r0;
+  r.done = so->done;
+  r.breakdown = so->breakdown | si->breakdown;
+  r.iter = so->iter;
+  r.steps = so->steps;
+  r.inner = so->pad0;
+  out[q] = r;
+}
+
 unsigned waves_grid(int n) { return static_cast<unsigned>((n + kWavesPB - 1) / kWavesPB); }
 
 }  // namespace
@@ -681,10 +787,12 @@ void vec(hipStream_t s, Prec p, int op, const BVec* vecs, const BItem* items, in
 }
 
 void apply_dot_tab(hipStream_t s, Prec p, const BVec* vecs, const BEnt* tab0, const BItem* rows,
-                   int n, double* partials, bool outer) {
+                   int n, double* partials, bool outer, bool all) {
   if (n <= 0) return;
   const dim3 g(waves_grid(n)), b(kLanes * kWavesPB);
-  if (outer)
+  if (outer && all)
+    hipLaunchKernelGGL((kb_apply_dot<P64, true, true>), g, b, 0, s, vecs, tab0, rows, n, partials);
+  else if (outer)
     hipLaunchKernelGGL((kb_apply_dot<P64, true>), g, b, 0, s, vecs, tab0, rows, n, partials);
   else if (p == P64)
     hipLaunchKernelGGL((kb_apply_dot<P64, false>), g, b, 0, s, vecs, tab0, rows, n, partials);
@@ -715,6 +823,23 @@ void ru_start(hipStream_t s, const BVec* vecs, int nprob) {
 
 void init_scalars(hipStream_t s, const BVec* vecs, int nprob, double eps) {
   hipLaunchKernelGGL(kb_init_scalars, dim3((nprob + 255) / 256), dim3(256), 0, s, vecs, nprob, eps);
+}
+
+void closures(hipStream_t s, const BVec* vecs, const BFace* faces, int nprob, int phase) {
+  hipLaunchKernelGGL(kb_closure, dim3(nprob), dim3(256), 0, s, vecs, faces, nprob, phase);
+}
+
+void resnorm(hipStream_t s, const BVec* vecs, const BFace* faces, const BItem* items, int n,
+             double* sums, double* maxs) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(kb_resnorm, dim3(n), dim3(256), 0, s, vecs, faces, items, n, sums, maxs);
+}
+
+void finish_reduce(hipStream_t s, const BVec* vecs, const int* rfirst, const int* rcnt,
+                   const double* rr_part, const int* efirst, const int* ecnt, const double* sums,
+                   const double* maxs, int nprob, BReport* out) {
+  hipLaunchKernelGGL(kb_finish_reduce, dim3(nprob), dim3(64), 0, s, vecs, rfirst, rcnt, rr_part,
+                     efirst, ecnt, sums, maxs, nprob, out);
 }
 
 void active_count(hipStream_t s, const BVec* vecs, int nprob, int inner, int* active) {

@@ -1,5 +1,6 @@
 // k_time.hip — explicit time-step stage kernels, the reference residual-norm kernel and the
 // single-workgroup lexicographic Gauss-Seidel used by the reference-semantics V-cycle.
+#include "faces_dev.hpp"
 #include "kernels_ext.hpp"
 
 namespace pdehip {
@@ -115,24 +116,7 @@ __global__ void __launch_bounds__(kBlock) k_resnorm(DevOp op, FaceData fd, const
     const int k = static_cast<int>(id / op.sxy);
     double r;
     if (DIM == 2 && is_boundary2(op, i, j)) {
-      int face; long long inner; long long q;
-      if (i == 0) { face = 0; inner = id + 1; q = j; }
-      else if (i == op.nx - 1) { face = 1; inner = id - 1; q = j; }
-      else if (j == 0) { face = 2; inner = id + op.nx; q = i; }
-      else { face = 3; inner = id - op.nx; q = i; }
-      const double h = fd.h[face];
-      const int kind = fd.kind[face];
-      if (kind == PDEHIP_BC_DIRICHLET) {
-        r = u[id] - fd.value[face][q];
-      } else if (kind == PDEHIP_BC_NEUMANN) {
-        const double g = fd.value[face][q];
-        r = u[id] - ((face & 1) ? u[inner] + h * g : u[inner] - h * g);
-      } else {
-        const double al = fd.alpha[face][q], be = fd.beta[face][q], ga = fd.gamma[face][q];
-        const double den = al + be / h;
-        if (fabs(den) <= 1e-12) continue;
-        r = u[id] - (ga + (be / h) * u[inner]) / den;
-      }
+      if (!resnorm_boundary2(op, fd, u, id, i, j, &r)) continue;
     } else {
       if (is_boundary2(op, i, j) || (DIM == 3 && (k == 0 || k == op.nz - 1))) continue;
       const W7<double> w = row_weights<DIM, KIND, double, double>(op, id);

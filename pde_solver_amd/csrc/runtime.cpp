@@ -1197,8 +1197,12 @@ void plan_finish_collect(Plan& P, pdehip_report* rep) {
     rep->rel_residual = r0 > 0 ? std::sqrt(rr) / r0 : std::sqrt(rr);
   }
   double a = 0.0, m = 0.0;
-  for (int i = 0; i < nb; ++i) { a += h[np + i]; m = std::max(m, h[np + nb + i]); }
-  rep->residual_l2 = std::sqrt(std::max(0.0, a));
+  for (int i = 0; i < nb; ++i) {
+    a += h[np + i];
+    const double v = h[np + nb + i];
+    if (v != v || v > m) m = v;   // NaN propagates
+  }
+  rep->residual_l2 = std::sqrt(a);
   rep->residual_linf = m;
 }
 

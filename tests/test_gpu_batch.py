@@ -76,6 +76,12 @@ def test_batch_engine_matches_plans_and_oracle(ctx, precision):
         assert reps[i].status == 0 and reps[i].converged
         assert reps[i].rel_residual < 2e-10
         assert d < tol_plan
+        # the batched finish (closures, reference residual norms) against the oracle's norms
+        # of the same downloaded solution
+        _, l2, linf = O.residual_norms(s, u)
+        noise = 64 * 2.2e-16 * (8.0 * (s.nx - 1) ** 2 + s.e) * np.abs(u).max()   # per-node rounding
+        assert abs(reps[i].residual_l2 - l2) <= 1e-3 * l2 + noise * np.sqrt(s.n)
+        assert abs(reps[i].residual_linf - linf) <= 1e-3 * linf + noise
         if s.nx <= 129:
             _, uo, _ = O.solve(s, opts(capi.FP64, 1e-13))
             assert rel(u, uo) < (1e-8 if precision == capi.FP64 else 5e-7)
