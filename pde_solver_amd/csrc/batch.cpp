@@ -62,13 +62,14 @@ struct BatchPlan {
   DBuf vecs;                               // BVec [n]
   DBuf dense;                              // BDense [n]
   std::vector<List> sweep, rows, crows, drows;   // per level
-  List el0, rows0;                         // level-0 element / row items (with ranges)
+  List el0, rows0;                         // level-0 element / strip items (with ranges)
   std::vector<LevelShape> shape;           // per level work-item shapes
   DBuf part, maxp;
   DBuf active;
   DBuf faces, psum, pmax, rep;             // finish: per-problem boundary data, partials, results
   BReport* hrep = nullptr;                 // pinned copy of rep
-  int* hactive = nullptr;
+  int* hactive = nullptr;                  // two poll slots
+  hipEvent_t evpoll[2] = {nullptr, nullptr};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph[2] = {nullptr, nullptr};   // captured PCG iterations (FP64 / inner)
   size_t bytes = 0;                        // device memory held by the batch and its plans
@@ -76,6 +77,8 @@ struct BatchPlan {
     for (hipGraphExec_t g : graph)
       if (g) (void)hipGraphExecDestroy(g);
     if (hactive) (void)hipHostFree(hactive);
+    for (hipEvent_t e : evpoll)
+      if (e) (void)hipEventDestroy(e);
     if (hrep) (void)hipHostFree(hrep);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -211,6 +214,7 @@ void autotune(BatchPlan& B) {
         const float t = time_ms(B, [&] {
           bk::sweep(s, pl, B.T(l), tmp.d(), tmp.n, 0, 0, false);
           bk::sweep(s, pl, B.T(l), tmp.d(), tmp.n, 0, 1, false);
+          bk::residual(s, pl, B.T(l), tmp.d(), tmp.n);
         }, reps);
         if (t < best) { best = t; pick = yc; }
       }
@@ -225,7 +229,6 @@ void autotune(BatchPlan& B) {
         List tmp;
         upload_list(ctx, s, tmp, row_items(B, l, rn, false));
         const float t = time_ms(B, [&] {
-          bk::residual(s, pl, B.T(l), tmp.d(), tmp.n);
           bk::prolong(s, pc, pl, B.T(l + 1), B.T(l), tmp.d(), tmp.n, s0);
         }, reps);
         if (t < best) { best = t; pick = rn; }
@@ -371,15 +374,18 @@ BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pd
       ec[p] = static_cast<int>(el.size()) - ef[p];
     }
     upload_list(ctx, s, B->el0, el, &ef, &ec);
-    std::vector<int> rf, rcn;
-    const std::vector<BItem> rw = row_items(*B, 0, B->shape[0].row_nodes, false, &rf, &rcn);
+    // level-0 strips of every problem (the smoother's items) with per-problem ranges
+    const std::vector<BItem> rw = sweep_items(*B, 0, B->shape[0].ychunk);
+    std::vector<int> rf(n, 0), rcn(n, 0);
+    for (int t = static_cast<int>(rw.size()) - 1; t >= 0; --t) { rf[rw[t].p] = t; rcn[rw[t].p] += 1; }
     upload_list(ctx, s, B->rows0, rw, &rf, &rcn);
   }
   const int np = std::max(B->el0.n, B->rows0.n);
   B->part.alloc(ctx, (size_t)np * sizeof(double));
   B->maxp.alloc(ctx, (size_t)np * sizeof(double));
-  B->active.alloc(ctx, sizeof(int));
-  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&B->hactive), sizeof(int)));
+  B->active.alloc(ctx, 2 * sizeof(int));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&B->hactive), 2 * sizeof(int)));
+  for (hipEvent_t& e : B->evpoll) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   {
     std::vector<BFace> fc(n);
     for (int p = 0; p < n; ++p) {
@@ -413,7 +419,7 @@ void vcycle(BatchPlan& B) {
       bk::sweep(s, B.prec(l), B.T(l), sw.d(), sw.n, 0, 0, it == 0);
       bk::sweep(s, B.prec(l), B.T(l), sw.d(), sw.n, 0, 1, false);
     }
-    bk::residual(s, B.prec(l), B.T(l), B.rows[l].d(), B.rows[l].n);
+    bk::residual(s, B.prec(l), B.T(l), sw.d(), sw.n);
     bk::restrict_(s, B.prec(l), B.prec(l + 1), B.T(l), B.T(l + 1), B.crows[l + 1].d(),
                   B.crows[l + 1].n, B.mixed && l == 0);
   }
@@ -429,12 +435,19 @@ void vcycle(BatchPlan& B) {
   }
 }
 
-int active(BatchPlan& B, int inner) {
+// Count of unfinished problems, enqueued into poll slot `slot` (device counter -> pinned host
+// word, then an event); poll_read waits for that event only.
+void poll_enqueue(BatchPlan& B, int inner, int slot) {
   const hipStream_t s = B.ctx->stream;
-  bk::active_count(s, B.V(), B.n, inner, B.active.as<int>());
-  HIPCHK(hipMemcpyAsync(B.hactive, B.active.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  return *B.hactive;
+  bk::active_count(s, B.V(), B.n, inner, B.active.as<int>() + slot);
+  HIPCHK(hipMemcpyAsync(B.hactive + slot, B.active.as<int>() + slot, sizeof(int),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(B.evpoll[slot], s));
+}
+
+int poll_read(BatchPlan& B, int slot) {
+  HIPCHK(hipEventSynchronize(B.evpoll[slot]));
+  return B.hactive[slot];
 }
 
 // PCG on every problem in lock step.  Krylov precision kp (P64: the FP64 path on the outer
@@ -476,7 +489,12 @@ void pcg(BatchPlan& B, Prec kp, int max_iter, double delta) {
       bk::vec(s, kp, bk::V_XPAY, B.V(), el.d(), el.n, nullptr, nullptr);
     }
   };
-  for (int it = 0; it < max_iter; it += every) {
+  // Polling is pipelined: the count after chunk c is read while chunk c + 1 is queued behind
+  // it, so the device never waits for the host to enqueue; a chunk that runs after every
+  // problem has finished only launches items that return at once.
+  int outstanding = -1;
+  int chunk = 0;
+  for (int it = 0; it < max_iter; it += every, ++chunk) {
     if (B.cfg.use_graph) {
       // the launch sequence of `every` iterations, captured once per Krylov precision (every
       // pointer and item list is resident, so the graph stays valid for the batch's life)
@@ -500,8 +518,12 @@ void pcg(BatchPlan& B, Prec kp, int max_iter, double delta) {
     } else {
       iterations();
     }
-    if (active(B, inner) == 0) break;
+    const int slot = chunk & 1;
+    poll_enqueue(B, inner, slot);
+    if (outstanding >= 0 && poll_read(B, outstanding) == 0) { outstanding = -1; break; }
+    outstanding = slot;
   }
+  if (outstanding >= 0) (void)poll_read(B, outstanding);
 }
 
 }  // namespace
@@ -589,7 +611,7 @@ double batch_time_kernel(BatchPlan& B, int kernel, int reps, double* algorithmic
       break;
     case 1:   // level-0 residual r = b - A u
       bytes = 3.0 * es * static_cast<double>(n0);
-      ms = time_ms(B, [&] { bk::residual(s, p0, B.T(0), B.rows[0].d(), B.rows[0].n); }, std::max(1, reps));
+      ms = time_ms(B, [&] { bk::residual(s, p0, B.T(0), B.sweep[0].d(), B.sweep[0].n); }, std::max(1, reps));
       break;
     case 2:   // Krylov apply q = A p with p.q partials
       bytes = 2.0 * (kp == P64 ? 8.0 : 4.0) * static_cast<double>(n0);

@@ -55,8 +55,9 @@ struct BReport {
 // Work item: problem p and a range (meaning depends on the list).
 struct BItem { int p, a, b, c; };
 
-// Work-item shape of one level: smoother strip height (rows), nodes per residual/prolongation
-// row block, coarse nodes per restriction block (whole rows, at least one).
+// Work-item shape of one level: strip height in rows of the smoother and the residual (and of
+// the level-0 operator applies), nodes per prolongation row block, coarse nodes per restriction
+// block (whole rows, at least one).
 struct LevelShape { int ychunk = 16; int row_nodes = 256; int crow_nodes = 128; };
 
 struct BatchPlan;
@@ -83,7 +84,7 @@ namespace bk {
 // between u and alt (dir 0: u -> alt, 1: alt -> u).  zero: u is zero (skip its loads).
 void sweep(hipStream_t s, Prec p, const BEnt* tab, const BItem* items, int n, int c1, int dir,
            bool zero);
-// r = b - A u over listed (problem, row range) items; MODE apply: y = A x with x.y partials.
+// r = b - A u over listed (problem, strip, row range) items (the smoother's items).
 void residual(hipStream_t s, Prec p, const BEnt* tab, const BItem* items, int n);
 // Coarse right-hand side (FW + folded closure transpose) for listed coarse (problem, row range).
 void restrict_(hipStream_t s, Prec pf, Prec pc, const BEnt* fine, const BEnt* coarse,
@@ -102,7 +103,7 @@ enum VecOp { V_APPLY_DOT = 0, V_CG_UPDATE = 1, V_DOT_RZ = 2, V_XPAY = 3, V_RESID
 // (V_APPLY_DOT and V_RESID64 are row kernels: apply_dot_tab.)
 void vec(hipStream_t s, Prec p, int op, const BVec* vecs, const BItem* items, int n,
          double* partials, double* maxpart);
-// Level-0 row items: inner q = A p with partials p.q (operator and scalars of tab0), or the
+// Level-0 strip items: inner q = A p with partials p.q (operator and scalars of tab0), or the
 // outer FP64 residual r64 = b64 - A64 x64 with partials r.r (all: also for finished problems).
 void apply_dot_tab(hipStream_t s, Prec p, const BVec* vecs, const BEnt* tab0, const BItem* rows,
                    int n, double* partials, bool outer, bool all = false);
@@ -110,7 +111,7 @@ void apply_dot_tab(hipStream_t s, Prec p, const BVec* vecs, const BEnt* tab0, co
 void reset_inner(hipStream_t s, const BVec* vecs, int nprob);
 // Mixed solve with residual replacement: inner scalars fresh, inner scale = initial residual.
 void ru_start(hipStream_t s, const BVec* vecs, int nprob);
-// Residual replacement over level-0 row items for problems whose inner scalars flag it:
+// Residual replacement over level-0 strip items for problems whose inner scalars flag it:
 // r32 = (b64 - A64 x64) / snorm, partials |b - A x|^2.
 void replace_residual(hipStream_t s, const BVec* vecs, const BItem* rows, int n, double* partials);
 void init_scalars(hipStream_t s, const BVec* vecs, int nprob, double eps);

@@ -6,6 +6,8 @@
 // problem), so problems of any size share a launch.  Items of a problem whose scalars say
 // "done" return at once.  Reductions write one partial per item; items of a problem are
 // contiguous, and the segmented finalize sums each problem's range in a fixed order.
+#include <type_traits>
+
 #include "batch.hpp"
 #include "faces_dev.hpp"
 
@@ -16,6 +18,7 @@ namespace {
 constexpr int kLanes = 64;
 constexpr int kTileX = kLanes - 4;   // smoother strip: 60 output columns, 2-node ring
 constexpr int kWavesPB = 4;          // wavefront items per 256-thread workgroup
+constexpr int kAhead = 1;            // row marches: rows loaded ahead of the row in use
 
 template <Prec P> struct PT;
 template <> struct PT<P64> { using T = double; using S = double; };
@@ -44,6 +47,40 @@ __device__ __forceinline__ W7<T> const_w(const DevOp& op) {
   w.C = T(op.w[0]); w.W = T(op.w[1]); w.E = T(op.w[2]); w.S = T(op.w[3]); w.N = T(op.w[4]);
   w.B = T(0); w.F = T(0);
   return w;
+}
+
+// Strip march over item (p, strip a, rows [b, c)): lanes cover columns a*60-2 .. a*60+61; x rows
+// j-1, j, j+1 (and the right-hand side row j) sit in registers, row j+2 (b row j+1) is loaded
+// one step ahead, x-neighbours come by lane shuffles.  f(i, j, id, xc, xw, xe, xs, xn, bj) runs
+// on the 60 output lanes whose column is inside the grid.  Loads are clamped to the grid.
+template <typename T, typename S, typename SB, typename F>
+__device__ __forceinline__ void strip_march(const S* __restrict__ x, const SB* __restrict__ b,
+                                            int nx, int ny, const BItem& it, F&& f) {
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int i = it.a * kTileX - 2 + lane;
+  const int ci = min(max(i, 0), nx - 1);
+  const bool out = lane >= 2 && lane < kLanes - 2 && i < nx;
+  auto ldx = [&](int j) -> T { return T(ldS(x, (long long)min(max(j, 0), ny - 1) * nx + ci)); };
+  auto ldb = [&](int j) -> T {
+    if constexpr (std::is_same<SB, void>::value) return T(0);
+    else return T(ldS(b, (long long)min(max(j, 0), ny - 1) * nx + ci));
+  };
+  // X[k] = x row j-1+k, Bq[k] = b row j+k; kAhead rows are in flight beyond those in use
+  T X[3 + kAhead], Bq[1 + kAhead];
+#pragma unroll
+  for (int k = 0; k < 2 + kAhead; ++k) X[k] = ldx(it.b - 1 + k);
+#pragma unroll
+  for (int k = 0; k < kAhead; ++k) Bq[k] = ldb(it.b + k);
+  for (int j = it.b; j < it.c; ++j) {
+    X[2 + kAhead] = ldx(j + 1 + kAhead);
+    Bq[kAhead] = ldb(j + kAhead);
+    const T xw = __shfl_up(X[1], 1, kLanes), xe = __shfl_down(X[1], 1, kLanes);
+    if (out) f(i, j, (long long)j * nx + i, X[1], xw, xe, X[0], X[2], Bq[0]);
+#pragma unroll
+    for (int k = 0; k < 2 + kAhead; ++k) X[k] = X[k + 1];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) Bq[k] = Bq[k + 1];
+  }
 }
 
 // ------------------------------------------------------------------------------- smoother
@@ -80,12 +117,22 @@ kb_sweep(const BEnt* __restrict__ tab, const BItem* __restrict__ items, int n, i
   };
   auto ld_b = [&](int j) -> T { return T(ldS(b, (long long)min(max(j, 0), ny - 1) * nx + ci)); };
   const W7<T> w0 = const_w<T>(op);
-  T U[4], B[3];
-  U[0] = U[1] = T(0); U[2] = ld_u(j0 - 2); U[3] = ld_u(j0 - 1);
-  B[0] = B[1] = T(0); B[2] = ld_b(j0 - 1);
+  // row queues: U[k] = x row p-2+k, B[k] = b row p-1+k (k < 4 / 3 in use); the rest are
+  // kAhead rows in flight
+  T U[4 + kAhead], B[3 + kAhead];
+  U[0] = U[1] = T(0);
+#pragma unroll
+  for (int k = 2; k < 4 + kAhead; ++k) U[k] = ld_u(j0 - 4 + k);
+  B[0] = B[1] = T(0);
+#pragma unroll
+  for (int k = 2; k < 3 + kAhead; ++k) B[k] = ld_b(j0 - 3 + k);
   for (int p = j0 - 1; p <= j1; ++p) {
-    U[0] = U[1]; U[1] = U[2]; U[2] = U[3]; U[3] = ld_u(p + 1);
-    B[0] = B[1]; B[1] = B[2]; B[2] = ld_b(p + 1);
+#pragma unroll
+    for (int k = 0; k < 3 + kAhead; ++k) U[k] = U[k + 1];
+    U[3 + kAhead] = ld_u(p + 1 + kAhead);
+#pragma unroll
+    for (int k = 0; k < 2 + kAhead; ++k) B[k] = B[k + 1];
+    B[2 + kAhead] = ld_b(p + 1 + kAhead);
     {
       const T uw = __shfl_up(U[2], 1, kLanes), ue = __shfl_down(U[2], 1, kLanes);
       if (r1 && p >= 1 && p <= ny - 2 && ((i + p) & 1) == c1) {
@@ -108,7 +155,7 @@ kb_sweep(const BEnt* __restrict__ tab, const BItem* __restrict__ items, int n, i
 }
 
 // ------------------------------------------------------------------------------- residual
-// r = b - A u on rows [a, b) of problem p (boundary rows b - u).
+// r = b - A u over strip item (p, strip, rows) of problem p (boundary rows b - u).
 template <Prec P>
 __global__ void __launch_bounds__(kLanes * kWavesPB)
 kb_residual(const BEnt* __restrict__ tab, const BItem* __restrict__ items, int n) {
@@ -120,29 +167,21 @@ kb_residual(const BEnt* __restrict__ tab, const BItem* __restrict__ items, int n
   const BEnt& e = tab[it.p];
   if (done(e.sc)) return;
   const DevOp& op = e.op;
-  const S* __restrict__ u = static_cast<const S*>(e.u);
-  const S* __restrict__ b = static_cast<const S*>(e.b);
   S* __restrict__ r = static_cast<S*>(e.r);
-  const int lane = threadIdx.x & (kLanes - 1);
-  const long long nx = op.nx;
   const W7<T> w0 = const_w<T>(op);
-  for (int j = it.a; j < it.b; ++j) {
-    for (int i = lane; i < op.nx; i += kLanes) {
-      const long long id = j * nx + i;
-      const T uc = T(ldS(u, id));
-      T out;
-      if (i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1) {
-        out = T(ldS(b, id)) - uc;
-      } else {
-        W7<T> wt = w0;
-        fold_center<2, T>(op, i, j, 0, wt);
-        const T ax = wt.C * uc + wt.E * T(ldS(u, id + 1)) + wt.W * T(ldS(u, id - 1)) +
-                     wt.N * T(ldS(u, id + nx)) + wt.S * T(ldS(u, id - nx));
-        out = T(ldS(b, id)) - ax;
-      }
-      r[id] = store_as<S>(static_cast<double>(out));
+  strip_march<T>(static_cast<const S*>(e.u), static_cast<const S*>(e.b), op.nx, op.ny, it,
+                 [&](int i, int j, long long id, T uc, T uw, T ue, T us, T un, T bj) {
+    T out;
+    if (i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1) {
+      out = bj - uc;
+    } else {
+      W7<T> wt = w0;
+      fold_center<2, T>(op, i, j, 0, wt);
+      const T ax = wt.C * uc + wt.E * ue + wt.W * uw + wt.N * un + wt.S * us;
+      out = bj - ax;
     }
-  }
+    r[id] = store_as<S>(static_cast<double>(out));
+  });
 }
 
 // ------------------------------------------------------------------------------- transfers
@@ -199,6 +238,7 @@ kb_restrict(const BEnt* __restrict__ ftab, const BEnt* __restrict__ ctab,
     int yi[3];
     double yw[3];
     const int my = fold_list(jc, c.ny, c.theta[2], c.theta[3], yi, yw);
+#pragma unroll 2
     for (int ic = lane; ic < c.nx; ic += kLanes) {
       if (ic == 0 || ic == c.nx - 1) { row[ic] = store_as<SC>(0.0); continue; }
       int xi[3];
@@ -244,6 +284,7 @@ kb_prolong(const BEnt* __restrict__ ctab, const BEnt* __restrict__ ftab,
     const double hy = oj ? 0.5 : 1.0;
     const SC* r0 = uc + (long long)y0 * c.nx;
     const SC* r1 = uc + (long long)y1 * c.nx;
+#pragma unroll 4
     for (int i = 1 + lane; i < f.nx - 1; i += kLanes) {
       const int ic = i >> 1, oi = i & 1;
       int x0 = ic, x1 = ic + 1;
@@ -313,6 +354,7 @@ kb_vec(const BVec* __restrict__ vecs, const BItem* __restrict__ items, int n, in
     S* pp = static_cast<S*>(v.p);
     const S* q = static_cast<const S*>(v.q);
     const S* z = static_cast<const S*>(v.z);
+#pragma unroll 4
     for (long long i = it.a + threadIdx.x; i < it.b; i += blockDim.x) {
       switch (op) {
         case V_CG_UPDATE: {
@@ -392,32 +434,27 @@ kb_apply_dot(const BVec* __restrict__ vecs, const BEnt* __restrict__ tab0,
   if (ALL || !done(sc)) {
     const DevOp& op = OUTER ? v.op64 : tab0[it.p].op;
     const S* x = OUTER ? reinterpret_cast<const S*>(v.x64) : static_cast<const S*>(v.p);
-    const S* b = OUTER ? reinterpret_cast<const S*>(v.b64) : nullptr;
     S* y = OUTER ? reinterpret_cast<S*>(v.r64) : static_cast<S*>(v.q);
-    const long long nx = op.nx;
     const W7<T> w0 = const_w<T>(op);
     const T e = T(op.e);
-    for (int j = it.a; j < it.b; ++j) {
-      for (int i = lane; i < op.nx; i += kLanes) {
-        const long long id = j * nx + i;
-        const T xc = T(ldS(x, id));
-        T ax;
-        if (i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1) {
-          ax = xc;
-        } else {
-          ax = apply_diff<2, T>(op, w0, e, i, j, 0, xc, T(ldS(x, id - 1)), T(ldS(x, id + 1)),
-                                T(ldS(x, id - nx)), T(ldS(x, id + nx)), T(0), T(0));
-        }
-        if constexpr (OUTER) {
-          const T rv = T(ldS(b, id)) - ax;
-          y[id] = store_as<S>(static_cast<double>(rv));
-          acc += static_cast<double>(rv) * static_cast<double>(rv);
-        } else {
-          y[id] = store_as<S>(static_cast<double>(ax));
-          acc += static_cast<double>(xc) * static_cast<double>(ax);
-        }
+    auto body = [&](int i, int j, long long id, T xc, T xw, T xe, T xs, T xn, T bj) {
+      T ax;
+      if (i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1) {
+        ax = xc;
+      } else {
+        ax = apply_diff<2, T>(op, w0, e, i, j, 0, xc, xw, xe, xs, xn, T(0), T(0));
       }
-    }
+      if constexpr (OUTER) {
+        const T rv = bj - ax;
+        y[id] = store_as<S>(static_cast<double>(rv));
+        acc += static_cast<double>(rv) * static_cast<double>(rv);
+      } else {
+        y[id] = store_as<S>(static_cast<double>(ax));
+        acc += static_cast<double>(xc) * static_cast<double>(ax);
+      }
+    };
+    if constexpr (OUTER) strip_march<T>(x, reinterpret_cast<const S*>(v.b64), op.nx, op.ny, it, body);
+    else strip_march<T>(x, static_cast<const void*>(nullptr), op.nx, op.ny, it, body);
   }
   acc = wave_sum(acc);
   if (lane == 0) partials[w] = acc;
@@ -438,29 +475,23 @@ kb_replace(const BVec* __restrict__ vecs, const BItem* __restrict__ items, int n
   const KScalars* si = v.si;
   if (!done(si) && si->flag_s) {
     const DevOp& op = v.op64;
-    const double* x = v.x64;
-    const double* b = v.b64;
     float* r = static_cast<float*>(v.r);
     const double inv = 1.0 / v.so->snorm;
-    const long long nx = op.nx;
     const W7<double> w0 = const_w<double>(op);
     const double e = op.e;
-    for (int j = it.a; j < it.b; ++j) {
-      for (int i = lane; i < op.nx; i += kLanes) {
-        const long long id = j * nx + i;
-        const double xc = x[id];
-        double ax;
-        if (i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1) {
-          ax = xc;
-        } else {
-          ax = apply_diff<2, double>(op, w0, e, i, j, 0, xc, x[id - 1], x[id + 1], x[id - nx],
-                                     x[id + nx], 0.0, 0.0);
-        }
-        const double rv = b[id] - ax;
-        r[id] = static_cast<float>(rv * inv);
-        acc += rv * rv;
+    strip_march<double>(v.x64, static_cast<const double*>(v.b64), op.nx, op.ny, it,
+                        [&](int i, int j, long long id, double xc, double xw, double xe, double xs,
+                            double xn, double bj) {
+      double ax;
+      if (i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1) {
+        ax = xc;
+      } else {
+        ax = apply_diff<2, double>(op, w0, e, i, j, 0, xc, xw, xe, xs, xn, 0.0, 0.0);
       }
-    }
+      const double rv = bj - ax;
+      r[id] = static_cast<float>(rv * inv);
+      acc += rv * rv;
+    });
   }
   acc = wave_sum(acc);
   if (lane == 0) partials[w] = acc;
