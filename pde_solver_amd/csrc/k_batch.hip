@@ -25,20 +25,48 @@ template <> struct PT<P64> { using T = double; using S = double; };
 template <> struct PT<P32> { using T = float; using S = float; };
 template <> struct PT<P16> { using T = float; using S = __half; };
 
+// Buffer access.  Pointers read from the problem tables are generic (flat) to the compiler, and
+// flat loads count against the LDS/scalar counter as well as the vector-memory one, so every
+// shuffle or scalar wait would also wait for the rows in flight.  Every batch buffer is device
+// memory: loads and stores go through the global address space.
+#define PDEHIP_GAS __attribute__((address_space(1)))
 template <typename S> __device__ __forceinline__ auto ldS(const S* p, long long i);
-template <> __device__ __forceinline__ auto ldS<double>(const double* p, long long i) { return p[i]; }
-template <> __device__ __forceinline__ auto ldS<float>(const float* p, long long i) { return p[i]; }
-template <> __device__ __forceinline__ auto ldS<__half>(const __half* p, long long i) { return __half2float(p[i]); }
+template <> __device__ __forceinline__ auto ldS<double>(const double* p, long long i) {
+  return ((const PDEHIP_GAS double*)p)[i];
+}
+template <> __device__ __forceinline__ auto ldS<float>(const float* p, long long i) {
+  return ((const PDEHIP_GAS float*)p)[i];
+}
+template <> __device__ __forceinline__ auto ldS<__half>(const __half* p, long long i) {
+  return __half2float(__ushort_as_half(((const PDEHIP_GAS unsigned short*)p)[i]));
+}
+template <> __device__ __forceinline__ auto ldS<int>(const int* p, long long i) {
+  return ((const PDEHIP_GAS int*)p)[i];
+}
+__device__ __forceinline__ void stS(double* p, long long i, double v) { ((PDEHIP_GAS double*)p)[i] = v; }
+__device__ __forceinline__ void stS(float* p, long long i, float v) { ((PDEHIP_GAS float*)p)[i] = v; }
+__device__ __forceinline__ void stS(__half* p, long long i, __half v) {
+  ((PDEHIP_GAS unsigned short*)p)[i] = __half_as_ushort(v);
+}
 
 template <typename S, typename T> __device__ __forceinline__ T rounded(T v);
 template <> __device__ __forceinline__ double rounded<double, double>(double v) { return v; }
 template <> __device__ __forceinline__ float rounded<float, float>(float v) { return v; }
 template <> __device__ __forceinline__ float rounded<__half, float>(float v) { return __half2float(__float2half(v)); }
 
-__device__ __forceinline__ bool done(const KScalars* sc) { return sc && (sc->done | sc->breakdown); }
+__device__ __forceinline__ bool done(const KScalars* sc) {
+  return sc && (ldS(&sc->done, 0) | ldS(&sc->breakdown, 0));
+}
 
+// Wavefront item index.  Workgroup slot b is dispatched to XCD b % 8; numbering the
+// workgroups XCD-contiguously ((b % 8) * per + b / 8, gridDim.x = 8 * per) puts consecutive
+// items (neighbouring strips and row chunks of one problem, which share halo rows and cache
+// lines) in the same XCD's L2.  Indices past the list return in the callers (w >= n).
+// The index is wave-uniform (readfirstlane), so the item and table reads are scalar loads.
 __device__ __forceinline__ int wave_item() {
-  return static_cast<int>(blockIdx.x) * kWavesPB + static_cast<int>(threadIdx.x / kLanes);
+  const int per = static_cast<int>(gridDim.x >> 3);
+  const int t = static_cast<int>(blockIdx.x & 7) * per + static_cast<int>(blockIdx.x >> 3);
+  return __builtin_amdgcn_readfirstlane(t * kWavesPB + static_cast<int>(threadIdx.x / kLanes));
 }
 
 template <typename T>
@@ -149,7 +177,7 @@ kb_sweep(const BEnt* __restrict__ tab, const BItem* __restrict__ items, int n, i
         fold_center<2, T>(op, i, q, 0, wt);
         U[1] = rounded<S, T>(gs_value<2, T>(wt, B[0], ue, uw, U[2], U[0], T(0), T(0)));
       }
-      if (out) xout[q * nx + i] = store_as<S>(static_cast<double>(U[1]));
+      if (out) stS(xout, q * nx + i, store_as<S>(static_cast<double>(U[1])));
     }
   }
 }
@@ -180,7 +208,7 @@ kb_residual(const BEnt* __restrict__ tab, const BItem* __restrict__ items, int n
       const T ax = wt.C * uc + wt.E * ue + wt.W * uw + wt.N * un + wt.S * us;
       out = bj - ax;
     }
-    r[id] = store_as<S>(static_cast<double>(out));
+    stS(r, id, store_as<S>(static_cast<double>(out)));
   });
 }
 
@@ -232,7 +260,7 @@ kb_restrict(const BEnt* __restrict__ ftab, const BEnt* __restrict__ ctab,
   for (int jc = it.a; jc < it.b; ++jc) {
     SC* row = bc + (long long)jc * c.nx;
     if (jc == 0 || jc == c.ny - 1) {
-      for (int ic = lane; ic < c.nx; ic += kLanes) row[ic] = store_as<SC>(0.0);
+      for (int ic = lane; ic < c.nx; ic += kLanes) stS(row, ic, store_as<SC>(0.0));
       continue;
     }
     int yi[3];
@@ -240,7 +268,7 @@ kb_restrict(const BEnt* __restrict__ ftab, const BEnt* __restrict__ ctab,
     const int my = fold_list(jc, c.ny, c.theta[2], c.theta[3], yi, yw);
 #pragma unroll 2
     for (int ic = lane; ic < c.nx; ic += kLanes) {
-      if (ic == 0 || ic == c.nx - 1) { row[ic] = store_as<SC>(0.0); continue; }
+      if (ic == 0 || ic == c.nx - 1) { stS(row, ic, store_as<SC>(0.0)); continue; }
       int xi[3];
       double xw[3];
       const int mx = fold_list(ic, c.nx, c.theta[0], c.theta[1], xi, xw);
@@ -250,7 +278,7 @@ kb_restrict(const BEnt* __restrict__ ftab, const BEnt* __restrict__ ctab,
           const double wt = yw[bb] * xw[q];
           if (wt != 0.0) acc += wt * fw2(f, rf, 2 * xi[q], 2 * yi[bb]);
         }
-      row[ic] = store_as<SC>(acc * s);
+      stS(row, ic, store_as<SC>(acc * s));
     }
   }
 }
@@ -304,7 +332,7 @@ kb_prolong(const BEnt* __restrict__ ctab, const BEnt* __restrict__ ftab,
         a += wy1 * (v * hx);
       }
       const long long id = (long long)j * f.nx + i;
-      uf[id] = store_as<SF>(static_cast<double>(ldS(uf, id)) + s * (a * hy));
+      stS(uf, id, store_as<SF>(static_cast<double>(ldS(uf, id)) + s * (a * hy)));
     }
   }
 }
@@ -327,9 +355,9 @@ kb_dense(const BEnt* __restrict__ tab, const BDense* __restrict__ dense,
   const int row = it.a;
   double acc = 0.0;
   for (int q = lane; q < d.m; q += kLanes)
-    acc += d.ainv[(long long)row * d.m + q] * static_cast<double>(ldS(b, d.map[q]));
+    acc += ldS(d.ainv, (long long)row * d.m + q) * static_cast<double>(ldS(b, ldS(d.map, q)));
   acc = wave_sum(acc);
-  if (lane == 0) x[d.map[row]] = store_as<S>(acc);
+  if (lane == 0) stS(x, ldS(d.map, row), store_as<S>(acc));
 }
 
 // ------------------------------------------------------------------------------- level-0 Krylov
@@ -347,67 +375,86 @@ kb_vec(const BVec* __restrict__ vecs, const BItem* __restrict__ items, int n, in
   const BVec& v = vecs[it.p];
   const KScalars* sc = inner ? v.si : v.so;
   double acc = 0.0, mx = 0.0;
-  const bool skip_all = done(sc);
-  if (!skip_all) {
+  if (!done(sc)) {
     S* x = static_cast<S*>(v.x);
     S* r = static_cast<S*>(v.r);
     S* pp = static_cast<S*>(v.p);
     const S* q = static_cast<const S*>(v.q);
     const S* z = static_cast<const S*>(v.z);
+    const long long i0 = it.a + threadIdx.x, i1 = it.b;
+    const int st = blockDim.x;
+    switch (op) {
+      case V_CG_UPDATE: {
+        const double alpha = ldS(&sc->alpha, 0);
+        const T a = T(alpha);
+        if (inner) {
+          // the inner correction goes straight into the FP64 solution: x64 += |r64| alpha p
+          const double ax = ldS(&v.so->snorm, 0) * alpha;
 #pragma unroll 4
-    for (long long i = it.a + threadIdx.x; i < it.b; i += blockDim.x) {
-      switch (op) {
-        case V_CG_UPDATE: {
-          const T a = T(sc->alpha);
-          if (inner) {
-            // the inner correction goes straight into the FP64 solution: x64 += |r64| alpha p
-            v.x64[i] += (v.so->snorm * sc->alpha) * static_cast<double>(ldS(pp, i));
-          } else {
-            x[i] = store_as<S>(static_cast<double>(T(ldS(x, i)) + a * T(ldS(pp, i))));
+          for (long long i = i0; i < i1; i += st) {
+            stS(v.x64, i, ldS(v.x64, i) + ax * static_cast<double>(ldS(pp, i)));
+            const T ri = T(ldS(r, i)) - a * T(ldS(q, i));
+            stS(r, i, store_as<S>(static_cast<double>(ri)));
+            acc += static_cast<double>(ri) * static_cast<double>(ri);
+            mx = nmax(mx, fabs(static_cast<double>(ri)));
           }
-          const T ri = T(ldS(r, i)) - a * T(ldS(q, i));
-          r[i] = store_as<S>(static_cast<double>(ri));
-          acc += static_cast<double>(ri) * static_cast<double>(ri);
-          mx = nmax(mx, fabs(static_cast<double>(ri)));
-          break;
+        } else {
+#pragma unroll 4
+          for (long long i = i0; i < i1; i += st) {
+            stS(x, i, store_as<S>(static_cast<double>(T(ldS(x, i)) + a * T(ldS(pp, i)))));
+            const T ri = T(ldS(r, i)) - a * T(ldS(q, i));
+            stS(r, i, store_as<S>(static_cast<double>(ri)));
+            acc += static_cast<double>(ri) * static_cast<double>(ri);
+            mx = nmax(mx, fabs(static_cast<double>(ri)));
+          }
         }
-        case V_DOT_RZ:
+        break;
+      }
+      case V_DOT_RZ:
+#pragma unroll 4
+        for (long long i = i0; i < i1; i += st)
           acc += static_cast<double>(ldS(r, i)) * static_cast<double>(ldS(z, i));
-          break;
-        case V_XPAY: {
-          const T bt = T(sc->beta);
-          pp[i] = store_as<S>(static_cast<double>(T(ldS(z, i)) + bt * T(ldS(pp, i))));
-          break;
-        }
-        case V_COPY_ZP:
-          pp[i] = z[i];
-          break;
-        case V_RR: {
+        break;
+      case V_XPAY: {
+        const T bt = T(ldS(&sc->beta, 0));
+#pragma unroll 4
+        for (long long i = i0; i < i1; i += st)
+          stS(pp, i, store_as<S>(static_cast<double>(T(ldS(z, i)) + bt * T(ldS(pp, i)))));
+        break;
+      }
+      case V_COPY_ZP:
+#pragma unroll 4
+        for (long long i = i0; i < i1; i += st) stS(pp, i, store_as<S>(static_cast<double>(ldS(z, i))));
+        break;
+      case V_RR:
+#pragma unroll 4
+        for (long long i = i0; i < i1; i += st) {
           const double ri = static_cast<double>(ldS(r, i));
           acc += ri * ri;
           mx = nmax(mx, fabs(ri));
-          break;
         }
-        case V_ZERO_X:
-          x[i] = store_as<S>(0.0);
-          break;
-        case V_TO32: {   // inner rhs = r64 / snorm (the fixed inner scale)
-          const double nr = v.so->snorm;
-          const double s = nr > 0.0 ? 1.0 / nr : 1.0;
-          r[i] = store_as<S>(v.r64[i] * s);
-          break;
-        }
-        case V_COPY_X0:
-          v.x64[i] = v.x0[i];
-          break;
-        default:
-          break;
+        break;
+      case V_ZERO_X:
+        for (long long i = i0; i < i1; i += st) stS(x, i, store_as<S>(0.0));
+        break;
+      case V_TO32: {   // inner rhs = r64 / snorm (the fixed inner scale)
+        const double nr = ldS(&v.so->snorm, 0);
+        const double sinv = nr > 0.0 ? 1.0 / nr : 1.0;
+#pragma unroll 4
+        for (long long i = i0; i < i1; i += st) stS(r, i, store_as<S>(ldS(v.r64, i) * sinv));
+        break;
       }
+      case V_COPY_X0:
+#pragma unroll 4
+        for (long long i = i0; i < i1; i += st) stS(v.x64, i, ldS(v.x0, i));
+        break;
+      default:
+        break;
     }
   }
   if (partials) {
-    const double s = block_sum(acc, lds);
-    if (threadIdx.x == 0) partials[w] = s;
+    const double bs = block_sum(acc, lds);
+    if (threadIdx.x == 0) partials[w] = bs;
   }
   if (maxpart) {
     const double m = block_max(mx, lds);
@@ -446,10 +493,10 @@ kb_apply_dot(const BVec* __restrict__ vecs, const BEnt* __restrict__ tab0,
       }
       if constexpr (OUTER) {
         const T rv = bj - ax;
-        y[id] = store_as<S>(static_cast<double>(rv));
+        stS(y, id, store_as<S>(static_cast<double>(rv)));
         acc += static_cast<double>(rv) * static_cast<double>(rv);
       } else {
-        y[id] = store_as<S>(static_cast<double>(ax));
+        stS(y, id, store_as<S>(static_cast<double>(ax)));
         acc += static_cast<double>(xc) * static_cast<double>(ax);
       }
     };
@@ -489,7 +536,7 @@ kb_replace(const BVec* __restrict__ vecs, const BItem* __restrict__ items, int n
         ax = apply_diff<2, double>(op, w0, e, i, j, 0, xc, xw, xe, xs, xn, 0.0, 0.0);
       }
       const double rv = bj - ax;
-      r[id] = static_cast<float>(rv * inv);
+      stS(r, id, static_cast<float>(rv * inv));
       acc += rv * rv;
     });
   }
@@ -741,7 +788,11 @@ kb_finish_reduce(const BVec* __restrict__ vecs, const int* __restrict__ rfirst,
   out[q] = r;
 }
 
-unsigned waves_grid(int n) { return static_cast<unsigned>((n + kWavesPB - 1) / kWavesPB); }
+// Workgroups for n wavefront items, rounded up to a multiple of 8 (XCD-contiguous numbering).
+unsigned waves_grid(int n) {
+  const int wg = (n + kWavesPB - 1) / kWavesPB;
+  return static_cast<unsigned>((wg + 7) / 8 * 8);
+}
 
 }  // namespace
 
