@@ -20,19 +20,24 @@ int blocks_for(long long n) {
 }
 
 // ----------------------------------------------------------------------------- BiCGStab
-__global__ void k_bicg_p(const double* __restrict__ r, double* __restrict__ p,
-                         const double* __restrict__ v, long long n, const KScalars* sc) {
+// Vector element type S is double (FP64 path) or float (mixed path, FP32 Krylov vectors);
+// arithmetic and reductions are in double.  In the mixed path the solution stays FP64 and the
+// x updates take a scale (the FP32 vectors hold the residual divided by it).
+template <typename S>
+__global__ void k_bicg_p(const S* __restrict__ r, S* __restrict__ p,
+                         const S* __restrict__ v, long long n, const KScalars* sc) {
   if (skip(sc)) return;
   const bool first = sc->iter == 0;
   const double beta = sc->beta, omega = sc->omega;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
-    p[i] = first ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
+    p[i] = static_cast<S>(first ? double(r[i]) : double(r[i]) + beta * (double(p[i]) - omega * double(v[i])));
 }
 
-__global__ void __launch_bounds__(kBlock) k_bicg_s(const double* __restrict__ r,
-                                                  const double* __restrict__ v,
-                                                  double* __restrict__ s, long long n,
+template <typename S>
+__global__ void __launch_bounds__(kBlock) k_bicg_s(const S* __restrict__ r,
+                                                  const S* __restrict__ v,
+                                                  S* __restrict__ s, long long n,
                                                   double* partials, const KScalars* sc) {
   __shared__ double lds[kBlock / kWave];
   if (skip(sc)) return;
@@ -40,54 +45,78 @@ __global__ void __launch_bounds__(kBlock) k_bicg_s(const double* __restrict__ r,
   double acc = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    const double si = r[i] - alpha * v[i];
+    const S si = static_cast<S>(double(r[i]) - alpha * double(v[i]));
     s[i] = si;
-    acc += si * si;
+    acc += double(si) * double(si);
   }
   const double t = block_sum(acc, lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// x += alpha*p_hat when ||s|| already met the target (krylov.cpp:143-148).
-__global__ void k_bicg_xs(double* __restrict__ x, const double* __restrict__ ph, long long n,
-                          const KScalars* sc) {
+// x += scale*alpha*p_hat when ||s|| already met the target (krylov.cpp:143-148).
+template <typename S>
+__global__ void k_bicg_xs(double* __restrict__ x, const S* __restrict__ ph, long long n,
+                          const KScalars* sc, const double* scale) {
   if (skip(sc) || !sc->flag_s) return;
-  const double alpha = sc->alpha;
+  const double alpha = sc->alpha * (scale ? *scale : 1.0);
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
-    x[i] += alpha * ph[i];
+    x[i] += alpha * double(ph[i]);
 }
 
-__global__ void __launch_bounds__(kBlock) k_dot2(const double* __restrict__ a,
-                                                const double* __restrict__ b,
-                                                const double* __restrict__ c, long long n,
+template <typename S>
+__global__ void __launch_bounds__(kBlock) k_dot2(const S* __restrict__ a,
+                                                const S* __restrict__ b,
+                                                const S* __restrict__ c, long long n,
                                                 double* p1, double* p2, const KScalars* sc) {
   __shared__ double lds[kBlock / kWave];
   if (skip(sc)) return;
   double ab = 0.0, ac = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    ab += a[i] * b[i];
-    ac += a[i] * c[i];
+    ab += double(a[i]) * double(b[i]);
+    ac += double(a[i]) * double(c[i]);
   }
   const double s1 = block_sum(ab, lds);
   const double s2 = block_sum(ac, lds);
   if (threadIdx.x == 0) { p1[blockIdx.x] = s1; p2[blockIdx.x] = s2; }
 }
 
-__global__ void k_bicg_x(double* __restrict__ x, const double* __restrict__ ph,
-                         const double* __restrict__ sh, long long n, const KScalars* sc) {
+template <typename S>
+__global__ void k_bicg_x(double* __restrict__ x, const S* __restrict__ ph,
+                         const S* __restrict__ sh, long long n, const KScalars* sc,
+                         const double* scale) {
   if (skip(sc)) return;
-  const double alpha = sc->alpha, omega = sc->omega;
+  const double f = scale ? *scale : 1.0;
+  const double alpha = sc->alpha * f, omega = sc->omega * f;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
-    x[i] += alpha * ph[i] + omega * sh[i];
+    x[i] += alpha * double(ph[i]) + omega * double(sh[i]);
+}
+
+// Mixed path: r = s - omega t (the recurrence), partials of r.r.
+__global__ void __launch_bounds__(kBlock) k_bicg_r(const float* __restrict__ sv,
+                                                  const float* __restrict__ t,
+                                                  float* __restrict__ r, long long n,
+                                                  double* partials, const KScalars* sc) {
+  __shared__ double lds[kBlock / kWave];
+  if (skip(sc)) return;
+  const double omega = sc->omega;
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float ri = static_cast<float>(double(sv[i]) - omega * double(t[i]));
+    r[i] = ri;
+    acc += double(ri) * double(ri);
+  }
+  const double bs = block_sum(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
 // Jacobi preconditioner: out = in / diag on interior rows (folded diagonal), identity on
 // boundary rows.
-template <int DIM, int KIND>
-__global__ void k_jacobi_pc(DevOp op, const double* __restrict__ in, double* __restrict__ out,
+template <int DIM, int KIND, typename S>
+__global__ void k_jacobi_pc(DevOp op, const S* __restrict__ in, S* __restrict__ out,
                             const KScalars* sc) {
   if (sc && skip(sc)) return;
   for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < op.n;
@@ -98,7 +127,7 @@ __global__ void k_jacobi_pc(DevOp op, const double* __restrict__ in, double* __r
     if (is_boundary2(op, i, j) || (DIM == 3 && (k == 0 || k == op.nz - 1))) { out[id] = in[id]; continue; }
     W7<double> w = row_weights<DIM, KIND, double, double>(op, id);
     fold_center<DIM, double>(op, i, j, k, w);
-    out[id] = in[id] / w.C;
+    out[id] = static_cast<S>(double(in[id]) / w.C);
   }
 }
 
@@ -106,8 +135,9 @@ __global__ void k_jacobi_pc(DevOp op, const double* __restrict__ in, double* __r
 constexpr int kMaxBasis = 64;
 
 // partials[i*nb + block] = V_i . w for i < cnt (V rows of length ld).
-__global__ void __launch_bounds__(kBlock) k_multidot(const double* __restrict__ V, long long ld,
-                                                    int cnt, const double* __restrict__ w,
+template <typename S>
+__global__ void __launch_bounds__(kBlock) k_multidot(const S* __restrict__ V, long long ld,
+                                                    int cnt, const S* __restrict__ w,
                                                     long long n, double* partials,
                                                     const KScalars* sc) {
   __shared__ double lds[kBlock / kWave];
@@ -118,10 +148,10 @@ __global__ void __launch_bounds__(kBlock) k_multidot(const double* __restrict__ 
     const int m = min(8, cnt - i0);
     for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
          q += (long long)gridDim.x * blockDim.x) {
-      const double wv = w[q];
+      const double wv = double(w[q]);
 #pragma unroll
       for (int t = 0; t < 8; ++t)
-        if (t < m) acc[t] += V[(long long)(i0 + t) * ld + q] * wv;
+        if (t < m) acc[t] += double(V[(long long)(i0 + t) * ld + q]) * wv;
     }
     for (int t = 0; t < m; ++t) {
       const double s = block_sum(acc[t], lds);
@@ -131,15 +161,16 @@ __global__ void __launch_bounds__(kBlock) k_multidot(const double* __restrict__ 
 }
 
 // w -= sum_i V_i h_i (h from device memory); also used with sign +1 for x updates.
-__global__ void k_multiaxpy(const double* __restrict__ V, long long ld, int cnt,
-                            const double* __restrict__ h, double* __restrict__ w, long long n,
+template <typename S>
+__global__ void k_multiaxpy(const S* __restrict__ V, long long ld, int cnt,
+                            const double* __restrict__ h, S* __restrict__ w, long long n,
                             double sign, int check_cycle, const KScalars* sc) {
   if (skip(sc) || (check_cycle && sc->cycle_done)) return;
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
        q += (long long)gridDim.x * blockDim.x) {
     double acc = 0.0;
-    for (int i = 0; i < cnt; ++i) acc += h[i] * V[(long long)i * ld + q];
-    w[q] += sign * acc;
+    for (int i = 0; i < cnt; ++i) acc += h[i] * double(V[(long long)i * ld + q]);
+    w[q] = static_cast<S>(double(w[q]) + sign * acc);
   }
 }
 
@@ -157,14 +188,15 @@ __global__ void __launch_bounds__(256) k_fin_multi(const double* partials, int n
 }
 
 // H[j+1][j] = ||w||, V_{j+1} = w / H[j+1][j]
-__global__ void k_gm_normalize(double* __restrict__ vnext, const double* __restrict__ w,
+template <typename S>
+__global__ void k_gm_normalize(S* __restrict__ vnext, const S* __restrict__ w,
                                long long n, const double* hval, const KScalars* sc) {
   if (skip(sc) || sc->cycle_done) return;
   const double h = *hval;
   if (!(h > 0.0)) return;
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
        q += (long long)gridDim.x * blockDim.x)
-    vnext[q] = w[q] / h;
+    vnext[q] = static_cast<S>(double(w[q]) / h);
 }
 
 __global__ void __launch_bounds__(256) k_fin_norm(const double* partials, int nb, double* out,
@@ -216,13 +248,14 @@ __global__ void k_backsub(const double* H, const double* g, double* y, int m, KS
 }
 
 // New cycle: V_0 = r / beta, g = [beta, 0, ...]; clears the cycle flags.
-__global__ void k_gm_start(double* __restrict__ v0, const double* __restrict__ r, long long n,
+template <typename S>
+__global__ void k_gm_start(S* __restrict__ v0, const double* __restrict__ r, long long n,
                            double* g, int m, KScalars* sc) {
   if (skip(sc)) return;
   const double beta = sc->metric;
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
        q += (long long)gridDim.x * blockDim.x)
-    v0[q] = r[q] / beta;
+    v0[q] = static_cast<S>(r[q] / beta);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     // cycle flags are reset by k_gm_reset before this kernel's first use
     g[0] = beta;
@@ -302,49 +335,79 @@ __global__ void k_bi_sdone(KScalars* sc) {
 
 int krylov_blocks(long long n) { return blocks_for(n); }
 
-void bicg_p(hipStream_t s, const double* r, double* p, const double* v, long long n, const KScalars* sc) {
-  hipLaunchKernelGGL(k_bicg_p, dim3(blocks_for(n)), dim3(kBlock), 0, s, r, p, v, n, sc);
+template <typename S>
+void bicg_p(hipStream_t s, const S* r, S* p, const S* v, long long n, const KScalars* sc) {
+  hipLaunchKernelGGL(k_bicg_p<S>, dim3(blocks_for(n)), dim3(kBlock), 0, s, r, p, v, n, sc);
 }
-void bicg_s(hipStream_t s, const double* r, const double* v, double* sv, long long n, double* partials, const KScalars* sc) {
-  hipLaunchKernelGGL(k_bicg_s, dim3(blocks_for(n)), dim3(kBlock), 0, s, r, v, sv, n, partials, sc);
+template <typename S>
+void bicg_s(hipStream_t s, const S* r, const S* v, S* sv, long long n, double* partials, const KScalars* sc) {
+  hipLaunchKernelGGL(k_bicg_s<S>, dim3(blocks_for(n)), dim3(kBlock), 0, s, r, v, sv, n, partials, sc);
 }
-void bicg_xs(hipStream_t s, double* x, const double* ph, long long n, const KScalars* sc) {
-  hipLaunchKernelGGL(k_bicg_xs, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, ph, n, sc);
+template <typename S>
+void bicg_xs(hipStream_t s, double* x, const S* ph, long long n, const KScalars* sc, const double* scale) {
+  hipLaunchKernelGGL(k_bicg_xs<S>, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, ph, n, sc, scale);
 }
-void dot2(hipStream_t s, const double* a, const double* b, const double* c, long long n, double* p1, double* p2, const KScalars* sc) {
-  hipLaunchKernelGGL(k_dot2, dim3(blocks_for(n)), dim3(kBlock), 0, s, a, b, c, n, p1, p2, sc);
+template <typename S>
+void dot2(hipStream_t s, const S* a, const S* b, const S* c, long long n, double* p1, double* p2, const KScalars* sc) {
+  hipLaunchKernelGGL(k_dot2<S>, dim3(blocks_for(n)), dim3(kBlock), 0, s, a, b, c, n, p1, p2, sc);
 }
-void bicg_x(hipStream_t s, double* x, const double* ph, const double* sh, long long n, const KScalars* sc) {
-  hipLaunchKernelGGL(k_bicg_x, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, ph, sh, n, sc);
+template <typename S>
+void bicg_x(hipStream_t s, double* x, const S* ph, const S* sh, long long n, const KScalars* sc, const double* scale) {
+  hipLaunchKernelGGL(k_bicg_x<S>, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, ph, sh, n, sc, scale);
+}
+template void bicg_p<double>(hipStream_t, const double*, double*, const double*, long long, const KScalars*);
+template void bicg_p<float>(hipStream_t, const float*, float*, const float*, long long, const KScalars*);
+template void bicg_s<double>(hipStream_t, const double*, const double*, double*, long long, double*, const KScalars*);
+template void bicg_s<float>(hipStream_t, const float*, const float*, float*, long long, double*, const KScalars*);
+template void bicg_xs<double>(hipStream_t, double*, const double*, long long, const KScalars*, const double*);
+template void bicg_xs<float>(hipStream_t, double*, const float*, long long, const KScalars*, const double*);
+template void dot2<double>(hipStream_t, const double*, const double*, const double*, long long, double*, double*, const KScalars*);
+template void dot2<float>(hipStream_t, const float*, const float*, const float*, long long, double*, double*, const KScalars*);
+template void bicg_x<double>(hipStream_t, double*, const double*, const double*, long long, const KScalars*, const double*);
+template void bicg_x<float>(hipStream_t, double*, const float*, const float*, long long, const KScalars*, const double*);
+void bicg_r(hipStream_t s, const float* sv, const float* t, float* r, long long n, double* partials, const KScalars* sc) {
+  hipLaunchKernelGGL(k_bicg_r, dim3(blocks_for(n)), dim3(kBlock), 0, s, sv, t, r, n, partials, sc);
 }
 void bi_sdone(hipStream_t s, KScalars* sc) { hipLaunchKernelGGL(k_bi_sdone, dim3(1), dim3(64), 0, s, sc); }
 void fin_k(hipStream_t s, KScalars* sc, const double* p1, const double* p2, int np, int op) {
   hipLaunchKernelGGL(k_fin_k, dim3(1), dim3(1024), 0, s, sc, p1, p2, np, op);
 }
 
-void jacobi_pc(hipStream_t s, const DevOp& op, const double* in, double* out, const KScalars* sc) {
+template <typename S>
+void jacobi_pc(hipStream_t s, const DevOp& op, const S* in, S* out, const KScalars* sc) {
   long long b = (op.n + kBlock - 1) / kBlock;
   if (b > 4096) b = 4096;
   const bool d3 = op.nz > 1;
-#define J_(D, K) hipLaunchKernelGGL((k_jacobi_pc<D, K>), dim3(b), dim3(kBlock), 0, s, op, in, out, sc)
+#define J_(D, K) hipLaunchKernelGGL((k_jacobi_pc<D, K, S>), dim3(b), dim3(kBlock), 0, s, op, in, out, sc)
   if (op.kind == PDEHIP_OP_POINTWISE) { if (d3) J_(3, PDEHIP_OP_POINTWISE); else J_(2, PDEHIP_OP_POINTWISE); }
   else if (op.kind == PDEHIP_OP_FLUX) { if (d3) J_(3, PDEHIP_OP_FLUX); else J_(2, PDEHIP_OP_FLUX); }
   else { if (d3) J_(3, PDEHIP_OP_CONST); else J_(2, PDEHIP_OP_CONST); }
 #undef J_
 }
+template void jacobi_pc<double>(hipStream_t, const DevOp&, const double*, double*, const KScalars*);
+template void jacobi_pc<float>(hipStream_t, const DevOp&, const float*, float*, const KScalars*);
 
-void multidot(hipStream_t s, const double* V, long long ld, int cnt, const double* w, long long n, double* partials, int nb, const KScalars* sc) {
-  hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, cnt, w, n, partials, sc);
+template <typename S>
+void multidot(hipStream_t s, const S* V, long long ld, int cnt, const S* w, long long n, double* partials, int nb, const KScalars* sc) {
+  hipLaunchKernelGGL(k_multidot<S>, dim3(nb), dim3(kBlock), 0, s, V, ld, cnt, w, n, partials, sc);
 }
-void multiaxpy(hipStream_t s, const double* V, long long ld, int cnt, const double* h, double* w, long long n, double sign, int check_cycle, const KScalars* sc) {
-  hipLaunchKernelGGL(k_multiaxpy, dim3(blocks_for(n)), dim3(kBlock), 0, s, V, ld, cnt, h, w, n, sign, check_cycle, sc);
+template <typename S>
+void multiaxpy(hipStream_t s, const S* V, long long ld, int cnt, const double* h, S* w, long long n, double sign, int check_cycle, const KScalars* sc) {
+  hipLaunchKernelGGL(k_multiaxpy<S>, dim3(blocks_for(n)), dim3(kBlock), 0, s, V, ld, cnt, h, w, n, sign, check_cycle, sc);
 }
+template void multidot<double>(hipStream_t, const double*, long long, int, const double*, long long, double*, int, const KScalars*);
+template void multidot<float>(hipStream_t, const float*, long long, int, const float*, long long, double*, int, const KScalars*);
+template void multiaxpy<double>(hipStream_t, const double*, long long, int, const double*, double*, long long, double, int, const KScalars*);
+template void multiaxpy<float>(hipStream_t, const float*, long long, int, const double*, float*, long long, double, int, const KScalars*);
 void fin_multi(hipStream_t s, const double* partials, int nb, int cnt, double* hcol, int add, const KScalars* sc) {
   hipLaunchKernelGGL(k_fin_multi, dim3(1), dim3(256), 0, s, partials, nb, cnt, hcol, add, sc);
 }
-void gm_normalize(hipStream_t s, double* vnext, const double* w, long long n, const double* hval, const KScalars* sc) {
-  hipLaunchKernelGGL(k_gm_normalize, dim3(blocks_for(n)), dim3(kBlock), 0, s, vnext, w, n, hval, sc);
+template <typename S>
+void gm_normalize(hipStream_t s, S* vnext, const S* w, long long n, const double* hval, const KScalars* sc) {
+  hipLaunchKernelGGL(k_gm_normalize<S>, dim3(blocks_for(n)), dim3(kBlock), 0, s, vnext, w, n, hval, sc);
 }
+template void gm_normalize<double>(hipStream_t, double*, const double*, long long, const double*, const KScalars*);
+template void gm_normalize<float>(hipStream_t, float*, const float*, long long, const double*, const KScalars*);
 void fin_norm(hipStream_t s, const double* partials, int nb, double* out, const KScalars* sc) {
   hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(256), 0, s, partials, nb, out, sc);
 }
@@ -357,9 +420,12 @@ void backsub(hipStream_t s, const double* H, const double* g, double* y, int m, 
 void gm_setcol(hipStream_t s, double* H, const double* hc, int j, int m, int cnt, const KScalars* sc) {
   hipLaunchKernelGGL(k_gm_setcol, dim3(1), dim3(64), 0, s, H, hc, j, m, cnt, sc);
 }
-void gm_start(hipStream_t s, double* v0, const double* r, long long n, double* g, int m, KScalars* sc) {
+template <typename S>
+void gm_start(hipStream_t s, S* v0, const double* r, long long n, double* g, int m, KScalars* sc) {
   hipLaunchKernelGGL(k_gm_reset, dim3(1), dim3(64), 0, s, sc);
-  hipLaunchKernelGGL(k_gm_start, dim3(blocks_for(n)), dim3(kBlock), 0, s, v0, r, n, g, m, sc);
+  hipLaunchKernelGGL(k_gm_start<S>, dim3(blocks_for(n)), dim3(kBlock), 0, s, v0, r, n, g, m, sc);
 }
+template void gm_start<double>(hipStream_t, double*, const double*, long long, double*, int, KScalars*);
+template void gm_start<float>(hipStream_t, float*, const double*, long long, double*, int, KScalars*);
 
 }  // namespace pdehip

@@ -9,6 +9,8 @@
 // threads of the block).  Each launch has >= 8 x #CU workgroups for the 257^3 level.
 #include <algorithm>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace pdehip {
@@ -28,6 +30,13 @@ template <> __device__ __forceinline__ auto ldT<float>(const float* p, long long
 template <> __device__ __forceinline__ auto ldT<__half>(const __half* p, long long i) { return __half2float(p[i]); }
 
 __device__ __forceinline__ bool skip(const KScalars* sc) { return sc && (sc->done | sc->breakdown); }
+
+// FP32/FP16 applies of operators whose rows sum to the reaction term use the difference form
+// (apply_diff): the sum form loses ~(8/h^2)/lambda of its FP32 digits on smooth vectors, which
+// makes the Krylov recursion drift from the true residual.  FP64 keeps the reference's sum.
+template <typename T, int KIND>
+constexpr bool kDiff = !std::is_same<T, double>::value &&
+                       (KIND == PDEHIP_OP_CONST || KIND == PDEHIP_OP_FLUX);
 
 // ----------------------------------------------------------------------------- apply / residual
 // MODE 0: y = A x (partials of x.y).  MODE 1: r = b - A x (partials of r.r).
@@ -57,9 +66,15 @@ __global__ void __launch_bounds__(kBlock) k_stencil(DevOp op, const void* __rest
         out = (MODE == 0) ? xc : (ldT(b, id) - xc);
       } else {
         W7<T> w = row_weights<2, KIND, T, K>(op, id);
-        fold_center<2, T>(op, i, j, 0, w);
-        const T ax = w.C * xc + w.E * ldT(x, id + 1) + w.W * ldT(x, id - 1) +
-                     w.N * ldT(x, id + op.nx) + w.S * ldT(x, id - op.nx);
+        T ax;
+        if constexpr (kDiff<T, KIND>) {
+          ax = apply_diff<2, T>(op, w, T(op.e), i, j, 0, xc, ldT(x, id - 1), ldT(x, id + 1),
+                                ldT(x, id - op.nx), ldT(x, id + op.nx), T(0), T(0));
+        } else {
+          fold_center<2, T>(op, i, j, 0, w);
+          ax = w.C * xc + w.E * ldT(x, id + 1) + w.W * ldT(x, id - 1) +
+               w.N * ldT(x, id + op.nx) + w.S * ldT(x, id - op.nx);
+        }
         out = (MODE == 0) ? ax : (ldT(b, id) - ax);
       }
       y[id] = store_as<S>(static_cast<double>(out));
@@ -84,9 +99,15 @@ __global__ void __launch_bounds__(kBlock) k_stencil(DevOp op, const void* __rest
           out = (MODE == 0) ? xc : (ldT(b, id) - xc);
         } else {
           W7<T> w = row_weights<3, KIND, T, K>(op, id);
-          fold_center<3, T>(op, i, j, k, w);
-          const T ax = w.C * xc + w.E * ldT(x, id + 1) + w.W * ldT(x, id - 1) +
-                       w.N * ldT(x, id + op.nx) + w.S * ldT(x, id - op.nx) + w.F * xp + w.B * xm;
+          T ax;
+          if constexpr (kDiff<T, KIND>) {
+            ax = apply_diff<3, T>(op, w, T(op.e), i, j, k, xc, ldT(x, id - 1), ldT(x, id + 1),
+                                  ldT(x, id - op.nx), ldT(x, id + op.nx), xm, xp);
+          } else {
+            fold_center<3, T>(op, i, j, k, w);
+            ax = w.C * xc + w.E * ldT(x, id + 1) + w.W * ldT(x, id - 1) +
+                 w.N * ldT(x, id + op.nx) + w.S * ldT(x, id - op.nx) + w.F * xp + w.B * xm;
+          }
           out = (MODE == 0) ? ax : (ldT(b, id) - ax);
         }
         y[id] = store_as<S>(static_cast<double>(out));

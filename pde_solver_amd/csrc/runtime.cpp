@@ -256,7 +256,7 @@ static void build_level_ops(Plan& P, Level& lv, const double* kappa64_dev, bool 
     lv.op64.e = p.e;
   }
   lv.op32 = lv.op64;
-  if (P.mixed) {
+  if (P.mixed || P.k32) {
     const double s = lv.sigma;
     for (double& wv : lv.op32.w) wv *= s;
     lv.op32.fx *= s; lv.op32.fy *= s; lv.op32.fz *= s; lv.op32.e *= s;
@@ -413,6 +413,7 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
   if (krylov && !P->folded && !all_dirichlet(p, g))
     throw Error(PDEHIP_ERR_UNSUPPORTED, "Krylov methods require Dirichlet boundaries on all sides");
   P->mixed = (m == PDEHIP_MGPCG) && o.precision == PDEHIP_MIXED;
+  P->k32 = (m == PDEHIP_BICGSTAB || m == PDEHIP_GMRES) && o.precision == PDEHIP_MIXED;
   if (mg && p.op_kind == PDEHIP_OP_POINTWISE)
     throw Error(PDEHIP_ERR_UNSUPPORTED, "multigrid needs constant or FLUX coefficients");
   if (m == PDEHIP_MG && o.mg_single_cycle &&
@@ -525,13 +526,23 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
     z8(P->r); z8(P->p); z8(P->q);
     if (m == PDEHIP_MGPCG && !P->mixed) z8(P->z);
     if (P->mixed) { z4(P->r32); z4(P->x32); z4(P->p32); z4(P->q32); z4(P->z32); }
+  } else if (m == PDEHIP_BICGSTAB && P->k32) {
+    z8(P->r);
+    const int nv = o.precond == PDEHIP_PC_JACOBI ? 8 : 6;
+    for (int t = 0; t < nv; ++t) z4(P->v32[t]);
   } else if (m == PDEHIP_BICGSTAB) {
     z8(P->r); z8(P->p); z8(P->q); z8(P->z); z8(P->w1); z8(P->w2); z8(P->w3);
     if (o.precond == PDEHIP_PC_JACOBI) z8(P->w4);
   } else if (m == PDEHIP_GMRES) {
     const int mm = std::max(1, std::min(o.gmres_restart, std::max(1, o.max_iter)));
-    z8(P->r); z8(P->w1); z8(P->w2); z8(P->w3);
-    P->V.alloc(ctx, (size_t)(mm + 1) * n8);
+    if (P->k32) {
+      z8(P->r);
+      z4(P->v32[0]); z4(P->v32[1]);
+      P->V32.alloc(ctx, (size_t)(mm + 1) * n4);
+    } else {
+      z8(P->r); z8(P->w1); z8(P->w2); z8(P->w3);
+      P->V.alloc(ctx, (size_t)(mm + 1) * n8);
+    }
     P->H.alloc(ctx, (size_t)(mm + 1) * mm * sizeof(double));
     P->cs.alloc(ctx, mm * sizeof(double));
     P->sn.alloc(ctx, mm * sizeof(double));
@@ -1124,6 +1135,187 @@ void plan_residual_norms(Plan& P, const double* u_dev, double* l2, double* linf)
   *linf = m;
 }
 
+// Mixed-precision BiCGStab: FP32 BiCGStab (the recurrence r = s - omega t, CUDA semantics)
+// whose x updates go straight into the FP64 solution, with FP64 residual replacement: every
+// check (check_every iterations) where the FP32 residual has fallen by inner_tol (default
+// 1e-2) since the last replacement, or claims convergence, r is replaced by (b - A x)/|r0|
+// computed in FP64, and only that true residual stops the solve.  The Krylov state (p, r_hat,
+// rho) carries on across replacements.  FP32 applies use the difference form.
+static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
+  const hipStream_t s = P.stream();
+  const long long n = P.g.n;
+  const DevOp& op64 = P.op();
+  const DevOp& op32 = P.L[0].op32;
+  init_scalars(P, 0, 1e-30);
+  init_scalars(P, 1, 1e-30);
+  KScalars* so = P.sc(0);
+  KScalars* si = P.sc(1);
+  const Launch L0{s, nullptr};
+  const Launch Li{s, si};
+  double* part = P.part.as<double>();
+  double* part2 = P.part2.as<double>();
+  double* x = P.x.as<double>();
+  double* r64 = P.r.as<double>();
+  const bool pc = P.opt.precond == PDEHIP_PC_JACOBI;
+  float* r = P.v32[0].as<float>();
+  float* rhat = P.v32[1].as<float>();
+  float* pv = P.v32[2].as<float>();
+  float* v = P.v32[3].as<float>();
+  float* sv = P.v32[4].as<float>();
+  float* t = P.v32[5].as<float>();
+  float* ph = pc ? P.v32[6].as<float>() : pv;
+  float* sh = pc ? P.v32[7].as<float>() : sv;
+  HIPCHK(hipMemcpyAsync(x, P.x0.get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  residual(L0, op64, P64, P.bvec.get(), x, r64, part);
+  finalize(L0, so, part, partial_blocks(op64), FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
+  const KScalars k0 = pull(P, 0);
+  const double delta = P.opt.inner_tol > 0 ? P.opt.inner_tol : 1e-2;
+  const int kb = krylov_blocks(n);
+  const int max_iter = std::max(1, P.opt.max_iter);
+  const int every = check_every(P);
+  int replacements = 0, iter_base = 0;
+  if (!k0.done) {
+    // inner vectors hold the residual divided by the fixed scale |r0|
+    to_f32_scaled(s, r64, r, n, &so->r0);
+    HIPCHK(hipMemcpyAsync(rhat, r, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    dot(L0, P32, r, r, n, part);
+    finalize(L0, si, part, vec_blocks(n), FIN_R0, k0.target / k0.r0, PDEHIP_TOL_ABS, nullptr);
+    double m_last = 1.0;
+    int launched = 0;
+    while (launched < max_iter) {
+      const int chunk = std::min(every, max_iter - launched);
+      for (int c = 0; c < chunk; ++c) {
+        dot(Li, P32, rhat, r, n, part);
+        fin_k(s, si, part, nullptr, vec_blocks(n), KFIN_BI_RHO);
+        bicg_p<float>(s, r, pv, v, n, si);
+        if (pc) jacobi_pc<float>(s, op64, pv, ph, si);
+        apply(Li, op32, P32, ph, v, nullptr);
+        dot(Li, P32, rhat, v, n, part);
+        fin_k(s, si, part, nullptr, vec_blocks(n), KFIN_BI_ALPHA);
+        bicg_s<float>(s, r, v, sv, n, part, si);
+        fin_k(s, si, part, nullptr, kb, KFIN_BI_S);
+        bicg_xs<float>(s, x, ph, n, si, &so->r0);
+        bi_sdone(s, si);
+        if (pc) jacobi_pc<float>(s, op64, sv, sh, si);
+        apply(Li, op32, P32, sh, t, nullptr);
+        dot2<float>(s, t, t, sv, n, part, part2, si);
+        fin_k(s, si, part, part2, kb, KFIN_BI_OMEGA);
+        bicg_x<float>(s, x, ph, sh, n, si, &so->r0);
+        bicg_r(s, sv, t, r, n, part, si);
+        fin_k(s, si, part, nullptr, kb, KFIN_BI_R);
+      }
+      launched += chunk;
+      if (cancelled(P)) break;
+      KScalars ki = pull(P, 1);
+      if (!(ki.done || ki.breakdown || ki.metric < delta * m_last)) continue;
+      // replacement: the FP64 true residual decides
+      residual(L0, op64, P64, P.bvec.get(), x, r64, part);
+      finalize(L0, so, part, partial_blocks(op64), FIN_CG_RR, 0, 0, nullptr);
+      ++replacements;
+      const KScalars ko = pull(P, 0);
+      if (ko.done || ko.breakdown) break;
+      to_f32_scaled(s, r64, r, n, &so->r0);
+      if (ki.breakdown) {
+        // rho or omega vanished: restart the recurrence from the true residual
+        HIPCHK(hipMemcpyAsync(rhat, r, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        iter_base += ki.iter;
+        ki.iter = 0;
+        ki.breakdown = 0;
+      }
+      ki.done = 0;
+      ki.flag_s = 0;
+      ki.metric = ko.metric / ko.r0;
+      m_last = ki.metric;
+      P.hsc[1] = ki;
+      HIPCHK(hipMemcpyAsync(si, &P.hsc[1], sizeof(KScalars), hipMemcpyHostToDevice, s));
+    }
+  }
+  const KScalars ko = pull(P, 0);
+  const KScalars ki = pull(P, 1);
+  rep->iterations = iter_base + ki.iter;
+  rep->refinements = replacements;
+  rep->inner_iterations = iter_base + ki.iter;
+  rep->final_metric = ko.metric;
+  rep->converged = ko.done;
+  if (cancelled(P) && !ko.done) throw Error(PDEHIP_ERR_CANCELLED, "solve cancelled");
+  if (!ko.done) rep->status = PDEHIP_ERR_NOT_CONVERGED;
+}
+
+// Mixed-precision restarted GMRES: the Arnoldi basis, the operator applies and the
+// preconditioner run in FP32 (dot products and the Hessenberg/Givens algebra in FP64); each
+// restart cycle adds its correction to the FP64 solution and recomputes the FP64 residual, so
+// the restarts are the refinement steps.
+static void solve_gmres_mixed(Plan& P, pdehip_report* rep) {
+  const hipStream_t s = P.stream();
+  const long long n = P.g.n;
+  const DevOp& op64 = P.op();
+  const DevOp& op32 = P.L[0].op32;
+  const int max_iter = std::max(1, P.opt.max_iter);
+  const int mm = std::max(1, std::min(P.opt.gmres_restart, max_iter));
+  init_scalars(P, 0, 1e-30);
+  KScalars* sc = P.sc(0);
+  const Launch L{s, sc};
+  const Launch L0{s, nullptr};
+  double* part = P.part.as<double>();
+  double* x = P.x.as<double>();
+  double* r = P.r.as<double>();
+  float* w = P.v32[0].as<float>();
+  float* zt = P.v32[1].as<float>();
+  float* V = P.V32.as<float>();
+  double *H = P.H.as<double>(), *cs = P.cs.as<double>(), *sn = P.sn.as<double>();
+  double *gv = P.gv.as<double>(), *yv = P.yv.as<double>();
+  const bool pc = P.opt.precond == PDEHIP_PC_JACOBI;
+  HIPCHK(hipMemcpyAsync(x, P.x0.get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  residual(L0, op64, P64, P.bvec.get(), x, r, part);
+  finalize(L0, sc, part, partial_blocks(op64), FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
+  const int kb = krylov_blocks(n);
+  int iter_total = 0, cycles = 0;
+  while (iter_total < max_iter) {
+    const KScalars& k0 = pull(P, 0);
+    if (k0.done || k0.breakdown || cancelled(P)) break;
+    const int m = std::min(mm, max_iter - iter_total);
+    ++cycles;
+    gm_start<float>(s, V, r, n, gv, m, sc);
+    for (int j = 0; j < m; ++j) {
+      const float* vj = V + (long long)j * n;
+      if (pc) { jacobi_pc<float>(s, op64, vj, zt, sc); apply(L, op32, P32, zt, w, nullptr); }
+      else apply(L, op32, P32, vj, w, nullptr);
+      double* hcol = P.hcb.as<double>();
+      for (int pass = 0; pass < 2; ++pass) {
+        multidot<float>(s, V, n, j + 1, w, n, part, kb, sc);
+        fin_multi(s, part, kb, j + 1, hcol + (pass ? 64 : 0), 0, sc);
+        multiaxpy<float>(s, V, n, j + 1, hcol + (pass ? 64 : 0), w, n, -1.0, 1, sc);
+      }
+      gm_setcol(s, H, hcol, j, m, j + 1, sc);
+      dot(L, P32, w, w, n, part);
+      fin_norm(s, part, vec_blocks(n), H + (long long)(j + 1) * m + j, sc);
+      gm_normalize<float>(s, V + (long long)(j + 1) * n, w, n, H + (long long)(j + 1) * m + j, sc);
+      givens(s, H, cs, sn, gv, j, m, sc);
+    }
+    backsub(s, H, gv, yv, m, sc);
+    fill(s, P32, zt, n, 0.0);
+    const KScalars& kc = pull(P, 0);
+    const int steps = kc.steps;
+    if (kc.breakdown) break;
+    multiaxpy<float>(s, V, n, steps, yv, zt, n, 1.0, 0, sc);
+    if (pc) { jacobi_pc<float>(s, op64, zt, w, sc); add_f32(s, x, w, n, nullptr, 1.0); }
+    else add_f32(s, x, zt, n, nullptr, 1.0);
+    residual(L, op64, P64, P.bvec.get(), x, r, part);
+    fin_k(s, sc, part, nullptr, partial_blocks(op64), KFIN_GM_R);
+    iter_total += steps;
+    HIPCHK(hipMemsetAsync(H, 0, P.H.bytes(), s));
+  }
+  const KScalars& k = pull(P, 0);
+  rep->iterations = iter_total;
+  rep->refinements = cycles;
+  rep->inner_iterations = iter_total;
+  rep->final_metric = k.metric;
+  rep->converged = k.done;
+  if (cancelled(P) && !k.done) throw Error(PDEHIP_ERR_CANCELLED, "solve cancelled");
+  if (k.breakdown && !k.done) rep->status = PDEHIP_ERR_BREAKDOWN;
+  else if (!k.done) rep->status = PDEHIP_ERR_NOT_CONVERGED;
+}
+
 void plan_solve(Plan& P, pdehip_report* rep) {
   std::memset(rep, 0, sizeof(*rep));
   const hipStream_t s = P.stream();
@@ -1135,8 +1327,14 @@ void plan_solve(Plan& P, pdehip_report* rep) {
     case PDEHIP_GAUSS_SEIDEL:
     case PDEHIP_SOR: solve_relax(P, rep); break;
     case PDEHIP_CG: solve_cg(P, false, rep); break;
-    case PDEHIP_BICGSTAB: solve_bicgstab(P, rep); break;
-    case PDEHIP_GMRES: solve_gmres(P, rep); break;
+    case PDEHIP_BICGSTAB:
+      if (P.k32) solve_bicgstab_mixed(P, rep);
+      else solve_bicgstab(P, rep);
+      break;
+    case PDEHIP_GMRES:
+      if (P.k32) solve_gmres_mixed(P, rep);
+      else solve_gmres(P, rep);
+      break;
     case PDEHIP_MG:
       if (P.opt.mg_single_cycle) solve_mg_reference(P, rep);
       else solve_mg_iter(P, rep);

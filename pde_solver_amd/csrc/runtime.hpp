@@ -145,6 +145,8 @@ struct Plan {
   // Krylov work vectors
   DBuf r, p, q, z, w1, w2, w3, w4;
   DBuf r32, x32, p32, q32, z32;
+  bool k32 = false;                // mixed BiCGStab/GMRES: FP32 Krylov vectors, FP64 x and residual
+  DBuf v32[8], V32;                // their FP32 vectors (GMRES: basis in V32)
   DBuf V, H, cs, sn, gv, yv, hcb;   // GMRES (hcb: two staged Hessenberg columns)
   DBuf part, part2, part3;     // reduction partials
   int npart = 0;
