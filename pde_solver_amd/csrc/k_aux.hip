@@ -51,7 +51,8 @@ __host__ __device__ inline long long face_size(const DevOp& op, int axis) {
 // ----------------------------------------------------------------------------- closures
 // Neumann/Robin closure of one axis, low face then high face per thread (reference
 // cpu_utils.cpp:630-720 / 774-901 for AXIS; OUTWARD: alpha*u + beta*du/dn = gamma).
-__global__ void k_closure_nr(DevOp op, FaceData fd, int axis, double* u, const KScalars* sc) {
+template <typename S>
+__global__ void k_closure_nr(DevOp op, FaceData fd, int axis, S* u, const KScalars* sc) {
   if (skip(sc)) return;
   const long long nq = face_size(op, axis);
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nq;
@@ -63,12 +64,13 @@ __global__ void k_closure_nr(DevOp op, FaceData fd, int axis, double* u, const K
       long long node, inner;
       face_node(op, axis, side, q, &node, &inner);
       double v;
-      if (closure_nr_value(fd, f, side, q, u[inner], &v)) u[node] = v;
+      if (closure_nr_value(fd, f, side, q, static_cast<double>(u[inner]), &v)) u[node] = static_cast<S>(v);
     }
   }
 }
 
-__global__ void k_closure_d(DevOp op, FaceData fd, int axis, double* u, const KScalars* sc) {
+template <typename S>
+__global__ void k_closure_d(DevOp op, FaceData fd, int axis, S* u, const KScalars* sc) {
   if (skip(sc)) return;
   const long long nq = face_size(op, axis);
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nq;
@@ -78,7 +80,7 @@ __global__ void k_closure_d(DevOp op, FaceData fd, int axis, double* u, const KS
       if (fd.kind[f] != PDEHIP_BC_DIRICHLET) continue;
       long long node, inner;
       face_node(op, axis, side, q, &node, &inner);
-      u[node] = fd.value[f][q];
+      u[node] = static_cast<S>(fd.value[f][q]);
     }
   }
 }
@@ -464,23 +466,29 @@ __global__ void __launch_bounds__(1024) k_finalize(KScalars* sc, const double* p
 }  // namespace
 
 // ============================================================================== launchers
-void closures_nr(const Launch& L, const DevOp& op, const FaceData& fd, double* u) {
+template <typename S>
+void closures_nr(const Launch& L, const DevOp& op, const FaceData& fd, S* u) {
   const int axes = op.nz > 1 ? 3 : 2;
   for (int a = 0; a < axes; ++a) {
     if (fd.kind[2 * a] == PDEHIP_BC_DIRICHLET && fd.kind[2 * a + 1] == PDEHIP_BC_DIRICHLET) continue;
-    hipLaunchKernelGGL(k_closure_nr, dim3(grid1(face_size(op, a), 1024)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_closure_nr<S>, dim3(grid1(face_size(op, a), 1024)), dim3(kBlock), 0,
                        L.stream, op, fd, a, u, L.sc);
   }
 }
 
-void closures_dirichlet(const Launch& L, const DevOp& op, const FaceData& fd, double* u) {
+template <typename S>
+void closures_dirichlet(const Launch& L, const DevOp& op, const FaceData& fd, S* u) {
   const int axes = op.nz > 1 ? 3 : 2;
   for (int a = 0; a < axes; ++a) {
     if (fd.kind[2 * a] != PDEHIP_BC_DIRICHLET && fd.kind[2 * a + 1] != PDEHIP_BC_DIRICHLET) continue;
-    hipLaunchKernelGGL(k_closure_d, dim3(grid1(face_size(op, a), 1024)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_closure_d<S>, dim3(grid1(face_size(op, a), 1024)), dim3(kBlock), 0,
                        L.stream, op, fd, a, u, L.sc);
   }
 }
+template void closures_nr<double>(const Launch&, const DevOp&, const FaceData&, double*);
+template void closures_nr<float>(const Launch&, const DevOp&, const FaceData&, float*);
+template void closures_dirichlet<double>(const Launch&, const DevOp&, const FaceData&, double*);
+template void closures_dirichlet<float>(const Launch&, const DevOp&, const FaceData&, float*);
 
 #define PDEHIP_BY_PREC(p, KER, ...)                                                   \
   switch (p) {                                                                        \

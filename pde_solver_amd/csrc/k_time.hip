@@ -181,7 +181,8 @@ __global__ void k_add_f32(double* __restrict__ x, const float* __restrict__ e, l
 
 // out32 = (float)(in64 * s), s = 1 / *norm
 __global__ void k_to_f32_scaled(const double* __restrict__ in, float* __restrict__ out, long long n,
-                                const double* norm) {
+                                const double* norm, const KScalars* sc) {
+  if (sc && (sc->done | sc->breakdown)) return;
   const double s = (*norm > 0.0) ? 1.0 / *norm : 1.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
@@ -230,7 +231,8 @@ void add_f32(hipStream_t s, double* x, const float* e, long long n, const double
 }
 
 void to_f32_scaled(hipStream_t s, const double* in, float* out, long long n, const double* norm) {
-  hipLaunchKernelGGL(k_to_f32_scaled, dim3(misc_blocks(n)), dim3(kBlock), 0, s, in, out, n, norm);
+  hipLaunchKernelGGL(k_to_f32_scaled, dim3(misc_blocks(n)), dim3(kBlock), 0, s, in, out, n, norm,
+                     static_cast<const KScalars*>(nullptr));
 }
 
 int maxabs_blocks(long long n) { return static_cast<int>(misc_blocks(n) > 1024 ? 1024 : misc_blocks(n)); }

@@ -91,8 +91,9 @@ struct FaceData {
   int convention;
 };
 // Reference closure order: axis by axis (x faces, y faces, z faces).
-void closures_nr(const Launch& L, const DevOp& op, const FaceData& fd, double* u);
-void closures_dirichlet(const Launch& L, const DevOp& op, const FaceData& fd, double* u);
+// (double: solutions; float: the FP32 state of a time series)
+template <typename S> void closures_nr(const Launch& L, const DevOp& op, const FaceData& fd, S* u);
+template <typename S> void closures_dirichlet(const Launch& L, const DevOp& op, const FaceData& fd, S* u);
 // ---- multigrid transfers
 // Coarse right-hand side: full weighting of the fine residual on every coarse node, then the
 // transpose of the homogeneous closure extension (coarse.theta) folded in, times scale

@@ -1136,11 +1136,13 @@ void plan_residual_norms(Plan& P, const double* u_dev, double* l2, double* linf)
 }
 
 // Mixed-precision BiCGStab: FP32 BiCGStab (the recurrence r = s - omega t, CUDA semantics)
-// whose x updates go straight into the FP64 solution, with FP64 residual replacement: every
-// check (check_every iterations) where the FP32 residual has fallen by inner_tol (default
-// 1e-2) since the last replacement, or claims convergence, r is replaced by (b - A x)/|r0|
-// computed in FP64, and only that true residual stops the solve.  The Krylov state (p, r_hat,
-// rho) carries on across replacements.  FP32 applies use the difference form.
+// whose x updates go straight into the FP64 solution, with FP64 residual replacement: at every
+// check (check_every iterations: 1, 4 or 16 by size) r is replaced by (b - A x)/|r0| computed
+// in FP64, and only that true residual stops the solve.  The Krylov state (p, r_hat, rho)
+// carries on across replacements; a breakdown restarts it from the true residual.  FP32
+// applies use the difference form.  On diffusion-dominated problems of 513^2 and more the FP32
+// iteration can diverge where FP64 BiCGStab converges; the solve then reports non-convergence
+// (restarted GMRES in mixed precision converges there with the FP64 iteration count).
 static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
   const hipStream_t s = P.stream();
   const long long n = P.g.n;
@@ -1169,7 +1171,6 @@ static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
   residual(L0, op64, P64, P.bvec.get(), x, r64, part);
   finalize(L0, so, part, partial_blocks(op64), FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
   const KScalars k0 = pull(P, 0);
-  const double delta = P.opt.inner_tol > 0 ? P.opt.inner_tol : 1e-2;
   const int kb = krylov_blocks(n);
   const int max_iter = std::max(1, P.opt.max_iter);
   const int every = check_every(P);
@@ -1180,7 +1181,6 @@ static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
     HIPCHK(hipMemcpyAsync(rhat, r, n * sizeof(float), hipMemcpyDeviceToDevice, s));
     dot(L0, P32, r, r, n, part);
     finalize(L0, si, part, vec_blocks(n), FIN_R0, k0.target / k0.r0, PDEHIP_TOL_ABS, nullptr);
-    double m_last = 1.0;
     int launched = 0;
     while (launched < max_iter) {
       const int chunk = std::min(every, max_iter - launched);
@@ -1207,13 +1207,12 @@ static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
       launched += chunk;
       if (cancelled(P)) break;
       KScalars ki = pull(P, 1);
-      if (!(ki.done || ki.breakdown || ki.metric < delta * m_last)) continue;
       // replacement: the FP64 true residual decides
       residual(L0, op64, P64, P.bvec.get(), x, r64, part);
       finalize(L0, so, part, partial_blocks(op64), FIN_CG_RR, 0, 0, nullptr);
       ++replacements;
       const KScalars ko = pull(P, 0);
-      if (ko.done || ko.breakdown) break;
+      if (ko.done || ko.breakdown || !(ko.metric < 1e3 * ko.r0)) break;   // converged / diverged
       to_f32_scaled(s, r64, r, n, &so->r0);
       if (ki.breakdown) {
         // rho or omega vanished: restart the recurrence from the true residual
@@ -1225,7 +1224,6 @@ static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
       ki.done = 0;
       ki.flag_s = 0;
       ki.metric = ko.metric / ko.r0;
-      m_last = ki.metric;
       P.hsc[1] = ki;
       HIPCHK(hipMemcpyAsync(si, &P.hsc[1], sizeof(KScalars), hipMemcpyHostToDevice, s));
     }
@@ -1426,49 +1424,71 @@ void op_apply(Ctx* ctx, const pdehip_problem& p, const double* x, double* y) {
 
 // ------------------------------------------------------------------ time series
 // SolvePDETimeSeries (solver.cpp:989-1329): 2D explicit steps, closures after each step.
-void time_series(Ctx* ctx, const pdehip_problem& p, const pdehip_time_config& tc,
-                 pdehip_frame_fn cb, void* user, double* u_out, pdehip_report* rep) {
-  if (p.nz > 1) throw Error(PDEHIP_ERR_UNSUPPORTED, "3D Time Series Solver not yet refactored.");
-  if (tc.dt <= 0.0) throw Error(PDEHIP_ERR_INVALID, "time step dt must be positive");
+namespace {
+
+void stage(hipStream_t s, const DevOp& op, const double* v, const double* un, double* acc,
+           double* out, const double* f, double fc, const NonlinearTerms& nl, double ut, double dt,
+           double c1, int mode) {
+  time_stage(s, op, v, un, acc, out, f, fc, nl, ut, dt, c1, mode);
+}
+void stage(hipStream_t s, const DevOp& op, const float* v, const float* un, float* acc,
+           float* out, const double* f, double fc, const NonlinearTerms& nl, double ut, double dt,
+           double c1, int mode) {
+  time_stage32(s, op, v, un, acc, out, f, fc, nl, ut, dt, c1, mode);
+}
+
+// The integration loop with state type S (double, or float for the FP32 state); stage
+// arithmetic is FP64 in both.  Frames and the result are widened to FP64 on the device.
+template <typename S>
+void run_series(Ctx* ctx, Plan& PL, const pdehip_problem& p, const pdehip_time_config& tc,
+                pdehip_frame_fn cb, void* user, double* u_out, pdehip_report* rep) {
   const bool has_ut = std::abs(p.ut) > 1e-12, has_utt = std::abs(p.utt) > 1e-12;
-  if (!has_ut && !has_utt) throw Error(PDEHIP_ERR_INVALID, "time-dependent solve requires u_t or u_tt term");
-  if (has_utt && p.initial && !p.initial_velocity)
-    throw Error(PDEHIP_ERR_INVALID, "checkpoint restart for u_tt requires velocity data");
-  pdehip_options o{};
-  o.method = PDEHIP_JACOBI;
-  o.max_iter = 1;
-  o.sor_omega = 1.5;
-  auto P = plan_create(ctx, p, o);
-  const Geo& g = P->g;
+  constexpr Prec PS = sizeof(S) == 8 ? P64 : P32;
+  const Geo& g = PL.g;
   const hipStream_t s = ctx->stream;
   const long long n = g.n;
-  const DevOp& op = P->op();
-  const FaceData& fd = P->faces.fd;
+  const DevOp& op = PL.op();
+  const FaceData& fd = PL.faces.fd;
   const Launch L0{s, nullptr};
-  DBuf u(ctx, n * 8), un(ctx, n * 8), acc(ctx, n * 8), ta(ctx, n * 8), tb(ctx, n * 8), vel;
-  if (p.initial) HIPCHK(hipMemcpyAsync(u.get(), p.initial, n * 8, hipMemcpyHostToDevice, s));
-  else fill(s, P64, u.get(), n, 0.0);
+  const size_t es = sizeof(S);
+  DBuf u(ctx, n * es), un(ctx, n * es), acc(ctx, n * es), ta(ctx, n * es), tb(ctx, n * es), vel;
+  DBuf stage64;
+  if (p.initial) {
+    if constexpr (PS == P64) {
+      HIPCHK(hipMemcpyAsync(u.get(), p.initial, n * 8, hipMemcpyHostToDevice, s));
+    } else {
+      stage64.alloc(ctx, n * 8);
+      HIPCHK(hipMemcpyAsync(stage64.get(), p.initial, n * 8, hipMemcpyHostToDevice, s));
+      convert(s, P64, stage64.get(), P32, u.get(), n, 1.0);
+    }
+  } else {
+    fill(s, PS, u.get(), n, 0.0);
+  }
   if (has_utt) {
     vel.alloc(ctx, n * 8);
     if (p.initial_velocity) HIPCHK(hipMemcpyAsync(vel.get(), p.initial_velocity, n * 8, hipMemcpyHostToDevice, s));
     else fill(s, P64, vel.get(), n, 0.0);
   }
-  closures_dirichlet(L0, op, fd, u.as<double>());
-  closures_nr(L0, op, fd, u.as<double>());
-  closures_dirichlet(L0, op, fd, u.as<double>());
+  closures_dirichlet(L0, op, fd, u.as<S>());
+  closures_nr(L0, op, fd, u.as<S>());
+  closures_dirichlet(L0, op, fd, u.as<S>());
   const int frames = std::max(1, tc.frames);
   const bool integ = !has_utt && tc.integrator != PDEHIP_TI_FORWARD_EULER;
   const double dt = tc.dt;
-  const double* f = P->f.as<double>();
-  const double fc = P->f_const, ut = p.ut;
-  const NonlinearTerms& nl = P->nl;
+  const double* f = PL.f.as<double>();
+  const double fc = PL.f_const, ut = p.ut;
+  const NonlinearTerms& nl = PL.nl;
   // frame buffering: up to buffer_frames frames stay on the device before one batched copy
   int buffer = tc.buffer_frames > 0 ? tc.buffer_frames : 1;
   buffer = std::max(1, std::min(buffer, frames));
-  DBuf fbuf(ctx, (size_t)buffer * n * 8);
-  std::vector<double> host((size_t)buffer * n), vhost;
+  DBuf fbuf(ctx, (size_t)(cb ? buffer : 1) * n * 8);
+  std::vector<double> host(cb ? (size_t)buffer * n : 0), vhost;
   int in_buf = 0, buf_first = 0;
   bool stop = false;
+  auto widen = [&](const S* src, double* dst) {
+    if constexpr (PS == P64) HIPCHK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, s));
+    else convert(s, P32, src, P64, dst, n, 1.0);
+  };
   auto flush = [&]() {
     if (in_buf == 0) return;
     HIPCHK(hipMemcpyAsync(host.data(), fbuf.get(), (size_t)in_buf * n * 8, hipMemcpyDeviceToHost, s));
@@ -1487,61 +1507,93 @@ void time_series(Ctx* ctx, const pdehip_problem& p, const pdehip_time_config& tc
   for (int frame = 0; frame < frames && !stop; ++frame) {
     if (cb) {
       if (in_buf == 0) buf_first = frame;
-      HIPCHK(hipMemcpyAsync(fbuf.as<double>() + (size_t)in_buf * n, u.get(), n * 8, hipMemcpyDeviceToDevice, s));
+      widen(u.as<S>(), fbuf.as<double>() + (size_t)in_buf * n);
       ++in_buf;
       // velocity is reported at the flush only; flush every frame for u_tt to keep it exact
       if (in_buf == buffer || has_utt) flush();
     }
     if (frame + 1 >= frames) break;
-    double* U = u.as<double>();
+    S* U = u.as<S>();
+    S* A = acc.as<S>();
+    S* Ta = ta.as<S>();
+    S* Tb = tb.as<S>();
     if (integ) {
       switch (tc.integrator) {
         case PDEHIP_TI_RK2:
-          time_stage(s, op, U, U, acc.as<double>(), ta.as<double>(), f, fc, nl, ut, dt, dt, 0);
-          time_stage(s, op, ta.as<double>(), U, acc.as<double>(), U, f, fc, nl, ut, dt, 0.5 * dt, 3);
+          stage(s, op, U, U, A, Ta, f, fc, nl, ut, dt, dt, 0);
+          stage(s, op, Ta, U, A, U, f, fc, nl, ut, dt, 0.5 * dt, 3);
           break;
         case PDEHIP_TI_RK4:
-          time_stage(s, op, U, U, acc.as<double>(), ta.as<double>(), f, fc, nl, ut, dt, 0.5 * dt, 0);
-          time_stage(s, op, ta.as<double>(), U, acc.as<double>(), tb.as<double>(), f, fc, nl, ut, dt, 0.5 * dt, 1);
-          time_stage(s, op, tb.as<double>(), U, acc.as<double>(), ta.as<double>(), f, fc, nl, ut, dt, dt, 1);
-          time_stage(s, op, ta.as<double>(), U, acc.as<double>(), U, f, fc, nl, ut, dt, dt / 6.0, 2);
+          stage(s, op, U, U, A, Ta, f, fc, nl, ut, dt, 0.5 * dt, 0);
+          stage(s, op, Ta, U, A, Tb, f, fc, nl, ut, dt, 0.5 * dt, 1);
+          stage(s, op, Tb, U, A, Ta, f, fc, nl, ut, dt, dt, 1);
+          stage(s, op, Ta, U, A, U, f, fc, nl, ut, dt, dt / 6.0, 2);
           break;
         case PDEHIP_TI_SSPRK2:
-          HIPCHK(hipMemcpyAsync(un.get(), U, n * 8, hipMemcpyDeviceToDevice, s));
-          time_stage(s, op, U, U, nullptr, ta.as<double>(), f, fc, nl, ut, dt, dt, 0);
-          time_stage(s, op, ta.as<double>(), un.as<double>(), nullptr, U, f, fc, nl, ut, dt, 0.0, 4);
+          HIPCHK(hipMemcpyAsync(un.get(), U, n * es, hipMemcpyDeviceToDevice, s));
+          stage(s, op, U, U, static_cast<S*>(nullptr), Ta, f, fc, nl, ut, dt, dt, 0);
+          stage(s, op, Ta, un.as<S>(), static_cast<S*>(nullptr), U, f, fc, nl, ut, dt, 0.0, 4);
           break;
         case PDEHIP_TI_SSPRK3:
-          HIPCHK(hipMemcpyAsync(un.get(), U, n * 8, hipMemcpyDeviceToDevice, s));
-          time_stage(s, op, U, U, nullptr, ta.as<double>(), f, fc, nl, ut, dt, dt, 0);
-          time_stage(s, op, ta.as<double>(), un.as<double>(), nullptr, tb.as<double>(), f, fc, nl, ut, dt, 0.0, 5);
-          time_stage(s, op, tb.as<double>(), un.as<double>(), nullptr, U, f, fc, nl, ut, dt, 0.0, 6);
+          HIPCHK(hipMemcpyAsync(un.get(), U, n * es, hipMemcpyDeviceToDevice, s));
+          stage(s, op, U, U, static_cast<S*>(nullptr), Ta, f, fc, nl, ut, dt, dt, 0);
+          stage(s, op, Ta, un.as<S>(), static_cast<S*>(nullptr), Tb, f, fc, nl, ut, dt, 0.0, 5);
+          stage(s, op, Tb, un.as<S>(), static_cast<S*>(nullptr), U, f, fc, nl, ut, dt, 0.0, 6);
           break;
         default:
-          time_stage(s, op, U, U, nullptr, ta.as<double>(), f, fc, nl, ut, dt, dt, 0);
-          HIPCHK(hipMemcpyAsync(U, ta.get(), n * 8, hipMemcpyDeviceToDevice, s));
+          stage(s, op, U, U, static_cast<S*>(nullptr), Ta, f, fc, nl, ut, dt, dt, 0);
+          HIPCHK(hipMemcpyAsync(U, Ta, n * es, hipMemcpyDeviceToDevice, s));
       }
     } else if (has_utt) {
-      time_utt(s, op, U, vel.as<double>(), ta.as<double>(), f, fc, nl, ut, p.utt, dt);
-      std::swap(u, ta);
+      if constexpr (PS == P64) {
+        time_utt(s, op, U, vel.as<double>(), Ta, f, fc, nl, ut, p.utt, dt);
+        std::swap(u, ta);
+      }
     } else {
-      time_stage(s, op, U, U, nullptr, ta.as<double>(), f, fc, nl, ut, dt, dt, 0);
+      stage(s, op, U, U, static_cast<S*>(nullptr), Ta, f, fc, nl, ut, dt, dt, 0);
       std::swap(u, ta);
     }
-    closures_nr(L0, op, fd, u.as<double>());
-    closures_dirichlet(L0, op, fd, u.as<double>());
+    closures_nr(L0, op, fd, u.as<S>());
+    closures_dirichlet(L0, op, fd, u.as<S>());
     ++steps;
   }
   flush();
   HIPCHK(hipStreamSynchronize(s));
   auto t1 = std::chrono::steady_clock::now();
-  if (u_out) HIPCHK(hipMemcpy(u_out, u.get(), n * 8, hipMemcpyDeviceToHost));
+  if (u_out) {
+    widen(u.as<S>(), fbuf.as<double>());
+    HIPCHK(hipMemcpyAsync(u_out, fbuf.get(), n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
   if (rep) {
     std::memset(rep, 0, sizeof(*rep));
     rep->iterations = steps;
     rep->converged = 1;
     rep->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   }
+}
+
+}  // namespace
+
+// Time series (SolvePDETimeSeries, solver.cpp:989-1329).  precision PDEHIP_MIXED keeps the
+// state in FP32 (stage arithmetic FP64); u_tt problems always run in FP64.
+void time_series(Ctx* ctx, const pdehip_problem& p, const pdehip_time_config& tc,
+                 pdehip_frame_fn cb, void* user, double* u_out, pdehip_report* rep) {
+  if (p.nz > 1) throw Error(PDEHIP_ERR_UNSUPPORTED, "3D Time Series Solver not yet refactored.");
+  if (tc.dt <= 0.0) throw Error(PDEHIP_ERR_INVALID, "time step dt must be positive");
+  const bool has_ut = std::abs(p.ut) > 1e-12, has_utt = std::abs(p.utt) > 1e-12;
+  if (!has_ut && !has_utt) throw Error(PDEHIP_ERR_INVALID, "time-dependent solve requires u_t or u_tt term");
+  if (has_utt && p.initial && !p.initial_velocity)
+    throw Error(PDEHIP_ERR_INVALID, "checkpoint restart for u_tt requires velocity data");
+  pdehip_options o{};
+  o.method = PDEHIP_JACOBI;
+  o.max_iter = 1;
+  o.sor_omega = 1.5;
+  auto P = plan_create(ctx, p, o);
+  if (tc.precision == PDEHIP_MIXED && !has_utt)
+    run_series<float>(ctx, *P, p, tc, cb, user, u_out, rep);
+  else
+    run_series<double>(ctx, *P, p, tc, cb, user, u_out, rep);
 }
 
 // ------------------------------------------------------------------ kernel timing

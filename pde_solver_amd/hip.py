@@ -152,6 +152,8 @@ class Context:
 
     def time_series(self, spec: capi.Spec, tc: capi.TimeConfig, keep_frames: bool = True,
                     on_frame=None):
+        """Frames (if keep_frames or on_frame), final state, report.  With keep_frames False
+        and no on_frame, no frame callback is installed: only the final state is copied."""
         p = spec.to_c()
         frames: List[np.ndarray] = []
 
@@ -163,7 +165,7 @@ class Context:
                 return 1 if on_frame(frame, t, arr) is not False else 0
             return 1
 
-        fn = capi.FRAME_FN(cb)
+        fn = capi.FRAME_FN(cb) if (keep_frames or on_frame is not None) else C.cast(None, capi.FRAME_FN)
         u = np.zeros(spec.n)
         rep = capi.Report()
         self._check(lib().pdehip_time_series(self.h, C.byref(p), C.byref(tc), fn, None,
