@@ -14,6 +14,7 @@ guess with the operator, hierarchy and right-hand side resident in HBM.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 257] [--no-cpu-baseline]
     python bench.py --only mixed|fp64 [--steps K] [--warmup W]   (one path only, for profiling)
     python bench.py --workload batch [--steps K] [--count 512]  (512 uneven 2D problems)
+    python bench.py --workload nonsym [--count 24] [--rd-steps 1000]
     python bench.py --kernels 4,3 [--precision mixed|fp64] [--steps K]
         (K launches of plan kernels only: 0 apply, 1 in-place sweep, 2 residual, 3 copy,
          4 fused sweep; for hardware-counter runs)
@@ -91,6 +92,145 @@ def make_batch_problems(count: int = 512, seed: int = 3):
         s.bc_convention = capi.BCCONV_OUTWARD
         specs.append(s)
     return specs
+
+
+def fourier_source(n, rng):
+    """Smooth seeded source: sum of 8 modes sin(pi p x + phi) sin(pi q y + psi), p, q ~ U{1..8},
+    amplitudes N(0, 1), phases U(0, 2 pi)."""
+    x = np.linspace(0.0, 1.0, n)
+    f = np.zeros((n, n))
+    for _ in range(8):
+        p, q = rng.integers(1, 9, size=2)
+        amp = rng.standard_normal()
+        phi, psi = rng.uniform(0.0, 2 * np.pi, size=2)
+        f += amp * np.outer(np.sin(np.pi * q * x + psi), np.sin(np.pi * p * x + phi))
+    return f
+
+
+def make_nonsym_problems(count: int = 128, seed: int = 4):
+    """Convection-diffusion -eps lap u + b.grad u = f, first-order upwind, Dirichlet 0:
+    n = 2^(7+K)+1 with K ~ U{0..3} (129..1025), eps = 10^U(-3,-1), b = (cos t, sin t) with
+    t ~ U(0, 2 pi), 8-mode Fourier source."""
+    from pde_solver_amd import capi
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(count):
+        n = 2 ** (7 + int(rng.integers(0, 4))) + 1
+        eps = float(10.0 ** rng.uniform(-3.0, -1.0))
+        th = float(rng.uniform(0.0, 2 * np.pi))
+        s = capi.Spec(n, n, a=-eps, b=-eps, c=float(np.cos(th)), d=float(np.sin(th)),
+                      convection=capi.CONV_UPWIND)
+        s.f_field = -fourier_source(n, rng)
+        specs.append(s)
+    return specs
+
+
+def bench_nonsym(args) -> int:
+    """Nonsymmetric workload: Jacobi-preconditioned BiCGStab and GMRES(30) to rel-resid 1e-8 on
+    the convection-diffusion problems (a bounded sample, one resident plan after the other),
+    mixed precision vs FP64; and reaction-diffusion u_t = lap u + u(1 - u) on 2049^2, 1000 RK4
+    steps at dt = 0.2 h^2, FP32 state vs FP64."""
+    from scipy.ndimage import gaussian_filter
+    from pde_solver_amd import capi, hip
+    from pde_solver_amd.capi import BC
+
+    ok, note = hip.is_available()
+    log(f"device: {note}")
+    if not ok:
+        raise SystemExit("no usable gfx9 device")
+    specs = make_nonsym_problems(128)[: args.count]
+    ctx = hip.Context()
+
+    def kopts(method, precision, tol=1e-8):
+        return capi.options(method, tol=tol, tol_mode=capi.TOL_REL, precond=capi.PC_JACOBI,
+                            bc_rows=capi.ROWS_FOLDED, gmres_restart=30, max_iter=10000,
+                            precision=precision)
+
+    results = {}
+    for method, mname in ((capi.BICGSTAB, "bicgstab"), (capi.GMRES, "gmres")):
+        sols = {}
+        for precision, pname in ((capi.FP64, "fp64"), (capi.MIXED, "mixed")):
+            plans = [ctx.plan(s, kopts(method, precision)) for s in specs]
+            lat, its, conv, us = [], [], 0, []
+            t0 = time.perf_counter()
+            for pl in plans:
+                t1 = time.perf_counter()
+                st, rep = pl.solve(check=False)
+                lat.append((time.perf_counter() - t1) * 1e3)
+                its.append(rep.inner_iterations if precision == capi.MIXED else rep.iterations)
+                conv += int(st == 0)
+            wall = time.perf_counter() - t0
+            for pl in plans:
+                us.append(pl.download())
+                pl.close()
+            sols[pname] = us
+            results[f"{mname}_{pname}"] = {
+                "solves_per_s": round(len(specs) / wall, 2), "converged": conv,
+                "problems": len(specs), "latency_ms_p50": round(pct(lat, 50), 2),
+                "latency_ms_p99": round(pct(lat, 99), 2), "iterations_total": int(sum(its))}
+            log(f"{mname} {pname}: {results[f'{mname}_{pname}']['solves_per_s']} solves/s, "
+                f"{conv}/{len(specs)} converged, {sum(its)} iterations")
+        errs = [float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+                for a, b in zip(sols["mixed"], sols["fp64"])]
+        results[f"{mname}_mixed"]["max_diff_vs_fp64"] = max(errs)
+
+    # reaction-diffusion, 2049^2, 1000 RK4 steps
+    n = 2049
+    h = 1.0 / (n - 1)
+    rd = capi.Spec(n, n, a=-1.0, b=-1.0, ut=1.0)
+    rd.nonlinear = [(capi.NL_POWER, -1.0, 1), (capi.NL_POWER, 1.0, 2)]   # N(u) = -u + u^2
+    rd.initial = gaussian_filter(np.random.default_rng(44).uniform(0.0, 1.0, (n, n)), 2.0,
+                                 mode="reflect")
+    rd.bcs = [BC.neumann(0.0)] * 4 + [BC(), BC()]
+    steps = args.rd_steps
+    rdres = {}
+    for precision, pname in ((capi.MIXED, "fp32"), (capi.FP64, "fp64")):
+        tc = capi.time_config(capi.TI_RK4, dt=0.2 * h * h, frames=steps + 1, precision=precision)
+        ctx.time_series(rd, capi.time_config(capi.TI_RK4, dt=0.2 * h * h, frames=3, precision=precision),
+                        keep_frames=False)
+        _, u, rep = ctx.time_series(rd, tc, keep_frames=False)
+        es = 4 if precision == capi.MIXED else 8
+        per_step = 17 * es * rd.n        # fused-stage RK4 compulsory bytes
+        rdres[pname] = {"steps": steps, "ms": round(rep.solve_ms, 2),
+                        "steps_per_s": round(steps / (rep.solve_ms / 1e3), 1),
+                        "GB_per_s": round(per_step * steps / (rep.solve_ms / 1e3) / 1e9, 1),
+                        "u_mean": float(u.mean())}
+        rdres[pname + "_u"] = u
+        log(f"RD {pname}: {rdres[pname]['ms']} ms for {steps} steps ({rdres[pname]['GB_per_s']} GB/s)")
+    rdres["fp32"]["max_rel_diff_vs_fp64"] = float(np.abs(rdres.pop("fp32_u") - rdres["fp64_u"]).max()
+                                                   / np.abs(rdres["fp64_u"]).max())
+    rdres.pop("fp64_u")
+
+    mixed_total = 2 * len(specs) / (len(specs) / results["bicgstab_mixed"]["solves_per_s"]
+                                     + len(specs) / results["gmres_mixed"]["solves_per_s"])
+    fp64_total = 2 * len(specs) / (len(specs) / results["bicgstab_fp64"]["solves_per_s"]
+                                    + len(specs) / results["gmres_fp64"]["solves_per_s"])
+    out = {
+        "metric": METRIC,
+        "value": round(mixed_total, 2),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(1e3 / mixed_total, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 Krylov vectors + f64 solution/residual",
+        "data": "synthetic",
+        "config": {"workload": f"nonsymmetric: the first {len(specs)} of 128 upwind "
+                               "convection-diffusion problems (129^2..1025^2, eps 10^U(-3,-1)), "
+                               "Jacobi-PC BiCGStab and GMRES(30) to rel-resid 1e-8, one resident "
+                               "plan after the other; RD 2049^2 RK4 1000 steps",
+                   "problems": len(specs)},
+        "krylov": results,
+        "fp64_solves_per_s": round(fp64_total, 2),
+        "reaction_diffusion_2049": rdres,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
 
 
 def batch_options(precision: int, tol: float):
@@ -308,9 +448,12 @@ def main() -> int:
     ap.add_argument("--kernels", default=None,
                     help="comma list of plan kernels to launch K times each (counter runs)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--workload", choices=["single", "batch"], default="single",
-                    help="single: the 257^3 solve (default); batch: 512 uneven 2D problems")
-    ap.add_argument("--count", type=int, default=512, help="batch workload: problems per step")
+    ap.add_argument("--workload", choices=["single", "batch", "nonsym"], default="single",
+                    help="single: the 257^3 solve (default); batch: 512 uneven 2D problems; "
+                         "nonsym: convection-diffusion Krylov + reaction-diffusion RK4")
+    ap.add_argument("--count", type=int, default=None,
+                    help="batch: problems per step (512); nonsym: problems timed (24 of 128)")
+    ap.add_argument("--rd-steps", type=int, default=1000, help="nonsym: RK4 steps at 2049^2")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
@@ -318,7 +461,13 @@ def main() -> int:
         return 2
 
     if args.workload == "batch":
+        if args.count is None:
+            args.count = 512
         return bench_batch(args)
+    if args.workload == "nonsym":
+        if args.count is None:
+            args.count = 24
+        return bench_nonsym(args)
 
     from pde_solver_amd import capi, hip
 

@@ -222,6 +222,43 @@ inline int plane_chunk(long long tiles, int nz, int capacity, int prologue, int 
 }
 
 // ------------------------------------------------------------------ reductions
+// u^p for an integer p by repeated squaring (|p| <= 16; pow beyond): u^1 and u^2 are the
+// correctly rounded values pow gives, higher powers differ by a few ulp at most.
+__device__ __forceinline__ double ipow(double u, int p) {
+  if (p > 16 || p < -16) return pow(u, static_cast<double>(p));
+  const bool inv = p < 0;
+  unsigned e = static_cast<unsigned>(inv ? -p : p);
+  double r = 1.0, b = u;
+  while (e) {
+    if (e & 1u) r *= b;
+    b *= b;
+    e >>= 1;
+  }
+  return inv ? 1.0 / r : r;
+}
+
+// Grid coordinates (i, j, k) of node id.  Grids of fewer than 2^31 nodes (every 2D grid up to
+// 46341^2, 3D up to 1290^3) use 32-bit unsigned divisions, a few instructions where the 64-bit
+// ones cost dozens per node; larger grids take the 64-bit path.
+struct IJK { int i, j, k; };
+__device__ __forceinline__ IJK coords(const DevOp& op, long long id) {
+  IJK c;
+  if (op.n < 2147483648LL) {
+    const unsigned u = static_cast<unsigned>(id);
+    const unsigned nx = static_cast<unsigned>(op.nx), ny = static_cast<unsigned>(op.ny);
+    const unsigned row = u / nx;
+    c.i = static_cast<int>(u - row * nx);
+    c.j = static_cast<int>(row % ny);
+    c.k = static_cast<int>(row / ny);
+  } else {
+    const long long row = id / op.nx;
+    c.i = static_cast<int>(id - row * op.nx);
+    c.j = static_cast<int>(row % op.ny);
+    c.k = static_cast<int>(row / op.ny);
+  }
+  return c;
+}
+
 // Maximum that propagates NaN (fmax drops it), so a non-finite value in a max-reduction
 // reaches the stop rule instead of reading as a small norm.
 template <typename T>

@@ -734,8 +734,8 @@ kb_resnorm(const BVec* __restrict__ vecs, const BFace* __restrict__ faces,
   const W7<double> wt = const_w<double>(op);
   double acc = 0.0, mx = 0.0;
   for (long long id = it.a + threadIdx.x; id < it.b; id += blockDim.x) {
-    const int i = static_cast<int>(id % op.nx);
-    const int j = static_cast<int>(id / op.nx);
+    const IJK c = coords(op, id);
+    const int i = c.i, j = c.j;
     double r;
     if (is_boundary2(op, i, j)) {
       if (!resnorm_boundary2(op, bf.fd, u, id, i, j, &r)) continue;

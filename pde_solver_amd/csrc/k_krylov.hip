@@ -121,9 +121,8 @@ __global__ void k_jacobi_pc(DevOp op, const S* __restrict__ in, S* __restrict__ 
   if (sc && skip(sc)) return;
   for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < op.n;
        id += (long long)gridDim.x * blockDim.x) {
-    const int i = static_cast<int>(id % op.nx);
-    const int j = static_cast<int>((id / op.nx) % op.ny);
-    const int k = static_cast<int>(id / op.sxy);
+    const IJK c = coords(op, id);
+    const int i = c.i, j = c.j, k = c.k;
     if (is_boundary2(op, i, j) || (DIM == 3 && (k == 0 || k == op.nz - 1))) { out[id] = in[id]; continue; }
     W7<double> w = row_weights<DIM, KIND, double, double>(op, id);
     fold_center<DIM, double>(op, i, j, k, w);

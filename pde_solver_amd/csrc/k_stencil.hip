@@ -58,8 +58,8 @@ __global__ void __launch_bounds__(kBlock) k_stencil(DevOp op, const void* __rest
     const long long n = op.n;
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < n;
          id += (long long)gridDim.x * blockDim.x) {
-      const int i = static_cast<int>(id % op.nx);
-      const int j = static_cast<int>(id / op.nx);
+      const IJK c = coords(op, id);
+      const int i = c.i, j = c.j;
       const T xc = ldT(x, id);
       T out;
       if (is_boundary2(op, i, j)) {
@@ -84,8 +84,8 @@ __global__ void __launch_bounds__(kBlock) k_stencil(DevOp op, const void* __rest
   } else {
     const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (p < op.sxy) {
-      const int i = static_cast<int>(p % op.nx);
-      const int j = static_cast<int>(p / op.nx);
+      const int j = static_cast<int>(static_cast<unsigned>(p) / static_cast<unsigned>(op.nx));
+      const int i = static_cast<int>(p) - j * op.nx;
       const bool bxy = is_boundary2(op, i, j);
       const int k0 = blockIdx.y * kZChunk;
       const int k1 = min(k0 + kZChunk, op.nz);
@@ -155,8 +155,8 @@ __global__ void __launch_bounds__(kBlock) k_rbgs(DevOp op, const void* __restric
   } else {
     const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (p >= op.sxy) return;
-    const int i = static_cast<int>(p % op.nx);
-    const int j = static_cast<int>(p / op.nx);
+    const int j = static_cast<int>(static_cast<unsigned>(p) / static_cast<unsigned>(op.nx));
+    const int i = static_cast<int>(p) - j * op.nx;
     if (is_boundary2(op, i, j)) return;
     const int k0 = max(1, (int)blockIdx.y * kZChunk);
     const int k1 = min((int)blockIdx.y * kZChunk + kZChunk, op.nz - 1);
@@ -179,7 +179,7 @@ __device__ __forceinline__ double eval_nonlinear(const NonlinearTerms& nl, doubl
   for (int t = 0; t < nl.n; ++t) {
     const pdehip_nonlinear& term = nl.t[t];
     switch (term.kind) {
-      case PDEHIP_NL_POWER: v += term.coeff * pow(u, static_cast<double>(term.power)); break;
+      case PDEHIP_NL_POWER: v += term.coeff * ipow(u, term.power); break;
       case PDEHIP_NL_SIN: v += term.coeff * sin(u); break;
       case PDEHIP_NL_COS: v += term.coeff * cos(u); break;
       case PDEHIP_NL_EXP: v += term.coeff * exp(u); break;
@@ -203,9 +203,8 @@ __global__ void __launch_bounds__(kBlock) k_jacobi_ref(DevOp op, const double* _
   const long long n = op.n;
   for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < n;
        id += (long long)gridDim.x * blockDim.x) {
-    const int i = static_cast<int>(id % op.nx);
-    const int j = static_cast<int>((id / op.nx) % op.ny);
-    const int k = static_cast<int>(id / op.sxy);
+    const IJK c = coords(op, id);
+    const int i = c.i, j = c.j, k = c.k;
     if (is_boundary2(op, i, j) || (DIM == 3 && (k == 0 || k == op.nz - 1))) continue;
     const W7<double> w = row_weights<DIM, KIND, double, double>(op, id);
     if (fabs(w.C) < 1e-12) { *degenerate = 1; continue; }
@@ -234,9 +233,8 @@ __global__ void __launch_bounds__(kBlock) k_sor_rb_ref(DevOp op, const double* _
   const long long n = op.n;
   for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < n;
        id += (long long)gridDim.x * blockDim.x) {
-    const int i = static_cast<int>(id % op.nx);
-    const int j = static_cast<int>((id / op.nx) % op.ny);
-    const int k = static_cast<int>(id / op.sxy);
+    const IJK c = coords(op, id);
+    const int i = c.i, j = c.j, k = c.k;
     if (((i + j + k) & 1) != color) continue;
     if (is_boundary2(op, i, j) || (DIM == 3 && (k == 0 || k == op.nz - 1))) continue;
     const W7<double> w = row_weights<DIM, KIND, double, double>(op, id);

@@ -6,17 +6,23 @@
 namespace pdehip {
 namespace {
 
+// Transcendental terms out of line: their code would otherwise raise every stage kernel's
+// register count (and lower its occupancy) whether or not a problem uses them.
+__device__ __noinline__ double eval_nl_other(int kind, double coeff, double u) {
+  switch (kind) {
+    case PDEHIP_NL_SIN: return coeff * sin(u);
+    case PDEHIP_NL_COS: return coeff * cos(u);
+    case PDEHIP_NL_EXP: return coeff * exp(u);
+    default: return coeff * fabs(u);
+  }
+}
+
 __device__ __forceinline__ double eval_nl(const NonlinearTerms& nl, double u) {
   double v = 0.0;
   for (int t = 0; t < nl.n; ++t) {
     const pdehip_nonlinear& term = nl.t[t];
-    switch (term.kind) {
-      case PDEHIP_NL_POWER: v += term.coeff * pow(u, static_cast<double>(term.power)); break;
-      case PDEHIP_NL_SIN: v += term.coeff * sin(u); break;
-      case PDEHIP_NL_COS: v += term.coeff * cos(u); break;
-      case PDEHIP_NL_EXP: v += term.coeff * exp(u); break;
-      default: v += term.coeff * fabs(u); break;
-    }
+    if (term.kind == PDEHIP_NL_POWER) v += term.coeff * ipow(u, term.power);
+    else v += eval_nl_other(term.kind, term.coeff, u);
   }
   return v;
 }
@@ -34,14 +40,17 @@ template <typename T> __device__ __forceinline__ double ldv(const T* p, long lon
 template <int KIND, typename S>
 __device__ __forceinline__ double dudt_at(const DevOp& op, const S* v, long long id, int i, int j,
                                          const double* f, double f_const, const NonlinearTerms& nl,
-                                         double ut) {
+                                         double ut, double inv_ut) {
   if (is_boundary2(op, i, j)) return 0.0;
   const W7<double> w = row_weights<2, KIND, double, double>(op, id);
   const double vc = ldv(v, id);
   const double opv = w.C * vc + (w.E * ldv(v, id + 1) + w.W * ldv(v, id - 1) +
                                  w.N * ldv(v, id + op.nx) + w.S * ldv(v, id - op.nx));
   const double rhs = opv + (f ? f[id] : f_const) + (nl.n ? eval_nl(nl, vc) : 0.0);
-  return safe_div(-rhs, ut);
+  // safe_div(-rhs, ut) with the reciprocal of the (uniform) u_t coefficient formed once
+  if (fabs(ut) < 1e-12) return 0.0;
+  const double r = -rhs * inv_ut;
+  return isfinite(r) ? r : 0.0;
 }
 
 // Stage kernel.  Per node:  k = F(v)
@@ -58,11 +67,12 @@ __global__ void __launch_bounds__(kBlock) k_stage(DevOp op, const S* __restrict_
                                                  S* __restrict__ out, const double* f,
                                                  double f_const, NonlinearTerms nl, double ut,
                                                  double dt, double c1, int mode) {
+  const double inv_ut = 1.0 / ut;
   for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < op.n;
        id += (long long)gridDim.x * blockDim.x) {
-    const int i = static_cast<int>(id % op.nx);
-    const int j = static_cast<int>(id / op.nx);
-    const double k = dudt_at<KIND, S>(op, v, id, i, j, f, f_const, nl, ut);
+    const IJK c = coords(op, id);
+    const int i = c.i, j = c.j;
+    const double k = dudt_at<KIND, S>(op, v, id, i, j, f, f_const, nl, ut, inv_ut);
     const double u0 = ldv(un, id);
     double o;
     switch (mode) {
@@ -85,8 +95,8 @@ __global__ void k_utt(DevOp op, const double* __restrict__ u, double* __restrict
                       double ut, double utt, double dt) {
   for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < op.n;
        id += (long long)gridDim.x * blockDim.x) {
-    const int i = static_cast<int>(id % op.nx);
-    const int j = static_cast<int>(id / op.nx);
+    const IJK c = coords(op, id);
+    const int i = c.i, j = c.j;
     if (is_boundary2(op, i, j)) { next[id] = u[id]; continue; }
     const W7<double> w = row_weights<2, KIND, double, double>(op, id);
     const double opv = w.C * u[id] + (w.E * u[id + 1] + w.W * u[id - 1] + w.N * u[id + op.nx] +
@@ -111,9 +121,8 @@ __global__ void __launch_bounds__(kBlock) k_resnorm(DevOp op, FaceData fd, const
   double acc = 0.0, mx = 0.0;
   for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < op.n;
        id += (long long)gridDim.x * blockDim.x) {
-    const int i = static_cast<int>(id % op.nx);
-    const int j = static_cast<int>((id / op.nx) % op.ny);
-    const int k = static_cast<int>(id / op.sxy);
+    const IJK c = coords(op, id);
+    const int i = c.i, j = c.j, k = c.k;
     double r;
     if (DIM == 2 && is_boundary2(op, i, j)) {
       if (!resnorm_boundary2(op, fd, u, id, i, j, &r)) continue;
@@ -162,9 +171,8 @@ __global__ void k_build_rhs(DevOp op, const double* f, double f_const, const dou
   const bool d3 = op.nz > 1;
   for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < op.n;
        id += (long long)gridDim.x * blockDim.x) {
-    const int i = static_cast<int>(id % op.nx);
-    const int j = static_cast<int>((id / op.nx) % op.ny);
-    const int k = static_cast<int>(id / op.sxy);
+    const IJK c = coords(op, id);
+    const int i = c.i, j = c.j, k = c.k;
     const bool bd = is_boundary2(op, i, j) || (d3 && (k == 0 || k == op.nz - 1));
     b[id] = bd ? bnd[id] : -(f ? f[id] : f_const);
   }
