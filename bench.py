@@ -15,6 +15,7 @@ guess with the operator, hierarchy and right-hand side resident in HBM.
     python bench.py --only mixed|fp64 [--steps K] [--warmup W]   (one path only, for profiling)
     python bench.py --workload batch [--steps K] [--count 512]  (512 uneven 2D problems)
     python bench.py --workload nonsym [--count 24] [--rd-steps 1000]
+    python bench.py --workload serve [--count 400] [--rates 50,200]
     python bench.py --kernels 4,3 [--precision mixed|fp64] [--steps K]
         (K launches of plan kernels only: 0 apply, 1 in-place sweep, 2 residual, 3 copy,
          4 fused sweep; for hardware-counter runs)
@@ -227,6 +228,153 @@ def bench_nonsym(args) -> int:
         "krylov": results,
         "fp64_solves_per_s": round(fp64_total, 2),
         "reaction_diffusion_2049": rdres,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+def make_serve_requests(count: int, seed: int = 5):
+    """Request stream: 2D (3 in 4) with n = 2^(5+K)+1, K ~ U{0..6}, or 3D with K ~ U{0..3};
+    operator Poisson / lognormal variable-coefficient diffusion (flux form) / upwind
+    convection-diffusion; MG-PCG for the symmetric ones, Jacobi-PC BiCGStab or GMRES(30) for
+    convection-diffusion (kept to 2D n <= 513 and 3D n <= 33, where GMRES(30) converges);
+    rel-resid 1e-8.  Coefficient fields and sources are drawn once per size and shared."""
+    from scipy.ndimage import gaussian_filter
+    from pde_solver_amd import capi
+    from pde_solver_amd.capi import BC
+
+    rng = np.random.default_rng(seed)
+    kappas, sources = {}, {}
+
+    def field(key, shape, maker):
+        if key not in maker[0]:
+            maker[0][key] = maker[1](shape)
+        return maker[0][key]
+
+    def kappa_of(shape):
+        g = gaussian_filter(np.random.default_rng(len(shape) * 1000 + shape[-1]).standard_normal(shape),
+                            4.0, mode="reflect")
+        g = (g - g.mean()) / max(g.std(), 1e-12)
+        return np.exp(0.5 * g)
+
+    def source_of(shape):
+        f = np.random.default_rng(shape[-1] + 7 * len(shape)).standard_normal(shape)
+        return f - f.mean()
+
+    reqs = []
+    for _ in range(count):
+        d3 = rng.uniform() < 0.25
+        n = 2 ** (5 + int(rng.integers(0, 4 if d3 else 7))) + 1
+        kind = int(rng.integers(0, 3))
+        if kind == 2 and n > (33 if d3 else 513):
+            kind = int(rng.integers(0, 2))
+        shape = (n, n, n) if d3 else (n, n)
+        if kind == 0:
+            s = capi.Spec(*shape[::-1], a=-1.0, b=-1.0, az=-1.0 if d3 else 0.0)
+        elif kind == 1:
+            s = capi.Spec(*shape[::-1], a=-1.0, op_kind=capi.OP_FLUX)
+            s.kappa = field(shape, shape, (kappas, kappa_of))
+        else:
+            eps = float(10.0 ** rng.uniform(-3.0, -1.0))
+            th = float(rng.uniform(0.0, 2 * np.pi))
+            s = capi.Spec(*shape[::-1], a=-eps, b=-eps, az=-eps if d3 else 0.0,
+                          c=float(np.cos(th)), d=float(np.sin(th)), convection=capi.CONV_UPWIND)
+        s.f_field = field(shape, shape, (sources, source_of))
+        if kind < 2 and d3:
+            s.bcs = [BC()] * 4 + [BC.robin(1.0, 1.0, 0.0)] * 2
+            s.bc_convention = capi.BCCONV_OUTWARD
+        elif kind < 2:
+            kinds = [BC.dirichlet(0.0), BC.neumann(0.0), BC.robin(1.0, 1.0, 0.0)]
+            faces = [kinds[k] for k in rng.integers(0, 3, size=4)]
+            if all(f.kind == capi.NEUMANN for f in faces):
+                faces[0] = BC.dirichlet(0.0)
+            s.bcs = faces + [BC(), BC()]
+            s.bc_convention = capi.BCCONV_OUTWARD
+        method = capi.MGPCG if kind < 2 else (capi.BICGSTAB if rng.uniform() < 0.5 else capi.GMRES)
+        reqs.append((s, method))
+    return reqs
+
+
+def bench_serve(args) -> int:
+    """Serving: Poisson arrivals at each rate, requests submitted to the continuous-batching
+    server as they arrive, latency from arrival to completion; mixed precision vs FP64."""
+    from pde_solver_amd import capi, hip
+
+    ok, note = hip.is_available()
+    log(f"device: {note}")
+    if not ok:
+        raise SystemExit("no usable gfx9 device")
+    reqs = make_serve_requests(args.count)
+    rates = [float(r) for r in args.rates.split(",")]
+    ctx = hip.Context()
+
+    def opts(method, precision):
+        if method == capi.MGPCG:
+            return capi.options(capi.MGPCG, tol=1e-8, tol_mode=capi.TOL_REL, precond=capi.PC_MG,
+                                bc_rows=capi.ROWS_FOLDED, precision=precision, max_iter=200)
+        return capi.options(method, tol=1e-8, tol_mode=capi.TOL_REL, precond=capi.PC_JACOBI,
+                            bc_rows=capi.ROWS_FOLDED, gmres_restart=30, max_iter=20000,
+                            precision=precision)
+
+    results = {}
+    for precision, pname in ((capi.MIXED, "mixed"), (capi.FP64, "fp64")):
+        for lam in rates:
+            srv = ctx.server(capi.ServerConfig(max_inflight=8, streams=4, cache_operators=1))
+            arr = np.cumsum(np.random.default_rng(55).exponential(1.0 / lam, size=len(reqs)))
+            pending = {}
+            lat, fails = [], 0
+            t0 = time.perf_counter()
+            k = 0
+            last_done = t0
+            while k < len(reqs) or pending:
+                now = time.perf_counter() - t0
+                while k < len(reqs) and arr[k] <= now:
+                    spec, method = reqs[k]
+                    pending[srv.submit(spec, opts(method, precision))] = arr[k]
+                    k += 1
+                for tk in list(pending):
+                    st, _, rep = srv.poll(tk)
+                    if st == capi.PENDING:
+                        continue
+                    tdone = time.perf_counter() - t0
+                    lat.append((tdone - pending.pop(tk)) * 1e3)
+                    fails += int(st != 0)
+                    last_done = tdone
+                if k < len(reqs) and not pending:
+                    time.sleep(max(0.0, min(arr[k] - (time.perf_counter() - t0), 0.01)))
+                else:
+                    time.sleep(0.0002)
+            stats = srv.stats()
+            srv.close()
+            key = f"{pname}_lambda{int(lam)}"
+            results[key] = {"requests": len(reqs), "failed": fails,
+                            "solves_per_s": round(len(reqs) / last_done, 1),
+                            "p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
+                            "cache_hits": int(stats["cache_hits"]),
+                            "peak_GiB": round(stats["peak_bytes"] / 2**30, 2)}
+            log(f"{key}: {results[key]}")
+    hi = max(rates)
+    out = {
+        "metric": METRIC,
+        "value": results[f"mixed_lambda{int(hi)}"]["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": 0,
+        "ms_per_step": results[f"mixed_lambda{int(hi)}"]["p50_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner solves + f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"serving: {len(reqs)} requests per rate, Poisson arrivals at "
+                               f"{args.rates} req/s, 2D 33^2..2049^2 and 3D 33^3..257^3, Poisson / "
+                               "var-coef / conv-diff, MG-CG / BiCGStab / GMRES, rel-resid 1e-8",
+                   "requests": len(reqs), "rates": rates},
+        "latency_ms": {"p50": results[f"mixed_lambda{int(hi)}"]["p50_ms"],
+                       "p99": results[f"mixed_lambda{int(hi)}"]["p99_ms"]},
+        "serving": results,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -448,9 +596,11 @@ def main() -> int:
     ap.add_argument("--kernels", default=None,
                     help="comma list of plan kernels to launch K times each (counter runs)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--workload", choices=["single", "batch", "nonsym"], default="single",
+    ap.add_argument("--workload", choices=["single", "batch", "nonsym", "serve"], default="single",
                     help="single: the 257^3 solve (default); batch: 512 uneven 2D problems; "
-                         "nonsym: convection-diffusion Krylov + reaction-diffusion RK4")
+                         "nonsym: convection-diffusion Krylov + reaction-diffusion RK4; "
+                         "serve: Poisson-arrival request stream")
+    ap.add_argument("--rates", default="50,200", help="serve: arrival rates, req/s")
     ap.add_argument("--count", type=int, default=None,
                     help="batch: problems per step (512); nonsym: problems timed (24 of 128)")
     ap.add_argument("--rd-steps", type=int, default=1000, help="nonsym: RK4 steps at 2049^2")
@@ -468,6 +618,10 @@ def main() -> int:
         if args.count is None:
             args.count = 24
         return bench_nonsym(args)
+    if args.workload == "serve":
+        if args.count is None:
+            args.count = 400
+        return bench_serve(args)
 
     from pde_solver_amd import capi, hip
 

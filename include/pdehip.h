@@ -168,7 +168,8 @@ typedef struct pdehip_report {
   int status;             /* enum pdehip_status */
   int iterations;         /* outer iterations (Krylov / relaxation sweeps / V-cycles) */
   int inner_iterations;   /* mixed: total FP32 inner iterations */
-  int refinements;        /* mixed: FP64 refinement steps */
+  int refinements;        /* mixed: FP64 refinement / residual-replacement steps; -1 when a
+                             diverging FP32 BiCGStab was completed by the FP64 iteration */
   int converged;
   double final_metric;    /* the quantity the stop rule tested (||r||_2 or max|du|) */
   double rel_residual;    /* ||b - A u||_2 / ||b||_2 in FP64 (FOLDED/REFERENCE rows) */

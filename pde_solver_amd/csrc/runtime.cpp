@@ -1141,8 +1141,9 @@ void plan_residual_norms(Plan& P, const double* u_dev, double* l2, double* linf)
 // in FP64, and only that true residual stops the solve.  The Krylov state (p, r_hat, rho)
 // carries on across replacements; a breakdown restarts it from the true residual.  FP32
 // applies use the difference form.  On diffusion-dominated problems of 513^2 and more the FP32
-// iteration can diverge where FP64 BiCGStab converges; the solve then reports non-convergence
-// (restarted GMRES in mixed precision converges there with the FP64 iteration count).
+// iteration can diverge where FP64 BiCGStab converges: the solve then completes with the FP64
+// BiCGStab from the initial guess (report: refinements = -1, inner_iterations = the FP32
+// iterations spent).  Restarted GMRES in mixed precision keeps the FP64 iteration count.
 static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
   const hipStream_t s = P.stream();
   const long long n = P.g.n;
@@ -1230,9 +1231,26 @@ static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
   }
   const KScalars ko = pull(P, 0);
   const KScalars ki = pull(P, 1);
-  rep->iterations = iter_base + ki.iter;
+  const int mixed_iters = iter_base + ki.iter;
+  if (!ko.done && !cancelled(P)) {
+    // the FP32 iteration diverged or stalled: the FP64 BiCGStab (reference semantics) solves
+    // the problem from the initial guess; its vectors are allocated on the first such case
+    const size_t n8 = n * sizeof(double);
+    for (DBuf* b : {&P.p, &P.q, &P.z, &P.w1, &P.w2, &P.w3, &P.w4}) {
+      if (b == &P.w4 && !pc) continue;
+      if (!*b) { b->alloc(P.ctx, n8); fill(s, P64, b->get(), n, 0.0); }
+    }
+    pdehip_report r64rep{};
+    solve_bicgstab(P, &r64rep);
+    *rep = r64rep;
+    rep->iterations = mixed_iters + r64rep.iterations;
+    rep->inner_iterations = mixed_iters;
+    rep->refinements = -1;   // marks the FP64 completion
+    return;
+  }
+  rep->iterations = mixed_iters;
   rep->refinements = replacements;
-  rep->inner_iterations = iter_base + ki.iter;
+  rep->inner_iterations = mixed_iters;
   rep->final_metric = ko.metric;
   rep->converged = ko.done;
   if (cancelled(P) && !ko.done) throw Error(PDEHIP_ERR_CANCELLED, "solve cancelled");

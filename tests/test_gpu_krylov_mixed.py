@@ -77,3 +77,14 @@ def test_mixed_krylov_3d(ctx, method):
     assert st == 0, ctx.error()
     _, uo, _ = O.solve(s, opts(method, capi.FP64, 1e-12))
     assert rel(u, uo) < 1e-7
+
+
+def test_mixed_bicgstab_diffusion_dominated_completes(ctx):
+    """eps = 0.1 at 513^2: the FP32 BiCGStab iteration diverges there; the solve must still
+    return the FP64 answer (completed by the FP64 iteration, refinements = -1) or converge."""
+    s = convdiff(513, 0.1, 1.0, seed=1)
+    st64, u64, _ = ctx.solve(s, opts(capi.BICGSTAB, capi.FP64, 1e-8))
+    st, u, rep = ctx.solve(s, opts(capi.BICGSTAB, capi.MIXED, 1e-8))
+    assert st == 0 and st64 == 0, ctx.error()
+    assert rep.rel_residual < 1.5e-8
+    assert rel(u, u64) < 1e-5
