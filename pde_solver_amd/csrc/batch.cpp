@@ -91,7 +91,7 @@ struct BatchPlan {
 
 namespace {
 
-// Work-item shapes.  Smoother items are 60-column strips of `ychunk` rows (a wavefront marches
+// Work-item shapes.  Smoother items are 124-column strips of `ychunk` rows (a wavefront marches
 // the rows, so short chunks trade ring re-loads for more items in flight); row items cover
 // whole rows, about `row_nodes` nodes each (at least one row); restriction items cover about
 // `crow_nodes` coarse nodes.
@@ -108,7 +108,7 @@ LevelShape default_shape(const BatchPlan& B, int l) {
 
 std::vector<BItem> sweep_items(const BatchPlan& B, int l, int ychunk) {
   std::vector<BItem> v;
-  const int strip = 60;
+  const int strip = 124;   // output columns of one wavefront strip (k_batch.hip)
   for (int p = 0; p < B.n; ++p) {
     if (l >= B.levels[p] - 1) continue;            // coarsest level: dense solve
     const Geo& g = B.plans[p]->L[l].g;

@@ -16,7 +16,9 @@ namespace bk {
 namespace {
 
 constexpr int kLanes = 64;
-constexpr int kTileX = kLanes - 4;   // smoother strip: 60 output columns, 2-node ring
+// Strips: each lane holds two adjacent columns, so a wavefront covers 128 columns, and the
+// 124 columns away from the 2-column ring on each side are output.
+constexpr int kTileX = 2 * kLanes - 4;
 constexpr int kWavesPB = 4;          // wavefront items per 256-thread workgroup
 constexpr int kAhead = 1;            // row marches: rows loaded ahead of the row in use
 
@@ -42,6 +44,26 @@ template <> __device__ __forceinline__ auto ldS<__half>(const __half* p, long lo
 }
 template <> __device__ __forceinline__ auto ldS<int>(const int* p, long long i) {
   return ((const PDEHIP_GAS int*)p)[i];
+}
+// Two adjacent elements p[i], p[i+1] in one global load / store (dwordx4, x2 or x1): two-lane
+// vectors with the element's own alignment, which global accesses of this width accept.
+template <typename S> struct V2;
+template <> struct V2<double> { typedef double type __attribute__((ext_vector_type(2))) __attribute__((aligned(8))); };
+template <> struct V2<float> { typedef float type __attribute__((ext_vector_type(2))) __attribute__((aligned(4))); };
+template <> struct V2<__half> {
+  typedef unsigned short type __attribute__((ext_vector_type(2))) __attribute__((aligned(2)));
+};
+template <typename S> struct Pair2 { S a, b; };
+template <typename S> __device__ __forceinline__ Pair2<S> ld2(const S* p, long long i) {
+  const typename V2<S>::type v = *((const PDEHIP_GAS typename V2<S>::type*)(p + i));
+  if constexpr (std::is_same<S, __half>::value) return Pair2<S>{__ushort_as_half(v.x), __ushort_as_half(v.y)};
+  else return Pair2<S>{v.x, v.y};
+}
+template <typename S> __device__ __forceinline__ void st2(S* p, long long i, S a, S b) {
+  typename V2<S>::type v;
+  if constexpr (std::is_same<S, __half>::value) { v.x = __half_as_ushort(a); v.y = __half_as_ushort(b); }
+  else { v.x = a; v.y = b; }
+  *((PDEHIP_GAS typename V2<S>::type*)(p + i)) = v;
 }
 __device__ __forceinline__ void stS(double* p, long long i, double v) { ((PDEHIP_GAS double*)p)[i] = v; }
 __device__ __forceinline__ void stS(float* p, long long i, float v) { ((PDEHIP_GAS float*)p)[i] = v; }
@@ -77,43 +99,66 @@ __device__ __forceinline__ W7<T> const_w(const DevOp& op) {
   return w;
 }
 
-// Strip march over item (p, strip a, rows [b, c)): lanes cover columns a*60-2 .. a*60+61; x rows
-// j-1, j, j+1 (and the right-hand side row j) sit in registers, row j+2 (b row j+1) is loaded
-// one step ahead, x-neighbours come by lane shuffles.  f(i, j, id, xc, xw, xe, xs, xn, bj) runs
-// on the 60 output lanes whose column is inside the grid.  Loads are clamped to the grid.
+template <typename T, typename S> __device__ __forceinline__ T cvt(S v);
+template <> __device__ __forceinline__ double cvt<double, double>(double v) { return v; }
+template <> __device__ __forceinline__ float cvt<float, float>(float v) { return v; }
+template <> __device__ __forceinline__ float cvt<float, __half>(__half v) { return __half2float(v); }
+
+// Columns c0, c0+1 (c0 even, possibly outside the grid) of the row at offset rowoff, clamped into
+// the grid, from one paired load of columns lo, lo+1 with lo = clamp(c0, 0, nx-2).
+template <typename T, typename S>
+__device__ __forceinline__ void ld_cols(const S* p, long long rowoff, int c0, int nx, T& v0, T& v1) {
+  const int lo = min(max(c0, 0), nx - 2);
+  const Pair2<S> q = ld2(p, rowoff + lo);
+  const T a = cvt<T, S>(q.a), b = cvt<T, S>(q.b);
+  v0 = min(max(c0, 0), nx - 1) == lo ? a : b;
+  v1 = min(max(c0 + 1, 0), nx - 1) == lo ? a : b;
+}
+
+// Strip march over item (p, strip a, rows [b, c)): lane L holds columns c0 = a*124-2+2L and
+// c0+1; x rows j-1, j, j+1 (and the right-hand side row j) sit in registers, the next rows are
+// loaded kAhead steps ahead, the x-neighbours across lanes come by shuffles.
+// f(i, j, id, xc, xw, xe, xs, xn, bj) runs on the 124 output columns (lanes 1..62) inside the
+// grid.  Loads are clamped to the grid.
 template <typename T, typename S, typename SB, typename F>
 __device__ __forceinline__ void strip_march(const S* __restrict__ x, const SB* __restrict__ b,
                                             int nx, int ny, const BItem& it, F&& f) {
   const int lane = threadIdx.x & (kLanes - 1);
-  const int i = it.a * kTileX - 2 + lane;
-  const int ci = min(max(i, 0), nx - 1);
-  const bool out = lane >= 2 && lane < kLanes - 2 && i < nx;
-  auto ldx = [&](int j) -> T { return T(ldS(x, (long long)min(max(j, 0), ny - 1) * nx + ci)); };
-  auto ldb = [&](int j) -> T {
-    if constexpr (std::is_same<SB, void>::value) return T(0);
-    else return T(ldS(b, (long long)min(max(j, 0), ny - 1) * nx + ci));
-  };
-  // X[k] = x row j-1+k, Bq[k] = b row j+k; kAhead rows are in flight beyond those in use
-  T X[3 + kAhead], Bq[1 + kAhead];
+  const int c0 = it.a * kTileX - 2 + 2 * lane;
+  const bool out = lane >= 1 && lane < kLanes - 1;
+  const bool out0 = out && c0 < nx, out1 = out && c0 + 1 < nx;
+  auto row = [&](int j) -> long long { return (long long)min(max(j, 0), ny - 1) * nx; };
+  // X0/X1[k] = x row j-1+k (columns c0, c0+1), B0/B1[k] = b row j+k
+  T X0[3 + kAhead], X1[3 + kAhead], B0[1 + kAhead], B1[1 + kAhead];
 #pragma unroll
-  for (int k = 0; k < 2 + kAhead; ++k) X[k] = ldx(it.b - 1 + k);
+  for (int k = 0; k < 2 + kAhead; ++k) ld_cols<T>(x, row(it.b - 1 + k), c0, nx, X0[k], X1[k]);
 #pragma unroll
-  for (int k = 0; k < kAhead; ++k) Bq[k] = ldb(it.b + k);
+  for (int k = 0; k < kAhead; ++k) {
+    if constexpr (std::is_same<SB, void>::value) { B0[k] = T(0); B1[k] = T(0); }
+    else ld_cols<T>(b, row(it.b + k), c0, nx, B0[k], B1[k]);
+  }
   for (int j = it.b; j < it.c; ++j) {
-    X[2 + kAhead] = ldx(j + 1 + kAhead);
-    Bq[kAhead] = ldb(j + kAhead);
-    const T xw = __shfl_up(X[1], 1, kLanes), xe = __shfl_down(X[1], 1, kLanes);
-    if (out) f(i, j, (long long)j * nx + i, X[1], xw, xe, X[0], X[2], Bq[0]);
+    ld_cols<T>(x, row(j + 1 + kAhead), c0, nx, X0[2 + kAhead], X1[2 + kAhead]);
+    if constexpr (std::is_same<SB, void>::value) { B0[kAhead] = T(0); B1[kAhead] = T(0); }
+    else ld_cols<T>(b, row(j + kAhead), c0, nx, B0[kAhead], B1[kAhead]);
+    const T w0 = __shfl_up(X1[1], 1, kLanes), e1 = __shfl_down(X0[1], 1, kLanes);
+    const long long id0 = (long long)j * nx + c0;
+    if (out0) f(c0, j, id0, X0[1], w0, X1[1], X0[0], X0[2], B0[0]);
+    if (out1) f(c0 + 1, j, id0 + 1, X1[1], X0[1], e1, X1[0], X1[2], B1[0]);
 #pragma unroll
-    for (int k = 0; k < 2 + kAhead; ++k) X[k] = X[k + 1];
+    for (int k = 0; k < 2 + kAhead; ++k) { X0[k] = X0[k + 1]; X1[k] = X1[k + 1]; }
 #pragma unroll
-    for (int k = 0; k < kAhead; ++k) Bq[k] = Bq[k + 1];
+    for (int k = 0; k < kAhead; ++k) { B0[k] = B0[k + 1]; B1[k] = B1[k + 1]; }
   }
 }
 
 // ------------------------------------------------------------------------------- smoother
-// One red-black sweep of item (p, strip a, rows [b, c)): the single-problem 2D fused sweep
-// (k_smooth.hip) with the operator and buffers of problem p.
+// One red-black sweep of item (p, strip a, rows [b, c)), both colours, out of place: first
+// colour on row p, second colour on row p-1 from the first colour's values.  Lane L holds
+// columns c0 = a*124-2+2L (even) and c0+1, which have opposite colours, so on every row exactly
+// one of the two updates and every lane does useful work: the element of colour c on row p is
+// (c + p) & 1, the same for all lanes.  First colour: every element but the outermost two
+// (their neighbours are outside the wavefront); second colour and output: lanes 1..62.
 template <Prec P, bool ZERO>
 __global__ void __launch_bounds__(kLanes * kWavesPB)
 kb_sweep(const BEnt* __restrict__ tab, const BItem* __restrict__ items, int n, int c1, int dir) {
@@ -129,55 +174,70 @@ kb_sweep(const BEnt* __restrict__ tab, const BItem* __restrict__ items, int n, i
   const S* __restrict__ xin = static_cast<const S*>(dir ? e.alt : e.u);
   S* __restrict__ xout = static_cast<S*>(dir ? e.u : e.alt);
   const int lane = threadIdx.x & (kLanes - 1);
-  const int i = it.a * kTileX - 2 + lane;
+  const int c0 = it.a * kTileX - 2 + 2 * lane;
   const int j0 = it.b, j1 = it.c;
-  const int ny = op.ny;
-  const long long nx = op.nx;
+  const int ny = op.ny, nx = op.nx;
   const int c2 = 1 - c1;
-  const bool in_x = i >= 0 && i < op.nx;
-  const bool r1 = lane >= 1 && lane <= kLanes - 2 && i >= 1 && i <= op.nx - 2;
-  const bool r2 = lane >= 2 && lane < kLanes - 2 && i >= 1 && i <= op.nx - 2;
-  const bool out = lane >= 2 && lane < kLanes - 2 && in_x;
-  const int ci = min(max(i, 0), op.nx - 1);
-  auto ld_u = [&](int j) -> T {
-    if constexpr (ZERO) return T(0);
-    else return T(ldS(xin, (long long)min(max(j, 0), ny - 1) * nx + ci));
+  const bool int0 = c0 >= 1 && c0 <= nx - 2, int1 = c0 + 1 >= 1 && c0 + 1 <= nx - 2;
+  const bool r1_0 = lane >= 1 && int0, r1_1 = lane <= kLanes - 2 && int1;
+  const bool mid = lane >= 1 && lane < kLanes - 1;
+  const bool r2_0 = mid && int0, r2_1 = mid && int1;
+  const bool out0 = mid && c0 < nx, out1 = mid && c0 + 1 < nx;
+  auto row = [&](int j) -> long long { return (long long)min(max(j, 0), ny - 1) * nx; };
+  auto ld_u = [&](int j, T& a, T& bb) {
+    if constexpr (ZERO) { a = T(0); bb = T(0); }
+    else ld_cols<T>(xin, row(j), c0, nx, a, bb);
   };
-  auto ld_b = [&](int j) -> T { return T(ldS(b, (long long)min(max(j, 0), ny - 1) * nx + ci)); };
   const W7<T> w0 = const_w<T>(op);
-  // row queues: U[k] = x row p-2+k, B[k] = b row p-1+k (k < 4 / 3 in use); the rest are
+  // row queues: U*[k] = x row p-2+k, B*[k] = b row p-1+k (k < 4 / 3 in use); the rest are
   // kAhead rows in flight
-  T U[4 + kAhead], B[3 + kAhead];
-  U[0] = U[1] = T(0);
+  T U0[4 + kAhead], U1[4 + kAhead], B0[3 + kAhead], B1[3 + kAhead];
+  U0[0] = U0[1] = U1[0] = U1[1] = T(0);
 #pragma unroll
-  for (int k = 2; k < 4 + kAhead; ++k) U[k] = ld_u(j0 - 4 + k);
-  B[0] = B[1] = T(0);
+  for (int k = 2; k < 4 + kAhead; ++k) ld_u(j0 - 4 + k, U0[k], U1[k]);
+  B0[0] = B0[1] = B1[0] = B1[1] = T(0);
 #pragma unroll
-  for (int k = 2; k < 3 + kAhead; ++k) B[k] = ld_b(j0 - 3 + k);
+  for (int k = 2; k < 3 + kAhead; ++k) ld_cols<T>(b, row(j0 - 3 + k), c0, nx, B0[k], B1[k]);
+  // GS update of column i on row j from its four neighbours
+  auto upd = [&](int i, int j, T bj, T ue, T uw, T un, T us) -> T {
+    W7<T> wt = w0;
+    fold_center<2, T>(op, i, j, 0, wt);
+    return rounded<S, T>(gs_value<2, T>(wt, bj, ue, uw, un, us, T(0), T(0)));
+  };
   for (int p = j0 - 1; p <= j1; ++p) {
 #pragma unroll
-    for (int k = 0; k < 3 + kAhead; ++k) U[k] = U[k + 1];
-    U[3 + kAhead] = ld_u(p + 1 + kAhead);
+    for (int k = 0; k < 3 + kAhead; ++k) { U0[k] = U0[k + 1]; U1[k] = U1[k + 1]; }
+    ld_u(p + 1 + kAhead, U0[3 + kAhead], U1[3 + kAhead]);
 #pragma unroll
-    for (int k = 0; k < 2 + kAhead; ++k) B[k] = B[k + 1];
-    B[2 + kAhead] = ld_b(p + 1 + kAhead);
-    {
-      const T uw = __shfl_up(U[2], 1, kLanes), ue = __shfl_down(U[2], 1, kLanes);
-      if (r1 && p >= 1 && p <= ny - 2 && ((i + p) & 1) == c1) {
-        W7<T> wt = w0;
-        fold_center<2, T>(op, i, p, 0, wt);
-        U[2] = rounded<S, T>(gs_value<2, T>(wt, B[1], ue, uw, U[3], U[1], T(0), T(0)));
+    for (int k = 0; k < 2 + kAhead; ++k) { B0[k] = B0[k + 1]; B1[k] = B1[k + 1]; }
+    ld_cols<T>(b, row(p + 1 + kAhead), c0, nx, B0[2 + kAhead], B1[2 + kAhead]);
+    if (p >= 1 && p <= ny - 2) {
+      if (((c1 + p) & 1) == 0) {   // column c0 has colour c1 on row p
+        const T uw = __shfl_up(U1[2], 1, kLanes);
+        if (r1_0) U0[2] = upd(c0, p, B0[1], U1[2], uw, U0[3], U0[1]);
+      } else {
+        const T ue = __shfl_down(U0[2], 1, kLanes);
+        if (r1_1) U1[2] = upd(c0 + 1, p, B1[1], ue, U0[2], U1[3], U1[1]);
       }
     }
     const int q = p - 1;
     if (q >= j0 && q < j1) {
-      const T uw = __shfl_up(U[1], 1, kLanes), ue = __shfl_down(U[1], 1, kLanes);
-      if (r2 && q >= 1 && q <= ny - 2 && ((i + q) & 1) == c2) {
-        W7<T> wt = w0;
-        fold_center<2, T>(op, i, q, 0, wt);
-        U[1] = rounded<S, T>(gs_value<2, T>(wt, B[0], ue, uw, U[2], U[0], T(0), T(0)));
+      if (q >= 1 && q <= ny - 2) {
+        if (((c2 + q) & 1) == 0) {
+          const T uw = __shfl_up(U1[1], 1, kLanes);
+          if (r2_0) U0[1] = upd(c0, q, B0[0], U1[1], uw, U0[2], U0[0]);
+        } else {
+          const T ue = __shfl_down(U0[1], 1, kLanes);
+          if (r2_1) U1[1] = upd(c0 + 1, q, B1[0], ue, U0[1], U1[2], U1[0]);
+        }
       }
-      if (out) stS(xout, q * nx + i, store_as<S>(static_cast<double>(U[1])));
+      const long long id0 = (long long)q * nx + c0;
+      if (out0 && out1) {
+        st2(xout, id0, store_as<S>(static_cast<double>(U0[1])), store_as<S>(static_cast<double>(U1[1])));
+      } else {
+        if (out0) stS(xout, id0, store_as<S>(static_cast<double>(U0[1])));
+        if (out1) stS(xout, id0 + 1, store_as<S>(static_cast<double>(U1[1])));
+      }
     }
   }
 }
