@@ -3,7 +3,7 @@
 // Set-up builds one resident Plan per problem (the same operators, folded boundary rows,
 // hierarchy and dense coarsest inverse a single solve would use), then the device tables and
 // the work-item lists:
-//   * per level l: smoother strips (problem, 60-column strip, row range), residual/prolongation
+//   * per level l: smoother strips (problem, 124-column strip, row range), residual/prolongation
 //     row blocks, restriction coarse-row blocks (only problems whose hierarchy continues below
 //     l) and dense-solve rows (problems whose coarsest level is l);
 //   * level 0: element blocks for the vector kernels and row blocks for the operator apply,
