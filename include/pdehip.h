@@ -159,7 +159,9 @@ typedef struct pdehip_options {
   int precond;            /* enum pdehip_precond (BiCGStab/GMRES/CG) */
   int precision;          /* enum pdehip_precision */
   int bc_rows;            /* enum pdehip_bc_rows */
-  double inner_tol;       /* mixed: relative reduction per FP32 inner solve (0 = default 1e-4) */
+  double inner_tol;       /* mixed MG-PCG (single and batched): residual-replacement threshold;
+                             the FP32 residual is replaced by the FP64 b - A x once it has
+                             fallen by this factor since the last replacement (0 = 1e-2) */
   int check_every;        /* iterations between host convergence checks (0 = default) */
   const volatile int* cancel;  /* optional cancel flag polled between iterations (pde_types.h:402) */
 } pdehip_options;

@@ -222,6 +222,64 @@ __global__ void __launch_bounds__(1024) k_fin_scale(const double* partials, int 
   }
 }
 
+// x64 += s x32, x32 = 0, when a replacement is due (gate clear).
+__global__ void k_flush_f32(double* __restrict__ x, float* __restrict__ e, long long n,
+                            const double* scale, const KScalars* gate) {
+  if (gate->done | gate->breakdown) return;
+  const double s = *scale;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    x[i] += s * static_cast<double>(e[i]);
+    e[i] = 0.0f;
+  }
+}
+
+// Residual-replacement bookkeeping of the mixed CG (one block): the inner recurrence runs on
+// r / snorm with snorm = |r0| fixed; a replacement is due once the recurrence residual has
+// fallen by `delta` below the largest one since the last replacement, or claims the outer
+// target.  Only a true (FP64) residual stops the solve.
+__global__ void __launch_bounds__(1024) k_fin_ru(KScalars* si, KScalars* so, KScalars* gate,
+                                                const double* partials, int np, int op,
+                                                double delta) {
+  __shared__ double lds[1024 / kWave];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partials[i];
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x != 0) return;
+  switch (op) {
+    case FIN_RU_START:
+      so->snorm = so->metric > 0.0 ? so->metric : 1.0;
+      so->steps = 0;
+      si->done = so->done | so->breakdown;
+      gate->done = 1;
+      break;
+    case FIN_RU_CHECK:
+      if (si->done | si->breakdown) { gate->done = 1; break; }
+      si->rr = s;
+      si->metric = sqrt(s);
+      si->iter += 1;
+      if (nonfinite(s)) { si->breakdown = 1; so->breakdown = 1; gate->done = 1; break; }
+      si->rho = fmax(si->rho, si->metric);
+      si->flag_s = (si->metric < delta * si->rho || si->metric * so->snorm < so->target) ? 1 : 0;
+      gate->done = si->flag_s ? 0 : 1;
+      break;
+    case FIN_RU_APPLY:
+      gate->done = 1;
+      if ((si->done | si->breakdown) || !si->flag_s) break;
+      so->metric = sqrt(s);
+      so->steps += 1;
+      so->iter = si->iter;
+      si->flag_s = 0;
+      if (nonfinite(s)) { so->breakdown = 1; si->breakdown = 1; break; }
+      si->metric = so->metric / so->snorm;
+      si->rho = si->metric;
+      if (so->metric < so->target) { so->done = 1; si->done = 1; }
+      break;
+    default:
+      break;
+  }
+}
+
 }  // namespace
 
 static long long misc_blocks(long long n) {
@@ -241,6 +299,23 @@ void add_f32(hipStream_t s, double* x, const float* e, long long n, const double
 void to_f32_scaled(hipStream_t s, const double* in, float* out, long long n, const double* norm) {
   hipLaunchKernelGGL(k_to_f32_scaled, dim3(misc_blocks(n)), dim3(kBlock), 0, s, in, out, n, norm,
                      static_cast<const KScalars*>(nullptr));
+}
+
+void to_f32_scaled_gated(hipStream_t s, const double* in, float* out, long long n,
+                         const double* norm, const KScalars* gate) {
+  hipLaunchKernelGGL(k_to_f32_scaled, dim3(misc_blocks(n)), dim3(kBlock), 0, s, in, out, n, norm,
+                     gate);
+}
+
+void flush_f32(hipStream_t s, double* x64, float* x32, long long n, const double* scale,
+               const KScalars* gate) {
+  hipLaunchKernelGGL(k_flush_f32, dim3(misc_blocks(n)), dim3(kBlock), 0, s, x64, x32, n, scale,
+                     gate);
+}
+
+void finalize_ru(hipStream_t s, KScalars* si, KScalars* so, KScalars* gate,
+                 const double* partials, int np, int op, double delta) {
+  hipLaunchKernelGGL(k_fin_ru, dim3(1), dim3(1024), 0, s, si, so, gate, partials, np, op, delta);
 }
 
 int maxabs_blocks(long long n) { return static_cast<int>(misc_blocks(n) > 1024 ? 1024 : misc_blocks(n)); }

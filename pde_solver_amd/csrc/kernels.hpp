@@ -130,9 +130,10 @@ enum FinOp {
   FIN_STORE = 5,       // out[slot] = sum
   FIN_MAX = 6,         // out[slot] = max
   FIN_RELAX = 7,       // metric = max; iter++; done = metric < tol (relaxation stop rule)
-  // mixed-precision CG with residual replacement (batched engine)
+  // mixed-precision CG with residual replacement (inner scalars, outer scalars)
   FIN_RU_CHECK = 8,    // as FIN_CG_RR without a stop; flag_s = replacement due
   FIN_RU_APPLY = 9,    // after a replacement: outer metric = sqrt(sum), stop on the true residual
+  FIN_RU_START = 10,   // outer snorm = outer metric (the inner scale); inner done = outer done
 };
 void finalize(const Launch& L, KScalars* sc, const double* partials, int nparts, int op,
               double tol, int tol_mode, double* out_slot);

@@ -25,6 +25,19 @@ void build_rhs(hipStream_t s, const DevOp& op, const double* f, double f_const, 
                double* b);
 void add_f32(hipStream_t s, double* x, const float* e, long long n, const double* scale, double hs);
 void to_f32_scaled(hipStream_t s, const double* in, float* out, long long n, const double* norm);
+// Mixed CG with residual replacement (single problem).  `gate` is a scalars block whose done
+// flag is clear exactly when a replacement is due: kernels launched with it (residual, the two
+// below) run only then.
+void to_f32_scaled_gated(hipStream_t s, const double* in, float* out, long long n,
+                         const double* norm, const KScalars* gate);
+// x64 += (*scale) x32, x32 = 0 (folds the FP32 correction accumulated since the last
+// replacement into the FP64 solution).
+void flush_f32(hipStream_t s, double* x64, float* x32, long long n, const double* scale,
+               const KScalars* gate);
+// FIN_RU_START / FIN_RU_CHECK (tol = the replacement threshold delta) / FIN_RU_APPLY on the
+// inner (si) and outer (so) scalars; gate->done = !(replacement due).
+void finalize_ru(hipStream_t s, KScalars* si, KScalars* so, KScalars* gate,
+                 const double* partials, int np, int op, double delta);
 int maxabs_blocks(long long n);
 void maxabs32(hipStream_t s, const float* a, long long n, double* partials);
 void fin_scale(hipStream_t s, const double* partials, int np, KScalars* sc);

@@ -567,8 +567,8 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
   P->part.alloc(ctx, (size_t)np * gm * sizeof(double));
   P->part2.alloc(ctx, (size_t)np * sizeof(double));
   P->part3.alloc(ctx, (size_t)2 * np * sizeof(double) + 64);
-  P->scal.alloc(ctx, 2 * sizeof(KScalars));
-  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&P->hsc), 2 * sizeof(KScalars)));
+  P->scal.alloc(ctx, 3 * sizeof(KScalars));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&P->hsc), 3 * sizeof(KScalars)));
   {
     DevOp rop = P->op();
     const size_t nst = (size_t)partial_blocks_max(P->op()) + 2 * (size_t)resnorm_blocks(rop) + 8;
@@ -746,85 +746,86 @@ static void solve_cg(Plan& P, bool use_mg, pdehip_report* rep) {
   else if (!k.done) rep->status = PDEHIP_ERR_NOT_CONVERGED;
 }
 
-// Mixed precision MG-PCG: FP64 iterative refinement around FP32 MG-PCG inner solves whose
-// V-cycles keep the coarse levels in FP16 (per-level sigma = h^2 scaling plus a per-cycle
-// input normalisation).  The outer residual b - A x, the solution accumulator, the outer
-// norms and the coarsest-level solve stay FP64.
+// Mixed precision MG-PCG with residual replacement: one FP32 PCG recurrence on r / |r0|
+// (V-cycles with FP16 or FP32 coarse levels under per-level sigma = h^2 scaling and a per-cycle
+// input normalisation) whose correction is accumulated in FP32 and folded into the FP64
+// solution whenever the FP32 residual is replaced by the true FP64 residual b - A x: once it has
+// fallen by `delta` (options.inner_tol, default 1e-2) since the last replacement, or claims the
+// target.  Unlike restarted refinement the Krylov space survives every replacement, so the
+// iteration count stays that of the FP64 solve.  Only a true residual stops the solve.  The
+// outer residual, the solution, the outer norms and the coarsest-level solve stay FP64.
 static void solve_mixed(Plan& P, pdehip_report* rep) {
   const hipStream_t s = P.stream();
-  const Geo& g = P.g;
-  const long long n = g.n;
+  const long long n = P.g.n;
   const DevOp& op64 = P.L[0].op64;
   const DevOp& op32 = P.L[0].op32;
   double* part = P.part.as<double>();
   double* part2 = P.part2.as<double>();
   init_scalars(P, 0, 1e-300);
   init_scalars(P, 1, 1e-300);
+  init_scalars(P, 2, 0.0);
   KScalars* so = P.sc(0);
   KScalars* si = P.sc(1);
+  KScalars* gate = P.sc(2);   // done clear exactly when a replacement is due
   const Launch L0{s, nullptr};
   const Launch Li{s, si};
-  HIPCHK(hipMemcpyAsync(P.x.get(), P.x0.get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
-  residual(L0, op64, P64, P.bvec.get(), P.x.get(), P.r.get(), part);
+  const Launch Lg{s, gate};
+  double* x64 = P.x.as<double>();
+  double* r64 = P.r.as<double>();
+  float* x32 = P.x32.as<float>();
+  float* r32 = P.r32.as<float>();
+  HIPCHK(hipMemcpyAsync(x64, P.x0.get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  residual(L0, op64, P64, P.bvec.get(), x64, r64, part);
   finalize(L0, so, part, partial_blocks(op64), FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
-  double inner_tol = P.opt.inner_tol > 0 ? P.opt.inner_tol : 1e-4;
-  const int max_ref = 30;
-  int inner_total = 0, refinements = 0;
+  const double delta = P.opt.inner_tol > 0 ? P.opt.inner_tol : 1e-2;
+  finalize_ru(s, si, so, gate, part, 0, FIN_RU_START, delta);
+  to_f32_scaled(s, r64, r32, n, &so->snorm);
+  fill(s, P32, x32, n, 0.0);
+  dot(Li, P32, r32, r32, n, part);
+  finalize(Li, si, part, vec_blocks(n), FIN_R0, 0.0, PDEHIP_TOL_ABS, nullptr);
+  maxabs32(s, r32, n, part2);
+  fin_scale(s, part2, maxabs_blocks(n), si);
+  vcycle(P, Li, r32, P.z32.get());
+  dot(Li, P32, r32, P.z32.get(), n, part);
+  finalize(Li, si, part, vec_blocks(n), FIN_CG_INIT, 0, 0, nullptr);
+  HIPCHK(hipMemcpyAsync(P.p32.get(), P.z32.get(), n * sizeof(float), hipMemcpyDeviceToDevice, s));
   const int every = check_every(P);
   const int max_iter = std::max(1, P.opt.max_iter);
-  for (int ref = 0; ref < max_ref; ++ref) {
-    const KScalars& ko = pull(P, 0);
-    if (ko.done || cancelled(P)) break;
-    const double rnorm = ko.metric;
-    ++refinements;
-    // inner: A e = r / ||r||, e0 = 0
-    to_f32_scaled(s, P.r.as<double>(), P.r32.as<float>(), n, &so->metric);
-    fill(s, P32, P.x32.get(), n, 0.0);
-    KScalars ki{};
-    ki.eps = 1e-300; ki.scale = 1.0; ki.inv_scale = 1.0;
-    P.hsc[1] = ki;
-    HIPCHK(hipMemcpyAsync(si, &P.hsc[1], sizeof(KScalars), hipMemcpyHostToDevice, s));
-    dot(L0, P32, P.r32.get(), P.r32.get(), n, part);
-    finalize(L0, si, part, vec_blocks(n), FIN_R0, inner_tol, PDEHIP_TOL_REL, nullptr);
-    maxabs32(s, P.r32.as<float>(), n, part2);
-    fin_scale(s, part2, maxabs_blocks(n), si);
-    vcycle(P, Li, P.r32.get(), P.z32.get());
-    dot(Li, P32, P.r32.get(), P.z32.get(), n, part);
-    finalize(Li, si, part, vec_blocks(n), FIN_CG_INIT, 0, 0, nullptr);
-    HIPCHK(hipMemcpyAsync(P.p32.get(), P.z32.get(), n * sizeof(float), hipMemcpyDeviceToDevice, s));
-    int launched = 0;
-    while (launched < max_iter) {
-      const int chunk = std::min(every, max_iter - launched);
-      for (int c = 0; c < chunk; ++c) {
-        apply(Li, op32, P32, P.p32.get(), P.q32.get(), part);
-        finalize(Li, si, part, partial_blocks(op32), FIN_CG_ALPHA, 0, 0, nullptr);
-        cg_update(Li, P32, P.x32.get(), P.r32.get(), P.p32.get(), P.q32.get(), n, part, part2);
-        finalize(Li, si, part, vec_blocks(n), FIN_CG_RR, 0, 0, nullptr);
-        fin_scale(s, part2, vec_blocks(n), si);
-        vcycle(P, Li, P.r32.get(), P.z32.get());
-        dot(Li, P32, P.r32.get(), P.z32.get(), n, part);
-        finalize(Li, si, part, vec_blocks(n), FIN_CG_BETA, 0, 0, nullptr);
-        cg_xpay(Li, P32, P.z32.get(), P.p32.get(), n);
-      }
-      launched += chunk;
-      const KScalars& k = pull(P, 1);
-      if (k.done || k.breakdown || cancelled(P)) break;
+  int launched = 0;
+  while (launched < max_iter) {
+    const int chunk = std::min(every, max_iter - launched);
+    for (int c = 0; c < chunk; ++c) {
+      apply(Li, op32, P32, P.p32.get(), P.q32.get(), part);
+      finalize(Li, si, part, partial_blocks(op32), FIN_CG_ALPHA, 0, 0, nullptr);
+      cg_update(Li, P32, x32, r32, P.p32.get(), P.q32.get(), n, part, part2);
+      finalize_ru(s, si, so, gate, part, vec_blocks(n), FIN_RU_CHECK, delta);
+      fin_scale(s, part2, vec_blocks(n), si);
+      // replacement (only when due): x += snorm x32, r = b - A x in FP64, r32 = r / snorm
+      flush_f32(s, x64, x32, n, &so->snorm, gate);
+      residual(Lg, op64, P64, P.bvec.get(), x64, r64, part);
+      to_f32_scaled_gated(s, r64, r32, n, &so->snorm, gate);
+      finalize_ru(s, si, so, gate, part, partial_blocks(op64), FIN_RU_APPLY, delta);
+      vcycle(P, Li, r32, P.z32.get());
+      dot(Li, P32, r32, P.z32.get(), n, part);
+      finalize(Li, si, part, vec_blocks(n), FIN_CG_BETA, 0, 0, nullptr);
+      cg_xpay(Li, P32, P.z32.get(), P.p32.get(), n);
     }
+    launched += chunk;
     const KScalars& k = pull(P, 1);
-    inner_total += k.iter;
-    add_f32(s, P.x.as<double>(), P.x32.as<float>(), n, nullptr, rnorm);
-    residual(L0, op64, P64, P.bvec.get(), P.x.get(), P.r.get(), part);
-    finalize(L0, so, part, partial_blocks(op64), FIN_CG_RR, 0, 0, nullptr);
-    if (k.breakdown) break;
+    if (k.done || k.breakdown || cancelled(P)) break;
   }
+  // an unconverged solve keeps the correction accumulated since the last replacement
+  flush_f32(s, x64, x32, n, &so->snorm, so);
+  const KScalars ki = pull(P, 1);
   const KScalars& ko = pull(P, 0);
-  rep->iterations = refinements;
-  rep->refinements = refinements;
-  rep->inner_iterations = inner_total;
+  rep->iterations = ki.iter;
+  rep->refinements = ko.steps;
+  rep->inner_iterations = ki.iter;
   rep->final_metric = ko.metric;
-  rep->converged = ko.done;
+  rep->converged = ko.done && !ko.breakdown;
   if (cancelled(P) && !ko.done) throw Error(PDEHIP_ERR_CANCELLED, "solve cancelled");
-  if (!ko.done) rep->status = PDEHIP_ERR_NOT_CONVERGED;
+  if (ko.breakdown || ki.breakdown) rep->status = PDEHIP_ERR_BREAKDOWN;
+  else if (!ko.done) rep->status = PDEHIP_ERR_NOT_CONVERGED;
 }
 
 // Iterated V-cycles (extension) until ||b - A u||_2 meets the target.

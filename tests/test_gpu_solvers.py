@@ -64,11 +64,30 @@ def test_mgpcg_mixed_varcoef3d_matches_fp64(ctx, n):
     s = varcoef3d(n, seed=1)
     st_o, u_o, _ = O.solve(s, mgpcg(tol=1e-13))
     st, u, rep = ctx.solve(s, mgpcg(tol=1e-10, precision=capi.MIXED))
-    print(f"  refinements {rep.refinements} inner {rep.inner_iterations} rel {rel(u, u_o):.2e} "
-          f"relres {rep.rel_residual:.2e}", flush=True)
+    _, _, rep64 = ctx.solve(s, mgpcg(tol=1e-10))
+    print(f"  replacements {rep.refinements} inner {rep.inner_iterations} (fp64 {rep64.iterations}) "
+          f"rel {rel(u, u_o):.2e} relres {rep.rel_residual:.2e}", flush=True)
     assert st == 0
     assert rel(u, u_o) < 1e-8
-    assert rep.refinements <= 6
+    assert rep.rel_residual < 1e-10
+    # residual replacement keeps one Krylov space: the FP32 recurrence needs about the FP64
+    # iteration count (a restarted refinement would need several more)
+    assert rep.inner_iterations <= rep64.iterations + 2
+    assert 1 <= rep.refinements <= 12
+
+
+def test_mgpcg_mixed_unconverged_keeps_correction(ctx):
+    """A mixed solve stopped by max_iter before any residual replacement still returns the
+    FP32 correction folded into the FP64 solution (status NOT_CONVERGED, residual reduced)."""
+    s = varcoef3d(33, seed=2)
+    opt = mgpcg(tol=1e-12, precision=capi.MIXED, inner_tol=1e-6)
+    opt.max_iter = 2
+    st, u, rep = ctx.solve(s, opt, check=False)
+    print(f"  status {st} inner {rep.inner_iterations} replacements {rep.refinements} "
+          f"relres {rep.rel_residual:.2e}", flush=True)
+    assert st == capi.ERR_NOT_CONVERGED
+    assert rep.refinements == 0
+    assert rep.rel_residual < 1e-2
 
 
 @pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
