@@ -298,6 +298,249 @@ k_rbgs_fused3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xi
   }
 }
 
+// ------------------------------------------------------------------------------- 3D, ring
+// Same tile, plane pipeline, colour order and per-node arithmetic as k_rbgs_fused3 (so the
+// same bits in x_out), restructured for 4-byte types so that the plane step carries no
+// bookkeeping:
+//   * the plane loop is unrolled kRing = 6 times and every register queue is a ring indexed
+//     by the step's phase (compile-time), so queues never shift (no register moves);
+//   * LDS slots (3 for x, 2 for kappa, 3 for the face planes) are compile-time too, and each
+//     LDS plane is padded by one row / column on every side, so every neighbour read is a
+//     fixed immediate offset from a per-thread base and no read needs a clamp;
+//   * global traffic goes through buffer instructions on a per-plane descriptor: the plane
+//     offset is scalar, the in-plane offset a per-thread constant (no 64-bit vector address
+//     arithmetic).  Stores of lanes outside the output tile use an offset past the
+//     descriptor's range, which the hardware discards (no exec-mask branches);
+//   * colour updates are computed on every lane and selected, instead of branching.
+// Loads run two planes ahead for x and one for b and kappa.  Steps past the chunk's last plane
+// (the loop runs a multiple of kRing steps) compute and discard; the host picks z chunks whose
+// step count is a multiple of kRing where it can.
+constexpr int kRing = 6;
+constexpr unsigned kBufRsrcWord3 = 0x00020000u;   // gfx9 raw buffer: 32-bit, no swizzle
+constexpr unsigned kDropOffset = 0x80000000u;     // past any plane's range: store discarded
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, long long elem,
+                                                             int esize, int bytes) {
+  char* p = const_cast<char*>(static_cast<const char*>(base)) + elem * esize;
+  return __builtin_amdgcn_make_buffer_rsrc(p, static_cast<short>(0), bytes,
+                                           static_cast<int>(kBufRsrcWord3));
+}
+
+template <typename S> struct BufIO;
+template <> struct BufIO<float> {
+  static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, unsigned byte_off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+  }
+  static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, unsigned byte_off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, byte_off, 0, 0);
+  }
+};
+template <> struct BufIO<__half> {
+  static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, unsigned byte_off) {
+    const unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(r, byte_off, 0, 0);
+    return __half2float(__ushort_as_half(h));
+  }
+  static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, unsigned byte_off, __half v) {
+    __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(v), r, byte_off, 0, 0);
+  }
+};
+
+template <int N> using IC = std::integral_constant<int, N>;
+
+template <int KIND, Prec P, int R, bool ZERO>
+__global__ void __launch_bounds__(kThreads)
+k_rbgs_ring3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v,
+             void* __restrict__ xout_v, int c1, int tiles_x, int tiles_y, int ntiles,
+             int kchunk, const KScalars* sc) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  static_assert(sizeof(T) == 4, "ring sweep: 4-byte arithmetic types only");
+  constexpr int TYE = R * kWaves;
+  constexpr int TY = TYE - 2 * kHalo;
+  constexpr bool HK = KIND == PDEHIP_OP_FLUX;
+  constexpr int KR = HK ? TYE : 1;
+  constexpr int PW = kLanes + 2;   // padded LDS row
+  __shared__ T lu[3][TYE + 2][PW];
+  __shared__ T lk[HK ? 2 : 1][KR + 2][PW];
+  __shared__ T lfe[HK ? 3 : 1][KR + 2][PW];
+  __shared__ T lfn[HK ? 3 : 1][KR + 2][PW];
+  if (skip(sc)) return;
+  const int t = xcd_tile();
+  if (t >= ntiles) return;
+  const int tx = t % tiles_x;
+  const int rest = t / tiles_x;
+  const int ty = rest % tiles_y;
+  const int cz = rest / tiles_y;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wv = threadIdx.x / kLanes;
+  const int i = tx * kTileX - kHalo + lane;
+  const int k0 = cz * kchunk;
+  const int k1 = min(k0 + kchunk, op.nz);
+  const int nz = op.nz;
+  const int c2 = 1 - c1;
+  const long long sxy = op.sxy;
+  const int pbytes = static_cast<int>(sxy * static_cast<long long>(sizeof(S)));
+  const int kbytes = static_cast<int>(sxy * static_cast<long long>(sizeof(T)));
+  const T fx = T(op.fx), fy = T(op.fy), fz = T(op.fz);
+
+  const bool int_x = i >= 1 && i <= op.nx - 2;
+  const bool in_x = i >= 0 && i < op.nx;
+  const bool ring_x = lane >= 1 && lane <= kLanes - 2;
+  const bool tile_x = lane >= kHalo && lane < kLanes - kHalo;
+  unsigned m_r1 = 0, m_r2 = 0, m_fxy = 0;
+  unsigned off_s[R], off_k[R], off_st[R];   // byte offsets in a plane (loads clamped in-grid)
+  const int ci = min(max(i, 0), op.nx - 1);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int lr = wv * R + r;
+    const int j = ty * TY - kHalo + lr;
+    const bool in_y = j >= 0 && j < op.ny;
+    const bool int_y = j >= 1 && j <= op.ny - 2;
+    const bool ring_y = lr >= 1 && lr <= TYE - 2;
+    const bool tile_y = lr >= kHalo && lr < TYE - kHalo;
+    const unsigned col = static_cast<unsigned>(min(max(j, 0), op.ny - 1) * op.nx + ci);
+    off_s[r] = col * static_cast<unsigned>(sizeof(S));
+    off_k[r] = col * static_cast<unsigned>(sizeof(T));
+    off_st[r] = (tile_x && tile_y && in_x && in_y) ? off_s[r] : kDropOffset;
+    if (ring_x && ring_y && int_x && int_y) m_r1 |= 1u << r;
+    if (tile_x && tile_y && int_x && int_y) m_r2 |= 1u << r;
+    if (i == 1 || i == op.nx - 2 || j == 1 || j == op.ny - 2) m_fxy |= 1u << r;
+  }
+  const int jrow0 = ty * TY - kHalo + wv * R;
+  // parity of (i + j) for row 0 of this thread; row r adds r
+  const int par0 = (i + jrow0) & 1;
+
+  auto kcl = [&](int k) -> long long { return static_cast<long long>(min(max(k, 0), nz - 1)) * sxy; };
+  auto ld_u = [&](int r, int k) -> T {
+    if constexpr (ZERO) return T(0);
+    else return BufIO<S>::ld(plane_rsrc(xin_v, kcl(k), sizeof(S), pbytes), off_s[r]);
+  };
+  auto ld_b = [&](int r, int k) -> T {
+    return BufIO<S>::ld(plane_rsrc(bv, kcl(k), sizeof(S), pbytes), off_s[r]);
+  };
+  auto ld_k = [&](int r, int k) -> T {
+    if constexpr (HK) return BufIO<T>::ld(plane_rsrc(op.kappa, kcl(k), sizeof(T), kbytes), off_k[r]);
+    else return T(0);
+  };
+
+  // Rings, with plane pstart + s at slot s mod L: U (L 6) holds x(p-2 .. p+3), B (L 3)
+  // b(p-1 .. p+1), K (L 3) kappa(p+1 .. p+3), ZF (L 6) front faces zf(p-2 .. p+1).
+  const int pstart = HK ? k0 - 3 : k0 - 1;
+  T U[R][6], B[R][3], K[R][3], ZF[R][6];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    U[r][4] = ld_u(r, pstart - 2);
+    U[r][5] = ld_u(r, pstart - 1);
+    U[r][0] = ld_u(r, pstart);
+    U[r][1] = ld_u(r, pstart + 1);
+    U[r][2] = ld_u(r, pstart + 2);
+    U[r][3] = T(0);
+    B[r][2] = ld_b(r, pstart - 1);
+    B[r][0] = ld_b(r, pstart);
+    B[r][1] = T(0);
+    K[r][0] = T(0);
+    K[r][1] = ld_k(r, pstart + 1);
+    K[r][2] = ld_k(r, pstart + 2);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) ZF[r][s] = T(0);
+  }
+
+  auto step = [&](auto phc, int p) {
+    constexpr int PH = decltype(phc)::value;
+    constexpr int u_m2 = (PH + 4) % 6, u_m1 = (PH + 5) % 6, u_0 = PH % 6, u_1 = (PH + 1) % 6,
+                  u_3 = (PH + 3) % 6;
+    constexpr int b_m1 = (PH + 2) % 3, b_0 = PH % 3, b_1 = (PH + 1) % 3;
+    constexpr int k_1 = (PH + 1) % 3, k_2 = (PH + 2) % 3, k_3 = PH % 3;
+    constexpr int z_m2 = (PH + 4) % 6, z_m1 = (PH + 5) % 6, z_0 = PH % 6, z_1 = (PH + 1) % 6;
+    constexpr int su = PH % 3, sq = (PH + 2) % 3, sk = PH % 2;
+    constexpr int f_p1 = (PH + 1) % 3, f_0 = PH % 3, f_m1 = (PH + 2) % 3;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int lr = wv * R + r;
+      U[r][u_3] = ld_u(r, p + 3);
+      B[r][b_1] = ld_b(r, p + 1);
+      if constexpr (HK) {
+        K[r][k_3] = ld_k(r, p + 3);
+        lk[sk][lr + 1][lane + 1] = K[r][k_1];
+      }
+      lu[su][lr + 1][lane + 1] = U[r][u_0];
+    }
+    __syncthreads();
+    if constexpr (HK) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int lr = wv * R + r;
+        const T kc = K[r][k_1];
+        ZF[r][z_1] = face_smooth<T>(kc, K[r][k_2], fz);
+        const T fe = face_smooth<T>(kc, lk[sk][lr + 1][lane + 2], fx);
+        lfe[f_p1][lr + 1][lane + 1] = lane < kLanes - 1 ? fe : T(0);
+        lfn[f_p1][lr + 1][lane + 1] = lr < TYE - 1 ? face_smooth<T>(kc, lk[sk][lr + 2][lane + 1], fy) : T(0);
+      }
+    }
+    // first colour on plane p (tile + ring)
+    if (p >= k0 - 1 && p >= 1 && p <= nz - 2) {
+      const int cp = (par0 + p) & 1;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int lr = wv * R + r;
+        const int j = jrow0 + r;
+        const bool upd = ((m_r1 >> r) & 1u) && ((cp + r) & 1) == c1;
+        T aw = T(0), ae = T(0), as = T(0), an = T(0);
+        if constexpr (HK) {
+          aw = lfe[f_0][lr + 1][lane]; ae = lfe[f_0][lr + 1][lane + 1];
+          as = lfn[f_0][lr][lane + 1]; an = lfn[f_0][lr + 1][lane + 1];
+        }
+        W7<T> w = weights_at<3, KIND, T>(op, 0, aw, ae, as, an, ZF[r][z_m1], ZF[r][z_0]);
+        if (((m_fxy >> r) & 1u) || p == 1 || p == nz - 2) fold_center<3, T>(op, i, j, p, w);
+        const T v = rounded<S, T>(gs_value<3, T>(w, B[r][b_0], lu[su][lr + 1][lane + 2],
+                                                 lu[su][lr + 1][lane], lu[su][lr + 2][lane + 1],
+                                                 lu[su][lr][lane + 1], U[r][u_1], U[r][u_m1]));
+        U[r][u_0] = upd ? v : U[r][u_0];
+        lu[su][lr + 1][lane + 1] = U[r][u_0];
+      }
+    }
+    // second colour on plane q = p - 1 (tile), then the tile's plane q goes out
+    const int q = p - 1;
+    if (q >= k0 && q < k1) {
+      if (q >= 1 && q <= nz - 2) {
+        const int cq = (par0 + q) & 1;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int lr = wv * R + r;
+          const int j = jrow0 + r;
+          const bool upd = ((m_r2 >> r) & 1u) && ((cq + r) & 1) == c2;
+          T aw = T(0), ae = T(0), as = T(0), an = T(0);
+          if constexpr (HK) {
+            aw = lfe[f_m1][lr + 1][lane]; ae = lfe[f_m1][lr + 1][lane + 1];
+            as = lfn[f_m1][lr][lane + 1]; an = lfn[f_m1][lr + 1][lane + 1];
+          }
+          W7<T> w = weights_at<3, KIND, T>(op, 0, aw, ae, as, an, ZF[r][z_m2], ZF[r][z_m1]);
+          if (((m_fxy >> r) & 1u) || q == 1 || q == nz - 2) fold_center<3, T>(op, i, j, q, w);
+          const T v = rounded<S, T>(gs_value<3, T>(w, B[r][b_m1], lu[sq][lr + 1][lane + 2],
+                                                   lu[sq][lr + 1][lane], lu[sq][lr + 2][lane + 1],
+                                                   lu[sq][lr][lane + 1], U[r][u_0], U[r][u_m2]));
+          U[r][u_m1] = upd ? v : U[r][u_m1];
+        }
+      }
+      const __amdgpu_buffer_rsrc_t ro = plane_rsrc(xout_v, static_cast<long long>(q) * sxy,
+                                                   sizeof(S), pbytes);
+#pragma unroll
+      for (int r = 0; r < R; ++r) BufIO<S>::st(ro, off_st[r], to_store<S, T>(U[r][u_m1]));
+    }
+  };
+
+  const int steps = k1 - pstart + 1;
+  for (int s0 = 0; s0 < steps; s0 += kRing) {
+    const int p = pstart + s0;
+    step(IC<0>{}, p);
+    step(IC<1>{}, p + 1);
+    step(IC<2>{}, p + 2);
+    step(IC<3>{}, p + 3);
+    step(IC<4>{}, p + 4);
+    step(IC<5>{}, p + 5);
+  }
+}
+
 // ------------------------------------------------------------------------------- 2D
 // One wavefront per 60-column strip; rows are the marching axis, x neighbours come from the
 // adjacent lanes (ds_bpermute), y neighbours from the thread's row queue.  No LDS, no barrier.
@@ -654,6 +897,47 @@ void launch3(const Launch& L, const DevOp& op, const void* b, const void* xin, v
 #undef PDEHIP_F3
 }
 
+// z chunk for the ring sweep: the step count of a chunk (kchunk + prologue + 1) is a multiple
+// of kRing, chosen like plane_chunk (whole rounds of resident workgroups) with the padded step
+// count as the cost of a chunk.
+int ring_chunk(long long tiles, int nz, int capacity, int prologue) {
+  int best_kc = nz;
+  long long best = -1;
+  for (int kc = 4; kc <= nz + kRing; ++kc) {
+    if ((kc + prologue + 1) % kRing != 0) continue;
+    const int chunks = (nz + kc - 1) / kc;
+    const long long n = tiles * chunks;
+    const long long rounds = (n + capacity - 1) / capacity;
+    const long long cost = rounds * (kc + prologue + 1);
+    if (best < 0 || cost < best) { best = cost; best_kc = kc; }
+    if (chunks == 1) break;
+  }
+  return best_kc;
+}
+
+template <int KIND, Prec P>
+void launch3_ring(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout,
+                  int c1, bool zero_in) {
+  constexpr int rows = 2;
+  const int ty = rows * kWaves - 2 * kHalo;
+  const int tiles_x = (op.nx + kTileX - 1) / kTileX;
+  const int tiles_y = (op.ny + ty - 1) / ty;
+  static const int occ = blocks_per_cu(k_rbgs_ring3<KIND, P, rows, false>, kThreads);
+  const int prologue = KIND == PDEHIP_OP_FLUX ? 3 : 1;
+  const int kchunk = g_tune.zchunk > 0 ? g_tune.zchunk
+                                       : ring_chunk((long long)tiles_x * tiles_y, op.nz,
+                                                    device_cus() * occ, prologue);
+  const int chunks = (op.nz + kchunk - 1) / kchunk;
+  const int ntiles = tiles_x * tiles_y * chunks;
+  const dim3 g(static_cast<unsigned>(8 * ((ntiles + 7) / 8)));
+  if (zero_in)
+    hipLaunchKernelGGL((k_rbgs_ring3<KIND, P, rows, true>), g, dim3(kThreads), 0, L.stream, op, b,
+                       xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc);
+  else
+    hipLaunchKernelGGL((k_rbgs_ring3<KIND, P, rows, false>), g, dim3(kThreads), 0, L.stream, op, b,
+                       xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc);
+}
+
 template <int KIND, Prec P>
 void launch3_pair(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout,
                   int c1, bool zero_in) {
@@ -701,14 +985,18 @@ void launch_kind(const Launch& L, const DevOp& op, Prec p, const void* b, const 
                  void* xout, int c1, bool zero_in) {
   if (op.nz > 1) {
     const bool pair = g_tune.pair != 0;
+    // ring sweep: buffer offsets of a plane must stay below 2^31 bytes
+    const bool ring = g_tune.ring != 0 && g_tune.rows == 2 && op.sxy < (1LL << 29);
     switch (p) {
       case P64: launch3<KIND, P64>(L, op, b, xin, xout, c1, zero_in); break;
       case P32:
         if (pair) launch3_pair<KIND, P32>(L, op, b, xin, xout, c1, zero_in);
+        else if (ring) launch3_ring<KIND, P32>(L, op, b, xin, xout, c1, zero_in);
         else launch3<KIND, P32>(L, op, b, xin, xout, c1, zero_in);
         break;
       default:
         if (pair) launch3_pair<KIND, P16>(L, op, b, xin, xout, c1, zero_in);
+        else if (ring) launch3_ring<KIND, P16>(L, op, b, xin, xout, c1, zero_in);
         else launch3<KIND, P16>(L, op, b, xin, xout, c1, zero_in);
         break;
     }

@@ -51,6 +51,7 @@ struct SmoothTuning {
   int zchunk = 0;    // 3D: z planes per workgroup (0: chosen from occupancy and grid size)
   int ychunk = 64;   // 2D: rows per wavefront (upper bound)
   int pair = 0;      // 3D FP32/FP16: column-pair (divergence-free) tiles
+  int ring = 1;      // 3D FP32/FP16: ring-buffered, 6-step unrolled sweep (k_rbgs_ring3)
 };
 void set_smooth_tuning(const SmoothTuning& t);
 SmoothTuning smooth_tuning();

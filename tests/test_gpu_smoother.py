@@ -11,7 +11,7 @@ from pde_solver_amd.capi import BC
 pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"smooth.fused": 1, "smooth.rows": 2, "smooth.zchunk": 0, "smooth.ychunk": 64,
-            "smooth.pair": 0}
+            "smooth.pair": 0, "smooth.ring": 1}
 
 
 @pytest.fixture(scope="module")
@@ -47,6 +47,8 @@ def const_problem(shape, seed):
 
 def solve(ctx, s, precision, fused, **tune):
     from pde_solver_amd import hip
+    for k, v in DEFAULTS.items():
+        hip.tune_set(k, v)
     hip.tune_set("smooth.fused", fused)
     for k, v in tune.items():
         hip.tune_set("smooth." + k, v)
@@ -65,6 +67,13 @@ CASES_3D = [
     ("flux", (33, 65, 129), {"rows": 2, "zchunk": 0, "pair": 1}),
     ("flux", (65, 33, 257), {"rows": 2, "zchunk": 8, "pair": 1}),
     ("const", (17, 129, 65), {"rows": 2, "zchunk": 0, "pair": 1}),
+    # FP32/FP16 levels at rows 2 run the ring sweep (k_rbgs_ring3) unless ring = 0: automatic
+    # chunks, a chunk whose step count is not a multiple of the 6-step unroll (dead steps), and
+    # the plane-march sweep it replaces
+    ("flux", (33, 65, 129), {"rows": 2, "zchunk": 0, "ring": 1}),
+    ("flux", (65, 33, 257), {"rows": 2, "zchunk": 9, "ring": 1}),
+    ("const", (17, 129, 65), {"rows": 2, "zchunk": 5, "ring": 1}),
+    ("flux", (33, 65, 129), {"rows": 2, "zchunk": 0, "ring": 0}),
 ]
 
 
