@@ -707,7 +707,7 @@ def main() -> int:
             traffic = json.load(open(tpath)).get("rbgs_sweep_fp32_bytes_per_launch")
         except Exception:
             traffic = None
-    roof = {"bound": "hbm", "kernel": f"k_rbgs_fused3 (red+black sweep, one launch), FP32, fine level {n}^3",
+    roof = {"bound": "hbm", "kernel": f"k_rbgs_ring3 (red+black sweep, one launch), FP32, fine level {n}^3",
             "achieved": round(gbs(bytes_mx, ms_mx), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs(bytes_mx, ms_mx) / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes": bytes_mx, "avg_ms": round(ms_mx, 4)}

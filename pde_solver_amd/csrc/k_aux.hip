@@ -190,6 +190,140 @@ __global__ void __launch_bounds__(kTX * kTY) k_restrict(DevOp f, const void* rfv
   }
 }
 
+// 3D restriction, one coarse node per thread (no z march, so no load waits on a carried
+// value): the 27-point full weighting of an interior coarse node reads fine nodes that are all
+// in range, so its 27 loads are issued unconditionally and wait once.  Coarse nodes next to a
+// face add the folded closure terms through the general path (fold lists, clipped weighting).
+template <typename SF>
+__device__ __forceinline__ double fw2_in(const SF* __restrict__ pl, int nx, int fi, int fj) {
+  const SF* r0 = pl + (long long)(fj - 1) * nx + fi;
+  const SF* r1 = r0 + nx;
+  const SF* r2 = r1 + nx;
+  const double a0 = 0.5 * ldd(r0, 0) + 0.25 * ldd(r0, -1) + 0.25 * ldd(r0, 1);
+  const double a1 = 0.5 * ldd(r1, 0) + 0.25 * ldd(r1, -1) + 0.25 * ldd(r1, 1);
+  const double a2 = 0.5 * ldd(r2, 0) + 0.25 * ldd(r2, -1) + 0.25 * ldd(r2, 1);
+  return 0.25 * a0 + 0.5 * a1 + 0.25 * a2;
+}
+
+template <Prec PF, Prec PC>
+__global__ void __launch_bounds__(kTX * kTY) k_restrict3(DevOp f, const void* rfv, DevOp c,
+                                                        void* bcv, double scale,
+                                                        const double* dscale,
+                                                        const KScalars* sc) {
+  using SF = typename PT<PF>::S;
+  using SC = typename PT<PC>::S;
+  if (skip(sc)) return;
+  const int ic = blockIdx.x * kTX + threadIdx.x;
+  const int jc = blockIdx.y * kTY + threadIdx.y;
+  const int kc = blockIdx.z;
+  if (ic >= c.nx || jc >= c.ny) return;
+  SC* bc = static_cast<SC*>(bcv);
+  const long long out = ((long long)kc * c.ny + jc) * c.nx + ic;
+  if (ic == 0 || jc == 0 || kc == 0 || ic == c.nx - 1 || jc == c.ny - 1 || kc == c.nz - 1) {
+    bc[out] = store_as<SC>(0.0);
+    return;
+  }
+  const SF* rf = static_cast<const SF*>(rfv);
+  const long long fsxy = (long long)f.nx * f.ny;
+  const int fi = 2 * ic, fj = 2 * jc, fk = 2 * kc;
+  double ql = fw2_in(rf + (fk - 1) * fsxy, f.nx, fi, fj);
+  double qm = fw2_in(rf + fk * fsxy, f.nx, fi, fj);
+  double qh = fw2_in(rf + (fk + 1) * fsxy, f.nx, fi, fj);
+  const bool fold_xy = ic == 1 || jc == 1 || ic == c.nx - 2 || jc == c.ny - 2;
+  int xi[3], yi[3];
+  double xw[3], yw[3];
+  int mx = 1, my = 1;
+  if (fold_xy) {
+    mx = fold_list(ic, c.nx, c.theta[0], c.theta[1], xi, xw);
+    my = fold_list(jc, c.ny, c.theta[2], c.theta[3], yi, yw);
+  }
+  // in-plane folded weighting of fine plane fk (the general form: every fold-list pair)
+  auto Q = [&](int k) -> double {
+    const SF* pl = rf + k * fsxy;
+    double acc = 0.0;
+    for (int b = 0; b < my; ++b)
+      for (int q = 0; q < mx; ++q) {
+        const double w = yw[b] * xw[q];
+        if (w != 0.0) acc += w * fw2(f, pl, 2 * xi[q], 2 * yi[b]);
+      }
+    return acc;
+  };
+  if (fold_xy) { ql = Q(fk - 1); qm = Q(fk); qh = Q(fk + 1); }
+  double v = 0.25 * ql + 0.5 * qm + 0.25 * qh;
+  if (kc == 1 && c.theta[4] != 0.0)
+    v += c.theta[4] * (0.5 * (fold_xy ? Q(0) : fw2_in(rf, f.nx, fi, fj)) + 0.25 * ql);
+  if (kc == c.nz - 2 && c.theta[5] != 0.0)
+    v += c.theta[5] * (0.25 * qh + 0.5 * (fold_xy ? Q(fk + 2) : fw2_in(rf + (fk + 2) * fsxy, f.nx, fi, fj)));
+  const double s = dscale ? scale * (*dscale) : scale;
+  bc[out] = store_as<SC>(v * s);
+}
+
+// 3D prolongation, one coarse interval per thread: fine planes 2kc and 2kc+1 of one (i, j)
+// column (the last interior plane alone when nz is even), both fine loads and the coarse
+// loads of planes kc and kc+1 issued before any arithmetic.
+template <Prec PC, Prec PF>
+__global__ void __launch_bounds__(kTX * kTY) k_prolong3(DevOp c, const void* ucv, DevOp f,
+                                                       void* ufv, double factor,
+                                                       const double* dscale,
+                                                       const KScalars* sc) {
+  using SC = typename PT<PC>::S;
+  using SF = typename PT<PF>::S;
+  if (skip(sc)) return;
+  const int i = blockIdx.x * kTX + threadIdx.x;
+  const int j = blockIdx.y * kTY + threadIdx.y;
+  const int kc = blockIdx.z;                 // fine planes 2kc, 2kc + 1 (interior ones)
+  if (i < 1 || j < 1 || i > f.nx - 2 || j > f.ny - 2) return;
+  const int ka = max(1, 2 * kc), kb = min(2 * kc + 2, f.nz - 1);   // [ka, kb)
+  if (ka >= kb) return;
+  const SC* uc = static_cast<const SC*>(ucv);
+  SF* uf = static_cast<SF*>(ufv);
+  const long long csxy = (long long)c.nx * c.ny;
+  const long long fsxy = (long long)f.nx * f.ny;
+  const int ic = i >> 1, oi = i & 1, jc = j >> 1, oj = j & 1;
+  int x0 = ic, x1 = ic + 1, y0 = jc, y1 = jc + 1;
+  double wx0 = 1.0, wx1 = 1.0, wy0 = 1.0, wy1 = 1.0;
+  if (x0 == 0) { wx0 = c.theta[0]; x0 = 1; }
+  if (x1 == c.nx - 1) { wx1 = c.theta[1]; x1 = c.nx - 2; }
+  if (y0 == 0) { wy0 = c.theta[2]; y0 = 1; }
+  if (y1 == c.ny - 1) { wy1 = c.theta[3]; y1 = c.ny - 2; }
+  const double hx = oi ? 0.5 : 1.0, hy = oj ? 0.5 : 1.0;
+  if (!oi) wx1 = 0.0;
+  if (!oj) wy1 = 0.0;
+  x1 = min(x1, c.nx - 2);   // in range even where its weight is zero
+  y1 = min(y1, c.ny - 2);
+  const long long id0 = ka * fsxy + (long long)j * f.nx + i;
+  const bool two = kb - ka == 2;
+  const double f0 = ldd(uf, id0);
+  const double f1 = two ? ldd(uf, id0 + fsxy) : 0.0;
+  // bilinear value of coarse plane kk through the z closure extension
+  auto P = [&](int kk) -> double {
+    double wz = 1.0;
+    if (kk == 0) { wz = c.theta[4]; kk = 1; } else if (kk == c.nz - 1) { wz = c.theta[5]; kk = c.nz - 2; }
+    const SC* pl = uc + kk * csxy;
+    const double l00 = ldd(pl + (long long)y0 * c.nx, x0), l01 = ldd(pl + (long long)y0 * c.nx, x1);
+    const double l10 = ldd(pl + (long long)y1 * c.nx, x0), l11 = ldd(pl + (long long)y1 * c.nx, x1);
+    double a = 0.0;
+    if (wy0 != 0.0) {
+      double v = wx0 != 0.0 ? wx0 * l00 : 0.0;
+      if (wx1 != 0.0) v += wx1 * l01;
+      a += wy0 * (v * hx);
+    }
+    if (wy1 != 0.0) {
+      double v = wx0 != 0.0 ? wx0 * l10 : 0.0;
+      if (wx1 != 0.0) v += wx1 * l11;
+      a += wy1 * (v * hx);
+    }
+    return wz == 0.0 ? 0.0 : wz * (a * hy);
+  };
+  const double s = dscale ? factor * (*dscale) : factor;
+  // fine plane k = 2kc + o sits at coarse kc (o = 0) or halfway to kc + 1 (o = 1)
+  const double p_lo = P(kc), p_hi = P(kc + 1);
+  const int o0 = ka & 1;
+  const double v0 = o0 ? 0.5 * (p_lo + p_hi) : p_lo;
+  uf[id0] = store_as<SF>(f0 + s * v0);
+  if (two) uf[id0 + fsxy] = store_as<SF>(f1 + s * (0.5 * (p_lo + p_hi)));
+}
+
 // fine interior += factor * trilinear interpolation of the coarse correction, whose face nodes
 // are taken as the homogeneous closure extension: theta times the inward neighbour per axis
 // (edges and corners the product), which is what extending x, then y, then z faces composes
@@ -500,6 +634,19 @@ template void closures_dirichlet<float>(const Launch&, const DevOp&, const FaceD
 void restrict_fw(const Launch& L, const DevOp& fine, Prec pf, const void* rf, const DevOp& coarse,
                  Prec pc, void* bc, double scale, const double* dev_scale) {
   const int gx = (coarse.nx + kTX - 1) / kTX, gy = (coarse.ny + kTY - 1) / kTY;
+  if (coarse.nz > 1) {
+    const dim3 g3(gx, gy, coarse.nz);
+    const dim3 b3(kTX, kTY);
+#define R3_(A, B) hipLaunchKernelGGL((k_restrict3<A, B>), g3, b3, 0, L.stream, fine, rf, coarse, bc, scale, dev_scale, L.sc)
+    if (pf == P64 && pc == P64) R3_(P64, P64);
+    else if (pf == P32 && pc == P32) R3_(P32, P32);
+    else if (pf == P32 && pc == P16) R3_(P32, P16);
+    else if (pf == P16 && pc == P16) R3_(P16, P16);
+    else if (pf == P64 && pc == P32) R3_(P64, P32);
+    else R3_(P16, P32);
+#undef R3_
+    return;
+  }
   const int zc = coarse.nz > 1 ? zchunk_for(gx, gy, coarse.nz) : 1;
   const dim3 g(gx, gy, coarse.nz > 1 ? (coarse.nz + zc - 1) / zc : 1);
   const dim3 blk(kTX, kTY);
@@ -516,6 +663,19 @@ void restrict_fw(const Launch& L, const DevOp& fine, Prec pf, const void* rf, co
 void prolong_add(const Launch& L, const DevOp& coarse, Prec pc, const void* uc, const DevOp& fine,
                  Prec pf, void* uf, double factor, const double* dev_scale) {
   const int gx = (fine.nx + kTX - 1) / kTX, gy = (fine.ny + kTY - 1) / kTY;
+  if (fine.nz > 1) {
+    const dim3 g3(gx, gy, (fine.nz + 1) / 2);
+    const dim3 b3(kTX, kTY);
+#define P3_(A, B) hipLaunchKernelGGL((k_prolong3<A, B>), g3, b3, 0, L.stream, coarse, uc, fine, uf, factor, dev_scale, L.sc)
+    if (pc == P64 && pf == P64) P3_(P64, P64);
+    else if (pc == P32 && pf == P32) P3_(P32, P32);
+    else if (pc == P16 && pf == P32) P3_(P16, P32);
+    else if (pc == P16 && pf == P16) P3_(P16, P16);
+    else if (pc == P32 && pf == P64) P3_(P32, P64);
+    else P3_(P32, P16);
+#undef P3_
+    return;
+  }
   const int zc = fine.nz > 1 ? zchunk_for(gx, gy, fine.nz) : 1;
   const dim3 g(gx, gy, fine.nz > 1 ? (fine.nz + zc - 1) / zc : 1);
   const dim3 blk(kTX, kTY);

@@ -903,7 +903,7 @@ void launch3(const Launch& L, const DevOp& op, const void* b, const void* xin, v
 int ring_chunk(long long tiles, int nz, int capacity, int prologue) {
   int best_kc = nz;
   long long best = -1;
-  for (int kc = 4; kc <= nz + kRing; ++kc) {
+  for (int kc = 2; kc <= nz + kRing; ++kc) {
     if ((kc + prologue + 1) % kRing != 0) continue;
     const int chunks = (nz + kc - 1) / kc;
     const long long n = tiles * chunks;
