@@ -232,7 +232,9 @@ void pdehip_plan_destroy(pdehip_plan* plan);
  * planes per 3D workgroup, 4..512, or 0 to choose from occupancy), "smooth.ychunk" (2D rows
  * per wavefront, 16..4096), "stencil.plane" (0/1: 3D apply and residual as plane-march
  * kernels), "smooth.pair" (0/1: 3D FP32/FP16 sweeps with column-pair tiles, no colour
- * divergence).  Results do not depend on any of these.
+ * divergence), "smooth.ring" (0/1, default 1: 3D FP32/FP16 sweeps by the ring-buffered,
+ * 6-step unrolled kernel), "stencil.ring" (0/1, default 1: the same for the 3D FP32/FP16
+ * apply and residual).  Results do not depend on any of these.
  * "mixed.fp16" (0/1, default 1): mixed-precision plans created afterwards store their coarse
  * multigrid levels in FP16 (1) or FP32 (0); this one changes the preconditioner's rounding.
  * Not thread-safe against running solves. */

@@ -12,6 +12,9 @@
 // after it was read, and a thread can only be there once every thread has passed the barrier
 // in between.  The FP64 operator divides exactly (face_coef); the FP32/FP16 operators keep
 // rho = 1/kappa and form each face with one hardware reciprocal, within their own rounding.
+#include <type_traits>
+
+#include "bufio.hpp"
 #include "kernels.hpp"
 
 namespace pdehip {
@@ -203,6 +206,193 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
   }
 }
 
+// The same plane march for 4-byte arithmetic (FP32/FP16 storage, CONST and FLUX), with the
+// bookkeeping taken out of the plane step: the loop is unrolled kRingP = 6 times and every
+// register queue is a ring indexed by the step's phase at compile time (x: 6 planes, loads
+// three steps ahead; b, kappa, front faces: 3), LDS slots are compile-time, LDS planes are
+// padded by one row / column on every side (neighbour reads at immediate offsets, no clamps),
+// global traffic goes through per-plane buffer descriptors, and stores of lanes outside the
+// output tile are discarded by the descriptor's range check.  Same tile, same arithmetic in
+// the same order as k_plane3, so the outputs and the partials are the same bits.  Steps past
+// the chunk compute and discard.
+constexpr int kRingP = 6;
+
+template <int KIND, Prec P, int MODE, int R>
+__global__ void __launch_bounds__(kThreads)
+k_plane3r(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, void* __restrict__ yv,
+          double* partials, int tiles_x, int tiles_y, int ntiles, int kchunk, const KScalars* sc) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  static_assert(sizeof(T) == 4, "ring plane march: 4-byte arithmetic types only");
+  constexpr int TYE = R * kWaves;
+  constexpr int TY = TYE - 2;
+  constexpr int TX = kLanes - 2;
+  constexpr bool HK = KIND == PDEHIP_OP_FLUX;
+  constexpr int KR = HK ? TYE : 1;
+  constexpr int PW = kLanes + 2;
+  __shared__ T lx[2][TYE + 2][PW];
+  __shared__ T lk[HK ? 2 : 1][KR + 2][PW];
+  __shared__ T lfe[HK ? 2 : 1][KR + 2][PW];
+  __shared__ T lfn[HK ? 2 : 1][KR + 2][PW];
+  __shared__ double lred[kWaves];
+  if (skip(sc)) return;
+  const int per = static_cast<int>(gridDim.x >> 3);
+  const int t = static_cast<int>(blockIdx.x & 7) * per + static_cast<int>(blockIdx.x >> 3);
+  if (t >= ntiles) {
+    if (partials && threadIdx.x == 0) partials[blockIdx.x] = 0.0;
+    return;
+  }
+  const int tx = t % tiles_x;
+  const int rest = t / tiles_x;
+  const int ty = rest % tiles_y;
+  const int cz = rest / tiles_y;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wv = threadIdx.x / kLanes;
+  const int i = tx * TX - 1 + lane;
+  const int k0 = cz * kchunk;
+  const int k1 = min(k0 + kchunk, op.nz);
+  const int nz = op.nz;
+  const long long sxy = op.sxy;
+  const int pbytes = static_cast<int>(sxy * static_cast<long long>(sizeof(S)));
+  const int kbytes = static_cast<int>(sxy * static_cast<long long>(sizeof(T)));
+  const T fx = T(op.fx), fy = T(op.fy), fz = T(op.fz);
+  const bool in_x = i >= 0 && i < op.nx;
+  const bool bnd_x = i == 0 || i == op.nx - 1;
+  const bool tile_x = lane >= 1 && lane <= kLanes - 2;
+
+  unsigned m_out = 0, m_bxy = 0, m_fxy = 0;
+  unsigned off_s[R], off_k[R], off_st[R];
+  const int ci = min(max(i, 0), op.nx - 1);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int lr = wv * R + r;
+    const int j = ty * TY - 1 + lr;
+    const bool in_y = j >= 0 && j < op.ny;
+    const unsigned col = static_cast<unsigned>(min(max(j, 0), op.ny - 1) * op.nx + ci);
+    off_s[r] = col * static_cast<unsigned>(sizeof(S));
+    off_k[r] = col * static_cast<unsigned>(sizeof(T));
+    const bool out = in_x && in_y && tile_x && lr >= 1 && lr <= TYE - 2;
+    off_st[r] = out ? off_s[r] : kDropOffset;
+    if (out) m_out |= 1u << r;
+    if (bnd_x || j == 0 || j == op.ny - 1) m_bxy |= 1u << r;
+    if (i == 1 || i == op.nx - 2 || j == 1 || j == op.ny - 2) m_fxy |= 1u << r;
+  }
+  const int jrow0 = ty * TY - 1 + wv * R;
+
+  auto kcl = [&](int k) -> long long { return static_cast<long long>(min(max(k, 0), nz - 1)) * sxy; };
+  auto ld_x = [&](int r, int k) -> T { return BufIO<S>::ld(plane_rsrc(xv, kcl(k), sizeof(S), pbytes), off_s[r]); };
+  auto ld_k = [&](int r, int k) -> T {
+    if constexpr (HK) return BufIO<T>::ld(plane_rsrc(op.kappa, kcl(k), sizeof(T), kbytes), off_k[r]);
+    else return T(0);
+  };
+  auto ld_b = [&](int r, int k) -> T {
+    if constexpr (MODE == 1) return BufIO<S>::ld(plane_rsrc(bv, kcl(k), sizeof(S), pbytes), off_s[r]);
+    else return T(0);
+  };
+
+  // Rings, plane pstart + s at slot s mod L: X (6) x(p-1 .. p+4), B (3) b(p .. p+2),
+  // K (3) kappa(p+1 .. p+3), ZF (3) zf(p-1 .. p+1) with zf(k) = face(kappa(k), kappa(k+1)).
+  const int pstart = HK ? k0 - 2 : k0;
+  T X[R][6], B[R][3], K[R][3], ZF[R][3];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    X[r][5] = ld_x(r, pstart - 1);
+    X[r][0] = ld_x(r, pstart);
+    X[r][1] = ld_x(r, pstart + 1);
+    X[r][2] = ld_x(r, pstart + 2);
+    X[r][3] = ld_x(r, pstart + 3);
+    X[r][4] = T(0);
+    B[r][0] = ld_b(r, pstart);
+    B[r][1] = ld_b(r, pstart + 1);
+    B[r][2] = T(0);
+    K[r][1] = ld_k(r, pstart + 1);
+    K[r][2] = ld_k(r, pstart + 2);
+    K[r][0] = T(0);
+    ZF[r][0] = ZF[r][1] = ZF[r][2] = T(0);
+  }
+  double acc = 0.0;
+
+  auto step = [&](auto phc, int p) {
+    constexpr int PH = decltype(phc)::value;
+    constexpr int x_m1 = (PH + 5) % 6, x_0 = PH % 6, x_1 = (PH + 1) % 6, x_4 = (PH + 4) % 6;
+    constexpr int b_0 = PH % 3, b_2 = (PH + 2) % 3;
+    constexpr int k_1 = (PH + 1) % 3, k_2 = (PH + 2) % 3, k_3 = PH % 3;
+    constexpr int z_m1 = (PH + 2) % 3, z_0 = PH % 3, z_1 = (PH + 1) % 3;
+    constexpr int sl = PH & 1, fs = (PH + 1) & 1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int lr = wv * R + r;
+      X[r][x_4] = ld_x(r, p + 4);
+      B[r][b_2] = ld_b(r, p + 2);
+      if constexpr (HK) {
+        K[r][k_3] = ld_k(r, p + 3);
+        lk[sl][lr + 1][lane + 1] = K[r][k_1];
+      }
+      lx[sl][lr + 1][lane + 1] = X[r][x_0];
+    }
+    __syncthreads();
+    if constexpr (HK) {
+      // faces of plane p+1 into slot (p+1) & 1
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int lr = wv * R + r;
+        const T kc = K[r][k_1];
+        ZF[r][z_1] = face_op<P, T>(kc, K[r][k_2], fz);
+        const T fe = face_op<P, T>(kc, lk[sl][lr + 1][lane + 2], fx);
+        lfe[fs][lr + 1][lane + 1] = lane < kLanes - 1 ? fe : T(0);
+        lfn[fs][lr + 1][lane + 1] = lr < TYE - 1 ? face_op<P, T>(kc, lk[sl][lr + 2][lane + 1], fy) : T(0);
+      }
+    }
+    if (p >= k0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int lr = wv * R + r;
+        const int j = jrow0 + r;
+        const T xc = X[r][x_0];
+        W7<T> w;
+        if constexpr (KIND == PDEHIP_OP_CONST) {
+          w.C = T(op.w[0]); w.W = T(op.w[1]); w.E = T(op.w[2]); w.S = T(op.w[3]);
+          w.N = T(op.w[4]); w.B = T(op.w[5]); w.F = T(op.w[6]);
+        } else {
+          w.W = lfe[sl][lr + 1][lane]; w.E = lfe[sl][lr + 1][lane + 1];
+          w.S = lfn[sl][lr][lane + 1]; w.N = lfn[sl][lr + 1][lane + 1];
+          T sum = w.W + w.E + w.S + w.N;
+          w.B = ZF[r][z_m1]; w.F = ZF[r][z_0];
+          sum += w.B + w.F;
+          w.C = T(op.e) - sum;
+        }
+        if (((m_fxy >> r) & 1u) || p == 1 || p == nz - 2) fold_center<3, T>(op, i, j, p, w);
+        const T ax = w.C * xc + w.E * lx[sl][lr + 1][lane + 2] + w.W * lx[sl][lr + 1][lane] +
+                     w.N * lx[sl][lr + 2][lane + 1] + w.S * lx[sl][lr][lane + 1] + w.F * X[r][x_1] +
+                     w.B * X[r][x_m1];
+        const bool bnd = ((m_bxy >> r) & 1u) || p == 0 || p == nz - 1;
+        const T ain = bnd ? xc : ax;
+        const T out = MODE == 0 ? ain : T(B[r][b_0] - ain);
+        const S so = store_as<S>(static_cast<double>(out));
+        BufIO<S>::st(plane_rsrc(yv, static_cast<long long>(p) * sxy, sizeof(S), pbytes), off_st[r], so);
+        if (partials && ((m_out >> r) & 1u))
+          acc += MODE == 0 ? static_cast<double>(xc) * static_cast<double>(out)
+                           : static_cast<double>(out) * static_cast<double>(out);
+      }
+    }
+  };
+
+  const int steps = k1 - pstart;
+  for (int s0 = 0; s0 < steps; s0 += kRingP) {
+    const int p = pstart + s0;
+    step(std::integral_constant<int, 0>{}, p);
+    if (s0 + 1 < steps) step(std::integral_constant<int, 1>{}, p + 1);
+    if (s0 + 2 < steps) step(std::integral_constant<int, 2>{}, p + 2);
+    if (s0 + 3 < steps) step(std::integral_constant<int, 3>{}, p + 3);
+    if (s0 + 4 < steps) step(std::integral_constant<int, 4>{}, p + 4);
+    if (s0 + 5 < steps) step(std::integral_constant<int, 5>{}, p + 5);
+  }
+  if (partials) {
+    const double s = block_sum(acc, lred);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  }
+}
+
 struct Shape { int tiles_x, tiles_y, chunks, ntiles, kchunk, blocks; };
 
 // Launch shape shared by every instantiation (the partials count must not depend on the
@@ -224,10 +414,21 @@ Shape shape_of(const DevOp& op) {
   return s;
 }
 
+bool g_plane_ring = true;
+
 template <int KIND, Prec P, int MODE>
 void launch(const Launch& L, const DevOp& op, const void* b, const void* x, void* y,
             double* partials) {
   const Shape s = shape_of(op);
+  // ring variant: 4-byte arithmetic, buffer offsets of a plane below 2^31 bytes
+  if constexpr (sizeof(typename PT<P>::T) == 4 && KIND != PDEHIP_OP_POINTWISE) {
+    if (g_plane_ring && op.sxy < (1LL << 29)) {
+      hipLaunchKernelGGL((k_plane3r<KIND, P, MODE, 2>), dim3(s.blocks), dim3(kThreads), 0,
+                         L.stream, op, b, x, y, partials, s.tiles_x, s.tiles_y, s.ntiles,
+                         s.kchunk, L.sc);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_plane3<KIND, P, MODE, 2>), dim3(s.blocks), dim3(kThreads), 0, L.stream,
                      op, b, x, y, partials, s.tiles_x, s.tiles_y, s.ntiles, s.kchunk, L.sc);
 }
@@ -254,6 +455,9 @@ void launch_mode(const Launch& L, const DevOp& op, Prec p, const void* b, const 
 }  // namespace
 
 int plane_blocks(const DevOp& op) { return shape_of(op).blocks; }
+
+void set_plane_ring(bool on) { g_plane_ring = on; }
+bool plane_ring() { return g_plane_ring; }
 
 void apply_plane(const Launch& L, const DevOp& op, Prec p, const void* x, void* y, double* partials) {
   launch_mode<0>(L, op, p, nullptr, x, y, partials);

@@ -23,6 +23,7 @@
 // contiguous run of tiles, which keeps ring re-reads inside one XCD's L2.
 #include <type_traits>
 
+#include "bufio.hpp"
 #include "kernels.hpp"
 
 namespace pdehip {
@@ -316,35 +317,6 @@ k_rbgs_fused3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xi
 // (the loop runs a multiple of kRing steps) compute and discard; the host picks z chunks whose
 // step count is a multiple of kRing where it can.
 constexpr int kRing = 6;
-constexpr unsigned kBufRsrcWord3 = 0x00020000u;   // gfx9 raw buffer: 32-bit, no swizzle
-constexpr unsigned kDropOffset = 0x80000000u;     // past any plane's range: store discarded
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, long long elem,
-                                                             int esize, int bytes) {
-  char* p = const_cast<char*>(static_cast<const char*>(base)) + elem * esize;
-  return __builtin_amdgcn_make_buffer_rsrc(p, static_cast<short>(0), bytes,
-                                           static_cast<int>(kBufRsrcWord3));
-}
-
-template <typename S> struct BufIO;
-template <> struct BufIO<float> {
-  static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, unsigned byte_off) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
-  }
-  static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, unsigned byte_off, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, byte_off, 0, 0);
-  }
-};
-template <> struct BufIO<__half> {
-  static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, unsigned byte_off) {
-    const unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(r, byte_off, 0, 0);
-    return __half2float(__ushort_as_half(h));
-  }
-  static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, unsigned byte_off, __half v) {
-    __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(v), r, byte_off, 0, 0);
-  }
-};
-
 template <int N> using IC = std::integral_constant<int, N>;
 
 template <int KIND, Prec P, int R, bool ZERO>

@@ -67,6 +67,9 @@ int plane_blocks(const DevOp& op);
 int partial_blocks_max(const DevOp& op);
 void set_plane_stencils(bool on);
 bool plane_stencils();
+// 3D FP32/FP16 plane march: the ring-buffered variant (k_plane3r) or k_plane3.
+void set_plane_ring(bool on);
+bool plane_ring();
 
 // Reference-semantics relaxation (FP64): interior update (-(f + N(u)) - sum w u_nb)/C.
 struct NonlinearTerms {

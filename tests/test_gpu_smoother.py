@@ -11,7 +11,7 @@ from pde_solver_amd.capi import BC
 pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"smooth.fused": 1, "smooth.rows": 2, "smooth.zchunk": 0, "smooth.ychunk": 64,
-            "smooth.pair": 0, "smooth.ring": 1}
+            "smooth.pair": 0, "smooth.ring": 1, "stencil.ring": 1}
 
 
 @pytest.fixture(scope="module")
@@ -51,7 +51,7 @@ def solve(ctx, s, precision, fused, **tune):
         hip.tune_set(k, v)
     hip.tune_set("smooth.fused", fused)
     for k, v in tune.items():
-        hip.tune_set("smooth." + k, v)
+        hip.tune_set(k if "." in k else "smooth." + k, v)
     opt = capi.options(capi.MGPCG, tol=1e-10, tol_mode=capi.TOL_REL, precond=capi.PC_MG,
                        bc_rows=capi.ROWS_FOLDED, precision=precision, max_iter=100)
     st, u, rep = ctx.solve(s, opt)
@@ -84,6 +84,20 @@ def test_fused_sweep_bitwise_3d(ctx, kind, shape, tune, precision):
     s = (flux_problem if kind == "flux" else const_problem)(shape, seed=sum(shape))
     u0, r0 = solve(ctx, s, precision, 0, **tune)
     u1, r1 = solve(ctx, s, precision, 1, **tune)
+    print(f"  iterations {r0.iterations}/{r1.iterations}  max diff {np.abs(u0 - u1).max():.3e}", flush=True)
+    assert r0.iterations == r1.iterations
+    assert np.array_equal(u0, u1)
+
+
+@pytest.mark.parametrize("kind,shape", [("flux", (33, 65, 129)), ("flux", (65, 33, 17)),
+                                        ("const", (17, 129, 65))])
+def test_plane_ring_bitwise(ctx, kind, shape):
+    """The ring-buffered FP32/FP16 plane march (apply + residual of every mixed 3D level)
+    against k_plane3: the same outputs and partials, so the same mixed solve bit for bit."""
+    print(f"plane ring {kind} {shape}", flush=True)
+    s = (flux_problem if kind == "flux" else const_problem)(shape, seed=sum(shape) + 1)
+    u0, r0 = solve(ctx, s, capi.MIXED, 1, **{"stencil.ring": 0})
+    u1, r1 = solve(ctx, s, capi.MIXED, 1, **{"stencil.ring": 1})
     print(f"  iterations {r0.iterations}/{r1.iterations}  max diff {np.abs(u0 - u1).max():.3e}", flush=True)
     assert r0.iterations == r1.iterations
     assert np.array_equal(u0, u1)
