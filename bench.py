@@ -175,6 +175,33 @@ def bench_nonsym(args) -> int:
         errs = [float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
                 for a, b in zip(sols["mixed"], sols["fp64"])]
         results[f"{mname}_mixed"]["max_diff_vs_fp64"] = max(errs)
+        if method == capi.BICGSTAB:
+            seq64 = sols["fp64"]
+
+    # batched BiCGStab: the same problems solved together in lock step by a resident batch
+    # (one launch per operation over every unfinished problem, 8 iterations per graph replay),
+    # on the sample above and on all 128 problems; FP64 and mixed (FP32 Krylov vectors, FP64
+    # solution and residual).  The first solve of a batch captures its graph and is not timed.
+    all_specs = make_nonsym_problems(128)
+    for label, sp in (("sample", specs), ("all", all_specs)):
+        for precision, pname in ((capi.FP64, "fp64"), (capi.MIXED, "mixed")):
+            bt = ctx.batch(sp, kopts(capi.BICGSTAB, precision), capi.BatchConfig(use_graph=1))
+            bt.solve(check=False)
+            t0 = time.perf_counter()
+            st, reps = bt.solve(check=False)
+            wall = time.perf_counter() - t0
+            key = f"bicgstab_batched_{label}_{pname}"
+            results[key] = {"solves_per_s": round(len(sp) / wall, 2), "batch_ms": round(wall * 1e3, 1),
+                            "converged": int(sum(r.status == 0 for r in reps)), "problems": len(sp),
+                            "iterations_max": int(max(r.iterations for r in reps)),
+                            "iterations_total": int(sum(r.iterations for r in reps))}
+            if label == "sample":
+                results[key]["max_diff_vs_sequential_fp64"] = max(
+                    float(np.linalg.norm(bt.download(i) - seq64[i]) / max(np.linalg.norm(seq64[i]), 1e-300))
+                    for i in range(len(sp)))
+            bt.close()
+            log(f"bicgstab batched {label} {pname}: {results[key]['solves_per_s']} solves/s "
+                f"({results[key]['batch_ms']} ms per batch), {results[key]['converged']}/{len(sp)} converged")
 
     # reaction-diffusion, 2049^2, 1000 RK4 steps
     n = 2049
@@ -227,6 +254,8 @@ def bench_nonsym(args) -> int:
                    "problems": len(specs)},
         "krylov": results,
         "fp64_solves_per_s": round(fp64_total, 2),
+        "bicgstab_batched_all_mixed_solves_per_s": results["bicgstab_batched_all_mixed"]["solves_per_s"],
+        "bicgstab_batched_all_fp64_solves_per_s": results["bicgstab_batched_all_fp64"]["solves_per_s"],
         "reaction_diffusion_2049": rdres,
     }
     print(json.dumps(out), flush=True)

@@ -261,27 +261,33 @@ int pdehip_time_series(pdehip_ctx* ctx, const pdehip_problem* prob, const pdehip
                        pdehip_frame_fn on_frame, void* user, double* u_out, pdehip_report* report);
 
 /* ---------------------------------------------------------------- batches */
-/* Solve `count` independent problems in one persistent launch (MG-CG, 2D CONST/FLUX,
- * FOLDED rows).  u_out[i] receives problem i.  reports may be NULL. */
+/* Solve `count` independent problems.  2D constant-coefficient linear sets solved by MG-PCG
+ * (no convection) or BiCGStab (no preconditioner or Jacobi) go through the lock-step batched
+ * engine below; other sets are solved one resident plan after the other.  u_out[i] receives
+ * problem i.  reports may be NULL. */
 typedef struct pdehip_batch_config {
   int precision;          /* PDEHIP_FP64 or PDEHIP_MIXED (pdehip_solve_batch; the resident
                              batch takes the precision from its options) */
-  int use_graph;          /* 1: the launch sequence of two PCG iterations is captured once in a
-                             hipGraph and replayed */
+  int use_graph;          /* 1: the launch sequence of two PCG (eight BiCGStab) iterations is
+                             captured once in a hipGraph and replayed */
   int autotune;           /* 1: per-level work-item shapes picked by timing candidates on the
                              batch's own buffers (cached per context by the batch's sizes) */
   int tile_x;             /* nodes per row block of the residual/prolongation/restriction
-                             kernels (whole rows; 0 = default 256/128) */
-  int tile_y;             /* smoother strip height in rows (0 = by level size) */
+                             kernels, or of the BiCGStab vector kernels (whole rows; 0 = default
+                             256/128, BiCGStab 4096) */
+  int tile_y;             /* smoother / operator-apply strip height in rows (0 = by level size,
+                             BiCGStab 16) */
   int waves_per_cu;       /* reserved, 0 */
 } pdehip_batch_config;
 int pdehip_solve_batch(pdehip_ctx* ctx, int count, const pdehip_problem* probs,
                        const pdehip_options* opt, const pdehip_batch_config* cfg,
                        double* const* u_out, pdehip_report* reports);
 /* Resident batch: operators, hierarchies and Krylov state of `count` problems kept in HBM and
- * solved together in lock step (2D constant-coefficient MG-PCG, FP64 or mixed precision): each
- * operation of an iteration is one launch over every unfinished problem.  No reference
- * counterpart (the reference runs a batch as sequential solves, src/main.cpp:2071-2097).
+ * solved together in lock step (2D constant-coefficient MG-PCG, or BiCGStab with the reference's
+ * recomputed residual, FP64 or mixed precision): each operation of an iteration is one launch
+ * over every unfinished problem.  No reference counterpart (the reference runs a batch as
+ * sequential solves, src/main.cpp:2071-2097).  Mixed BiCGStab keeps the Krylov vectors and the
+ * applies in FP32 and the solution and every iteration's residual b - A x in FP64.
  * pdehip_batch_solve fills count reports and returns PDEHIP_OK when every problem converged,
  * else the status of the first one that did not; download copies problem `index`'s solution. */
 typedef struct pdehip_batch pdehip_batch;
@@ -293,7 +299,7 @@ int pdehip_batch_download(pdehip_batch* batch, int index, double* u_out);
 size_t pdehip_batch_bytes(const pdehip_batch* batch);
 /* Measurement: mean duration of one batched level-0 launch over every problem, timed with HIP
  * events on the batch stream.  kernel: 0 smoother red+black sweep, 1 residual, 2 Krylov
- * apply + dot, 3 FP64 residual.  algorithmic_bytes (may be NULL): compulsory bytes per launch.
+ * apply + dot, 3 FP64 residual (a BiCGStab batch: 2 and 3 only).  algorithmic_bytes (may be NULL): compulsory bytes per launch.
  * Overwrites the batch's work vectors and scalars; the next pdehip_batch_solve starts afresh. */
 int pdehip_batch_time_kernel(pdehip_batch* batch, int kernel, int reps, double* avg_ms,
                              double* algorithmic_bytes);

@@ -55,6 +55,7 @@ struct BatchPlan {
   pdehip_options opt{};
   pdehip_batch_config cfg{};
   bool mixed = false;
+  bool krylov = false;                     // BiCGStab batch (no hierarchy)
   int L = 0;                               // deepest hierarchy
   std::vector<std::unique_ptr<Plan>> plans;
   std::vector<int> levels;                 // per problem
@@ -63,6 +64,8 @@ struct BatchPlan {
   DBuf dense;                              // BDense [n]
   std::vector<List> sweep, rows, crows, drows;   // per level
   List el0, rows0;                         // level-0 element / strip items (with ranges)
+  List rb0;                                // BiCGStab: level-0 row-block items (with ranges)
+  DBuf kvecs;                              // BiCGStab: BKVec [n]
   std::vector<LevelShape> shape;           // per level work-item shapes
   DBuf part, maxp;
   DBuf active;
@@ -85,6 +88,7 @@ struct BatchPlan {
   }
   const BEnt* T(int l) const { return tab.as<BEnt>() + (size_t)l * n; }
   const BVec* V() const { return vecs.as<BVec>(); }
+  const BKVec* K() const { return kvecs.as<BKVec>(); }
   Prec coarse = P16;                       // mixed: storage of levels >= 1 (from the plans)
   Prec prec(int l) const { return mixed ? (l == 0 ? P32 : coarse) : P64; }
 };
@@ -112,6 +116,18 @@ std::vector<BItem> sweep_items(const BatchPlan& B, int l, int ychunk) {
   for (int p = 0; p < B.n; ++p) {
     if (l >= B.levels[p] - 1) continue;            // coarsest level: dense solve
     const Geo& g = B.plans[p]->L[l].g;
+    for (int sx = 0; sx * strip < g.nx; ++sx)
+      for (int j0 = 0; j0 < g.ny; j0 += ychunk) v.push_back(BItem{p, sx, j0, std::min(g.ny, j0 + ychunk)});
+  }
+  return v;
+}
+
+// Level-0 strips of every problem (the operator applies' items), whatever its hierarchy.
+std::vector<BItem> strip_items0(const BatchPlan& B, int ychunk) {
+  std::vector<BItem> v;
+  const int strip = 124;
+  for (int p = 0; p < B.n; ++p) {
+    const Geo& g = B.plans[p]->g;
     for (int sx = 0; sx * strip < g.nx; ++sx)
       for (int j0 = 0; j0 < g.ny; j0 += ychunk) v.push_back(BItem{p, sx, j0, std::min(g.ny, j0 + ychunk)});
   }
@@ -260,18 +276,148 @@ bool batch_eligible(int count, const pdehip_problem* probs, const pdehip_options
                     std::string* why) {
   auto no = [&](const char* r) { if (why) *why = r; return false; };
   if (count < 1) return no("empty batch");
-  if (opt.method != PDEHIP_MGPCG) return no("batched engine runs MG-PCG");
-  if (opt.mg_pre < 2 || opt.mg_pre % 2 || opt.mg_post < 2 || opt.mg_post % 2)
+  const bool bicg = opt.method == PDEHIP_BICGSTAB;
+  if (opt.method != PDEHIP_MGPCG && !bicg) return no("batched engine runs MG-PCG or BiCGStab");
+  if (bicg && opt.precond != PDEHIP_PC_NONE && opt.precond != PDEHIP_PC_JACOBI)
+    return no("batched BiCGStab takes no preconditioner or Jacobi");
+  if (!bicg && (opt.mg_pre < 2 || opt.mg_pre % 2 || opt.mg_post < 2 || opt.mg_post % 2))
     return no("batched engine needs even pre/post smoothing counts");
   for (int i = 0; i < count; ++i) {
     const pdehip_problem& p = probs[i];
     if (p.nz > 1) return no("batched engine is 2D");
     if (p.op_kind != PDEHIP_OP_CONST) return no("batched engine needs constant coefficients");
-    if (p.c != 0.0 || p.d != 0.0) return no("batched engine needs no convection");
+    if (!bicg && (p.c != 0.0 || p.d != 0.0)) return no("batched MG-PCG needs no convection");
     if (p.n_nonlinear > 0) return no("batched engine needs linear problems");
   }
   return true;
 }
+
+namespace {
+
+// Finish buffers shared by both engines: per-problem boundary data, residual-norm partials,
+// the device report array and its pinned copy; polling slots and events.
+void setup_finish(BatchPlan& B) {
+  Ctx* ctx = B.ctx;
+  const hipStream_t s = ctx->stream;
+  const int n = B.n;
+  B.active.alloc(ctx, 2 * sizeof(int));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&B.hactive), 2 * sizeof(int)));
+  for (hipEvent_t& e : B.evpoll) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  std::vector<BFace> fc(n);
+  for (int p = 0; p < n; ++p) {
+    const Plan& P = *B.plans[p];
+    fc[p] = BFace{P.faces.fd, P.f ? P.f.as<double>() : nullptr, P.f_const};
+  }
+  B.faces.alloc(ctx, n * sizeof(BFace));
+  HIPCHK(hipMemcpyAsync(B.faces.get(), fc.data(), n * sizeof(BFace), hipMemcpyHostToDevice, s));
+  B.psum.alloc(ctx, (size_t)std::max(1, B.el0.n) * sizeof(double));
+  B.pmax.alloc(ctx, (size_t)std::max(1, B.el0.n) * sizeof(double));
+  B.rep.alloc(ctx, n * sizeof(BReport));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&B.hrep), n * sizeof(BReport)));
+}
+
+// Element items (4096 nodes) and level-0 strip items of every problem, with per-problem ranges.
+void level0_lists(BatchPlan& B, int ychunk) {
+  Ctx* ctx = B.ctx;
+  const hipStream_t s = ctx->stream;
+  const int n = B.n;
+  std::vector<BItem> el;
+  std::vector<int> ef(n), ec(n);
+  for (int p = 0; p < n; ++p) {
+    const Geo& g = B.plans[p]->g;
+    ef[p] = static_cast<int>(el.size());
+    for (long long e0 = 0; e0 < g.n; e0 += 4096)
+      el.push_back(BItem{p, static_cast<int>(e0), static_cast<int>(std::min(g.n, e0 + 4096)), 0});
+    ec[p] = static_cast<int>(el.size()) - ef[p];
+  }
+  upload_list(ctx, s, B.el0, el, &ef, &ec);
+  const std::vector<BItem> rw = strip_items0(B, ychunk);
+  std::vector<int> rf(n, 0), rcn(n, 0);
+  for (int t = static_cast<int>(rw.size()) - 1; t >= 0; --t) { rf[rw[t].p] = t; rcn[rw[t].p] += 1; }
+  upload_list(ctx, s, B.rows0, rw, &rf, &rcn);
+}
+
+// BiCGStab batch: one resident plan per problem (operator, folded boundary rows, right-hand
+// side, Krylov vectors), the BKVec table, row-block items for the vector kernels, level-0
+// strips for the applies, and the finish buffers of the MG-PCG engine.
+BatchPtr create_krylov(BatchPtr B, Ctx* ctx, int count, const pdehip_problem* probs,
+                       const pdehip_options& opt, size_t used0) {
+  const hipStream_t s = ctx->stream;
+  B->krylov = true;
+  B->L = 1;
+  const bool jac = opt.precond == PDEHIP_PC_JACOBI;
+  std::vector<BVec> vecs(count);
+  std::vector<BKVec> kv(count);
+  for (int i = 0; i < count; ++i) {
+    B->plans.push_back(plan_create(ctx, probs[i], opt));
+    Plan& P = *B->plans.back();
+    B->levels.push_back(1);
+    BVec& v = vecs[i];
+    std::memset(&v, 0, sizeof(v));
+    v.x64 = P.x.as<double>();
+    v.r64 = P.r.as<double>();
+    v.b64 = P.bvec.as<double>();
+    v.x0 = P.x0.as<double>();
+    v.so = P.sc(0);
+    v.si = P.sc(1);
+    v.n = P.g.n;
+    v.nx = P.g.nx;
+    v.ny = P.g.ny;
+    v.op64 = P.L[0].op64;
+    BKVec& k = kv[i];
+    std::memset(&k, 0, sizeof(k));
+    if (B->mixed) {
+      k.r = P.v32[0].get(); k.rh = P.v32[1].get(); k.p = P.v32[2].get(); k.v = P.v32[3].get();
+      k.s = P.v32[4].get(); k.t = P.v32[5].get();
+      k.ph = jac ? P.v32[6].get() : k.p;
+      k.sh = jac ? P.v32[7].get() : k.s;
+      k.op = P.L[0].op32;
+    } else {
+      k.r = P.r.get(); k.rh = P.z.get(); k.p = P.p.get(); k.v = P.q.get();
+      k.s = P.w1.get(); k.t = P.w2.get();
+      k.ph = jac ? P.w4.get() : k.p;
+      k.sh = jac ? P.w3.get() : k.s;
+      k.op = P.L[0].op64;
+    }
+    k.x = P.x.as<double>();
+    k.b = P.bvec.as<double>();
+    k.sc = P.sc(0);
+    k.op64 = P.L[0].op64;
+    k.nx = P.g.nx;
+    k.ny = P.g.ny;
+    k.jacobi = jac ? 1 : 0;
+  }
+  B->vecs.alloc(ctx, count * sizeof(BVec));
+  B->kvecs.alloc(ctx, count * sizeof(BKVec));
+  HIPCHK(hipMemcpyAsync(B->vecs.get(), vecs.data(), count * sizeof(BVec), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(B->kvecs.get(), kv.data(), count * sizeof(BKVec), hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventCreate(&B->ev0));
+  HIPCHK(hipEventCreate(&B->ev1));
+  const int ychunk = B->cfg.tile_y > 0 ? B->cfg.tile_y : 16;
+  level0_lists(*B, ychunk);
+  {
+    const int nodes = B->cfg.tile_x > 0 ? B->cfg.tile_x : 4096;
+    std::vector<BItem> rb;
+    std::vector<int> f(count), c(count);
+    for (int p = 0; p < count; ++p) {
+      const Geo& g = B->plans[p]->g;
+      f[p] = static_cast<int>(rb.size());
+      const int rc = std::max(1, nodes / g.nx);
+      for (int j0 = 0; j0 < g.ny; j0 += rc) rb.push_back(BItem{p, j0, std::min(g.ny, j0 + rc), 0});
+      c[p] = static_cast<int>(rb.size()) - f[p];
+    }
+    upload_list(ctx, s, B->rb0, rb, &f, &c);
+  }
+  const int np = std::max({B->el0.n, B->rows0.n, B->rb0.n});
+  B->part.alloc(ctx, (size_t)np * sizeof(double));
+  B->maxp.alloc(ctx, (size_t)np * sizeof(double));
+  setup_finish(*B);
+  HIPCHK(hipStreamSynchronize(s));
+  B->bytes = ctx->used.load() - used0;
+  return B;
+}
+
+}  // namespace
 
 BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pdehip_options& opt,
                       const pdehip_batch_config* cfg) {
@@ -286,6 +432,7 @@ BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pd
   const hipStream_t s = ctx->stream;
   const size_t used0 = ctx->used.load();
   B->plans.reserve(count);
+  if (opt.method == PDEHIP_BICGSTAB) return create_krylov(std::move(B), ctx, count, probs, opt, used0);
   for (int i = 0; i < count; ++i) {
     B->plans.push_back(plan_create(ctx, probs[i], opt));
     Plan& P = *B->plans.back();
@@ -363,42 +510,12 @@ BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pd
   HIPCHK(hipEventCreate(&B->ev1));
   upload_shapes(*B, &dense);
   if (B->cfg.autotune) autotune(*B);
-  {
-    std::vector<BItem> el;
-    std::vector<int> ef(n), ec(n);
-    for (int p = 0; p < n; ++p) {
-      const Geo& g = B->plans[p]->g;
-      ef[p] = static_cast<int>(el.size());
-      for (long long e0 = 0; e0 < g.n; e0 += 4096)
-        el.push_back(BItem{p, static_cast<int>(e0), static_cast<int>(std::min(g.n, e0 + 4096)), 0});
-      ec[p] = static_cast<int>(el.size()) - ef[p];
-    }
-    upload_list(ctx, s, B->el0, el, &ef, &ec);
-    // level-0 strips of every problem (the smoother's items) with per-problem ranges
-    const std::vector<BItem> rw = sweep_items(*B, 0, B->shape[0].ychunk);
-    std::vector<int> rf(n, 0), rcn(n, 0);
-    for (int t = static_cast<int>(rw.size()) - 1; t >= 0; --t) { rf[rw[t].p] = t; rcn[rw[t].p] += 1; }
-    upload_list(ctx, s, B->rows0, rw, &rf, &rcn);
-  }
+  // level-0 element items and the smoother's level-0 strips, with per-problem ranges
+  level0_lists(*B, B->shape[0].ychunk);
   const int np = std::max(B->el0.n, B->rows0.n);
   B->part.alloc(ctx, (size_t)np * sizeof(double));
   B->maxp.alloc(ctx, (size_t)np * sizeof(double));
-  B->active.alloc(ctx, 2 * sizeof(int));
-  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&B->hactive), 2 * sizeof(int)));
-  for (hipEvent_t& e : B->evpoll) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  {
-    std::vector<BFace> fc(n);
-    for (int p = 0; p < n; ++p) {
-      const Plan& P = *B->plans[p];
-      fc[p] = BFace{P.faces.fd, P.f ? P.f.as<double>() : nullptr, P.f_const};
-    }
-    B->faces.alloc(ctx, n * sizeof(BFace));
-    HIPCHK(hipMemcpyAsync(B->faces.get(), fc.data(), n * sizeof(BFace), hipMemcpyHostToDevice, s));
-    B->psum.alloc(ctx, (size_t)std::max(1, B->el0.n) * sizeof(double));
-    B->pmax.alloc(ctx, (size_t)std::max(1, B->el0.n) * sizeof(double));
-    B->rep.alloc(ctx, n * sizeof(BReport));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&B->hrep), n * sizeof(BReport)));
-  }
+  setup_finish(*B);
   HIPCHK(hipStreamSynchronize(s));
   B->bytes = ctx->used.load() - used0;
   return B;
@@ -526,6 +643,73 @@ void pcg(BatchPlan& B, Prec kp, int max_iter, double delta) {
   if (outstanding >= 0) (void)poll_read(B, outstanding);
 }
 
+// BiCGStab on every problem in lock step (reference BiCGStabSolve, krylov.cpp:82-185: the
+// residual recomputed from the solution every iteration; Jacobi preconditioning optional), the
+// launch sequence of the single-problem path (runtime.cpp solve_bicgstab) with every launch
+// covering every unfinished problem.  Mixed precision: FP32 Krylov vectors and applies, FP64
+// solution and residual.
+void bicgstab(BatchPlan& B, int max_iter) {
+  const hipStream_t s = B.ctx->stream;
+  const Prec kp = B.mixed ? P32 : P64;
+  double* p1 = B.part.as<double>();
+  double* p2 = B.maxp.as<double>();
+  const List& rb = B.rb0;
+  const List& rw = B.rows0;
+  const BKVec* K = B.K();
+  auto fin = [&](const List& Lst, int op, const double* q2) {
+    bk::kfinalize(s, K, Lst.first.as<int>(), Lst.cnt.as<int>(), B.n, p1, q2, op, 0.0, 0);
+  };
+  bk::kvec(s, kp, bk::KK_RH, K, rb.d(), rb.n, nullptr);
+  const int every = 8;
+  auto iterations = [&] {
+    for (int c = 0; c < every; ++c) {
+      bk::kvec(s, kp, bk::KK_RHO, K, rb.d(), rb.n, p1);
+      fin(rb, bk::KK_FRHO, nullptr);
+      bk::kvec(s, kp, bk::KK_P, K, rb.d(), rb.n, nullptr);
+      bk::kapply(s, kp, 0, K, rw.d(), rw.n, p1, nullptr);
+      fin(rw, bk::KK_FALPHA, nullptr);
+      bk::kvec(s, kp, bk::KK_S, K, rb.d(), rb.n, p1);
+      fin(rb, bk::KK_FS, nullptr);
+      bk::kvec(s, kp, bk::KK_XS, K, rb.d(), rb.n, nullptr);
+      bk::kapply(s, kp, 1, K, rw.d(), rw.n, p1, p2);
+      fin(rw, bk::KK_FOMEGA, p2);
+      bk::kvec(s, kp, bk::KK_X, K, rb.d(), rb.n, nullptr);
+      bk::kapply(s, kp, 2, K, rw.d(), rw.n, p1, nullptr);
+      fin(rw, bk::KK_FR, nullptr);
+    }
+  };
+  int outstanding = -1;
+  int chunk = 0;
+  for (int it = 0; it < max_iter; it += every, ++chunk) {
+    if (B.cfg.use_graph) {
+      hipGraphExec_t& ge = B.graph[0];
+      if (!ge) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          iterations();
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIPCHK(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(e);
+      }
+      HIPCHK(hipGraphLaunch(ge, s));
+    } else {
+      iterations();
+    }
+    const int slot = chunk & 1;
+    poll_enqueue(B, 0, slot);
+    if (outstanding >= 0 && poll_read(B, outstanding) == 0) { outstanding = -1; break; }
+    outstanding = slot;
+  }
+  if (outstanding >= 0) (void)poll_read(B, outstanding);
+}
+
 }  // namespace
 
 int batch_solve(BatchPlan& B, pdehip_report* reports) {
@@ -535,23 +719,31 @@ int batch_solve(BatchPlan& B, pdehip_report* reports) {
   const List& rw = B.rows0;
   const List& el = B.el0;
   const int max_iter = std::max(1, B.opt.max_iter);
-  bk::init_scalars(s, B.V(), n, 1e-300);
+  bk::init_scalars(s, B.V(), n, B.krylov ? 1e-30 : 1e-300);
   HIPCHK(hipEventRecord(B.ev0, s));
   bk::vec(s, P64, bk::V_COPY_X0, B.V(), el.d(), el.n, nullptr, nullptr);
-  bk::apply_dot_tab(s, P64, B.V(), B.T(0), rw.d(), rw.n, part, true);
-  bk::finalize(s, B.V(), rw.first.as<int>(), rw.cnt.as<int>(), n, part, nullptr, FIN_R0, 0,
-               B.opt.tol, B.opt.tol_mode);
-  if (!B.mixed) {
-    pcg(B, P64, max_iter, 0.0);
+  if (B.krylov) {
+    // r = b - A x0 in FP64 (stored in the Krylov precision), |r0| and the target
+    bk::kapply(s, B.mixed ? P32 : P64, 2, B.K(), rw.d(), rw.n, part, nullptr);
+    bk::kfinalize(s, B.K(), rw.first.as<int>(), rw.cnt.as<int>(), n, part, nullptr, bk::KK_R0,
+                  B.opt.tol, B.opt.tol_mode);
+    bicgstab(B, max_iter);
   } else {
-    // inner scale fixed at |r0|; inner stop only through replacements (target 0 here)
-    const double delta = B.opt.inner_tol > 0 ? B.opt.inner_tol : 1e-2;
-    bk::ru_start(s, B.V(), n);
-    bk::vec(s, P32, bk::V_TO32, B.V(), el.d(), el.n, nullptr, nullptr);
-    bk::vec(s, P32, bk::V_RR, B.V(), el.d(), el.n, part, B.maxp.as<double>());
-    bk::finalize(s, B.V(), el.first.as<int>(), el.cnt.as<int>(), n, part, B.maxp.as<double>(),
-                 FIN_R0, 1, 0.0, PDEHIP_TOL_ABS);
-    pcg(B, P32, max_iter, delta);
+    bk::apply_dot_tab(s, P64, B.V(), B.T(0), rw.d(), rw.n, part, true);
+    bk::finalize(s, B.V(), rw.first.as<int>(), rw.cnt.as<int>(), n, part, nullptr, FIN_R0, 0,
+                 B.opt.tol, B.opt.tol_mode);
+    if (!B.mixed) {
+      pcg(B, P64, max_iter, 0.0);
+    } else {
+      // inner scale fixed at |r0|; inner stop only through replacements (target 0 here)
+      const double delta = B.opt.inner_tol > 0 ? B.opt.inner_tol : 1e-2;
+      bk::ru_start(s, B.V(), n);
+      bk::vec(s, P32, bk::V_TO32, B.V(), el.d(), el.n, nullptr, nullptr);
+      bk::vec(s, P32, bk::V_RR, B.V(), el.d(), el.n, part, B.maxp.as<double>());
+      bk::finalize(s, B.V(), el.first.as<int>(), el.cnt.as<int>(), n, part, B.maxp.as<double>(),
+                   FIN_R0, 1, 0.0, PDEHIP_TOL_ABS);
+      pcg(B, P32, max_iter, delta);
+    }
   }
   HIPCHK(hipEventRecord(B.ev1, s));
   // finish, batched: residual of the solved system, boundary closures, reference residual
@@ -576,7 +768,13 @@ int batch_solve(BatchPlan& B, pdehip_report* reports) {
     rep.residual_l2 = std::sqrt(k.l2sq);
     rep.residual_linf = k.linf;
     rep.converged = k.done && !k.breakdown;
-    if (B.mixed) {
+    if (B.krylov) {
+      rep.iterations = k.iter;
+      if (B.mixed) {   // every iteration's residual is the FP64 b - A x
+        rep.inner_iterations = k.iter;
+        rep.refinements = k.iter;
+      }
+    } else if (B.mixed) {
       rep.refinements = k.steps;
       rep.iterations = k.steps;
       rep.inner_iterations = k.inner;
@@ -603,6 +801,22 @@ double batch_time_kernel(BatchPlan& B, int kernel, int reps, double* algorithmic
   double bytes = 0.0;
   float ms = 0.f;
   double* part = B.part.as<double>();
+  if (B.krylov) {
+    const double ek = kp == P64 ? 8.0 : 4.0;
+    if (kernel == 2) {          // v = A ph with rh.v partials (read ph, rh; write v)
+      bytes = 3.0 * ek * static_cast<double>(n0);
+      ms = time_ms(B, [&] { bk::kapply(s, kp, 0, B.K(), B.rows0.d(), B.rows0.n, part, nullptr); },
+                   std::max(1, reps));
+    } else if (kernel == 3) {   // r = b - A x in FP64 (read x, b; write r)
+      bytes = (16.0 + ek) * static_cast<double>(n0);
+      ms = time_ms(B, [&] { bk::kapply(s, kp, 2, B.K(), B.rows0.d(), B.rows0.n, part, nullptr); },
+                   std::max(1, reps));
+    } else {
+      throw Error(PDEHIP_ERR_INVALID, "batch_time_kernel: a BiCGStab batch times kernels 2 and 3");
+    }
+    if (algorithmic_bytes) *algorithmic_bytes = bytes;
+    return ms;
+  }
   switch (kernel) {
     case 0:   // level-0 smoother: one red + black sweep, out of place (read u, b; write alt)
       bytes = 3.0 * es * static_cast<double>(n0);

@@ -41,6 +41,18 @@ struct BVec {
   DevOp op64;                    // level-0 FP64 operator (outer residual)
 };
 
+// Level-0 vectors of one problem of a batched BiCGStab (reference BiCGStabSolve semantics:
+// the residual is recomputed from the FP64 solution every iteration).  Krylov vectors are FP64,
+// or FP32 in mixed precision; the solution, right-hand side and residual arithmetic stay FP64.
+struct BKVec {
+  void *r, *rh, *p, *v, *s, *t, *ph, *sh;   // ph / sh alias p / s without a preconditioner
+  double *x, *b;
+  KScalars* sc;
+  DevOp op;       // the applies' operator (FP32 difference form in mixed precision)
+  DevOp op64;     // FP64 operator: residual and Jacobi diagonal
+  int nx, ny, jacobi, pad;
+};
+
 // Finish data of one problem: boundary data, source.
 struct BFace { FaceData fd; const double* f; double f_const; };
 
@@ -65,7 +77,8 @@ struct BatchDeleter { void operator()(BatchPlan* b) const; };
 using BatchPtr = std::unique_ptr<BatchPlan, BatchDeleter>;
 BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pdehip_options& opt,
                       const pdehip_batch_config* cfg);
-// Returns PDEHIP_OK, or the status of the first problem that did not converge.
+// Returns PDEHIP_OK, or the status of the first problem that did not converge.  MG-PCG batches
+// run the lock-step MG-PCG; BiCGStab batches the lock-step BiCGStab.
 int batch_solve(BatchPlan& B, pdehip_report* reports);
 void batch_download(BatchPlan& B, int i, double* u_out);
 // Mean duration (ms) of one launch of a batched level-0 kernel over every problem (HIP events
@@ -74,7 +87,8 @@ void batch_download(BatchPlan& B, int i, double* u_out);
 double batch_time_kernel(BatchPlan& B, int kernel, int reps, double* algorithmic_bytes);
 size_t batch_bytes(const BatchPlan& B);
 void batch_destroy(BatchPtr& B);
-// True when the batched engine handles this set (2D, constant-coefficient operators, MG-PCG).
+// True when the batched engine handles this set: 2D constant-coefficient linear operators, and
+// MG-PCG without convection, or BiCGStab (no preconditioner or Jacobi).
 bool batch_eligible(int count, const pdehip_problem* probs, const pdehip_options& opt,
                     std::string* why);
 
@@ -131,6 +145,21 @@ void resnorm(hipStream_t s, const BVec* vecs, const BFace* faces, const BItem* i
 void finish_reduce(hipStream_t s, const BVec* vecs, const int* rfirst, const int* rcnt,
                    const double* rr_part, const int* efirst, const int* ecnt, const double* sums,
                    const double* maxs, int nprob, BReport* out);
+// ---- batched BiCGStab (k_batch.hip).  Row-block items (p, j0, j1, 0) for the vector kernels,
+// level-0 strip items for the applies; partials per item, segmented finalize per problem.
+enum KkVec { KK_RH = 0, KK_RHO = 1, KK_P = 2, KK_S = 3, KK_XS = 4, KK_X = 5 };
+// rh = r / partials rh.r / p = r + beta (p - omega v), ph = D^-1 p / s = r - alpha v,
+// sh = D^-1 s, partials s.s / x += alpha ph (problems whose |s| met the target) /
+// x += alpha ph + omega sh.
+void kvec(hipStream_t s, Prec p, int op, const BKVec* vecs, const BItem* items, int n,
+          double* partials);
+// mode 0: v = A ph, partials rh.v; 1: t = A sh, partials t.t and t.s (skipped for problems whose
+// |s| met the target); 2: r = b - A64 x in FP64, partials |r|^2.
+void kapply(hipStream_t s, Prec p, int mode, const BKVec* vecs, const BItem* items, int n,
+            double* p1, double* p2);
+enum KkFin { KK_R0 = 0, KK_FRHO = 1, KK_FALPHA = 2, KK_FS = 3, KK_FOMEGA = 4, KK_FR = 5 };
+void kfinalize(hipStream_t s, const BKVec* vecs, const int* first, const int* cnt, int nprob,
+               const double* p1, const double* p2, int op, double tol, int tol_mode);
 }  // namespace bk
 
 }  // namespace pdehip

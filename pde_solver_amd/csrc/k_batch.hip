@@ -743,6 +743,212 @@ __global__ void kb_active(const BVec* __restrict__ vecs, int nprob, int inner, i
   if (threadIdx.x == 0) *active = static_cast<int>(s);
 }
 
+// ------------------------------------------------------------------------ batched BiCGStab
+// Vector kernels over row-block items (p, rows [a, b)): one wavefront walks the rows, lanes the
+// columns, so the Jacobi diagonal of every node follows from (i, j) without a division.  The
+// arithmetic per node is the single-problem path's (k_krylov.hip): FP64 combinations of the
+// stored values, rounded once to the vector precision, division by the folded diagonal.
+__device__ __forceinline__ double kk_diag(const DevOp& op, int i, int j) {
+  W7<double> w = const_w<double>(op);
+  fold_center<2, double>(op, i, j, 0, w);
+  return w.C;
+}
+
+template <typename S, int OP>
+__global__ void __launch_bounds__(kLanes * kWavesPB)
+kk_vec(const BKVec* __restrict__ vecs, const BItem* __restrict__ items, int n, double* partials) {
+  const int w = wave_item();
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BKVec& v = vecs[it.p];
+  const int lane = threadIdx.x & (kLanes - 1);
+  double acc = 0.0;
+  const KScalars* sc = v.sc;
+  if (!done(sc)) {
+    const int nx = v.nx, ny = v.ny;
+    S* r = static_cast<S*>(v.r);
+    S* rh = static_cast<S*>(v.rh);
+    S* p = static_cast<S*>(v.p);
+    S* vv = static_cast<S*>(v.v);
+    S* sv = static_cast<S*>(v.s);
+    S* ph = static_cast<S*>(v.ph);
+    S* sh = static_cast<S*>(v.sh);
+    const double alpha = ldS(&sc->alpha, 0), beta = ldS(&sc->beta, 0), omega = ldS(&sc->omega, 0);
+    const bool first = ldS(&sc->iter, 0) == 0;
+    const bool fs = ldS(&sc->flag_s, 0) != 0;
+    const bool jac = v.jacobi != 0;
+    for (int j = it.a; j < it.b && (OP != KK_XS || fs); ++j) {
+      const long long row = (long long)j * nx;
+      const bool bj = j == 0 || j == ny - 1;
+      for (int i = lane; i < nx; i += kLanes) {
+        const long long id = row + i;
+        const bool bnd = bj || i == 0 || i == nx - 1;
+        if constexpr (OP == KK_RH) {
+          stS(rh, id, ldS(r, id));
+        } else if constexpr (OP == KK_RHO) {
+          acc += double(ldS(rh, id)) * double(ldS(r, id));
+        } else if constexpr (OP == KK_P) {
+          const S pn = static_cast<S>(first ? double(ldS(r, id))
+                                            : double(ldS(r, id)) + beta * (double(ldS(p, id)) - omega * double(ldS(vv, id))));
+          stS(p, id, pn);
+          if (jac) stS(ph, id, bnd ? pn : static_cast<S>(double(pn) / kk_diag(v.op64, i, j)));
+        } else if constexpr (OP == KK_S) {
+          const S si = static_cast<S>(double(ldS(r, id)) - alpha * double(ldS(vv, id)));
+          stS(sv, id, si);
+          acc += double(si) * double(si);
+          if (jac) stS(sh, id, bnd ? si : static_cast<S>(double(si) / kk_diag(v.op64, i, j)));
+        } else if constexpr (OP == KK_XS) {
+          stS(v.x, id, ldS(v.x, id) + alpha * double(ldS(ph, id)));
+        } else {
+          stS(v.x, id, ldS(v.x, id) + (alpha * double(ldS(ph, id)) + omega * double(ldS(sh, id))));
+        }
+      }
+    }
+  }
+  if constexpr (OP == KK_RHO || OP == KK_S) {
+    acc = wave_sum(acc);
+    if (lane == 0) partials[w] = acc;
+  }
+}
+
+// Applies over level-0 strip items.  MODE 0: v = A ph (+ rh.v); 1: t = A sh (+ t.t, t.s);
+// 2: r = b - A x in FP64 (+ |r|^2).  FP64 vectors use the sum form with the folded centre
+// (k_stencil's FP64 rows); FP32 vectors the difference form (its FP32 rows).
+template <typename S, int MODE>
+__global__ void __launch_bounds__(kLanes * kWavesPB)
+kk_apply(const BKVec* __restrict__ vecs, const BItem* __restrict__ items, int n, double* p1,
+         double* p2) {
+  using T = std::conditional_t<MODE == 2, double, S>;
+  const int w = wave_item();
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BKVec& v = vecs[it.p];
+  const int lane = threadIdx.x & (kLanes - 1);
+  double a1 = 0.0, a2 = 0.0;
+  const KScalars* sc = v.sc;
+  bool run = !done(sc);
+  if (MODE == 1 && run) run = ldS(&sc->flag_s, 0) == 0;
+  if (run) {
+    const DevOp& op = MODE == 2 ? v.op64 : v.op;
+    const W7<T> w0 = const_w<T>(op);
+    const T e = T(op.e);
+    auto ax_of = [&](int i, int j, T xc, T xw, T xe, T xs, T xn) -> T {
+      if (i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1) return xc;
+      if constexpr (std::is_same<T, double>::value) {
+        W7<T> wf = w0;
+        fold_center<2, T>(op, i, j, 0, wf);
+        return wf.C * xc + wf.E * xe + wf.W * xw + wf.N * xn + wf.S * xs;
+      } else {
+        return apply_diff<2, T>(op, w0, e, i, j, 0, xc, xw, xe, xs, xn, T(0), T(0));
+      }
+    };
+    if constexpr (MODE == 0) {
+      S* out = static_cast<S*>(v.v);
+      const S* rh = static_cast<const S*>(v.rh);
+      strip_march<T>(static_cast<const S*>(v.ph), static_cast<const void*>(nullptr), op.nx, op.ny, it,
+                     [&](int i, int j, long long id, T xc, T xw, T xe, T xs, T xn, T) {
+        const S o = static_cast<S>(ax_of(i, j, xc, xw, xe, xs, xn));
+        stS(out, id, o);
+        a1 += double(ldS(rh, id)) * double(o);
+      });
+    } else if constexpr (MODE == 1) {
+      S* out = static_cast<S*>(v.t);
+      const S* sv = static_cast<const S*>(v.s);
+      strip_march<T>(static_cast<const S*>(v.sh), static_cast<const void*>(nullptr), op.nx, op.ny, it,
+                     [&](int i, int j, long long id, T xc, T xw, T xe, T xs, T xn, T) {
+        const S o = static_cast<S>(ax_of(i, j, xc, xw, xe, xs, xn));
+        stS(out, id, o);
+        a1 += double(o) * double(o);
+        a2 += double(o) * double(ldS(sv, id));
+      });
+    } else {
+      S* out = static_cast<S*>(v.r);
+      strip_march<double>(v.x, static_cast<const double*>(v.b), op.nx, op.ny, it,
+                          [&](int i, int j, long long id, double xc, double xw, double xe, double xs,
+                              double xn, double bj) {
+        const double rv = bj - ax_of(i, j, xc, xw, xe, xs, xn);
+        stS(out, id, static_cast<S>(rv));
+        a1 += rv * rv;
+      });
+    }
+  }
+  a1 = wave_sum(a1);
+  if (lane == 0) p1[w] = a1;
+  if constexpr (MODE == 1) {
+    a2 = wave_sum(a2);
+    if (lane == 0) p2[w] = a2;
+  }
+}
+
+// Segmented finalize of the batched BiCGStab: problem blockIdx.x sums its items' partials in a
+// fixed order and applies the single-problem path's scalar step (k_krylov.hip k_fin_k).
+__global__ void __launch_bounds__(256)
+kk_finalize(const BKVec* __restrict__ vecs, const int* __restrict__ first,
+            const int* __restrict__ cnt, int nprob, const double* __restrict__ p1,
+            const double* __restrict__ p2, int op, double tol, int tol_mode) {
+  __shared__ double lds[256 / kWave];
+  const int q = blockIdx.x;
+  if (q >= nprob) return;
+  KScalars* sc = vecs[q].sc;
+  if (op != KK_R0 && (sc->done | sc->breakdown)) return;
+  const int f = first[q], c = cnt[q];
+  double a1 = 0.0, a2 = 0.0;
+  for (int i = threadIdx.x; i < c; i += blockDim.x) {
+    a1 += p1[f + i];
+    if (p2) a2 += p2[f + i];
+  }
+  const double s1 = block_sum(a1, lds);
+  const double s2 = p2 ? block_sum(a2, lds) : 0.0;
+  if (threadIdx.x != 0) return;
+  switch (op) {
+    case KK_R0: {
+      const double r0 = sqrt(s1);
+      if (nonfinite(s1)) { sc->breakdown = 1; break; }
+      sc->r0 = r0;
+      sc->metric = r0;
+      sc->target = tol_mode == PDEHIP_TOL_REL ? tol * r0 : tol;
+      if (r0 < sc->target || r0 == 0.0) sc->done = 1;
+      break;
+    }
+    case KK_FRHO:
+      sc->rho = s1;
+      if (fabs(s1) < sc->eps) { sc->breakdown = 1; break; }
+      if (sc->iter > 0) sc->beta = (s1 / sc->rho_old) * (sc->alpha / sc->omega);
+      break;
+    case KK_FALPHA:
+      if (fabs(s1) < sc->eps) { sc->breakdown = 1; break; }
+      sc->alpha = sc->rho / s1;
+      break;
+    case KK_FS:
+      sc->snorm = sqrt(s1);
+      if (nonfinite(s1)) { sc->breakdown = 1; break; }
+      if (sc->snorm < sc->target) sc->flag_s = 1;
+      break;
+    case KK_FOMEGA:
+      if (sc->flag_s) {   // |s| met the target: x += alpha ph already done, stop here
+        sc->metric = sc->snorm;
+        sc->iter += 1;
+        sc->done = 1;
+        break;
+      }
+      sc->tt = s1;
+      sc->ts = s2;
+      if (fabs(s1) < sc->eps) { sc->breakdown = 1; break; }
+      sc->omega = s2 / s1;
+      break;
+    case KK_FR:
+      sc->metric = sqrt(s1);
+      sc->iter += 1;
+      if (nonfinite(s1)) { sc->breakdown = 1; break; }
+      if (sc->metric < sc->target) { sc->done = 1; break; }
+      if (fabs(sc->omega) < sc->eps) { sc->breakdown = 1; break; }
+      sc->rho_old = sc->rho;
+      break;
+    default:
+      break;
+  }
+}
+
 // ------------------------------------------------------------------------------- finish
 // Boundary closures of problem blockIdx.x's FP64 solution, one face pair per launch (phase).
 __global__ void __launch_bounds__(256)
@@ -986,6 +1192,45 @@ void finish_reduce(hipStream_t s, const BVec* vecs, const int* rfirst, const int
 
 void active_count(hipStream_t s, const BVec* vecs, int nprob, int inner, int* active) {
   hipLaunchKernelGGL(kb_active, dim3(1), dim3(1024), 0, s, vecs, nprob, inner, active);
+}
+
+
+void kvec(hipStream_t s, Prec p, int op, const BKVec* vecs, const BItem* items, int n,
+          double* partials) {
+  if (n <= 0) return;
+  const dim3 g(waves_grid(n)), b(kLanes * kWavesPB);
+#define KV_(S, OP) hipLaunchKernelGGL((kk_vec<S, OP>), g, b, 0, s, vecs, items, n, partials)
+#define KV_ALL(S)                                \
+  switch (op) {                                  \
+    case KK_RH: KV_(S, KK_RH); break;            \
+    case KK_RHO: KV_(S, KK_RHO); break;          \
+    case KK_P: KV_(S, KK_P); break;              \
+    case KK_S: KV_(S, KK_S); break;              \
+    case KK_XS: KV_(S, KK_XS); break;            \
+    default: KV_(S, KK_X); break;                \
+  }
+  if (p == P64) { KV_ALL(double) } else { KV_ALL(float) }
+#undef KV_ALL
+#undef KV_
+}
+
+void kapply(hipStream_t s, Prec p, int mode, const BKVec* vecs, const BItem* items, int n,
+            double* p1, double* p2) {
+  if (n <= 0) return;
+  const dim3 g(waves_grid(n)), b(kLanes * kWavesPB);
+#define KA_(S, M) hipLaunchKernelGGL((kk_apply<S, M>), g, b, 0, s, vecs, items, n, p1, p2)
+  if (p == P64) {
+    if (mode == 0) KA_(double, 0); else if (mode == 1) KA_(double, 1); else KA_(double, 2);
+  } else {
+    if (mode == 0) KA_(float, 0); else if (mode == 1) KA_(float, 1); else KA_(float, 2);
+  }
+#undef KA_
+}
+
+void kfinalize(hipStream_t s, const BKVec* vecs, const int* first, const int* cnt, int nprob,
+               const double* p1, const double* p2, int op, double tol, int tol_mode) {
+  hipLaunchKernelGGL(kk_finalize, dim3(nprob), dim3(256), 0, s, vecs, first, cnt, nprob, p1, p2,
+                     op, tol, tol_mode);
 }
 
 }  // namespace bk
