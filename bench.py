@@ -150,10 +150,10 @@ def bench_nonsym(args) -> int:
 
     results = {}
     for method, mname in ((capi.BICGSTAB, "bicgstab"), (capi.GMRES, "gmres")):
-        sols = {}
+        sols, oks = {}, {}
         for precision, pname in ((capi.FP64, "fp64"), (capi.MIXED, "mixed")):
             plans = [ctx.plan(s, kopts(method, precision)) for s in specs]
-            lat, its, conv, us = [], [], 0, []
+            lat, its, conv, us, ok = [], [], 0, [], []
             t0 = time.perf_counter()
             for pl in plans:
                 t1 = time.perf_counter()
@@ -161,11 +161,13 @@ def bench_nonsym(args) -> int:
                 lat.append((time.perf_counter() - t1) * 1e3)
                 its.append(rep.inner_iterations if precision == capi.MIXED else rep.iterations)
                 conv += int(st == 0)
+                ok.append(st == 0)
             wall = time.perf_counter() - t0
             for pl in plans:
                 us.append(pl.download())
                 pl.close()
             sols[pname] = us
+            oks[pname] = ok
             results[f"{mname}_{pname}"] = {
                 "solves_per_s": round(len(specs) / wall, 2), "converged": conv,
                 "problems": len(specs), "latency_ms_p50": round(pct(lat, 50), 2),
@@ -176,7 +178,7 @@ def bench_nonsym(args) -> int:
                 for a, b in zip(sols["mixed"], sols["fp64"])]
         results[f"{mname}_mixed"]["max_diff_vs_fp64"] = max(errs)
         if method == capi.BICGSTAB:
-            seq64 = sols["fp64"]
+            seq64, seq64_ok = sols["fp64"], oks["fp64"]
 
     # batched BiCGStab: the same problems solved together in lock step by a resident batch
     # (one launch per operation over every unfinished problem, 8 iterations per graph replay),
@@ -195,10 +197,10 @@ def bench_nonsym(args) -> int:
                             "converged": int(sum(r.status == 0 for r in reps)), "problems": len(sp),
                             "iterations_max": int(max(r.iterations for r in reps)),
                             "iterations_total": int(sum(r.iterations for r in reps))}
-            if label == "sample":
-                results[key]["max_diff_vs_sequential_fp64"] = max(
-                    float(np.linalg.norm(bt.download(i) - seq64[i]) / max(np.linalg.norm(seq64[i]), 1e-300))
-                    for i in range(len(sp)))
+            if label == "sample":   # over the problems both solves converged on
+                diffs = [float(np.linalg.norm(bt.download(i) - seq64[i]) / max(np.linalg.norm(seq64[i]), 1e-300))
+                         for i in range(len(sp)) if reps[i].status == 0 and seq64_ok[i]]
+                results[key]["max_diff_vs_sequential_fp64"] = max(diffs) if diffs else None
             bt.close()
             log(f"bicgstab batched {label} {pname}: {results[key]['solves_per_s']} solves/s "
                 f"({results[key]['batch_ms']} ms per batch), {results[key]['converged']}/{len(sp)} converged")

@@ -65,7 +65,8 @@ struct BatchPlan {
   std::vector<List> sweep, rows, crows, drows;   // per level
   List el0, rows0;                         // level-0 element / strip items (with ranges)
   List rb0;                                // BiCGStab: level-0 row-block items (with ranges)
-  DBuf kvecs;                              // BiCGStab: BKVec [n]
+  DBuf kvecs;                              // BiCGStab: BKVec [n] (the batch's precision)
+  DBuf kvecs64;                            // mixed BiCGStab: FP64 views for the completion pass
   std::vector<LevelShape> shape;           // per level work-item shapes
   DBuf part, maxp;
   DBuf active;
@@ -89,6 +90,7 @@ struct BatchPlan {
   const BEnt* T(int l) const { return tab.as<BEnt>() + (size_t)l * n; }
   const BVec* V() const { return vecs.as<BVec>(); }
   const BKVec* K() const { return kvecs.as<BKVec>(); }
+  const BKVec* K64() const { return kvecs64.as<BKVec>(); }
   Prec coarse = P16;                       // mixed: storage of levels >= 1 (from the plans)
   Prec prec(int l) const { return mixed ? (l == 0 ? P32 : coarse) : P64; }
 };
@@ -337,6 +339,40 @@ void level0_lists(BatchPlan& B, int ychunk) {
   upload_list(ctx, s, B.rows0, rw, &rf, &rcn);
 }
 
+// Per-problem BiCGStab vectors: the plan's FP64 Krylov vectors, or (fp32) its FP32 ones; the
+// solution, right-hand side and initial guess are FP64 either way.  A mixed-precision plan
+// allocates its FP64 Krylov vectors only when a completion pass needs them
+// (ensure_fp64_views).
+BKVec krylov_view(Plan& P, bool jac, bool fp32) {
+  BKVec k;
+  std::memset(&k, 0, sizeof(k));
+  if (fp32) {
+    k.r = P.v32[0].get(); k.rh = P.v32[1].get(); k.p = P.v32[2].get(); k.v = P.v32[3].get();
+    k.s = P.v32[4].get(); k.t = P.v32[5].get();
+    k.ph = jac ? P.v32[6].get() : k.p;
+    k.sh = jac ? P.v32[7].get() : k.s;
+    k.op = P.L[0].op32;
+  } else {
+    k.r = P.r.get(); k.rh = P.z.get(); k.p = P.p.get(); k.v = P.q.get();
+    k.s = P.w1.get(); k.t = P.w2.get();
+    k.ph = jac ? P.w4.get() : k.p;
+    k.sh = jac ? P.w3.get() : k.s;
+    k.op = P.L[0].op64;
+  }
+  k.x = P.x.as<double>();
+  k.b = P.bvec.as<double>();
+  k.x0 = P.x0.as<double>();
+  k.sc = P.sc(0);
+  k.op64 = P.L[0].op64;
+  k.nx = P.g.nx;
+  k.ny = P.g.ny;
+  k.jacobi = jac ? 1 : 0;
+  const void* need[] = {k.r, k.rh, k.p, k.v, k.s, k.t, k.ph, k.sh, k.x, k.b, k.x0, k.sc};
+  for (const void* q : need)
+    if (!q) throw Error(PDEHIP_ERR_DEVICE, "batched BiCGStab: a Krylov vector of a plan is missing");
+  return k;
+}
+
 // BiCGStab batch: one resident plan per problem (operator, folded boundary rows, right-hand
 // side, Krylov vectors), the BKVec table, row-block items for the vector kernels, level-0
 // strips for the applies, and the finish buffers of the MG-PCG engine.
@@ -364,28 +400,7 @@ BatchPtr create_krylov(BatchPtr B, Ctx* ctx, int count, const pdehip_problem* pr
     v.nx = P.g.nx;
     v.ny = P.g.ny;
     v.op64 = P.L[0].op64;
-    BKVec& k = kv[i];
-    std::memset(&k, 0, sizeof(k));
-    if (B->mixed) {
-      k.r = P.v32[0].get(); k.rh = P.v32[1].get(); k.p = P.v32[2].get(); k.v = P.v32[3].get();
-      k.s = P.v32[4].get(); k.t = P.v32[5].get();
-      k.ph = jac ? P.v32[6].get() : k.p;
-      k.sh = jac ? P.v32[7].get() : k.s;
-      k.op = P.L[0].op32;
-    } else {
-      k.r = P.r.get(); k.rh = P.z.get(); k.p = P.p.get(); k.v = P.q.get();
-      k.s = P.w1.get(); k.t = P.w2.get();
-      k.ph = jac ? P.w4.get() : k.p;
-      k.sh = jac ? P.w3.get() : k.s;
-      k.op = P.L[0].op64;
-    }
-    k.x = P.x.as<double>();
-    k.b = P.bvec.as<double>();
-    k.sc = P.sc(0);
-    k.op64 = P.L[0].op64;
-    k.nx = P.g.nx;
-    k.ny = P.g.ny;
-    k.jacobi = jac ? 1 : 0;
+    kv[i] = krylov_view(P, jac, B->mixed);
   }
   B->vecs.alloc(ctx, count * sizeof(BVec));
   B->kvecs.alloc(ctx, count * sizeof(BKVec));
@@ -646,16 +661,39 @@ void pcg(BatchPlan& B, Prec kp, int max_iter, double delta) {
 // BiCGStab on every problem in lock step (reference BiCGStabSolve, krylov.cpp:82-185: the
 // residual recomputed from the solution every iteration; Jacobi preconditioning optional), the
 // launch sequence of the single-problem path (runtime.cpp solve_bicgstab) with every launch
-// covering every unfinished problem.  Mixed precision: FP32 Krylov vectors and applies, FP64
-// solution and residual.
-void bicgstab(BatchPlan& B, int max_iter) {
+// covering every unfinished problem.  Unlike the reference, a breakdown restarts the problem
+// from its true residual (at most 8 times) instead of ending its solve.  Mixed precision: FP32
+// Krylov vectors and applies, FP64 solution and residual.
+// Mixed BiCGStab batch: the FP64 Krylov vectors of every plan (allocated once, zeroed) and the
+// FP64 view table the completion pass runs on.
+void ensure_fp64_views(BatchPlan& B) {
+  if (B.kvecs64) return;
+  Ctx* ctx = B.ctx;
+  const hipStream_t s = ctx->stream;
+  const size_t used0 = ctx->used.load();
+  const bool jac = B.opt.precond == PDEHIP_PC_JACOBI;
+  std::vector<BKVec> kv(B.n);
+  for (int i = 0; i < B.n; ++i) {
+    Plan& P = *B.plans[i];
+    const size_t n8 = static_cast<size_t>(P.g.n) * sizeof(double);
+    for (DBuf* b : {&P.p, &P.q, &P.z, &P.w1, &P.w2, &P.w3, &P.w4}) {
+      if (b == &P.w4 && !jac) continue;
+      if (!*b) { b->alloc(ctx, n8); fill(s, P64, b->get(), P.g.n, 0.0); }
+    }
+    kv[i] = krylov_view(P, jac, false);
+  }
+  B.kvecs64.alloc(ctx, B.n * sizeof(BKVec));
+  HIPCHK(hipMemcpyAsync(B.kvecs64.get(), kv.data(), B.n * sizeof(BKVec), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  B.bytes += ctx->used.load() - used0;
+}
+
+void bicgstab(BatchPlan& B, int max_iter, Prec kp, const BKVec* K, int graph_slot) {
   const hipStream_t s = B.ctx->stream;
-  const Prec kp = B.mixed ? P32 : P64;
   double* p1 = B.part.as<double>();
   double* p2 = B.maxp.as<double>();
   const List& rb = B.rb0;
   const List& rw = B.rows0;
-  const BKVec* K = B.K();
   auto fin = [&](const List& Lst, int op, const double* q2) {
     bk::kfinalize(s, K, Lst.first.as<int>(), Lst.cnt.as<int>(), B.n, p1, q2, op, 0.0, 0);
   };
@@ -663,6 +701,8 @@ void bicgstab(BatchPlan& B, int max_iter) {
   const int every = 8;
   auto iterations = [&] {
     for (int c = 0; c < every; ++c) {
+      bk::kvec(s, kp, bk::KK_RESTART, K, rb.d(), rb.n, nullptr);
+      bk::krestart(s, K, B.n);
       bk::kvec(s, kp, bk::KK_RHO, K, rb.d(), rb.n, p1);
       fin(rb, bk::KK_FRHO, nullptr);
       bk::kvec(s, kp, bk::KK_P, K, rb.d(), rb.n, nullptr);
@@ -682,7 +722,7 @@ void bicgstab(BatchPlan& B, int max_iter) {
   int chunk = 0;
   for (int it = 0; it < max_iter; it += every, ++chunk) {
     if (B.cfg.use_graph) {
-      hipGraphExec_t& ge = B.graph[0];
+      hipGraphExec_t& ge = B.graph[graph_slot];
       if (!ge) {
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -724,10 +764,28 @@ int batch_solve(BatchPlan& B, pdehip_report* reports) {
   bk::vec(s, P64, bk::V_COPY_X0, B.V(), el.d(), el.n, nullptr, nullptr);
   if (B.krylov) {
     // r = b - A x0 in FP64 (stored in the Krylov precision), |r0| and the target
-    bk::kapply(s, B.mixed ? P32 : P64, 2, B.K(), rw.d(), rw.n, part, nullptr);
+    const Prec kp = B.mixed ? P32 : P64;
+    bk::kapply(s, kp, 2, B.K(), rw.d(), rw.n, part, nullptr);
     bk::kfinalize(s, B.K(), rw.first.as<int>(), rw.cnt.as<int>(), n, part, nullptr, bk::KK_R0,
                   B.opt.tol, B.opt.tol_mode);
-    bicgstab(B, max_iter);
+    bicgstab(B, max_iter, kp, B.K(), 0);
+    if (B.mixed) {
+      // FP64 completion: problems the FP32 recurrence did not bring to the target start again
+      // from x0 in FP64 (the single-problem path's fallback, refinements = -1 in the report)
+      bk::kfailed(s, B.K(), n, B.active.as<int>());
+      HIPCHK(hipMemcpyAsync(B.hactive, B.active.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (B.hactive[0] > 0) {
+        ensure_fp64_views(B);
+        const List& rb = B.rb0;
+        bk::kreset_failed(s, B.K64(), n, 1e-30);
+        bk::kvec(s, P64, bk::KK_X0, B.K64(), rb.d(), rb.n, nullptr);
+        bk::kapply(s, P64, 2, B.K64(), rw.d(), rw.n, part, nullptr);
+        bk::kfinalize(s, B.K64(), rw.first.as<int>(), rw.cnt.as<int>(), n, part, nullptr,
+                      bk::KK_R0, B.opt.tol, B.opt.tol_mode);
+        bicgstab(B, max_iter, P64, B.K64(), 1);
+      }
+    }
   } else {
     bk::apply_dot_tab(s, P64, B.V(), B.T(0), rw.d(), rw.n, part, true);
     bk::finalize(s, B.V(), rw.first.as<int>(), rw.cnt.as<int>(), n, part, nullptr, FIN_R0, 0,
@@ -770,7 +828,9 @@ int batch_solve(BatchPlan& B, pdehip_report* reports) {
     rep.converged = k.done && !k.breakdown;
     if (B.krylov) {
       rep.iterations = k.iter;
-      if (B.mixed) {   // every iteration's residual is the FP64 b - A x
+      if (B.mixed && k.steps < 0) {   // completed by the FP64 pass
+        rep.refinements = -1;
+      } else if (B.mixed) {           // every iteration's residual is the FP64 b - A x
         rep.inner_iterations = k.iter;
         rep.refinements = k.iter;
       }

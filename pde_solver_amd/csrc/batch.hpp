@@ -46,7 +46,7 @@ struct BVec {
 // or FP32 in mixed precision; the solution, right-hand side and residual arithmetic stay FP64.
 struct BKVec {
   void *r, *rh, *p, *v, *s, *t, *ph, *sh;   // ph / sh alias p / s without a preconditioner
-  double *x, *b;
+  double *x, *b, *x0;
   KScalars* sc;
   DevOp op;       // the applies' operator (FP32 difference form in mixed precision)
   DevOp op64;     // FP64 operator: residual and Jacobi diagonal
@@ -147,10 +147,12 @@ void finish_reduce(hipStream_t s, const BVec* vecs, const int* rfirst, const int
                    const double* maxs, int nprob, BReport* out);
 // ---- batched BiCGStab (k_batch.hip).  Row-block items (p, j0, j1, 0) for the vector kernels,
 // level-0 strip items for the applies; partials per item, segmented finalize per problem.
-enum KkVec { KK_RH = 0, KK_RHO = 1, KK_P = 2, KK_S = 3, KK_XS = 4, KK_X = 5 };
+enum KkVec { KK_RH = 0, KK_RHO = 1, KK_P = 2, KK_S = 3, KK_XS = 4, KK_X = 5, KK_RESTART = 6,
+             KK_X0 = 7 };
 // rh = r / partials rh.r / p = r + beta (p - omega v), ph = D^-1 p / s = r - alpha v,
 // sh = D^-1 s, partials s.s / x += alpha ph (problems whose |s| met the target) /
-// x += alpha ph + omega sh.
+// x += alpha ph + omega sh / rh = r for problems with a restart pending after a breakdown /
+// x = x0.
 void kvec(hipStream_t s, Prec p, int op, const BKVec* vecs, const BItem* items, int n,
           double* partials);
 // mode 0: v = A ph, partials rh.v; 1: t = A sh, partials t.t and t.s (skipped for problems whose
@@ -158,6 +160,13 @@ void kvec(hipStream_t s, Prec p, int op, const BKVec* vecs, const BItem* items, 
 void kapply(hipStream_t s, Prec p, int mode, const BKVec* vecs, const BItem* items, int n,
             double* p1, double* p2);
 enum KkFin { KK_R0 = 0, KK_FRHO = 1, KK_FALPHA = 2, KK_FS = 3, KK_FOMEGA = 4, KK_FR = 5 };
+// Scalars of a fresh start for problems whose restart is pending (after KK_RESTART).
+void krestart(hipStream_t s, const BKVec* vecs, int nprob);
+// Mixed batches: every problem that did not converge gets fresh scalars (eps) and steps = -1,
+// for the FP64 completion pass; converged problems keep theirs.
+void kreset_failed(hipStream_t s, const BKVec* vecs, int nprob, double eps);
+// Count of problems that did not converge, written to *out.
+void kfailed(hipStream_t s, const BKVec* vecs, int nprob, int* out);
 void kfinalize(hipStream_t s, const BKVec* vecs, const int* first, const int* cnt, int nprob,
                const double* p1, const double* p2, int op, double tol, int tol_mode);
 }  // namespace bk

@@ -737,7 +737,8 @@ __global__ void kb_active(const BVec* __restrict__ vecs, int nprob, int inner, i
   double a = 0.0;
   for (int q = threadIdx.x; q < nprob; q += blockDim.x) {
     const KScalars* sc = inner ? vecs[q].si : vecs[q].so;
-    a += (sc->done | sc->breakdown) ? 0.0 : 1.0;
+    // a BiCGStab problem broken down with a restart pending is still active
+    a += (sc->done | (sc->breakdown & !sc->cycle_done)) ? 0.0 : 1.0;
   }
   const double s = block_sum(a, lds);
   if (threadIdx.x == 0) *active = static_cast<int>(s);
@@ -764,7 +765,10 @@ kk_vec(const BKVec* __restrict__ vecs, const BItem* __restrict__ items, int n, d
   const int lane = threadIdx.x & (kLanes - 1);
   double acc = 0.0;
   const KScalars* sc = v.sc;
-  if (!done(sc)) {
+  // a restart pending after a breakdown copies the residual into rh although the problem is
+  // marked broken down until the scalars are reset
+  const bool run = OP == KK_RESTART ? ldS(&sc->cycle_done, 0) != 0 && !ldS(&sc->done, 0) : !done(sc);
+  if (run) {
     const int nx = v.nx, ny = v.ny;
     S* r = static_cast<S*>(v.r);
     S* rh = static_cast<S*>(v.rh);
@@ -774,7 +778,7 @@ kk_vec(const BKVec* __restrict__ vecs, const BItem* __restrict__ items, int n, d
     S* ph = static_cast<S*>(v.ph);
     S* sh = static_cast<S*>(v.sh);
     const double alpha = ldS(&sc->alpha, 0), beta = ldS(&sc->beta, 0), omega = ldS(&sc->omega, 0);
-    const bool first = ldS(&sc->iter, 0) == 0;
+    const bool first = ldS(&sc->pad0, 0) == 0;   // first iteration since the (re)start
     const bool fs = ldS(&sc->flag_s, 0) != 0;
     const bool jac = v.jacobi != 0;
     for (int j = it.a; j < it.b && (OP != KK_XS || fs); ++j) {
@@ -783,8 +787,10 @@ kk_vec(const BKVec* __restrict__ vecs, const BItem* __restrict__ items, int n, d
       for (int i = lane; i < nx; i += kLanes) {
         const long long id = row + i;
         const bool bnd = bj || i == 0 || i == nx - 1;
-        if constexpr (OP == KK_RH) {
+        if constexpr (OP == KK_RH || OP == KK_RESTART) {
           stS(rh, id, ldS(r, id));
+        } else if constexpr (OP == KK_X0) {
+          stS(v.x, id, ldS(v.x0, id));
         } else if constexpr (OP == KK_RHO) {
           acc += double(ldS(rh, id)) * double(ldS(r, id));
         } else if constexpr (OP == KK_P) {
@@ -880,6 +886,57 @@ kk_apply(const BKVec* __restrict__ vecs, const BItem* __restrict__ items, int n,
   }
 }
 
+// A vanishing rho, r^.v, t.t or omega stops the single-problem BiCGStab (reference semantics).
+// The batch instead restarts the problem from its true residual (r^ = r, p = r), at most
+// kKkRestarts times: the rest of the iteration skips it (broken down), and kk_restart clears
+// the flag before the next one.  The residual is b - A x of the current x at every such point.
+// A non-finite dot product before the solution update (an overflowing FP32 recurrence) is
+// restarted the same way; a non-finite true residual ends the problem.
+constexpr int kKkRestarts = 8;
+__device__ __forceinline__ void kk_breakdown(KScalars* sc) {
+  sc->breakdown = 1;
+  if (sc->pad1 < kKkRestarts) sc->cycle_done = 1;
+}
+
+// Restart of every problem with a pending restart (after its rh = r copy): scalars of a fresh
+// start, one more restart counted.
+__global__ void kk_restart(const BKVec* __restrict__ vecs, int nprob) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nprob) return;
+  KScalars* sc = vecs[q].sc;
+  if (!sc->cycle_done || sc->done) return;
+  sc->cycle_done = 0;
+  sc->breakdown = 0;
+  sc->pad0 = 0;
+  sc->pad1 += 1;
+}
+
+// Number of problems that did not converge (not done, or broken down), one block.
+__global__ void __launch_bounds__(1024) kk_failed(const BKVec* __restrict__ vecs, int nprob,
+                                                 int* out) {
+  __shared__ double lds[1024 / kWave];
+  double a = 0.0;
+  for (int q = threadIdx.x; q < nprob; q += blockDim.x) {
+    const KScalars* sc = vecs[q].sc;
+    a += (sc->done && !sc->breakdown) ? 0.0 : 1.0;
+  }
+  const double t = block_sum(a, lds);
+  if (threadIdx.x == 0) *out = static_cast<int>(t);
+}
+
+__global__ void kk_reset_failed(const BKVec* __restrict__ vecs, int nprob, double eps) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nprob) return;
+  KScalars* sc = vecs[q].sc;
+  if (sc->done && !sc->breakdown) return;
+  KScalars k{};
+  k.eps = eps;
+  k.scale = 1.0;
+  k.inv_scale = 1.0;
+  k.steps = -1;
+  *sc = k;
+}
+
 // Segmented finalize of the batched BiCGStab: problem blockIdx.x sums its items' partials in a
 // fixed order and applies the single-problem path's scalar step (k_krylov.hip k_fin_k).
 __global__ void __launch_bounds__(256)
@@ -890,7 +947,7 @@ kk_finalize(const BKVec* __restrict__ vecs, const int* __restrict__ first,
   const int q = blockIdx.x;
   if (q >= nprob) return;
   KScalars* sc = vecs[q].sc;
-  if (op != KK_R0 && (sc->done | sc->breakdown)) return;
+  if (sc->done | sc->breakdown) return;
   const int f = first[q], c = cnt[q];
   double a1 = 0.0, a2 = 0.0;
   for (int i = threadIdx.x; i < c; i += blockDim.x) {
@@ -912,16 +969,16 @@ kk_finalize(const BKVec* __restrict__ vecs, const int* __restrict__ first,
     }
     case KK_FRHO:
       sc->rho = s1;
-      if (fabs(s1) < sc->eps) { sc->breakdown = 1; break; }
-      if (sc->iter > 0) sc->beta = (s1 / sc->rho_old) * (sc->alpha / sc->omega);
+      if (fabs(s1) < sc->eps || nonfinite(s1)) { kk_breakdown(sc); break; }
+      if (sc->pad0 > 0) sc->beta = (s1 / sc->rho_old) * (sc->alpha / sc->omega);
       break;
     case KK_FALPHA:
-      if (fabs(s1) < sc->eps) { sc->breakdown = 1; break; }
+      if (fabs(s1) < sc->eps || nonfinite(s1)) { kk_breakdown(sc); break; }
       sc->alpha = sc->rho / s1;
       break;
     case KK_FS:
       sc->snorm = sqrt(s1);
-      if (nonfinite(s1)) { sc->breakdown = 1; break; }
+      if (nonfinite(s1)) { kk_breakdown(sc); break; }
       if (sc->snorm < sc->target) sc->flag_s = 1;
       break;
     case KK_FOMEGA:
@@ -933,15 +990,16 @@ kk_finalize(const BKVec* __restrict__ vecs, const int* __restrict__ first,
       }
       sc->tt = s1;
       sc->ts = s2;
-      if (fabs(s1) < sc->eps) { sc->breakdown = 1; break; }
+      if (fabs(s1) < sc->eps || nonfinite(s1) || nonfinite(s2)) { kk_breakdown(sc); break; }
       sc->omega = s2 / s1;
       break;
     case KK_FR:
       sc->metric = sqrt(s1);
       sc->iter += 1;
+      sc->pad0 += 1;
       if (nonfinite(s1)) { sc->breakdown = 1; break; }
       if (sc->metric < sc->target) { sc->done = 1; break; }
-      if (fabs(sc->omega) < sc->eps) { sc->breakdown = 1; break; }
+      if (fabs(sc->omega) < sc->eps) { kk_breakdown(sc); break; }
       sc->rho_old = sc->rho;
       break;
     default:
@@ -1207,6 +1265,8 @@ void kvec(hipStream_t s, Prec p, int op, const BKVec* vecs, const BItem* items, 
     case KK_P: KV_(S, KK_P); break;              \
     case KK_S: KV_(S, KK_S); break;              \
     case KK_XS: KV_(S, KK_XS); break;            \
+    case KK_RESTART: KV_(S, KK_RESTART); break;  \
+    case KK_X0: KV_(S, KK_X0); break;            \
     default: KV_(S, KK_X); break;                \
   }
   if (p == P64) { KV_ALL(double) } else { KV_ALL(float) }
@@ -1225,6 +1285,18 @@ void kapply(hipStream_t s, Prec p, int mode, const BKVec* vecs, const BItem* ite
     if (mode == 0) KA_(float, 0); else if (mode == 1) KA_(float, 1); else KA_(float, 2);
   }
 #undef KA_
+}
+
+void kfailed(hipStream_t s, const BKVec* vecs, int nprob, int* out) {
+  hipLaunchKernelGGL(kk_failed, dim3(1), dim3(1024), 0, s, vecs, nprob, out);
+}
+
+void kreset_failed(hipStream_t s, const BKVec* vecs, int nprob, double eps) {
+  hipLaunchKernelGGL(kk_reset_failed, dim3((nprob + 255) / 256), dim3(256), 0, s, vecs, nprob, eps);
+}
+
+void krestart(hipStream_t s, const BKVec* vecs, int nprob) {
+  hipLaunchKernelGGL(kk_restart, dim3((nprob + 255) / 256), dim3(256), 0, s, vecs, nprob);
 }
 
 void kfinalize(hipStream_t s, const BKVec* vecs, const int* first, const int* cnt, int nprob,

@@ -104,6 +104,25 @@ def test_batch_bicgstab_mixed(ctx):
     assert stm == capi.OK
 
 
+def test_batch_bicgstab_mixed_completion_pass(ctx):
+    """Problems the FP32 pass leaves unconverged start again from x0 in FP64 (refinements = -1,
+    as the single-problem mixed path).  With an iteration cap far below what any problem needs,
+    neither pass converges, so the completion pass runs on every problem: finite solutions
+    reduced from the zero guess, status NOT_CONVERGED."""
+    specs = problems([65, 129, 33], seed=41)
+    opt = kopts(precision=capi.MIXED, tol=1e-10)
+    opt.max_iter = 4
+    st, us, reps = ctx.solve_batch(specs, opt, check=False)
+    for r, u in zip(reps, us):
+        print(f"  status {r.status} refinements {r.refinements} its {r.iterations} relres {r.rel_residual:.2e}",
+              flush=True)
+        assert r.status == capi.ERR_NOT_CONVERGED
+        assert r.refinements == -1
+        assert np.all(np.isfinite(u))
+        assert 0.0 < r.rel_residual < 1.0
+    assert st == capi.ERR_NOT_CONVERGED
+
+
 def test_batch_bicgstab_graph_and_resident(ctx):
     """The resident batch with graph replay gives the same bits as without, on two solves in a
     row; kernel timing runs on a BiCGStab batch and the next solve starts afresh."""
