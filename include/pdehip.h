@@ -287,7 +287,11 @@ int pdehip_solve_batch(pdehip_ctx* ctx, int count, const pdehip_problem* probs,
  * recomputed residual, FP64 or mixed precision): each operation of an iteration is one launch
  * over every unfinished problem.  No reference counterpart (the reference runs a batch as
  * sequential solves, src/main.cpp:2071-2097).  Mixed BiCGStab keeps the Krylov vectors and the
- * applies in FP32 and the solution and every iteration's residual b - A x in FP64.
+ * applies in FP32 and the solution and every iteration's residual b - A x in FP64; problems it
+ * leaves unconverged start again from the initial guess in FP64 (report.refinements = -1).
+ * A BiCGStab breakdown restarts the problem from its true residual (at most 8 times) instead of
+ * ending it.  options.max_iter is rounded up to whole polling chunks (2 MG-PCG / 8 BiCGStab
+ * iterations).
  * pdehip_batch_solve fills count reports and returns PDEHIP_OK when every problem converged,
  * else the status of the first one that did not; download copies problem `index`'s solution. */
 typedef struct pdehip_batch pdehip_batch;
