@@ -121,6 +121,7 @@ int pdehip_create(int device, size_t hbm_budget_bytes, pdehip_ctx** out) {
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
     ctx->c.budget = hbm_budget_bytes ? hbm_budget_bytes : static_cast<size_t>(0.8 * free_b);
+    ctx->c.pool_cap = ctx->c.budget / 8;   // released blocks kept for reuse (no hipFree syncs)
     HIPCHK(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
     return PDEHIP_OK;
   });

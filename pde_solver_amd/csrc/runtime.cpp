@@ -31,6 +31,14 @@ std::atomic<bool> g_mixed_fp16{true};
 void set_mixed_fp16(bool on) { g_mixed_fp16.store(on); }
 bool mixed_fp16() { return g_mixed_fp16.load(); }
 
+void Ctx::trim_pool() {
+  std::lock_guard<std::mutex> g(pool_mu);
+  for (auto& kv : pool)
+    for (void* p : kv.second) (void)hipFree(p);
+  pool.clear();
+  pooled = 0;
+}
+
 void DBuf::alloc(Ctx* c, size_t bytes) {
   release();
   if (bytes == 0) return;
@@ -43,10 +51,26 @@ void DBuf::alloc(Ctx* c, size_t bytes) {
   size_t pk = c->peak.load();
   while (now > pk && !c->peak.compare_exchange_weak(pk, now)) {}
   void* p = nullptr;
-  const hipError_t e = hipMalloc(&p, bytes);
-  if (e != hipSuccess) {
-    c->used.fetch_sub(bytes);
-    hip_check(e, "hipMalloc");
+  if (c->pool_cap) {
+    std::lock_guard<std::mutex> g(c->pool_mu);
+    auto it = c->pool.find(bytes);
+    if (it != c->pool.end() && !it->second.empty()) {
+      p = it->second.back();
+      it->second.pop_back();
+      c->pooled -= bytes;
+    }
+  }
+  if (!p) {
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess && c->pooled) {   // give the pooled blocks back and try once more
+      (void)hipGetLastError();
+      c->trim_pool();
+      e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess) {
+      c->used.fetch_sub(bytes);
+      hip_check(e, "hipMalloc");
+    }
   }
   ctx_ = c;
   p_ = p;
@@ -55,7 +79,16 @@ void DBuf::alloc(Ctx* c, size_t bytes) {
 
 void DBuf::release() {
   if (p_) {
-    (void)hipFree(p_);
+    bool kept = false;
+    if (ctx_->pool_cap) {
+      std::lock_guard<std::mutex> g(ctx_->pool_mu);
+      if (ctx_->pooled + bytes_ <= ctx_->pool_cap) {
+        ctx_->pool[bytes_].push_back(p_);
+        ctx_->pooled += bytes_;
+        kept = true;
+      }
+    }
+    if (!kept) (void)hipFree(p_);
     ctx_->used.fetch_sub(bytes_);
   }
   p_ = nullptr;
@@ -467,10 +500,7 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
 
   // ---- source, boundary data, right-hand side
   P->f_const = p.f;
-  if (p.f_field) {
-    upload(ctx, P->f, p.f_field, g.n, s);
-    P->hf.assign(p.f_field, p.f_field + g.n);
-  }
+  if (p.f_field) upload(ctx, P->f, p.f_field, g.n, s);
   const Launch L0{s, nullptr};
   P->x0.alloc(ctx, g.n * sizeof(double));
   fill(s, P64, P->x0.get(), g.n, 0.0);

@@ -45,6 +45,15 @@ struct Ctx {
   std::mutex err_mu;
   std::mutex tune_mu;                // batch work-item shapes picked by the autotuner
   std::map<std::string, std::vector<int>> batch_shapes;
+  // Released device blocks kept for reuse by size (up to pool_cap bytes; 0 = no pooling).
+  // Every buffer of a context is used on its stream only, so a block handed out again is
+  // ordered after its previous user's work; hipFree would synchronise the whole device.
+  std::mutex pool_mu;
+  std::map<size_t, std::vector<void*>> pool;
+  size_t pooled = 0;
+  size_t pool_cap = 0;
+  void trim_pool();
+  ~Ctx() { trim_pool(); }
   void set_error(const std::string& e) {
     std::lock_guard<std::mutex> g(err_mu);
     last_error = e;
@@ -156,7 +165,6 @@ struct Plan {
   bool fin_resid = false;      // plan_finish_enqueue staged the true-residual partials
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   size_t bytes_held = 0;
-  std::vector<double> hf;      // host copy of the source (set_source)
 
   ~Plan();
   KScalars* sc(int i = 0) const { return scal.as<KScalars>() + i; }

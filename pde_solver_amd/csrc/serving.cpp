@@ -238,6 +238,9 @@ pdehip_server* server_create(Ctx* ctx, const pdehip_server_config* cfg) {
     w->ctx.device = ctx->device;
     w->ctx.cus = ctx->cus;
     w->ctx.budget = 0;   // admission is the gate
+    // a worker's released plan buffers are reused by its next plans: a hipFree would
+    // synchronise the device, stalling every other worker's stream
+    w->ctx.pool_cap = s->budget / (8 * static_cast<size_t>(nw));
     (void)hipSetDevice(ctx->device);
     HIPCHK(hipStreamCreateWithFlags(&w->ctx.stream, hipStreamNonBlocking));
     s->ws.push_back(std::move(w));
