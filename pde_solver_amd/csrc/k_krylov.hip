@@ -141,21 +141,23 @@ __global__ void __launch_bounds__(kBlock) k_multidot(const S* __restrict__ V, lo
                                                     const KScalars* sc) {
   __shared__ double lds[kBlock / kWave];
   if (skip(sc) || sc->cycle_done) return;
+  // block (x, y): element blocks x of basis vectors 8y .. 8y+7, so every group of eight runs
+  // in parallel (gridDim.y = ceil(cnt / 8))
   const int nb = gridDim.x;
-  for (int i0 = 0; i0 < cnt; i0 += 8) {
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int m = min(8, cnt - i0);
-    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
-         q += (long long)gridDim.x * blockDim.x) {
-      const double wv = double(w[q]);
+  const int i0 = blockIdx.y * 8;
+  if (i0 >= cnt) return;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int m = min(8, cnt - i0);
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const double wv = double(w[q]);
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
-        if (t < m) acc[t] += double(V[(long long)(i0 + t) * ld + q]) * wv;
-    }
-    for (int t = 0; t < m; ++t) {
-      const double s = block_sum(acc[t], lds);
-      if (threadIdx.x == 0) partials[(long long)(i0 + t) * nb + blockIdx.x] = s;
-    }
+    for (int t = 0; t < 8; ++t)
+      if (t < m) acc[t] += double(V[(long long)(i0 + t) * ld + q]) * wv;
+  }
+  for (int t = 0; t < m; ++t) {
+    const double s = block_sum(acc[t], lds);
+    if (threadIdx.x == 0) partials[(long long)(i0 + t) * nb + blockIdx.x] = s;
   }
 }
 
@@ -174,16 +176,17 @@ __global__ void k_multiaxpy(const S* __restrict__ V, long long ld, int cnt,
 }
 
 // Sum the multi-dot partials into hcol[i] (accumulate when add != 0).
+// One block per dot product (blockIdx.x = i), each summing its nb partials in a fixed order.
 __global__ void __launch_bounds__(256) k_fin_multi(const double* partials, int nb, int cnt,
                                                   double* hcol, int add, const KScalars* sc) {
   __shared__ double lds[256 / kWave];
   if (skip(sc) || sc->cycle_done) return;
-  for (int i = 0; i < cnt; ++i) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[(long long)i * nb + b];
-    const double s = block_sum(acc, lds);
-    if (threadIdx.x == 0) hcol[i] = add ? hcol[i] + s : s;
-  }
+  const int i = blockIdx.x;
+  if (i >= cnt) return;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[(long long)i * nb + b];
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) hcol[i] = add ? hcol[i] + s : s;
 }
 
 // H[j+1][j] = ||w||, V_{j+1} = w / H[j+1][j]
@@ -388,7 +391,8 @@ template void jacobi_pc<float>(hipStream_t, const DevOp&, const float*, float*, 
 
 template <typename S>
 void multidot(hipStream_t s, const S* V, long long ld, int cnt, const S* w, long long n, double* partials, int nb, const KScalars* sc) {
-  hipLaunchKernelGGL(k_multidot<S>, dim3(nb), dim3(kBlock), 0, s, V, ld, cnt, w, n, partials, sc);
+  hipLaunchKernelGGL(k_multidot<S>, dim3(nb, std::max(1, (cnt + 7) / 8)), dim3(kBlock), 0, s, V, ld,
+                     cnt, w, n, partials, sc);
 }
 template <typename S>
 void multiaxpy(hipStream_t s, const S* V, long long ld, int cnt, const double* h, S* w, long long n, double sign, int check_cycle, const KScalars* sc) {
@@ -399,7 +403,8 @@ template void multidot<float>(hipStream_t, const float*, long long, int, const f
 template void multiaxpy<double>(hipStream_t, const double*, long long, int, const double*, double*, long long, double, int, const KScalars*);
 template void multiaxpy<float>(hipStream_t, const float*, long long, int, const double*, float*, long long, double, int, const KScalars*);
 void fin_multi(hipStream_t s, const double* partials, int nb, int cnt, double* hcol, int add, const KScalars* sc) {
-  hipLaunchKernelGGL(k_fin_multi, dim3(1), dim3(256), 0, s, partials, nb, cnt, hcol, add, sc);
+  hipLaunchKernelGGL(k_fin_multi, dim3(std::max(1, cnt)), dim3(256), 0, s, partials, nb, cnt, hcol,
+                     add, sc);
 }
 template <typename S>
 void gm_normalize(hipStream_t s, S* vnext, const S* w, long long n, const double* hval, const KScalars* sc) {
