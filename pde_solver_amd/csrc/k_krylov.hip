@@ -275,19 +275,8 @@ __global__ void k_gm_reset(KScalars* sc) {
   if (threadIdx.x == 0) { sc->cycle_done = 0; sc->steps = 0; }
 }
 
-// Stage-2 reductions for BiCGStab and the true-residual checks.
-__global__ void __launch_bounds__(1024) k_fin_k(KScalars* sc, const double* p1, const double* p2,
-                                               int np, int op) {
-  __shared__ double lds[1024 / kWave];
-  if (skip(sc)) return;
-  double a1 = 0.0, a2 = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) {
-    a1 += p1[i];
-    if (p2) a2 += p2[i];
-  }
-  const double s1 = block_sum(a1, lds);
-  const double s2 = p2 ? block_sum(a2, lds) : 0.0;
-  if (threadIdx.x != 0) return;
+// The scalar step after the BiCGStab / GMRES reductions (s1, s2: the reduced sums).
+__device__ void fin_apply(KScalars* sc, double s1, double s2, int op) {
   switch (op) {
     case KFIN_BI_RHO:
       sc->rho = s1;
@@ -323,6 +312,69 @@ __global__ void __launch_bounds__(1024) k_fin_k(KScalars* sc, const double* p1, 
       if (sc->metric < sc->target) sc->done = 1;
       break;
   }
+}
+
+// Stage-2 reductions for BiCGStab and the true-residual checks.
+__global__ void __launch_bounds__(1024) k_fin_k(KScalars* sc, const double* p1, const double* p2,
+                                               int np, int op) {
+  __shared__ double lds[1024 / kWave];
+  if (skip(sc)) return;
+  double a1 = 0.0, a2 = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    a1 += p1[i];
+    if (p2) a2 += p2[i];
+  }
+  const double s1 = block_sum(a1, lds);
+  const double s2 = p2 ? block_sum(a2, lds) : 0.0;
+  if (threadIdx.x == 0) fin_apply(sc, s1, s2, op);
+}
+
+// Jacobi scaling of one node: in / folded diagonal on interior rows, identity on boundary rows
+// (the k_jacobi_pc rule).
+template <int DIM, int KIND, typename S>
+__device__ __forceinline__ S jacobi_node(const DevOp& op, long long id, S in) {
+  const IJK c = coords(op, id);
+  if (is_boundary2(op, c.i, c.j) || (DIM == 3 && (c.k == 0 || c.k == op.nz - 1))) return in;
+  W7<double> w = row_weights<DIM, KIND, double, double>(op, id);
+  fold_center<DIM, double>(op, c.i, c.j, c.k, w);
+  return static_cast<S>(double(in) / w.C);
+}
+
+// p = r + beta (p - omega v) and, with the Jacobi preconditioner, ph = D^-1 p in the same pass.
+template <int DIM, int KIND, bool PC, typename S>
+__global__ void __launch_bounds__(kBlock) k_bicg_p_pc(DevOp op, const S* __restrict__ r, S* __restrict__ p,
+                                                     const S* __restrict__ v, S* __restrict__ ph,
+                                                     const KScalars* sc) {
+  if (skip(sc)) return;
+  const bool first = sc->iter == 0;
+  const double beta = sc->beta, omega = sc->omega;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < op.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const S pi = static_cast<S>(first ? double(r[i]) : double(r[i]) + beta * (double(p[i]) - omega * double(v[i])));
+    p[i] = pi;
+    if constexpr (PC) ph[i] = jacobi_node<DIM, KIND, S>(op, i, pi);
+  }
+}
+
+// s = r - alpha v with partials of s.s and, with the Jacobi preconditioner, sh = D^-1 s in the
+// same pass.
+template <int DIM, int KIND, bool PC, typename S>
+__global__ void __launch_bounds__(kBlock) k_bicg_s_pc(DevOp op, const S* __restrict__ r,
+                                                     const S* __restrict__ v, S* __restrict__ sv,
+                                                     S* __restrict__ sh, double* partials, const KScalars* sc) {
+  __shared__ double lds[kBlock / kWave];
+  if (skip(sc)) return;
+  const double alpha = sc->alpha;
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < op.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const S si = static_cast<S>(double(r[i]) - alpha * double(v[i]));
+    sv[i] = si;
+    if constexpr (PC) sh[i] = jacobi_node<DIM, KIND, S>(op, i, si);
+    acc += double(si) * double(si);
+  }
+  const double t = block_sum(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
 __global__ void k_bi_sdone(KScalars* sc) {
@@ -374,6 +426,40 @@ void bi_sdone(hipStream_t s, KScalars* sc) { hipLaunchKernelGGL(k_bi_sdone, dim3
 void fin_k(hipStream_t s, KScalars* sc, const double* p1, const double* p2, int np, int op) {
   hipLaunchKernelGGL(k_fin_k, dim3(1), dim3(1024), 0, s, sc, p1, p2, np, op);
 }
+
+// Kernel variants by (dimension, operator kind) when the Jacobi scaling is fused in.
+#define PC_DISPATCH(KER, ...)                                                                  \
+  do {                                                                                         \
+    const dim3 g_(blocks_for(op.n)), b_(kBlock);                                               \
+    const bool d3_ = op.nz > 1;                                                                \
+    if (!pc) hipLaunchKernelGGL((KER<2, PDEHIP_OP_CONST, false, S>), g_, b_, 0, s, __VA_ARGS__); \
+    else if (op.kind == PDEHIP_OP_POINTWISE) {                                                 \
+      if (d3_) hipLaunchKernelGGL((KER<3, PDEHIP_OP_POINTWISE, true, S>), g_, b_, 0, s, __VA_ARGS__); \
+      else hipLaunchKernelGGL((KER<2, PDEHIP_OP_POINTWISE, true, S>), g_, b_, 0, s, __VA_ARGS__); \
+    } else if (op.kind == PDEHIP_OP_FLUX) {                                                    \
+      if (d3_) hipLaunchKernelGGL((KER<3, PDEHIP_OP_FLUX, true, S>), g_, b_, 0, s, __VA_ARGS__); \
+      else hipLaunchKernelGGL((KER<2, PDEHIP_OP_FLUX, true, S>), g_, b_, 0, s, __VA_ARGS__);   \
+    } else {                                                                                   \
+      if (d3_) hipLaunchKernelGGL((KER<3, PDEHIP_OP_CONST, true, S>), g_, b_, 0, s, __VA_ARGS__); \
+      else hipLaunchKernelGGL((KER<2, PDEHIP_OP_CONST, true, S>), g_, b_, 0, s, __VA_ARGS__);  \
+    }                                                                                          \
+  } while (0)
+template <typename S>
+void bicg_p_pc(hipStream_t s, const DevOp& op, const S* r, S* p, const S* v, S* ph, bool pc, const KScalars* sc) {
+  PC_DISPATCH(k_bicg_p_pc, op, r, p, v, ph, sc);
+}
+template <typename S>
+void bicg_s_pc(hipStream_t s, const DevOp& op, const S* r, const S* v, S* sv, S* sh, bool pc,
+               double* partials, const KScalars* sc) {
+  PC_DISPATCH(k_bicg_s_pc, op, r, v, sv, sh, partials, sc);
+}
+#undef PC_DISPATCH
+#define INST_(S)                                                                                          \
+  template void bicg_p_pc<S>(hipStream_t, const DevOp&, const S*, S*, const S*, S*, bool, const KScalars*); \
+  template void bicg_s_pc<S>(hipStream_t, const DevOp&, const S*, const S*, S*, S*, bool, double*, const KScalars*);
+INST_(double)
+INST_(float)
+#undef INST_
 
 template <typename S>
 void jacobi_pc(hipStream_t s, const DevOp& op, const S* in, S* out, const KScalars* sc) {

@@ -21,6 +21,10 @@ template <typename S> void bicg_x(hipStream_t s, double* x, const S* ph, const S
 void bicg_r(hipStream_t s, const float* sv, const float* t, float* r, long long n, double* partials, const KScalars* sc);
 void bi_sdone(hipStream_t s, KScalars* sc);
 void fin_k(hipStream_t s, KScalars* sc, const double* p1, const double* p2, int np, int op);
+// bicg_p_pc = bicg_p, plus the Jacobi scaling of p into ph when pc; bicg_s_pc = bicg_s, plus
+// the Jacobi scaling of s into sh when pc (one pass over the vectors instead of two).
+template <typename S> void bicg_p_pc(hipStream_t s, const DevOp& op, const S* r, S* p, const S* v, S* ph, bool pc, const KScalars* sc);
+template <typename S> void bicg_s_pc(hipStream_t s, const DevOp& op, const S* r, const S* v, S* sv, S* sh, bool pc, double* partials, const KScalars* sc);
 template <typename S> void jacobi_pc(hipStream_t s, const DevOp& op, const S* in, S* out, const KScalars* sc);
 template <typename S> void multidot(hipStream_t s, const S* V, long long ld, int cnt, const S* w, long long n, double* partials, int nb, const KScalars* sc);
 template <typename S> void multiaxpy(hipStream_t s, const S* V, long long ld, int cnt, const double* h, S* w, long long n, double sign, int check_cycle, const KScalars* sc);

@@ -1050,16 +1050,14 @@ static void solve_bicgstab(Plan& P, pdehip_report* rep) {
     for (int c = 0; c < chunk; ++c) {
       dot(L, P64, rhat, r, n, part);
       fin_k(s, sc, part, nullptr, vec_blocks(n), KFIN_BI_RHO);
-      bicg_p(s, r, pv, v, n, sc);
-      if (pc) jacobi_pc(s, op, pv, ph, sc);
+      bicg_p_pc(s, op, r, pv, v, ph, pc, sc);     // with the Jacobi scaling of p
       apply(L, op, P64, ph, v, nullptr);
       dot(L, P64, rhat, v, n, part);
       fin_k(s, sc, part, nullptr, vec_blocks(n), KFIN_BI_ALPHA);
-      bicg_s(s, r, v, sv, n, part, sc);
+      bicg_s_pc(s, op, r, v, sv, shv, pc, part, sc);   // with the Jacobi scaling of s
       fin_k(s, sc, part, nullptr, kb, KFIN_BI_S);
       bicg_xs(s, x, ph, n, sc);
       bi_sdone(s, sc);
-      if (pc) jacobi_pc(s, op, sv, shv, sc);
       apply(L, op, P64, shv, t, nullptr);
       dot2(s, t, t, sv, n, part, part2, sc);
       fin_k(s, sc, part, part2, kb, KFIN_BI_OMEGA);
@@ -1219,16 +1217,14 @@ static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
       for (int c = 0; c < chunk; ++c) {
         dot(Li, P32, rhat, r, n, part);
         fin_k(s, si, part, nullptr, vec_blocks(n), KFIN_BI_RHO);
-        bicg_p<float>(s, r, pv, v, n, si);
-        if (pc) jacobi_pc<float>(s, op64, pv, ph, si);
+        bicg_p_pc<float>(s, op64, r, pv, v, ph, pc, si);
         apply(Li, op32, P32, ph, v, nullptr);
         dot(Li, P32, rhat, v, n, part);
         fin_k(s, si, part, nullptr, vec_blocks(n), KFIN_BI_ALPHA);
-        bicg_s<float>(s, r, v, sv, n, part, si);
+        bicg_s_pc<float>(s, op64, r, v, sv, sh, pc, part, si);
         fin_k(s, si, part, nullptr, kb, KFIN_BI_S);
         bicg_xs<float>(s, x, ph, n, si, &so->r0);
         bi_sdone(s, si);
-        if (pc) jacobi_pc<float>(s, op64, sv, sh, si);
         apply(Li, op32, P32, sh, t, nullptr);
         dot2<float>(s, t, t, sv, n, part, part2, si);
         fin_k(s, si, part, part2, kb, KFIN_BI_OMEGA);

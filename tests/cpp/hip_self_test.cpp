@@ -247,6 +247,70 @@ int main() {
               << " frames=" << frames_seen << " " << out.error << "\n";
     if (!pass) ++g_fail;
   }
+  // Restart equivalence (tests/checkpoint_equivalence.cpp): a series restarted from frame 2's
+  // grid (and velocity, for u_tt) at t_start = that frame's time reproduces the remaining frames
+  // bit for bit.  Heat with RK4 and Neumann/Robin faces; wave (u_tt) with forward Euler.
+  for (int wave = 0; wave < 2; ++wave) {
+    ++g_total;
+    const int n = 17, total = 7, ck = 2;
+    SolveInput in;
+    in.domain = Unit(n);
+    in.pde.a = -1.0; in.pde.b = -1.0;
+    if (wave) in.pde.utt = 1.0; else in.pde.ut = 1.0;
+    in.pde.f = 0.5;
+    in.bc = DirichletAll(Linear(0.1, 0.2, 0.0));
+    in.bc.left.kind = BCKind::Neumann;
+    in.bc.left.value = Linear(0.3, 0.0, 0.0);
+    in.bc.right.kind = BCKind::Robin;
+    in.bc.right.alpha = Linear(1.0, 0.0, 0.0);
+    in.bc.right.beta = Linear(0.5, 0.0, 0.0);
+    in.bc.right.gamma = Linear(0.2, 0.0, 0.0);
+    in.initial_grid.resize(n * n);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i)
+        in.initial_grid[j * n + i] = std::sin(M_PI * i / (n - 1.0)) * std::cos(1.3 * j / (n - 1.0));
+    // an initial grid for u_tt needs a velocity as well (solver.cpp:1111)
+    if (wave) in.initial_velocity.assign(in.initial_grid.begin(), in.initial_grid.end());
+    in.time.enabled = true;
+    in.time.t_start = 0.25;
+    in.time.dt = wave ? 0.2 / (n - 1.0) : 0.2 / ((n - 1.0) * (n - 1.0));
+    in.time.frames = total;
+    in.time.integrator = wave ? TimeIntegrator::ForwardEuler : TimeIntegrator::RK4;
+    std::vector<std::vector<double>> frames;
+    std::vector<double> ck_vel;
+    double ck_t = 0.0;
+    SolveOutput base = SolvePDETimeSeriesHip(in, [&](int fr, double t, const std::vector<double>& g,
+                                                     const std::vector<double>* v) {
+      frames.push_back(g);
+      if (fr == ck) {
+        ck_t = t;
+        if (v) ck_vel = *v;
+      }
+      return true;
+    });
+    SolveInput re = in;
+    re.time.t_start = ck_t;
+    re.time.frames = total - ck;
+    re.initial_grid = frames.size() > ck ? frames[ck] : std::vector<double>();
+    re.initial_velocity = ck_vel;
+    std::vector<std::vector<double>> again;
+    SolveOutput out = SolvePDETimeSeriesHip(re, [&](int, double, const std::vector<double>& g,
+                                                    const std::vector<double>*) {
+      again.push_back(g);
+      return true;
+    });
+    bool same = base.error.empty() && out.error.empty() && (int)frames.size() == total &&
+                (int)again.size() == total - ck && (!wave || ck_vel.size() == (size_t)n * n);
+    double moved = 0.0;
+    for (size_t q = 0; same && q < again.size(); ++q) {
+      same = again[q] == frames[ck + q];
+      for (size_t i = 0; i < again[q].size(); ++i) moved = std::max(moved, std::abs(again[q][i] - frames[0][i]));
+    }
+    const bool pass = same && moved > 1e-6;   // the state does evolve
+    std::cout << (pass ? "[OK] " : "[FAIL] ") << "restart equivalence (" << (wave ? "u_tt, FE" : "u_t, RK4")
+              << ") frames " << frames.size() << "/" << again.size() << " " << base.error << out.error << "\n";
+    if (!pass) ++g_fail;
+  }
   std::cout << (g_total - g_fail) << "/" << g_total << " passed\n";
   return g_fail == 0 ? 0 : 1;
 }
