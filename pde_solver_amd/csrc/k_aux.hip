@@ -205,37 +205,32 @@ __device__ __forceinline__ double fw2_in(const SF* __restrict__ pl, int nx, int 
   return 0.25 * a0 + 0.5 * a1 + 0.25 * a2;
 }
 
-template <Prec PF, Prec PC>
-__global__ void __launch_bounds__(kTX * kTY) k_restrict3(DevOp f, const void* rfv, DevOp c,
-                                                        void* bcv, double scale,
-                                                        const double* dscale,
-                                                        const KScalars* sc) {
-  using SF = typename PT<PF>::S;
-  using SC = typename PT<PC>::S;
-  if (skip(sc)) return;
-  const int ic = blockIdx.x * kTX + threadIdx.x;
-  const int jc = blockIdx.y * kTY + threadIdx.y;
-  const int kc = blockIdx.z;
-  if (ic >= c.nx || jc >= c.ny) return;
-  SC* bc = static_cast<SC*>(bcv);
+// One coarse node of the 3D restriction.  RIM: the node may sit next to an x/y face, so the
+// folded closure terms are taken through the general path; interior nodes (RIM false) never
+// need it, and their waves carry no folded lanes.
+template <typename SF, typename SC, bool RIM>
+__device__ __forceinline__ void restrict3_node(const DevOp& f, const SF* __restrict__ rf, const DevOp& c,
+                                               SC* __restrict__ bc, int ic, int jc, int kc, double s) {
   const long long out = ((long long)kc * c.ny + jc) * c.nx + ic;
   if (ic == 0 || jc == 0 || kc == 0 || ic == c.nx - 1 || jc == c.ny - 1 || kc == c.nz - 1) {
     bc[out] = store_as<SC>(0.0);
     return;
   }
-  const SF* rf = static_cast<const SF*>(rfv);
   const long long fsxy = (long long)f.nx * f.ny;
   const int fi = 2 * ic, fj = 2 * jc, fk = 2 * kc;
   double ql = fw2_in(rf + (fk - 1) * fsxy, f.nx, fi, fj);
   double qm = fw2_in(rf + fk * fsxy, f.nx, fi, fj);
   double qh = fw2_in(rf + (fk + 1) * fsxy, f.nx, fi, fj);
-  const bool fold_xy = ic == 1 || jc == 1 || ic == c.nx - 2 || jc == c.ny - 2;
+  bool fold_xy = false;
   int xi[3], yi[3];
   double xw[3], yw[3];
   int mx = 1, my = 1;
-  if (fold_xy) {
-    mx = fold_list(ic, c.nx, c.theta[0], c.theta[1], xi, xw);
-    my = fold_list(jc, c.ny, c.theta[2], c.theta[3], yi, yw);
+  if constexpr (RIM) {
+    fold_xy = ic == 1 || jc == 1 || ic == c.nx - 2 || jc == c.ny - 2;
+    if (fold_xy) {
+      mx = fold_list(ic, c.nx, c.theta[0], c.theta[1], xi, xw);
+      my = fold_list(jc, c.ny, c.theta[2], c.theta[3], yi, yw);
+    }
   }
   // in-plane folded weighting of fine plane fk (the general form: every fold-list pair)
   auto Q = [&](int k) -> double {
@@ -254,8 +249,58 @@ __global__ void __launch_bounds__(kTX * kTY) k_restrict3(DevOp f, const void* rf
     v += c.theta[4] * (0.5 * (fold_xy ? Q(0) : fw2_in(rf, f.nx, fi, fj)) + 0.25 * ql);
   if (kc == c.nz - 2 && c.theta[5] != 0.0)
     v += c.theta[5] * (0.25 * qh + 0.5 * (fold_xy ? Q(fk + 2) : fw2_in(rf + (fk + 2) * fsxy, f.nx, fi, fj)));
-  const double s = dscale ? scale * (*dscale) : scale;
   bc[out] = store_as<SC>(v * s);
+}
+
+// Rim of a coarse plane: the nodes outside the interior box [2, nx-3] x [2, ny-3], numbered
+// row by row (rows 0, 1, ny-2, ny-1 whole, then 4 nodes of every middle row); every node when
+// the box is empty.
+__device__ __forceinline__ bool rim_node(int nx, int ny, long long r, int* ic, int* jc) {
+  const int bx = nx - 4, by = ny - 4;
+  if (bx <= 0 || by <= 0) {
+    if (r >= (long long)nx * ny) return false;
+    *jc = (int)(r / nx); *ic = (int)(r % nx);
+    return true;
+  }
+  if (r < 2LL * nx) { *jc = (int)(r / nx); *ic = (int)(r % nx); return true; }
+  if (r < 4LL * nx) { const long long q = r - 2LL * nx; *jc = ny - 2 + (int)(q / nx); *ic = (int)(q % nx); return true; }
+  const long long q = r - 4LL * nx;
+  if (q >= 4LL * by) return false;
+  *jc = 2 + (int)(q / 4);
+  const int t = (int)(q % 4);
+  *ic = t < 2 ? t : nx - 4 + t;
+  return true;
+}
+
+// 3D restriction, one coarse node per thread (no z march, so no load waits on a carried
+// value): the 27-point full weighting of an interior coarse node reads fine nodes that are all
+// in range, so its 27 loads are issued unconditionally and wait once.  Blocks y < gy_in cover
+// the interior box; the blocks above it cover the rim, whose nodes next to a face add the
+// folded closure terms through the general path (fold lists, clipped weighting).
+template <Prec PF, Prec PC>
+__global__ void __launch_bounds__(kTX * kTY) k_restrict3(DevOp f, const void* rfv, DevOp c,
+                                                        void* bcv, double scale,
+                                                        const double* dscale,
+                                                        const KScalars* sc, int gy_in) {
+  using SF = typename PT<PF>::S;
+  using SC = typename PT<PC>::S;
+  if (skip(sc)) return;
+  const SF* rf = static_cast<const SF*>(rfv);
+  SC* bc = static_cast<SC*>(bcv);
+  const double s = dscale ? scale * (*dscale) : scale;
+  const int kc = blockIdx.z;
+  if ((int)blockIdx.y < gy_in) {
+    const int ic = 2 + blockIdx.x * kTX + threadIdx.x;
+    const int jc = 2 + blockIdx.y * kTY + threadIdx.y;
+    if (ic > c.nx - 3 || jc > c.ny - 3) return;
+    restrict3_node<SF, SC, false>(f, rf, c, bc, ic, jc, kc, s);
+  } else {
+    const long long r = ((long long)(blockIdx.y - gy_in) * gridDim.x + blockIdx.x) * (kTX * kTY) +
+                        threadIdx.y * kTX + threadIdx.x;
+    int ic, jc;
+    if (!rim_node(c.nx, c.ny, r, &ic, &jc)) return;
+    restrict3_node<SF, SC, true>(f, rf, c, bc, ic, jc, kc, s);
+  }
 }
 
 // 3D prolongation, one coarse interval per thread: fine planes 2kc and 2kc+1 of one (i, j)
@@ -635,9 +680,17 @@ void restrict_fw(const Launch& L, const DevOp& fine, Prec pf, const void* rf, co
                  Prec pc, void* bc, double scale, const double* dev_scale) {
   const int gx = (coarse.nx + kTX - 1) / kTX, gy = (coarse.ny + kTY - 1) / kTY;
   if (coarse.nz > 1) {
-    const dim3 g3(gx, gy, coarse.nz);
+    // interior box blocks, then rim blocks (rim_node numbering) stacked above them in y
+    const int bx = coarse.nx - 4, by = coarse.ny - 4;
+    const bool box = bx > 0 && by > 0;
+    const int gxi = box ? (bx + kTX - 1) / kTX : 1;
+    const int gy_in = box ? (by + kTY - 1) / kTY : 0;
+    const long long rim = box ? 4LL * coarse.nx + 4LL * by : (long long)coarse.nx * coarse.ny;
+    const long long per_row = (long long)gxi * kTX * kTY;
+    const int gy_rim = static_cast<int>((rim + per_row - 1) / per_row);
+    const dim3 g3(gxi, gy_in + gy_rim, coarse.nz);
     const dim3 b3(kTX, kTY);
-#define R3_(A, B) hipLaunchKernelGGL((k_restrict3<A, B>), g3, b3, 0, L.stream, fine, rf, coarse, bc, scale, dev_scale, L.sc)
+#define R3_(A, B) hipLaunchKernelGGL((k_restrict3<A, B>), g3, b3, 0, L.stream, fine, rf, coarse, bc, scale, dev_scale, L.sc, gy_in)
     if (pf == P64 && pc == P64) R3_(P64, P64);
     else if (pf == P32 && pc == P32) R3_(P32, P32);
     else if (pf == P32 && pc == P16) R3_(P32, P16);

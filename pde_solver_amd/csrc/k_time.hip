@@ -143,6 +143,36 @@ __global__ void __launch_bounds__(kBlock) k_resnorm(DevOp op, FaceData fd, const
   if (threadIdx.x == 0) { sums[blockIdx.x] = s; maxs[blockIdx.x] = m; }
 }
 
+// 3D form of k_resnorm (interior rows only): blocks take whole x rows (k, j) in turn, so node
+// coordinates come from one division per row instead of two 64-bit divisions per node.
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) k_resnorm3(DevOp op, const double* __restrict__ u,
+                                                    const double* f, double f_const, double* sums,
+                                                    double* maxs, int* degenerate) {
+  __shared__ double lds[kBlock / kWave];
+  double acc = 0.0, mx = 0.0;
+  const int rows_j = op.ny - 2;
+  const long long rows = (long long)rows_j * (op.nz - 2);
+  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int j = 1 + (int)(row % rows_j);
+    const int k = 1 + (int)(row / rows_j);
+    const long long base = (long long)k * op.sxy + (long long)j * op.nx;
+    for (int i = 1 + threadIdx.x; i <= op.nx - 2; i += blockDim.x) {
+      const long long id = base + i;
+      const W7<double> w = row_weights<3, KIND, double, double>(op, id);
+      if (fabs(w.C) < 1e-12) { *degenerate = 1; continue; }
+      double s = w.E * u[id + 1] + w.W * u[id - 1] + w.N * u[id + op.nx] + w.S * u[id - op.nx];
+      s = s + w.F * u[id + op.sxy] + w.B * u[id - op.sxy];
+      const double r = w.C * u[id] + s + (f ? f[id] : f_const);
+      acc += r * r;
+      mx = nmax(mx, fabs(r));
+    }
+  }
+  const double s = block_sum(acc, lds);
+  const double m = block_max(mx, lds);
+  if (threadIdx.x == 0) { sums[blockIdx.x] = s; maxs[blockIdx.x] = m; }
+}
+
 // Lexicographic Gauss-Seidel of the reference V-cycle (multigrid.cpp:69-94) for one 2D level:
 // one workgroup walks the anti-diagonals d = i + j in order, so every node sees its left and
 // lower neighbours already updated and its right and upper neighbours not yet, exactly the
@@ -378,9 +408,15 @@ void residual_norms(hipStream_t s, const DevOp& op, const FaceData& fd, const do
   const dim3 g(resnorm_blocks(op));
   const bool d3 = op.nz > 1;
 #define RN_(D, K) hipLaunchKernelGGL((k_resnorm<D, K>), g, dim3(kBlock), 0, s, op, fd, u, f, f_const, nl, sums, maxs, degenerate)
-  if (op.kind == PDEHIP_OP_POINTWISE) { if (d3) RN_(3, PDEHIP_OP_POINTWISE); else RN_(2, PDEHIP_OP_POINTWISE); }
+#define RN3_(K) hipLaunchKernelGGL((k_resnorm3<K>), g, dim3(kBlock), 0, s, op, u, f, f_const, sums, maxs, degenerate)
+  if (d3 && op.nx >= 3 && op.ny >= 3 && op.nz >= 3) {
+    if (op.kind == PDEHIP_OP_POINTWISE) RN3_(PDEHIP_OP_POINTWISE);
+    else if (op.kind == PDEHIP_OP_FLUX) RN3_(PDEHIP_OP_FLUX);
+    else RN3_(PDEHIP_OP_CONST);
+  } else if (op.kind == PDEHIP_OP_POINTWISE) { if (d3) RN_(3, PDEHIP_OP_POINTWISE); else RN_(2, PDEHIP_OP_POINTWISE); }
   else if (op.kind == PDEHIP_OP_FLUX) { if (d3) RN_(3, PDEHIP_OP_FLUX); else RN_(2, PDEHIP_OP_FLUX); }
   else { if (d3) RN_(3, PDEHIP_OP_CONST); else RN_(2, PDEHIP_OP_CONST); }
+#undef RN3_
 #undef RN_
 }
 
