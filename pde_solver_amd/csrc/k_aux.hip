@@ -303,9 +303,12 @@ __global__ void __launch_bounds__(kTX * kTY) k_restrict3(DevOp f, const void* rf
   }
 }
 
-// 3D prolongation, one coarse interval per thread: fine planes 2kc and 2kc+1 of one (i, j)
-// column (the last interior plane alone when nz is even), both fine loads and the coarse
-// loads of planes kc and kc+1 issued before any arithmetic.
+// 3D prolongation, kProlongZ coarse intervals per thread: fine planes 2kc .. 2kc+2*kProlongZ-1
+// of one (i, j) column (interior planes only) and the coarse planes kc .. kc+kProlongZ they lie
+// between.  Every load is issued before any arithmetic (plane indices clamped in range; a
+// value that is not needed is not stored), so a thread keeps 2*kProlongZ + 4*(kProlongZ+1)
+// loads in flight instead of 10.  The arithmetic of each fine node is the one-interval form.
+constexpr int kProlongZ = 4;
 template <Prec PC, Prec PF>
 __global__ void __launch_bounds__(kTX * kTY) k_prolong3(DevOp c, const void* ucv, DevOp f,
                                                        void* ufv, double factor,
@@ -316,14 +319,13 @@ __global__ void __launch_bounds__(kTX * kTY) k_prolong3(DevOp c, const void* ucv
   if (skip(sc)) return;
   const int i = blockIdx.x * kTX + threadIdx.x;
   const int j = blockIdx.y * kTY + threadIdx.y;
-  const int kc = blockIdx.z;                 // fine planes 2kc, 2kc + 1 (interior ones)
+  const int kc0 = blockIdx.z * kProlongZ;
   if (i < 1 || j < 1 || i > f.nx - 2 || j > f.ny - 2) return;
-  const int ka = max(1, 2 * kc), kb = min(2 * kc + 2, f.nz - 1);   // [ka, kb)
-  if (ka >= kb) return;
   const SC* uc = static_cast<const SC*>(ucv);
   SF* uf = static_cast<SF*>(ufv);
   const long long csxy = (long long)c.nx * c.ny;
   const long long fsxy = (long long)f.nx * f.ny;
+  const long long col = (long long)j * f.nx + i;
   const int ic = i >> 1, oi = i & 1, jc = j >> 1, oj = j & 1;
   int x0 = ic, x1 = ic + 1, y0 = jc, y1 = jc + 1;
   double wx0 = 1.0, wx1 = 1.0, wy0 = 1.0, wy1 = 1.0;
@@ -336,37 +338,59 @@ __global__ void __launch_bounds__(kTX * kTY) k_prolong3(DevOp c, const void* ucv
   if (!oj) wy1 = 0.0;
   x1 = min(x1, c.nx - 2);   // in range even where its weight is zero
   y1 = min(y1, c.ny - 2);
-  const long long id0 = ka * fsxy + (long long)j * f.nx + i;
-  const bool two = kb - ka == 2;
-  const double f0 = ldd(uf, id0);
-  const double f1 = two ? ldd(uf, id0 + fsxy) : 0.0;
-  // bilinear value of coarse plane kk through the z closure extension
-  auto P = [&](int kk) -> double {
-    double wz = 1.0;
-    if (kk == 0) { wz = c.theta[4]; kk = 1; } else if (kk == c.nz - 1) { wz = c.theta[5]; kk = c.nz - 2; }
+  // loads: coarse corners of planes kc0 .. kc0+kProlongZ (through the z closure extension),
+  // fine values of planes 2kc0 .. 2kc0+2kProlongZ-1
+  double l00[kProlongZ + 1], l01[kProlongZ + 1], l10[kProlongZ + 1], l11[kProlongZ + 1];
+  double wz[kProlongZ + 1];
+#pragma unroll
+  for (int q = 0; q <= kProlongZ; ++q) {
+    int kk = min(kc0 + q, c.nz - 1);
+    wz[q] = 1.0;
+    if (kk == 0) { wz[q] = c.theta[4]; kk = 1; } else if (kk == c.nz - 1) { wz[q] = c.theta[5]; kk = c.nz - 2; }
     const SC* pl = uc + kk * csxy;
-    const double l00 = ldd(pl + (long long)y0 * c.nx, x0), l01 = ldd(pl + (long long)y0 * c.nx, x1);
-    const double l10 = ldd(pl + (long long)y1 * c.nx, x0), l11 = ldd(pl + (long long)y1 * c.nx, x1);
+    l00[q] = ldd(pl + (long long)y0 * c.nx, x0);
+    l01[q] = ldd(pl + (long long)y0 * c.nx, x1);
+    l10[q] = ldd(pl + (long long)y1 * c.nx, x0);
+    l11[q] = ldd(pl + (long long)y1 * c.nx, x1);
+  }
+  double fv[2 * kProlongZ];
+#pragma unroll
+  for (int q = 0; q < 2 * kProlongZ; ++q) {
+    const int k = min(max(2 * kc0 + q, 1), f.nz - 2);
+    fv[q] = ldd(uf, k * fsxy + col);
+  }
+  // bilinear value of coarse plane kc0 + q
+  auto P = [&](int q) -> double {
     double a = 0.0;
     if (wy0 != 0.0) {
-      double v = wx0 != 0.0 ? wx0 * l00 : 0.0;
-      if (wx1 != 0.0) v += wx1 * l01;
+      double v = wx0 != 0.0 ? wx0 * l00[q] : 0.0;
+      if (wx1 != 0.0) v += wx1 * l01[q];
       a += wy0 * (v * hx);
     }
     if (wy1 != 0.0) {
-      double v = wx0 != 0.0 ? wx0 * l10 : 0.0;
-      if (wx1 != 0.0) v += wx1 * l11;
+      double v = wx0 != 0.0 ? wx0 * l10[q] : 0.0;
+      if (wx1 != 0.0) v += wx1 * l11[q];
       a += wy1 * (v * hx);
     }
-    return wz == 0.0 ? 0.0 : wz * (a * hy);
+    return wz[q] == 0.0 ? 0.0 : wz[q] * (a * hy);
   };
   const double s = dscale ? factor * (*dscale) : factor;
-  // fine plane k = 2kc + o sits at coarse kc (o = 0) or halfway to kc + 1 (o = 1)
-  const double p_lo = P(kc), p_hi = P(kc + 1);
-  const int o0 = ka & 1;
-  const double v0 = o0 ? 0.5 * (p_lo + p_hi) : p_lo;
-  uf[id0] = store_as<SF>(f0 + s * v0);
-  if (two) uf[id0 + fsxy] = store_as<SF>(f1 + s * (0.5 * (p_lo + p_hi)));
+  double p_lo = P(0);
+#pragma unroll
+  for (int q = 0; q < kProlongZ; ++q) {
+    const double p_hi = P(q + 1);
+    // fine plane 2kc + o sits at coarse kc (o = 0) or halfway to kc + 1 (o = 1)
+    const int kc = kc0 + q;
+    const int ka = max(1, 2 * kc), kb = min(2 * kc + 2, f.nz - 1);   // [ka, kb)
+    if (ka < kb) {
+      const int o0 = ka & 1;      // 1 only for kc = 0 (ka = 1)
+      const double v0 = o0 ? 0.5 * (p_lo + p_hi) : p_lo;
+      const double fa = o0 ? fv[2 * q + 1] : fv[2 * q];   // fine plane ka = 2kc + o0
+      uf[ka * fsxy + col] = store_as<SF>(fa + s * v0);
+      if (kb - ka == 2) uf[(ka + 1) * fsxy + col] = store_as<SF>(fv[2 * q + 1] + s * (0.5 * (p_lo + p_hi)));
+    }
+    p_lo = p_hi;
+  }
 }
 
 // fine interior += factor * trilinear interpolation of the coarse correction, whose face nodes
@@ -717,7 +741,8 @@ void prolong_add(const Launch& L, const DevOp& coarse, Prec pc, const void* uc, 
                  Prec pf, void* uf, double factor, const double* dev_scale) {
   const int gx = (fine.nx + kTX - 1) / kTX, gy = (fine.ny + kTY - 1) / kTY;
   if (fine.nz > 1) {
-    const dim3 g3(gx, gy, (fine.nz + 1) / 2);
+    const int ncz = (fine.nz + 1) / 2;   // coarse intervals (the last one holds no interior plane)
+    const dim3 g3(gx, gy, (ncz + kProlongZ - 1) / kProlongZ);
     const dim3 b3(kTX, kTY);
 #define P3_(A, B) hipLaunchKernelGGL((k_prolong3<A, B>), g3, b3, 0, L.stream, coarse, uc, fine, uf, factor, dev_scale, L.sc)
     if (pc == P64 && pf == P64) P3_(P64, P64);

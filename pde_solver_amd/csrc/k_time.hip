@@ -144,7 +144,9 @@ __global__ void __launch_bounds__(kBlock) k_resnorm(DevOp op, FaceData fd, const
 }
 
 // 3D form of k_resnorm (interior rows only): blocks take whole x rows (k, j) in turn, so node
-// coordinates come from one division per row instead of two 64-bit divisions per node.
+// coordinates come from one division per row instead of two 64-bit divisions per node.  (Four
+// rows in flight per thread measured slower: the FP64 face divisions of four nodes cost the
+// occupancy.)
 template <int KIND>
 __global__ void __launch_bounds__(kBlock) k_resnorm3(DevOp op, const double* __restrict__ u,
                                                     const double* f, double f_const, double* sums,
