@@ -754,16 +754,20 @@ static void solve_cg(Plan& P, bool use_mg, pdehip_report* rep) {
   while (launched < max_iter) {
     const int chunk = std::min(every, max_iter - launched);
     for (int c = 0; c < chunk; ++c) {
+      // The previous iteration's preconditioning and direction update open this one, so a
+      // chunk ends at the stopping test and no launch is queued behind the converged one.
+      if (launched + c > 0) {
+        if (use_mg) {
+          vcycle(P, L, P.r.get(), z);
+          dot(L, P64, P.r.get(), z, n, part);
+        }
+        finalize(L, sc, part, vec_blocks(n), FIN_CG_BETA, 0, 0, nullptr);
+        cg_xpay(L, P64, z, P.p.get(), n);
+      }
       apply(L, op, P64, P.p.get(), P.q.get(), part);
       finalize(L, sc, part, partial_blocks(op), FIN_CG_ALPHA, 0, 0, nullptr);
       cg_update(L, P64, P.x.get(), P.r.get(), P.p.get(), P.q.get(), n, part, nullptr);
       finalize(L, sc, part, vec_blocks(n), FIN_CG_RR, 0, 0, nullptr);
-      if (use_mg) {
-        vcycle(P, L, P.r.get(), z);
-        dot(L, P64, P.r.get(), z, n, part);
-      }
-      finalize(L, sc, part, vec_blocks(n), FIN_CG_BETA, 0, 0, nullptr);
-      cg_xpay(L, P64, z, P.p.get(), n);
     }
     launched += chunk;
     const KScalars& k = pull(P, 0);
@@ -825,6 +829,14 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
   while (launched < max_iter) {
     const int chunk = std::min(every, max_iter - launched);
     for (int c = 0; c < chunk; ++c) {
+      // preconditioning and direction update of the previous iteration first: a chunk ends
+      // at the replacement, the only place the solve can stop
+      if (launched + c > 0) {
+        vcycle(P, Li, r32, P.z32.get());
+        dot(Li, P32, r32, P.z32.get(), n, part);
+        finalize(Li, si, part, vec_blocks(n), FIN_CG_BETA, 0, 0, nullptr);
+        cg_xpay(Li, P32, P.z32.get(), P.p32.get(), n);
+      }
       apply(Li, op32, P32, P.p32.get(), P.q32.get(), part);
       finalize(Li, si, part, partial_blocks(op32), FIN_CG_ALPHA, 0, 0, nullptr);
       cg_update(Li, P32, x32, r32, P.p32.get(), P.q32.get(), n, part, part2);
@@ -835,10 +847,6 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
       residual(Lg, op64, P64, P.bvec.get(), x64, r64, part);
       to_f32_scaled_gated(s, r64, r32, n, &so->snorm, gate);
       finalize_ru(s, si, so, gate, part, partial_blocks(op64), FIN_RU_APPLY, delta);
-      vcycle(P, Li, r32, P.z32.get());
-      dot(Li, P32, r32, P.z32.get(), n, part);
-      finalize(Li, si, part, vec_blocks(n), FIN_CG_BETA, 0, 0, nullptr);
-      cg_xpay(Li, P32, P.z32.get(), P.p32.get(), n);
     }
     launched += chunk;
     const KScalars& k = pull(P, 1);
@@ -848,6 +856,8 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
   flush_f32(s, x64, x32, n, &so->snorm, so);
   const KScalars ki = pull(P, 1);
   const KScalars& ko = pull(P, 0);
+  // converged at a replacement: r64 and its partials are b - A x of the final x already
+  P.resid_fresh = ko.done && !ko.breakdown && !ki.breakdown;
   rep->iterations = ki.iter;
   rep->refinements = ko.steps;
   rep->inner_iterations = ki.iter;
@@ -1365,6 +1375,7 @@ void plan_solve(Plan& P, pdehip_report* rep) {
   init_scalars(P, 0, 0.0);
   init_scalars(P, 1, 0.0);
   HIPCHK(hipEventRecord(P.ev0, s));
+  P.resid_fresh = false;
   switch (P.opt.method) {
     case PDEHIP_JACOBI:
     case PDEHIP_GAUSS_SEIDEL:
@@ -1402,7 +1413,7 @@ void plan_finish_enqueue(Plan& P) {
   double* part = P.part.as<double>();
   const int mth = P.opt.method;
   P.fin_resid = P.r && mth != PDEHIP_JACOBI && mth != PDEHIP_GAUSS_SEIDEL && mth != PDEHIP_SOR;
-  if (P.fin_resid) residual(L0, P.op(), P64, P.bvec.get(), P.x.get(), P.r.get(), part);
+  if (P.fin_resid && !P.resid_fresh) residual(L0, P.op(), P64, P.bvec.get(), P.x.get(), P.r.get(), part);
   if (P.folded) {
     closures_nr(L0, P.op(), P.faces.fd, P.x.as<double>());
     closures_dirichlet(L0, P.op(), P.faces.fd, P.x.as<double>());

@@ -163,6 +163,7 @@ struct Plan {
   KScalars* hsc = nullptr;     // pinned host mirror
   double* hstage = nullptr;    // pinned staging of the report's reductions
   bool fin_resid = false;      // plan_finish_enqueue staged the true-residual partials
+  bool resid_fresh = false;    // the solver left r = b - A x of the final x and its partials
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   size_t bytes_held = 0;
 
