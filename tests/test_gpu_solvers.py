@@ -209,3 +209,26 @@ def test_mg_linear_solution_robin_neumann(ctx):
     x = np.linspace(0.0, 1.0, 33)
     assert np.abs(u - (1.0 + x)[None, :]).max() < 1e-11
     assert abs(rep.iterations - rep_o.iterations) <= 1
+
+
+@pytest.mark.parametrize("shape", [(65, 9, 17), (9, 33, 5), (33, 17, 9)])
+def test_mgpcg_fp64_nonsquare3d_matches_oracle(ctx, shape):
+    """Flattened boxes: coarse levels whose interior box (all but two nodes per face) is thin or
+    empty, so the restriction runs on its face-rim blocks alone."""
+    nx, ny, nz = shape
+    print(f"mgpcg fp64 nonsquare 3D {shape}", flush=True)
+    s = capi.Spec(nx, ny, nz, a=-1.0, op_kind=capi.OP_FLUX)
+    rng = np.random.default_rng(nx + 3 * ny + 7 * nz)
+    s.kappa = np.exp(0.3 * rng.standard_normal(s.shape))
+    s.f_field = rng.standard_normal(s.shape)
+    s.bcs = [BC.dirichlet(0.0), BC.robin(1.0, 1.0, 0.0), BC.dirichlet(0.1), BC.neumann(0.0),
+             BC.robin(1.0, 2.0, 0.0), BC.dirichlet(0.0)]
+    s.bc_convention = capi.BCCONV_OUTWARD
+    st_o, u_o, rep_o = O.solve(s, mgpcg())
+    assert st_o == 0
+    for precision, tol in ((capi.FP64, 1e-10), (capi.MIXED, 1e-8)):
+        st, u, rep = ctx.solve(s, mgpcg(tol=1e-12 if precision == capi.FP64 else 1e-10, precision=precision))
+        print(f"  precision {precision} its {rep.iterations} (oracle {rep_o.iterations}) rel {rel(u, u_o):.2e}",
+              flush=True)
+        assert st == 0
+        assert rel(u, u_o) < tol
