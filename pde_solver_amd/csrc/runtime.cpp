@@ -633,6 +633,18 @@ static void init_scalars(Plan& P, int i, double eps) {
   HIPCHK(hipMemcpyAsync(P.sc(i), &P.hsc[i], sizeof(KScalars), hipMemcpyHostToDevice, P.stream()));
 }
 
+// Scalar blocks 0 .. count-1 (eps[i] each) in one host-to-device copy.
+static void init_scalars_n(Plan& P, int count, const double* eps) {
+  for (int i = 0; i < count; ++i) {
+    KScalars k{};
+    k.eps = eps[i];
+    k.scale = 1.0;
+    k.inv_scale = 1.0;
+    P.hsc[i] = k;
+  }
+  HIPCHK(hipMemcpyAsync(P.sc(0), P.hsc, count * sizeof(KScalars), hipMemcpyHostToDevice, P.stream()));
+}
+
 void plan_init_scalars(Plan& P, int i, double eps) { init_scalars(P, i, eps); }
 
 static const KScalars& pull(Plan& P, int i) {
@@ -795,9 +807,8 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
   const DevOp& op32 = P.L[0].op32;
   double* part = P.part.as<double>();
   double* part2 = P.part2.as<double>();
-  init_scalars(P, 0, 1e-300);
-  init_scalars(P, 1, 1e-300);
-  init_scalars(P, 2, 0.0);
+  const double eps3[3] = {1e-300, 1e-300, 0.0};
+  init_scalars_n(P, 3, eps3);
   KScalars* so = P.sc(0);
   KScalars* si = P.sc(1);
   KScalars* gate = P.sc(2);   // done clear exactly when a replacement is due
@@ -1372,8 +1383,8 @@ static void solve_gmres_mixed(Plan& P, pdehip_report* rep) {
 void plan_solve(Plan& P, pdehip_report* rep) {
   std::memset(rep, 0, sizeof(*rep));
   const hipStream_t s = P.stream();
-  init_scalars(P, 0, 0.0);
-  init_scalars(P, 1, 0.0);
+  const double eps2[2] = {0.0, 0.0};
+  init_scalars_n(P, 2, eps2);
   HIPCHK(hipEventRecord(P.ev0, s));
   P.resid_fresh = false;
   switch (P.opt.method) {
