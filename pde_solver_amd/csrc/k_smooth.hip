@@ -319,11 +319,14 @@ k_rbgs_fused3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xi
 constexpr int kRing = 6;
 template <int N> using IC = std::integral_constant<int, N>;
 
-template <int KIND, Prec P, int R, bool ZERO>
+// DOT: also the partial of b . x_out over the nodes the workgroup stores (one per block; the
+// stored nodes of all blocks cover the grid once), for the CG dot product r . z when b is the
+// residual and x_out the preconditioned one.
+template <int KIND, Prec P, int R, bool ZERO, bool DOT = false>
 __global__ void __launch_bounds__(kThreads)
 k_rbgs_ring3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v,
              void* __restrict__ xout_v, int c1, int tiles_x, int tiles_y, int ntiles,
-             int kchunk, const KScalars* sc) {
+             int kchunk, const KScalars* sc, double* partials = nullptr) {
   using T = typename PT<P>::T;
   using S = typename PT<P>::S;
   static_assert(sizeof(T) == 4, "ring sweep: 4-byte arithmetic types only");
@@ -338,7 +341,12 @@ k_rbgs_ring3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin
   __shared__ T lfn[HK ? 3 : 1][KR + 2][PW];
   if (skip(sc)) return;
   const int t = xcd_tile();
-  if (t >= ntiles) return;
+  if (t >= ntiles) {
+    if constexpr (DOT) {
+      if (threadIdx.x == 0) partials[blockIdx.x] = 0.0;
+    }
+    return;
+  }
   const int tx = t % tiles_x;
   const int rest = t / tiles_x;
   const int ty = rest % tiles_y;
@@ -346,6 +354,7 @@ k_rbgs_ring3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin
   const int lane = threadIdx.x & (kLanes - 1);
   const int wv = threadIdx.x / kLanes;
   const int i = tx * kTileX - kHalo + lane;
+  double dacc = 0.0;   // DOT
   const int k0 = cz * kchunk;
   const int k1 = min(k0 + kchunk, op.nz);
   const int nz = op.nz;
@@ -498,6 +507,12 @@ k_rbgs_ring3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin
                                                    sizeof(S), pbytes);
 #pragma unroll
       for (int r = 0; r < R; ++r) BufIO<S>::st(ro, off_st[r], to_store<S, T>(U[r][u_m1]));
+      if constexpr (DOT) {
+        // U holds S-representable values (rounded updates or loaded x_in): the stored values
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (off_st[r] != kDropOffset) dacc += double(B[r][b_m1]) * double(U[r][u_m1]);
+      }
     }
   };
 
@@ -510,6 +525,11 @@ k_rbgs_ring3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin
     step(IC<3>{}, p + 3);
     step(IC<4>{}, p + 4);
     step(IC<5>{}, p + 5);
+  }
+  if constexpr (DOT) {
+    __shared__ double lred[kWaves];
+    const double tot = block_sum(dacc, lred);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
   }
 }
 
@@ -888,8 +908,8 @@ int ring_chunk(long long tiles, int nz, int capacity, int prologue) {
 }
 
 template <int KIND, Prec P>
-void launch3_ring(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout,
-                  int c1, bool zero_in) {
+int launch3_ring(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout,
+                 int c1, bool zero_in, double* partials = nullptr, int cap = 0) {
   constexpr int rows = 2;
   const int ty = rows * kWaves - 2 * kHalo;
   const int tiles_x = (op.nx + kTileX - 1) / kTileX;
@@ -902,12 +922,20 @@ void launch3_ring(const Launch& L, const DevOp& op, const void* b, const void* x
   const int chunks = (op.nz + kchunk - 1) / kchunk;
   const int ntiles = tiles_x * tiles_y * chunks;
   const dim3 g(static_cast<unsigned>(8 * ((ntiles + 7) / 8)));
+  if (partials) {
+    // the dot variant: one partial per block, when they fit the caller's buffer
+    if (zero_in || static_cast<int>(g.x) > cap) return 0;
+    hipLaunchKernelGGL((k_rbgs_ring3<KIND, P, rows, false, true>), g, dim3(kThreads), 0, L.stream,
+                       op, b, xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc, partials);
+    return static_cast<int>(g.x);
+  }
   if (zero_in)
     hipLaunchKernelGGL((k_rbgs_ring3<KIND, P, rows, true>), g, dim3(kThreads), 0, L.stream, op, b,
-                       xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc);
+                       xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc, nullptr);
   else
     hipLaunchKernelGGL((k_rbgs_ring3<KIND, P, rows, false>), g, dim3(kThreads), 0, L.stream, op, b,
-                       xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc);
+                       xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc, nullptr);
+  return 0;
 }
 
 template <int KIND, Prec P>
@@ -985,6 +1013,20 @@ void launch_kind(const Launch& L, const DevOp& op, Prec p, const void* b, const 
 
 void set_smooth_tuning(const SmoothTuning& t) { g_tune = t; }
 SmoothTuning smooth_tuning() { return g_tune; }
+
+int rbgs_sweep_dot(const Launch& L, const DevOp& op, Prec p, const void* b, const void* xin,
+                   void* xout, int first_color, double* partials, int cap) {
+  const bool ring = g_tune.ring != 0 && g_tune.rows == 2 && g_tune.pair == 0 &&
+                    op.sxy < (1LL << 29) && op.nz > 1 && (p == P32 || p == P16);
+  if (!ring) return 0;
+  if (op.kind == PDEHIP_OP_CONST)
+    return p == P32 ? launch3_ring<PDEHIP_OP_CONST, P32>(L, op, b, xin, xout, first_color, false, partials, cap)
+                    : launch3_ring<PDEHIP_OP_CONST, P16>(L, op, b, xin, xout, first_color, false, partials, cap);
+  if (op.kind == PDEHIP_OP_FLUX)
+    return p == P32 ? launch3_ring<PDEHIP_OP_FLUX, P32>(L, op, b, xin, xout, first_color, false, partials, cap)
+                    : launch3_ring<PDEHIP_OP_FLUX, P16>(L, op, b, xin, xout, first_color, false, partials, cap);
+  return 0;
+}
 
 void rbgs_sweep(const Launch& L, const DevOp& op, Prec p, const void* b, const void* xin,
                 void* xout, int first_color, bool zero_in) {

@@ -44,6 +44,11 @@ void rbgs(const Launch& L, const DevOp& op, Prec p, const void* b, void* x, int 
 // is known to be zero everywhere (its loads are skipped; x_in may be null).  (k_smooth.hip)
 void rbgs_sweep(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x_in,
                 void* x_out, int first_color, bool zero_in);
+// rbgs_sweep (x_in not zero) that also leaves the partials of b . x_out, one per block, when
+// the 3D FP32/FP16 ring sweep serves this level and its grid fits `cap` partials.  Returns the
+// number of partials written, or 0 after launching nothing (the caller sweeps and dots apart).
+int rbgs_sweep_dot(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x_in,
+                   void* x_out, int first_color, double* partials, int cap);
 // Launch shape of the fused sweep (process-wide).
 struct SmoothTuning {
   int fused = 1;     // 0: multigrid smooths with the two-launch in-place sweep

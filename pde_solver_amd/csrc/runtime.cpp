@@ -666,12 +666,20 @@ static int check_every(const Plan& P) {
 // `sweeps` red-black sweeps on level lv (colour c1 first in each sweep), result in u.  Pairs of
 // sweeps run as fused out-of-place sweeps u -> alt -> u; an odd last sweep runs in place.
 // zero: u is known to be zero on entry (it need not hold zeros when sweeps >= 2).
+// dot (optional): the last sweep, when it is a fused one, also leaves the partials of b . u
+// there; *dot_np = their count (0: none written).
 static void smooth(const Launch& L, Level& lv, const DevOp& op, const void* b, void* u,
-                   int sweeps, int c1, bool zero) {
+                   int sweeps, int c1, bool zero, double* dot = nullptr, int dot_cap = 0,
+                   int* dot_np = nullptr) {
   int done = 0;
+  if (dot_np) *dot_np = 0;
   if (lv.alt && smooth_tuning().fused) {
     for (; done + 2 <= sweeps; done += 2) {
       rbgs_sweep(L, op, lv.prec, b, u, lv.alt.get(), c1, zero);
+      if (dot && done + 2 == sweeps) {
+        *dot_np = rbgs_sweep_dot(L, op, lv.prec, b, lv.alt.get(), u, c1, dot, dot_cap);
+        if (*dot_np > 0) { zero = false; continue; }
+      }
       rbgs_sweep(L, op, lv.prec, b, lv.alt.get(), u, c1, false);
       zero = false;
     }
@@ -684,8 +692,11 @@ static void smooth(const Launch& L, Level& lv, const DevOp& op, const void* b, v
 }
 
 // One V-cycle on b0 (level-0 right-hand side) into u0 (level-0 correction from a zero guess;
-// u0's contents on entry are ignored).
-static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0) {
+// u0's contents on entry are ignored).  dot (optional): partials of b0 . u0 left by the last
+// level-0 sweep, *dot_np of them (0 when that sweep could not produce them).
+static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0, double* dot = nullptr,
+                   int* dot_np = nullptr) {
+  if (dot_np) *dot_np = 0;
   const int nl = static_cast<int>(P.L.size());
   const int pre = std::max(0, P.opt.mg_pre), post = std::max(0, P.opt.mg_post);
   KScalars* sc = const_cast<KScalars*>(L.sc);
@@ -724,7 +735,8 @@ static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0) {
     void* ul = l == 0 ? u0 : f.u.get();
     prolong_add(L, co, c.prec, c.u.get(), fo, f.prec, ul, c.sigma / f.sigma,
                 (P.mixed && l == 0) ? &sc->inv_scale : nullptr);
-    smooth(L, f, fo, bl, ul, post, 1, false);
+    if (l == 0 && dot) smooth(L, f, fo, bl, ul, post, 1, false, dot, P.npart, dot_np);
+    else smooth(L, f, fo, bl, ul, post, 1, false);
   }
 }
 
@@ -830,9 +842,18 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
   finalize(Li, si, part, vec_blocks(n), FIN_R0, 0.0, PDEHIP_TOL_ABS, nullptr);
   maxabs32(s, r32, n, part2);
   fin_scale(s, part2, maxabs_blocks(n), si);
-  vcycle(P, Li, r32, P.z32.get());
-  dot(Li, P32, r32, P.z32.get(), n, part);
-  finalize(Li, si, part, vec_blocks(n), FIN_CG_INIT, 0, 0, nullptr);
+  // r . z comes from the V-cycle's last sweep where it can (else one dot launch)
+  auto precondition = [&]() -> int {
+    int np = 0;
+    vcycle(P, Li, r32, P.z32.get(), part, &np);
+    if (np == 0) {
+      dot(Li, P32, r32, P.z32.get(), n, part);
+      np = vec_blocks(n);
+    }
+    return np;
+  };
+  int npz = precondition();
+  finalize(Li, si, part, npz, FIN_CG_INIT, 0, 0, nullptr);
   HIPCHK(hipMemcpyAsync(P.p32.get(), P.z32.get(), n * sizeof(float), hipMemcpyDeviceToDevice, s));
   const int every = check_every(P);
   const int max_iter = std::max(1, P.opt.max_iter);
@@ -843,9 +864,8 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
       // preconditioning and direction update of the previous iteration first: a chunk ends
       // at the replacement, the only place the solve can stop
       if (launched + c > 0) {
-        vcycle(P, Li, r32, P.z32.get());
-        dot(Li, P32, r32, P.z32.get(), n, part);
-        finalize(Li, si, part, vec_blocks(n), FIN_CG_BETA, 0, 0, nullptr);
+        npz = precondition();
+        finalize(Li, si, part, npz, FIN_CG_BETA, 0, 0, nullptr);
         cg_xpay(Li, P32, P.z32.get(), P.p32.get(), n);
       }
       apply(Li, op32, P32, P.p32.get(), P.q32.get(), part);
