@@ -88,3 +88,30 @@ def test_mixed_bicgstab_diffusion_dominated_completes(ctx):
     assert st == 0 and st64 == 0, ctx.error()
     assert rep.rel_residual < 1.5e-8
     assert rel(u, u64) < 1e-5
+
+
+@pytest.mark.parametrize("case", ["pointwise2d", "flux3d"])
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+def test_jacobi_bicgstab_operator_kinds(ctx, case, precision):
+    """Jacobi-preconditioned BiCGStab (the Jacobi scaling carried by the p and s updates) on a
+    point-wise variable-coefficient 2D operator and a 3D flux operator with Robin faces,
+    against the oracle."""
+    rng = np.random.default_rng(17)
+    if case == "pointwise2d":
+        n = 49
+        s = capi.Spec(n, n, op_kind=capi.OP_POINTWISE, c=0.4, d=-0.2)
+        s.coef = {capi.COEF_A: -1.0 - rng.random(s.shape), capi.COEF_B: -0.5 - rng.random(s.shape),
+                  capi.COEF_E: rng.random(s.shape)}
+    else:
+        n = 17
+        s = capi.Spec(n, n, n, a=-1.0, e=1.0, op_kind=capi.OP_FLUX)
+        s.kappa = np.exp(0.4 * rng.standard_normal(s.shape))
+        s.bcs = [BC.dirichlet(0.0)] * 4 + [BC.robin(1.0, 1.0, 0.0)] * 2
+        s.bc_convention = capi.BCCONV_OUTWARD
+    s.f_field = rng.standard_normal(s.shape)
+    print(f"jacobi bicgstab {case} precision={precision}", flush=True)
+    st, u, rep = ctx.solve(s, opts(capi.BICGSTAB, precision, 1e-10))
+    assert st == 0, ctx.error()
+    _, uo, ro = O.solve(s, opts(capi.BICGSTAB, capi.FP64, 1e-12))
+    print(f"  its {rep.iterations} (oracle {ro.iterations}) rel {rel(u, uo):.2e}", flush=True)
+    assert rel(u, uo) < (1e-8 if precision == capi.FP64 else 1e-7)
