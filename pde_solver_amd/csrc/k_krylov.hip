@@ -23,36 +23,6 @@ int blocks_for(long long n) {
 // Vector element type S is double (FP64 path) or float (mixed path, FP32 Krylov vectors);
 // arithmetic and reductions are in double.  In the mixed path the solution stays FP64 and the
 // x updates take a scale (the FP32 vectors hold the residual divided by it).
-template <typename S>
-__global__ void k_bicg_p(const S* __restrict__ r, S* __restrict__ p,
-                         const S* __restrict__ v, long long n, const KScalars* sc) {
-  if (skip(sc)) return;
-  const bool first = sc->iter == 0;
-  const double beta = sc->beta, omega = sc->omega;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    p[i] = static_cast<S>(first ? double(r[i]) : double(r[i]) + beta * (double(p[i]) - omega * double(v[i])));
-}
-
-template <typename S>
-__global__ void __launch_bounds__(kBlock) k_bicg_s(const S* __restrict__ r,
-                                                  const S* __restrict__ v,
-                                                  S* __restrict__ s, long long n,
-                                                  double* partials, const KScalars* sc) {
-  __shared__ double lds[kBlock / kWave];
-  if (skip(sc)) return;
-  const double alpha = sc->alpha;
-  double acc = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    const S si = static_cast<S>(double(r[i]) - alpha * double(v[i]));
-    s[i] = si;
-    acc += double(si) * double(si);
-  }
-  const double t = block_sum(acc, lds);
-  if (threadIdx.x == 0) partials[blockIdx.x] = t;
-}
-
 // x += scale*alpha*p_hat when ||s|| already met the target (krylov.cpp:143-148).
 template <typename S>
 __global__ void k_bicg_xs(double* __restrict__ x, const S* __restrict__ ph, long long n,
@@ -390,14 +360,6 @@ __global__ void k_bi_sdone(KScalars* sc) {
 int krylov_blocks(long long n) { return blocks_for(n); }
 
 template <typename S>
-void bicg_p(hipStream_t s, const S* r, S* p, const S* v, long long n, const KScalars* sc) {
-  hipLaunchKernelGGL(k_bicg_p<S>, dim3(blocks_for(n)), dim3(kBlock), 0, s, r, p, v, n, sc);
-}
-template <typename S>
-void bicg_s(hipStream_t s, const S* r, const S* v, S* sv, long long n, double* partials, const KScalars* sc) {
-  hipLaunchKernelGGL(k_bicg_s<S>, dim3(blocks_for(n)), dim3(kBlock), 0, s, r, v, sv, n, partials, sc);
-}
-template <typename S>
 void bicg_xs(hipStream_t s, double* x, const S* ph, long long n, const KScalars* sc, const double* scale) {
   hipLaunchKernelGGL(k_bicg_xs<S>, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, ph, n, sc, scale);
 }
@@ -409,10 +371,6 @@ template <typename S>
 void bicg_x(hipStream_t s, double* x, const S* ph, const S* sh, long long n, const KScalars* sc, const double* scale) {
   hipLaunchKernelGGL(k_bicg_x<S>, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, ph, sh, n, sc, scale);
 }
-template void bicg_p<double>(hipStream_t, const double*, double*, const double*, long long, const KScalars*);
-template void bicg_p<float>(hipStream_t, const float*, float*, const float*, long long, const KScalars*);
-template void bicg_s<double>(hipStream_t, const double*, const double*, double*, long long, double*, const KScalars*);
-template void bicg_s<float>(hipStream_t, const float*, const float*, float*, long long, double*, const KScalars*);
 template void bicg_xs<double>(hipStream_t, double*, const double*, long long, const KScalars*, const double*);
 template void bicg_xs<float>(hipStream_t, double*, const float*, long long, const KScalars*, const double*);
 template void dot2<double>(hipStream_t, const double*, const double*, const double*, long long, double*, double*, const KScalars*);

@@ -11,8 +11,6 @@ enum KFinOp { KFIN_BI_RHO = 0, KFIN_BI_ALPHA = 1, KFIN_BI_S = 2, KFIN_BI_OMEGA =
 int krylov_blocks(long long n);
 // Vector element type S: double (FP64 path) or float (mixed path).  x stays FP64; scale (device
 // pointer or null) multiplies the x updates of the mixed path.
-template <typename S> void bicg_p(hipStream_t s, const S* r, S* p, const S* v, long long n, const KScalars* sc);
-template <typename S> void bicg_s(hipStream_t s, const S* r, const S* v, S* sv, long long n, double* partials, const KScalars* sc);
 template <typename S> void bicg_xs(hipStream_t s, double* x, const S* ph, long long n, const KScalars* sc, const double* scale = nullptr);
 template <typename S> void dot2(hipStream_t s, const S* a, const S* b, const S* c, long long n, double* p1, double* p2, const KScalars* sc);
 template <typename S> void bicg_x(hipStream_t s, double* x, const S* ph, const S* sh, long long n, const KScalars* sc, const double* scale = nullptr);
@@ -21,8 +19,8 @@ template <typename S> void bicg_x(hipStream_t s, double* x, const S* ph, const S
 void bicg_r(hipStream_t s, const float* sv, const float* t, float* r, long long n, double* partials, const KScalars* sc);
 void bi_sdone(hipStream_t s, KScalars* sc);
 void fin_k(hipStream_t s, KScalars* sc, const double* p1, const double* p2, int np, int op);
-// bicg_p_pc = bicg_p, plus the Jacobi scaling of p into ph when pc; bicg_s_pc = bicg_s, plus
-// the Jacobi scaling of s into sh when pc (one pass over the vectors instead of two).
+// bicg_p_pc: p = r + beta (p - omega v) (p = r on the first iteration), plus ph = D^-1 p when pc;
+// bicg_s_pc: s = r - alpha v with partials of s.s, plus sh = D^-1 s when pc.
 template <typename S> void bicg_p_pc(hipStream_t s, const DevOp& op, const S* r, S* p, const S* v, S* ph, bool pc, const KScalars* sc);
 template <typename S> void bicg_s_pc(hipStream_t s, const DevOp& op, const S* r, const S* v, S* sv, S* sh, bool pc, double* partials, const KScalars* sc);
 template <typename S> void jacobi_pc(hipStream_t s, const DevOp& op, const S* in, S* out, const KScalars* sc);
