@@ -205,7 +205,7 @@ int pdehip_plan_set_source(pdehip_plan* plan, const double* f_field) {
   if (!plan) return PDEHIP_ERR_INVALID;
   return guarded(plan->owner, [&]() {
     std::lock_guard<std::mutex> g(plan->owner->c.mu);
-    plan_set_source(*plan->p, f_field);
+    plan_set_source(*plan->p, f_field, plan->p->f_const);
     return PDEHIP_OK;
   });
 }
@@ -231,7 +231,7 @@ int pdehip_plan_download(pdehip_plan* plan, double* u_out) {
   });
 }
 
-size_t pdehip_plan_bytes(const pdehip_plan* plan) { return plan ? plan->p->bytes_held : 0; }
+size_t pdehip_plan_bytes(const pdehip_plan* plan) { return plan ? plan_bytes(*plan->p) : 0; }
 
 int pdehip_plan_time_kernel(pdehip_plan* plan, int kernel, int precision, int reps, double* avg_ms,
                             double* algorithmic_bytes) {

@@ -101,8 +101,6 @@ __global__ void k_jacobi_pc(DevOp op, const S* __restrict__ in, S* __restrict__ 
 }
 
 // ----------------------------------------------------------------------------- GMRES
-constexpr int kMaxBasis = 64;
-
 // partials[i*nb + block] = V_i . w for i < cnt (V rows of length ld).
 template <typename S>
 __global__ void __launch_bounds__(kBlock) k_multidot(const S* __restrict__ V, long long ld,
@@ -235,10 +233,11 @@ __global__ void k_gm_start(S* __restrict__ v0, const double* __restrict__ r, lon
   }
 }
 
-__global__ void k_gm_setcol(double* H, const double* hc, int j, int m, int cnt,
+// H[i][j] = hc[i] + hc[stride + i]: the two CGS passes' coefficients staged stride apart.
+__global__ void k_gm_setcol(double* H, const double* hc, int stride, int j, int m, int cnt,
                             const KScalars* sc) {
   if (skip(sc) || sc->cycle_done) return;
-  for (int i = threadIdx.x; i < cnt; i += blockDim.x) H[i * m + j] = hc[i] + hc[64 + i];
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) H[i * m + j] = hc[i] + hc[stride + i];
 }
 
 __global__ void k_gm_reset(KScalars* sc) {
@@ -465,8 +464,9 @@ void givens(hipStream_t s, double* H, double* cs, double* sn, double* g, int j, 
 void backsub(hipStream_t s, const double* H, const double* g, double* y, int m, KScalars* sc) {
   hipLaunchKernelGGL(k_backsub, dim3(1), dim3(1), 0, s, H, g, y, m, sc);
 }
-void gm_setcol(hipStream_t s, double* H, const double* hc, int j, int m, int cnt, const KScalars* sc) {
-  hipLaunchKernelGGL(k_gm_setcol, dim3(1), dim3(64), 0, s, H, hc, j, m, cnt, sc);
+void gm_setcol(hipStream_t s, double* H, const double* hc, int stride, int j, int m, int cnt,
+               const KScalars* sc) {
+  hipLaunchKernelGGL(k_gm_setcol, dim3(1), dim3(64), 0, s, H, hc, stride, j, m, cnt, sc);
 }
 template <typename S>
 void gm_start(hipStream_t s, S* v0, const double* r, long long n, double* g, int m, KScalars* sc) {

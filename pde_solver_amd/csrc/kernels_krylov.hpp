@@ -31,7 +31,9 @@ template <typename S> void gm_normalize(hipStream_t s, S* vnext, const S* w, lon
 void fin_norm(hipStream_t s, const double* partials, int nb, double* out, const KScalars* sc);
 void givens(hipStream_t s, double* H, double* cs, double* sn, double* g, int j, int m, KScalars* sc);
 void backsub(hipStream_t s, const double* H, const double* g, double* y, int m, KScalars* sc);
-void gm_setcol(hipStream_t s, double* H, const double* hc, int j, int m, int cnt, const KScalars* sc);
+// hc holds two staged columns of cnt coefficients, `stride` doubles apart.
+void gm_setcol(hipStream_t s, double* H, const double* hc, int stride, int j, int m, int cnt,
+               const KScalars* sc);
 template <typename S> void gm_start(hipStream_t s, S* v0, const double* r, long long n, double* g, int m, KScalars* sc);
 
 }  // namespace pdehip

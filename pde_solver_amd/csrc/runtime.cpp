@@ -578,7 +578,7 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
     P->sn.alloc(ctx, mm * sizeof(double));
     P->gv.alloc(ctx, (mm + 1) * sizeof(double));
     P->yv.alloc(ctx, mm * sizeof(double));
-    P->hcb.alloc(ctx, 2 * 64 * sizeof(double));
+    P->hcb.alloc(ctx, 2 * (size_t)(mm + 1) * sizeof(double));   // two staged CGS columns
     HIPCHK(hipMemsetAsync(P->H.get(), 0, P->H.bytes(), s));
   } else if (m == PDEHIP_MG) {
     z8(P->r); z8(P->q);
@@ -607,19 +607,42 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
   HIPCHK(hipEventCreate(&P->ev0));
   HIPCHK(hipEventCreate(&P->ev1));
   HIPCHK(hipStreamSynchronize(s));
-  P->bytes_held = ctx->used.load();
   return P;
 }
 
-void plan_set_source(Plan& P, const double* f_host) {
+size_t plan_bytes(const Plan& P) {
+  size_t t = P.faces.buf.bytes() + P.f.bytes() + P.bvec.bytes() + P.x.bytes() + P.x0.bytes() +
+             P.bnd.bytes();
+  for (const Level& lv : P.L)
+    for (const DBuf* b : {&lv.kap64, &lv.kap32, &lv.pw, &lv.u, &lv.b, &lv.r, &lv.alt, &lv.ainv, &lv.map})
+      t += b->bytes();
+  for (const DBuf* b : {&P.r, &P.p, &P.q, &P.z, &P.w1, &P.w2, &P.w3, &P.w4, &P.r32, &P.x32, &P.p32,
+                        &P.q32, &P.z32, &P.V32, &P.V, &P.H, &P.cs, &P.sn, &P.gv, &P.yv, &P.hcb,
+                        &P.part, &P.part2, &P.part3, &P.scal})
+    t += b->bytes();
+  for (const DBuf& b : P.v32) t += b.bytes();
+  return t;
+}
+
+void plan_set_source(Plan& P, const double* f_host, double f_const) {
   const hipStream_t s = P.stream();
+  P.f_const = f_const;
   if (f_host) {
     if (!P.f) P.f.alloc(P.ctx, P.g.n * sizeof(double));
     HIPCHK(hipMemcpyAsync(P.f.get(), f_host, P.g.n * sizeof(double), hipMemcpyHostToDevice, s));
   } else {
     P.f.release();
   }
-  build_rhs(s, P.op(), P.f.as<double>(), P.f_const, P.x0.as<double>(), P.bvec.as<double>());
+  // Boundary rows of b carry the Dirichlet datum.  Folded plans keep it in x0 (their initial
+  // guess); reference-row plans start from x = 0, so their datum is rebuilt from the faces.
+  const double* bnd = P.x0.as<double>();
+  if (!P.folded) {
+    if (!P.bnd) P.bnd.alloc(P.ctx, P.g.n * sizeof(double));
+    fill(s, P64, P.bnd.get(), P.g.n, 0.0);
+    closures_dirichlet(Launch{s, nullptr}, P.op(), P.faces.fd, P.bnd.as<double>());
+    bnd = P.bnd.as<double>();
+  }
+  build_rhs(s, P.op(), P.f.as<double>(), P.f_const, bnd, P.bvec.as<double>());
   HIPCHK(hipStreamSynchronize(s));
 }
 
@@ -1150,13 +1173,13 @@ static void solve_gmres(Plan& P, pdehip_report* rep) {
       const double* vj = V + (long long)j * n;
       if (pc) { jacobi_pc(s, op, vj, zt, sc); apply(L, op, P64, zt, w, nullptr); }
       else apply(L, op, P64, vj, w, nullptr);
-      double* hcol = P.hcb.as<double>();  // two passes of (j+1) coefficients
+      double* hcol = P.hcb.as<double>();  // two passes of (j+1) coefficients, mm+1 apart
       for (int pass = 0; pass < 2; ++pass) {
         multidot(s, V, n, j + 1, w, n, part, kb, sc);
-        fin_multi(s, part, kb, j + 1, hcol + (pass ? 64 : 0), 0, sc);
-        multiaxpy(s, V, n, j + 1, hcol + (pass ? 64 : 0), w, n, -1.0, 1, sc);
+        fin_multi(s, part, kb, j + 1, hcol + (pass ? mm + 1 : 0), 0, sc);
+        multiaxpy(s, V, n, j + 1, hcol + (pass ? mm + 1 : 0), w, n, -1.0, 1, sc);
       }
-      gm_setcol(s, H, hcol, j, m, j + 1, sc);   // H[i][j] = h1[i] + h2[i]
+      gm_setcol(s, H, hcol, mm + 1, j, m, j + 1, sc);   // H[i][j] = h1[i] + h2[i]
       dot(L, P64, w, w, n, part);
       fin_norm(s, part, vec_blocks(n), H + (long long)(j + 1) * m + j, sc);
       gm_normalize(s, V + (long long)(j + 1) * n, w, n, H + (long long)(j + 1) * m + j, sc);
@@ -1367,10 +1390,10 @@ static void solve_gmres_mixed(Plan& P, pdehip_report* rep) {
       double* hcol = P.hcb.as<double>();
       for (int pass = 0; pass < 2; ++pass) {
         multidot<float>(s, V, n, j + 1, w, n, part, kb, sc);
-        fin_multi(s, part, kb, j + 1, hcol + (pass ? 64 : 0), 0, sc);
-        multiaxpy<float>(s, V, n, j + 1, hcol + (pass ? 64 : 0), w, n, -1.0, 1, sc);
+        fin_multi(s, part, kb, j + 1, hcol + (pass ? mm + 1 : 0), 0, sc);
+        multiaxpy<float>(s, V, n, j + 1, hcol + (pass ? mm + 1 : 0), w, n, -1.0, 1, sc);
       }
-      gm_setcol(s, H, hcol, j, m, j + 1, sc);
+      gm_setcol(s, H, hcol, mm + 1, j, m, j + 1, sc);
       dot(L, P32, w, w, n, part);
       fin_norm(s, part, vec_blocks(n), H + (long long)(j + 1) * m + j, sc);
       gm_normalize<float>(s, V + (long long)(j + 1) * n, w, n, H + (long long)(j + 1) * m + j, sc);

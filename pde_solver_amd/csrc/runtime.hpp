@@ -149,7 +149,8 @@ struct Plan {
   NonlinearTerms nl{};
   DBuf f;                  // per-node source (or empty: constant prob.f)
   double f_const = 0.0;
-  DBuf bvec, x, x0;        // FP64 right-hand side, solution, boundary data
+  DBuf bvec, x, x0;        // FP64 right-hand side, solution, initial guess (folded: boundary data)
+  DBuf bnd;                // reference-row plans: Dirichlet datum for rebuilding b (on demand)
   std::vector<Level> L;    // L[0] is the fine level (operator for every method)
   // Krylov work vectors
   DBuf r, p, q, z, w1, w2, w3, w4;
@@ -165,7 +166,6 @@ struct Plan {
   bool fin_resid = false;      // plan_finish_enqueue staged the true-residual partials
   bool resid_fresh = false;    // the solver left r = b - A x of the final x and its partials
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  size_t bytes_held = 0;
 
   ~Plan();
   KScalars* sc(int i = 0) const { return scal.as<KScalars>() + i; }
@@ -174,7 +174,8 @@ struct Plan {
 };
 
 std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehip_options& o);
-void plan_set_source(Plan& P, const double* f_host);
+// Replace the source (f_host: per-node field, or NULL for the constant f_const) and rebuild b.
+void plan_set_source(Plan& P, const double* f_host, double f_const);
 void plan_solve(Plan& P, pdehip_report* rep);
 // The tail of plan_solve split in two: enqueue the post-solve work (true residual, boundary
 // reconstruction, residual norms, staging copies) without waiting, then fill the report after
@@ -184,6 +185,8 @@ void plan_finish_collect(Plan& P, pdehip_report* rep);
 // Scalars of a plan (device) and the plan's KScalars initialisation, for batched drivers.
 void plan_init_scalars(Plan& P, int i, double eps);
 void plan_download(Plan& P, double* u_out);
+// Device bytes the plan's own buffers hold now.
+size_t plan_bytes(const Plan& P);
 void plan_residual_norms(Plan& P, const double* u_dev, double* l2, double* linf);
 
 // Standalone entry points.

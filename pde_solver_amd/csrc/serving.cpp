@@ -83,9 +83,19 @@ struct OwnedProblem {
         if (!samples[fc][t].empty()) h = hash_bytes(h, samples[fc][t].data(), samples[fc][t].size() * 8);
       }
     }
-    pdehip_options oc = o;
-    oc.cancel = nullptr;
-    h = hash_bytes(h, &oc, sizeof(oc));
+    h = mix(h, (uint64_t)std::max(0, std::min(p.n_nonlinear, PDEHIP_MAX_NONLINEAR)));
+    for (int t = 0; t < std::min(p.n_nonlinear, PDEHIP_MAX_NONLINEAR); ++t) {
+      h = mix(h, (uint64_t)p.nonlinear[t].kind * 1000003ULL + (uint64_t)(int64_t)p.nonlinear[t].power);
+      h = hash_d(h, p.nonlinear[t].coeff);
+    }
+    h = hash_d(h, p.ut);
+    h = hash_d(h, p.utt);
+    // named option fields (the struct's padding bytes are not part of the key)
+    for (int v : {o.method, o.max_iter, o.tol_mode, o.gmres_restart, o.mg_pre, o.mg_post,
+                  o.mg_coarse_iters, o.mg_max_levels, o.mg_single_cycle, o.precond, o.precision,
+                  o.bc_rows, o.check_every})
+      h = mix(h, (uint64_t)(int64_t)v);
+    for (double v : {o.tol, o.sor_omega, o.inner_tol}) h = hash_d(h, v);
     return h;
   }
 };
@@ -179,13 +189,11 @@ void pdehip_server::run(int wi) {
           break;
         }
       if (plan) {
-        plan_set_source(*plan, req->prob.p.f_field);
-        plan->f_const = req->prob.p.f;
+        plan_set_source(*plan, req->prob.p.f_field, req->prob.p.f);
       } else {
         auto P = plan_create(&w.ctx, req->prob.p, req->opt);
         plan = P.get();
         if (cfg.cache_operators) {
-          w.cached_bytes += P->bytes_held;
           w.lru.emplace_front(key, std::move(P));
         } else {
           w.lru.emplace_front(0, std::move(P));
@@ -194,17 +202,26 @@ void pdehip_server::run(int wi) {
       plan_solve(*plan, &req->rep);
       st = req->rep.status;
       if (req->u_out) plan_download(*plan, req->u_out);
-      if (!cfg.cache_operators) { w.lru.clear(); w.cached_bytes = 0; }
-      // keep the resident cache within a quarter of the budget
-      while (cfg.cache_operators && w.lru.size() > 1 && w.ctx.used.load() > budget / 4) {
+      if (!cfg.cache_operators) w.lru.clear();
+      // keep this worker's resident cache within a quarter of the budget
+      auto held = [&] {
+        size_t t = 0;
+        for (auto& e : w.lru) t += plan_bytes(*e.second);
+        return t;
+      };
+      w.cached_bytes = held();
+      while (cfg.cache_operators && w.lru.size() > 1 && w.cached_bytes > budget / 4) {
         w.lru.pop_back();
+        w.cached_bytes = held();
       }
     } catch (const Error& e) {
       st = e.code;
       w.lru.clear();
+      w.cached_bytes = 0;
     } catch (const std::exception&) {
       st = PDEHIP_ERR_DEVICE;
       w.lru.clear();
+      w.cached_bytes = 0;
     }
     {
       std::lock_guard<std::mutex> lk(mu);

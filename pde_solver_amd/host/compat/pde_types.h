@@ -3,8 +3,8 @@
 // meaning and defaults follow the reference header (include/pde_types.h:10-58, 148-159,
 // 183-238, 282-345, 385-430) so the plug-in compiles unchanged against the reference's own
 // header in-tree.  Members of the reference the HIP path never reads (shapes, multi-field
-// coupling, conservation-law settings) are omitted here; the hip_* members of SolverConfig
-// are the plug-in's extensions (they are added to the reference header when integrating).
+// coupling, conservation-law settings) are omitted here.  Options of the HIP backend that the
+// reference has no member for are set through HipSetSolverOptions (include/hip_solver.h).
 #ifndef PDE_TYPES_H
 #define PDE_TYPES_H
 
@@ -126,13 +126,6 @@ struct SolverConfig {
   int mg_max_levels = 10;
   AdvectionScheme advection_scheme = AdvectionScheme::Centered;
   bool advection_cfl_check = false;
-  // HIP backend extensions
-  int hip_precision = 0;            // 0: FP64, 1: mixed (FP32/FP16 inner, FP64 refinement)
-  bool hip_mg_pcg = false;          // MultigridVcycle as MG-preconditioned CG
-  bool hip_relative_tol = false;    // tol relative to the initial residual
-  bool hip_outward_robin = false;   // alpha*u + beta*du/dn = gamma with the outward normal
-  bool hip_jacobi_precond = false;  // Jacobi right preconditioner for BiCGStab/GMRES
-  bool hip_mg_single_cycle = false; // MultigridVcycle: exactly one V-cycle (CPU semantics)
 };
 
 enum class TimeIntegrator { ForwardEuler, RK2, RK4, SSPRK2, SSPRK3, BackwardEuler, CrankNicolson, IMEX };

@@ -44,6 +44,9 @@ namespace {
 std::mutex g_ctx_mu;
 pdehip_ctx* g_ctx = nullptr;
 
+std::mutex g_opt_mu;
+HipSolverOptions g_opt;
+
 pdehip_ctx* context(std::string* err) {
   std::lock_guard<std::mutex> g(g_ctx_mu);
   if (!g_ctx) {
@@ -73,8 +76,6 @@ struct Assembled {
   std::vector<double> face[6][4];
   std::string error;
 };
-
-bool is_linear_or_latex_finite(const BoundaryCondition::Expression&) { return true; }
 
 // BoundaryCondition::Expression -> pdehip_expr: linear form, or per-node samples of the LaTeX
 // with the linear fallback of EvalExpr (reference cpu_utils.cpp:521-530).
@@ -121,7 +122,7 @@ bool sample_field(const std::string& latex, double constant, const Grid& g, cons
   return true;
 }
 
-bool assemble(const SolveInput& in, Assembled* A) {
+bool assemble(const SolveInput& in, const HipSolverOptions& hopt, Assembled* A) {
   const Domain& d = in.domain;
   if (d.nx < 3 || d.ny < 3) { A->error = "grid must be at least 3x3"; return false; }
   if (d.nz > 1 && d.nz < 3) { A->error = "grid must be at least 3x3x3"; return false; }
@@ -191,7 +192,7 @@ bool assemble(const SolveInput& in, Assembled* A) {
     for (int t = 0; t < 4; ++t)
       if (!convert_expr(*ex[t], g, d, f, dst[t], &A->face[f][t], &A->error)) return false;
   }
-  p.bc_convention = in.solver.hip_outward_robin ? PDEHIP_BCCONV_OUTWARD : PDEHIP_BCCONV_AXIS;
+  p.bc_convention = hopt.outward_robin ? PDEHIP_BCCONV_OUTWARD : PDEHIP_BCCONV_AXIS;
   p.n_nonlinear = 0;
   for (const NonlinearTerm& t : in.nonlinear) {
     if (p.n_nonlinear >= PDEHIP_MAX_NONLINEAR) { A->error = "too many nonlinear terms"; return false; }
@@ -255,6 +256,16 @@ const char* method_failure(SolveMethod m) {
 
 }  // namespace
 
+void HipSetSolverOptions(const HipSolverOptions& options) {
+  std::lock_guard<std::mutex> g(g_opt_mu);
+  g_opt = options;
+}
+
+HipSolverOptions HipGetSolverOptions() {
+  std::lock_guard<std::mutex> g(g_opt_mu);
+  return g_opt;
+}
+
 bool HipIsAvailable(std::string* note) {
   char buf[256] = {0};
   const int ok = pdehip_is_available(buf, sizeof(buf));
@@ -262,35 +273,33 @@ bool HipIsAvailable(std::string* note) {
   return ok != 0;
 }
 
-SolveOutput SolvePDEHip(const SolveInput& input) {
-  if (std::abs(input.pde.ut) > 1e-12 || std::abs(input.pde.utt) > 1e-12)
-    return {"time-dependent PDEs require the time-series solver", {}};
-  std::string err;
-  pdehip_ctx* ctx = context(&err);
-  if (!ctx) return {"HIP unavailable: " + err, {}};
-  Assembled A;
-  if (!assemble(input, &A)) return {A.error, {}};
+namespace {
+
+// SolverConfig (+ the HIP options) -> pdehip_options, with the plug-in's method checks.
+// Returns an error message, or an empty string.
+std::string build_options(const SolveInput& input, const HipSolverOptions& hopt,
+                          const Assembled& A, pdehip_options* out) {
   const SolverConfig& s = input.solver;
-  pdehip_options o;
+  pdehip_options& o = *out;
   pdehip_default_options(&o);
   o.max_iter = std::max(1, s.max_iter);
   o.tol = s.tol;
-  o.tol_mode = s.hip_relative_tol ? PDEHIP_TOL_REL : PDEHIP_TOL_ABS;
+  o.tol_mode = hopt.relative_tol ? PDEHIP_TOL_REL : PDEHIP_TOL_ABS;
   o.sor_omega = s.sor_omega;
   o.gmres_restart = s.gmres_restart;
   o.mg_pre = s.mg_pre_smooth;
   o.mg_post = s.mg_post_smooth;
   o.mg_coarse_iters = s.mg_coarse_iters;
   o.mg_max_levels = s.mg_max_levels;
-  o.precision = s.hip_precision ? PDEHIP_MIXED : PDEHIP_FP64;
-  o.precond = s.hip_jacobi_precond ? PDEHIP_PC_JACOBI : PDEHIP_PC_NONE;
+  o.precision = hopt.mixed_precision ? PDEHIP_MIXED : PDEHIP_FP64;
+  o.precond = hopt.jacobi_precond ? PDEHIP_PC_JACOBI : PDEHIP_PC_NONE;
   const bool dirichlet = all_dirichlet(A.p);
   switch (s.method) {
     case SolveMethod::Jacobi: o.method = PDEHIP_JACOBI; break;
     case SolveMethod::GaussSeidel: o.method = PDEHIP_GAUSS_SEIDEL; break;
     case SolveMethod::SOR:
       o.method = PDEHIP_SOR;
-      if (s.sor_omega <= 0.0 || s.sor_omega >= 2.0) return {"SOR omega must be in (0,2)", {}};
+      if (s.sor_omega <= 0.0 || s.sor_omega >= 2.0) return "SOR omega must be in (0,2)";
       break;
     case SolveMethod::CG:
       // Folded rows: identical iterates to the reference rows for zero Dirichlet data, and a
@@ -299,7 +308,7 @@ SolveOutput SolvePDEHip(const SolveInput& input) {
       o.bc_rows = PDEHIP_ROWS_FOLDED;
       if (A.p.op_kind != PDEHIP_OP_CONST || std::abs(A.p.c) > 1e-12 || std::abs(A.p.d) > 1e-12 ||
           std::abs(A.p.dz) > 1e-12)
-        return {"CG requires symmetric operators with constant coefficients (no convection/mixed/higher-order terms)", {}};
+        return "CG requires symmetric operators with constant coefficients (no convection/mixed/higher-order terms)";
       break;
     case SolveMethod::BiCGStab:
       o.method = PDEHIP_BICGSTAB;
@@ -310,29 +319,110 @@ SolveOutput SolvePDEHip(const SolveInput& input) {
       o.bc_rows = dirichlet ? PDEHIP_ROWS_REFERENCE : PDEHIP_ROWS_FOLDED;
       break;
     case SolveMethod::MultigridVcycle:
-      o.method = s.hip_mg_pcg ? PDEHIP_MGPCG : PDEHIP_MG;
-      o.mg_single_cycle = s.hip_mg_single_cycle ? 1 : 0;
+      o.method = hopt.mg_pcg ? PDEHIP_MGPCG : PDEHIP_MG;
+      o.mg_single_cycle = hopt.mg_single_cycle ? 1 : 0;
       o.bc_rows = PDEHIP_ROWS_FOLDED;
       o.precond = PDEHIP_PC_MG;
       break;
   }
-  if (!input.nonlinear.empty() && o.method > PDEHIP_SOR)
-    return {"Krylov solvers require linear PDEs", {}};
+  if (!input.nonlinear.empty() && o.method > PDEHIP_SOR) return "Krylov solvers require linear PDEs";
+  return std::string();
+}
+
+// Status of a finished solve -> SolveOutput (error string or residual norms).
+void finish_output(int st, SolveMethod m, const pdehip_report& rep, pdehip_ctx* ctx, SolveOutput* out) {
+  if (st == PDEHIP_ERR_CANCELLED) { *out = {"solve cancelled", {}}; return; }
+  if (st == PDEHIP_ERR_NOT_CONVERGED) { *out = {method_failure(m), {}}; return; }
+  if (st != PDEHIP_OK) {
+    const char* e = pdehip_last_error(ctx);
+    *out = {std::string(e && *e ? e : "HIP solve failed"), {}};
+    return;
+  }
+  out->residual_l2 = rep.residual_l2;
+  out->residual_linf = rep.residual_linf;
+}
+
+bool same_options(const pdehip_options& a, const pdehip_options& b) {
+  return a.method == b.method && a.max_iter == b.max_iter && a.tol == b.tol &&
+         a.tol_mode == b.tol_mode && a.sor_omega == b.sor_omega &&
+         a.gmres_restart == b.gmres_restart && a.mg_pre == b.mg_pre && a.mg_post == b.mg_post &&
+         a.mg_coarse_iters == b.mg_coarse_iters && a.mg_max_levels == b.mg_max_levels &&
+         a.mg_single_cycle == b.mg_single_cycle && a.precond == b.precond &&
+         a.precision == b.precision && a.bc_rows == b.bc_rows;
+}
+
+}  // namespace
+
+SolveOutput SolvePDEHip(const SolveInput& input) {
+  if (std::abs(input.pde.ut) > 1e-12 || std::abs(input.pde.utt) > 1e-12)
+    return {"time-dependent PDEs require the time-series solver", {}};
+  std::string err;
+  pdehip_ctx* ctx = context(&err);
+  if (!ctx) return {"HIP unavailable: " + err, {}};
+  const HipSolverOptions hopt = HipGetSolverOptions();
+  Assembled A;
+  if (!assemble(input, hopt, &A)) return {A.error, {}};
+  pdehip_options o;
+  err = build_options(input, hopt, A, &o);
+  if (!err.empty()) return {err, {}};
   CancelBridge cancel(input.cancel);
   o.cancel = cancel.flag();
   SolveOutput out;
   out.grid.assign(static_cast<size_t>(A.p.nx) * A.p.ny * A.p.nz, 0.0);
   pdehip_report rep{};
   const int st = pdehip_solve(ctx, &A.p, &o, out.grid.data(), &rep);
-  if (st == PDEHIP_ERR_CANCELLED) return {"solve cancelled", {}};
-  if (st == PDEHIP_ERR_NOT_CONVERGED) return {method_failure(s.method), {}};
-  if (st != PDEHIP_OK) {
-    const char* e = pdehip_last_error(ctx);
-    return {std::string(e && *e ? e : "HIP solve failed"), {}};
-  }
-  out.residual_l2 = rep.residual_l2;
-  out.residual_linf = rep.residual_linf;
+  finish_output(st, input.solver.method, rep, ctx, &out);
   return out;
+}
+
+std::vector<SolveOutput> SolvePDEBatchHip(const std::vector<SolveInput>& inputs) {
+  std::vector<SolveOutput> outs(inputs.size());
+  std::string err;
+  pdehip_ctx* ctx = context(&err);
+  if (!ctx) {
+    for (auto& o : outs) o = {"HIP unavailable: " + err, {}};
+    return outs;
+  }
+  const HipSolverOptions hopt = HipGetSolverOptions();
+  std::vector<Assembled> A(inputs.size());
+  std::vector<pdehip_options> opt(inputs.size());
+  std::vector<char> ok(inputs.size(), 0);
+  for (size_t i = 0; i < inputs.size(); ++i) {
+    const SolveInput& in = inputs[i];
+    if (std::abs(in.pde.ut) > 1e-12 || std::abs(in.pde.utt) > 1e-12) {
+      outs[i] = {"time-dependent PDEs require the time-series solver", {}};
+      continue;
+    }
+    if (!assemble(in, hopt, &A[i])) { outs[i] = {A[i].error, {}}; continue; }
+    err = build_options(in, hopt, A[i], &opt[i]);
+    if (!err.empty()) { outs[i] = {err, {}}; continue; }
+    ok[i] = 1;
+  }
+  // runs of consecutive assembled inputs with equal options: one pdehip_solve_batch each
+  for (size_t i = 0; i < inputs.size();) {
+    if (!ok[i]) { ++i; continue; }
+    size_t j = i + 1;
+    while (j < inputs.size() && ok[j] && same_options(opt[j], opt[i])) ++j;
+    const int count = static_cast<int>(j - i);
+    std::vector<pdehip_problem> probs(count);
+    std::vector<double*> grids(count);
+    std::vector<pdehip_report> reps(count);
+    for (int t = 0; t < count; ++t) {
+      probs[t] = A[i + t].p;
+      outs[i + t].grid.assign(static_cast<size_t>(probs[t].nx) * probs[t].ny * probs[t].nz, 0.0);
+      grids[t] = outs[i + t].grid.data();
+    }
+    for (auto& r : reps) r.status = PDEHIP_PENDING;   // left so if the batch stops early
+    const int st = pdehip_solve_batch(ctx, count, probs.data(), &opt[i], nullptr, grids.data(),
+                                      reps.data());
+    for (int t = 0; t < count; ++t) {
+      const int sti = reps[t].status != PDEHIP_PENDING ? reps[t].status
+                                                       : (st != PDEHIP_OK ? st : PDEHIP_ERR_DEVICE);
+      finish_output(sti, inputs[i + t].solver.method, reps[t], ctx, &outs[i + t]);
+    }
+    i = j;
+  }
+  return outs;
 }
 
 namespace {
@@ -360,7 +450,7 @@ SolveOutput SolvePDETimeSeriesHip(const SolveInput& input, const FrameCallback& 
   pdehip_ctx* ctx = context(&err);
   if (!ctx) return {"HIP unavailable: " + err, {}};
   Assembled A;
-  if (!assemble(input, &A)) return {A.error, {}};
+  if (!assemble(input, HipGetSolverOptions(), &A)) return {A.error, {}};
   pdehip_time_config tc{};
   switch (input.time.integrator) {
     case TimeIntegrator::ForwardEuler: tc.integrator = PDEHIP_TI_FORWARD_EULER; break;

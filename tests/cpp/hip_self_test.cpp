@@ -172,10 +172,14 @@ int main() {
     Configure(&in, SolveMethod::CG, 4000, 1e-10);
     Run("3D Poisson (CG)", in, ex, 2e-3);
     Configure(&in, SolveMethod::MultigridVcycle, 100, 1e-10);
-    in.solver.hip_mg_pcg = true;
+    HipSolverOptions ho;
+    ho.mg_pcg = true;
+    HipSetSolverOptions(ho);
     Run("3D Poisson (MG-PCG)", in, ex, 2e-3);
-    in.solver.hip_precision = 1;
+    ho.mixed_precision = true;
+    HipSetSolverOptions(ho);
     Run("3D Poisson (MG-PCG mixed precision)", in, ex, 2e-3);
+    HipSetSolverOptions(HipSolverOptions());
   }
   {
     // Robin (outward) faces: u = cos(pi x/2)... use a closed form u = 1 + x with
@@ -189,13 +193,15 @@ int main() {
     in.bc.right.kind = BCKind::Neumann;  // du/dx = 1 (axis convention = outward on the high face)
     in.bc.right.value = Linear(1.0, 0, 0);
     Configure(&in, SolveMethod::CG, 4000, 1e-11);
-    in.solver.hip_outward_robin = true;
+    HipSolverOptions ho;
+    ho.outward_robin = true;
+    HipSetSolverOptions(ho);
     Run("Linear solution, Robin + Neumann faces (CG folded rows)", in,
         [](double x, double, double) { return 1.0 + x; }, 1e-8);
     // absolute L2 tolerance: 1e-11 sits at the FP64 rounding floor of rows scaled by 1/h^2
     Configure(&in, SolveMethod::MultigridVcycle, 200, 1e-10);
-    in.solver.hip_outward_robin = true;
     Run("Linear solution, Robin + Neumann faces (MG)", in, [](double x, double, double) { return 1.0 + x; }, 1e-8);
+    HipSetSolverOptions(HipSolverOptions());
   }
   // Capability matrix: HIP lifts the 2D/Dirichlet-only gates, keeps the shape gate.
   {
@@ -245,6 +251,46 @@ int main() {
     std::cout << (pass ? "[OK] " : "[FAIL] ") << "heat decay time series ("
               << (ti == TimeIntegrator::RK4 ? "RK4" : "FE") << ") u_mid=" << mid << " expect=" << expect
               << " frames=" << frames_seen << " " << out.error << "\n";
+    if (!pass) ++g_fail;
+  }
+  // Batch entry (the batch runner's replacement, src/main.cpp:2071-2097): uneven 2D MG-PCG
+  // inputs with shared options go through the lock-step engine together, a GMRES input on its
+  // own; each answer matches the input solved alone to the solve tolerance.
+  {
+    ++g_total;
+    std::vector<SolveInput> ins;
+    for (int n : {33, 65, 17, 129}) {
+      SolveInput in;
+      in.domain = Unit(n);
+      in.pde.a = -1.0; in.pde.b = -1.0; in.pde.e = 0.1 * n;
+      in.pde.rhs_latex = "-2*\\pi^2*\\sin(\\pi*x)*\\sin(\\pi*y)";
+      in.bc = DirichletAll(Linear(0.0, 0.0, 0.0));
+      Configure(&in, SolveMethod::MultigridVcycle, 100, 1e-10);
+      ins.push_back(in);
+    }
+    Configure(&ins.back(), SolveMethod::GMRES, 4000, 1e-9);
+    HipSolverOptions ho;
+    ho.mg_pcg = true;
+    ho.relative_tol = true;
+    HipSetSolverOptions(ho);
+    const std::vector<SolveOutput> outs = SolvePDEBatchHip(ins);
+    double worst = 0.0;
+    bool ok = outs.size() == ins.size();
+    for (size_t i = 0; ok && i < ins.size(); ++i) {
+      const SolveOutput one = SolvePDEHip(ins[i]);
+      ok = outs[i].error.empty() && one.error.empty() && outs[i].grid.size() == one.grid.size();
+      double num = 0.0, den = 0.0;
+      for (size_t q = 0; ok && q < one.grid.size(); ++q) {
+        num += (outs[i].grid[q] - one.grid[q]) * (outs[i].grid[q] - one.grid[q]);
+        den += one.grid[q] * one.grid[q];
+      }
+      if (ok) worst = std::max(worst, std::sqrt(num / std::max(den, 1e-300)));
+      if (!ok) std::cout << "  batch input " << i << ": " << outs[i].error << one.error << "\n";
+    }
+    HipSetSolverOptions(HipSolverOptions());
+    const bool pass = ok && worst < 1e-7;
+    std::cout << (pass ? "[OK] " : "[FAIL] ") << "batch entry (3 MG-PCG in lock step + 1 GMRES) "
+              << "max rel diff vs single solves " << worst << "\n";
     if (!pass) ++g_fail;
   }
   // Restart equivalence (tests/checkpoint_equivalence.cpp): a series restarted from frame 2's

@@ -203,6 +203,10 @@ CASES: List[Case] = [
     Case("gmres_restart_quirk_33", "solve", _case_convdiff(capi.GMRES, restart=100, tol=1e-9),
          ref_fails=True),
     Case("jacobi_poisson_17", "solve", _case_relax(capi.JACOBI, 17, "dirichlet")),
+    # the plain CPU workload: 129^2 Poisson, Dirichlet 0, f = 2 pi^2 sin(pi x) sin(pi y), Jacobi
+    # to max|du| < 1e-10 from u = 0 (about 49.5k sweeps; rate cos(pi h))
+    Case("jacobi_poisson_129", "solve", _case_relax(capi.JACOBI, 129, "dirichlet", tol=1e-10,
+                                                    max_iter=60000)),
     Case("jacobi_mixed_bc_17", "solve", _case_relax(capi.JACOBI, 17, "mixed")),
     Case("sor_mixed_bc_21", "solve", _case_relax(capi.SOR, 21, "mixed", omega=1.7)),
     Case("gs_helmholtz_var_17", "solve", _case_relax(capi.GAUSS_SEIDEL, 17, "mixed", var=True, e=-2.0)),

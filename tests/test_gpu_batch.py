@@ -82,9 +82,8 @@ def test_batch_engine_matches_plans_and_oracle(ctx, precision):
         noise = 64 * 2.2e-16 * (8.0 * (s.nx - 1) ** 2 + s.e) * np.abs(u).max()   # per-node rounding
         assert abs(reps[i].residual_l2 - l2) <= 1e-3 * l2 + noise * np.sqrt(s.n)
         assert abs(reps[i].residual_linf - linf) <= 1e-3 * linf + noise
-        if s.nx <= 129:
-            _, uo, _ = O.solve(s, opts(capi.FP64, 1e-13))
-            assert rel(u, uo) < (1e-8 if precision == capi.FP64 else 5e-7)
+        _, uo, _ = O.solve(s, opts(capi.FP64, 1e-13))
+        assert rel(u, uo) < (1e-8 if precision == capi.FP64 else 5e-7)
     b.close()
 
 
@@ -114,10 +113,14 @@ def test_batch_autotuned_graph_matches_default(ctx, precision):
     _, r1 = b1.solve()
     u1 = [b1.download(i) for i in range(len(specs))]
     _, r2 = b1.solve()
-    for i in range(len(specs)):
+    for i, s in enumerate(specs):
         assert r1[i].status == 0 and r1[i].rel_residual < 2e-10
         assert abs(r1[i].inner_iterations - r0[i].inner_iterations) <= 1
         assert rel(u1[i], u0[i]) < 2e-8
+        # every size against the oracle, 513^2 and 1025^2 included
+        _, uo, _ = O.solve(s, opts(capi.FP64, 1e-13))
+        print(f"  n={s.nx} rel vs oracle {rel(u1[i], uo):.2e}", flush=True)
+        assert rel(u1[i], uo) < (1e-8 if precision == capi.FP64 else 5e-7)
         assert np.array_equal(u1[i], b1.download(i))
         assert r2[i].iterations == r1[i].iterations
     b1.close()

@@ -87,3 +87,29 @@ def test_hip_time_series_matches_reference(ctx, case):
     for f in range(ref.shape[0]):
         assert rel(frames[f], ref[f]) < 1e-12, (f, rel(frames[f], ref[f]))
     assert rel(u, ref[-1]) < 1e-12
+
+
+def test_cfg1_mms_closed_form(ctx):
+    """The plain CPU workload on the device: 129^2 Poisson with f = 2 pi^2 sin(pi x) sin(pi y).
+    Its right-hand side is an eigenvector of the 5-point Dirichlet Laplacian, so the discrete
+    solution is (2 pi^2 / lambda_h) sin(pi x) sin(pi y), lambda_h = (8/h^2) sin^2(pi h / 2):
+    CG's max error against the continuous solution is 2 pi^2 / lambda_h - 1 = 5.020092e-05.
+    Jacobi stops on max|du| < 1e-10, an algebraic error below 1e-10 / (1 - cos(pi h))."""
+    spec, opt, _, _ = K.case("cg_poisson_129").build()
+    h = 1.0 / 128
+    closed = 2 * np.pi ** 2 / ((8 / h ** 2) * np.sin(np.pi * h / 2) ** 2) - 1.0
+    assert closed == pytest.approx(5.020092e-05, abs=5e-12)
+    X, Y, _ = spec.coords()
+    exact = np.sin(np.pi * X) * np.sin(np.pi * Y)
+    st, u, rep = ctx.solve(spec, opt)
+    err = np.abs(u - exact).max()
+    print(f"  CG iterations {rep.iterations} MMS Linf {err:.9e} (closed form {closed:.9e})", flush=True)
+    assert abs(err - closed) < 1e-10
+    spec, opt, _, _ = K.case("jacobi_poisson_129").build()
+    st, u, rep = ctx.solve(spec, opt)
+    err = np.abs(u - exact).max()
+    g = load("jacobi_poisson_129")
+    print(f"  Jacobi sweeps {rep.iterations} MMS Linf {err:.9e}", flush=True)
+    assert st == 0 and 49000 <= rep.iterations <= 50000
+    assert abs(err - closed) < 1e-10 / (1 - np.cos(np.pi * h))
+    assert rel(u, g["u"]) < 1e-10

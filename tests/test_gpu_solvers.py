@@ -232,3 +232,62 @@ def test_mgpcg_fp64_nonsquare3d_matches_oracle(ctx, shape):
               flush=True)
         assert st == 0
         assert rel(u, u_o) < tol
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+def test_flux_two_layer_conductor_harmonic_face(ctx, dim, precision):
+    """FLUX faces are harmonic means of their two nodes (FP64 operator, and the FP32 reciprocal
+    form of the mixed path): a two-layer conductor is reproduced at every node."""
+    from tests.closed_forms import two_layer_conductor
+    s, exact = two_layer_conductor(21, 8, 1.0, 10.0, ny=9, nz=9 if dim == 3 else 1)
+    st, u, rep = ctx.solve(s, mgpcg(tol=1e-12 if precision == capi.FP64 else 1e-10,
+                                    precision=precision))
+    err = np.abs(u - exact).max()
+    print(f"  dim {dim} precision {precision} its {rep.iterations} max err {err:.2e}", flush=True)
+    assert st == 0
+    assert err < (1e-10 if precision == capi.FP64 else 1e-8)
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+def test_gmres_long_restart_matches_oracle(ctx, precision):
+    """GMRES with a restart length above 64 (the Hessenberg staging is sized by the restart)."""
+    s = capi.Spec(33, 33, a=-0.05, b=-0.05, c=0.7, d=0.4, convection=capi.CONV_UPWIND)
+    s.f_field = np.random.default_rng(21).standard_normal(s.shape)
+    opt = capi.options(capi.GMRES, tol=1e-10, tol_mode=capi.TOL_REL, precond=capi.PC_JACOBI,
+                       bc_rows=capi.ROWS_FOLDED, gmres_restart=80, max_iter=2000,
+                       precision=precision)
+    st_o, u_o, rep_o = O.solve(s, capi.options(capi.GMRES, tol=1e-12, tol_mode=capi.TOL_REL,
+                                               precond=capi.PC_JACOBI, bc_rows=capi.ROWS_FOLDED,
+                                               gmres_restart=80, max_iter=2000))
+    st, u, rep = ctx.solve(s, opt)
+    print(f"  precision {precision} its {rep.iterations} (oracle {rep_o.iterations}) "
+          f"rel {rel(u, u_o):.2e}", flush=True)
+    assert st == 0 and st_o == 0
+    assert rel(u, u_o) < (1e-8 if precision == capi.FP64 else 1e-7)
+    if precision == capi.FP64:
+        _, _, rep_same = O.solve(s, opt)   # the same tolerance: the same iteration count
+        assert abs(rep.iterations - rep_same.iterations) <= 2
+
+
+def test_plan_set_source_reference_rows_keep_dirichlet_data(ctx):
+    """A reference-row BiCGStab plan (x0 = 0) with non-zero Dirichlet data: every new source
+    rebuilds b with the boundary datum, and the plan's answers equal one-shot solves bitwise."""
+    s = capi.Spec(33, 33, a=-1.0, b=-1.0, c=0.5)
+    s.bcs = [BC.dirichlet(1.0, 0.5), BC.dirichlet(0.2), BC.dirichlet(0.0, 0.0, 1.0),
+             BC.dirichlet(-0.3)] + [BC(), BC()]
+    opt = capi.options(capi.BICGSTAB, tol=1e-11, max_iter=5000)
+    rng = np.random.default_rng(31)
+    s.f_field = rng.standard_normal(s.shape)
+    plan = ctx.plan(s, opt)
+    for _ in range(2):
+        f = rng.standard_normal(s.shape)
+        plan.set_source(f)
+        plan.solve()
+        u_plan = plan.download()
+        s.f_field = f
+        _, u_one, _ = ctx.solve(s, opt)
+        _, u_o, _ = O.solve(s, opt)
+        assert np.array_equal(u_plan, u_one)
+        assert rel(u_plan, u_o) < 1e-9
+    plan.close()

@@ -193,3 +193,22 @@ def test_bicgstab_gmres_jacobi_pc_upwind_dense():
                                              bc_rows=capi.ROWS_FOLDED, gmres_restart=30, max_iter=5000))
         assert st == 0, (m, st)
         np.testing.assert_allclose(u.reshape(-1), x_ref, rtol=1e-8, atol=1e-10)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_flux_two_layer_conductor_harmonic_face(dim):
+    """The FLUX operator's face conductivity is the harmonic mean of its two nodes: a two-layer
+    conductor is reproduced to rounding at every node, which an arithmetic-mean face misses by
+    far more than the tolerance (both solutions compared below)."""
+    from tests.closed_forms import arithmetic_mean_solution, two_layer_conductor
+    nx, m, k1, k2 = 21, 8, 1.0, 10.0
+    s, exact = two_layer_conductor(nx, m, k1, k2, ny=9, nz=9 if dim == 3 else 1)
+    arith = arithmetic_mean_solution(nx, m, k1, k2)
+    assert np.abs(arith - exact.reshape(-1, nx)[0]).max() > 1e-2
+    for method in (capi.MGPCG, capi.CG):
+        opt = capi.options(method, tol=1e-13, tol_mode=capi.TOL_REL, bc_rows=capi.ROWS_FOLDED,
+                           precond=capi.PC_MG if method == capi.MGPCG else capi.PC_NONE,
+                           max_iter=5000)
+        st, u, rep = O.solve(s, opt)
+        assert st == 0
+        assert np.abs(u - exact).max() < 1e-11, (method, np.abs(u - exact).max())
