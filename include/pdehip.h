@@ -24,6 +24,8 @@
  *                               per solve, src/backends/cuda/solvers/relaxation.cu:272-316)
  *   pdehip_solve_batch       <- the sequential batch runner src/main.cpp:2071-2097
  *   pdehip_server_*          <- no reference counterpart (request loop, continuous batching)
+ *   pdehip_residual_norms_noalloc <- no reference counterpart (allocation-free variant of
+ *                               pdehip_residual_norms for callers that cannot allocate)
  *
  * Status codes map onto the reference's SolveOutput.error idiom
  * (src/backends/cuda/CudaSolve.cu:33-45); pdehip_last_error() returns the
@@ -205,6 +207,18 @@ int pdehip_apply(pdehip_ctx* ctx, const pdehip_problem* prob, const double* x, d
 /* ComputeResidualNorms on the device (residual.cpp:153-583, Cartesian, no shapes/integrals). */
 int pdehip_residual_norms(pdehip_ctx* ctx, const pdehip_problem* prob, const double* u,
                           double* l2, double* linf);
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): pdehip_residual_norms
+ * on device-resident inputs.  u_dev and every array member of *prob (f_field, kappa, face
+ * samples) are DEVICE pointers; l2 and linf are host outputs.  The call allocates nothing: its
+ * partial sums, flags and sampled boundary data live in a static scratch array of the library
+ * (2.13 MB per device, held until the process ends).  Calls that overlap on one device share
+ * that array and race.  CONST and FLUX operators only (PDEHIP_ERR_UNSUPPORTED for POINTWISE);
+ * a 2D grid side above 16385 nodes is rejected (PDEHIP_ERR_INVALID).  Results equal
+ * pdehip_residual_norms on the same data. */
+int pdehip_residual_norms_noalloc(pdehip_ctx* ctx, const pdehip_problem* prob, const double* u_dev,
+                                  double* l2, double* linf);
 
 /* ---------------------------------------------------------------- resident plans */
 int pdehip_plan_create(pdehip_ctx* ctx, const pdehip_problem* prob, const pdehip_options* opt,

@@ -123,6 +123,7 @@ int pdehip_create(int device, size_t hbm_budget_bytes, pdehip_ctx** out) {
     ctx->c.budget = hbm_budget_bytes ? hbm_budget_bytes : static_cast<size_t>(0.8 * free_b);
     ctx->c.pool_cap = ctx->c.budget / 8;   // released blocks kept for reuse (no hipFree syncs)
     HIPCHK(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+    noalloc_init(device);   // the static scratch array's address on this device
     return PDEHIP_OK;
   });
   if (st == PDEHIP_OK) *out = ctx.release();
@@ -182,6 +183,17 @@ int pdehip_residual_norms(pdehip_ctx* ctx, const pdehip_problem* prob, const dou
     DBuf du(&ctx->c, plan->g.n * sizeof(double));
     HIPCHK(hipMemcpyAsync(du.get(), u, plan->g.n * sizeof(double), hipMemcpyHostToDevice, ctx->c.stream));
     plan_residual_norms(*plan, du.as<double>(), l2, linf);
+    return PDEHIP_OK;
+  });
+}
+
+int pdehip_residual_norms_noalloc(pdehip_ctx* ctx, const pdehip_problem* prob, const double* u_dev,
+                                  double* l2, double* linf) {
+  if (!ctx || !prob || !u_dev || !l2 || !linf) return PDEHIP_ERR_INVALID;
+  return guarded(ctx, [&]() {
+    std::lock_guard<std::mutex> g(ctx->c.mu);
+    HIPCHK(hipSetDevice(ctx->c.device));
+    residual_norms_noalloc(&ctx->c, *prob, u_dev, l2, linf);
     return PDEHIP_OK;
   });
 }

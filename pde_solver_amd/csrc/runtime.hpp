@@ -189,6 +189,12 @@ void plan_download(Plan& P, double* u_out);
 size_t plan_bytes(const Plan& P);
 void plan_residual_norms(Plan& P, const double* u_dev, double* l2, double* linf);
 
+// Allocation-free residual norms (noalloc.hip): looks up the static scratch array's address
+// on `device` (call with that device current), and the entry itself.
+void noalloc_init(int device);
+void residual_norms_noalloc(Ctx* ctx, const pdehip_problem& p, const double* u_dev, double* l2,
+                            double* linf);
+
 // Standalone entry points.
 void op_apply(Ctx* ctx, const pdehip_problem& p, const double* x, double* y);
 void time_series(Ctx* ctx, const pdehip_problem& p, const pdehip_time_config& tc,

@@ -41,6 +41,7 @@ def lib():
         L.pdehip_solve.argtypes = [VP, P, O, DP, R]
         L.pdehip_apply.argtypes = [VP, P, DP, DP]
         L.pdehip_residual_norms.argtypes = [VP, P, DP, DP, DP]
+        L.pdehip_residual_norms_noalloc.argtypes = [VP, P, VP, DP, DP]
         L.pdehip_plan_create.argtypes = [VP, P, O, C.POINTER(VP)]
         L.pdehip_plan_set_source.argtypes = [VP, DP]
         L.pdehip_plan_solve.argtypes = [VP, R]
@@ -149,6 +150,13 @@ class Context:
         l2, linf = C.c_double(), C.c_double()
         self._check(lib().pdehip_residual_norms(self.h, C.byref(p), capi.dptr(uu), C.byref(l2), C.byref(linf)))
         return l2.value, linf.value
+
+    def residual_norms_noalloc(self, prob: capi.Problem, u_dev_ptr: int):
+        """pdehip_residual_norms_noalloc: prob's arrays and u are device addresses (ints)."""
+        l2, linf = C.c_double(), C.c_double()
+        st = lib().pdehip_residual_norms_noalloc(self.h, C.byref(prob), C.c_void_p(u_dev_ptr),
+                                                 C.byref(l2), C.byref(linf))
+        return st, l2.value, linf.value
 
     def time_series(self, spec: capi.Spec, tc: capi.TimeConfig, keep_frames: bool = True,
                     on_frame=None):
