@@ -419,6 +419,136 @@ def batch_options(precision: int, tol: float):
                         max_iter=200, inner_tol=1e-4)
 
 
+def max_rel_err(us, refs):
+    return max(float(np.linalg.norm(u - r) / max(np.linalg.norm(r), 1e-300)) for u, r in zip(us, refs))
+
+
+def batch_run(ctx, specs, precision, tol, fp16, steps, warmup, u_ref, roofline=False):
+    """One resident lock-step batch of every problem (autotuned shapes, graph replay): solves/s,
+    batch p50/p99, error against u_ref and, with roofline, the level-0 launches' HIP-event
+    timing."""
+    from pde_solver_amd import capi, hip
+    tuned = capi.BatchConfig(autotune=1, use_graph=1)
+    hip.tune_set("mixed.fp16", fp16)
+    try:
+        t0 = time.perf_counter()
+        b = ctx.batch(specs, batch_options(precision, tol), tuned)
+        setup = time.perf_counter() - t0
+    finally:
+        hip.tune_set("mixed.fp16", 1)
+    for _ in range(warmup):
+        b.solve()
+    lat, dev = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        _, reps = b.solve()
+        lat.append((time.perf_counter() - t1) * 1e3)
+        dev.append(reps[0].solve_ms)
+    wall = time.perf_counter() - t0
+    us = [b.download(i) for i in range(len(specs))]
+    out = {"solves_per_s": round(len(specs) * steps / wall, 1),
+           "batch_ms_p50": round(pct(lat, 50), 3), "batch_ms_p99": round(pct(lat, 99), 3),
+           "device_ms_p50": round(pct(dev, 50), 3), "setup_s": round(setup, 2),
+           "max_err_vs_fp64_oracle": max_rel_err(us, u_ref),
+           "max_rel_residual": max(r.rel_residual for r in reps),
+           "all_converged": all(r.status == 0 for r in reps),
+           "resident_GiB": round(b.bytes() / 2**30, 2),
+           "iterations_max": max(r.iterations for r in reps),
+           "inner_iterations_max": max(r.inner_iterations for r in reps)}
+    if roofline:
+        kern = {}
+        for k, name in ((0, "sweep"), (1, "residual"), (2, "apply_dot"), (3, "residual_fp64")):
+            ms, nb = b.time_kernel(k, 10)
+            kern[name] = {"avg_ms": round(ms, 4), "GB/s": round(nb / (ms * 1e-3) / 1e9, 1),
+                          "algorithmic_bytes": nb}
+        out["level0_kernels"] = kern
+    b.close()
+    return out
+
+
+def batch_plain(ctx, specs, u_ref):
+    """The plain FP64 path: one resident plan per problem, solved one after the other."""
+    from pde_solver_amd import capi
+    plans = [ctx.plan(s, batch_options(capi.FP64, 1e-8)) for s in specs]
+    for pl in plans[:8]:
+        pl.solve()
+    lat = []
+    t0 = time.perf_counter()
+    for pl in plans:
+        t1 = time.perf_counter()
+        pl.solve()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    wall = time.perf_counter() - t0
+    err = max_rel_err([pl.download() for pl in plans], u_ref)
+    for pl in plans:
+        pl.close()
+    return {"solves_per_s": round(len(specs) / wall, 1),
+            "latency_ms_p50": round(pct(lat, 50), 3), "latency_ms_p99": round(pct(lat, 99), 3),
+            "max_err_vs_fp64_oracle": err}
+
+
+def batch_cpu(specs, seconds):
+    """The CPU oracle's FP64 MG-PCG over the batch's problems in batch order (a random size
+    mix) until about `seconds` of work."""
+    from oracle import oracle as O
+    from pde_solver_amd import capi
+    threads = cpu_threads()
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    done = 0
+    while done < len(specs) and (time.perf_counter() - t0 < seconds or done < 8):
+        O.solve(specs[done], batch_options(capi.FP64, 1e-8))
+        done += 1
+    total = time.perf_counter() - t0
+    cpu = {"value": round(done / total, 2), "unit": "solves/s", "cores": threads,
+           "kind": "port", "build": CPU_BUILD["flags"],
+           "sample": f"the first {done} problems of the batch "
+                     f"({sum(s.n for s in specs[:done]) / 1e6:.1f}M nodes), FP64 MG-PCG "
+                     f"to rel 1e-8, one after the other, {total:.1f}s wall"}
+    log(f"cpu baseline (batch): {cpu['value']} solves/s on {threads} threads (sample)")
+    return cpu
+
+
+_THREADS = []
+
+
+def cpu_threads() -> int:
+    """Host cores this process may use, read once: after the OpenMP runtime binds its threads
+    (OMP_PROC_BIND=close) the main thread's own affinity is a single core."""
+    if not _THREADS:
+        t = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        _THREADS.append(max(1, min(t, 64)))
+    return _THREADS[0]
+
+
+CPU_BUILD = {"flags": "-O3 -march=x86-64-v3 (prebuilt)"}
+
+
+def cpu_setup() -> None:
+    """The CPU baseline's conditions: OpenMP threads bound close, one per host core this
+    process may use, and the oracle rebuilt with -march=native on this host (the prebuilt
+    x86-64-v3 library when no compiler is there).  Called before anything loads OpenMP."""
+    import subprocess
+    import tempfile
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ["OMP_PLACES"] = "cores"
+    os.environ["OMP_NUM_THREADS"] = str(cpu_threads())
+    out = tempfile.mkdtemp(prefix="oracle_native_")
+    try:
+        r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native",
+                            f"NATIVE_OUT={out}"], capture_output=True, text=True, timeout=180)
+        lib = os.path.join(out, "liboracle.so")
+        if r.returncode == 0 and os.path.exists(lib):
+            from oracle import oracle as O
+            O.use_library(lib)
+            CPU_BUILD["flags"] = "-O3 -march=native (built on this host)"
+        else:
+            log(f"native oracle build failed, using the prebuilt library: {r.stderr[-300:]}")
+    except Exception as e:   # no make / compiler on this host
+        log(f"native oracle build unavailable ({e}), using the prebuilt library")
+
+
 def bench_batch(args) -> int:
     """Batched uneven sizes: 512 2D problems per step.  Mixed batched (FP32 MG-PCG with FP64
     residual replacement) vs FP64 batched vs the plain FP64 path (one resident plan solved after
@@ -447,46 +577,8 @@ def bench_batch(args) -> int:
     ref.close()
     log(f"FP64 oracle batch (rel-resid 1e-12) in {time.perf_counter() - t:.1f}s incl. set-up")
 
-    def errs(us):
-        return max(float(np.linalg.norm(u - r) / max(np.linalg.norm(r), 1e-300)) for u, r in zip(us, u_ref))
-
     def run(precision, tol, fp16, steps, roofline=False):
-        hip.tune_set("mixed.fp16", fp16)
-        try:
-            t0 = time.perf_counter()
-            b = ctx.batch(specs, batch_options(precision, tol), tuned)
-            setup = time.perf_counter() - t0
-        finally:
-            hip.tune_set("mixed.fp16", 1)
-        for _ in range(args.warmup):
-            b.solve()
-        lat, dev = [], []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            t1 = time.perf_counter()
-            _, reps = b.solve()
-            lat.append((time.perf_counter() - t1) * 1e3)
-            dev.append(reps[0].solve_ms)
-        wall = time.perf_counter() - t0
-        us = [b.download(i) for i in range(len(specs))]
-        out = {"solves_per_s": round(len(specs) * steps / wall, 1),
-               "batch_ms_p50": round(pct(lat, 50), 3), "batch_ms_p99": round(pct(lat, 99), 3),
-               "device_ms_p50": round(pct(dev, 50), 3), "setup_s": round(setup, 2),
-               "max_err_vs_fp64_oracle": errs(us),
-               "max_rel_residual": max(r.rel_residual for r in reps),
-               "all_converged": all(r.status == 0 for r in reps),
-               "resident_GiB": round(b.bytes() / 2**30, 2),
-               "iterations_max": max(r.iterations for r in reps),
-               "inner_iterations_max": max(r.inner_iterations for r in reps)}
-        if roofline:
-            kern = {}
-            for k, name in ((0, "sweep"), (1, "residual"), (2, "apply_dot"), (3, "residual_fp64")):
-                ms, nb = b.time_kernel(k, 10)
-                kern[name] = {"avg_ms": round(ms, 4), "GB/s": round(nb / (ms * 1e-3) / 1e9, 1),
-                              "algorithmic_bytes": nb}
-            out["level0_kernels"] = kern
-        b.close()
-        return out
+        return batch_run(ctx, specs, precision, tol, fp16, steps, args.warmup, u_ref, roofline)
 
     mixed = run(capi.MIXED, 1e-8, 0, args.steps, roofline=True)
     log(f"mixed batched (FP32 coarse): {mixed['solves_per_s']} solves/s, batch p50 "
@@ -500,45 +592,9 @@ def bench_batch(args) -> int:
     log(f"at rel-resid 1e-10: mixed {tight_mixed['solves_per_s']} (err "
         f"{tight_mixed['max_err_vs_fp64_oracle']:.1e}), fp64 {tight_fp64['solves_per_s']} solves/s")
 
-    # plain FP64 path: one resident plan per problem, solved one after the other
-    plans = [ctx.plan(s, batch_options(capi.FP64, 1e-8)) for s in specs]
-    for pl in plans[:8]:
-        pl.solve()
-    lat = []
-    t0 = time.perf_counter()
-    for pl in plans:
-        t1 = time.perf_counter()
-        pl.solve()
-        lat.append((time.perf_counter() - t1) * 1e3)
-    wall_plain = time.perf_counter() - t0
-    err_plain = errs([pl.download() for pl in plans])
-    for pl in plans:
-        pl.close()
-    plain = {"solves_per_s": round(len(specs) / wall_plain, 1),
-             "latency_ms_p50": round(pct(lat, 50), 3), "latency_ms_p99": round(pct(lat, 99), 3),
-             "max_err_vs_fp64_oracle": err_plain}
+    plain = batch_plain(ctx, specs, u_ref)
     log(f"plain fp64 (sequential resident plans): {plain['solves_per_s']} solves/s")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        from oracle import oracle as O
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        threads = max(1, min(threads, 64))
-        O.set_threads(threads)
-        # problems in batch order (the order is random, so the size mix is the batch's) until
-        # about 15 s of CPU work
-        t0 = time.perf_counter()
-        done = 0
-        while done < len(specs) and (time.perf_counter() - t0 < 15.0 or done < 8):
-            O.solve(specs[done], batch_options(capi.FP64, 1e-8))
-            done += 1
-        total = time.perf_counter() - t0
-        cpu = {"value": round(done / total, 2), "unit": "solves/s", "cores": threads,
-               "kind": "port",
-               "sample": f"the first {done} problems of the batch "
-                         f"({sum(s.n for s in specs[:done]) / 1e6:.1f}M nodes), FP64 MG-PCG "
-                         f"to rel 1e-8, one after the other, {total:.1f}s wall"}
-        log(f"cpu baseline: {cpu['value']} solves/s on {threads} threads (sample)")
+    cpu = None if args.no_cpu_baseline else batch_cpu(specs, 15.0)
 
     sw = mixed["level0_kernels"]["sweep"]
     out = {
@@ -605,14 +661,175 @@ def pct(v, q):
     return float(np.percentile(np.asarray(v), q))
 
 
-def cpu_baseline(spec, n_threads: int):
+def cpu_baseline(spec, n_threads: int, runs: int = 5):
+    """The oracle's FP64 MG-PCG on the same problem: median solve-phase time of `runs` solves
+    (the oracle's own steady_clock around the solve; set-up reported separately)."""
     from oracle import oracle as O
     from pde_solver_amd import capi
     O.set_threads(n_threads)
-    t = time.perf_counter()
-    st, u, rep = O.solve(spec, options(capi.FP64, 1e-8))
-    total = time.perf_counter() - t
-    return {"status": st, "solve_s": rep.solve_ms / 1e3, "total_s": total, "iterations": rep.iterations}, u
+    solve_s, total_s = [], []
+    for _ in range(runs):
+        t = time.perf_counter()
+        st, u, rep = O.solve(spec, options(capi.FP64, 1e-8))
+        total_s.append(time.perf_counter() - t)
+        solve_s.append(rep.solve_ms / 1e3)
+    return {"status": st, "solve_s": statistics.median(solve_s), "total_s": statistics.median(total_s),
+            "iterations": rep.iterations, "runs": runs}, u
+
+
+def measure_traffic(n: int):
+    """HBM bytes per launch of the fine-level FP32 sweep from two rocprofv3 counter passes over
+    this script's kernel-timing mode (FETCH_SIZE, then WRITE_SIZE), each scaled by what the
+    same counter reads for the FP32 copy of one level-0 vector (N * 4 bytes read and written).
+    Returns (bytes, detail) or (None, reason)."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    if not shutil.which("rocprofv3"):
+        return None, "rocprofv3 not found"
+    med = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix="pmc_")
+        cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", counter, "--output-format",
+               "csv", "-d", out, "-o", "pmc", "--", sys.executable, os.path.join(ROOT, "bench.py"),
+               "--kernels", "4,3", "--precision", "mixed", "--steps", "4", "--n", str(n)]
+        env = dict(os.environ, TMPDIR="/tmp")
+        log(f"traffic: {counter} pass ...")
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=180, env=env, cwd="/tmp")
+        except subprocess.TimeoutExpired:
+            return None, f"{counter} pass timed out"
+        files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+        if r.returncode not in (0, 139) or not files:
+            return None, f"{counter} pass failed (rc {r.returncode})"
+        vals = {}
+        with open(files[0]) as fh:
+            for row in csv.DictReader(fh):
+                name = row.get("Kernel_Name", "")
+                if row.get("Counter_Name") != counter:
+                    continue
+                key = "sweep" if "k_rbgs_ring3" in name else ("copy" if "k_convert" in name else None)
+                if key:
+                    vals.setdefault(key, []).append(float(row["Counter_Value"]))
+        if "sweep" not in vals or "copy" not in vals:
+            return None, f"{counter}: kernels not found in the counter output"
+        med[counter] = {k: statistics.median(v) for k, v in vals.items()}
+        shutil.rmtree(out, ignore_errors=True)
+    nbytes = float(n) ** 3 * 4.0
+    scale_f = nbytes / (med["FETCH_SIZE"]["copy"] * 1024.0)
+    scale_w = nbytes / (med["WRITE_SIZE"]["copy"] * 1024.0)
+    traffic = med["FETCH_SIZE"]["sweep"] * 1024.0 * scale_f + med["WRITE_SIZE"]["sweep"] * 1024.0 * scale_w
+    detail = {"fetch_KiB": med["FETCH_SIZE"]["sweep"], "write_KiB": med["WRITE_SIZE"]["sweep"],
+              "copy_fetch_KiB": med["FETCH_SIZE"]["copy"], "copy_write_KiB": med["WRITE_SIZE"]["copy"],
+              "fetch_scale": round(scale_f, 4), "write_scale": round(scale_w, 4)}
+    return traffic, detail
+
+
+def bench_cfg1(ctx, with_cpu: bool):
+    """The plain CPU workload on both sides: 129^2 Poisson, Dirichlet 0, f = 2 pi^2 sin sin;
+    CG to relative residual 1e-10 and Jacobi to max|du| < 1e-10.  GPU: median of 5 (CG) or 3
+    (Jacobi) one-shot solves, set-up included as a caller sees it.  CPU: the oracle around the
+    whole solve call (the right-hand side build is negligible at this size), median of 5 CG
+    solves, and a 5000-sweep Jacobi sample whose per-sweep time is scaled to the full count."""
+    from pde_solver_amd import capi
+    n = 129
+    s = capi.Spec(n, n, a=-1.0, b=-1.0)
+    X, Y, _ = s.coords()
+    exact = np.sin(np.pi * X) * np.sin(np.pi * Y)
+    s.f_field = -2 * np.pi ** 2 * exact
+    h = 1.0 / (n - 1)
+    closed = 2 * np.pi ** 2 / ((8 / h ** 2) * np.sin(np.pi * h / 2) ** 2) - 1.0
+    cg = capi.options(capi.CG, tol=1e-10, tol_mode=capi.TOL_REL, max_iter=2000)
+    jac = capi.options(capi.JACOBI, tol=1e-10, max_iter=60000)
+    out = {"workload": "129^2 Poisson, Dirichlet 0, f = 2 pi^2 sin(pi x) sin(pi y); CG to "
+                       "rel-resid 1e-10, Jacobi to max|du| < 1e-10",
+           "mms_closed_form": closed}
+    for name, opt in (("cg", cg), ("jacobi", jac)):
+        ctx.solve(s, opt)
+        ms = []
+        for _ in range(5 if name == "cg" else 3):
+            t = time.perf_counter()
+            st, u, rep = ctx.solve(s, opt)
+            ms.append((time.perf_counter() - t) * 1e3)
+        out[f"gpu_{name}"] = {"ms_median": round(statistics.median(ms), 3), "iterations": rep.iterations,
+                              "device_ms": round(rep.solve_ms, 3), "status": st,
+                              "mms_linf": float(np.abs(u - exact).max())}
+    log(f"cfg1: GPU CG {out['gpu_cg']['ms_median']} ms ({out['gpu_cg']['iterations']} its), Jacobi "
+        f"{out['gpu_jacobi']['ms_median']} ms ({out['gpu_jacobi']['iterations']} sweeps)")
+    out["_cpu"] = (s, cg, exact) if with_cpu else None
+    return out
+
+
+def bench_cfg1_cpu(out):
+    """The CPU half of bench_cfg1 (run after every GPU measurement)."""
+    from pde_solver_amd import capi
+    s, cg, exact = out.pop("_cpu")
+    from oracle import oracle as O
+    O.set_threads(cpu_threads())
+    ms = []
+    for _ in range(5):
+        t = time.perf_counter()
+        st, u, rep = O.solve(s, cg)
+        ms.append((time.perf_counter() - t) * 1e3)
+    out["cpu_cg"] = {"ms_median": round(statistics.median(ms), 3), "iterations": rep.iterations,
+                     "mms_linf": float(np.abs(u - exact).max()), "cores": cpu_threads(),
+                     "build": CPU_BUILD["flags"]}
+    sample = capi.options(capi.JACOBI, tol=0.0, max_iter=5000)
+    ms = []
+    for _ in range(3):
+        t = time.perf_counter()
+        _, _, rep = O.solve(s, sample)
+        ms.append((time.perf_counter() - t) * 1e3)
+    per = statistics.median(ms) / rep.iterations
+    full = out["gpu_jacobi"]["iterations"]
+    out["cpu_jacobi"] = {"ms_per_sweep": round(per, 5), "sweeps": full,
+                         "ms_full_estimate": round(per * full, 1), "cores": cpu_threads(),
+                         "sample": f"median of 3 runs of {rep.iterations} sweeps",
+                         "build": CPU_BUILD["flags"]}
+    log(f"cfg1: CPU CG {out['cpu_cg']['ms_median']} ms, Jacobi {per * full:.0f} ms (from a sample)")
+
+
+def batched_summary(ctx, steps: int, warmup: int, with_cpu: bool):
+    """The batched workload (512 uneven 2D problems) in the default line: mixed batched and
+    FP64 batched lock-step solves, the plain FP64 path, the batched sweep's HIP-event roofline
+    and the CPU oracle on a bounded sample.  Errors are against the FP64 batch at 1e-12."""
+    from pde_solver_amd import capi
+    specs = make_batch_problems(512)
+    ref = ctx.batch(specs, batch_options(capi.FP64, 1e-12), capi.BatchConfig(autotune=1, use_graph=1))
+    ref.solve()
+    u_ref = [ref.download(i) for i in range(len(specs))]
+    ref.close()
+    mixed = batch_run(ctx, specs, capi.MIXED, 1e-8, 0, steps, warmup, u_ref, roofline=True)
+    fp64 = batch_run(ctx, specs, capi.FP64, 1e-8, 1, steps, warmup, u_ref)
+    plain = batch_plain(ctx, specs, u_ref)
+    # rel-resid 1e-8 leaves errors up to ~2e-8 on the near-singular problems; at 2e-9 both
+    # batched paths stay within 1e-8 of the FP64 solution
+    acc = {"tol": 2e-9,
+           "mixed_batched": batch_run(ctx, specs, capi.MIXED, 2e-9, 0, steps, warmup, u_ref),
+           "fp64_batched": batch_run(ctx, specs, capi.FP64, 2e-9, 1, steps, warmup, u_ref)}
+    acc["speedup_mixed_vs_fp64_batched"] = round(acc["mixed_batched"]["solves_per_s"] /
+                                                 acc["fp64_batched"]["solves_per_s"], 2)
+    acc["speedup_mixed_batched_vs_plain_fp64"] = round(acc["mixed_batched"]["solves_per_s"] /
+                                                       plain["solves_per_s"], 2)
+    sw = mixed.pop("level0_kernels")["sweep"]
+    out = {"workload": "512 2D problems, n log-uniform in 33..1025, half Poisson / half Helmholtz "
+                       "c~U(0,50), 8-mode Fourier RHS, D/N/R per face, MG-PCG V(2,2) to rel-resid 1e-8",
+           "nodes": sum(s.n for s in specs),
+           "mixed_batched": mixed, "fp64_batched": fp64, "plain_fp64": plain,
+           "speedup_mixed_batched_vs_plain_fp64": round(mixed["solves_per_s"] / plain["solves_per_s"], 2),
+           "speedup_mixed_batched_vs_fp64_batched": round(mixed["solves_per_s"] / fp64["solves_per_s"], 2),
+           "roofline": {"bound": "hbm", "kernel": "kb_sweep<P32> over every problem's level 0",
+                        "achieved": sw["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(sw["GB/s"] / HBM_PEAK_GBS, 4),
+                        "algorithmic_bytes": sw["algorithmic_bytes"], "avg_ms": sw["avg_ms"]},
+           "at_error_1e-8": acc,
+           "cpu_baseline": None,
+           "_cpu": specs if with_cpu else None}
+    log(f"batched: mixed {mixed['solves_per_s']}, fp64 {fp64['solves_per_s']}, plain fp64 "
+        f"{plain['solves_per_s']} solves/s")
+    return out
 
 
 def main() -> int:
@@ -635,12 +852,18 @@ def main() -> int:
     ap.add_argument("--count", type=int, default=None,
                     help="batch: problems per step (512); nonsym: problems timed (24 of 128)")
     ap.add_argument("--rd-steps", type=int, default=1000, help="nonsym: RK4 steps at 2049^2")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure roofline.traffic")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload: only the 257^3 line (no 129^2 or batched rows)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
               "(--gpus must be 1)", file=sys.stderr)
         return 2
 
+    if not args.no_cpu_baseline and not args.kernels and not args.only:
+        cpu_setup()
     if args.workload == "batch":
         if args.count is None:
             args.count = 512
@@ -653,6 +876,12 @@ def main() -> int:
         if args.count is None:
             args.count = 400
         return bench_serve(args)
+
+    # counter passes first, in child processes, before this process touches the GPU
+    traffic, traffic_detail = None, "not measured (--no-traffic)"
+    if not args.kernels and not args.only and not args.no_traffic:
+        traffic, traffic_detail = measure_traffic(args.n)
+        log(f"traffic: {traffic} bytes per sweep ({traffic_detail})")
 
     from pde_solver_amd import capi, hip
 
@@ -731,16 +960,12 @@ def main() -> int:
     ms_rs, bytes_rs = plan64.time_kernel(2, capi.FP64, reps=20)
     ms_cp, bytes_cp = plan64.time_kernel(3, capi.FP64, reps=20)
     gbs = lambda b, ms: b / (ms * 1e-3) / 1e9  # noqa: E731
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "roofline_traffic.json")
-    if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get("rbgs_sweep_fp32_bytes_per_launch")
-        except Exception:
-            traffic = None
     roof = {"bound": "hbm", "kernel": f"k_rbgs_ring3 (red+black sweep, one launch), FP32, fine level {n}^3",
             "achieved": round(gbs(bytes_mx, ms_mx), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs(bytes_mx, ms_mx) / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "frac": round(gbs(bytes_mx, ms_mx) / HBM_PEAK_GBS, 4),
+            "traffic": round(traffic) if traffic else None,
+            "traffic_over_algorithmic": round(traffic / bytes_mx, 3) if traffic else None,
+            "traffic_counters": traffic_detail,
             "algorithmic_bytes": bytes_mx, "avg_ms": round(ms_mx, 4)}
     kernels = {
         "rbgs_sweep_fp64": {"avg_ms": round(ms_64, 4), "GB/s": round(gbs(bytes_64, ms_64), 1)},
@@ -752,20 +977,49 @@ def main() -> int:
     log(f"roofline: rbgs fp32 {roof['achieved']} GB/s, rbgs fp64 {kernels['rbgs_sweep_fp64']['GB/s']} GB/s, "
         f"apply fp64 {kernels['apply_fp64']['GB/s']} GB/s, copy {kernels['stream_copy_fp64']['GB/s']} GB/s")
 
-    # ---- CPU baseline: the oracle's FP64 MG-PCG (OpenMP) on the same problem
+    # ---- a tolerance at which both paths' error against the FP64 solution is <= 1e-8 (rel-resid
+    # 1e-8 leaves ~1.5e-8 on this problem)
+    tol_acc = 2e-9
+    acc = {"tol": tol_acc}
+    for name, prec in (("mixed", capi.MIXED), ("fp64", capi.FP64)):
+        pl = ctx.plan(spec, options(prec, tol_acc))
+        w, la, _, rp = timed_solves(pl, 5, 1)
+        acc[name] = {"solves_per_s": round(5 / w, 3), "p99_ms": round(pct(la, 99), 3),
+                     "err_vs_fp64_oracle": float(np.linalg.norm(pl.download() - u_ref) / nref),
+                     "rel_residual": rp[-1].rel_residual,
+                     "iterations": rp[-1].inner_iterations if prec == capi.MIXED else rp[-1].iterations}
+        pl.close()
+    acc["speedup_mixed_vs_fp64"] = round(acc["mixed"]["solves_per_s"] / acc["fp64"]["solves_per_s"], 3)
+    log(f"at tol {tol_acc}: mixed {acc['mixed']['solves_per_s']} (err {acc['mixed']['err_vs_fp64_oracle']:.1e}), "
+        f"fp64 {acc['fp64']['solves_per_s']} (err {acc['fp64']['err_vs_fp64_oracle']:.1e})")
+
+    plan64.close()
+    planmx.close()
+    extras = {}
+    if not args.no_extras:
+        extras["plain_cpu_workload_129"] = bench_cfg1(ctx, not args.no_cpu_baseline)
+        extras["batched_512"] = batched_summary(ctx, 5, 2, not args.no_cpu_baseline)
+
+    # ---- CPU baselines last (the OpenMP runtime binds this process's main thread): the oracle's
+    # FP64 MG-PCG on the same problem, median of 5; then the 129^2 and batched rows' CPU legs
     cpu = None
     if not args.no_cpu_baseline:
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        threads = max(1, min(threads, 64))
-        log(f"cpu baseline: oracle FP64 MG-PCG on {threads} threads ...")
+        threads = cpu_threads()
+        log(f"cpu baseline: oracle FP64 MG-PCG on {threads} threads, 5 solves ...")
         res, u_cpu = cpu_baseline(spec, threads)
         cpu = {"value": round(1.0 / res["solve_s"], 4), "unit": "solves/s", "cores": threads,
-               "kind": "port",
-               "sample": f"one full {n}^3 FP64 MG-PCG solve to rel-resid 1e-8 "
+               "kind": "port", "build": CPU_BUILD["flags"],
+               "omp": f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} OMP_PLACES={os.environ.get('OMP_PLACES')}",
+               "sample": f"median of {res['runs']} full {n}^3 FP64 MG-PCG solves to rel-resid 1e-8 "
                          f"({res['iterations']} iterations, solve phase {res['solve_s']:.2f}s, "
                          f"set-up excluded; {res['total_s']:.2f}s with set-up)",
                "rel_l2_vs_gpu_fp64": float(np.linalg.norm(u_cpu - u_64) / np.linalg.norm(u_64))}
         log(f"cpu baseline: {cpu['value']} solves/s")
+        if extras:
+            bench_cfg1_cpu(extras["plain_cpu_workload_129"])
+            extras["batched_512"]["cpu_baseline"] = batch_cpu(extras["batched_512"].pop("_cpu"), 15.0)
+    for e in extras.values():
+        e.pop("_cpu", None)
 
     value = args.steps / wall_mx
     out = {
@@ -795,13 +1049,13 @@ def main() -> int:
         "mixed": {"refinements": reps_mx[-1].refinements, "inner_iterations": reps_mx[-1].inner_iterations,
                   "rel_residual": reps_mx[-1].rel_residual, "err_vs_fp64_oracle": err_mx},
         "speedup_mixed_vs_fp64": round(value / (steps64 / wall_64), 3),
+        "at_error_1e-8": acc,
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        **extras,
     }
     print(json.dumps(out), flush=True)
-    plan64.close()
-    planmx.close()
     ctx.close()
     return 0
 

@@ -45,6 +45,14 @@ SolveOutput SolvePDEHip(const SolveInput& input);
 // batch; any other input is solved like SolvePDEHip.
 std::vector<SolveOutput> SolvePDEBatchHip(const std::vector<SolveInput>& inputs);
 
+// Legacy-VTK STRUCTURED_POINTS file of a solution grid (the layout the reference's
+// WriteVtkStructuredPoints writes, src/vtk_writer.cpp:28-60: DIMENSIONS / ORIGIN / SPACING,
+// point data x fastest) with `digits` significant digits per value.  17 digits round-trip FP64
+// exactly; the reference's writer keeps 6 decimals of a float.  Returns false and sets *error
+// when the grid does not match the domain or the file cannot be written.
+bool HipWriteVtk(const std::string& path, const Domain& domain, const std::vector<double>& grid,
+                 int digits, std::string* error);
+
 // Replaces SolvePDETimeSeriesMetal (src/backends/metal/MetalSolve.h:21-23) and the CPU
 // SolvePDETimeSeries (include/solver.h:13-14) for 2D explicit time stepping.
 SolveOutput SolvePDETimeSeriesHip(const SolveInput& input, const FrameCallback& on_frame,

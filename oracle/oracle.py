@@ -21,6 +21,15 @@ _oracle = None
 _ref = None
 
 
+def use_library(path: str) -> None:
+    """Load the oracle from another build of oracle.cpp (for example the -march=native one the
+    CPU baseline builds on the measuring host).  Must be called before the first oracle call."""
+    global ORACLE_SO
+    if _oracle is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    ORACLE_SO = path
+
+
 def _load_oracle():
     global _oracle
     if _oracle is None:

@@ -7,6 +7,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -264,6 +265,34 @@ void HipSetSolverOptions(const HipSolverOptions& options) {
 HipSolverOptions HipGetSolverOptions() {
   std::lock_guard<std::mutex> g(g_opt_mu);
   return g_opt;
+}
+
+bool HipWriteVtk(const std::string& path, const Domain& domain, const std::vector<double>& grid,
+                 int digits, std::string* error) {
+  auto fail = [&](const std::string& m) {
+    if (error) *error = m;
+    return false;
+  };
+  const int nx = domain.nx, ny = domain.ny, nz = std::max(1, domain.nz);
+  const size_t n = static_cast<size_t>(nx) * ny * nz;
+  if (nx < 1 || ny < 1 || grid.size() != n) return fail("grid size does not match domain");
+  std::FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) return fail("failed to open output file");
+  const int d = std::max(1, std::min(digits, 17));
+  auto step = [](double lo, double hi, int m) { return m > 1 ? (hi - lo) / (m - 1) : 1.0; };
+  std::fprintf(f, "# vtk DataFile Version 3.0\npde_solution\nASCII\nDATASET STRUCTURED_POINTS\n");
+  std::fprintf(f, "DIMENSIONS %d %d %d\n", nx, ny, nz);
+  std::fprintf(f, "ORIGIN %.17g %.17g %.17g\n", domain.xmin, domain.ymin, domain.zmin);
+  std::fprintf(f, "SPACING %.17g %.17g %.17g\n", step(domain.xmin, domain.xmax, nx),
+               step(domain.ymin, domain.ymax, ny), step(domain.zmin, domain.zmax, nz));
+  std::fprintf(f, "POINT_DATA %zu\nSCALARS solution double 1\nLOOKUP_TABLE default\n", n);
+  for (size_t row = 0; row < n / nx; ++row) {
+    for (int i = 0; i < nx; ++i)
+      std::fprintf(f, i + 1 < nx ? "%.*g " : "%.*g\n", d, grid[row * nx + i]);
+  }
+  const bool ok = std::ferror(f) == 0;
+  if (std::fclose(f) != 0 || !ok) return fail("failed to write output file");
+  return true;
 }
 
 bool HipIsAvailable(std::string* note) {

@@ -2,6 +2,9 @@
 // SolveWithBackend(..., BackendKind::HIP), plus HIP-only cases (3D Krylov/multigrid with Robin
 // faces, the time-series entry).  Prints one [OK]/[FAIL] line per case; exit code 0 when all pass.
 #include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
 #include <functional>
 #include <iostream>
 #include <string>
@@ -84,7 +87,8 @@ void Run(const std::string& name, const SolveInput& in,
 
 }  // namespace
 
-int main() {
+int main(int argc, char** argv) {
+  const std::string vtk_path = argc > 1 ? argv[1] : "/tmp/pdehip_cfg1_129.vtk";
   std::string note;
   if (!HipIsAvailable(&note)) {
     std::cout << "[FAIL] HIP unavailable: " << note << "\n";
@@ -251,6 +255,49 @@ int main() {
     std::cout << (pass ? "[OK] " : "[FAIL] ") << "heat decay time series ("
               << (ti == TimeIntegrator::RK4 ? "RK4" : "FE") << ") u_mid=" << mid << " expect=" << expect
               << " frames=" << frames_seen << " " << out.error << "\n";
+    if (!pass) ++g_fail;
+  }
+  // The plain CPU workload end to end on HIP: 129^2 Poisson, f = 2 pi^2 sin sin, CG to rel-resid
+  // 1e-10, written as .vtk and read back.  Max error against sin sin is the closed form
+  // 2 pi^2 / lambda_h - 1 = 5.020092e-05; 17 digits round-trip the grid exactly.
+  {
+    ++g_total;
+    SolveInput in;
+    in.domain = Unit(129);
+    in.pde.a = -1.0; in.pde.b = -1.0;
+    in.pde.rhs_latex = "-2*\\pi^2*\\sin(\\pi*x)*\\sin(\\pi*y)";
+    in.bc = DirichletAll(Linear(0.0, 0.0, 0.0));
+    Configure(&in, SolveMethod::CG, 2000, 1e-10);
+    HipSolverOptions ho;
+    ho.relative_tol = true;
+    HipSetSolverOptions(ho);
+    BackendKind sel = BackendKind::Auto;
+    std::string sel_note;
+    const SolveOutput out = SolveWithBackend(in, BackendKind::HIP, &sel, &sel_note);
+    HipSetSolverOptions(HipSolverOptions());
+    const std::string path = vtk_path;
+    std::string err;
+    bool ok = out.error.empty() && HipWriteVtk(path, in.domain, out.grid, 17, &err);
+    std::vector<double> back;
+    int dims[3] = {0, 0, 0};
+    if (ok) {
+      std::ifstream f(path);
+      std::string line, word;
+      while (std::getline(f, line) && line.rfind("LOOKUP_TABLE", 0) != 0) {
+        std::istringstream ls(line);
+        ls >> word;
+        if (word == "DIMENSIONS") ls >> dims[0] >> dims[1] >> dims[2];
+      }
+      double v;
+      while (f >> v) back.push_back(v);
+    }
+    ok = ok && dims[0] == 129 && dims[1] == 129 && dims[2] == 1 && back == out.grid;
+    const double e = ok ? ErrInf(in.domain, back, [](double x, double y, double) {
+      return std::sin(M_PI * x) * std::sin(M_PI * y);
+    }) : -1.0;
+    const bool pass = ok && std::abs(e - 5.020092e-05) < 1e-10;
+    std::cout << (pass ? "[OK] " : "[FAIL] ") << "129^2 Poisson CG on HIP, .vtk round trip (" << path
+              << ") MMS Linf=" << e << " (closed form 5.020092e-05) " << out.error << err << "\n";
     if (!pass) ++g_fail;
   }
   // Batch entry (the batch runner's replacement, src/main.cpp:2071-2097): uneven 2D MG-PCG

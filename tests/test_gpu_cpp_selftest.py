@@ -11,7 +11,12 @@ BIN = os.path.join(os.path.dirname(__file__), "..", "pde_solver_amd", "bin", "hi
 
 def test_cpp_self_test_through_solve_with_backend():
     assert os.path.exists(BIN), "build the host layer first (python -m pde_solver_amd.build)"
-    r = subprocess.run([BIN], capture_output=True, text=True, timeout=240)
+    # the 129^2 workload's .vtk output goes to gpurun_out/ when that folder exists (kept as a
+    # result of the run), else to a temporary file
+    out_dir = os.path.join(os.path.dirname(__file__), "..", "gpurun_out")
+    vtk = os.path.join(out_dir, "cfg1_poisson_129_cg_hip.vtk") if os.path.isdir(out_dir) else \
+        "/tmp/pdehip_cfg1_129.vtk"
+    r = subprocess.run([BIN, vtk], capture_output=True, text=True, timeout=240)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "[FAIL]" not in r.stdout
