@@ -349,43 +349,47 @@ def bench_serve(args) -> int:
                             precision=precision)
 
     results = {}
-    for precision, pname in ((capi.MIXED, "mixed"), (capi.FP64, "fp64")):
-        for lam in rates:
-            srv = ctx.server(capi.ServerConfig(max_inflight=8, streams=4, cache_operators=1))
-            arr = np.cumsum(np.random.default_rng(55).exponential(1.0 / lam, size=len(reqs)))
-            pending = {}
-            lat, fails = [], 0
-            t0 = time.perf_counter()
-            k = 0
-            last_done = t0
-            while k < len(reqs) or pending:
-                now = time.perf_counter() - t0
-                while k < len(reqs) and arr[k] <= now:
-                    spec, method = reqs[k]
-                    pending[srv.submit(spec, opts(method, precision))] = arr[k]
-                    k += 1
-                for tk in list(pending):
-                    st, _, rep = srv.poll(tk)
-                    if st == capi.PENDING:
-                        continue
-                    tdone = time.perf_counter() - t0
-                    lat.append((tdone - pending.pop(tk)) * 1e3)
-                    fails += int(st != 0)
-                    last_done = tdone
-                if k < len(reqs) and not pending:
-                    time.sleep(max(0.0, min(arr[k] - (time.perf_counter() - t0), 0.01)))
-                else:
-                    time.sleep(0.0002)
-            stats = srv.stats()
-            srv.close()
-            key = f"{pname}_lambda{int(lam)}"
-            results[key] = {"requests": len(reqs), "failed": fails,
-                            "solves_per_s": round(len(reqs) / last_done, 1),
-                            "p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
-                            "cache_hits": int(stats["cache_hits"]),
-                            "peak_GiB": round(stats["peak_bytes"] / 2**30, 2)}
-            log(f"{key}: {results[key]}")
     hi = max(rates)
+    runs = [(capi.MIXED, "mixed", lam, 2) for lam in rates] + \
+           [(capi.FP64, "fp64", lam, 2) for lam in rates] + \
+           [(capi.MIXED, "mixed_no_batch_lanes", hi, -1), (capi.FP64, "fp64_no_batch_lanes", hi, -1)]
+    for precision, pname, lam, lanes in runs:
+        srv = ctx.server(capi.ServerConfig(max_inflight=256, streams=4, cache_operators=1,
+                                           batch_lanes=lanes, batch_max=64))
+        arr = np.cumsum(np.random.default_rng(55).exponential(1.0 / lam, size=len(reqs)))
+        pending = {}
+        lat, fails = [], 0
+        t0 = time.perf_counter()
+        k = 0
+        last_done = t0
+        while k < len(reqs) or pending:
+            now = time.perf_counter() - t0
+            while k < len(reqs) and arr[k] <= now:
+                spec, method = reqs[k]
+                pending[srv.submit(spec, opts(method, precision))] = arr[k]
+                k += 1
+            for tk in list(pending):
+                st, _, rep = srv.poll(tk)
+                if st == capi.PENDING:
+                    continue
+                tdone = time.perf_counter() - t0
+                lat.append((tdone - pending.pop(tk)) * 1e3)
+                fails += int(st != 0)
+                last_done = tdone
+            if k < len(reqs) and not pending:
+                time.sleep(max(0.0, min(arr[k] - (time.perf_counter() - t0), 0.01)))
+            else:
+                time.sleep(0.0002)
+        stats = srv.stats()
+        srv.close()
+        key = f"{pname}_lambda{int(lam)}"
+        results[key] = {"requests": len(reqs), "failed": fails,
+                        "solves_per_s": round(len(reqs) / last_done, 1),
+                        "p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
+                        "cache_hits": int(stats["cache_hits"]),
+                        "batched": int(stats["batched"]), "lane_batches": int(stats["batches"]),
+                        "peak_GiB": round(stats["peak_bytes"] / 2**30, 2)}
+        log(f"{key}: {results[key]}")
     out = {
         "metric": METRIC,
         "value": results[f"mixed_lambda{int(hi)}"]["solves_per_s"],
@@ -848,7 +852,7 @@ def main() -> int:
                     help="single: the 257^3 solve (default); batch: 512 uneven 2D problems; "
                          "nonsym: convection-diffusion Krylov + reaction-diffusion RK4; "
                          "serve: Poisson-arrival request stream")
-    ap.add_argument("--rates", default="50,200", help="serve: arrival rates, req/s")
+    ap.add_argument("--rates", default="50,200,500,1000,2000", help="serve: arrival rates, req/s")
     ap.add_argument("--count", type=int, default=None,
                     help="batch: problems per step (512); nonsym: problems timed (24 of 128)")
     ap.add_argument("--rd-steps", type=int, default=1000, help="nonsym: RK4 steps at 2049^2")

@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define PDEHIP_ABI_VERSION 1
+#define PDEHIP_ABI_VERSION 2
 
 /* ---------------------------------------------------------------- status */
 enum pdehip_status {
@@ -325,16 +325,35 @@ void pdehip_batch_destroy(pdehip_batch* batch);
 
 /* ---------------------------------------------------------------- serving */
 typedef struct pdehip_server pdehip_server;
+/* The server runs two kinds of device lanes, each on its own HIP stream (default priority):
+ *  - plan workers (`streams`): one resident plan per request (any problem the backend solves);
+ *  - batch lanes (`batch_lanes`): requests the lock-step batched engine takes (2D CONST operators,
+ *    MG-PCG without convection with even smoothing counts, or FP64 BiCGStab) are gathered, as
+ *    many as are queued with the same options (up to `batch_max`), and solved together: one
+ *    launch per operation over every member.  While one lane's batch runs, arrivals start on
+ *    the next free lane.  Batch members keep fixed work-item shapes, so a request's result is
+ *    bitwise the same alone and in a mix.
+ * Admission: a request starts when the bytes reserved by running requests plus its own fit
+ * the budget; its bytes are the measured size of an earlier plan of the same shape and
+ * options, else an estimate.  Deadlines (timeout_ms) stop running solves on the device: plan
+ * workers through the solve's cancel flag, batch lanes by marking the member done between
+ * polling chunks; such requests end with PDEHIP_ERR_TIMEOUT. */
 typedef struct pdehip_server_config {
   size_t hbm_budget_bytes;  /* admission budget (0 = context budget) */
-  int max_inflight;         /* concurrent requests on the device (0 = default 8) */
-  int streams;              /* HIP streams used for overlap (0 = default 4) */
+  int max_inflight;         /* requests running on the device at once (0 = default 256) */
+  int streams;              /* plan-worker streams (0 = default 4) */
   int cache_operators;      /* keep operators/hierarchies resident across requests */
-  double timeout_ms;        /* per-request deadline, 0 = none */
+  double timeout_ms;        /* per-request deadline from submission, 0 = none */
+  int batch_lanes;          /* batch-lane streams (0 = default 2; -1 = none: every request
+                               runs on a plan worker) */
+  int batch_max;            /* most requests in one lane batch (0 = default 64) */
 } pdehip_server_config;
 typedef struct pdehip_server_stats {
   uint64_t submitted, completed, rejected, cache_hits;
   size_t bytes_in_use, peak_bytes;
+  uint64_t batched;         /* requests solved in a batch lane */
+  uint64_t batches;         /* lane batches run */
+  uint64_t timed_out;       /* requests ended by their deadline */
 } pdehip_server_stats;
 int pdehip_server_create(pdehip_ctx* ctx, const pdehip_server_config* cfg, pdehip_server** out);
 /* The problem's host arrays and u_out must stay valid until the request completes. */

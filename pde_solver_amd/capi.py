@@ -110,12 +110,14 @@ class BatchConfig(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_inflight", C.c_int), ("streams", C.c_int),
-                ("cache_operators", C.c_int), ("timeout_ms", C.c_double)]
+                ("cache_operators", C.c_int), ("timeout_ms", C.c_double),
+                ("batch_lanes", C.c_int), ("batch_max", C.c_int)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
-                ("cache_hits", C.c_uint64), ("bytes_in_use", C.c_size_t), ("peak_bytes", C.c_size_t)]
+                ("cache_hits", C.c_uint64), ("bytes_in_use", C.c_size_t), ("peak_bytes", C.c_size_t),
+                ("batched", C.c_uint64), ("batches", C.c_uint64), ("timed_out", C.c_uint64)]
 
 
 FRAME_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_double, DP, DP)

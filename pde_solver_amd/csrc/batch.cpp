@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 
@@ -76,6 +77,7 @@ struct BatchPlan {
   hipEvent_t evpoll[2] = {nullptr, nullptr};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph[2] = {nullptr, nullptr};   // captured PCG iterations (FP64 / inner)
+  std::function<void()> on_chunk;          // host hook after each polling chunk is enqueued
   size_t bytes = 0;                        // device memory held by the batch and its plans
   ~BatchPlan() {
     for (hipGraphExec_t g : graph)
@@ -294,6 +296,12 @@ bool batch_eligible(int count, const pdehip_problem* probs, const pdehip_options
   return true;
 }
 
+bool batch_plan_ok(const Plan& P, const pdehip_options& opt) {
+  if (opt.method == PDEHIP_BICGSTAB) return true;
+  const int nl = static_cast<int>(P.L.size());
+  return nl >= 2 && P.L[nl - 1].m > 0;
+}
+
 namespace {
 
 // Finish buffers shared by both engines: per-problem boundary data, residual-norm partials,
@@ -339,6 +347,16 @@ void level0_lists(BatchPlan& B, int ychunk) {
   upload_list(ctx, s, B.rows0, rw, &rf, &rcn);
 }
 
+// Problem i's plan: the caller's resident one when given (moved into the batch), else new.
+std::unique_ptr<Plan> take_or_create(Ctx* ctx, const pdehip_problem* probs, const pdehip_options& opt,
+                                     std::vector<std::unique_ptr<Plan>>* given, int i) {
+  if (given && i < static_cast<int>(given->size()) && (*given)[i]) {
+    if ((*given)[i]->ctx != ctx) throw Error(PDEHIP_ERR_INVALID, "a batch's plans must share its context");
+    return std::move((*given)[i]);
+  }
+  return plan_create(ctx, probs[i], opt);
+}
+
 // Per-problem BiCGStab vectors: the plan's FP64 Krylov vectors, or (fp32) its FP32 ones; the
 // solution, right-hand side and initial guess are FP64 either way.  A mixed-precision plan
 // allocates its FP64 Krylov vectors only when a completion pass needs them
@@ -377,7 +395,8 @@ BKVec krylov_view(Plan& P, bool jac, bool fp32) {
 // side, Krylov vectors), the BKVec table, row-block items for the vector kernels, level-0
 // strips for the applies, and the finish buffers of the MG-PCG engine.
 BatchPtr create_krylov(BatchPtr B, Ctx* ctx, int count, const pdehip_problem* probs,
-                       const pdehip_options& opt, size_t used0) {
+                       const pdehip_options& opt, size_t used0,
+                       std::vector<std::unique_ptr<Plan>>* given) {
   const hipStream_t s = ctx->stream;
   B->krylov = true;
   B->L = 1;
@@ -385,7 +404,7 @@ BatchPtr create_krylov(BatchPtr B, Ctx* ctx, int count, const pdehip_problem* pr
   std::vector<BVec> vecs(count);
   std::vector<BKVec> kv(count);
   for (int i = 0; i < count; ++i) {
-    B->plans.push_back(plan_create(ctx, probs[i], opt));
+    B->plans.push_back(take_or_create(ctx, probs, opt, given, i));
     Plan& P = *B->plans.back();
     B->levels.push_back(1);
     BVec& v = vecs[i];
@@ -435,7 +454,7 @@ BatchPtr create_krylov(BatchPtr B, Ctx* ctx, int count, const pdehip_problem* pr
 }  // namespace
 
 BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pdehip_options& opt,
-                      const pdehip_batch_config* cfg) {
+                      const pdehip_batch_config* cfg, std::vector<std::unique_ptr<Plan>>* plans) {
   std::string why;
   if (!batch_eligible(count, probs, opt, &why)) throw Error(PDEHIP_ERR_UNSUPPORTED, why);
   BatchPtr B(new BatchPlan());
@@ -447,9 +466,10 @@ BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pd
   const hipStream_t s = ctx->stream;
   const size_t used0 = ctx->used.load();
   B->plans.reserve(count);
-  if (opt.method == PDEHIP_BICGSTAB) return create_krylov(std::move(B), ctx, count, probs, opt, used0);
+  if (opt.method == PDEHIP_BICGSTAB)
+    return create_krylov(std::move(B), ctx, count, probs, opt, used0, plans);
   for (int i = 0; i < count; ++i) {
-    B->plans.push_back(plan_create(ctx, probs[i], opt));
+    B->plans.push_back(take_or_create(ctx, probs, opt, plans, i));
     Plan& P = *B->plans.back();
     const int nl = static_cast<int>(P.L.size());
     if (nl < 2 || P.L[nl - 1].m <= 0)
@@ -651,6 +671,7 @@ void pcg(BatchPlan& B, Prec kp, int max_iter, double delta) {
       iterations();
     }
     const int slot = chunk & 1;
+    if (B.on_chunk) B.on_chunk();
     poll_enqueue(B, inner, slot);
     if (outstanding >= 0 && poll_read(B, outstanding) == 0) { outstanding = -1; break; }
     outstanding = slot;
@@ -743,6 +764,7 @@ void bicgstab(BatchPlan& B, int max_iter, Prec kp, const BKVec* K, int graph_slo
       iterations();
     }
     const int slot = chunk & 1;
+    if (B.on_chunk) B.on_chunk();
     poll_enqueue(B, 0, slot);
     if (outstanding >= 0 && poll_read(B, outstanding) == 0) { outstanding = -1; break; }
     outstanding = slot;
@@ -907,6 +929,20 @@ double batch_time_kernel(BatchPlan& B, int kernel, int reps, double* algorithmic
 void batch_download(BatchPlan& B, int i, double* u_out) { plan_download(*B.plans.at(i), u_out); }
 
 size_t batch_bytes(const BatchPlan& B) { return B.bytes; }
+
+std::vector<std::unique_ptr<Plan>> batch_release_plans(BatchPlan& B) {
+  (void)hipStreamSynchronize(B.ctx->stream);
+  std::vector<std::unique_ptr<Plan>> out = std::move(B.plans);
+  B.plans.clear();
+  return out;
+}
+
+void batch_set_chunk_hook(BatchPlan& B, std::function<void()> hook) { B.on_chunk = std::move(hook); }
+
+void batch_cancel(BatchPlan& B, int i) {
+  if (i < 0 || i >= B.n) throw Error(PDEHIP_ERR_INVALID, "batch_cancel: problem index out of range");
+  bk::cancel(B.ctx->stream, B.V(), i);
+}
 
 void BatchDeleter::operator()(BatchPlan* b) const { delete b; }
 

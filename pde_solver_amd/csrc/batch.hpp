@@ -11,6 +11,7 @@
 // segmented per problem (items of a problem are contiguous) and deterministic.
 #pragma once
 
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -75,8 +76,20 @@ struct LevelShape { int ychunk = 16; int row_nodes = 256; int crow_nodes = 128; 
 struct BatchPlan;
 struct BatchDeleter { void operator()(BatchPlan* b) const; };
 using BatchPtr = std::unique_ptr<BatchPlan, BatchDeleter>;
+// plans (optional): resident plans of the problems, created on ctx with opt (moved into the
+// batch where non-null; the rest are created).  batch_release_plans gives them back.
 BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pdehip_options& opt,
-                      const pdehip_batch_config* cfg);
+                      const pdehip_batch_config* cfg,
+                      std::vector<std::unique_ptr<Plan>>* plans = nullptr);
+// The batch's plans, in problem order, after its work has finished (the batch keeps none).
+std::vector<std::unique_ptr<Plan>> batch_release_plans(BatchPlan& B);
+// Called on the host after each polling chunk of a solve has been enqueued (for example to
+// cancel problems whose deadline passed); the stream order puts anything it enqueues between
+// that chunk and the next.
+void batch_set_chunk_hook(BatchPlan& B, std::function<void()> hook);
+// Stop problem i from the next enqueued launch on: its scalars are marked broken down, so every
+// later kernel skips it and its report says PDEHIP_ERR_BREAKDOWN.
+void batch_cancel(BatchPlan& B, int i);
 // Returns PDEHIP_OK, or the status of the first problem that did not converge.  MG-PCG batches
 // run the lock-step MG-PCG; BiCGStab batches the lock-step BiCGStab.
 int batch_solve(BatchPlan& B, pdehip_report* reports);
@@ -91,6 +104,10 @@ void batch_destroy(BatchPtr& B);
 // MG-PCG without convection, or BiCGStab (no preconditioner or Jacobi).
 bool batch_eligible(int count, const pdehip_problem* probs, const pdehip_options& opt,
                     std::string* why);
+
+// True when a plan built for this batch's options can join it (MG-PCG plans need a dense
+// coarsest level).
+bool batch_plan_ok(const Plan& P, const pdehip_options& opt);
 
 // ---- batched kernels (k_batch.hip)
 namespace bk {
@@ -136,6 +153,8 @@ void finalize(hipStream_t s, const BVec* vecs, const int* first, const int* cnt,
               int tol_mode);
 // Count of problems whose selected scalars are not done (written to *active).
 void active_count(hipStream_t s, const BVec* vecs, int nprob, int inner, int* active);
+// Problem p's outer and inner scalars marked broken down with no restart pending.
+void cancel(hipStream_t s, const BVec* vecs, int p);
 // Finish: boundary closures of every problem's FP64 solution (phase 0/1: Neumann/Robin x/y
 // faces, 2/3: Dirichlet x/y faces, launched in that order), the reference residual norms over
 // element items (partials per item), and the per-problem gather into BReport.

@@ -744,6 +744,17 @@ __global__ void kb_active(const BVec* __restrict__ vecs, int nprob, int inner, i
   if (threadIdx.x == 0) *active = static_cast<int>(s);
 }
 
+// A problem stopped from the host (a deadline): both scalar blocks broken down, no restart.
+__global__ void kb_cancel(const BVec* __restrict__ vecs, int p) {
+  if (threadIdx.x != 0) return;
+  KScalars* scs[2] = {vecs[p].so, vecs[p].si};
+  for (KScalars* sc : scs) {
+    if (!sc) continue;
+    sc->breakdown = 1;
+    sc->cycle_done = 0;
+  }
+}
+
 // ------------------------------------------------------------------------ batched BiCGStab
 // Vector kernels over row-block items (p, rows [a, b)): one wavefront walks the rows, lanes the
 // columns, so the Jacobi diagonal of every node follows from (i, j) without a division.  The
@@ -1250,6 +1261,10 @@ void finish_reduce(hipStream_t s, const BVec* vecs, const int* rfirst, const int
 
 void active_count(hipStream_t s, const BVec* vecs, int nprob, int inner, int* active) {
   hipLaunchKernelGGL(kb_active, dim3(1), dim3(1024), 0, s, vecs, nprob, inner, active);
+}
+
+void cancel(hipStream_t s, const BVec* vecs, int p) {
+  hipLaunchKernelGGL(kb_cancel, dim3(1), dim3(64), 0, s, vecs, p);
 }
 
 

@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
     lib = C.CDLL(LIB)
     missing = [name for name in declared() if not hasattr(lib, name)]
     assert not missing, missing
-    assert lib.pdehip_abi_version() == 1
+    assert lib.pdehip_abi_version() == 2
 
 
 def test_struct_layouts_match_header():

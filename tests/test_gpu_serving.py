@@ -110,6 +110,11 @@ def test_server_mix_matches_oracle_and_alone(ctx):
     stats = srv.stats()
     srv.close()
     assert stats["completed"] == len(reqs) and stats["rejected"] == 0
+    # the 2D CONST MG-PCG and FP64 BiCGStab requests ran in batch lanes, the rest on plan workers
+    n_lane = sum(1 for m in MIX if m[1] == 2 and m[0] != "varcoef" and
+                 (m[3] == capi.MGPCG or (m[3] == capi.BICGSTAB and m[4] == capi.FP64)))
+    print(f"  batched {stats['batched']} of {len(reqs)} in {stats['batches']} lane batches", flush=True)
+    assert stats["batched"] == n_lane > 0
     for i, ((s, o), (st, u, rep)) in enumerate(zip(reqs, mixed)):
         o_ref = capi.options(o.method, tol=1e-13, tol_mode=capi.TOL_REL, precond=o.precond,
                              bc_rows=capi.ROWS_FOLDED, gmres_restart=30, max_iter=o.max_iter * 2)
@@ -138,13 +143,80 @@ def test_server_deadline_reports_timeout(ctx):
     s, o = request("varcoef", 3, 33, capi.MGPCG, capi.FP64, seed=2)
     st, _, rep = srv.wait(srv.submit(s, o))
     assert st == capi.ERR_TIMEOUT
+    assert srv.stats()["timed_out"] == 1
     srv.close()
+
+
+def slow_poisson(n, method):
+    """Poisson without a preconditioner to rel-resid 1e-10: hundreds (BiCGStab) to thousands
+    (GMRES(30)) of Krylov iterations, so a deadline of a fifth of the solve lands inside it."""
+    s = capi.Spec(n, n, a=-1.0, b=-1.0)
+    s.f_field = np.random.default_rng(n).standard_normal(s.shape)
+    s.bcs = [BC.dirichlet(0.0)] * 4 + [BC(), BC()]
+    return s, capi.options(method, tol=1e-10, tol_mode=capi.TOL_REL, precond=capi.PC_NONE,
+                           gmres_restart=30, max_iter=20000)
+
+
+@pytest.mark.parametrize("method", [capi.BICGSTAB, capi.GMRES])
+def test_server_deadline_stops_running_solve(ctx, method):
+    """A deadline that passes while the solve runs ends it on the device: a batch lane marks the
+    member broken down between polling chunks (BiCGStab), a plan worker's watchdog raises the
+    solve's cancel flag (GMRES).  The request ends with PDEHIP_ERR_TIMEOUT after far fewer
+    iterations than the full solve, and the server keeps serving."""
+    import time
+    s, o = slow_poisson(129, method)
+    srv = server(ctx)
+    t = time.perf_counter()
+    st_full, _, rep_full = srv.wait(srv.submit(s, o))
+    full_ms = (time.perf_counter() - t) * 1e3
+    srv.close()
+    srv = server(ctx, timeout_ms=max(5.0, 0.2 * full_ms))
+    t = time.perf_counter()
+    st, _, rep = srv.wait(srv.submit(s, o))
+    ms = (time.perf_counter() - t) * 1e3
+    print(f"  method {method}: full {rep_full.iterations} its {full_ms:.0f} ms; with deadline status "
+          f"{st} after {rep.iterations} its {ms:.0f} ms", flush=True)
+    assert st_full == 0
+    assert st == capi.ERR_TIMEOUT
+    assert ms < 0.7 * full_ms
+    assert rep.iterations < rep_full.iterations
+    stats = srv.stats()
+    assert stats["timed_out"] == 1
+    if method == capi.BICGSTAB:
+        assert stats["batched"] == 1
+    # the server still serves after a cancelled solve
+    s2, o2 = request("poisson", 2, 33, capi.MGPCG, capi.FP64, seed=3)
+    srv.close()
+    srv = server(ctx)
+    st2, _, _ = srv.wait(srv.submit(s2, o2))
+    assert st2 == 0
+    srv.close()
+
+
+def test_server_lane_batches_compatible_arrivals(ctx):
+    """Requests queued together with the same options share one lane batch; a request with other
+    options forms its own; each result equals the request served alone."""
+    specs = [request("poisson", 2, n, capi.MGPCG, capi.MIXED, seed=40 + i)
+             for i, n in enumerate((33, 65, 129, 65, 33, 129))]
+    other = request("poisson", 2, 65, capi.MGPCG, capi.FP64, seed=50)
+    srv = server(ctx, batch_lanes=1)
+    alone = [srv.wait(srv.submit(s, o))[1].copy() for s, o in specs + [other]]
+    before = srv.stats()["batches"]
+    tickets = [srv.submit(s, o) for s, o in specs + [other]]
+    outs = [srv.wait(t) for t in tickets]
+    stats = srv.stats()
+    srv.close()
+    print(f"  lane batches for the mix: {stats['batches'] - before}", flush=True)
+    assert stats["batches"] - before <= 4   # far fewer batches than requests
+    for (st, u, _), a in zip(outs, alone):
+        assert st == 0 and np.array_equal(u, a)
 
 
 def test_server_cache_hit_with_only_constant_source_changed(ctx):
     """Two requests with the same operator and constant sources f = 1 and f = -3: the second
-    hits the resident operator and must use its own f."""
-    srv = server(ctx, streams=1)
+    hits the resident operator and must use its own f (plan workers only: bitwise against the
+    one-shot solve)."""
+    srv = server(ctx, streams=1, batch_lanes=-1)
     outs, alone = [], []
     for f in (1.0, -3.0):
         s = capi.Spec(33, 33, a=-1.0, b=-1.0)
@@ -159,6 +231,25 @@ def test_server_cache_hit_with_only_constant_source_changed(ctx):
     for u, a in zip(outs, alone):
         assert np.array_equal(u, a)
     assert rel(outs[1], -3.0 * outs[0]) < 1e-9
+
+
+def test_server_lane_cache_hit_with_only_constant_source_changed(ctx):
+    """The same through one batch lane: the second request reuses the lane's resident operator
+    with its own constant source (within the solve tolerance of the one-shot solves)."""
+    srv = server(ctx, batch_lanes=1)
+    outs = []
+    for f in (1.0, -3.0):
+        s = capi.Spec(65, 65, a=-1.0, b=-1.0)
+        s.f = f
+        s.bcs = [BC.dirichlet(0.0)] * 4 + [BC(), BC()]
+        o = capi.options(capi.MGPCG, tol=1e-10, tol_mode=capi.TOL_REL, precond=capi.PC_MG,
+                         bc_rows=capi.ROWS_FOLDED)
+        outs.append((srv.wait(srv.submit(s, o))[1].copy(), ctx.solve(s, o)[1]))
+    assert srv.stats()["cache_hits"] == 1 and srv.stats()["batched"] == 2
+    srv.close()
+    for u, a in outs:
+        assert rel(u, a) < 1e-8
+    assert rel(outs[1][0], -3.0 * outs[0][0]) < 1e-8
 
 
 def test_server_nonlinear_terms_are_part_of_the_operator(ctx):

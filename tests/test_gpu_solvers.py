@@ -163,15 +163,18 @@ def test_server_results_identical_alone_and_in_mix(ctx):
         s.f_field = rng.standard_normal(s.shape)
         specs.append(s)
     opt = mgpcg(tol=1e-10)
-    alone = [ctx.solve(s, opt)[1] for s in specs]
+    one_shot = [ctx.solve(s, opt)[1] for s in specs]
     cfg = capi.ServerConfig()
     cfg.max_inflight, cfg.streams, cfg.cache_operators = 8, 4, 1
     srv = ctx.server(cfg)
+    alone = [srv.wait(srv.submit(s, opt))[1].copy() for s in specs]
     tickets = [srv.submit(s, opt) for s in specs]
     outs = [srv.wait(t) for t in tickets]
-    for (st, u, rep), a in zip(outs, alone):
+    assert srv.stats()["batched"] == 2 * len(specs)   # 2D MG-PCG requests run in batch lanes
+    for (st, u, rep), a, o in zip(outs, alone, one_shot):
         assert st == 0
         assert np.array_equal(u, a)
+        assert rel(u, o) < 1e-8
     srv.close()
 
 
