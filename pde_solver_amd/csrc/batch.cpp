@@ -709,7 +709,7 @@ void ensure_fp64_views(BatchPlan& B) {
   B.bytes += ctx->used.load() - used0;
 }
 
-void bicgstab(BatchPlan& B, int max_iter, Prec kp, const BKVec* K, int graph_slot) {
+void bicgstab(BatchPlan& B, int max_iter, Prec kp, const BKVec* K, int graph_slot, int stall = 0) {
   const hipStream_t s = B.ctx->stream;
   double* p1 = B.part.as<double>();
   double* p2 = B.maxp.as<double>();
@@ -736,7 +736,8 @@ void bicgstab(BatchPlan& B, int max_iter, Prec kp, const BKVec* K, int graph_slo
       fin(rw, bk::KK_FOMEGA, p2);
       bk::kvec(s, kp, bk::KK_X, K, rb.d(), rb.n, nullptr);
       bk::kapply(s, kp, 2, K, rw.d(), rw.n, p1, nullptr);
-      fin(rw, bk::KK_FR, nullptr);
+      bk::kfinalize(s, K, rw.first.as<int>(), rw.cnt.as<int>(), B.n, p1, nullptr, bk::KK_FR,
+                    static_cast<double>(stall), 0);
     }
   };
   int outstanding = -1;
@@ -790,21 +791,22 @@ int batch_solve(BatchPlan& B, pdehip_report* reports) {
     bk::kapply(s, kp, 2, B.K(), rw.d(), rw.n, part, nullptr);
     bk::kfinalize(s, B.K(), rw.first.as<int>(), rw.cnt.as<int>(), n, part, nullptr, bk::KK_R0,
                   B.opt.tol, B.opt.tol_mode);
-    bicgstab(B, max_iter, kp, B.K(), 0);
+    bicgstab(B, max_iter, kp, B.K(), 0, B.mixed ? bk::kMixedStallWindow : 0);
     if (B.mixed) {
-      // FP64 completion: problems the FP32 recurrence did not bring to the target start again
-      // from x0 in FP64 (the single-problem path's fallback, refinements = -1 in the report)
+      // FP64 completion: problems the FP32 recurrence did not bring to the target (diverged,
+      // stalled for the window, or out of iterations) continue in FP64 with the first pass's
+      // target, from their current solution when the pass cut the true residual by 1e3, else
+      // from x0 (refinements = -1 in the report)
       bk::kfailed(s, B.K(), n, B.active.as<int>());
       HIPCHK(hipMemcpyAsync(B.hactive, B.active.get(), sizeof(int), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
       if (B.hactive[0] > 0) {
         ensure_fp64_views(B);
-        const List& rb = B.rb0;
         bk::kreset_failed(s, B.K64(), n, 1e-30);
-        bk::kvec(s, P64, bk::KK_X0, B.K64(), rb.d(), rb.n, nullptr);
+        bk::kvec(s, P64, bk::KK_X0, B.K64(), B.rb0.d(), B.rb0.n, nullptr);
         bk::kapply(s, P64, 2, B.K64(), rw.d(), rw.n, part, nullptr);
         bk::kfinalize(s, B.K64(), rw.first.as<int>(), rw.cnt.as<int>(), n, part, nullptr,
-                      bk::KK_R0, B.opt.tol, B.opt.tol_mode);
+                      bk::KK_R0K, 0.0, 0);
         bicgstab(B, max_iter, P64, B.K64(), 1);
       }
     }

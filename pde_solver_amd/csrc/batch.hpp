@@ -178,11 +178,17 @@ void kvec(hipStream_t s, Prec p, int op, const BKVec* vecs, const BItem* items, 
 // |s| met the target); 2: r = b - A64 x in FP64, partials |r|^2.
 void kapply(hipStream_t s, Prec p, int mode, const BKVec* vecs, const BItem* items, int n,
             double* p1, double* p2);
-enum KkFin { KK_R0 = 0, KK_FRHO = 1, KK_FALPHA = 2, KK_FS = 3, KK_FOMEGA = 4, KK_FR = 5 };
+// KK_FR with tol > 0: stall window in iterations (see kk_finalize); KK_R0K: the completion
+// pass's start, keeping the first pass's r0 and target.
+enum KkFin { KK_R0 = 0, KK_FRHO = 1, KK_FALPHA = 2, KK_FS = 3, KK_FOMEGA = 4, KK_FR = 5,
+             KK_R0K = 6 };
+// FP32 pass of a mixed BiCGStab batch: iterations without a halving of the true residual after
+// which a problem leaves the pass for the FP64 completion.
+constexpr int kMixedStallWindow = 100;
 // Scalars of a fresh start for problems whose restart is pending (after KK_RESTART).
 void krestart(hipStream_t s, const BKVec* vecs, int nprob);
 // Mixed batches: every problem that did not converge gets fresh scalars (eps) and steps = -1,
-// for the FP64 completion pass; converged problems keep theirs.
+// keeping r0 and the target, for the FP64 completion pass; converged problems keep theirs.
 void kreset_failed(hipStream_t s, const BKVec* vecs, int nprob, double eps);
 // Count of problems that did not converge, written to *out.
 void kfailed(hipStream_t s, const BKVec* vecs, int nprob, int* out);

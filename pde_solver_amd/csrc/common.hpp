@@ -326,6 +326,9 @@ struct KScalars {
   int cycle_done;    // GMRES: the current restart cycle is finished
   int steps;         // GMRES: Arnoldi steps of the current cycle
   int pad0, pad1;
+  double best;       // mixed BiCGStab: smallest true residual so far ...
+  int best_iter;     // ... reached at this iteration
+  int stalled;       // 1 once it has not halved for the stall window
 };
 
 }  // namespace pdehip

@@ -778,7 +778,9 @@ kk_vec(const BKVec* __restrict__ vecs, const BItem* __restrict__ items, int n, d
   const KScalars* sc = v.sc;
   // a restart pending after a breakdown copies the residual into rh although the problem is
   // marked broken down until the scalars are reset
-  const bool run = OP == KK_RESTART ? ldS(&sc->cycle_done, 0) != 0 && !ldS(&sc->done, 0) : !done(sc);
+  // KK_X0 resets only the problems a mixed batch's completion restarts from x0 (steps == -1)
+  const bool run = OP == KK_RESTART ? ldS(&sc->cycle_done, 0) != 0 && !ldS(&sc->done, 0)
+                 : (OP == KK_X0 ? !done(sc) && ldS(&sc->steps, 0) == -1 : !done(sc));
   if (run) {
     const int nx = v.nx, ny = v.ny;
     S* r = static_cast<S*>(v.r);
@@ -940,11 +942,16 @@ __global__ void kk_reset_failed(const BKVec* __restrict__ vecs, int nprob, doubl
   if (q >= nprob) return;
   KScalars* sc = vecs[q].sc;
   if (sc->done && !sc->breakdown) return;
+  // continue from the current solution when the first pass cut the true residual by 1e3
+  // (steps = -2), else start again from x0 (steps = -1)
+  const bool keep = isfinite(sc->metric) && sc->metric < 1e-3 * sc->r0;
   KScalars k{};
   k.eps = eps;
   k.scale = 1.0;
   k.inv_scale = 1.0;
-  k.steps = -1;
+  k.steps = keep ? -2 : -1;
+  k.r0 = sc->r0;           // the first pass's initial residual and target carry over
+  k.target = sc->target;
   *sc = k;
 }
 
@@ -1010,8 +1017,29 @@ kk_finalize(const BKVec* __restrict__ vecs, const int* __restrict__ first,
       sc->pad0 += 1;
       if (nonfinite(s1)) { sc->breakdown = 1; break; }
       if (sc->metric < sc->target) { sc->done = 1; break; }
+      // stall window (tol > 0, the FP32 pass of a mixed batch): a true residual that has not
+      // halved for `tol` iterations ends the pass for this problem (no restart); the FP64
+      // completion continues from its current solution
+      if (tol > 0.0) {
+        if (sc->best_iter == 0 || sc->metric < 0.5 * sc->best) {
+          sc->best = sc->metric;
+          sc->best_iter = sc->iter;
+        } else if (sc->iter - sc->best_iter >= static_cast<int>(tol)) {
+          sc->stalled = 1;
+          sc->breakdown = 1;
+          sc->cycle_done = 0;
+          break;
+        }
+      }
       if (fabs(sc->omega) < sc->eps) { kk_breakdown(sc); break; }
       sc->rho_old = sc->rho;
+      break;
+    case KK_R0K:
+      // restart of the FP64 completion from the current solution: the initial residual and the
+      // target of the first pass are kept
+      if (nonfinite(s1)) { sc->breakdown = 1; break; }
+      sc->metric = sqrt(s1);
+      if (sc->metric < sc->target) sc->done = 1;
       break;
     default:
       break;

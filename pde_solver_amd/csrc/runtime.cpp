@@ -1086,7 +1086,9 @@ static void solve_relax(Plan& P, pdehip_report* rep) {
 }
 
 // BiCGStab (reference krylov.cpp:82-185) with optional Jacobi right preconditioner.
-static void solve_bicgstab(Plan& P, pdehip_report* rep) {
+// from_x: start from the current solution x (not x0) with the absolute target abs_target (the
+// FP64 completion of a mixed solve).
+static void solve_bicgstab(Plan& P, pdehip_report* rep, bool from_x = false, double abs_target = 0.0) {
   const hipStream_t s = P.stream();
   const long long n = P.g.n;
   const DevOp& op = P.op();
@@ -1101,9 +1103,12 @@ static void solve_bicgstab(Plan& P, pdehip_report* rep) {
   const bool pc = P.opt.precond == PDEHIP_PC_JACOBI;
   double* ph = pc ? P.w4.as<double>() : pv;
   double* shv = pc ? sh : sv;
-  HIPCHK(hipMemcpyAsync(x, P.x0.get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  if (!from_x) HIPCHK(hipMemcpyAsync(x, P.x0.get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
   residual(L0, op, P64, P.bvec.get(), x, r, part);
-  finalize(L0, sc, part, partial_blocks(op), FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
+  if (from_x)
+    finalize(L0, sc, part, partial_blocks(op), FIN_R0, abs_target, PDEHIP_TOL_ABS, nullptr);
+  else
+    finalize(L0, sc, part, partial_blocks(op), FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
   HIPCHK(hipMemcpyAsync(rhat, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   const int kb = krylov_blocks(n);
   const int max_iter = std::max(1, P.opt.max_iter);
@@ -1269,6 +1274,11 @@ static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
   const int max_iter = std::max(1, P.opt.max_iter);
   const int every = check_every(P);
   int replacements = 0, iter_base = 0;
+  // stall window: a true residual that has not halved for this many iterations ends the FP32
+  // pass (as a growth beyond 1e3 |r0| does); the FP64 completion continues from the current x
+  constexpr int kStallWindow = 100;
+  double best = k0.r0;
+  int best_at = 0;
   if (!k0.done) {
     // inner vectors hold the residual divided by the fixed scale |r0|
     to_f32_scaled(s, r64, r, n, &so->r0);
@@ -1305,6 +1315,12 @@ static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
       ++replacements;
       const KScalars ko = pull(P, 0);
       if (ko.done || ko.breakdown || !(ko.metric < 1e3 * ko.r0)) break;   // converged / diverged
+      if (ko.metric < 0.5 * best) {
+        best = ko.metric;
+        best_at = launched;
+      } else if (launched - best_at >= kStallWindow) {
+        break;   // stalled
+      }
       to_f32_scaled(s, r64, r, n, &so->r0);
       if (ki.breakdown) {
         // rho or omega vanished: restart the recurrence from the true residual
@@ -1324,15 +1340,25 @@ static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
   const KScalars ki = pull(P, 1);
   const int mixed_iters = iter_base + ki.iter;
   if (!ko.done && !cancelled(P)) {
-    // the FP32 iteration diverged or stalled: the FP64 BiCGStab (reference semantics) solves
-    // the problem from the initial guess; its vectors are allocated on the first such case
+    // the FP32 iteration diverged, stalled or ran out of iterations: the FP64 BiCGStab
+    // (reference semantics) continues from the current FP64 solution when the FP32 pass cut the
+    // true residual by 1e3 (else from the initial guess), to the first pass's target; its
+    // vectors are allocated on the first such case
     const size_t n8 = n * sizeof(double);
     for (DBuf* b : {&P.p, &P.q, &P.z, &P.w1, &P.w2, &P.w3, &P.w4}) {
       if (b == &P.w4 && !pc) continue;
       if (!*b) { b->alloc(P.ctx, n8); fill(s, P64, b->get(), n, 0.0); }
     }
     pdehip_report r64rep{};
-    solve_bicgstab(P, &r64rep);
+    const bool keep_x = std::isfinite(ko.metric) && ko.metric < 1e-3 * ko.r0;
+    solve_bicgstab(P, &r64rep, keep_x, ko.target);
+    if (keep_x) {
+      // the report's relative residual is against the first pass's initial residual
+      KScalars kf = pull(P, 0);
+      kf.r0 = ko.r0;
+      P.hsc[0] = kf;
+      HIPCHK(hipMemcpyAsync(P.sc(0), &P.hsc[0], sizeof(KScalars), hipMemcpyHostToDevice, s));
+    }
     *rep = r64rep;
     rep->iterations = mixed_iters + r64rep.iterations;
     rep->inner_iterations = mixed_iters;
