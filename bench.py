@@ -217,17 +217,32 @@ def bench_nonsym(args) -> int:
     rdres = {}
     for precision, pname in ((capi.MIXED, "fp32"), (capi.FP64, "fp64")):
         tc = capi.time_config(capi.TI_RK4, dt=0.2 * h * h, frames=steps + 1, precision=precision)
-        ctx.time_series(rd, capi.time_config(capi.TI_RK4, dt=0.2 * h * h, frames=3, precision=precision),
-                        keep_frames=False)
-        _, u, rep = ctx.time_series(rd, tc, keep_frames=False)
         es = 4 if precision == capi.MIXED else 8
-        per_step = 17 * es * rd.n        # fused-stage RK4 compulsory bytes
-        rdres[pname] = {"steps": steps, "ms": round(rep.solve_ms, 2),
-                        "steps_per_s": round(steps / (rep.solve_ms / 1e3), 1),
-                        "GB_per_s": round(per_step * steps / (rep.solve_ms / 1e3) / 1e9, 1),
-                        "u_mean": float(u.mean())}
-        rdres[pname + "_u"] = u
-        log(f"RD {pname}: {rdres[pname]['ms']} ms for {steps} steps ({rdres[pname]['GB_per_s']} GB/s)")
+        for fused in (1, 0):
+            hip.tune_set("time.fused", fused)
+            ctx.time_series(rd, capi.time_config(capi.TI_RK4, dt=0.2 * h * h, frames=3, precision=precision),
+                            keep_frames=False)
+            _, u, rep = ctx.time_series(rd, tc, keep_frames=False)
+            sec = rep.solve_ms / 1e3
+            if fused:
+                # one launch per step: u_n read once, u_{n+1} written once
+                rdres[pname] = {"steps": steps, "ms": round(rep.solve_ms, 2),
+                                "us_per_step": round(rep.solve_ms * 1e3 / steps, 2),
+                                "steps_per_s": round(steps / sec, 1),
+                                "GB_per_s": round(2 * es * rd.n * steps / sec / 1e9, 1),
+                                "hbm_frac": round(2 * es * rd.n * steps / sec / 1e9 / HBM_PEAK_GBS, 3),
+                                "stage_equivalent_GB_per_s": round(17 * es * rd.n * steps / sec / 1e9, 1),
+                                "u_mean": float(u.mean())}
+                rdres[pname + "_u"] = u
+            else:
+                # one launch per stage (17 state accesses per node and step)
+                rdres[pname]["per_stage_launches_ms"] = round(rep.solve_ms, 2)
+                rdres[pname]["per_stage_GB_per_s"] = round(17 * es * rd.n * steps / sec / 1e9, 1)
+                rdres[pname]["fused_equals_per_stage"] = bool(np.array_equal(u, rdres[pname + "_u"]))
+        hip.tune_set("time.fused", 1)
+        log(f"RD {pname}: fused {rdres[pname]['ms']} ms for {steps} steps ({rdres[pname]['GB_per_s']} GB/s), "
+            f"per-stage launches {rdres[pname]['per_stage_launches_ms']} ms, "
+            f"bitwise equal: {rdres[pname]['fused_equals_per_stage']}")
     rdres["fp32"]["max_rel_diff_vs_fp64"] = float(np.abs(rdres.pop("fp32_u") - rdres["fp64_u"]).max()
                                                    / np.abs(rdres["fp64_u"]).max())
     rdres.pop("fp64_u")

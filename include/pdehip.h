@@ -249,6 +249,8 @@ void pdehip_plan_destroy(pdehip_plan* plan);
  * divergence), "smooth.ring" (0/1, default 1: 3D FP32/FP16 sweeps by the ring-buffered,
  * 6-step unrolled kernel), "stencil.ring" (0/1, default 1: the same for the 3D FP32/FP16
  * apply and residual).  Results do not depend on any of these.
+ * "time.fused" (0/1, default 1): RK4 time series take one launch per step (the four stages and
+ * the closures, temporally blocked in LDS) instead of one per stage; the same bits either way.
  * "mixed.fp16" (0/1, default 1): mixed-precision plans created afterwards store their coarse
  * multigrid levels in FP16 (1) or FP32 (0); this one changes the preconditioner's rounding.
  * Not thread-safe against running solves. */

@@ -1,5 +1,7 @@
 // k_time.hip — explicit time-step stage kernels, the reference residual-norm kernel and the
 // single-workgroup lexicographic Gauss-Seidel used by the reference-semantics V-cycle.
+#include <type_traits>
+
 #include "faces_dev.hpp"
 #include "kernels_ext.hpp"
 
@@ -17,14 +19,71 @@ __device__ __noinline__ double eval_nl_other(int kind, double coeff, double u) {
   }
 }
 
+// Which nonlinear terms a problem has, fixed per launch: none, integer powers 0..3 only
+// (evaluated as one cubic), or any kind.
+enum NlMode { NL_NONE = 0, NL_CUBIC = 1, NL_ANY = 2 };
+
+int nl_mode(const NonlinearTerms& nl) {
+  if (nl.n <= 0) return NL_NONE;
+  for (int t = 0; t < nl.n; ++t)
+    if (nl.t[t].kind != PDEHIP_NL_POWER || nl.t[t].power < 0 || nl.t[t].power > 3) return NL_ANY;
+  return NL_CUBIC;
+}
+
+// An NL_CUBIC problem's terms as the cubic a0 + a1 u + a2 u^2 + a3 u^3 (coefficients of equal
+// powers summed), evaluated by Horner's rule: three FMAs per node instead of a per-term
+// selection.  It rounds differently from the term-by-term sum (by an ulp or so of the terms),
+// the same way in every kernel that takes this path.
+struct Cubic {
+  double a[4];
+};
+
+Cubic cubic_of(const NonlinearTerms& nl) {
+  Cubic c{};
+  for (int t = 0; t < nl.n; ++t) c.a[nl.t[t].power] += nl.t[t].coeff;
+  return c;
+}
+
+__device__ __forceinline__ double eval_cubic(const Cubic& c, double u) {
+  return fma(fma(fma(c.a[3], u, c.a[2]), u, c.a[1]), u, c.a[0]);
+}
+
+// Any terms, in order, summed from 0.
 __device__ __forceinline__ double eval_nl(const NonlinearTerms& nl, double u) {
   double v = 0.0;
-  for (int t = 0; t < nl.n; ++t) {
+#pragma unroll
+  for (int t = 0; t < PDEHIP_MAX_NONLINEAR; ++t) {
+    if (t >= nl.n) break;
     const pdehip_nonlinear& term = nl.t[t];
     if (term.kind == PDEHIP_NL_POWER) v += term.coeff * ipow(u, term.power);
     else v += eval_nl_other(term.kind, term.coeff, u);
   }
   return v;
+}
+
+// N(u) under the launch's mode.
+template <int NLM>
+__device__ __forceinline__ double eval_terms(const NonlinearTerms& nl, const Cubic& cb, double u) {
+  if constexpr (NLM == NL_NONE) return 0.0;
+  else if constexpr (NLM == NL_CUBIC) return eval_cubic(cb, u);
+  else return eval_nl(nl, u);
+}
+
+// Launch f(KIND, NLM) with both as compile-time constants.
+template <typename F>
+void by_kind_nl(const DevOp& op, const NonlinearTerms& nl, F&& f) {
+  using PW = std::integral_constant<int, PDEHIP_OP_POINTWISE>;
+  using FX = std::integral_constant<int, PDEHIP_OP_FLUX>;
+  using CT = std::integral_constant<int, PDEHIP_OP_CONST>;
+  const int m = nl_mode(nl);
+  auto with_kind = [&](auto K) {
+    if (m == NL_NONE) f(K, std::integral_constant<int, NL_NONE>{});
+    else if (m == NL_CUBIC) f(K, std::integral_constant<int, NL_CUBIC>{});
+    else f(K, std::integral_constant<int, NL_ANY>{});
+  };
+  if (op.kind == PDEHIP_OP_POINTWISE) with_kind(PW{});
+  else if (op.kind == PDEHIP_OP_FLUX) with_kind(FX{});
+  else with_kind(CT{});
 }
 
 __device__ __forceinline__ double safe_div(double num, double den) {  // safe_math.h:48-58
@@ -35,23 +94,38 @@ __device__ __forceinline__ double safe_div(double num, double den) {  // safe_ma
 
 template <typename T> __device__ __forceinline__ double ldv(const T* p, long long i) { return static_cast<double>(p[i]); }
 
-// du/dt = -(A v + f + N(v))/ut on interior nodes (fill_dudt, reference solver.cpp:1182-1195),
-// zero on the boundary; then one of the stage combinations below.
-template <int KIND, typename S>
-__device__ __forceinline__ double dudt_at(const DevOp& op, const S* v, long long id, int i, int j,
-                                         const double* f, double f_const, const NonlinearTerms& nl,
-                                         double ut, double inv_ut) {
-  if (is_boundary2(op, i, j)) return 0.0;
+// du/dt = -(A v + f + N(v))/ut at interior node id from its value and its four neighbours'
+// (fill_dudt, reference solver.cpp:1182-1195).  One definition for the per-stage kernel and the
+// fused step, so both round alike.
+template <int KIND, int NLM>
+__device__ __forceinline__ double dudt_values(const DevOp& op, long long id, double vc, double ve,
+                                             double vw, double vn, double vs, const double* f,
+                                             double f_const, const NonlinearTerms& nl,
+                                             const Cubic& cb, double ut, double inv_ut) {
   const W7<double> w = row_weights<2, KIND, double, double>(op, id);
-  const double vc = ldv(v, id);
-  const double opv = w.C * vc + (w.E * ldv(v, id + 1) + w.W * ldv(v, id - 1) +
-                                 w.N * ldv(v, id + op.nx) + w.S * ldv(v, id - op.nx));
-  const double rhs = opv + (f ? f[id] : f_const) + (nl.n ? eval_nl(nl, vc) : 0.0);
+  const double opv = w.C * vc + (w.E * ve + w.W * vw + w.N * vn + w.S * vs);
+  const double rhs = opv + (f ? f[id] : f_const) + eval_terms<NLM>(nl, cb, vc);
   // safe_div(-rhs, ut) with the reciprocal of the (uniform) u_t coefficient formed once
   if (fabs(ut) < 1e-12) return 0.0;
   const double r = -rhs * inv_ut;
   return isfinite(r) ? r : 0.0;
 }
+
+// The same, zero on the boundary, with the values read from v.
+template <int KIND, int NLM, typename S>
+__device__ __forceinline__ double dudt_at(const DevOp& op, const S* v, long long id, int i, int j,
+                                         const double* f, double f_const, const NonlinearTerms& nl,
+                                         const Cubic& cb, double ut, double inv_ut) {
+  if (is_boundary2(op, i, j)) return 0.0;
+  return dudt_values<KIND, NLM>(op, id, ldv(v, id), ldv(v, id + 1), ldv(v, id - 1), ldv(v, id + op.nx),
+                           ldv(v, id - op.nx), f, f_const, nl, cb, ut, inv_ut);
+}
+
+// RK4 stage combinations (k: the stage's du/dt, u0: u_n, a: the accumulator) with explicit
+// fused multiply-adds, shared by the per-stage kernel and the fused step.
+__device__ __forceinline__ double stage_out(double u0, double c1, double k) { return fma(c1, k, u0); }
+__device__ __forceinline__ double stage_acc(double a, double k) { return fma(2.0, k, a); }
+__device__ __forceinline__ double stage_last(double u0, double c1, double a, double k) { return fma(c1, a + k, u0); }
 
 // Stage kernel.  Per node:  k = F(v)
 //   mode 0 (FE / RK4 s1 / RK2 s1 / SSP s1): acc = k (if acc), out = un + c1*k
@@ -61,30 +135,141 @@ __device__ __forceinline__ double dudt_at(const DevOp& op, const S* v, long long
 //   mode 4 (SSPRK2 s2):                       out = 0.5*un + 0.5*(v + dt*k)
 //   mode 5 (SSPRK3 s2):                       out = 0.75*un + 0.25*(v + dt*k)
 //   mode 6 (SSPRK3 s3):                       out = (1/3)*un + (2/3)*(v + dt*k)
-template <int KIND, typename S>
+template <int KIND, int NLM, typename S>
 __global__ void __launch_bounds__(kBlock) k_stage(DevOp op, const S* __restrict__ v,
                                                  const S* __restrict__ un, S* __restrict__ acc,
                                                  S* __restrict__ out, const double* f,
-                                                 double f_const, NonlinearTerms nl, double ut,
-                                                 double dt, double c1, int mode) {
+                                                 double f_const, NonlinearTerms nl, Cubic cb,
+                                                 double ut, double dt, double c1, int mode) {
   const double inv_ut = 1.0 / ut;
   for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < op.n;
        id += (long long)gridDim.x * blockDim.x) {
     const IJK c = coords(op, id);
     const int i = c.i, j = c.j;
-    const double k = dudt_at<KIND, S>(op, v, id, i, j, f, f_const, nl, ut, inv_ut);
+    const double k = dudt_at<KIND, NLM, S>(op, v, id, i, j, f, f_const, nl, cb, ut, inv_ut);
     const double u0 = ldv(un, id);
     double o;
     switch (mode) {
-      case 0: if (acc) acc[id] = static_cast<S>(k); o = u0 + c1 * k; break;
-      case 1: { const double a = ldv(acc, id) + 2.0 * k; acc[id] = static_cast<S>(a); o = u0 + c1 * k; break; }
-      case 2: o = u0 + c1 * ((ldv(acc, id)) + k); break;
-      case 3: o = u0 + c1 * (ldv(acc, id) + k); break;
+      case 0: if (acc) acc[id] = static_cast<S>(k); o = stage_out(u0, c1, k); break;
+      case 1: { const double a = stage_acc(ldv(acc, id), k); acc[id] = static_cast<S>(a); o = stage_out(u0, c1, k); break; }
+      case 2: o = stage_last(u0, c1, ldv(acc, id), k); break;
+      case 3: o = stage_last(u0, c1, ldv(acc, id), k); break;
       case 4: o = 0.5 * u0 + 0.5 * (ldv(v, id) + dt * k); break;
       case 5: o = 0.75 * u0 + 0.25 * (ldv(v, id) + dt * k); break;
       default: o = (1.0 / 3.0) * u0 + (2.0 / 3.0) * (ldv(v, id) + dt * k); break;
     }
     out[id] = static_cast<S>(o);
+  }
+}
+
+// One whole RK4 step of a 2D time series per launch (temporal blocking).  A workgroup owns a
+// TX x TY tile (TX = 54) and holds u on the tile plus a 5-node ring, EX = 64 columns by EY
+// rows, in LDS.  Wave w (of 4) owns rows w, w + 4, ... and lane l column l: the same nodes in
+// every stage, so u_n and the RK4 accumulator stay in registers and LDS holds only the stage
+// input and output.  Stage r runs on the extended tile shrunk by r nodes per side, so no stage
+// needs another workgroup's values: per node, u_n is read once and u_{n+1} written once
+// instead of the 17 accesses of the per-stage launches.  Stage values are rounded to the state
+// type S where the per-stage kernels store them (k1 and the accumulator, each stage input, the
+// result), with the same shared arithmetic, so the result is theirs bit for bit.  The closures
+// that follow a step (Neumann / Robin x faces, then y faces, then Dirichlet x and y faces;
+// reference solver.cpp:1300-1329) run in LDS.  u_{n+1} is formed on the tile plus one ring
+// because a face node's inward neighbour can sit in the next tile (a grid side of 1 + a
+// multiple of the tile width); the x-face closure runs on that ring as well, since the y-face
+// closure reads its results.
+template <int KIND, int NLM, typename S, int TY>
+__global__ void __launch_bounds__(256) k_rk4_step(DevOp op, FaceData fd, const S* __restrict__ u,
+                                                 S* __restrict__ out, const double* f, double f_const,
+                                                 NonlinearTerms nl, Cubic cb, double ut, double h2,
+                                                 double h1, double h6, int tiles_x) {
+  constexpr int H = 5, EX = 64, TX = EX - 2 * H, EY = TY + 2 * H, KR = EY / 4;
+  static_assert(EY % 4 == 0, "the rows are dealt to four waves");
+  __shared__ S L0[EX * EY], L1[EX * EY];
+  const int nx = op.nx, ny = op.ny;
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int i0 = static_cast<int>(blockIdx.x % tiles_x) * TX - H;
+  const int j0 = static_cast<int>(blockIdx.x / tiles_x) * TY - H;
+  const int i = i0 + lane;
+  const bool col_in = i >= 0 && i < nx;
+  const bool col_face = i == 0 || i == nx - 1;
+  S u0[KR], acc[KR];
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int ey = wave + 4 * k, j = j0 + ey;
+    const S v = (col_in && j >= 0 && j < ny) ? u[(long long)j * nx + i] : S(0);
+    u0[k] = v;
+    acc[k] = S(0);
+    L0[ey * EX + lane] = v;
+  }
+  __syncthreads();
+  const double inv_ut = 1.0 / ut;
+  // stage r reads L0, L1, L0, L1 (u_n, T1, T2, T3) and writes L1, L0, L1, L0 (T1, T2, T3,
+  // u_{n+1}); u_n itself comes from registers after stage 1
+#pragma unroll 1
+  for (int r = 1; r <= 4; ++r) {
+    const S* vin = (r & 1) ? L0 : L1;
+    S* vout = (r & 1) ? L1 : L0;
+    const double c1 = r == 3 ? h1 : h2;
+    const bool lane_on = col_in && lane >= r && lane < EX - r;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      const int ey = wave + 4 * k, j = j0 + ey;
+      if (ey < r || ey >= EY - r || j < 0 || j >= ny) continue;   // the same for the whole wave
+      if (!lane_on) continue;
+      const int q = ey * EX + lane;
+      double kv = 0.0;
+      if (!col_face && j != 0 && j != ny - 1)
+        kv = dudt_values<KIND, NLM>(op, (long long)j * nx + i, double(vin[q]), double(vin[q + 1]),
+                                    double(vin[q - 1]), double(vin[q + EX]), double(vin[q - EX]),
+                                    f, f_const, nl, cb, ut, inv_ut);
+      const double un = double(u0[k]);
+      if (r == 1) {
+        acc[k] = static_cast<S>(kv);
+        vout[q] = static_cast<S>(stage_out(un, c1, kv));
+      } else if (r < 4) {
+        acc[k] = static_cast<S>(stage_acc(double(acc[k]), kv));
+        vout[q] = static_cast<S>(stage_out(un, c1, kv));
+      } else {
+        vout[q] = static_cast<S>(stage_last(un, h6, double(acc[k]), kv));
+      }
+    }
+    __syncthreads();
+  }
+  // closures on L0 = u_{n+1}: phase 0 (x faces, Neumann / Robin) on the tile plus one ring,
+  // the later phases on the tile; only workgroups whose tile or ring holds a face node
+  const bool on_face = i0 + H - 1 <= 0 || i0 + EX - H >= nx - 1 || j0 + H - 1 <= 0 ||
+                       j0 + EY - H >= ny - 1;
+#pragma unroll 1
+  for (int phase = 0; on_face && phase < 4; ++phase) {
+    const int axis = phase & 1;
+    const bool dirichlet = phase >= 2;
+    const int lo = phase == 0 ? H - 1 : H;     // region [lo, EX - lo) x [lo, EY - lo)
+    const bool lane_on = col_in && lane >= lo && lane < EX - lo;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      const int ey = wave + 4 * k, j = j0 + ey;
+      if (ey < lo || ey >= EY - lo || j < 0 || j >= ny) continue;
+      if (!lane_on) continue;
+      const int pos = axis == 0 ? i : j, last = axis == 0 ? nx - 1 : ny - 1;
+      if (pos != 0 && pos != last) continue;
+      const int side = pos == 0 ? 0 : 1, face = 2 * axis + side;
+      const long long fq = axis == 0 ? j : i;
+      const int q = ey * EX + lane;
+      if (dirichlet) {
+        if (fd.kind[face] == PDEHIP_BC_DIRICHLET) L0[q] = static_cast<S>(fd.value[face][fq]);
+      } else if (fd.kind[face] != PDEHIP_BC_DIRICHLET) {
+        const int inner = axis == 0 ? q + (side ? -1 : 1) : q + (side ? -EX : EX);
+        double v;
+        if (closure_nr_value(fd, face, side, fq, double(L0[inner]), &v)) L0[q] = static_cast<S>(v);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int ey = wave + 4 * k, j = j0 + ey;
+    if (ey < H || ey >= EY - H || j >= ny) continue;
+    if (lane >= H && lane < EX - H && i < nx) out[(long long)j * nx + i] = L0[ey * EX + lane];
   }
 }
 
@@ -365,12 +550,11 @@ void time_stage(hipStream_t s, const DevOp& op, const double* v, const double* u
                 double dt, double c1, int mode) {
   long long blocks = (op.n + kBlock - 1) / kBlock;
   if (blocks > 8192) blocks = 8192;
-  if (op.kind == PDEHIP_OP_POINTWISE)
-    hipLaunchKernelGGL((k_stage<PDEHIP_OP_POINTWISE, double>), dim3(blocks), dim3(kBlock), 0, s, op, v, un, acc, out, f, f_const, nl, ut, dt, c1, mode);
-  else if (op.kind == PDEHIP_OP_FLUX)
-    hipLaunchKernelGGL((k_stage<PDEHIP_OP_FLUX, double>), dim3(blocks), dim3(kBlock), 0, s, op, v, un, acc, out, f, f_const, nl, ut, dt, c1, mode);
-  else
-    hipLaunchKernelGGL((k_stage<PDEHIP_OP_CONST, double>), dim3(blocks), dim3(kBlock), 0, s, op, v, un, acc, out, f, f_const, nl, ut, dt, c1, mode);
+  const Cubic cb = cubic_of(nl);
+  by_kind_nl(op, nl, [&](auto K, auto M) {
+    hipLaunchKernelGGL((k_stage<decltype(K)::value, decltype(M)::value, double>), dim3(blocks),
+                       dim3(kBlock), 0, s, op, v, un, acc, out, f, f_const, nl, cb, ut, dt, c1, mode);
+  });
 }
 
 void time_stage32(hipStream_t s, const DevOp& op, const float* v, const float* un, float* acc,
@@ -378,12 +562,44 @@ void time_stage32(hipStream_t s, const DevOp& op, const float* v, const float* u
                   double dt, double c1, int mode) {
   long long blocks = (op.n + kBlock - 1) / kBlock;
   if (blocks > 8192) blocks = 8192;
-  if (op.kind == PDEHIP_OP_POINTWISE)
-    hipLaunchKernelGGL((k_stage<PDEHIP_OP_POINTWISE, float>), dim3(blocks), dim3(kBlock), 0, s, op, v, un, acc, out, f, f_const, nl, ut, dt, c1, mode);
-  else if (op.kind == PDEHIP_OP_FLUX)
-    hipLaunchKernelGGL((k_stage<PDEHIP_OP_FLUX, float>), dim3(blocks), dim3(kBlock), 0, s, op, v, un, acc, out, f, f_const, nl, ut, dt, c1, mode);
-  else
-    hipLaunchKernelGGL((k_stage<PDEHIP_OP_CONST, float>), dim3(blocks), dim3(kBlock), 0, s, op, v, un, acc, out, f, f_const, nl, ut, dt, c1, mode);
+  const Cubic cb = cubic_of(nl);
+  by_kind_nl(op, nl, [&](auto K, auto M) {
+    hipLaunchKernelGGL((k_stage<decltype(K)::value, decltype(M)::value, float>), dim3(blocks),
+                       dim3(kBlock), 0, s, op, v, un, acc, out, f, f_const, nl, cb, ut, dt, c1, mode);
+  });
+}
+
+namespace {
+bool g_fused_steps = true;
+
+// Tiles of 54 x TY nodes (64-column extended tiles); TY = 22 keeps each workgroup's LDS at
+// 16 KB (FP32) / 32 KB (FP64) and a 2049^2 grid at 3572 workgroups.
+template <typename S, int TY>
+void rk4_step_launch(hipStream_t s, const DevOp& op, const FaceData& fd, const S* u, S* out,
+                     const double* f, double fc, const NonlinearTerms& nl, double ut, double dt) {
+  constexpr int TX = 54;
+  const int tx = (op.nx + TX - 1) / TX, ty = (op.ny + TY - 1) / TY;
+  const dim3 g(static_cast<unsigned>(tx * ty));
+  const double h2 = 0.5 * dt, h6 = dt / 6.0;
+  const Cubic cb = cubic_of(nl);
+  by_kind_nl(op, nl, [&](auto K, auto M) {
+    hipLaunchKernelGGL((k_rk4_step<decltype(K)::value, decltype(M)::value, S, TY>), g,
+                       dim3(256), 0, s, op, fd, u, out, f, fc, nl, cb, ut, h2, dt, h6, tx);
+  });
+}
+}  // namespace
+
+void set_fused_steps(bool on) { g_fused_steps = on; }
+bool fused_steps() { return g_fused_steps; }
+
+void time_rk4_step(hipStream_t s, const DevOp& op, const FaceData& fd, const double* u, double* out,
+                   const double* f, double fc, const NonlinearTerms& nl, double ut, double dt) {
+  rk4_step_launch<double, 22>(s, op, fd, u, out, f, fc, nl, ut, dt);
+}
+
+void time_rk4_step(hipStream_t s, const DevOp& op, const FaceData& fd, const float* u, float* out,
+                   const double* f, double fc, const NonlinearTerms& nl, double ut, double dt) {
+  rk4_step_launch<float, 22>(s, op, fd, u, out, f, fc, nl, ut, dt);
 }
 
 void time_utt(hipStream_t s, const DevOp& op, const double* u, double* vel, double* next,

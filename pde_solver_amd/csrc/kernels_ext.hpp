@@ -8,6 +8,14 @@ namespace pdehip {
 void time_stage(hipStream_t s, const DevOp& op, const double* v, const double* un, double* acc,
                 double* out, const double* f, double f_const, const NonlinearTerms& nl, double ut,
                 double dt, double c1, int mode);
+// One whole RK4 step (four stages and the closures that follow) in one launch, u -> out
+// (2D; the same bits as the per-stage launches).  fused_steps: whether time series use it.
+void time_rk4_step(hipStream_t s, const DevOp& op, const FaceData& fd, const double* u, double* out,
+                   const double* f, double fc, const NonlinearTerms& nl, double ut, double dt);
+void time_rk4_step(hipStream_t s, const DevOp& op, const FaceData& fd, const float* u, float* out,
+                   const double* f, double fc, const NonlinearTerms& nl, double ut, double dt);
+void set_fused_steps(bool on);
+bool fused_steps();
 void time_stage32(hipStream_t s, const DevOp& op, const float* v, const float* un, float* acc,
                   float* out, const double* f, double f_const, const NonlinearTerms& nl, double ut,
                   double dt, double c1, int mode);

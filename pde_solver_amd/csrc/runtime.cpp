@@ -1653,6 +1653,7 @@ void run_series(Ctx* ctx, Plan& PL, const pdehip_problem& p, const pdehip_time_c
     S* A = acc.as<S>();
     S* Ta = ta.as<S>();
     S* Tb = tb.as<S>();
+    bool closed = false;   // the step's kernel applied the closures itself
     if (integ) {
       switch (tc.integrator) {
         case PDEHIP_TI_RK2:
@@ -1660,6 +1661,13 @@ void run_series(Ctx* ctx, Plan& PL, const pdehip_problem& p, const pdehip_time_c
           stage(s, op, Ta, U, A, U, f, fc, nl, ut, dt, 0.5 * dt, 3);
           break;
         case PDEHIP_TI_RK4:
+          if (fused_steps()) {
+            // the whole step and its closures in one launch (same bits)
+            time_rk4_step(s, op, fd, U, Ta, f, fc, nl, ut, dt);
+            std::swap(u, ta);
+            closed = true;
+            break;
+          }
           stage(s, op, U, U, A, Ta, f, fc, nl, ut, dt, 0.5 * dt, 0);
           stage(s, op, Ta, U, A, Tb, f, fc, nl, ut, dt, 0.5 * dt, 1);
           stage(s, op, Tb, U, A, Ta, f, fc, nl, ut, dt, dt, 1);
@@ -1689,8 +1697,10 @@ void run_series(Ctx* ctx, Plan& PL, const pdehip_problem& p, const pdehip_time_c
       stage(s, op, U, U, static_cast<S*>(nullptr), Ta, f, fc, nl, ut, dt, dt, 0);
       std::swap(u, ta);
     }
-    closures_nr(L0, op, fd, u.as<S>());
-    closures_dirichlet(L0, op, fd, u.as<S>());
+    if (!closed) {
+      closures_nr(L0, op, fd, u.as<S>());
+      closures_dirichlet(L0, op, fd, u.as<S>());
+    }
     ++steps;
   }
   flush();
