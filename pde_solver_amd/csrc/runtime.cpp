@@ -1239,9 +1239,11 @@ void plan_residual_norms(Plan& P, const double* u_dev, double* l2, double* linf)
 // in FP64, and only that true residual stops the solve.  The Krylov state (p, r_hat, rho)
 // carries on across replacements; a breakdown restarts it from the true residual.  FP32
 // applies use the difference form.  On diffusion-dominated problems of 513^2 and more the FP32
-// iteration can diverge where FP64 BiCGStab converges: the solve then completes with the FP64
-// BiCGStab from the initial guess (report: refinements = -1, inner_iterations = the FP32
-// iterations spent).  Restarted GMRES in mixed precision keeps the FP64 iteration count.
+// iteration can diverge or stall where FP64 BiCGStab converges: a true residual above 1e3 |r0|,
+// or one that has not halved for 100 iterations, ends the FP32 pass, and the solve completes
+// with the FP64 BiCGStab, from the current solution when the FP32 pass cut the true residual
+// by 1e3 and from the initial guess otherwise (report: refinements = -1, inner_iterations = the
+// FP32 iterations spent).  Restarted GMRES in mixed precision keeps the FP64 iteration count.
 static void solve_bicgstab_mixed(Plan& P, pdehip_report* rep) {
   const hipStream_t s = P.stream();
   const long long n = P.g.n;
