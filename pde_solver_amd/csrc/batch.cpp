@@ -56,7 +56,10 @@ struct BatchPlan {
   pdehip_options opt{};
   pdehip_batch_config cfg{};
   bool mixed = false;
-  bool krylov = false;                     // BiCGStab batch (no hierarchy)
+  bool krylov = false;                     // BiCGStab or GMRES batch (no hierarchy)
+  bool gmres = false;                      // GMRES(mm) batch
+  int mm = 0;                              // GMRES restart length
+  DBuf gvecs;                              // GMRES: BGVec [n]
   int L = 0;                               // deepest hierarchy
   std::vector<std::unique_ptr<Plan>> plans;
   std::vector<int> levels;                 // per problem
@@ -76,7 +79,7 @@ struct BatchPlan {
   int* hactive = nullptr;                  // two poll slots
   hipEvent_t evpoll[2] = {nullptr, nullptr};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};   // captured PCG iterations (FP64 / inner)
+  hipGraphExec_t graph[2] = {nullptr, nullptr};   // captured iterations (FP64 / inner) or GMRES cycle
   std::function<void()> on_chunk;          // host hook after each polling chunk is enqueued
   size_t bytes = 0;                        // device memory held by the batch and its plans
   ~BatchPlan() {
@@ -93,6 +96,7 @@ struct BatchPlan {
   const BVec* V() const { return vecs.as<BVec>(); }
   const BKVec* K() const { return kvecs.as<BKVec>(); }
   const BKVec* K64() const { return kvecs64.as<BKVec>(); }
+  const BGVec* G() const { return gvecs.as<BGVec>(); }
   Prec coarse = P16;                       // mixed: storage of levels >= 1 (from the plans)
   Prec prec(int l) const { return mixed ? (l == 0 ? P32 : coarse) : P64; }
 };
@@ -280,10 +284,12 @@ bool batch_eligible(int count, const pdehip_problem* probs, const pdehip_options
                     std::string* why) {
   auto no = [&](const char* r) { if (why) *why = r; return false; };
   if (count < 1) return no("empty batch");
-  const bool bicg = opt.method == PDEHIP_BICGSTAB;
-  if (opt.method != PDEHIP_MGPCG && !bicg) return no("batched engine runs MG-PCG or BiCGStab");
+  const bool gm = opt.method == PDEHIP_GMRES;
+  const bool bicg = opt.method == PDEHIP_BICGSTAB || gm;   // the Krylov engines
+  if (opt.method != PDEHIP_MGPCG && !bicg) return no("batched engine runs MG-PCG, BiCGStab or GMRES");
   if (bicg && opt.precond != PDEHIP_PC_NONE && opt.precond != PDEHIP_PC_JACOBI)
-    return no("batched BiCGStab takes no preconditioner or Jacobi");
+    return no("batched BiCGStab / GMRES take no preconditioner or Jacobi");
+  if (gm && opt.precision == PDEHIP_MIXED) return no("batched GMRES runs in FP64");
   if (!bicg && (opt.mg_pre < 2 || opt.mg_pre % 2 || opt.mg_post < 2 || opt.mg_post % 2))
     return no("batched engine needs even pre/post smoothing counts");
   for (int i = 0; i < count; ++i) {
@@ -297,7 +303,7 @@ bool batch_eligible(int count, const pdehip_problem* probs, const pdehip_options
 }
 
 bool batch_plan_ok(const Plan& P, const pdehip_options& opt) {
-  if (opt.method == PDEHIP_BICGSTAB) return true;
+  if (opt.method == PDEHIP_BICGSTAB || opt.method == PDEHIP_GMRES) return true;
   const int nl = static_cast<int>(P.L.size());
   return nl >= 2 && P.L[nl - 1].m > 0;
 }
@@ -391,6 +397,34 @@ BKVec krylov_view(Plan& P, bool jac, bool fp32) {
   return k;
 }
 
+// Per-problem GMRES data: the plan's FP64 basis, Hessenberg system and work vectors.
+BGVec gmres_view(Plan& P, bool jac) {
+  BGVec g;
+  std::memset(&g, 0, sizeof(g));
+  g.V = P.V.as<double>();
+  g.H = P.H.as<double>();
+  g.cs = P.cs.as<double>();
+  g.sn = P.sn.as<double>();
+  g.g = P.gv.as<double>();
+  g.y = P.yv.as<double>();
+  g.hc = P.hcb.as<double>();
+  g.x = P.x.as<double>();
+  g.r = P.r.as<double>();
+  g.b = P.bvec.as<double>();
+  g.x0 = P.x0.as<double>();
+  g.w = P.w1.as<double>();
+  g.sc = P.sc(0);
+  g.op = P.L[0].op64;
+  g.n = P.g.n;
+  g.nx = P.g.nx;
+  g.ny = P.g.ny;
+  g.jacobi = jac ? 1 : 0;
+  const void* need[] = {g.V, g.H, g.cs, g.sn, g.g, g.y, g.hc, g.x, g.r, g.b, g.x0, g.w, g.sc};
+  for (const void* q : need)
+    if (!q) throw Error(PDEHIP_ERR_DEVICE, "batched GMRES: a vector of a plan is missing");
+  return g;
+}
+
 // BiCGStab batch: one resident plan per problem (operator, folded boundary rows, right-hand
 // side, Krylov vectors), the BKVec table, row-block items for the vector kernels, level-0
 // strips for the applies, and the finish buffers of the MG-PCG engine.
@@ -399,10 +433,13 @@ BatchPtr create_krylov(BatchPtr B, Ctx* ctx, int count, const pdehip_problem* pr
                        std::vector<std::unique_ptr<Plan>>* given) {
   const hipStream_t s = ctx->stream;
   B->krylov = true;
+  B->gmres = opt.method == PDEHIP_GMRES;
+  B->mm = std::max(1, std::min(opt.gmres_restart, std::max(1, opt.max_iter)));
   B->L = 1;
   const bool jac = opt.precond == PDEHIP_PC_JACOBI;
   std::vector<BVec> vecs(count);
-  std::vector<BKVec> kv(count);
+  std::vector<BKVec> kv(B->gmres ? 0 : count);
+  std::vector<BGVec> gv(B->gmres ? count : 0);
   for (int i = 0; i < count; ++i) {
     B->plans.push_back(take_or_create(ctx, probs, opt, given, i));
     Plan& P = *B->plans.back();
@@ -419,12 +456,18 @@ BatchPtr create_krylov(BatchPtr B, Ctx* ctx, int count, const pdehip_problem* pr
     v.nx = P.g.nx;
     v.ny = P.g.ny;
     v.op64 = P.L[0].op64;
-    kv[i] = krylov_view(P, jac, B->mixed);
+    if (B->gmres) gv[i] = gmres_view(P, jac);
+    else kv[i] = krylov_view(P, jac, B->mixed);
   }
   B->vecs.alloc(ctx, count * sizeof(BVec));
-  B->kvecs.alloc(ctx, count * sizeof(BKVec));
   HIPCHK(hipMemcpyAsync(B->vecs.get(), vecs.data(), count * sizeof(BVec), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(B->kvecs.get(), kv.data(), count * sizeof(BKVec), hipMemcpyHostToDevice, s));
+  if (B->gmres) {
+    B->gvecs.alloc(ctx, count * sizeof(BGVec));
+    HIPCHK(hipMemcpyAsync(B->gvecs.get(), gv.data(), count * sizeof(BGVec), hipMemcpyHostToDevice, s));
+  } else {
+    B->kvecs.alloc(ctx, count * sizeof(BKVec));
+    HIPCHK(hipMemcpyAsync(B->kvecs.get(), kv.data(), count * sizeof(BKVec), hipMemcpyHostToDevice, s));
+  }
   HIPCHK(hipEventCreate(&B->ev0));
   HIPCHK(hipEventCreate(&B->ev1));
   const int ychunk = B->cfg.tile_y > 0 ? B->cfg.tile_y : 16;
@@ -442,7 +485,8 @@ BatchPtr create_krylov(BatchPtr B, Ctx* ctx, int count, const pdehip_problem* pr
     }
     upload_list(ctx, s, B->rb0, rb, &f, &c);
   }
-  const int np = std::max({B->el0.n, B->rows0.n, B->rb0.n});
+  // GMRES: the CGS partials hold (mm + 1) sums per row-block item
+  const int np = std::max({B->el0.n, B->rows0.n, B->rb0.n * (B->gmres ? B->mm + 1 : 1)});
   B->part.alloc(ctx, (size_t)np * sizeof(double));
   B->maxp.alloc(ctx, (size_t)np * sizeof(double));
   setup_finish(*B);
@@ -466,7 +510,7 @@ BatchPtr batch_create(Ctx* ctx, int count, const pdehip_problem* probs, const pd
   const hipStream_t s = ctx->stream;
   const size_t used0 = ctx->used.load();
   B->plans.reserve(count);
-  if (opt.method == PDEHIP_BICGSTAB)
+  if (opt.method == PDEHIP_BICGSTAB || opt.method == PDEHIP_GMRES)
     return create_krylov(std::move(B), ctx, count, probs, opt, used0, plans);
   for (int i = 0; i < count; ++i) {
     B->plans.push_back(take_or_create(ctx, probs, opt, plans, i));
@@ -773,6 +817,75 @@ void bicgstab(BatchPlan& B, int max_iter, Prec kp, const BKVec* K, int graph_slo
   if (outstanding >= 0) (void)poll_read(B, outstanding);
 }
 
+// GMRES(mm) on every problem in lock step: the single-problem path's cycle (runtime.cpp
+// solve_gmres: V_0 = r / |r|; per step w = A M^-1 V_j, two classical Gram-Schmidt passes, the
+// norm, the Givens step; then back substitution, x += M^-1 V y and the true residual) with every
+// launch covering every problem.  A problem whose residual estimate meets the target leaves the
+// cycle; the true residual at the cycle's end ends it.  One cycle is captured as a graph and
+// replayed; the host reads the count of unfinished problems after each cycle (pipelined).
+void gmres(BatchPlan& B, int max_iter) {
+  const hipStream_t s = B.ctx->stream;
+  const int mm = B.mm;
+  const BGVec* G = B.G();
+  double* p1 = B.part.as<double>();
+  double* p2 = B.maxp.as<double>();
+  const List& rb = B.rb0;
+  const List& rw = B.rows0;
+  const int* rbf = rb.first.as<int>();
+  const int* rbc = rb.cnt.as<int>();
+  auto cycle = [&](int m) {
+    bk::gcycle(s, G, B.n, bk::GC_START, mm);
+    bk::gvec(s, bk::KG_START, G, rb.d(), rb.n, 0, mm, nullptr, nullptr);
+    for (int j = 0; j < m; ++j) {
+      bk::gapply(s, 0, G, rw.d(), rw.n, j, nullptr);
+      bk::gvec(s, bk::KG_DOT, G, rb.d(), rb.n, j, mm, p1, nullptr);
+      bk::gfinalize(s, G, rbf, rbc, rb.n, B.n, p1, bk::GF_H1, j, mm, 0.0, 0);
+      bk::gvec(s, bk::KG_AXPY1, G, rb.d(), rb.n, j, mm, nullptr, nullptr);
+      bk::gvec(s, bk::KG_DOT, G, rb.d(), rb.n, j, mm, p1, nullptr);
+      bk::gfinalize(s, G, rbf, rbc, rb.n, B.n, p1, bk::GF_H2, j, mm, 0.0, 0);
+      bk::gvec(s, bk::KG_AXPY2, G, rb.d(), rb.n, j, mm, nullptr, p2);
+      bk::gfinalize(s, G, rbf, rbc, rb.n, B.n, p2, bk::GF_NORM, j, mm, 0.0, 0);
+      bk::gvec(s, bk::KG_NORMALIZE, G, rb.d(), rb.n, j, mm, nullptr, nullptr);
+    }
+    bk::gcycle(s, G, B.n, bk::GC_BACKSUB, mm);
+    bk::gvec(s, bk::KG_UPDATE, G, rb.d(), rb.n, 0, mm, nullptr, nullptr);
+    bk::gapply(s, 1, G, rw.d(), rw.n, 0, p1);
+    bk::gfinalize(s, G, rw.first.as<int>(), rw.cnt.as<int>(), rw.n, B.n, p1, bk::GF_R, 0, mm, 0.0, 0);
+  };
+  int outstanding = -1;
+  int chunk = 0;
+  for (int it = 0; it < max_iter; it += mm, ++chunk) {
+    const int m = std::min(mm, max_iter - it);
+    if (B.cfg.use_graph && m == mm) {
+      hipGraphExec_t& ge = B.graph[0];
+      if (!ge) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          cycle(m);
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIPCHK(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(e);
+      }
+      HIPCHK(hipGraphLaunch(ge, s));
+    } else {
+      cycle(m);
+    }
+    const int slot = chunk & 1;
+    if (B.on_chunk) B.on_chunk();
+    poll_enqueue(B, 0, slot);
+    if (outstanding >= 0 && poll_read(B, outstanding) == 0) { outstanding = -1; break; }
+    outstanding = slot;
+  }
+  if (outstanding >= 0) (void)poll_read(B, outstanding);
+}
+
 }  // namespace
 
 int batch_solve(BatchPlan& B, pdehip_report* reports) {
@@ -785,7 +898,13 @@ int batch_solve(BatchPlan& B, pdehip_report* reports) {
   bk::init_scalars(s, B.V(), n, B.krylov ? 1e-30 : 1e-300);
   HIPCHK(hipEventRecord(B.ev0, s));
   bk::vec(s, P64, bk::V_COPY_X0, B.V(), el.d(), el.n, nullptr, nullptr);
-  if (B.krylov) {
+  if (B.gmres) {
+    // r = b - A x0, |r0| and the target, then the restart cycles
+    bk::gapply(s, 1, B.G(), rw.d(), rw.n, 0, part);
+    bk::gfinalize(s, B.G(), rw.first.as<int>(), rw.cnt.as<int>(), rw.n, n, part, bk::GF_R0, 0,
+                  B.mm, B.opt.tol, B.opt.tol_mode);
+    gmres(B, max_iter);
+  } else if (B.krylov) {
     // r = b - A x0 in FP64 (stored in the Krylov precision), |r0| and the target
     const Prec kp = B.mixed ? P32 : P64;
     bk::kapply(s, kp, 2, B.K(), rw.d(), rw.n, part, nullptr);
@@ -885,6 +1004,7 @@ double batch_time_kernel(BatchPlan& B, int kernel, int reps, double* algorithmic
   double bytes = 0.0;
   float ms = 0.f;
   double* part = B.part.as<double>();
+  if (B.gmres) throw Error(PDEHIP_ERR_UNSUPPORTED, "batch_time_kernel: not for GMRES batches");
   if (B.krylov) {
     const double ek = kp == P64 ? 8.0 : 4.0;
     if (kernel == 2) {          // v = A ph with rh.v partials (read ph, rh; write v)

@@ -54,6 +54,20 @@ struct BKVec {
   int nx, ny, jacobi, pad;
 };
 
+// Level-0 data of one problem of a batched GMRES(m) (reference GMRESSolve, krylov.cpp:187-333,
+// in the single-problem path's form: classical Gram-Schmidt applied twice, right Jacobi
+// preconditioning optional).  FP64 throughout: the Arnoldi basis V ((m+1) vectors of n), the
+// Hessenberg matrix H ((m+1) x m, leading dimension m), the Givens rotations, the least-squares
+// right-hand side g, the solution y, and the two CGS passes' coefficients hc (m+1 apart).
+struct BGVec {
+  double *V, *H, *cs, *sn, *g, *y, *hc;
+  double *x, *r, *b, *x0, *w;
+  KScalars* sc;
+  DevOp op;       // FP64 operator (folded rows)
+  long long n;
+  int nx, ny, jacobi, pad;
+};
+
 // Finish data of one problem: boundary data, source.
 struct BFace { FaceData fd; const double* f; double f_const; };
 
@@ -101,7 +115,8 @@ double batch_time_kernel(BatchPlan& B, int kernel, int reps, double* algorithmic
 size_t batch_bytes(const BatchPlan& B);
 void batch_destroy(BatchPtr& B);
 // True when the batched engine handles this set: 2D constant-coefficient linear operators, and
-// MG-PCG without convection, or BiCGStab (no preconditioner or Jacobi).
+// MG-PCG without convection, BiCGStab (no preconditioner or Jacobi), or FP64 GMRES(m) (no
+// preconditioner or Jacobi).
 bool batch_eligible(int count, const pdehip_problem* probs, const pdehip_options& opt,
                     std::string* why);
 
@@ -194,6 +209,29 @@ void kreset_failed(hipStream_t s, const BKVec* vecs, int nprob, double eps);
 void kfailed(hipStream_t s, const BKVec* vecs, int nprob, int* out);
 void kfinalize(hipStream_t s, const BKVec* vecs, const int* first, const int* cnt, int nprob,
                const double* p1, const double* p2, int op, double tol, int tol_mode);
+// ---- batched GMRES(m) (k_batch.hip).  Row-block items for the vector kernels, level-0 strips
+// for the applies; `col` is the Arnoldi step of the current cycle, the same for every problem.
+// KG_DOT: partials[i * n + item] = V_i . w for i <= col (launched over groups of 8 vectors);
+// KG_AXPY1 / KG_AXPY2: w -= sum_i hc_pass[i] V_i (AXPY2 also writes the partials w . w to
+// partials2); KG_NORMALIZE: V_{col+1} = w / |w| (|w| from hc[2 m + 1]); KG_START: V_0 = r / |r|;
+// KG_UPDATE: x += M^-1 sum_{i < steps} y_i V_i.
+enum KgVec { KG_DOT = 0, KG_AXPY1 = 1, KG_AXPY2 = 2, KG_NORMALIZE = 3, KG_START = 4, KG_UPDATE = 5 };
+void gvec(hipStream_t s, int op, const BGVec* vecs, const BItem* items, int n, int col, int mm,
+          double* partials, double* partials2);
+// mode 0: w = A M^-1 V_col; mode 1: r = b - A x, partials |r|^2 per item.
+void gapply(hipStream_t s, int mode, const BGVec* vecs, const BItem* items, int n, int col,
+            double* partials);
+// Segmented finalize per problem: GF_H1 / GF_H2 sum the CGS coefficients of step col (GF_H2
+// also sets H[i][col] = h1[i] + h2[i]); GF_NORM sets H[col+1][col] = |w| and applies the Givens
+// step; GF_R0 / GF_R: the (initial) true residual, target and stop test.
+enum KgFin { GF_H1 = 0, GF_H2 = 1, GF_NORM = 2, GF_R0 = 3, GF_R = 4 };
+void gfinalize(hipStream_t s, const BGVec* vecs, const int* first, const int* cnt, int nitems,
+               int nprob, const double* partials, int op, int col, int mm, double tol,
+               int tol_mode);
+// GC_START: a new cycle (g = [|r|, 0, ...], H = 0, cycle flags cleared); GC_BACKSUB: y from the
+// triangular system of the cycle's steps.
+enum KgCycle { GC_START = 0, GC_BACKSUB = 1 };
+void gcycle(hipStream_t s, const BGVec* vecs, int nprob, int op, int mm);
 }  // namespace bk
 
 }  // namespace pdehip

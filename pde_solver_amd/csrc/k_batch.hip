@@ -1046,6 +1046,272 @@ kk_finalize(const BKVec* __restrict__ vecs, const int* __restrict__ first,
   }
 }
 
+// ------------------------------------------------------------------------ batched GMRES(m)
+// The single-problem path's GMRES (runtime.cpp solve_gmres, k_krylov.hip) with every launch
+// covering every problem of the batch: vector kernels over row-block items, applies over
+// level-0 strips, per-problem segmented reductions, and the per-problem scalar work (Givens
+// steps, back substitution) in one block per problem.  Problems leave a cycle when their
+// residual estimate meets the target (cycle_done) and the batch when the true residual at the
+// cycle's end does (done).
+
+// Right Jacobi preconditioner at node (i, j): division by the folded diagonal on interior rows,
+// identity on boundary rows (k_krylov.hip k_jacobi_pc).
+__device__ __forceinline__ double kg_pc(const BGVec& v, int i, int j, double x) {
+  if (!v.jacobi || i == 0 || j == 0 || i == v.nx - 1 || j == v.ny - 1) return x;
+  return x / kk_diag(v.op, i, j);
+}
+
+__device__ __forceinline__ bool kg_out_of_cycle(const KScalars* sc) {
+  return done(sc) || ldS(&sc->cycle_done, 0) != 0;
+}
+
+template <int OP>
+__global__ void __launch_bounds__(kLanes * kWavesPB)
+kg_vec(const BGVec* __restrict__ vecs, const BItem* __restrict__ items, int n, int col, int mm,
+       double* partials, double* partials2) {
+  const int w = wave_item();
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BGVec& v = vecs[it.p];
+  const int lane = threadIdx.x & (kLanes - 1);
+  const KScalars* sc = v.sc;
+  const long long nn = v.n;
+  const int nx = v.nx;
+  if constexpr (OP == KG_DOT) {
+    // V_i . w for the group i = 8 blockIdx.y .. +7 (i <= col)
+    const int i0 = static_cast<int>(blockIdx.y) * 8;
+    const int cnt = min(8, col + 1 - i0);
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (!kg_out_of_cycle(sc)) {
+      for (int j = it.a; j < it.b; ++j) {
+        for (int i = lane; i < nx; i += kLanes) {
+          const long long id = (long long)j * nx + i;
+          const double wv = ldS(v.w, id);
+#pragma unroll
+          for (int t = 0; t < 8; ++t)
+            if (t < cnt) acc[t] += ldS(v.V, (long long)(i0 + t) * nn + id) * wv;
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      if (t >= cnt) break;
+      const double a = wave_sum(acc[t]);
+      if (lane == 0) partials[(long long)(i0 + t) * n + w] = a;
+    }
+  } else {
+    const bool in_cycle = OP == KG_START || OP == KG_UPDATE;
+    const bool run = in_cycle ? !done(sc) : !kg_out_of_cycle(sc);
+    double ww = 0.0;
+    if (run) {
+      const double* hc = v.hc + (OP == KG_AXPY2 ? mm + 1 : 0);
+      // |w| of this step, kept at hc[2 mm + 1] (the Givens step has already zeroed H[col+1][col])
+      const double hn = OP == KG_NORMALIZE ? ldS(v.hc, 2 * mm + 1) : 1.0;
+      const double beta = OP == KG_START ? ldS(&sc->metric, 0) : 1.0;
+      const int steps = OP == KG_UPDATE ? ldS(&sc->steps, 0) : 0;
+      const double sign = -1.0;
+      if (OP != KG_NORMALIZE || hn > 0.0) {
+        for (int j = it.a; j < it.b; ++j) {
+          for (int i = lane; i < nx; i += kLanes) {
+            const long long id = (long long)j * nx + i;
+            if constexpr (OP == KG_AXPY1 || OP == KG_AXPY2) {
+              double acc = 0.0;
+              for (int q = 0; q <= col; ++q) acc += ldS(hc, q) * ldS(v.V, (long long)q * nn + id);
+              const double wn = ldS(v.w, id) + sign * acc;
+              stS(v.w, id, wn);
+              if (OP == KG_AXPY2) ww += wn * wn;
+            } else if constexpr (OP == KG_NORMALIZE) {
+              stS(v.V, (long long)(col + 1) * nn + id, ldS(v.w, id) / hn);
+            } else if constexpr (OP == KG_START) {
+              stS(v.V, id, ldS(v.r, id) / beta);
+            } else {   // KG_UPDATE: z = 0 + sum y_q V_q, x += M^-1 z
+              double acc = 0.0;
+              for (int q = 0; q < steps; ++q) acc += ldS(v.y, q) * ldS(v.V, (long long)q * nn + id);
+              const double z = 0.0 + 1.0 * acc;
+              stS(v.x, id, ldS(v.x, id) + 1.0 * kg_pc(v, i, j, z));
+            }
+          }
+        }
+      }
+    }
+    if constexpr (OP == KG_AXPY2) {
+      ww = wave_sum(ww);
+      if (lane == 0) partials2[w] = ww;
+    }
+  }
+}
+
+// MODE 0: w = A M^-1 V_col (the preconditioner applied to each stencil value on the fly);
+// MODE 1: r = b - A x with partials |r|^2.  FP64 rows in sum form with the folded centre, as
+// the single-problem FP64 apply.
+template <int MODE>
+__global__ void __launch_bounds__(kLanes * kWavesPB)
+kg_apply(const BGVec* __restrict__ vecs, const BItem* __restrict__ items, int n, int col,
+         double* partials) {
+  const int w = wave_item();
+  if (w >= n) return;
+  const BItem it = items[w];
+  const BGVec& v = vecs[it.p];
+  const int lane = threadIdx.x & (kLanes - 1);
+  const KScalars* sc = v.sc;
+  double a1 = 0.0;
+  const bool run = MODE == 0 ? !kg_out_of_cycle(sc) : !done(sc);
+  if (run) {
+    const DevOp& op = v.op;
+    const W7<double> w0 = const_w<double>(op);
+    auto ax_of = [&](int i, int j, double xc, double xw, double xe, double xs, double xn) -> double {
+      if (i == 0 || j == 0 || i == op.nx - 1 || j == op.ny - 1) return xc;
+      W7<double> wf = w0;
+      fold_center<2, double>(op, i, j, 0, wf);
+      return wf.C * xc + wf.E * xe + wf.W * xw + wf.N * xn + wf.S * xs;
+    };
+    if constexpr (MODE == 0) {
+      const double* src = v.V + (long long)col * v.n;
+      strip_march<double>(src, static_cast<const void*>(nullptr), op.nx, op.ny, it,
+                          [&](int i, int j, long long id, double xc, double xw, double xe,
+                              double xs, double xn, double) {
+        double o;
+        if (v.jacobi)
+          o = ax_of(i, j, kg_pc(v, i, j, xc), kg_pc(v, i - 1, j, xw), kg_pc(v, i + 1, j, xe),
+                    kg_pc(v, i, j - 1, xs), kg_pc(v, i, j + 1, xn));
+        else
+          o = ax_of(i, j, xc, xw, xe, xs, xn);
+        stS(v.w, id, o);
+      });
+    } else {
+      strip_march<double>(v.x, static_cast<const double*>(v.b), op.nx, op.ny, it,
+                          [&](int i, int j, long long id, double xc, double xw, double xe,
+                              double xs, double xn, double bj) {
+        const double rv = bj - ax_of(i, j, xc, xw, xe, xs, xn);
+        stS(v.r, id, rv);
+        a1 += rv * rv;
+      });
+    }
+  }
+  if constexpr (MODE == 1) {
+    a1 = wave_sum(a1);
+    if (lane == 0) partials[w] = a1;
+  }
+}
+
+// Givens step col of one problem (k_krylov.hip k_givens: the reference's rotation convention,
+// krylov.cpp:262-295).  A breakdown ends the problem (no restart pending).
+__device__ void kg_givens(const BGVec& v, KScalars* sc, int col, int mm) {
+  double* H = v.H;
+  double* cs = v.cs;
+  double* sn = v.sn;
+  double* g = v.g;
+  for (int i = 0; i < col; ++i) {
+    const double v1 = H[i * mm + col], v2 = H[(i + 1) * mm + col];
+    H[i * mm + col] = cs[i] * v1 - sn[i] * v2;
+    H[(i + 1) * mm + col] = sn[i] * v1 + cs[i] * v2;
+  }
+  const double h1 = H[(col + 1) * mm + col], h0 = H[col * mm + col];
+  const double denom = sqrt(h0 * h0 + h1 * h1);
+  if (denom < 1e-30) { sc->breakdown = 1; sc->cycle_done = 0; return; }
+  cs[col] = h0 / denom;
+  sn[col] = -h1 / denom;
+  H[col * mm + col] = cs[col] * h0 - sn[col] * h1;
+  H[(col + 1) * mm + col] = 0.0;
+  const double g0 = g[col], g1 = g[col + 1];
+  g[col] = cs[col] * g0 - sn[col] * g1;
+  g[col + 1] = sn[col] * g0 + cs[col] * g1;
+  sc->steps = col + 1;
+  sc->iter += 1;
+  sc->metric = fabs(g[col + 1]);
+  if (fabs(g[col + 1]) < sc->target) sc->cycle_done = 1;
+}
+
+// Segmented finalize, one 256-thread block per problem.  The coefficient sums (GF_H1/GF_H2)
+// take one wavefront per basis vector; every sum runs over the problem's items in a fixed order.
+__global__ void __launch_bounds__(256)
+kg_finalize(const BGVec* __restrict__ vecs, const int* __restrict__ first,
+            const int* __restrict__ cnt, int nitems, int nprob, const double* __restrict__ partials,
+            int op, int col, int mm, double tol, int tol_mode) {
+  __shared__ double lds[256 / kWave];
+  const int q = blockIdx.x;
+  if (q >= nprob) return;
+  const BGVec& v = vecs[q];
+  KScalars* sc = v.sc;
+  if (sc->done | sc->breakdown) return;
+  if ((op == GF_H1 || op == GF_H2 || op == GF_NORM) && sc->cycle_done) return;
+  const int f = first[q], c = cnt[q];
+  if (op == GF_H1 || op == GF_H2) {
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wv = static_cast<int>(threadIdx.x / kLanes);
+    for (int i = wv; i <= col; i += static_cast<int>(blockDim.x / kLanes)) {
+      double a = 0.0;
+      for (int t = lane; t < c; t += kLanes) a += partials[(long long)i * nitems + f + t];
+      a = wave_sum(a);
+      if (lane == 0) {
+        if (op == GF_H1) {
+          v.hc[i] = a;
+        } else {
+          v.hc[mm + 1 + i] = a;
+          v.H[(long long)i * mm + col] = v.hc[i] + a;
+        }
+      }
+    }
+    return;
+  }
+  double a = 0.0;
+  for (int t = threadIdx.x; t < c; t += blockDim.x) a += partials[f + t];
+  const double s1 = block_sum(a, lds);
+  if (threadIdx.x != 0) return;
+  switch (op) {
+    case GF_NORM:
+      v.H[(long long)(col + 1) * mm + col] = sqrt(s1);
+      v.hc[2 * mm + 1] = sqrt(s1);   // for the normalisation, which runs after the rotation
+      kg_givens(v, sc, col, mm);
+      break;
+    case GF_R0: {
+      const double r0 = sqrt(s1);
+      if (nonfinite(s1)) { sc->breakdown = 1; break; }
+      sc->r0 = r0;
+      sc->metric = r0;
+      sc->target = tol_mode == PDEHIP_TOL_REL ? tol * r0 : tol;
+      if (r0 < sc->target || r0 == 0.0) sc->done = 1;
+      break;
+    }
+    case GF_R:
+      sc->metric = sqrt(s1);
+      if (nonfinite(s1)) { sc->breakdown = 1; sc->cycle_done = 0; break; }
+      if (sc->metric < sc->target) sc->done = 1;
+      break;
+    default:
+      break;
+  }
+}
+
+// Per-problem cycle bookkeeping, one 64-thread block per problem.
+__global__ void kg_cycle(const BGVec* __restrict__ vecs, int nprob, int op, int mm) {
+  const int q = blockIdx.x;
+  if (q >= nprob) return;
+  const BGVec& v = vecs[q];
+  KScalars* sc = v.sc;
+  if (sc->done | sc->breakdown) return;
+  if (op == GC_START) {
+    const double beta = sc->metric;
+    for (int t = threadIdx.x; t < (mm + 1) * mm; t += blockDim.x) v.H[t] = 0.0;
+    for (int t = threadIdx.x; t <= mm; t += blockDim.x) v.g[t] = t == 0 ? beta : 0.0;
+    if (threadIdx.x == 0) { sc->cycle_done = 0; sc->steps = 0; }
+    return;
+  }
+  if (threadIdx.x != 0) return;
+  // back substitution over the cycle's steps (k_krylov.hip k_backsub)
+  const double* H = v.H;
+  const double* g = v.g;
+  double* y = v.y;
+  const int dim = sc->steps;
+  for (int i = dim - 1; i >= 0; --i) {
+    double sum = g[i];
+    for (int t = i + 1; t < dim; ++t) sum -= H[i * mm + t] * y[t];
+    if (fabs(H[i * mm + i]) < 1e-30) { sc->breakdown = 1; sc->cycle_done = 0; return; }
+    y[i] = sum / H[i * mm + i];
+  }
+  for (int i = dim; i < mm; ++i) y[i] = 0.0;
+}
+
 // ------------------------------------------------------------------------------- finish
 // Boundary closures of problem blockIdx.x's FP64 solution, one face pair per launch (phase).
 __global__ void __launch_bounds__(256)
@@ -1346,6 +1612,42 @@ void kfinalize(hipStream_t s, const BKVec* vecs, const int* first, const int* cn
                const double* p1, const double* p2, int op, double tol, int tol_mode) {
   hipLaunchKernelGGL(kk_finalize, dim3(nprob), dim3(256), 0, s, vecs, first, cnt, nprob, p1, p2,
                      op, tol, tol_mode);
+}
+
+void gvec(hipStream_t s, int op, const BGVec* vecs, const BItem* items, int n, int col, int mm,
+          double* partials, double* partials2) {
+  if (n <= 0) return;
+  const dim3 b(kLanes * kWavesPB);
+  const unsigned gx = waves_grid(n);
+#define KG_(OP, GY) hipLaunchKernelGGL((kg_vec<OP>), dim3(gx, GY), b, 0, s, vecs, items, n, col, mm, partials, partials2)
+  switch (op) {
+    case KG_DOT: KG_(KG_DOT, static_cast<unsigned>((col + 1 + 7) / 8)); break;
+    case KG_AXPY1: KG_(KG_AXPY1, 1u); break;
+    case KG_AXPY2: KG_(KG_AXPY2, 1u); break;
+    case KG_NORMALIZE: KG_(KG_NORMALIZE, 1u); break;
+    case KG_START: KG_(KG_START, 1u); break;
+    default: KG_(KG_UPDATE, 1u); break;
+  }
+#undef KG_
+}
+
+void gapply(hipStream_t s, int mode, const BGVec* vecs, const BItem* items, int n, int col,
+            double* partials) {
+  if (n <= 0) return;
+  const dim3 g(waves_grid(n)), b(kLanes * kWavesPB);
+  if (mode == 0) hipLaunchKernelGGL((kg_apply<0>), g, b, 0, s, vecs, items, n, col, partials);
+  else hipLaunchKernelGGL((kg_apply<1>), g, b, 0, s, vecs, items, n, col, partials);
+}
+
+void gfinalize(hipStream_t s, const BGVec* vecs, const int* first, const int* cnt, int nitems,
+               int nprob, const double* partials, int op, int col, int mm, double tol,
+               int tol_mode) {
+  hipLaunchKernelGGL(kg_finalize, dim3(nprob), dim3(256), 0, s, vecs, first, cnt, nitems, nprob,
+                     partials, op, col, mm, tol, tol_mode);
+}
+
+void gcycle(hipStream_t s, const BGVec* vecs, int nprob, int op, int mm) {
+  hipLaunchKernelGGL(kg_cycle, dim3(nprob), dim3(64), 0, s, vecs, nprob, op, mm);
 }
 
 }  // namespace bk
