@@ -179,6 +179,8 @@ def bench_nonsym(args) -> int:
         results[f"{mname}_mixed"]["max_diff_vs_fp64"] = max(errs)
         if method == capi.BICGSTAB:
             seq64, seq64_ok = sols["fp64"], oks["fp64"]
+        else:
+            gm64, gm64_ok = sols["fp64"], oks["fp64"]
 
     # batched BiCGStab: the same problems solved together in lock step by a resident batch
     # (one launch per operation over every unfinished problem, 8 iterations per graph replay),
@@ -204,6 +206,27 @@ def bench_nonsym(args) -> int:
             bt.close()
             log(f"bicgstab batched {label} {pname}: {results[key]['solves_per_s']} solves/s "
                 f"({results[key]['batch_ms']} ms per batch), {results[key]['converged']}/{len(sp)} converged")
+
+    # batched GMRES(30), FP64: per-problem Arnoldi bases and Hessenberg systems in lock step, one
+    # graph per restart cycle (mixed-precision GMRES is not batched)
+    for label, sp in (("sample", specs), ("all", all_specs)):
+        bt = ctx.batch(sp, kopts(capi.GMRES, capi.FP64), capi.BatchConfig(use_graph=1))
+        bt.solve(check=False)
+        t0 = time.perf_counter()
+        st, reps = bt.solve(check=False)
+        wall = time.perf_counter() - t0
+        key = f"gmres_batched_{label}_fp64"
+        results[key] = {"solves_per_s": round(len(sp) / wall, 2), "batch_ms": round(wall * 1e3, 1),
+                        "converged": int(sum(r.status == 0 for r in reps)), "problems": len(sp),
+                        "iterations_max": int(max(r.iterations for r in reps)),
+                        "iterations_total": int(sum(r.iterations for r in reps))}
+        if label == "sample":
+            diffs = [float(np.linalg.norm(bt.download(i) - gm64[i]) / max(np.linalg.norm(gm64[i]), 1e-300))
+                     for i in range(len(sp)) if reps[i].status == 0 and gm64_ok[i]]
+            results[key]["max_diff_vs_sequential_fp64"] = max(diffs) if diffs else None
+        bt.close()
+        log(f"gmres batched {label} fp64: {results[key]['solves_per_s']} solves/s "
+            f"({results[key]['batch_ms']} ms per batch), {results[key]['converged']}/{len(sp)} converged")
 
     # reaction-diffusion, 2049^2, 1000 RK4 steps
     n = 2049
@@ -273,6 +296,7 @@ def bench_nonsym(args) -> int:
         "fp64_solves_per_s": round(fp64_total, 2),
         "bicgstab_batched_all_mixed_solves_per_s": results["bicgstab_batched_all_mixed"]["solves_per_s"],
         "bicgstab_batched_all_fp64_solves_per_s": results["bicgstab_batched_all_fp64"]["solves_per_s"],
+        "gmres_batched_all_fp64_solves_per_s": results["gmres_batched_all_fp64"]["solves_per_s"],
         "reaction_diffusion_2049": rdres,
     }
     print(json.dumps(out), flush=True)

@@ -69,6 +69,7 @@ struct BatchPlan {
   std::vector<List> sweep, rows, crows, drows;   // per level
   List el0, rows0;                         // level-0 element / strip items (with ranges)
   List rb0;                                // BiCGStab: level-0 row-block items (with ranges)
+  List gel;                                // GMRES: level-0 element items (with ranges)
   DBuf kvecs;                              // BiCGStab: BKVec [n] (the batch's precision)
   DBuf kvecs64;                            // mixed BiCGStab: FP64 views for the completion pass
   std::vector<LevelShape> shape;           // per level work-item shapes
@@ -485,8 +486,21 @@ BatchPtr create_krylov(BatchPtr B, Ctx* ctx, int count, const pdehip_problem* pr
     }
     upload_list(ctx, s, B->rb0, rb, &f, &c);
   }
-  // GMRES: the CGS partials hold (mm + 1) sums per row-block item
-  const int np = std::max({B->el0.n, B->rows0.n, B->rb0.n * (B->gmres ? B->mm + 1 : 1)});
+  if (B->gmres) {
+    // element items of the GMRES vector kernels (bk::kGmresItemNodes nodes at most)
+    std::vector<BItem> el;
+    std::vector<int> f(count), c(count);
+    for (int p = 0; p < count; ++p) {
+      const long long nn = B->plans[p]->g.n;
+      f[p] = static_cast<int>(el.size());
+      for (long long e0 = 0; e0 < nn; e0 += bk::kGmresItemNodes)
+        el.push_back(BItem{p, static_cast<int>(e0), static_cast<int>(std::min(nn, e0 + bk::kGmresItemNodes)), 0});
+      c[p] = static_cast<int>(el.size()) - f[p];
+    }
+    upload_list(ctx, s, B->gel, el, &f, &c);
+  }
+  // GMRES: the CGS partials hold (mm + 1) sums per element item
+  const int np = std::max({B->el0.n, B->rows0.n, B->rb0.n, B->gel.n * (B->gmres ? B->mm + 1 : 1)});
   B->part.alloc(ctx, (size_t)np * sizeof(double));
   B->maxp.alloc(ctx, (size_t)np * sizeof(double));
   setup_finish(*B);
@@ -829,7 +843,7 @@ void gmres(BatchPlan& B, int max_iter) {
   const BGVec* G = B.G();
   double* p1 = B.part.as<double>();
   double* p2 = B.maxp.as<double>();
-  const List& rb = B.rb0;
+  const List& rb = B.gel;
   const List& rw = B.rows0;
   const int* rbf = rb.first.as<int>();
   const int* rbc = rb.cnt.as<int>();

@@ -209,13 +209,15 @@ void kreset_failed(hipStream_t s, const BKVec* vecs, int nprob, double eps);
 void kfailed(hipStream_t s, const BKVec* vecs, int nprob, int* out);
 void kfinalize(hipStream_t s, const BKVec* vecs, const int* first, const int* cnt, int nprob,
                const double* p1, const double* p2, int op, double tol, int tol_mode);
-// ---- batched GMRES(m) (k_batch.hip).  Row-block items for the vector kernels, level-0 strips
+// ---- batched GMRES(m) (k_batch.hip).  Element items for the vector kernels, level-0 strips
 // for the applies; `col` is the Arnoldi step of the current cycle, the same for every problem.
 // KG_DOT: partials[i * n + item] = V_i . w for i <= col (launched over groups of 8 vectors);
 // KG_AXPY1 / KG_AXPY2: w -= sum_i hc_pass[i] V_i (AXPY2 also writes the partials w . w to
 // partials2); KG_NORMALIZE: V_{col+1} = w / |w| (|w| from hc[2 m + 1]); KG_START: V_0 = r / |r|;
 // KG_UPDATE: x += M^-1 sum_{i < steps} y_i V_i.
 enum KgVec { KG_DOT = 0, KG_AXPY1 = 1, KG_AXPY2 = 2, KG_NORMALIZE = 3, KG_START = 4, KG_UPDATE = 5 };
+// The vector kernels' items are element ranges of at most this many nodes.
+constexpr int kGmresItemNodes = 512;
 void gvec(hipStream_t s, int op, const BGVec* vecs, const BItem* items, int n, int col, int mm,
           double* partials, double* partials2);
 // mode 0: w = A M^-1 V_col; mode 1: r = b - A x, partials |r|^2 per item.

@@ -1061,10 +1061,16 @@ __device__ __forceinline__ double kg_pc(const BGVec& v, int i, int j, double x) 
   return x / kk_diag(v.op, i, j);
 }
 
+constexpr int kGmPer = 8;                  // nodes per lane in a GMRES vector item
+static_assert(kGmPer * kLanes == kGmresItemNodes, "GMRES vector items: 8 nodes per lane");
+
 __device__ __forceinline__ bool kg_out_of_cycle(const KScalars* sc) {
   return done(sc) || ldS(&sc->cycle_done, 0) != 0;
 }
 
+// Vector kernels over element items (p, [a, b)) of at most kGmresItemNodes: lane L holds nodes
+// a + 64 k + L (k < kGmPer) in registers, so each basis vector's loads for all of them are in
+// flight together (a per-node loop over the basis would wait on one load pair at a time).
 template <int OP>
 __global__ void __launch_bounds__(kLanes * kWavesPB)
 kg_vec(const BGVec* __restrict__ vecs, const BItem* __restrict__ items, int n, int col, int mm,
@@ -1076,20 +1082,29 @@ kg_vec(const BGVec* __restrict__ vecs, const BItem* __restrict__ items, int n, i
   const int lane = threadIdx.x & (kLanes - 1);
   const KScalars* sc = v.sc;
   const long long nn = v.n;
-  const int nx = v.nx;
+  long long ids[kGmPer];
+  bool in[kGmPer];
+#pragma unroll
+  for (int k = 0; k < kGmPer; ++k) {
+    ids[k] = (long long)it.a + k * kLanes + lane;
+    in[k] = ids[k] < it.b;
+    if (!in[k]) ids[k] = it.a;   // loads stay in the item; results are not stored
+  }
   if constexpr (OP == KG_DOT) {
     // V_i . w for the group i = 8 blockIdx.y .. +7 (i <= col)
     const int i0 = static_cast<int>(blockIdx.y) * 8;
     const int cnt = min(8, col + 1 - i0);
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (!kg_out_of_cycle(sc)) {
-      for (int j = it.a; j < it.b; ++j) {
-        for (int i = lane; i < nx; i += kLanes) {
-          const long long id = (long long)j * nx + i;
-          const double wv = ldS(v.w, id);
+      double wv[kGmPer];
 #pragma unroll
-          for (int t = 0; t < 8; ++t)
-            if (t < cnt) acc[t] += ldS(v.V, (long long)(i0 + t) * nn + id) * wv;
+      for (int k = 0; k < kGmPer; ++k) wv[k] = in[k] ? ldS(v.w, ids[k]) : 0.0;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        if (t < cnt) {
+          const double* Vi = v.V + (long long)(i0 + t) * nn;
+#pragma unroll
+          for (int k = 0; k < kGmPer; ++k) acc[t] += ldS(Vi, ids[k]) * wv[k];
         }
       }
     }
@@ -1104,33 +1119,58 @@ kg_vec(const BGVec* __restrict__ vecs, const BItem* __restrict__ items, int n, i
     const bool run = in_cycle ? !done(sc) : !kg_out_of_cycle(sc);
     double ww = 0.0;
     if (run) {
-      const double* hc = v.hc + (OP == KG_AXPY2 ? mm + 1 : 0);
-      // |w| of this step, kept at hc[2 mm + 1] (the Givens step has already zeroed H[col+1][col])
-      const double hn = OP == KG_NORMALIZE ? ldS(v.hc, 2 * mm + 1) : 1.0;
-      const double beta = OP == KG_START ? ldS(&sc->metric, 0) : 1.0;
-      const int steps = OP == KG_UPDATE ? ldS(&sc->steps, 0) : 0;
-      const double sign = -1.0;
-      if (OP != KG_NORMALIZE || hn > 0.0) {
-        for (int j = it.a; j < it.b; ++j) {
-          for (int i = lane; i < nx; i += kLanes) {
-            const long long id = (long long)j * nx + i;
-            if constexpr (OP == KG_AXPY1 || OP == KG_AXPY2) {
-              double acc = 0.0;
-              for (int q = 0; q <= col; ++q) acc += ldS(hc, q) * ldS(v.V, (long long)q * nn + id);
-              const double wn = ldS(v.w, id) + sign * acc;
-              stS(v.w, id, wn);
-              if (OP == KG_AXPY2) ww += wn * wn;
-            } else if constexpr (OP == KG_NORMALIZE) {
-              stS(v.V, (long long)(col + 1) * nn + id, ldS(v.w, id) / hn);
-            } else if constexpr (OP == KG_START) {
-              stS(v.V, id, ldS(v.r, id) / beta);
-            } else {   // KG_UPDATE: z = 0 + sum y_q V_q, x += M^-1 z
-              double acc = 0.0;
-              for (int q = 0; q < steps; ++q) acc += ldS(v.y, q) * ldS(v.V, (long long)q * nn + id);
-              const double z = 0.0 + 1.0 * acc;
-              stS(v.x, id, ldS(v.x, id) + 1.0 * kg_pc(v, i, j, z));
-            }
+      if constexpr (OP == KG_AXPY1 || OP == KG_AXPY2 || OP == KG_UPDATE) {
+        // z = sum_q c_q V_q over the nodes in registers, the basis vector by vector
+        const double* c = OP == KG_UPDATE ? v.y : v.hc + (OP == KG_AXPY2 ? mm + 1 : 0);
+        const int cnt = OP == KG_UPDATE ? ldS(&sc->steps, 0) : col + 1;
+        double acc[kGmPer];
+#pragma unroll
+        for (int k = 0; k < kGmPer; ++k) acc[k] = 0.0;
+        // four basis vectors' loads in flight at once; each node still sums q in order
+        int q = 0;
+        for (; q + 4 <= cnt; q += 4) {
+          double c4[4], x4[4][kGmPer];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            c4[u] = ldS(c, q + u);
+            const double* Vq = v.V + (long long)(q + u) * nn;
+#pragma unroll
+            for (int k = 0; k < kGmPer; ++k) x4[u][k] = ldS(Vq, ids[k]);
           }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < kGmPer; ++k) acc[k] += c4[u] * x4[u][k];
+        }
+        for (; q < cnt; ++q) {
+          const double cq = ldS(c, q);
+          const double* Vq = v.V + (long long)q * nn;
+#pragma unroll
+          for (int k = 0; k < kGmPer; ++k) acc[k] += cq * ldS(Vq, ids[k]);
+        }
+        const double sign = -1.0;
+#pragma unroll
+        for (int k = 0; k < kGmPer; ++k) {
+          if (!in[k]) continue;
+          if constexpr (OP == KG_UPDATE) {   // x += M^-1 (0 + z)
+            const int j = static_cast<int>(ids[k] / v.nx), i = static_cast<int>(ids[k] - (long long)j * v.nx);
+            const double z = 0.0 + 1.0 * acc[k];
+            stS(v.x, ids[k], ldS(v.x, ids[k]) + 1.0 * kg_pc(v, i, j, z));
+          } else {                          // w -= z
+            const double wn = ldS(v.w, ids[k]) + sign * acc[k];
+            stS(v.w, ids[k], wn);
+            if (OP == KG_AXPY2) ww += wn * wn;
+          }
+        }
+      } else {
+        // |w| of this step, kept at hc[2 mm + 1] (the Givens step has already zeroed H[col+1][col])
+        const double hn = OP == KG_NORMALIZE ? ldS(v.hc, 2 * mm + 1) : ldS(&sc->metric, 0);
+        if (OP != KG_NORMALIZE || hn > 0.0) {
+          const double* src = OP == KG_NORMALIZE ? v.w : v.r;
+          double* dst = v.V + (OP == KG_NORMALIZE ? (long long)(col + 1) * nn : 0);
+#pragma unroll
+          for (int k = 0; k < kGmPer; ++k)
+            if (in[k]) stS(dst, ids[k], ldS(src, ids[k]) / hn);
         }
       }
     }
@@ -1222,13 +1262,15 @@ __device__ void kg_givens(const BGVec& v, KScalars* sc, int col, int mm) {
   if (fabs(g[col + 1]) < sc->target) sc->cycle_done = 1;
 }
 
-// Segmented finalize, one 256-thread block per problem.  The coefficient sums (GF_H1/GF_H2)
-// take one wavefront per basis vector; every sum runs over the problem's items in a fixed order.
-__global__ void __launch_bounds__(256)
+// Segmented finalize, one 512-thread block per problem.  The coefficient sums (GF_H1/GF_H2)
+// take one wavefront per basis vector, with eight partial loads in flight per lane; every sum
+// runs over the problem's items in a fixed order.
+constexpr int kGfThreads = 512;
+__global__ void __launch_bounds__(kGfThreads)
 kg_finalize(const BGVec* __restrict__ vecs, const int* __restrict__ first,
             const int* __restrict__ cnt, int nitems, int nprob, const double* __restrict__ partials,
             int op, int col, int mm, double tol, int tol_mode) {
-  __shared__ double lds[256 / kWave];
+  __shared__ double lds[kGfThreads / kWave];
   const int q = blockIdx.x;
   if (q >= nprob) return;
   const BGVec& v = vecs[q];
@@ -1241,7 +1283,9 @@ kg_finalize(const BGVec* __restrict__ vecs, const int* __restrict__ first,
     const int wv = static_cast<int>(threadIdx.x / kLanes);
     for (int i = wv; i <= col; i += static_cast<int>(blockDim.x / kLanes)) {
       double a = 0.0;
-      for (int t = lane; t < c; t += kLanes) a += partials[(long long)i * nitems + f + t];
+      const double* pi = partials + (long long)i * nitems + f;
+#pragma unroll 8
+      for (int t = lane; t < c; t += kLanes) a += pi[t];
       a = wave_sum(a);
       if (lane == 0) {
         if (op == GF_H1) {
@@ -1642,8 +1686,8 @@ void gapply(hipStream_t s, int mode, const BGVec* vecs, const BItem* items, int 
 void gfinalize(hipStream_t s, const BGVec* vecs, const int* first, const int* cnt, int nitems,
                int nprob, const double* partials, int op, int col, int mm, double tol,
                int tol_mode) {
-  hipLaunchKernelGGL(kg_finalize, dim3(nprob), dim3(256), 0, s, vecs, first, cnt, nitems, nprob,
-                     partials, op, col, mm, tol, tol_mode);
+  hipLaunchKernelGGL(kg_finalize, dim3(nprob), dim3(kGfThreads), 0, s, vecs, first, cnt, nitems,
+                     nprob, partials, op, col, mm, tol, tol_mode);
 }
 
 void gcycle(hipStream_t s, const BGVec* vecs, int nprob, int op, int mm) {
