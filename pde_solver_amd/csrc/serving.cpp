@@ -148,9 +148,9 @@ std::string shape_key(const pdehip_problem& p, const pdehip_options& o) {
          std::to_string(o.gmres_restart) + ":" + std::to_string(o.mg_max_levels);
 }
 
-// The batch lanes take 2D CONST problems the lock-step engine solves: MG-PCG (any precision) or
-// FP64 BiCGStab.  (Mixed BiCGStab batches add an FP64 completion pass that restarts unconverged
-// members; those requests run on plan workers.)
+// The batch lanes take 2D CONST problems the lock-step engine solves: MG-PCG (any precision),
+// FP64 BiCGStab or FP64 GMRES.  (Mixed BiCGStab batches add an FP64 completion pass that
+// restarts unconverged members; those requests run on plan workers.)
 bool lane_eligible(const pdehip_problem& p, const pdehip_options& o) {
   if (o.method == PDEHIP_BICGSTAB && o.precision == PDEHIP_MIXED) return false;
   return batch_eligible(1, &p, o, nullptr);

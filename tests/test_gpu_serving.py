@@ -110,9 +110,10 @@ def test_server_mix_matches_oracle_and_alone(ctx):
     stats = srv.stats()
     srv.close()
     assert stats["completed"] == len(reqs) and stats["rejected"] == 0
-    # the 2D CONST MG-PCG and FP64 BiCGStab requests ran in batch lanes, the rest on plan workers
+    # the 2D CONST MG-PCG, FP64 BiCGStab and FP64 GMRES requests ran in batch lanes, the rest on
+    # plan workers
     n_lane = sum(1 for m in MIX if m[1] == 2 and m[0] != "varcoef" and
-                 (m[3] == capi.MGPCG or (m[3] == capi.BICGSTAB and m[4] == capi.FP64)))
+                 (m[3] == capi.MGPCG or (m[3] in (capi.BICGSTAB, capi.GMRES) and m[4] == capi.FP64)))
     print(f"  batched {stats['batched']} of {len(reqs)} in {stats['batches']} lane batches", flush=True)
     assert stats["batched"] == n_lane > 0
     for i, ((s, o), (st, u, rep)) in enumerate(zip(reqs, mixed)):
