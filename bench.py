@@ -899,6 +899,8 @@ def main() -> int:
                     help="skip the two rocprofv3 counter passes that measure roofline.traffic")
     ap.add_argument("--no-extras", action="store_true",
                     help="default workload: only the 257^3 line (no 129^2 or batched rows)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="launch-shape knob for A/B timing (pdehip_tune_set), repeatable")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
@@ -932,6 +934,10 @@ def main() -> int:
     log(f"device: {note}")
     if not ok:
         raise SystemExit("no usable gfx9 device")
+    for kv in args.tune:
+        key, val = kv.split("=")
+        hip.tune_set(key, int(val))
+        log(f"tuning {key} = {val}")
     n = args.n
     t = time.perf_counter()
     spec = make_problem(n)

@@ -248,7 +248,12 @@ void pdehip_plan_destroy(pdehip_plan* plan);
  * kernels), "smooth.pair" (0/1: 3D FP32/FP16 sweeps with column-pair tiles, no colour
  * divergence), "smooth.ring" (0/1, default 1: 3D FP32/FP16 sweeps by the ring-buffered,
  * 6-step unrolled kernel), "stencil.ring" (0/1, default 1: the same for the 3D FP32/FP16
- * apply and residual).  Results do not depend on any of these.
+ * apply and residual), "smooth.box" (0..1025, default 65: coarse 3D levels of mixed-precision
+ * V-cycles whose sides are all at most this smooth on LDS boxes, the prolongation folded into
+ * the first post-smoothing launch; 0: off), "smooth.box2" (0..1025, default 65: of those, the
+ * levels whose sides are all at most this take two sweeps per launch, the others one),
+ * "smooth.box_z" (4 or 8, default 8: interior planes per box).  Results do not depend on any of
+ * these.
  * "time.fused" (0/1, default 1): RK4 time series take one launch per step (the four stages and
  * the closures, temporally blocked in LDS) instead of one per stage; the same bits either way.
  * "mixed.fp16" (0/1, default 1): mixed-precision plans created afterwards store their coarse

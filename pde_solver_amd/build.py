@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpdehip.so")
 BUILD = os.path.join(ROOT, "_build")
 
-SOURCES = ["k_stencil.hip", "k_smooth.hip", "k_plane.hip", "k_aux.hip", "k_time.hip", "k_krylov.hip", "k_batch.hip",
+SOURCES = ["k_stencil.hip", "k_smooth.hip", "k_box.hip", "k_plane.hip", "k_aux.hip", "k_time.hip", "k_krylov.hip", "k_batch.hip",
            "noalloc.hip",
            "runtime.cpp", "serving.cpp", "batch.cpp", "capi.cpp"]
 

@@ -1,6 +1,7 @@
 // k_aux.hip — boundary closures, multigrid transfers, coarsest-level solve, vector kernels and
 // the deterministic two-stage reductions whose second stage updates device-resident scalars.
 #include "faces_dev.hpp"
+#include "transfer_dev.hpp"
 
 namespace pdehip {
 namespace {
@@ -326,32 +327,19 @@ __global__ void __launch_bounds__(kTX * kTY) k_prolong3(DevOp c, const void* ucv
   const long long csxy = (long long)c.nx * c.ny;
   const long long fsxy = (long long)f.nx * f.ny;
   const long long col = (long long)j * f.nx + i;
-  const int ic = i >> 1, oi = i & 1, jc = j >> 1, oj = j & 1;
-  int x0 = ic, x1 = ic + 1, y0 = jc, y1 = jc + 1;
-  double wx0 = 1.0, wx1 = 1.0, wy0 = 1.0, wy1 = 1.0;
-  if (x0 == 0) { wx0 = c.theta[0]; x0 = 1; }
-  if (x1 == c.nx - 1) { wx1 = c.theta[1]; x1 = c.nx - 2; }
-  if (y0 == 0) { wy0 = c.theta[2]; y0 = 1; }
-  if (y1 == c.ny - 1) { wy1 = c.theta[3]; y1 = c.ny - 2; }
-  const double hx = oi ? 0.5 : 1.0, hy = oj ? 0.5 : 1.0;
-  if (!oi) wx1 = 0.0;
-  if (!oj) wy1 = 0.0;
-  x1 = min(x1, c.nx - 2);   // in range even where its weight is zero
-  y1 = min(y1, c.ny - 2);
+  const ProlongCol w = prolong_col(c, i, j);
   // loads: coarse corners of planes kc0 .. kc0+kProlongZ (through the z closure extension),
   // fine values of planes 2kc0 .. 2kc0+2kProlongZ-1
   double l00[kProlongZ + 1], l01[kProlongZ + 1], l10[kProlongZ + 1], l11[kProlongZ + 1];
   double wz[kProlongZ + 1];
 #pragma unroll
   for (int q = 0; q <= kProlongZ; ++q) {
-    int kk = min(kc0 + q, c.nz - 1);
-    wz[q] = 1.0;
-    if (kk == 0) { wz[q] = c.theta[4]; kk = 1; } else if (kk == c.nz - 1) { wz[q] = c.theta[5]; kk = c.nz - 2; }
+    const int kk = prolong_zplane(c, min(kc0 + q, c.nz - 1), &wz[q]);
     const SC* pl = uc + kk * csxy;
-    l00[q] = ldd(pl + (long long)y0 * c.nx, x0);
-    l01[q] = ldd(pl + (long long)y0 * c.nx, x1);
-    l10[q] = ldd(pl + (long long)y1 * c.nx, x0);
-    l11[q] = ldd(pl + (long long)y1 * c.nx, x1);
+    l00[q] = ldd(pl + (long long)w.y0 * c.nx, w.x0);
+    l01[q] = ldd(pl + (long long)w.y0 * c.nx, w.x1);
+    l10[q] = ldd(pl + (long long)w.y1 * c.nx, w.x0);
+    l11[q] = ldd(pl + (long long)w.y1 * c.nx, w.x1);
   }
   double fv[2 * kProlongZ];
 #pragma unroll
@@ -359,35 +347,19 @@ __global__ void __launch_bounds__(kTX * kTY) k_prolong3(DevOp c, const void* ucv
     const int k = min(max(2 * kc0 + q, 1), f.nz - 2);
     fv[q] = ldd(uf, k * fsxy + col);
   }
-  // bilinear value of coarse plane kc0 + q
-  auto P = [&](int q) -> double {
-    double a = 0.0;
-    if (wy0 != 0.0) {
-      double v = wx0 != 0.0 ? wx0 * l00[q] : 0.0;
-      if (wx1 != 0.0) v += wx1 * l01[q];
-      a += wy0 * (v * hx);
-    }
-    if (wy1 != 0.0) {
-      double v = wx0 != 0.0 ? wx0 * l10[q] : 0.0;
-      if (wx1 != 0.0) v += wx1 * l11[q];
-      a += wy1 * (v * hx);
-    }
-    return wz[q] == 0.0 ? 0.0 : wz[q] * (a * hy);
-  };
   const double s = dscale ? factor * (*dscale) : factor;
-  double p_lo = P(0);
+  double p_lo = prolong_plane(w, l00[0], l01[0], l10[0], l11[0], wz[0]);
 #pragma unroll
   for (int q = 0; q < kProlongZ; ++q) {
-    const double p_hi = P(q + 1);
+    const double p_hi = prolong_plane(w, l00[q + 1], l01[q + 1], l10[q + 1], l11[q + 1], wz[q + 1]);
     // fine plane 2kc + o sits at coarse kc (o = 0) or halfway to kc + 1 (o = 1)
     const int kc = kc0 + q;
     const int ka = max(1, 2 * kc), kb = min(2 * kc + 2, f.nz - 1);   // [ka, kb)
     if (ka < kb) {
       const int o0 = ka & 1;      // 1 only for kc = 0 (ka = 1)
-      const double v0 = o0 ? 0.5 * (p_lo + p_hi) : p_lo;
       const double fa = o0 ? fv[2 * q + 1] : fv[2 * q];   // fine plane ka = 2kc + o0
-      uf[ka * fsxy + col] = store_as<SF>(fa + s * v0);
-      if (kb - ka == 2) uf[(ka + 1) * fsxy + col] = store_as<SF>(fv[2 * q + 1] + s * (0.5 * (p_lo + p_hi)));
+      uf[ka * fsxy + col] = store_as<SF>(prolong_fine(fa, s, o0 != 0, p_lo, p_hi));
+      if (kb - ka == 2) uf[(ka + 1) * fsxy + col] = store_as<SF>(prolong_fine(fv[2 * q + 1], s, true, p_lo, p_hi));
     }
     p_lo = p_hi;
   }

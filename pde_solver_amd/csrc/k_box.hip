@@ -1,0 +1,286 @@
+// k_box.hip — multigrid smoothing of the coarse 3D levels on boxes for gfx950: one or two
+// red-black Gauss-Seidel sweeps per launch, temporally blocked in LDS.
+//
+// The plane-march sweep (k_smooth.hip) is built for large grids: a workgroup walks a z chunk one
+// plane at a time with one barrier per plane, so a launch costs its prologue and a few plane
+// steps of latency whatever the grid's size.  On the coarse levels of a V-cycle (129^3 and
+// below) the level sits in L2 / Infinity Cache and that latency is the whole cost.  Here a
+// workgroup owns a kBX x kBY x BZ box of interior nodes and loads the box plus a ring of
+// H = 2 * sweeps nodes into LDS at once (every load of the box in flight together), then runs
+// the colour stages of its sweeps in LDS with one barrier per stage:
+//   stage s (0 .. 2*sweeps-1) updates colour c1 (s even) or the other colour (s odd) on the box
+//   plus a ring of 2*sweeps-1-s nodes; the neighbours it reads sit one node further out and
+//   were left by stage s-1, so the box's own nodes come out as sweeps of the whole grid leave
+//   them, and no workgroup waits for another.
+// Input modes: x_in from memory; x_in known to be zero (no x loads); or x_in plus the
+// prolongation of the coarse-level correction, formed on load from a coarse box staged in LDS,
+// so the V-cycle's prolongation and the post-smoothing sweeps after it take one launch.
+// Every update uses the plane-march sweep's per-node arithmetic (weights_at, fold_center,
+// gs_value, face_smooth; values rounded to the storage type after each update) and the
+// prolongation uses the prolongation kernel's (transfer_dev.hpp), so x_out equals bit for bit
+// what the prolongation launch and the plane-march sweeps leave.
+#include <type_traits>
+
+#include "kernels.hpp"
+#include "smooth_dev.hpp"
+#include "transfer_dev.hpp"
+
+namespace pdehip {
+namespace {
+
+constexpr int kBoxThreads = 512;
+constexpr int kBX = 16, kBY = 8;   // interior nodes of a box along x and y (z: BZ)
+
+template <int N> using IC = std::integral_constant<int, N>;
+
+template <typename S> __device__ __forceinline__ float back(S v);
+template <> __device__ __forceinline__ float back<float>(float v) { return v; }
+template <> __device__ __forceinline__ float back<__half>(__half v) { return __half2float(v); }
+
+// IN: 0 x_in from memory, 1 x_in is zero, 2 x_in plus factor * prolong(uc) on interior nodes.
+template <int KIND, Prec P, int NS, int BZ, int IN>
+__global__ void __launch_bounds__(kBoxThreads)
+k_box_rbgs(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v,
+           void* __restrict__ xout_v, int c1, int nbx, int nby, int nboxes, DevOp cop,
+           const void* __restrict__ uc_v, double factor, const double* dscale,
+           const KScalars* sc) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  static_assert(sizeof(T) == 4, "box sweep: 4-byte arithmetic types only");
+  constexpr bool HK = KIND == PDEHIP_OP_FLUX;
+  constexpr int H = 2 * NS;
+  constexpr int EX = kBX + 2 * H, EY = kBY + 2 * H, EZ = BZ + 2 * H;
+  constexpr int EXY = EX * EY, EN = EXY * EZ;
+  // coarse nodes under the extended box (prolongation on load)
+  constexpr int CX = EX / 2 + 2, CY = EY / 2 + 2, CZ = EZ / 2 + 2, CN = CX * CY * CZ;
+  // x and b in LDS in the storage type (every value the sweeps hold is representable in it),
+  // rho = 1/kappa in the arithmetic type
+  __shared__ S xs[EN];
+  __shared__ S bs[EN];
+  __shared__ T rs[HK ? EN : 1];
+  __shared__ S cs[IN == 2 ? CN : 1];
+  if (skip(sc)) return;
+  const int t = xcd_tile();
+  if (t >= nboxes) return;
+  const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
+  const int ox = 1 + bx * kBX, oy = 1 + by * kBY, oz = 1 + bz * BZ;   // first interior node
+  const int gx0 = ox - H, gy0 = oy - H, gz0 = oz - H;                  // grid node of LDS (0,0,0)
+  const int nx = op.nx, ny = op.ny, nz = op.nz;
+  const S* __restrict__ b = static_cast<const S*>(bv);
+  const S* __restrict__ xin = static_cast<const S*>(xin_v);
+  const T* __restrict__ kap = static_cast<const T*>(op.kappa);
+  S* __restrict__ xout = static_cast<S*>(xout_v);
+  const int tid = static_cast<int>(threadIdx.x);
+  auto gid = [&](int i, int j, int k) -> long long { return ((long long)k * ny + j) * nx + i; };
+
+  const int cx0 = gx0 >> 1, cy0 = gy0 >> 1, cz0 = gz0 >> 1;   // floor
+  // the extended box (and the coarse box): every load issued before the first LDS store
+  constexpr int NC = IN == 2 ? (CN + kBoxThreads - 1) / kBoxThreads : 1;
+  S cv[NC];
+  bool inc[NC];
+  if constexpr (IN == 2) {
+    const S* __restrict__ uc = static_cast<const S*>(uc_v);
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const int e = min(tid + q * kBoxThreads, CN - 1);
+      const int ci = cx0 + e % CX, cj = cy0 + (e / CX) % CY, ck = cz0 + e / (CX * CY);
+      inc[q] = ci >= 0 && ci < cop.nx && cj >= 0 && cj < cop.ny && ck >= 0 && ck < cop.nz;
+      cv[q] = uc[inc[q] ? ((long long)ck * cop.ny + cj) * cop.nx + ci : 0];
+    }
+  }
+  constexpr int NL = (EN + kBoxThreads - 1) / kBoxThreads;
+  S xv[NL], bv_[NL];
+  T rv[HK ? NL : 1];
+  bool inb[NL];
+#pragma unroll
+  for (int q = 0; q < NL; ++q) {
+    const int e = min(tid + q * kBoxThreads, EN - 1);
+    const int gi = gx0 + e % EX, gj = gy0 + (e / EX) % EY, gk = gz0 + e / EXY;
+    inb[q] = gi >= 0 && gi < nx && gj >= 0 && gj < ny && gk >= 0 && gk < nz;
+    const long long id = inb[q] ? gid(gi, gj, gk) : 0;
+    if constexpr (IN != 1) xv[q] = xin[id];
+    bv_[q] = b[id];
+    if constexpr (HK) rv[q] = kap[id];
+  }
+#pragma unroll
+  for (int q = 0; q < NL; ++q) {
+    const int e = tid + q * kBoxThreads;
+    if (EN % kBoxThreads == 0 || e < EN) {
+      if constexpr (IN != 1) xs[e] = inb[q] ? xv[q] : S(0.0f);
+      else xs[e] = S(0.0f);
+      bs[e] = inb[q] ? bv_[q] : S(0.0f);
+      if constexpr (HK) rs[e] = inb[q] ? rv[q] : T(0);
+    }
+  }
+  if constexpr (IN == 2) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const int e = tid + q * kBoxThreads;
+      if (CN % kBoxThreads == 0 || e < CN) cs[e] = inc[q] ? cv[q] : S(0.0f);
+    }
+  }
+  if constexpr (IN == 2) {
+    __syncthreads();
+    const double s = dscale ? factor * (*dscale) : factor;
+    auto C = [&](int k, int j, int i) -> double {
+      return static_cast<double>(back<S>(cs[((k - cz0) * CY + (j - cy0)) * CX + (i - cx0)]));
+    };
+    // values first, stores after (each node reads only the coarse box and itself)
+    S pv[NL];
+    bool upd[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+      const int e = min(tid + q * kBoxThreads, EN - 1);
+      const int gi = gx0 + e % EX, gj = gy0 + (e / EX) % EY, gk = gz0 + e / EXY;
+      upd[q] = (EN % kBoxThreads == 0 || tid + q * kBoxThreads < EN) && gi >= 1 && gi <= nx - 2 &&
+               gj >= 1 && gj <= ny - 2 && gk >= 1 && gk <= nz - 2;
+      if (!upd[q]) continue;
+      const ProlongCol w = prolong_col(cop, gi, gj);
+      const int kc = gk >> 1;
+      const bool half = (gk & 1) != 0;
+      double wz0, wz1 = 0.0;
+      const int k0 = prolong_zplane(cop, kc, &wz0);
+      const double p_lo = prolong_plane(w, C(k0, w.y0, w.x0), C(k0, w.y0, w.x1),
+                                        C(k0, w.y1, w.x0), C(k0, w.y1, w.x1), wz0);
+      double p_hi = 0.0;
+      if (half) {
+        const int k1 = prolong_zplane(cop, kc + 1, &wz1);
+        p_hi = prolong_plane(w, C(k1, w.y0, w.x0), C(k1, w.y0, w.x1), C(k1, w.y1, w.x0),
+                             C(k1, w.y1, w.x1), wz1);
+      }
+      const double fa = static_cast<double>(back<S>(xs[e]));
+      pv[q] = store_as<S>(prolong_fine(fa, s, half, p_lo, p_hi));
+    }
+#pragma unroll
+    for (int q = 0; q < NL; ++q)
+      if (upd[q]) xs[tid + q * kBoxThreads] = pv[q];
+  }
+  __syncthreads();
+
+  const T fx = T(op.fx), fy = T(op.fy), fz = T(op.fz);
+  const int c2 = 1 - c1;
+  auto stage = [&](auto stc) {
+    constexpr int st = decltype(stc)::value;
+    constexpr int r = 2 * NS - 1 - st;                 // ring of this stage
+    constexpr int RX = kBX + 2 * r, RY = kBY + 2 * r, RZ = BZ + 2 * r, HX = RX / 2;
+    constexpr int NI = HX * RY * RZ;
+    constexpr int o = H - r;                           // LDS offset of the stage's region
+    constexpr int NQ = (NI + kBoxThreads - 1) / kBoxThreads;
+    const int colour = (st & 1) ? c2 : c1;
+    // every update of the stage first (a stage reads only the other colour), stores after
+    S nv[NQ];
+    int ne[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int it = tid + q * kBoxThreads;
+      ne[q] = -1;
+      if (NI % kBoxThreads != 0 && it >= NI) continue;
+      const int hx = it % HX, row = it / HX;
+      const int ey = o + row % RY, ez = o + row / RY;
+      const int gj = gy0 + ey, gk = gz0 + ez;
+      // the region's nodes of this colour in row (ey, ez): every other x, starting at par
+      const int par = (colour + gx0 + o + gj + gk) & 1;
+      const int ex = o + 2 * hx + par;
+      const int gi = gx0 + ex;
+      if (gi < 1 || gi > nx - 2 || gj < 1 || gj > ny - 2 || gk < 1 || gk > nz - 2) continue;
+      const int e = (ez * EY + ey) * EX + ex;
+      T aw = T(0), ae = T(0), as = T(0), an = T(0), ab = T(0), af = T(0);
+      if constexpr (HK) {
+        const T rc = rs[e];
+        aw = face_smooth<T>(rc, rs[e - 1], fx);
+        ae = face_smooth<T>(rc, rs[e + 1], fx);
+        as = face_smooth<T>(rc, rs[e - EX], fy);
+        an = face_smooth<T>(rc, rs[e + EX], fy);
+        ab = face_smooth<T>(rc, rs[e - EXY], fz);
+        af = face_smooth<T>(rc, rs[e + EXY], fz);
+      }
+      W7<T> w = weights_at<3, KIND, T>(op, 0, aw, ae, as, an, ab, af);
+      fold_center<3, T>(op, gi, gj, gk, w);
+      const T v = gs_value<3, T>(w, back<S>(bs[e]), back<S>(xs[e + 1]), back<S>(xs[e - 1]),
+                                 back<S>(xs[e + EX]), back<S>(xs[e - EX]), back<S>(xs[e + EXY]),
+                                 back<S>(xs[e - EXY]));
+      nv[q] = to_store<S, T>(v);
+      ne[q] = e;
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      if (ne[q] >= 0) xs[ne[q]] = nv[q];
+    __syncthreads();
+  };
+  stage(IC<0>{});
+  stage(IC<1>{});
+  if constexpr (NS == 2) {
+    stage(IC<2>{});
+    stage(IC<3>{});
+  }
+
+  // out: the box's interior nodes, plus the boundary layer beside it on the grid faces it touches
+  const int xl = ox == 1 ? 0 : ox, yl = oy == 1 ? 0 : oy, zl = oz == 1 ? 0 : oz;
+  const int xh = ox + kBX >= nx - 1 ? nx : ox + kBX;
+  const int yh = oy + kBY >= ny - 1 ? ny : oy + kBY;
+  const int zh = oz + BZ >= nz - 1 ? nz : oz + BZ;
+  constexpr int OX = kBX + 2, OY = kBY + 2, OZ = BZ + 2, ON = OX * OY * OZ;
+  for (int it = tid; it < ON; it += kBoxThreads) {
+    const int gi = ox - 1 + it % OX, gj = oy - 1 + (it / OX) % OY, gk = oz - 1 + it / (OX * OY);
+    if (gi < xl || gi >= xh || gj < yl || gj >= yh || gk < zl || gk >= zh) continue;
+    const int e = ((gk - gz0) * EY + (gj - gy0)) * EX + (gi - gx0);
+    xout[gid(gi, gj, gk)] = xs[e];
+  }
+}
+
+template <int KIND, Prec P, int NS, int BZ, int IN>
+void launch(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout, int c1,
+            const DevOp& cop, const void* uc, double factor, const double* dscale) {
+  const int nbx = (op.nx - 2 + kBX - 1) / kBX;
+  const int nby = (op.ny - 2 + kBY - 1) / kBY;
+  const int nbz = (op.nz - 2 + BZ - 1) / BZ;
+  const int nboxes = nbx * nby * nbz;
+  const dim3 g(static_cast<unsigned>(8 * ((nboxes + 7) / 8)));
+  hipLaunchKernelGGL((k_box_rbgs<KIND, P, NS, BZ, IN>), g, dim3(kBoxThreads), 0, L.stream, op, b,
+                     xin, xout, c1, nbx, nby, nboxes, cop, uc, factor, dscale, L.sc);
+}
+
+template <int KIND, Prec P, int NS, int BZ>
+void by_input(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout, int c1,
+              int in_mode, const DevOp& cop, const void* uc, double factor, const double* dscale) {
+  if (in_mode == 1) launch<KIND, P, NS, BZ, 1>(L, op, b, xin, xout, c1, cop, uc, factor, dscale);
+  else if (in_mode == 2) launch<KIND, P, NS, BZ, 2>(L, op, b, xin, xout, c1, cop, uc, factor, dscale);
+  else launch<KIND, P, NS, BZ, 0>(L, op, b, xin, xout, c1, cop, uc, factor, dscale);
+}
+
+template <int KIND, Prec P>
+void by_shape(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout, int c1,
+              int sweeps, int bz, int in_mode, const DevOp& cop, const void* uc, double factor,
+              const double* dscale) {
+  if (sweeps == 2) {
+    if (bz == 4) by_input<KIND, P, 2, 4>(L, op, b, xin, xout, c1, in_mode, cop, uc, factor, dscale);
+    else by_input<KIND, P, 2, 8>(L, op, b, xin, xout, c1, in_mode, cop, uc, factor, dscale);
+  } else {
+    if (bz == 4) by_input<KIND, P, 1, 4>(L, op, b, xin, xout, c1, in_mode, cop, uc, factor, dscale);
+    else by_input<KIND, P, 1, 8>(L, op, b, xin, xout, c1, in_mode, cop, uc, factor, dscale);
+  }
+}
+
+}  // namespace
+
+bool box_sweeps(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x_in,
+                void* x_out, int first_color, int sweeps, int in_mode, const DevOp* coarse,
+                Prec pc, const void* uc, double factor, const double* dev_scale) {
+  if (op.nz < 3 || op.nx < 3 || op.ny < 3 || sweeps < 1 || sweeps > 2 || p == P64) return false;
+  if (op.kind != PDEHIP_OP_CONST && op.kind != PDEHIP_OP_FLUX) return false;
+  if (op.n >= (1LL << 31)) return false;
+  if (in_mode == 2 && (!coarse || pc != p || coarse->nz < 3)) return false;
+  if (in_mode != 1 && x_in == x_out) return false;   // other boxes read x_in's ring
+  const DevOp cop = coarse ? *coarse : op;
+  const int bz = smooth_tuning().box_z == 4 ? 4 : 8;
+  if (op.kind == PDEHIP_OP_CONST) {
+    if (p == P32) by_shape<PDEHIP_OP_CONST, P32>(L, op, b, x_in, x_out, first_color, sweeps, bz, in_mode, cop, uc, factor, dev_scale);
+    else by_shape<PDEHIP_OP_CONST, P16>(L, op, b, x_in, x_out, first_color, sweeps, bz, in_mode, cop, uc, factor, dev_scale);
+  } else {
+    if (p == P32) by_shape<PDEHIP_OP_FLUX, P32>(L, op, b, x_in, x_out, first_color, sweeps, bz, in_mode, cop, uc, factor, dev_scale);
+    else by_shape<PDEHIP_OP_FLUX, P16>(L, op, b, x_in, x_out, first_color, sweeps, bz, in_mode, cop, uc, factor, dev_scale);
+  }
+  return true;
+}
+
+}  // namespace pdehip

@@ -57,9 +57,22 @@ struct SmoothTuning {
   int ychunk = 64;   // 2D: rows per wavefront (upper bound)
   int pair = 0;      // 3D FP32/FP16: column-pair (divergence-free) tiles
   int ring = 1;      // 3D FP32/FP16: ring-buffered, 6-step unrolled sweep (k_rbgs_ring3)
+  int box = 65;      // 3D FP32/FP16 coarse levels of mixed V-cycles with every side <= box
+                     // smooth on boxes (k_box.hip); 0: plane-march sweeps on every level
+  int box2 = 65;     // ... two sweeps per box launch where every side is <= box2, else one
+  int box_z = 8;     // interior planes of a box (4 or 8)
 };
 void set_smooth_tuning(const SmoothTuning& t);
 SmoothTuning smooth_tuning();
+// `sweeps` (1 or 2) red-black sweeps (colour first_color first) of a 3D FP32/FP16 CONST or FLUX
+// level on boxes, temporally blocked in LDS (k_box.hip), out of place.  in_mode 0: from x_in;
+// 1: from zero (x_in not read); 2: from x_in + factor (* *dev_scale) * the prolongation of the
+// coarse correction uc (level `coarse`, precision pc == p), formed on load.  x_out equals what
+// prolong_add and rbgs_sweep launches would leave.  Returns false (nothing launched) when the
+// level is not one the box kernels take.
+bool box_sweeps(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x_in,
+                void* x_out, int first_color, int sweeps, int in_mode, const DevOp* coarse,
+                Prec pc, const void* uc, double factor, const double* dev_scale);
 // Number of blocks `partials`-producing kernels use for this op (<= kMaxPartials).
 int partial_blocks(const DevOp& op);
 // 3D apply / residual as plane marches (k_plane.hip); apply()/residual() route 3D levels there

@@ -714,15 +714,62 @@ static void smooth(const Launch& L, Level& lv, const DevOp& op, const void* b, v
   }
 }
 
+// Box smoothing (k_box.hip) serves this level: a coarse 3D FP32/FP16 level of a mixed V-cycle
+// whose sides are all within the tuning bound.
+static bool box_level(const Plan& P, const Level& lv, int l) {
+  const SmoothTuning t = smooth_tuning();
+  return P.mixed && l > 0 && t.box > 0 && t.fused && lv.alt && lv.g.nz > 1 && lv.prec != P64 &&
+         lv.g.nx <= t.box && lv.g.ny <= t.box && lv.g.nz <= t.box &&
+         (lv.op32.kind == PDEHIP_OP_CONST || lv.op32.kind == PDEHIP_OP_FLUX);
+}
+
+// `sweeps` box sweeps (colour c1 first) on level lv, two per launch on levels within the
+// smooth.box2 bound (else one): from a zero guess (src
+// null), or from src plus the prolongation `pin` of the next level's correction folded into the
+// first launch.  Results alternate between u and alt and never land in src; returns the buffer
+// that holds the last one.
+struct ProlongIn {
+  const DevOp* op;
+  Prec prec;
+  const void* u;
+  double factor;
+};
+static void* box_smooth(const Launch& L, Level& lv, const DevOp& op, const void* b, void* src,
+                        int sweeps, int c1, const ProlongIn* pin) {
+  void* u = lv.u.get();
+  void* alt = lv.alt.get();
+  void* out = src == u ? alt : u;
+  const int b2 = smooth_tuning().box2;
+  const int per = lv.g.nx <= b2 && lv.g.ny <= b2 && lv.g.nz <= b2 ? 2 : 1;
+  bool first = true;
+  for (int left = sweeps; left > 0;) {
+    const int ns = left >= per ? per : 1;
+    const int mode = first ? (src ? (pin ? 2 : 0) : 1) : 0;
+    const bool ok = box_sweeps(L, op, lv.prec, b, src, out, c1, ns, mode, pin ? pin->op : nullptr,
+                               pin ? pin->prec : lv.prec, pin ? pin->u : nullptr,
+                               pin ? pin->factor : 1.0, nullptr);
+    if (!ok) throw Error(PDEHIP_ERR_DEVICE, "box smoothing refused a level it was chosen for");
+    first = false;
+    left -= ns;
+    src = out;
+    out = out == u ? alt : u;
+  }
+  return src;
+}
+
 // One V-cycle on b0 (level-0 right-hand side) into u0 (level-0 correction from a zero guess;
 // u0's contents on entry are ignored).  dot (optional): partials of b0 . u0 left by the last
-// level-0 sweep, *dot_np of them (0 when that sweep could not produce them).
+// level-0 sweep, *dot_np of them (0 when that sweep could not produce them).  Coarse levels
+// that box_level() picks smooth on boxes: pre-smoothing from zero in one launch, and the
+// prolongation of the coarser correction with the post-smoothing in one more; their correction
+// then ends in u or alt, which the finer level's prolongation reads.
 static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0, double* dot = nullptr,
                    int* dot_np = nullptr) {
   if (dot_np) *dot_np = 0;
   const int nl = static_cast<int>(P.L.size());
   const int pre = std::max(0, P.opt.mg_pre), post = std::max(0, P.opt.mg_post);
   KScalars* sc = const_cast<KScalars*>(L.sc);
+  std::vector<void*> corr(nl);   // level l's correction: after pre-smoothing, then after post-smoothing
   for (int l = 0; l < nl - 1; ++l) {
     Level& f = P.L[l];
     Level& c = P.L[l + 1];
@@ -731,8 +778,13 @@ static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0, double* d
     const void* bl = l == 0 ? b0 : f.b.get();
     void* ul = l == 0 ? u0 : f.u.get();
     // every level's correction starts from zero; smooth() never reads it
-    smooth(L, f, fo, bl, ul, pre, 0, true);
-    if (pre == 0) fill(L.stream, f.prec, ul, f.g.n, 0.0);
+    if (pre > 0 && box_level(P, f, l)) {
+      ul = box_smooth(L, f, fo, bl, nullptr, pre, 0, nullptr);
+    } else {
+      smooth(L, f, fo, bl, ul, pre, 0, true);
+      if (pre == 0) fill(L.stream, f.prec, ul, f.g.n, 0.0);
+    }
+    corr[l] = ul;
     residual(L, fo, f.prec, bl, ul, f.r.get(), nullptr);
     restrict_fw(L, fo, f.prec, f.r.get(), co, c.prec, c.b.get(), 1.0,
                 (P.mixed && l == 0) ? &sc->scale : nullptr);
@@ -749,14 +801,20 @@ static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0, double* d
     for (int it = 0; it < half; ++it) { rbgs(L, cop, cl.prec, cb, cu, 0); rbgs(L, cop, cl.prec, cb, cu, 1); }
     for (int it = 0; it < half; ++it) { rbgs(L, cop, cl.prec, cb, cu, 1); rbgs(L, cop, cl.prec, cb, cu, 0); }
   }
+  corr[nl - 1] = cu;
   for (int l = nl - 2; l >= 0; --l) {
     Level& f = P.L[l];
     Level& c = P.L[l + 1];
     const DevOp& fo = P.mixed ? f.op32 : f.op64;
     const DevOp& co = P.mixed ? c.op32 : c.op64;
     const void* bl = l == 0 ? b0 : f.b.get();
-    void* ul = l == 0 ? u0 : f.u.get();
-    prolong_add(L, co, c.prec, c.u.get(), fo, f.prec, ul, c.sigma / f.sigma,
+    void* ul = corr[l];
+    if (post > 0 && box_level(P, f, l)) {
+      const ProlongIn pin{&co, c.prec, corr[l + 1], c.sigma / f.sigma};
+      corr[l] = box_smooth(L, f, fo, bl, ul, post, 1, &pin);
+      continue;
+    }
+    prolong_add(L, co, c.prec, corr[l + 1], fo, f.prec, ul, c.sigma / f.sigma,
                 (P.mixed && l == 0) ? &sc->inv_scale : nullptr);
     if (l == 0 && dot) smooth(L, f, fo, bl, ul, post, 1, false, dot, P.npart, dot_np);
     else smooth(L, f, fo, bl, ul, post, 1, false);

@@ -1,7 +1,8 @@
 """The fused out-of-place red-black sweep (one launch per sweep) against the two-launch in-place
 sweep: every multigrid solve must come out bitwise identical, whatever the launch shape, on
 grids that span several tiles and z chunks in every direction (non-cubic, so a swapped axis
-shows), for FP64 and for the mixed FP32/FP16 hierarchy."""
+shows), for FP64 and for the mixed FP32/FP16 hierarchy.  The box smoothing of the coarse mixed
+levels is held to the same bar against the plane-march sweeps."""
 import numpy as np
 import pytest
 
@@ -11,7 +12,8 @@ from pde_solver_amd.capi import BC
 pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"smooth.fused": 1, "smooth.rows": 2, "smooth.zchunk": 0, "smooth.ychunk": 64,
-            "smooth.pair": 0, "smooth.ring": 1, "stencil.ring": 1}
+            "smooth.pair": 0, "smooth.ring": 1, "stencil.ring": 1, "smooth.box": 65,
+            "smooth.box2": 65, "smooth.box_z": 8, "mixed.fp16": 1}
 
 
 @pytest.fixture(scope="module")
@@ -45,7 +47,7 @@ def const_problem(shape, seed):
     return s
 
 
-def solve(ctx, s, precision, fused, **tune):
+def solve(ctx, s, precision, fused, pre=2, post=2, **tune):
     from pde_solver_amd import hip
     for k, v in DEFAULTS.items():
         hip.tune_set(k, v)
@@ -53,7 +55,8 @@ def solve(ctx, s, precision, fused, **tune):
     for k, v in tune.items():
         hip.tune_set(k if "." in k else "smooth." + k, v)
     opt = capi.options(capi.MGPCG, tol=1e-10, tol_mode=capi.TOL_REL, precond=capi.PC_MG,
-                       bc_rows=capi.ROWS_FOLDED, precision=precision, max_iter=100)
+                       bc_rows=capi.ROWS_FOLDED, precision=precision, max_iter=100,
+                       mg_pre=pre, mg_post=post)
     st, u, rep = ctx.solve(s, opt)
     assert st == 0, ctx.error()
     return u, rep
@@ -98,6 +101,38 @@ def test_plane_ring_bitwise(ctx, kind, shape):
     s = (flux_problem if kind == "flux" else const_problem)(shape, seed=sum(shape) + 1)
     u0, r0 = solve(ctx, s, capi.MIXED, 1, **{"stencil.ring": 0})
     u1, r1 = solve(ctx, s, capi.MIXED, 1, **{"stencil.ring": 1})
+    print(f"  iterations {r0.iterations}/{r1.iterations}  max diff {np.abs(u0 - u1).max():.3e}", flush=True)
+    assert r0.iterations == r1.iterations
+    assert np.array_equal(u0, u1)
+
+
+BOX_CASES = [
+    # (kind, (nz, ny, nx), box_z, FP16 coarse levels, pre, post, box2)
+    ("flux", (33, 65, 129), 8, 1, 2, 2, 65),
+    ("flux", (65, 33, 17), 4, 1, 2, 2, 65),
+    ("flux", (129, 65, 33), 8, 0, 2, 2, 65),
+    ("const", (17, 129, 65), 8, 1, 2, 2, 65),
+    ("const", (65, 65, 65), 4, 0, 2, 2, 65),
+    ("flux", (33, 65, 129), 8, 1, 1, 3, 65),
+    ("flux", (65, 33, 129), 4, 1, 3, 1, 65),
+    ("flux", (65, 129, 129), 8, 1, 2, 2, 33),
+    ("const", (129, 65, 129), 4, 0, 3, 2, 0),
+]
+
+
+@pytest.mark.parametrize("kind,shape,bz,fp16,pre,post,box2", BOX_CASES)
+def test_box_smoothing_bitwise(ctx, kind, shape, bz, fp16, pre, post, box2):
+    """Coarse levels of the mixed V-cycle on LDS boxes (two sweeps per launch, the
+    prolongation folded into the first post-smoothing launch) against the plane-march sweeps
+    and the prolongation kernel: the same mixed solve bit for bit, on grids whose coarse levels
+    span several boxes in every direction and end in partial boxes, FP16 and FP32 coarse
+    storage, odd sweep counts and levels with one sweep per launch (buffers alternating)."""
+    print(f"box {kind} {shape} box_z={bz} fp16={fp16} V({pre},{post}) box2={box2}", flush=True)
+    s = (flux_problem if kind == "flux" else const_problem)(shape, seed=sum(shape) + 7)
+    u0, r0 = solve(ctx, s, capi.MIXED, 1, pre, post, **{"smooth.box": 0, "mixed.fp16": fp16})
+    u1, r1 = solve(ctx, s, capi.MIXED, 1, pre, post,
+                   **{"smooth.box": 129, "smooth.box2": box2, "smooth.box_z": bz,
+                      "mixed.fp16": fp16})
     print(f"  iterations {r0.iterations}/{r1.iterations}  max diff {np.abs(u0 - u1).max():.3e}", flush=True)
     assert r0.iterations == r1.iterations
     assert np.array_equal(u0, u1)
