@@ -247,7 +247,10 @@ void pdehip_plan_destroy(pdehip_plan* plan);
  * per wavefront, 16..4096), "stencil.plane" (0/1: 3D apply and residual as plane-march
  * kernels), "smooth.pair" (0/1: 3D FP32/FP16 sweeps with column-pair tiles, no colour
  * divergence), "smooth.ring" (0/1, default 1: 3D FP32/FP16 sweeps by the ring-buffered,
- * 6-step unrolled kernel), "stencil.ring" (0/1, default 1: the same for the 3D FP32/FP16
+ * 6-step unrolled kernel), "smooth.reg" (0/1, default 1: of the ring sweeps, the one with
+ * in-plane neighbours in registers, four rows per wave), "smooth.dot" (0/1, default 1: the last
+ * level-0 sweep of a mixed V-cycle also leaves the partials of r.z), "stencil.ring" (0/1,
+ * default 1: the same as "smooth.ring" for the 3D FP32/FP16
  * apply and residual), "smooth.box" (0..1025, default 65: coarse 3D levels of mixed-precision
  * V-cycles whose sides are all at most this smooth on LDS boxes, the prolongation folded into
  * the first post-smoothing launch; 0: off), "smooth.box2" (0..1025, default 65: of those, the

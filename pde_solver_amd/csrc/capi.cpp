@@ -265,6 +265,8 @@ int pdehip_tune_set(const char* key, int value) {
   else if (k == "smooth.ychunk" && value >= 16 && value <= 4096) t.ychunk = value;
   else if (k == "smooth.pair" && (value == 0 || value == 1)) t.pair = value;
   else if (k == "smooth.ring" && (value == 0 || value == 1)) t.ring = value;
+  else if (k == "smooth.reg" && (value == 0 || value == 1)) t.reg = value;
+  else if (k == "smooth.dot" && (value == 0 || value == 1)) t.dot = value;
   else if (k == "smooth.box" && value >= 0 && value <= 1025) t.box = value;
   else if (k == "smooth.box2" && value >= 0 && value <= 1025) t.box2 = value;
   else if (k == "smooth.box_z" && (value == 4 || value == 8)) t.box_z = value;
@@ -287,6 +289,8 @@ int pdehip_tune_get(const char* key, int* value) {
   else if (k == "smooth.ychunk") *value = t.ychunk;
   else if (k == "smooth.pair") *value = t.pair;
   else if (k == "smooth.ring") *value = t.ring;
+  else if (k == "smooth.reg") *value = t.reg;
+  else if (k == "smooth.dot") *value = t.dot;
   else if (k == "smooth.box") *value = t.box;
   else if (k == "smooth.box2") *value = t.box2;
   else if (k == "smooth.box_z") *value = t.box_z;

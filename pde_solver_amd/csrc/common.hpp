@@ -88,6 +88,10 @@ template <> __device__ __forceinline__ double face_coef<double>(double kc, doubl
   return s > 0.0 ? f * (2.0 * kc * kn) / s : 0.0;
 }
 template <> __device__ __forceinline__ float face_coef<float>(float rc, float rn, float f) {
+  // rounded here: a kernel that sums faces straight from registers must not fuse this product
+  // into the sum (an FMA skips its rounding) where a kernel that keeps faces in LDS cannot, or
+  // the two round the same operator differently whenever f is not a power of two
+#pragma clang fp contract(off)
   return (f + f) * __builtin_amdgcn_rcpf(rc + rn);
 }
 

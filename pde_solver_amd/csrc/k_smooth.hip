@@ -473,6 +473,251 @@ k_rbgs_ring3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin
   }
 }
 
+// ------------------------------------------------------------------------------- 3D, register rows
+// k_rbgs_reg3: the ring sweep's plane pipeline, colour order and per-node arithmetic (so the same
+// bits in x_out), with the in-plane neighbours held in registers.  A wave owns kRR consecutive
+// rows of a 64 x (8 kRR) extended tile (lane = column; output tile 60 x (8 kRR - 4)):
+//   * x neighbours come from the adjacent lanes by DPP wave shifts, y neighbours from the
+//     thread's own rows; only a wave's first and last rows meet another wave's rows, through
+//     LDS slots of two rows per wave (x) and one row per wave (rho, north faces);
+//   * every node forms its east and north faces of a plane once; the west face is the west
+//     lane's east face (DPP), the south face the row below's north face;
+//   * register rings indexed by the step's phase of a 6-step unrolled plane loop, buffer loads
+//     and stores on per-plane descriptors, colour updates computed on every lane and selected,
+//     as in k_rbgs_ring3.
+// One barrier per plane: x slots are rewritten three planes later, rho slots two, face slots
+// three, each after a barrier that every reader of the old contents has passed.
+constexpr int kRR = 4;
+
+__device__ __forceinline__ float dpp_west(float v) {   // lane l gets lane l-1 (lane 0: itself)
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v),
+                                                               __builtin_bit_cast(int, v), 0x138,
+                                                               0xf, 0xf, false));
+}
+__device__ __forceinline__ float dpp_east(float v) {   // lane l gets lane l+1 (lane 63: itself)
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v),
+                                                               __builtin_bit_cast(int, v), 0x130,
+                                                               0xf, 0xf, false));
+}
+
+template <int KIND, Prec P, bool ZERO, bool DOT = false>
+__global__ void __launch_bounds__(kThreads)
+k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v,
+            void* __restrict__ xout_v, int c1, int tiles_x, int tiles_y, int ntiles,
+            int kchunk, const KScalars* sc, double* partials = nullptr) {
+  using T = typename PT<P>::T;
+  using S = typename PT<P>::S;
+  static_assert(sizeof(T) == 4, "register-row sweep: 4-byte arithmetic types only");
+  constexpr int R = kRR;
+  constexpr int TYE = R * kWaves;
+  constexpr int TY = TYE - 2 * kHalo;
+  constexpr bool HK = KIND == PDEHIP_OP_FLUX;
+  __shared__ T lxb[3][kWaves][2][kLanes];          // x of each wave's first / last row
+  __shared__ T lkb[HK ? 2 : 1][kWaves][kLanes];    // rho of each wave's first row
+  __shared__ T lfn[HK ? 3 : 1][kWaves][kLanes];    // north faces of each wave's last row
+  if (skip(sc)) return;
+  const int t = xcd_tile();
+  if (t >= ntiles) {
+    if constexpr (DOT) {
+      if (threadIdx.x == 0) partials[blockIdx.x] = 0.0;
+    }
+    return;
+  }
+  const int tx = t % tiles_x;
+  const int rest = t / tiles_x;
+  const int ty = rest % tiles_y;
+  const int cz = rest / tiles_y;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wv = threadIdx.x / kLanes;
+  const int wdn = (wv + kWaves - 1) & (kWaves - 1);   // the wave below (wraps: its values unused)
+  const int wup = (wv + 1) & (kWaves - 1);            // the wave above
+  const int i = tx * kTileX - kHalo + lane;
+  double dacc = 0.0;   // DOT
+  const int k0 = cz * kchunk;
+  const int k1 = min(k0 + kchunk, op.nz);
+  const int nz = op.nz;
+  const int c2 = 1 - c1;
+  const long long sxy = op.sxy;
+  const int pbytes = static_cast<int>(sxy * static_cast<long long>(sizeof(S)));
+  const int kbytes = static_cast<int>(sxy * static_cast<long long>(sizeof(T)));
+  const T fx = T(op.fx), fy = T(op.fy), fz = T(op.fz);
+
+  const bool int_x = i >= 1 && i <= op.nx - 2;
+  const bool in_x = i >= 0 && i < op.nx;
+  const bool ring_x = lane >= 1 && lane <= kLanes - 2;
+  const bool tile_x = lane >= kHalo && lane < kLanes - kHalo;
+  unsigned m_r1 = 0, m_r2 = 0, m_fxy = 0;
+  unsigned off_s[R], off_k[R], off_st[R];
+  const int ci = min(max(i, 0), op.nx - 1);
+  const int jrow0 = ty * TY - kHalo + wv * R;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int lr = wv * R + r;
+    const int j = jrow0 + r;
+    const bool in_y = j >= 0 && j < op.ny;
+    const bool int_y = j >= 1 && j <= op.ny - 2;
+    const bool ring_y = lr >= 1 && lr <= TYE - 2;
+    const bool tile_y = lr >= kHalo && lr < TYE - kHalo;
+    const unsigned col = static_cast<unsigned>(min(max(j, 0), op.ny - 1) * op.nx + ci);
+    off_s[r] = col * static_cast<unsigned>(sizeof(S));
+    off_k[r] = col * static_cast<unsigned>(sizeof(T));
+    off_st[r] = (tile_x && tile_y && in_x && in_y) ? off_s[r] : kDropOffset;
+    if (ring_x && ring_y && int_x && int_y) m_r1 |= 1u << r;
+    if (tile_x && tile_y && int_x && int_y) m_r2 |= 1u << r;
+    if (i == 1 || i == op.nx - 2 || j == 1 || j == op.ny - 2) m_fxy |= 1u << r;
+  }
+  const int par0 = (i + jrow0) & 1;
+  const bool top_ok = wv * R + R - 1 < TYE - 1;   // the last row has a north neighbour in the tile
+
+  auto kcl = [&](int k) -> long long { return static_cast<long long>(min(max(k, 0), nz - 1)) * sxy; };
+  auto ld_u = [&](int r, int k) -> T {
+    if constexpr (ZERO) return T(0);
+    else return BufIO<S>::ld(plane_rsrc(xin_v, kcl(k), sizeof(S), pbytes), off_s[r]);
+  };
+  auto ld_b = [&](int r, int k) -> T {
+    return BufIO<S>::ld(plane_rsrc(bv, kcl(k), sizeof(S), pbytes), off_s[r]);
+  };
+  auto ld_k = [&](int r, int k) -> T {
+    if constexpr (HK) return BufIO<T>::ld(plane_rsrc(op.kappa, kcl(k), sizeof(T), kbytes), off_k[r]);
+    else return T(0);
+  };
+
+  // Rings, with plane pstart + s at slot s mod L: U (6) x(p-2 .. p+3), B (3) b(p-1 .. p+1),
+  // K (3) rho(p+1 .. p+3), ZF (6) zf(p-2 .. p+1), FE / FN (3) faces of planes p-1 .. p+1.
+  const int pstart = HK ? k0 - 3 : k0 - 1;
+  T U[R][6], B[R][3], K[R][3], ZF[R][6], FE[R][3], FN[R][3];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    U[r][4] = ld_u(r, pstart - 2);
+    U[r][5] = ld_u(r, pstart - 1);
+    U[r][0] = ld_u(r, pstart);
+    U[r][1] = ld_u(r, pstart + 1);
+    U[r][2] = ld_u(r, pstart + 2);
+    U[r][3] = T(0);
+    B[r][2] = ld_b(r, pstart - 1);
+    B[r][0] = ld_b(r, pstart);
+    B[r][1] = T(0);
+    K[r][0] = T(0);
+    K[r][1] = ld_k(r, pstart + 1);
+    K[r][2] = ld_k(r, pstart + 2);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) ZF[r][s] = T(0);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) { FE[r][s] = T(0); FN[r][s] = T(0); }
+  }
+
+  auto step = [&](auto phc, int p) {
+    constexpr int PH = decltype(phc)::value;
+    constexpr int u_m2 = (PH + 4) % 6, u_m1 = (PH + 5) % 6, u_0 = PH % 6, u_1 = (PH + 1) % 6,
+                  u_3 = (PH + 3) % 6;
+    constexpr int b_m1 = (PH + 2) % 3, b_0 = PH % 3, b_1 = (PH + 1) % 3;
+    constexpr int k_1 = (PH + 1) % 3, k_2 = (PH + 2) % 3, k_3 = PH % 3;
+    constexpr int z_m2 = (PH + 4) % 6, z_m1 = (PH + 5) % 6, z_0 = PH % 6, z_1 = (PH + 1) % 6;
+    constexpr int f_p1 = (PH + 1) % 3, f_0 = PH % 3, f_m1 = (PH + 2) % 3;
+    constexpr int sx = PH % 3, sxq = (PH + 2) % 3, sk = PH % 2;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      U[r][u_3] = ld_u(r, p + 3);
+      B[r][b_1] = ld_b(r, p + 1);
+      if constexpr (HK) K[r][k_3] = ld_k(r, p + 3);
+    }
+    lxb[sx][wv][0][lane] = U[0][u_0];
+    lxb[sx][wv][1][lane] = U[R - 1][u_0];
+    if constexpr (HK) lkb[sk][wv][lane] = K[0][k_1];
+    __syncthreads();
+    if constexpr (HK) {
+      // faces of plane p+1: east and north of every node, front in the ring
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const T kc = K[r][k_1];
+        ZF[r][z_1] = face_smooth<T>(kc, K[r][k_2], fz);
+        FE[r][f_p1] = face_smooth<T>(kc, dpp_east(kc), fx);
+        const T kn = r < R - 1 ? K[r + 1][k_1] : lkb[sk][wup][lane];
+        FN[r][f_p1] = face_smooth<T>(kc, kn, fy);
+      }
+      lfn[f_p1][wv][lane] = top_ok ? FN[R - 1][f_p1] : T(0);
+    }
+    // first colour on plane p (tile + ring)
+    if (p >= k0 - 1 && p >= 1 && p <= nz - 2) {
+      const int cp = (par0 + p) & 1;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int j = jrow0 + r;
+        const bool upd = ((m_r1 >> r) & 1u) && ((cp + r) & 1) == c1;
+        T aw = T(0), ae = T(0), as = T(0), an = T(0);
+        if constexpr (HK) {
+          ae = FE[r][f_0];
+          aw = dpp_west(ae);
+          an = FN[r][f_0];
+          as = r > 0 ? FN[r - 1][f_0] : lfn[f_0][wdn][lane];
+        }
+        W7<T> w = weights_at<3, KIND, T>(op, 0, aw, ae, as, an, ZF[r][z_m1], ZF[r][z_0]);
+        if (((m_fxy >> r) & 1u) || p == 1 || p == nz - 2) fold_center<3, T>(op, i, j, p, w);
+        const T xc = U[r][u_0];
+        const T xn = r < R - 1 ? U[r + 1][u_0] : lxb[sx][wup][0][lane];
+        const T xs = r > 0 ? U[r - 1][u_0] : lxb[sx][wdn][1][lane];
+        const T v = rounded<S, T>(gs_value<3, T>(w, B[r][b_0], dpp_east(xc), dpp_west(xc), xn, xs,
+                                                 U[r][u_1], U[r][u_m1]));
+        U[r][u_0] = upd ? v : xc;
+      }
+      lxb[sx][wv][0][lane] = U[0][u_0];
+      lxb[sx][wv][1][lane] = U[R - 1][u_0];
+    }
+    // second colour on plane q = p - 1 (tile), then the tile's plane q goes out
+    const int q = p - 1;
+    if (q >= k0 && q < k1) {
+      if (q >= 1 && q <= nz - 2) {
+        const int cq = (par0 + q) & 1;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int j = jrow0 + r;
+          const bool upd = ((m_r2 >> r) & 1u) && ((cq + r) & 1) == c2;
+          T aw = T(0), ae = T(0), as = T(0), an = T(0);
+          if constexpr (HK) {
+            ae = FE[r][f_m1];
+            aw = dpp_west(ae);
+            an = FN[r][f_m1];
+            as = r > 0 ? FN[r - 1][f_m1] : lfn[f_m1][wdn][lane];
+          }
+          W7<T> w = weights_at<3, KIND, T>(op, 0, aw, ae, as, an, ZF[r][z_m2], ZF[r][z_m1]);
+          if (((m_fxy >> r) & 1u) || q == 1 || q == nz - 2) fold_center<3, T>(op, i, j, q, w);
+          const T xc = U[r][u_m1];
+          const T xn = r < R - 1 ? U[r + 1][u_m1] : lxb[sxq][wup][0][lane];
+          const T xs = r > 0 ? U[r - 1][u_m1] : lxb[sxq][wdn][1][lane];
+          const T v = rounded<S, T>(gs_value<3, T>(w, B[r][b_m1], dpp_east(xc), dpp_west(xc), xn,
+                                                   xs, U[r][u_0], U[r][u_m2]));
+          U[r][u_m1] = upd ? v : xc;
+        }
+      }
+      const __amdgpu_buffer_rsrc_t ro = plane_rsrc(xout_v, static_cast<long long>(q) * sxy,
+                                                   sizeof(S), pbytes);
+#pragma unroll
+      for (int r = 0; r < R; ++r) BufIO<S>::st(ro, off_st[r], to_store<S, T>(U[r][u_m1]));
+      if constexpr (DOT) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (off_st[r] != kDropOffset) dacc += double(B[r][b_m1]) * double(U[r][u_m1]);
+      }
+    }
+  };
+
+  const int steps = k1 - pstart + 1;
+  for (int s0 = 0; s0 < steps; s0 += kRing) {
+    const int p = pstart + s0;
+    step(IC<0>{}, p);
+    step(IC<1>{}, p + 1);
+    step(IC<2>{}, p + 2);
+    step(IC<3>{}, p + 3);
+    step(IC<4>{}, p + 4);
+    step(IC<5>{}, p + 5);
+  }
+  if constexpr (DOT) {
+    __shared__ double lred[kWaves];
+    const double tot = block_sum(dacc, lred);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+  }
+}
+
 // ------------------------------------------------------------------------------- 2D
 // One wavefront per 60-column strip; rows are the marching axis, x neighbours come from the
 // adjacent lanes (ds_bpermute), y neighbours from the thread's row queue.  No LDS, no barrier.
@@ -879,6 +1124,35 @@ int launch3_ring(const Launch& L, const DevOp& op, const void* b, const void* xi
 }
 
 template <int KIND, Prec P>
+int launch3_reg(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout,
+                int c1, bool zero_in, double* partials = nullptr, int cap = 0) {
+  const int ty = kRR * kWaves - 2 * kHalo;
+  const int tiles_x = (op.nx + kTileX - 1) / kTileX;
+  const int tiles_y = (op.ny + ty - 1) / ty;
+  static const int occ = blocks_per_cu(k_rbgs_reg3<KIND, P, false>, kThreads);
+  const int prologue = KIND == PDEHIP_OP_FLUX ? 3 : 1;
+  const int kchunk = g_tune.zchunk > 0 ? g_tune.zchunk
+                                       : ring_chunk((long long)tiles_x * tiles_y, op.nz,
+                                                    device_cus() * occ, prologue);
+  const int chunks = (op.nz + kchunk - 1) / kchunk;
+  const int ntiles = tiles_x * tiles_y * chunks;
+  const dim3 g(static_cast<unsigned>(8 * ((ntiles + 7) / 8)));
+  if (partials) {
+    if (zero_in || static_cast<int>(g.x) > cap) return 0;
+    hipLaunchKernelGGL((k_rbgs_reg3<KIND, P, false, true>), g, dim3(kThreads), 0, L.stream, op, b,
+                       xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc, partials);
+    return static_cast<int>(g.x);
+  }
+  if (zero_in)
+    hipLaunchKernelGGL((k_rbgs_reg3<KIND, P, true>), g, dim3(kThreads), 0, L.stream, op, b, xin,
+                       xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc, nullptr);
+  else
+    hipLaunchKernelGGL((k_rbgs_reg3<KIND, P, false>), g, dim3(kThreads), 0, L.stream, op, b, xin,
+                       xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc, nullptr);
+  return 0;
+}
+
+template <int KIND, Prec P>
 void launch3_pair(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout,
                   int c1, bool zero_in) {
   constexpr int rows = 2;
@@ -931,11 +1205,13 @@ void launch_kind(const Launch& L, const DevOp& op, Prec p, const void* b, const 
       case P64: launch3<KIND, P64>(L, op, b, xin, xout, c1, zero_in); break;
       case P32:
         if (pair) launch3_pair<KIND, P32>(L, op, b, xin, xout, c1, zero_in);
+        else if (ring && g_tune.reg) launch3_reg<KIND, P32>(L, op, b, xin, xout, c1, zero_in);
         else if (ring) launch3_ring<KIND, P32>(L, op, b, xin, xout, c1, zero_in);
         else launch3<KIND, P32>(L, op, b, xin, xout, c1, zero_in);
         break;
       default:
         if (pair) launch3_pair<KIND, P16>(L, op, b, xin, xout, c1, zero_in);
+        else if (ring && g_tune.reg) launch3_reg<KIND, P16>(L, op, b, xin, xout, c1, zero_in);
         else if (ring) launch3_ring<KIND, P16>(L, op, b, xin, xout, c1, zero_in);
         else launch3<KIND, P16>(L, op, b, xin, xout, c1, zero_in);
         break;
@@ -956,9 +1232,18 @@ SmoothTuning smooth_tuning() { return g_tune; }
 
 int rbgs_sweep_dot(const Launch& L, const DevOp& op, Prec p, const void* b, const void* xin,
                    void* xout, int first_color, double* partials, int cap) {
-  const bool ring = g_tune.ring != 0 && g_tune.rows == 2 && g_tune.pair == 0 &&
+  const bool ring = g_tune.ring != 0 && g_tune.rows == 2 && g_tune.pair == 0 && g_tune.dot != 0 &&
                     op.sxy < (1LL << 29) && op.nz > 1 && (p == P32 || p == P16);
   if (!ring) return 0;
+  if (g_tune.reg) {
+    if (op.kind == PDEHIP_OP_CONST)
+      return p == P32 ? launch3_reg<PDEHIP_OP_CONST, P32>(L, op, b, xin, xout, first_color, false, partials, cap)
+                      : launch3_reg<PDEHIP_OP_CONST, P16>(L, op, b, xin, xout, first_color, false, partials, cap);
+    if (op.kind == PDEHIP_OP_FLUX)
+      return p == P32 ? launch3_reg<PDEHIP_OP_FLUX, P32>(L, op, b, xin, xout, first_color, false, partials, cap)
+                      : launch3_reg<PDEHIP_OP_FLUX, P16>(L, op, b, xin, xout, first_color, false, partials, cap);
+    return 0;
+  }
   if (op.kind == PDEHIP_OP_CONST)
     return p == P32 ? launch3_ring<PDEHIP_OP_CONST, P32>(L, op, b, xin, xout, first_color, false, partials, cap)
                     : launch3_ring<PDEHIP_OP_CONST, P16>(L, op, b, xin, xout, first_color, false, partials, cap);

@@ -57,6 +57,8 @@ struct SmoothTuning {
   int ychunk = 64;   // 2D: rows per wavefront (upper bound)
   int pair = 0;      // 3D FP32/FP16: column-pair (divergence-free) tiles
   int ring = 1;      // 3D FP32/FP16: ring-buffered, 6-step unrolled sweep (k_rbgs_ring3)
+  int reg = 1;       // ... with in-plane neighbours in registers, 4 rows per wave (k_rbgs_reg3)
+  int dot = 1;       // the last level-0 sweep of a mixed V-cycle also leaves the r.z partials
   int box = 65;      // 3D FP32/FP16 coarse levels of mixed V-cycles with every side <= box
                      // smooth on boxes (k_box.hip); 0: plane-march sweeps on every level
   int box2 = 65;     // ... two sweeps per box launch where every side is <= box2, else one

@@ -12,7 +12,7 @@ from pde_solver_amd.capi import BC
 pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"smooth.fused": 1, "smooth.rows": 2, "smooth.zchunk": 0, "smooth.ychunk": 64,
-            "smooth.pair": 0, "smooth.ring": 1, "stencil.ring": 1, "smooth.box": 65,
+            "smooth.pair": 0, "smooth.ring": 1, "smooth.reg": 1, "stencil.ring": 1, "smooth.box": 65,
             "smooth.box2": 65, "smooth.box_z": 8, "mixed.fp16": 1}
 
 
@@ -70,13 +70,20 @@ CASES_3D = [
     ("flux", (33, 65, 129), {"rows": 2, "zchunk": 0, "pair": 1}),
     ("flux", (65, 33, 257), {"rows": 2, "zchunk": 8, "pair": 1}),
     ("const", (17, 129, 65), {"rows": 2, "zchunk": 0, "pair": 1}),
-    # FP32/FP16 levels at rows 2 run the ring sweep (k_rbgs_ring3) unless ring = 0: automatic
+    # FP32/FP16 levels at rows 2 run the ring sweep (k_rbgs_ring3) with reg = 0: automatic
     # chunks, a chunk whose step count is not a multiple of the 6-step unroll (dead steps), and
     # the plane-march sweep it replaces
-    ("flux", (33, 65, 129), {"rows": 2, "zchunk": 0, "ring": 1}),
-    ("flux", (65, 33, 257), {"rows": 2, "zchunk": 9, "ring": 1}),
-    ("const", (17, 129, 65), {"rows": 2, "zchunk": 5, "ring": 1}),
+    ("flux", (33, 65, 129), {"rows": 2, "zchunk": 0, "ring": 1, "reg": 0}),
+    ("flux", (65, 33, 257), {"rows": 2, "zchunk": 9, "ring": 1, "reg": 0}),
+    ("const", (17, 129, 65), {"rows": 2, "zchunk": 5, "ring": 1, "reg": 0}),
     ("flux", (33, 65, 129), {"rows": 2, "zchunk": 0, "ring": 0}),
+    ("flux", (37, 37, 37), {"rows": 2, "zchunk": 0, "ring": 1, "reg": 0}),
+    # the register-row sweep (k_rbgs_reg3, the default): automatic chunks, dead steps, short
+    # chunks, sides 4m+1 (1/h^2 not a power of two)
+    ("flux", (33, 65, 129), {"rows": 2, "zchunk": 0, "reg": 1}),
+    ("flux", (65, 33, 257), {"rows": 2, "zchunk": 9, "reg": 1}),
+    ("const", (17, 129, 65), {"rows": 2, "zchunk": 5, "reg": 1}),
+    ("flux", (69, 61, 37), {"rows": 2, "zchunk": 0, "reg": 1}),
 ]
 
 
@@ -115,6 +122,8 @@ BOX_CASES = [
     ("const", (65, 65, 65), 4, 0, 2, 2, 65),
     ("flux", (33, 65, 129), 8, 1, 1, 3, 65),
     ("flux", (65, 33, 129), 4, 1, 3, 1, 65),
+    # sides 4m+1 whose third level is even and whose 1/h^2 is not a power of two
+    ("flux", (69, 61, 37), 8, 1, 2, 2, 65),
     ("flux", (65, 129, 129), 8, 1, 2, 2, 33),
     ("const", (129, 65, 129), 4, 0, 3, 2, 0),
 ]
