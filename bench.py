@@ -753,7 +753,8 @@ def measure_traffic(n: int):
                 name = row.get("Kernel_Name", "")
                 if row.get("Counter_Name") != counter:
                     continue
-                key = "sweep" if "k_rbgs_ring3" in name else ("copy" if "k_convert" in name else None)
+                sweep = "k_rbgs_reg3" in name or "k_rbgs_ring3" in name
+                key = "sweep" if sweep else ("copy" if "k_convert" in name else None)
                 if key:
                     vals.setdefault(key, []).append(float(row["Counter_Value"]))
         if "sweep" not in vals or "copy" not in vals:
@@ -1009,7 +1010,8 @@ def main() -> int:
     ms_rs, bytes_rs = plan64.time_kernel(2, capi.FP64, reps=20)
     ms_cp, bytes_cp = plan64.time_kernel(3, capi.FP64, reps=20)
     gbs = lambda b, ms: b / (ms * 1e-3) / 1e9  # noqa: E731
-    roof = {"bound": "hbm", "kernel": f"k_rbgs_ring3 (red+black sweep, one launch), FP32, fine level {n}^3",
+    sweep_kernel = "k_rbgs_reg3" if hip.tune_get("smooth.reg") else "k_rbgs_ring3"
+    roof = {"bound": "hbm", "kernel": f"{sweep_kernel} (red+black sweep, one launch), FP32, fine level {n}^3",
             "achieved": round(gbs(bytes_mx, ms_mx), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs(bytes_mx, ms_mx) / HBM_PEAK_GBS, 4),
             "traffic": round(traffic) if traffic else None,

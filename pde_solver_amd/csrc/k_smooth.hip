@@ -486,7 +486,9 @@ k_rbgs_ring3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin
 //     and stores on per-plane descriptors, colour updates computed on every lane and selected,
 //     as in k_rbgs_ring3.
 // One barrier per plane: x slots are rewritten three planes later, rho slots two, face slots
-// three, each after a barrier that every reader of the old contents has passed.
+// three, each after a barrier that every reader of the old contents has passed.  A step forms
+// the faces of the next plane after its colour stages, so the register rings hold two face
+// planes and three front-face planes.
 constexpr int kRR = 4;
 
 __device__ __forceinline__ float dpp_west(float v) {   // lane l gets lane l-1 (lane 0: itself)
@@ -501,7 +503,7 @@ __device__ __forceinline__ float dpp_east(float v) {   // lane l gets lane l+1 (
 }
 
 template <int KIND, Prec P, bool ZERO, bool DOT = false>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads, 2)   // two workgroups per CU: at most 128 VGPRs
 k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v,
             void* __restrict__ xout_v, int c1, int tiles_x, int tiles_y, int ntiles,
             int kchunk, const KScalars* sc, double* partials = nullptr) {
@@ -583,9 +585,11 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
   };
 
   // Rings, with plane pstart + s at slot s mod L: U (6) x(p-2 .. p+3), B (3) b(p-1 .. p+1),
-  // K (3) rho(p+1 .. p+3), ZF (6) zf(p-2 .. p+1), FE / FN (3) faces of planes p-1 .. p+1.
+  // K (3) rho(p+1 .. p+3), ZF (3) zf(p-2 .. p) and FE / FN (2) faces of planes p-1 .. p during
+  // the colour stages of step p; the faces of plane p+1 are formed after them, in the slots the
+  // second stage has just finished with.
   const int pstart = HK ? k0 - 3 : k0 - 1;
-  T U[R][6], B[R][3], K[R][3], ZF[R][6], FE[R][3], FN[R][3];
+  T U[R][6], B[R][3], K[R][3], ZF[R][3], FE[R][2], FN[R][2];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     U[r][4] = ld_u(r, pstart - 2);
@@ -601,9 +605,9 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
     K[r][1] = ld_k(r, pstart + 1);
     K[r][2] = ld_k(r, pstart + 2);
 #pragma unroll
-    for (int s = 0; s < 6; ++s) ZF[r][s] = T(0);
+    for (int s = 0; s < 3; ++s) ZF[r][s] = T(0);
 #pragma unroll
-    for (int s = 0; s < 3; ++s) { FE[r][s] = T(0); FN[r][s] = T(0); }
+    for (int s = 0; s < 2; ++s) { FE[r][s] = T(0); FN[r][s] = T(0); }
   }
 
   auto step = [&](auto phc, int p) {
@@ -612,8 +616,9 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
                   u_3 = (PH + 3) % 6;
     constexpr int b_m1 = (PH + 2) % 3, b_0 = PH % 3, b_1 = (PH + 1) % 3;
     constexpr int k_1 = (PH + 1) % 3, k_2 = (PH + 2) % 3, k_3 = PH % 3;
-    constexpr int z_m2 = (PH + 4) % 6, z_m1 = (PH + 5) % 6, z_0 = PH % 6, z_1 = (PH + 1) % 6;
-    constexpr int f_p1 = (PH + 1) % 3, f_0 = PH % 3, f_m1 = (PH + 2) % 3;
+    constexpr int z_m2 = (PH + 1) % 3, z_m1 = (PH + 2) % 3, z_0 = PH % 3;
+    constexpr int f_0 = PH % 2, f_m1 = (PH + 1) % 2;
+    constexpr int g_p1 = (PH + 1) % 3, g_0 = PH % 3, g_m1 = (PH + 2) % 3;   // lfn slots
     constexpr int sx = PH % 3, sxq = (PH + 2) % 3, sk = PH % 2;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -625,18 +630,6 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
     lxb[sx][wv][1][lane] = U[R - 1][u_0];
     if constexpr (HK) lkb[sk][wv][lane] = K[0][k_1];
     __syncthreads();
-    if constexpr (HK) {
-      // faces of plane p+1: east and north of every node, front in the ring
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const T kc = K[r][k_1];
-        ZF[r][z_1] = face_smooth<T>(kc, K[r][k_2], fz);
-        FE[r][f_p1] = face_smooth<T>(kc, dpp_east(kc), fx);
-        const T kn = r < R - 1 ? K[r + 1][k_1] : lkb[sk][wup][lane];
-        FN[r][f_p1] = face_smooth<T>(kc, kn, fy);
-      }
-      lfn[f_p1][wv][lane] = top_ok ? FN[R - 1][f_p1] : T(0);
-    }
     // first colour on plane p (tile + ring)
     if (p >= k0 - 1 && p >= 1 && p <= nz - 2) {
       const int cp = (par0 + p) & 1;
@@ -649,7 +642,7 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
           ae = FE[r][f_0];
           aw = dpp_west(ae);
           an = FN[r][f_0];
-          as = r > 0 ? FN[r - 1][f_0] : lfn[f_0][wdn][lane];
+          as = r > 0 ? FN[r - 1][f_0] : lfn[g_0][wdn][lane];
         }
         W7<T> w = weights_at<3, KIND, T>(op, 0, aw, ae, as, an, ZF[r][z_m1], ZF[r][z_0]);
         if (((m_fxy >> r) & 1u) || p == 1 || p == nz - 2) fold_center<3, T>(op, i, j, p, w);
@@ -677,7 +670,7 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
             ae = FE[r][f_m1];
             aw = dpp_west(ae);
             an = FN[r][f_m1];
-            as = r > 0 ? FN[r - 1][f_m1] : lfn[f_m1][wdn][lane];
+            as = r > 0 ? FN[r - 1][f_m1] : lfn[g_m1][wdn][lane];
           }
           W7<T> w = weights_at<3, KIND, T>(op, 0, aw, ae, as, an, ZF[r][z_m2], ZF[r][z_m1]);
           if (((m_fxy >> r) & 1u) || q == 1 || q == nz - 2) fold_center<3, T>(op, i, j, q, w);
@@ -698,6 +691,19 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
         for (int r = 0; r < R; ++r)
           if (off_st[r] != kDropOffset) dacc += double(B[r][b_m1]) * double(U[r][u_m1]);
       }
+    }
+    if constexpr (HK) {
+      // faces of plane p+1: east and north of every node, front in the ring (into the slots of
+      // plane p-1, whose last reader was the second colour above)
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const T kc = K[r][k_1];
+        ZF[r][z_m2] = face_smooth<T>(kc, K[r][k_2], fz);
+        FE[r][f_m1] = face_smooth<T>(kc, dpp_east(kc), fx);
+        const T kn = r < R - 1 ? K[r + 1][k_1] : lkb[sk][wup][lane];
+        FN[r][f_m1] = face_smooth<T>(kc, kn, fy);
+      }
+      lfn[g_p1][wv][lane] = top_ok ? FN[R - 1][f_m1] : T(0);
     }
   };
 
