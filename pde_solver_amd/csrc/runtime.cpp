@@ -404,6 +404,8 @@ Plan::~Plan() {
   if (hstage) (void)hipHostFree(hstage);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
+  for (hipEvent_t e : evq)
+    if (e) (void)hipEventDestroy(e);
 }
 
 static bool all_dirichlet(const pdehip_problem& p, const Geo& g) {
@@ -598,7 +600,7 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
   P->part2.alloc(ctx, (size_t)np * sizeof(double));
   P->part3.alloc(ctx, (size_t)2 * np * sizeof(double) + 64);
   P->scal.alloc(ctx, 3 * sizeof(KScalars));
-  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&P->hsc), 3 * sizeof(KScalars)));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&P->hsc), 9 * sizeof(KScalars)));
   {
     DevOp rop = P->op();
     const size_t nst = (size_t)partial_blocks_max(P->op()) + 2 * (size_t)resnorm_blocks(rop) + 8;
@@ -606,6 +608,7 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
   }
   HIPCHK(hipEventCreate(&P->ev0));
   HIPCHK(hipEventCreate(&P->ev1));
+  for (hipEvent_t& e : P->evq) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(hipStreamSynchronize(s));
   return P;
 }
@@ -677,6 +680,64 @@ static const KScalars& pull(Plan& P, int i) {
 }
 
 static bool cancelled(const Plan& P) { return P.opt.cancel && *P.opt.cancel; }
+
+// Convergence checks that trail the device by one chunk.  After each chunk of iterations the
+// plan's first `nsc` scalar blocks are copied to a pinned slot and an event is recorded; the host
+// reads the previous chunk's copy while the current chunk runs, so the GPU does not idle through
+// a host round trip per chunk.  When the residual trend says the chunk in flight may reach the
+// target, the host waits for it before queuing the next one.  A chunk queued behind a converged
+// one finds done set on the device and each of its kernels returns at once, so results never
+// depend on how far ahead the host ran.
+class TrailingCheck {
+ public:
+  // ratio(k): residual over its target after a chunk (from the copied scalar blocks)
+  TrailingCheck(Plan& P, int nsc, int every) : P_(P), nsc_(nsc), every_(every) {}
+  // Called after a chunk has been queued.  Returns true when the loop must end: a checked chunk
+  // converged or broke down, or the solve was cancelled.
+  template <typename Stop, typename Ratio>
+  bool after_chunk(Stop stop, Ratio ratio) {
+    const int cur = slot_;
+    HIPCHK(hipMemcpyAsync(&P_.hsc[3 + 3 * cur], P_.sc(0), nsc_ * sizeof(KScalars),
+                          hipMemcpyDeviceToHost, P_.stream()));
+    HIPCHK(hipEventRecord(P_.evq[cur], P_.stream()));
+    slot_ ^= 1;
+    if (pending_ >= 0) {
+      if (check(pending_, stop, ratio)) return true;
+      pending_ = -1;
+    }
+    if (ahead_ok()) {
+      pending_ = cur;
+      return false;
+    }
+    return check(cur, stop, ratio);
+  }
+
+ private:
+  template <typename Stop, typename Ratio>
+  bool check(int sl, Stop stop, Ratio ratio) {
+    HIPCHK(hipEventSynchronize(P_.evq[sl]));
+    const KScalars* k = &P_.hsc[3 + 3 * sl];
+    if (stop(k) || cancelled(P_)) return true;
+    hist_.push_back(ratio(k));
+    return false;
+  }
+  // the chunk in flight cannot converge even if the residual fell `kFast` per iteration (or
+  // faster, as observed) from the last checked value: at least a factor 10 would remain
+  bool ahead_ok() const {
+    if (hist_.empty()) return false;
+    double q = 0.05;
+    for (size_t i = 1; i < hist_.size(); ++i)
+      if (hist_[i - 1] > 0.0 && hist_[i] > 0.0)
+        q = std::min(q, std::pow(hist_[i] / hist_[i - 1], 1.0 / every_));
+    q = std::max(q, 1e-3);
+    const double r = hist_.back();
+    return std::isfinite(r) && r * std::pow(q, every_) > 10.0;
+  }
+  Plan& P_;
+  int nsc_, every_;
+  int slot_ = 0, pending_ = -1;
+  std::vector<double> hist_;
+};
 
 static int check_every(const Plan& P) {
   if (P.opt.check_every > 0) return P.opt.check_every;
@@ -856,11 +917,12 @@ static void solve_cg(Plan& P, bool use_mg, pdehip_report* rep) {
   const int every = check_every(P);
   const int max_iter = std::max(1, P.opt.max_iter);
   int launched = 0;
+  TrailingCheck tc(P, 1, every);
   while (launched < max_iter) {
     const int chunk = std::min(every, max_iter - launched);
     for (int c = 0; c < chunk; ++c) {
       // The previous iteration's preconditioning and direction update open this one, so a
-      // chunk ends at the stopping test and no launch is queued behind the converged one.
+      // chunk ends at the stopping test.
       if (launched + c > 0) {
         if (use_mg) {
           vcycle(P, L, P.r.get(), z);
@@ -875,8 +937,9 @@ static void solve_cg(Plan& P, bool use_mg, pdehip_report* rep) {
       finalize(L, sc, part, vec_blocks(n), FIN_CG_RR, 0, 0, nullptr);
     }
     launched += chunk;
-    const KScalars& k = pull(P, 0);
-    if (k.done || k.breakdown || cancelled(P)) break;
+    if (tc.after_chunk([](const KScalars* k) { return k[0].done || k[0].breakdown; },
+                       [](const KScalars* k) { return k[0].metric / k[0].target; }))
+      break;
   }
   const KScalars& k = pull(P, 0);
   report_from(k, rep);
@@ -939,6 +1002,7 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
   const int every = check_every(P);
   const int max_iter = std::max(1, P.opt.max_iter);
   int launched = 0;
+  TrailingCheck tc(P, 2, every);
   while (launched < max_iter) {
     const int chunk = std::min(every, max_iter - launched);
     for (int c = 0; c < chunk; ++c) {
@@ -961,8 +1025,10 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
       finalize_ru(s, si, so, gate, part, partial_blocks(op64), FIN_RU_APPLY, delta);
     }
     launched += chunk;
-    const KScalars& k = pull(P, 1);
-    if (k.done || k.breakdown || cancelled(P)) break;
+    // blocks 0 (outer) and 1 (inner): the inner residual is relative to the outer snorm
+    if (tc.after_chunk([](const KScalars* k) { return k[1].done || k[1].breakdown; },
+                       [](const KScalars* k) { return k[1].metric * k[0].snorm / k[0].target; }))
+      break;
   }
   // an unconverged solve keeps the correction accumulated since the last replacement
   flush_f32(s, x64, x32, n, &so->snorm, so);

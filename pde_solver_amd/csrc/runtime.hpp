@@ -161,11 +161,12 @@ struct Plan {
   DBuf part, part2, part3;     // reduction partials
   int npart = 0;
   DBuf scal;                   // KScalars x 2 (outer/inner)
-  KScalars* hsc = nullptr;     // pinned host mirror
+  KScalars* hsc = nullptr;     // pinned host mirror (3 scalar blocks, then 2 x 3 trailing-check slots)
   double* hstage = nullptr;    // pinned staging of the report's reductions
   bool fin_resid = false;      // plan_finish_enqueue staged the true-residual partials
   bool resid_fresh = false;    // the solver left r = b - A x of the final x and its partials
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evq[2] = {nullptr, nullptr};   // trailing convergence checks
 
   ~Plan();
   KScalars* sc(int i = 0) const { return scal.as<KScalars>() + i; }
