@@ -255,8 +255,9 @@ void pdehip_plan_destroy(pdehip_plan* plan);
  * V-cycles whose sides are all at most this smooth on LDS boxes, the prolongation folded into
  * the first post-smoothing launch; 0: off), "smooth.box2" (0..1025, default 65: of those, the
  * levels whose sides are all at most this take two sweeps per launch, the others one),
- * "smooth.box_z" (4 or 8, default 8: interior planes per box).  Results do not depend on any of
- * these.
+ * "smooth.box_z" (4 or 8, default 8: interior planes per box), "smooth.box_rr" (0..4097,
+ * default 65: mixed 3D levels whose sides are all at most this form their residual and its
+ * restriction in one launch; 0: off).  Results do not depend on any of these.
  * "time.fused" (0/1, default 1): RK4 time series take one launch per step (the four stages and
  * the closures, temporally blocked in LDS) instead of one per stage; the same bits either way.
  * "mixed.fp16" (0/1, default 1): mixed-precision plans created afterwards store their coarse

@@ -187,6 +187,20 @@ __device__ __forceinline__ T apply_diff(const DevOp& op, const W7<T>& w, T e, in
   return a;
 }
 
+// A x on one interior row as a fixed FMA chain (centre, E, W, N, S, F, B), shared by every
+// kernel that forms the operator from weights at hand, so they round it the same way.
+template <typename T>
+__device__ __forceinline__ T apply_row(const W7<T>& w, T xc, T xe, T xw, T xn, T xs, T xf, T xb) {
+  T a = w.C * xc;
+  a = fma(w.E, xe, a);
+  a = fma(w.W, xw, a);
+  a = fma(w.N, xn, a);
+  a = fma(w.S, xs, a);
+  a = fma(w.F, xf, a);
+  a = fma(w.B, xb, a);
+  return a;
+}
+
 // Gauss-Seidel value of one interior row: (b - sum_nb w_nb u_nb) * recip(C), the neighbour sum
 // as a fixed FMA chain (E, W, N, S, F, B).  Every smoother kernel uses this, so the fused and
 // the in-place sweeps agree bit for bit whatever the compiler's contraction choices.

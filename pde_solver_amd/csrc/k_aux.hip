@@ -100,16 +100,6 @@ int zchunk_for(int gx, int gy, int nz) {
   return zc;
 }
 
-// Per-axis fold list of coarse coordinate c on an axis of n nodes: the node itself (weight 1)
-// plus the face node it absorbs (weight theta) when it is the first/last interior node.
-__device__ __forceinline__ int fold_list(int c, int n, double th_lo, double th_hi, int* idx, double* w) {
-  idx[0] = c; w[0] = 1.0;
-  int m = 1;
-  if (c == 1) { idx[m] = 0; w[m] = th_lo; ++m; }
-  if (c == n - 2) { idx[m] = n - 1; w[m] = th_hi; ++m; }
-  return m;
-}
-
 // 3 x 3 in-plane full weighting of fine plane `pl` at (fi, fj), clipped at the plane edges.
 template <typename SF>
 __device__ __forceinline__ double fw2(const DevOp& f, const SF* pl, int fi, int fj) {
@@ -191,24 +181,9 @@ __global__ void __launch_bounds__(kTX * kTY) k_restrict(DevOp f, const void* rfv
   }
 }
 
-// 3D restriction, one coarse node per thread (no z march, so no load waits on a carried
-// value): the 27-point full weighting of an interior coarse node reads fine nodes that are all
-// in range, so its 27 loads are issued unconditionally and wait once.  Coarse nodes next to a
-// face add the folded closure terms through the general path (fold lists, clipped weighting).
-template <typename SF>
-__device__ __forceinline__ double fw2_in(const SF* __restrict__ pl, int nx, int fi, int fj) {
-  const SF* r0 = pl + (long long)(fj - 1) * nx + fi;
-  const SF* r1 = r0 + nx;
-  const SF* r2 = r1 + nx;
-  const double a0 = 0.5 * ldd(r0, 0) + 0.25 * ldd(r0, -1) + 0.25 * ldd(r0, 1);
-  const double a1 = 0.5 * ldd(r1, 0) + 0.25 * ldd(r1, -1) + 0.25 * ldd(r1, 1);
-  const double a2 = 0.5 * ldd(r2, 0) + 0.25 * ldd(r2, -1) + 0.25 * ldd(r2, 1);
-  return 0.25 * a0 + 0.5 * a1 + 0.25 * a2;
-}
-
-// One coarse node of the 3D restriction.  RIM: the node may sit next to an x/y face, so the
-// folded closure terms are taken through the general path; interior nodes (RIM false) never
-// need it, and their waves carry no folded lanes.
+// One coarse node of the 3D restriction (restrict_value, transfer_dev.hpp) reading the fine
+// residual from memory; coarse boundary nodes are zero.  The 27 loads of an interior node are
+// issued unconditionally and wait once.
 template <typename SF, typename SC, bool RIM>
 __device__ __forceinline__ void restrict3_node(const DevOp& f, const SF* __restrict__ rf, const DevOp& c,
                                                SC* __restrict__ bc, int ic, int jc, int kc, double s) {
@@ -218,39 +193,8 @@ __device__ __forceinline__ void restrict3_node(const DevOp& f, const SF* __restr
     return;
   }
   const long long fsxy = (long long)f.nx * f.ny;
-  const int fi = 2 * ic, fj = 2 * jc, fk = 2 * kc;
-  double ql = fw2_in(rf + (fk - 1) * fsxy, f.nx, fi, fj);
-  double qm = fw2_in(rf + fk * fsxy, f.nx, fi, fj);
-  double qh = fw2_in(rf + (fk + 1) * fsxy, f.nx, fi, fj);
-  bool fold_xy = false;
-  int xi[3], yi[3];
-  double xw[3], yw[3];
-  int mx = 1, my = 1;
-  if constexpr (RIM) {
-    fold_xy = ic == 1 || jc == 1 || ic == c.nx - 2 || jc == c.ny - 2;
-    if (fold_xy) {
-      mx = fold_list(ic, c.nx, c.theta[0], c.theta[1], xi, xw);
-      my = fold_list(jc, c.ny, c.theta[2], c.theta[3], yi, yw);
-    }
-  }
-  // in-plane folded weighting of fine plane fk (the general form: every fold-list pair)
-  auto Q = [&](int k) -> double {
-    const SF* pl = rf + k * fsxy;
-    double acc = 0.0;
-    for (int b = 0; b < my; ++b)
-      for (int q = 0; q < mx; ++q) {
-        const double w = yw[b] * xw[q];
-        if (w != 0.0) acc += w * fw2(f, pl, 2 * xi[q], 2 * yi[b]);
-      }
-    return acc;
-  };
-  if (fold_xy) { ql = Q(fk - 1); qm = Q(fk); qh = Q(fk + 1); }
-  double v = 0.25 * ql + 0.5 * qm + 0.25 * qh;
-  if (kc == 1 && c.theta[4] != 0.0)
-    v += c.theta[4] * (0.5 * (fold_xy ? Q(0) : fw2_in(rf, f.nx, fi, fj)) + 0.25 * ql);
-  if (kc == c.nz - 2 && c.theta[5] != 0.0)
-    v += c.theta[5] * (0.25 * qh + 0.5 * (fold_xy ? Q(fk + 2) : fw2_in(rf + (fk + 2) * fsxy, f.nx, fi, fj)));
-  bc[out] = store_as<SC>(v * s);
+  auto r = [&](int i, int j, int k) -> double { return ldd(rf, k * fsxy + (long long)j * f.nx + i); };
+  bc[out] = store_as<SC>(restrict_value<RIM>(f, c, r, ic, jc, kc, s));
 }
 
 // Rim of a coarse plane: the nodes outside the interior box [2, nx-3] x [2, ny-3], numbered

@@ -228,6 +228,158 @@ k_box_rbgs(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v
   }
 }
 
+// ------------------------------------------------------------------ residual + restriction
+// One workgroup per box of kCX x kCY x kCZ coarse nodes: the fine residual r = b - A x (the
+// plane kernels' per-node arithmetic: faces by face_coef, apply_row, r rounded to the storage
+// type) on every fine node the box's full weighting reads, the face nodes its folded closure
+// terms read included, kept in LDS; then each coarse node's value (restrict_value,
+// transfer_dev.hpp) from there.  The fine residual never goes to memory, and a level's residual
+// and restriction take one launch.
+constexpr int kCX = 8, kCY = 4, kCZ = 4;
+constexpr int kRRThreads = 256;
+
+template <int KIND, Prec PF, Prec PC>
+__global__ void __launch_bounds__(kRRThreads)
+k_box_rr(DevOp f, const void* __restrict__ bv, const void* __restrict__ xv, DevOp c,
+         void* __restrict__ bcv, double scale, const double* dscale, int ncx, int ncy,
+         int nboxes, const KScalars* sc) {
+  using T = typename PT<PF>::T;
+  using S = typename PT<PF>::S;
+  using SC = typename PT<PC>::S;
+  static_assert(sizeof(T) == 4, "box residual-restriction: 4-byte arithmetic types only");
+  constexpr bool HK = KIND == PDEHIP_OP_FLUX;
+  // fine residual region [2*I0 - 2, 2*(I0 + kCX - 1) + 2] per axis, x region one node wider
+  constexpr int RX = 2 * kCX + 3, RY = 2 * kCY + 3, RZ = 2 * kCZ + 3, RN = RX * RY * RZ;
+  constexpr int XX = RX + 2, XY = RY + 2, XZ = RZ + 2, XXY = XX * XY, XN = XXY * XZ;
+  __shared__ S xs[XN];
+  __shared__ T rs[HK ? XN : 1];
+  __shared__ S bs[RN];
+  __shared__ S res[RN];
+  if (skip(sc)) return;
+  const int t = xcd_tile();
+  if (t >= nboxes) return;
+  const int I0 = (t % ncx) * kCX, rest = t / ncx, J0 = (rest % ncy) * kCY, K0 = (rest / ncy) * kCZ;
+  const int rx0 = 2 * I0 - 2, ry0 = 2 * J0 - 2, rz0 = 2 * K0 - 2;   // residual region origin
+  const int nx = f.nx, ny = f.ny, nz = f.nz;
+  const S* __restrict__ b = static_cast<const S*>(bv);
+  const S* __restrict__ x = static_cast<const S*>(xv);
+  const T* __restrict__ kap = static_cast<const T*>(f.kappa);
+  SC* __restrict__ bc = static_cast<SC*>(bcv);
+  const int tid = static_cast<int>(threadIdx.x);
+  auto gid = [&](int i, int j, int k) -> long long { return ((long long)k * ny + j) * nx + i; };
+  auto in_grid = [&](int i, int j, int k) {
+    return i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz;
+  };
+
+  // loads: x and rho on the x region, b on the residual region, every load before any store
+  constexpr int NX = (XN + kRRThreads - 1) / kRRThreads;
+  constexpr int NB = (RN + kRRThreads - 1) / kRRThreads;
+  S xv_[NX], bv_[NB];
+  T rv[HK ? NX : 1];
+  bool inx[NX], inb[NB];
+#pragma unroll
+  for (int q = 0; q < NX; ++q) {
+    const int e = min(tid + q * kRRThreads, XN - 1);
+    const int gi = rx0 - 1 + e % XX, gj = ry0 - 1 + (e / XX) % XY, gk = rz0 - 1 + e / XXY;
+    inx[q] = in_grid(gi, gj, gk);
+    const long long id = inx[q] ? gid(gi, gj, gk) : 0;
+    xv_[q] = x[id];
+    if constexpr (HK) rv[q] = kap[id];
+  }
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int e = min(tid + q * kRRThreads, RN - 1);
+    const int gi = rx0 + e % RX, gj = ry0 + (e / RX) % RY, gk = rz0 + e / (RX * RY);
+    inb[q] = in_grid(gi, gj, gk);
+    bv_[q] = b[inb[q] ? gid(gi, gj, gk) : 0];
+  }
+#pragma unroll
+  for (int q = 0; q < NX; ++q) {
+    const int e = tid + q * kRRThreads;
+    if (XN % kRRThreads == 0 || e < XN) {
+      xs[e] = inx[q] ? xv_[q] : S(0.0f);
+      if constexpr (HK) rs[e] = inx[q] ? rv[q] : T(0);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int e = tid + q * kRRThreads;
+    if (RN % kRRThreads == 0 || e < RN) bs[e] = inb[q] ? bv_[q] : S(0.0f);
+  }
+  __syncthreads();
+
+  // fine residual on the residual region (identity rows on the boundary, zero off the grid)
+  const T fx = T(f.fx), fy = T(f.fy), fz = T(f.fz);
+  for (int e = tid; e < RN; e += kRRThreads) {
+    const int lx = e % RX, ly = (e / RX) % RY, lz = e / (RX * RY);
+    const int gi = rx0 + lx, gj = ry0 + ly, gk = rz0 + lz;
+    T out = T(0);
+    if (in_grid(gi, gj, gk)) {
+      const int ex = ((lz + 1) * XY + (ly + 1)) * XX + (lx + 1);
+      const T xc = back<S>(xs[ex]);
+      const T bb = back<S>(bs[e]);
+      if (gi == 0 || gj == 0 || gk == 0 || gi == nx - 1 || gj == ny - 1 || gk == nz - 1) {
+        out = T(bb - xc);
+      } else {
+        T aw = T(0), ae = T(0), as = T(0), an = T(0), ab = T(0), af = T(0);
+        if constexpr (HK) {
+          const T rc = rs[ex];
+          aw = face_coef<T>(rc, rs[ex - 1], fx);
+          ae = face_coef<T>(rc, rs[ex + 1], fx);
+          as = face_coef<T>(rc, rs[ex - XX], fy);
+          an = face_coef<T>(rc, rs[ex + XX], fy);
+          ab = face_coef<T>(rc, rs[ex - XXY], fz);
+          af = face_coef<T>(rc, rs[ex + XXY], fz);
+        }
+        W7<T> w = weights_at<3, KIND, T>(f, 0, aw, ae, as, an, ab, af);
+        fold_center<3, T>(f, gi, gj, gk, w);
+        const T ax = apply_row<T>(w, xc, back<S>(xs[ex + 1]), back<S>(xs[ex - 1]),
+                                  back<S>(xs[ex + XX]), back<S>(xs[ex - XX]),
+                                  back<S>(xs[ex + XXY]), back<S>(xs[ex - XXY]));
+        out = T(bb - ax);
+      }
+    }
+    res[e] = store_as<S>(static_cast<double>(out));
+  }
+  __syncthreads();
+
+  // the box's coarse nodes
+  const double s = dscale ? scale * (*dscale) : scale;
+  auto r = [&](int i, int j, int k) -> double {
+    return static_cast<double>(back<S>(res[((k - rz0) * RY + (j - ry0)) * RX + (i - rx0)]));
+  };
+  for (int q = tid; q < kCX * kCY * kCZ; q += kRRThreads) {
+    const int ic = I0 + q % kCX, jc = J0 + (q / kCX) % kCY, kc = K0 + q / (kCX * kCY);
+    if (ic >= c.nx || jc >= c.ny || kc >= c.nz) continue;
+    const long long out = ((long long)kc * c.ny + jc) * c.nx + ic;
+    if (ic == 0 || jc == 0 || kc == 0 || ic == c.nx - 1 || jc == c.ny - 1 || kc == c.nz - 1) {
+      bc[out] = store_as<SC>(0.0);
+      continue;
+    }
+    bc[out] = store_as<SC>(restrict_value<true>(f, c, r, ic, jc, kc, s));
+  }
+}
+
+template <int KIND, Prec PF, Prec PC>
+void launch_rr(const Launch& L, const DevOp& f, const void* b, const void* x, const DevOp& c,
+               void* bc, double scale, const double* dscale) {
+  const int ncx = (c.nx + kCX - 1) / kCX, ncy = (c.ny + kCY - 1) / kCY, ncz = (c.nz + kCZ - 1) / kCZ;
+  const int nboxes = ncx * ncy * ncz;
+  const dim3 g(static_cast<unsigned>(8 * ((nboxes + 7) / 8)));
+  hipLaunchKernelGGL((k_box_rr<KIND, PF, PC>), g, dim3(kRRThreads), 0, L.stream, f, b, x, c, bc,
+                     scale, dscale, ncx, ncy, nboxes, L.sc);
+}
+
+template <int KIND>
+bool rr_by_prec(const Launch& L, const DevOp& f, Prec pf, const void* b, const void* x,
+                const DevOp& c, Prec pc, void* bc, double scale, const double* dscale) {
+  if (pf == P32 && pc == P32) launch_rr<KIND, P32, P32>(L, f, b, x, c, bc, scale, dscale);
+  else if (pf == P32 && pc == P16) launch_rr<KIND, P32, P16>(L, f, b, x, c, bc, scale, dscale);
+  else if (pf == P16 && pc == P16) launch_rr<KIND, P16, P16>(L, f, b, x, c, bc, scale, dscale);
+  else return false;
+  return true;
+}
+
 template <int KIND, Prec P, int NS, int BZ, int IN>
 void launch(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout, int c1,
             const DevOp& cop, const void* uc, double factor, const double* dscale) {
@@ -262,6 +414,18 @@ void by_shape(const Launch& L, const DevOp& op, const void* b, const void* xin, 
 }
 
 }  // namespace
+
+bool box_resrestrict(const Launch& L, const DevOp& fine, Prec pf, const void* b, const void* x,
+                     const DevOp& coarse, Prec pc, void* bc, double scale, const double* dev_scale) {
+  if (fine.nz < 3 || coarse.nz < 3 || fine.n >= (1LL << 31)) return false;
+  if (2 * coarse.nx - 1 != fine.nx || 2 * coarse.ny - 1 != fine.ny || 2 * coarse.nz - 1 != fine.nz)
+    return false;
+  if (fine.kind == PDEHIP_OP_CONST)
+    return rr_by_prec<PDEHIP_OP_CONST>(L, fine, pf, b, x, coarse, pc, bc, scale, dev_scale);
+  if (fine.kind == PDEHIP_OP_FLUX)
+    return rr_by_prec<PDEHIP_OP_FLUX>(L, fine, pf, b, x, coarse, pc, bc, scale, dev_scale);
+  return false;
+}
 
 bool box_sweeps(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x_in,
                 void* x_out, int first_color, int sweeps, int in_mode, const DevOp* coarse,

@@ -188,9 +188,8 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
             w.C = T(op.e) - sum;
           }
           if (((m_fxy >> r) & 1u) || p == 1 || p == nz - 2) fold_center<3, T>(op, i, j, p, w);
-          const T ax = w.C * xc + w.E * lx[sl][lr][lane + 1] + w.W * lx[sl][lr][lane - 1] +
-                       w.N * lx[sl][lr + 1][lane] + w.S * lx[sl][lr - 1][lane] + w.F * X[r][2] +
-                       w.B * X[r][0];
+          const T ax = apply_row<T>(w, xc, lx[sl][lr][lane + 1], lx[sl][lr][lane - 1],
+                                    lx[sl][lr + 1][lane], lx[sl][lr - 1][lane], X[r][2], X[r][0]);
           out = MODE == 0 ? ax : T(B[r][0] - ax);
         }
         y[p * sxy + col[r]] = store_as<S>(static_cast<double>(out));
@@ -362,9 +361,9 @@ k_plane3r(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, vo
           w.C = T(op.e) - sum;
         }
         if (((m_fxy >> r) & 1u) || p == 1 || p == nz - 2) fold_center<3, T>(op, i, j, p, w);
-        const T ax = w.C * xc + w.E * lx[sl][lr + 1][lane + 2] + w.W * lx[sl][lr + 1][lane] +
-                     w.N * lx[sl][lr + 2][lane + 1] + w.S * lx[sl][lr][lane + 1] + w.F * X[r][x_1] +
-                     w.B * X[r][x_m1];
+        const T ax = apply_row<T>(w, xc, lx[sl][lr + 1][lane + 2], lx[sl][lr + 1][lane],
+                                  lx[sl][lr + 2][lane + 1], lx[sl][lr][lane + 1], X[r][x_1],
+                                  X[r][x_m1]);
         const bool bnd = ((m_bxy >> r) & 1u) || p == 0 || p == nz - 1;
         const T ain = bnd ? xc : ax;
         const T out = MODE == 0 ? ain : T(B[r][b_0] - ain);

@@ -63,6 +63,8 @@ struct SmoothTuning {
                      // smooth on boxes (k_box.hip); 0: plane-march sweeps on every level
   int box2 = 65;     // ... two sweeps per box launch where every side is <= box2, else one
   int box_z = 8;     // interior planes of a box (4 or 8)
+  int box_rr = 65;   // 3D FP32/FP16 levels of mixed V-cycles with every side <= box_rr form
+                     // the residual and its restriction in one launch (k_box.hip); 0: off
 };
 void set_smooth_tuning(const SmoothTuning& t);
 SmoothTuning smooth_tuning();
@@ -75,6 +77,12 @@ SmoothTuning smooth_tuning();
 bool box_sweeps(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x_in,
                 void* x_out, int first_color, int sweeps, int in_mode, const DevOp* coarse,
                 Prec pc, const void* uc, double factor, const double* dev_scale);
+// restrict_fw of the residual b - A x of a 3D FP32/FP16 CONST or FLUX level in one launch
+// (k_box.hip): the fine residual is formed in LDS and never stored.  The coarse values equal
+// what residual() then restrict_fw() leave.  Returns false (nothing launched) when the level
+// pair is not one the kernel takes.
+bool box_resrestrict(const Launch& L, const DevOp& fine, Prec pf, const void* b, const void* x,
+                     const DevOp& coarse, Prec pc, void* bc, double scale, const double* dev_scale);
 // Number of blocks `partials`-producing kernels use for this op (<= kMaxPartials).
 int partial_blocks(const DevOp& op);
 // 3D apply / residual as plane marches (k_plane.hip); apply()/residual() route 3D levels there

@@ -846,9 +846,14 @@ static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0, double* d
       if (pre == 0) fill(L.stream, f.prec, ul, f.g.n, 0.0);
     }
     corr[l] = ul;
-    residual(L, fo, f.prec, bl, ul, f.r.get(), nullptr);
-    restrict_fw(L, fo, f.prec, f.r.get(), co, c.prec, c.b.get(), 1.0,
-                (P.mixed && l == 0) ? &sc->scale : nullptr);
+    const double* rscale = (P.mixed && l == 0) ? &sc->scale : nullptr;
+    const int brr = smooth_tuning().box_rr;
+    const bool fused_rr = P.mixed && f.g.nx <= brr && f.g.ny <= brr && f.g.nz <= brr &&
+                          box_resrestrict(L, fo, f.prec, bl, ul, co, c.prec, c.b.get(), 1.0, rscale);
+    if (!fused_rr) {
+      residual(L, fo, f.prec, bl, ul, f.r.get(), nullptr);
+      restrict_fw(L, fo, f.prec, f.r.get(), co, c.prec, c.b.get(), 1.0, rscale);
+    }
   }
   Level& cl = P.L[nl - 1];
   const DevOp& cop = P.mixed ? cl.op32 : cl.op64;

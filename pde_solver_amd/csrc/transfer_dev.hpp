@@ -68,4 +68,92 @@ __device__ __forceinline__ double prolong_fine(double fa, double s, bool half, d
   return fma(s, v, fa);
 }
 
+// ---------------------------------------------------------------------------- restriction
+// Coarse right-hand side of one interior coarse node: full weighting of the fine residual, with
+// the transpose of the homogeneous closure extension folded in (coarse nodes next to a folded
+// face absorb theta times the weighted value of the face nodes beyond them; tensor product over
+// the axes).  `r(i, j, k)` gives the fine residual at fine node (i, j, k).  The explicit FMA
+// order makes the restriction kernel and the fused residual-restriction round alike.
+
+// Per-axis fold list of coarse coordinate c on an axis of n nodes: the node itself (weight 1)
+// plus the face node it absorbs (weight theta) when it is the first/last interior node.
+__device__ __forceinline__ int fold_list(int c, int n, double th_lo, double th_hi, int* idx, double* w) {
+  idx[0] = c; w[0] = 1.0;
+  int m = 1;
+  if (c == 1) { idx[m] = 0; w[m] = th_lo; ++m; }
+  if (c == n - 2) { idx[m] = n - 1; w[m] = th_hi; ++m; }
+  return m;
+}
+
+// 0.5 c + 0.25 w + 0.25 e
+__device__ __forceinline__ double fw_row3(double w, double c, double e) {
+  const double a = fma(0.25, w, 0.5 * c);
+  return fma(0.25, e, a);
+}
+
+// 3 x 3 in-plane weighting at (fi, fj) of fine plane fk, all nine nodes inside the grid
+template <typename R>
+__device__ __forceinline__ double fw_in(const R& r, int fi, int fj, int fk) {
+  const double a0 = fw_row3(r(fi - 1, fj - 1, fk), r(fi, fj - 1, fk), r(fi + 1, fj - 1, fk));
+  const double a1 = fw_row3(r(fi - 1, fj, fk), r(fi, fj, fk), r(fi + 1, fj, fk));
+  const double a2 = fw_row3(r(fi - 1, fj + 1, fk), r(fi, fj + 1, fk), r(fi + 1, fj + 1, fk));
+  return fma(0.25, a2, fma(0.5, a1, 0.25 * a0));
+}
+
+// the same, clipped at the plane's edges
+template <typename R>
+__device__ __forceinline__ double fw_clip(const DevOp& f, const R& r, int fi, int fj, int fk) {
+  const double w1[3] = {0.25, 0.5, 0.25};
+  double acc = 0.0;
+  for (int dj = -1; dj <= 1; ++dj) {
+    const int j = fj + dj;
+    if (j < 0 || j >= f.ny) continue;
+    double a = 0.5 * r(fi, j, fk);
+    if (fi > 0) a = fma(0.25, r(fi - 1, j, fk), a);
+    if (fi < f.nx - 1) a = fma(0.25, r(fi + 1, j, fk), a);
+    acc = fma(w1[dj + 1], a, acc);
+  }
+  return acc;
+}
+
+// Coarse value at interior coarse node (ic, jc, kc) of a 3D level, times s.  RIM: the node may
+// sit next to an x/y face, whose folded closure terms take the general path (fold lists,
+// clipped weighting); interior nodes (RIM false) never need it.
+template <bool RIM, typename R>
+__device__ __forceinline__ double restrict_value(const DevOp& f, const DevOp& c, const R& r, int ic,
+                                                 int jc, int kc, double s) {
+  const int fi = 2 * ic, fj = 2 * jc, fk = 2 * kc;
+  double ql = fw_in(r, fi, fj, fk - 1);
+  double qm = fw_in(r, fi, fj, fk);
+  double qh = fw_in(r, fi, fj, fk + 1);
+  bool fold_xy = false;
+  int xi[3], yi[3];
+  double xw[3], yw[3];
+  int mx = 1, my = 1;
+  xi[0] = ic; yi[0] = jc; xw[0] = 1.0; yw[0] = 1.0;
+  if constexpr (RIM) {
+    fold_xy = ic == 1 || jc == 1 || ic == c.nx - 2 || jc == c.ny - 2;
+    if (fold_xy) {
+      mx = fold_list(ic, c.nx, c.theta[0], c.theta[1], xi, xw);
+      my = fold_list(jc, c.ny, c.theta[2], c.theta[3], yi, yw);
+    }
+  }
+  auto Q = [&](int k) -> double {
+    double acc = 0.0;
+    for (int b = 0; b < my; ++b)
+      for (int q = 0; q < mx; ++q) {
+        const double w = yw[b] * xw[q];
+        if (w != 0.0) acc = fma(w, fw_clip(f, r, 2 * xi[q], 2 * yi[b], k), acc);
+      }
+    return acc;
+  };
+  if (fold_xy) { ql = Q(fk - 1); qm = Q(fk); qh = Q(fk + 1); }
+  double v = fma(0.25, qh, fma(0.5, qm, 0.25 * ql));
+  if (kc == 1 && c.theta[4] != 0.0)
+    v = fma(c.theta[4], fma(0.25, ql, 0.5 * (fold_xy ? Q(0) : fw_in(r, fi, fj, 0))), v);
+  if (kc == c.nz - 2 && c.theta[5] != 0.0)
+    v = fma(c.theta[5], fma(0.5, fold_xy ? Q(fk + 2) : fw_in(r, fi, fj, fk + 2), 0.25 * qh), v);
+  return v * s;
+}
+
 }  // namespace pdehip

@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"smooth.fused": 1, "smooth.rows": 2, "smooth.zchunk": 0, "smooth.ychunk": 64,
             "smooth.pair": 0, "smooth.ring": 1, "smooth.reg": 1, "stencil.ring": 1, "smooth.box": 65,
-            "smooth.box2": 65, "smooth.box_z": 8, "mixed.fp16": 1}
+            "smooth.box2": 65, "smooth.box_z": 8, "smooth.box_rr": 65, "mixed.fp16": 1}
 
 
 @pytest.fixture(scope="module")
@@ -142,6 +142,33 @@ def test_box_smoothing_bitwise(ctx, kind, shape, bz, fp16, pre, post, box2):
     u1, r1 = solve(ctx, s, capi.MIXED, 1, pre, post,
                    **{"smooth.box": 129, "smooth.box2": box2, "smooth.box_z": bz,
                       "mixed.fp16": fp16})
+    print(f"  iterations {r0.iterations}/{r1.iterations}  max diff {np.abs(u0 - u1).max():.3e}", flush=True)
+    assert r0.iterations == r1.iterations
+    assert np.array_equal(u0, u1)
+
+
+RR_CASES = [
+    # (kind, (nz, ny, nx), FP16 coarse levels, largest fine side fused)
+    ("flux", (33, 65, 129), 1, 129),
+    ("flux", (65, 129, 129), 0, 129),
+    ("const", (129, 65, 33), 1, 129),
+    ("flux", (69, 61, 37), 1, 129),
+    ("flux", (65, 65, 257), 1, 257),
+    ("const", (17, 33, 65), 0, 257),
+]
+
+
+@pytest.mark.parametrize("kind,shape,fp16,brr", RR_CASES)
+def test_box_resrestrict_bitwise(ctx, kind, shape, fp16, brr):
+    """The residual and its restriction in one launch (the fine residual formed in LDS)
+    against the residual and restriction kernels: the same mixed solve bit for bit, on grids
+    whose coarse levels end in partial boxes, with folded faces on every axis, FP16 and FP32
+    coarse storage, and the fine level's FP32 -> FP16 transfer (scaled by the V-cycle's input
+    normalisation) when the bound admits it."""
+    print(f"box rr {kind} {shape} fp16={fp16} bound={brr}", flush=True)
+    s = (flux_problem if kind == "flux" else const_problem)(shape, seed=sum(shape) + 11)
+    u0, r0 = solve(ctx, s, capi.MIXED, 1, **{"smooth.box_rr": 0, "mixed.fp16": fp16})
+    u1, r1 = solve(ctx, s, capi.MIXED, 1, **{"smooth.box_rr": brr, "mixed.fp16": fp16})
     print(f"  iterations {r0.iterations}/{r1.iterations}  max diff {np.abs(u0 - u1).max():.3e}", flush=True)
     assert r0.iterations == r1.iterations
     assert np.array_equal(u0, u1)
