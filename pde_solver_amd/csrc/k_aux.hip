@@ -396,7 +396,7 @@ __global__ void __launch_bounds__(kBlock) k_coarse_dense(const double* __restric
   const int row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (row >= m) return;
   double acc = 0.0;
-  for (int q = lane; q < m; q += kWave) acc += ainv[(long long)row * m + q] * ldd(b, map[q]);
+  for (int q = lane; q < m; q += kWave) acc = fma(ainv[(long long)row * m + q], ldd(b, map[q]), acc);
   acc = wave_sum(acc);
   if (lane == 0) x[map[row]] = store_as<S>(acc);
 }
