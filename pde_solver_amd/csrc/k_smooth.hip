@@ -587,8 +587,9 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
   // Rings, with plane pstart + s at slot s mod L: U (6) x(p-2 .. p+3), B (3) b(p-1 .. p+1),
   // K (3) rho(p+1 .. p+3), ZF (3) zf(p-2 .. p) and FE / FN (2) faces of planes p-1 .. p during
   // the colour stages of step p; the faces of plane p+1 are formed after them, in the slots the
-  // second stage has just finished with.
-  const int pstart = HK ? k0 - 3 : k0 - 1;
+  // second stage has just finished with.  The first step is the first colour's ring plane
+  // k0 - 1; the faces and front faces it needs are formed before the loop.
+  const int pstart = k0 - 1;
   T U[R][6], B[R][3], K[R][3], ZF[R][3], FE[R][2], FN[R][2];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -608,6 +609,22 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
     for (int s = 0; s < 3; ++s) ZF[r][s] = T(0);
 #pragma unroll
     for (int s = 0; s < 2; ++s) { FE[r][s] = T(0); FN[r][s] = T(0); }
+  }
+  if constexpr (HK) {
+    // faces of plane k0 - 1 (slot 0) and front faces zf(k0 - 2) (slot 2), zf(k0 - 1) (slot 0)
+    T km2[R], km1[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) { km2[r] = ld_k(r, pstart - 1); km1[r] = ld_k(r, pstart); }
+    lkb[1][wv][lane] = km1[0];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      ZF[r][2] = face_smooth<T>(km2[r], km1[r], fz);
+      ZF[r][0] = face_smooth<T>(km1[r], K[r][1], fz);
+      FE[r][0] = face_smooth<T>(km1[r], dpp_east(km1[r]), fx);
+      FN[r][0] = face_smooth<T>(km1[r], r < R - 1 ? km1[r + 1] : lkb[1][wup][lane], fy);
+    }
+    lfn[0][wv][lane] = top_ok ? FN[R - 1][0] : T(0);
   }
 
   auto step = [&](auto phc, int p) {
@@ -707,15 +724,17 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
     }
   };
 
+  // steps k0 - 1 .. k1 (the last one holds only the second colour of plane k1 - 1); the step
+  // count is the same in every thread, so every barrier is reached by all of them
   const int steps = k1 - pstart + 1;
   for (int s0 = 0; s0 < steps; s0 += kRing) {
     const int p = pstart + s0;
     step(IC<0>{}, p);
-    step(IC<1>{}, p + 1);
-    step(IC<2>{}, p + 2);
-    step(IC<3>{}, p + 3);
-    step(IC<4>{}, p + 4);
-    step(IC<5>{}, p + 5);
+    if (s0 + 1 < steps) step(IC<1>{}, p + 1);
+    if (s0 + 2 < steps) step(IC<2>{}, p + 2);
+    if (s0 + 3 < steps) step(IC<3>{}, p + 3);
+    if (s0 + 4 < steps) step(IC<4>{}, p + 4);
+    if (s0 + 5 < steps) step(IC<5>{}, p + 5);
   }
   if constexpr (DOT) {
     __shared__ double lred[kWaves];
@@ -1136,10 +1155,10 @@ int launch3_reg(const Launch& L, const DevOp& op, const void* b, const void* xin
   const int tiles_x = (op.nx + kTileX - 1) / kTileX;
   const int tiles_y = (op.ny + ty - 1) / ty;
   static const int occ = blocks_per_cu(k_rbgs_reg3<KIND, P, false>, kThreads);
-  const int prologue = KIND == PDEHIP_OP_FLUX ? 3 : 1;
+  // a chunk of kchunk planes takes kchunk + 2 plane steps (any count: no dead steps)
   const int kchunk = g_tune.zchunk > 0 ? g_tune.zchunk
-                                       : ring_chunk((long long)tiles_x * tiles_y, op.nz,
-                                                    device_cus() * occ, prologue);
+                                       : plane_chunk((long long)tiles_x * tiles_y, op.nz,
+                                                     device_cus() * occ, 2, 2);
   const int chunks = (op.nz + kchunk - 1) / kchunk;
   const int ntiles = tiles_x * tiles_y * chunks;
   const dim3 g(static_cast<unsigned>(8 * ((ntiles + 7) / 8)));
