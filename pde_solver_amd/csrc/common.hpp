@@ -347,6 +347,8 @@ struct KScalars {
   double best;       // mixed BiCGStab: smallest true residual so far ...
   int best_iter;     // ... reached at this iteration
   int stalled;       // 1 once it has not halved for the stall window
+  int xsel;          // mixed CG, fused replacement: which of the two FP64 solution buffers is current
+  int fresh;         // ... 1 from a replacement until the next FP32 update: the FP32 correction is void
 };
 
 }  // namespace pdehip

@@ -470,7 +470,8 @@ __global__ void __launch_bounds__(kBlock) k_dot(const void* av, const void* bv, 
 template <Prec P>
 __global__ void __launch_bounds__(kBlock) k_cg_update(void* xv, void* rv, const void* pvv,
                                                      const void* qv, long long n, double* partials,
-                                                     double* maxpart, const KScalars* sc) {
+                                                     double* maxpart, const KScalars* sc,
+                                                     const int* fresh) {
   using T = typename PT<P>::T;
   using S = typename PT<P>::S;
   __shared__ double lds[kBlock / kWave];
@@ -480,10 +481,12 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(void* xv, void* rv, const 
   const S* p = static_cast<const S*>(pvv);
   const S* q = static_cast<const S*>(qv);
   const T alpha = T(sc->alpha);
+  const bool void_x = fresh && *fresh;   // x holds a correction already folded in: take it as 0
   double acc = 0.0, mx = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    const T xi = T(ldd(x, i)) + alpha * T(ldd(p, i));
+    const T x0 = void_x ? T(0) : T(ldd(x, i));
+    const T xi = x0 + alpha * T(ldd(p, i));
     const T ri = T(ldd(r, i)) - alpha * T(ldd(q, i));
     x[i] = store_as<S>(xi);
     r[i] = store_as<S>(ri);
@@ -730,9 +733,10 @@ void finalize(const Launch& L, KScalars* sc, const double* partials, int nparts,
 }
 
 void cg_update(const Launch& L, Prec p, void* x, void* r, const void* pv, const void* q, long long n,
-               double* partials, double* maxpart) {
+               double* partials, double* maxpart, const int* fresh) {
   const dim3 g(vec_blocks(n));
-  PDEHIP_BY_PREC(p, k_cg_update, g, dim3(kBlock), 0, L.stream, x, r, pv, q, n, partials, maxpart, L.sc);
+  PDEHIP_BY_PREC(p, k_cg_update, g, dim3(kBlock), 0, L.stream, x, r, pv, q, n, partials, maxpart,
+                 L.sc, fresh);
 }
 
 void cg_xpay(const Launch& L, Prec p, const void* z, void* pv, long long n) {

@@ -41,11 +41,23 @@ __device__ __forceinline__ T face_op(T kc, T kn, T f) {
   return face_coef<T>(kc, kn, f);
 }
 
-// MODE 0: y = A x, partials of x.y.  MODE 1: r = b - A x, partials of r.r.
+// Residual replacement of a mixed solve (MODE 2): the two FP64 solution buffers, the FP32
+// correction, the scaled FP32 residual out and the outer scalars (xsel, snorm).
+struct ReplArgs {
+  double* x[2];
+  const float* e;
+  float* r32;
+  const KScalars* so;
+};
+
+// MODE 0: y = A x, partials of x.y.  MODE 1: r = b - A x, partials of r.r.  MODE 2 (FP64 only):
+// MODE 1 on x = x[sel] + s e, which also goes to x[1 - sel], and r / s to r32.
 template <int KIND, Prec P, int MODE, int R>
 __global__ void __launch_bounds__(kThreads)
 k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, void* __restrict__ yv,
-         double* partials, int tiles_x, int tiles_y, int ntiles, int kchunk, const KScalars* sc) {
+         double* partials, int tiles_x, int tiles_y, int ntiles, int kchunk, const KScalars* sc,
+         ReplArgs ra) {
+  static_assert(MODE != 2 || P == P64, "fused replacement: FP64 operator only");
   using T = typename PT<P>::T;
   using S = typename PT<P>::S;
   constexpr int TYE = R * kWaves;
@@ -69,6 +81,15 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
   const S* __restrict__ b = static_cast<const S*>(bv);
   const T* __restrict__ kap = static_cast<const T*>(op.kappa);
   S* __restrict__ y = static_cast<S*>(yv);
+  double* __restrict__ xo = nullptr;
+  double rs = 0.0, rinv = 1.0;
+  if constexpr (MODE == 2) {
+    const int sel = ra.so->xsel;
+    x = ra.x[sel];
+    xo = ra.x[sel ^ 1];
+    rs = ra.so->snorm;
+    rinv = rs > 0.0 ? 1.0 / rs : 1.0;
+  }
   const int tx = t % tiles_x;
   const int rest = t / tiles_x;
   const int ty = rest % tiles_y;
@@ -102,13 +123,17 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
   const int jrow0 = ty * TY - 1 + wv * R;
 
   auto kcl = [&](int k) -> long long { return (long long)min(max(k, 0), nz - 1) * sxy; };
-  auto ld_x = [&](int r, int k) -> T { return T(ldS(x, kcl(k) + col[r])); };
+  auto ld_x = [&](int r, int k) -> T {
+    const long long id = kcl(k) + col[r];
+    if constexpr (MODE == 2) return fma(rs, static_cast<double>(ra.e[id]), x[id]);
+    else return T(ldS(x, id));
+  };
   auto ld_k = [&](int r, int k) -> T {
     if constexpr (HK) return kap[kcl(k) + col[r]];
     else return T(0);
   };
   auto ld_b = [&](int r, int k) -> T {
-    if constexpr (MODE == 1) return T(ldS(b, kcl(k) + col[r]));
+    if constexpr (MODE >= 1) return T(ldS(b, kcl(k) + col[r]));
     else return T(0);
   };
 
@@ -193,6 +218,10 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
           out = MODE == 0 ? ax : T(B[r][0] - ax);
         }
         y[p * sxy + col[r]] = store_as<S>(static_cast<double>(out));
+        if constexpr (MODE == 2) {
+          xo[p * sxy + col[r]] = xc;
+          ra.r32[p * sxy + col[r]] = static_cast<float>(out * rinv);
+        }
         if (partials)
           acc += MODE == 0 ? static_cast<double>(xc) * static_cast<double>(out)
                            : static_cast<double>(out) * static_cast<double>(out);
@@ -429,7 +458,8 @@ void launch(const Launch& L, const DevOp& op, const void* b, const void* x, void
     }
   }
   hipLaunchKernelGGL((k_plane3<KIND, P, MODE, 2>), dim3(s.blocks), dim3(kThreads), 0, L.stream,
-                     op, b, x, y, partials, s.tiles_x, s.tiles_y, s.ntiles, s.kchunk, L.sc);
+                     op, b, x, y, partials, s.tiles_x, s.tiles_y, s.ntiles, s.kchunk, L.sc,
+                     ReplArgs{});
 }
 
 template <int MODE>
@@ -465,6 +495,24 @@ void apply_plane(const Launch& L, const DevOp& op, Prec p, const void* x, void* 
 void residual_plane(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x, void* r,
                     double* partials) {
   launch_mode<1>(L, op, p, b, x, r, partials);
+}
+
+bool replace_plane(const Launch& Lgate, const DevOp& op64, const double* b, double* const x[2],
+                   const float* e, double* r64, float* r32, const KScalars* so, double* partials) {
+  if (op64.nz <= 1 || !plane_stencils()) return false;
+  const Shape s = shape_of(op64);
+  const ReplArgs ra{{x[0], x[1]}, e, r32, so};
+#define PDEHIP_REPL_(K)                                                                         \
+  hipLaunchKernelGGL((k_plane3<K, P64, 2, 2>), dim3(s.blocks), dim3(kThreads), 0, Lgate.stream, \
+                     op64, b, nullptr, r64, partials, s.tiles_x, s.tiles_y, s.ntiles, s.kchunk,  \
+                     Lgate.sc, ra)
+  switch (op64.kind) {
+    case PDEHIP_OP_CONST: PDEHIP_REPL_(PDEHIP_OP_CONST); break;
+    case PDEHIP_OP_POINTWISE: PDEHIP_REPL_(PDEHIP_OP_POINTWISE); break;
+    default: PDEHIP_REPL_(PDEHIP_OP_FLUX); break;
+  }
+#undef PDEHIP_REPL_
+  return true;
 }
 
 }  // namespace pdehip

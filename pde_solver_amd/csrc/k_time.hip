@@ -446,7 +446,7 @@ __global__ void k_flush_f32(double* __restrict__ x, float* __restrict__ e, long 
   const double s = *scale;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    x[i] += s * static_cast<double>(e[i]);
+    x[i] = fma(s, static_cast<double>(e[i]), x[i]);   // as the fused replacement forms it
     e[i] = 0.0f;
   }
 }
@@ -472,6 +472,7 @@ __global__ void __launch_bounds__(1024) k_fin_ru(KScalars* si, KScalars* so, KSc
       break;
     case FIN_RU_CHECK:
       if (si->done | si->breakdown) { gate->done = 1; break; }
+      so->fresh = 0;   // the CG update before this check has restarted the FP32 correction
       si->rr = s;
       si->metric = sqrt(s);
       si->iter += 1;
@@ -481,8 +482,15 @@ __global__ void __launch_bounds__(1024) k_fin_ru(KScalars* si, KScalars* so, KSc
       gate->done = si->flag_s ? 0 : 1;
       break;
     case FIN_RU_APPLY:
+    case FIN_RU_APPLY_PP:
       gate->done = 1;
       if ((si->done | si->breakdown) || !si->flag_s) break;
+      if (op == FIN_RU_APPLY_PP) {
+        // the fused replacement ran: its output buffer is now the solution, and the FP32
+        // correction it folded in is void until the next CG update
+        so->xsel ^= 1;
+        so->fresh = 1;
+      }
       so->metric = sqrt(s);
       so->steps += 1;
       so->iter = si->iter;

@@ -91,6 +91,14 @@ void apply_plane(const Launch& L, const DevOp& op, Prec p, const void* x, void* 
 void residual_plane(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x, void* r,
                     double* partials);
 int plane_blocks(const DevOp& op);
+// Residual replacement of a mixed solve in one 3D plane march (FP64 operator op64): with
+// sel = so->xsel, s = so->snorm, x = x[sel] + s e (x[sel] and e read only), the march writes
+// x[1 - sel] = x, r64 = b - A x, r32 = r64 / s and the partials of r64.r64; a set gate (done or
+// breakdown) skips it.  Every output node is written by one thread and no input is written, so
+// neighbouring tiles never see a half-updated solution.  Returns false (nothing launched) when
+// the operator is not a 3D plane-march one.
+bool replace_plane(const Launch& Lgate, const DevOp& op64, const double* b, double* const x[2],
+                   const float* e, double* r64, float* r32, const KScalars* so, double* partials);
 // Partials capacity that covers either routing of this op.
 int partial_blocks_max(const DevOp& op);
 void set_plane_stencils(bool on);
@@ -166,13 +174,18 @@ enum FinOp {
   FIN_RU_CHECK = 8,    // as FIN_CG_RR without a stop; flag_s = replacement due
   FIN_RU_APPLY = 9,    // after a replacement: outer metric = sqrt(sum), stop on the true residual
   FIN_RU_START = 10,   // outer snorm = outer metric (the inner scale); inner done = outer done
+  FIN_RU_APPLY_PP = 11,  // FIN_RU_APPLY after a fused replacement: a replacement that ran also
+                         // switches the current FP64 solution buffer (xsel) and voids the FP32
+                         // correction (fresh) until the next CG update
 };
 void finalize(const Launch& L, KScalars* sc, const double* partials, int nparts, int op,
               double tol, int tol_mode, double* out_slot);
 
 // CG updates: x += alpha p, r -= alpha q, partials of r.r (and max|r| into maxpart if non-null).
+// With `fresh` non-null and *fresh set, x is taken as zero (x = alpha p): the FP32 correction
+// of a mixed solve after a fused replacement.
 void cg_update(const Launch& L, Prec p, void* x, void* r, const void* pv, const void* q,
-               long long n, double* partials, double* maxpart);
+               long long n, double* partials, double* maxpart, const int* fresh = nullptr);
 // p = z + beta p
 void cg_xpay(const Launch& L, Prec p, const void* z, void* pv, long long n);
 // y += a x with a from device scalar (mode) or constant

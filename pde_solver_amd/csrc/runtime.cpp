@@ -27,9 +27,12 @@ void hip_check(hipError_t e, const char* what) {
 // ------------------------------------------------------------------ buffers
 namespace {
 std::atomic<bool> g_mixed_fp16{true};
+std::atomic<bool> g_fused_replace{true};
 }  // namespace
 void set_mixed_fp16(bool on) { g_mixed_fp16.store(on); }
 bool mixed_fp16() { return g_mixed_fp16.load(); }
+void set_fused_replace(bool on) { g_fused_replace.store(on); }
+bool fused_replace() { return g_fused_replace.load(); }
 
 void Ctx::trim_pool() {
   std::lock_guard<std::mutex> g(pool_mu);
@@ -555,9 +558,10 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
   auto z8 = [&](DBuf& b) { b.alloc(ctx, n8); fill(s, P64, b.get(), g.n, 0.0); };
   auto z4 = [&](DBuf& b) { b.alloc(ctx, n4); fill(s, P32, b.get(), g.n, 0.0); };
   if (m == PDEHIP_CG || m == PDEHIP_MGPCG) {
-    z8(P->r); z8(P->p); z8(P->q);
+    z8(P->r);
+    if (!P->mixed) { z8(P->p); z8(P->q); }
     if (m == PDEHIP_MGPCG && !P->mixed) z8(P->z);
-    if (P->mixed) { z4(P->r32); z4(P->x32); z4(P->p32); z4(P->q32); z4(P->z32); }
+    if (P->mixed) { z8(P->xb); z4(P->r32); z4(P->x32); z4(P->p32); z4(P->q32); z4(P->z32); }
   } else if (m == PDEHIP_BICGSTAB && P->k32) {
     z8(P->r);
     const int nv = o.precond == PDEHIP_PC_JACOBI ? 8 : 6;
@@ -619,7 +623,7 @@ size_t plan_bytes(const Plan& P) {
   for (const Level& lv : P.L)
     for (const DBuf* b : {&lv.kap64, &lv.kap32, &lv.pw, &lv.u, &lv.b, &lv.r, &lv.alt, &lv.ainv, &lv.map})
       t += b->bytes();
-  for (const DBuf* b : {&P.r, &P.p, &P.q, &P.z, &P.w1, &P.w2, &P.w3, &P.w4, &P.r32, &P.x32, &P.p32,
+  for (const DBuf* b : {&P.r, &P.p, &P.q, &P.z, &P.w1, &P.w2, &P.w3, &P.w4, &P.xb, &P.r32, &P.x32, &P.p32,
                         &P.q32, &P.z32, &P.V32, &P.V, &P.H, &P.cs, &P.sn, &P.gv, &P.yv, &P.hcb,
                         &P.part, &P.part2, &P.part3, &P.scal})
     t += b->bytes();
@@ -980,6 +984,9 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
   double* r64 = P.r.as<double>();
   float* x32 = P.x32.as<float>();
   float* r32 = P.r32.as<float>();
+  // fused replacement (3D plane marches): the solution alternates between P.x and P.xb
+  double* const xs[2] = {x64, P.xb.as<double>()};
+  const bool fused = P.xb && op64.nz > 1 && plane_stencils() && fused_replace();
   HIPCHK(hipMemcpyAsync(x64, P.x0.get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
   residual(L0, op64, P64, P.bvec.get(), x64, r64, part);
   finalize(L0, so, part, partial_blocks(op64), FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
@@ -1020,14 +1027,20 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
       }
       apply(Li, op32, P32, P.p32.get(), P.q32.get(), part);
       finalize(Li, si, part, partial_blocks(op32), FIN_CG_ALPHA, 0, 0, nullptr);
-      cg_update(Li, P32, x32, r32, P.p32.get(), P.q32.get(), n, part, part2);
+      cg_update(Li, P32, x32, r32, P.p32.get(), P.q32.get(), n, part, part2,
+                fused ? &so->fresh : nullptr);
       finalize_ru(s, si, so, gate, part, vec_blocks(n), FIN_RU_CHECK, delta);
       fin_scale(s, part2, vec_blocks(n), si);
       // replacement (only when due): x += snorm x32, r = b - A x in FP64, r32 = r / snorm
-      flush_f32(s, x64, x32, n, &so->snorm, gate);
-      residual(Lg, op64, P64, P.bvec.get(), x64, r64, part);
-      to_f32_scaled_gated(s, r64, r32, n, &so->snorm, gate);
-      finalize_ru(s, si, so, gate, part, partial_blocks(op64), FIN_RU_APPLY, delta);
+      if (fused) {
+        replace_plane(Lg, op64, P.bvec.as<double>(), xs, x32, r64, r32, so, part);
+        finalize_ru(s, si, so, gate, part, partial_blocks(op64), FIN_RU_APPLY_PP, delta);
+      } else {
+        flush_f32(s, x64, x32, n, &so->snorm, gate);
+        residual(Lg, op64, P64, P.bvec.get(), x64, r64, part);
+        to_f32_scaled_gated(s, r64, r32, n, &so->snorm, gate);
+        finalize_ru(s, si, so, gate, part, partial_blocks(op64), FIN_RU_APPLY, delta);
+      }
     }
     launched += chunk;
     // blocks 0 (outer) and 1 (inner): the inner residual is relative to the outer snorm
@@ -1036,9 +1049,16 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
       break;
   }
   // an unconverged solve keeps the correction accumulated since the last replacement
-  flush_f32(s, x64, x32, n, &so->snorm, so);
+  if (!fused) flush_f32(s, x64, x32, n, &so->snorm, so);
   const KScalars ki = pull(P, 1);
   const KScalars& ko = pull(P, 0);
+  if (fused) {
+    // the current solution buffer back into P.x, then (unconverged, correction not void) the
+    // correction accumulated since the last replacement
+    if (ko.xsel)
+      HIPCHK(hipMemcpyAsync(x64, xs[1], n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (!(ko.done | ko.breakdown) && !ko.fresh) flush_f32(s, x64, x32, n, &so->snorm, so);
+  }
   // converged at a replacement: r64 and its partials are b - A x of the final x already
   P.resid_fresh = ko.done && !ko.breakdown && !ki.breakdown;
   rep->iterations = ki.iter;

@@ -31,6 +31,9 @@ void hip_check(hipError_t e, const char* what);
 // is created.
 void set_mixed_fp16(bool on);
 bool mixed_fp16();
+// mixed MG-PCG on 3D plane-march levels: residual replacement in one launch (else three)
+void set_fused_replace(bool on);
+bool fused_replace();
 
 struct Ctx {
   int device = 0;
@@ -155,6 +158,7 @@ struct Plan {
   // Krylov work vectors
   DBuf r, p, q, z, w1, w2, w3, w4;
   DBuf r32, x32, p32, q32, z32;
+  DBuf xb;                         // mixed MG-PCG: the second FP64 solution buffer of the fused replacement
   bool k32 = false;                // mixed BiCGStab/GMRES: FP32 Krylov vectors, FP64 x and residual
   DBuf v32[8], V32;                // their FP32 vectors (GMRES: basis in V32)
   DBuf V, H, cs, sn, gv, yv, hcb;   // GMRES (hcb: two staged Hessenberg columns)

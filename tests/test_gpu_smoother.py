@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"smooth.fused": 1, "smooth.rows": 2, "smooth.zchunk": 0, "smooth.ychunk": 64,
             "smooth.pair": 0, "smooth.ring": 1, "smooth.reg": 1, "stencil.ring": 1, "smooth.box": 65,
-            "smooth.box2": 65, "smooth.box_z": 8, "smooth.box_rr": 65, "mixed.fp16": 1}
+            "smooth.box2": 65, "smooth.box_z": 8, "smooth.box_rr": 65, "mixed.fp16": 1,
+            "mixed.replace": 1}
 
 
 @pytest.fixture(scope="module")
@@ -47,7 +48,7 @@ def const_problem(shape, seed):
     return s
 
 
-def solve(ctx, s, precision, fused, pre=2, post=2, **tune):
+def solve(ctx, s, precision, fused, pre=2, post=2, max_iter=100, **tune):
     from pde_solver_amd import hip
     for k, v in DEFAULTS.items():
         hip.tune_set(k, v)
@@ -55,10 +56,10 @@ def solve(ctx, s, precision, fused, pre=2, post=2, **tune):
     for k, v in tune.items():
         hip.tune_set(k if "." in k else "smooth." + k, v)
     opt = capi.options(capi.MGPCG, tol=1e-10, tol_mode=capi.TOL_REL, precond=capi.PC_MG,
-                       bc_rows=capi.ROWS_FOLDED, precision=precision, max_iter=100,
+                       bc_rows=capi.ROWS_FOLDED, precision=precision, max_iter=max_iter,
                        mg_pre=pre, mg_post=post)
-    st, u, rep = ctx.solve(s, opt)
-    assert st == 0, ctx.error()
+    st, u, rep = ctx.solve(s, opt, check=False)
+    assert st == (0 if max_iter >= 100 else capi.ERR_NOT_CONVERGED), ctx.error()
     return u, rep
 
 
@@ -207,3 +208,23 @@ def test_plane_apply_3d_matches_oracle(ctx, kind, shape):
     err = np.abs(y - y_o).max() / np.abs(y_o).max()
     print(f"  max rel diff {err:.2e}", flush=True)
     assert err < 1e-13
+
+
+@pytest.mark.parametrize("kind,shape,max_iter", [
+    ("flux", (33, 65, 129), 100), ("flux", (69, 61, 37), 100), ("const", (17, 129, 65), 100),
+    # stopped by max_iter: after a replacement (the FP32 correction void) and between two
+    ("flux", (33, 65, 129), 3), ("flux", (33, 65, 129), 4), ("flux", (69, 61, 37), 5)])
+def test_fused_replacement_bitwise(ctx, kind, shape, max_iter):
+    """Mixed MG-PCG residual replacement in one plane-march launch (the FP32 correction folded
+    into the other FP64 solution buffer, FP64 residual, scaled FP32 copy) against the three
+    launches it replaces: the same solution, iterations, replacements and final residual bit
+    for bit, converged or stopped by max_iter."""
+    print(f"fused replacement {kind} {shape} max_iter={max_iter}", flush=True)
+    s = (flux_problem if kind == "flux" else const_problem)(shape, seed=sum(shape) + 7)
+    u0, r0 = solve(ctx, s, capi.MIXED, 1, max_iter=max_iter, **{"mixed.replace": 0})
+    u1, r1 = solve(ctx, s, capi.MIXED, 1, max_iter=max_iter, **{"mixed.replace": 1})
+    print(f"  iterations {r0.iterations}/{r1.iterations} refinements {r0.refinements}/"
+          f"{r1.refinements} max diff {np.abs(u0 - u1).max():.3e}", flush=True)
+    assert (r0.iterations, r0.refinements) == (r1.iterations, r1.refinements)
+    assert r0.final_metric == r1.final_metric
+    assert np.array_equal(u0, u1)
