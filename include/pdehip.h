@@ -264,7 +264,9 @@ void pdehip_plan_destroy(pdehip_plan* plan);
  * multigrid levels in FP16 (1) or FP32 (0); this one changes the preconditioner's rounding.
  * "mixed.replace" (0/1, default 1): mixed MG-PCG on 3D grids replaces its FP32 residual in one
  * launch (fold the FP32 correction into the FP64 solution, FP64 residual, scaled FP32 copy) or
- * in three; the same bits either way.
+ * in three; the same bits either way.  "mixed.xpay" (0/1, default 1): mixed MG-PCG on 3D grids
+ * forms the new search direction inside the operator apply (one launch instead of two); the
+ * same bits either way.
  * Not thread-safe against running solves. */
 int pdehip_tune_set(const char* key, int value);
 int pdehip_tune_get(const char* key, int* value);

@@ -511,7 +511,7 @@ __global__ void k_xpay(const void* zv, void* pvv, long long n, const KScalars* s
   const T beta = T(sc->beta);
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
-    p[i] = store_as<S>(T(ldd(z, i)) + beta * T(ldd(p, i)));
+    p[i] = store_as<S>(fma(beta, T(ldd(p, i)), T(ldd(z, i))));   // as the fused apply forms it
 }
 
 template <Prec P>

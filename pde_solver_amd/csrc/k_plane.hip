@@ -243,15 +243,24 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
 // output tile are discarded by the descriptor's range check.  Same tile, same arithmetic in
 // the same order as k_plane3, so the outputs and the partials are the same bits.  Steps past
 // the chunk compute and discard.
+//
+// MODE 3 (FP32): the CG direction update folded into the apply: with bv = z and xv = p, every
+// loaded node forms p' = z + beta p (beta = sc->beta, the FMA of the separate update), the
+// march writes p' to pov and y = A p' with the partials of p'.y.  p' of a plane is formed just
+// before its first use, so the z and p loads keep their three-plane lead.  pov must not be xv:
+// neighbouring tiles read xv's halo.
 constexpr int kRingP = 6;
 
 template <int KIND, Prec P, int MODE, int R>
 __global__ void __launch_bounds__(kThreads)
 k_plane3r(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, void* __restrict__ yv,
-          double* partials, int tiles_x, int tiles_y, int ntiles, int kchunk, const KScalars* sc) {
+          double* partials, int tiles_x, int tiles_y, int ntiles, int kchunk, const KScalars* sc,
+          void* __restrict__ pov) {
   using T = typename PT<P>::T;
   using S = typename PT<P>::S;
   static_assert(sizeof(T) == 4, "ring plane march: 4-byte arithmetic types only");
+  static_assert(MODE != 3 || P == P32, "fused direction update: FP32 vectors only");
+  constexpr bool XP = MODE == 3;
   constexpr int TYE = R * kWaves;
   constexpr int TY = TYE - 2;
   constexpr int TX = kLanes - 2;
@@ -317,18 +326,38 @@ k_plane3r(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, vo
     if constexpr (MODE == 1) return BufIO<S>::ld(plane_rsrc(bv, kcl(k), sizeof(S), pbytes), off_s[r]);
     else return T(0);
   };
+  // MODE 3: z into X, p into XQ; beta
+  auto ld_z = [&](int r, int k) -> T {
+    if constexpr (XP) return BufIO<S>::ld(plane_rsrc(bv, kcl(k), sizeof(S), pbytes), off_s[r]);
+    else return T(0);
+  };
+  const T beta = XP ? T(sc->beta) : T(0);
 
   // Rings, plane pstart + s at slot s mod L: X (6) x(p-1 .. p+4), B (3) b(p .. p+2),
   // K (3) kappa(p+1 .. p+3), ZF (3) zf(p-1 .. p+1) with zf(k) = face(kappa(k), kappa(k+1)).
+  // MODE 3: X holds z until the step that first uses the plane turns it into p' with XQ's p.
   const int pstart = HK ? k0 - 2 : k0;
-  T X[R][6], B[R][3], K[R][3], ZF[R][3];
+  T X[R][6], B[R][3], K[R][3], ZF[R][3], XQ[R][6];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    X[r][5] = ld_x(r, pstart - 1);
-    X[r][0] = ld_x(r, pstart);
-    X[r][1] = ld_x(r, pstart + 1);
-    X[r][2] = ld_x(r, pstart + 2);
-    X[r][3] = ld_x(r, pstart + 3);
+    if constexpr (XP) {
+#pragma unroll
+      for (int s = 0; s < 6; ++s) XQ[r][s] = T(0);
+      X[r][5] = fma(beta, ld_x(r, pstart - 1), ld_z(r, pstart - 1));
+      X[r][0] = fma(beta, ld_x(r, pstart), ld_z(r, pstart));
+      X[r][1] = ld_z(r, pstart + 1);
+      X[r][2] = ld_z(r, pstart + 2);
+      X[r][3] = ld_z(r, pstart + 3);
+      XQ[r][1] = ld_x(r, pstart + 1);
+      XQ[r][2] = ld_x(r, pstart + 2);
+      XQ[r][3] = ld_x(r, pstart + 3);
+    } else {
+      X[r][5] = ld_x(r, pstart - 1);
+      X[r][0] = ld_x(r, pstart);
+      X[r][1] = ld_x(r, pstart + 1);
+      X[r][2] = ld_x(r, pstart + 2);
+      X[r][3] = ld_x(r, pstart + 3);
+    }
     X[r][4] = T(0);
     B[r][0] = ld_b(r, pstart);
     B[r][1] = ld_b(r, pstart + 1);
@@ -350,7 +379,13 @@ k_plane3r(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, vo
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int lr = wv * R + r;
-      X[r][x_4] = ld_x(r, p + 4);
+      if constexpr (XP) {
+        X[r][x_1] = fma(beta, XQ[r][x_1], X[r][x_1]);   // p' of plane p+1, first used below
+        X[r][x_4] = ld_z(r, p + 4);
+        XQ[r][x_4] = ld_x(r, p + 4);
+      } else {
+        X[r][x_4] = ld_x(r, p + 4);
+      }
       B[r][b_2] = ld_b(r, p + 2);
       if constexpr (HK) {
         K[r][k_3] = ld_k(r, p + 3);
@@ -395,11 +430,13 @@ k_plane3r(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, vo
                                   X[r][x_m1]);
         const bool bnd = ((m_bxy >> r) & 1u) || p == 0 || p == nz - 1;
         const T ain = bnd ? xc : ax;
-        const T out = MODE == 0 ? ain : T(B[r][b_0] - ain);
+        const T out = MODE == 1 ? T(B[r][b_0] - ain) : ain;
         const S so = store_as<S>(static_cast<double>(out));
         BufIO<S>::st(plane_rsrc(yv, static_cast<long long>(p) * sxy, sizeof(S), pbytes), off_st[r], so);
+        if constexpr (XP)
+          BufIO<S>::st(plane_rsrc(pov, static_cast<long long>(p) * sxy, sizeof(S), pbytes), off_st[r], xc);
         if (partials && ((m_out >> r) & 1u))
-          acc += MODE == 0 ? static_cast<double>(xc) * static_cast<double>(out)
+          acc += MODE != 1 ? static_cast<double>(xc) * static_cast<double>(out)
                            : static_cast<double>(out) * static_cast<double>(out);
       }
     }
@@ -453,7 +490,7 @@ void launch(const Launch& L, const DevOp& op, const void* b, const void* x, void
     if (g_plane_ring && op.sxy < (1LL << 29)) {
       hipLaunchKernelGGL((k_plane3r<KIND, P, MODE, 2>), dim3(s.blocks), dim3(kThreads), 0,
                          L.stream, op, b, x, y, partials, s.tiles_x, s.tiles_y, s.ntiles,
-                         s.kchunk, L.sc);
+                         s.kchunk, L.sc, nullptr);
       return;
     }
   }
@@ -495,6 +532,23 @@ void apply_plane(const Launch& L, const DevOp& op, Prec p, const void* x, void* 
 void residual_plane(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x, void* r,
                     double* partials) {
   launch_mode<1>(L, op, p, b, x, r, partials);
+}
+
+bool apply_xpay_plane(const Launch& L, const DevOp& op, Prec p, const void* z, const void* p_in,
+                      void* p_out, void* y, double* partials) {
+  if (op.nz <= 1 || !plane_stencils() || !g_plane_ring || p != P32 || op.sxy >= (1LL << 29) ||
+      op.kind == PDEHIP_OP_POINTWISE || p_in == p_out)
+    return false;
+  const Shape s = shape_of(op);
+  if (op.kind == PDEHIP_OP_CONST)
+    hipLaunchKernelGGL((k_plane3r<PDEHIP_OP_CONST, P32, 3, 2>), dim3(s.blocks), dim3(kThreads), 0,
+                       L.stream, op, z, p_in, y, partials, s.tiles_x, s.tiles_y, s.ntiles, s.kchunk,
+                       L.sc, p_out);
+  else
+    hipLaunchKernelGGL((k_plane3r<PDEHIP_OP_FLUX, P32, 3, 2>), dim3(s.blocks), dim3(kThreads), 0,
+                       L.stream, op, z, p_in, y, partials, s.tiles_x, s.tiles_y, s.ntiles, s.kchunk,
+                       L.sc, p_out);
+  return true;
 }
 
 bool replace_plane(const Launch& Lgate, const DevOp& op64, const double* b, double* const x[2],

@@ -91,6 +91,11 @@ void apply_plane(const Launch& L, const DevOp& op, Prec p, const void* x, void* 
 void residual_plane(const Launch& L, const DevOp& op, Prec p, const void* b, const void* x, void* r,
                     double* partials);
 int plane_blocks(const DevOp& op);
+// CG direction update and apply in one 3D plane march (FP32 CONST / FLUX): p_out = z + beta p_in
+// (beta from L.sc, the same FMA as cg_xpay), y = A p_out, partials of p_out.y as apply().  p_out
+// must differ from p_in.  Returns false (nothing launched) where the march does not apply.
+bool apply_xpay_plane(const Launch& L, const DevOp& op, Prec p, const void* z, const void* p_in,
+                      void* p_out, void* y, double* partials);
 // Residual replacement of a mixed solve in one 3D plane march (FP64 operator op64): with
 // sel = so->xsel, s = so->snorm, x = x[sel] + s e (x[sel] and e read only), the march writes
 // x[1 - sel] = x, r64 = b - A x, r32 = r64 / s and the partials of r64.r64; a set gate (done or

@@ -28,11 +28,14 @@ void hip_check(hipError_t e, const char* what) {
 namespace {
 std::atomic<bool> g_mixed_fp16{true};
 std::atomic<bool> g_fused_replace{true};
+std::atomic<bool> g_fused_xpay{true};
 }  // namespace
 void set_mixed_fp16(bool on) { g_mixed_fp16.store(on); }
 bool mixed_fp16() { return g_mixed_fp16.load(); }
 void set_fused_replace(bool on) { g_fused_replace.store(on); }
 bool fused_replace() { return g_fused_replace.load(); }
+void set_fused_xpay(bool on) { g_fused_xpay.store(on); }
+bool fused_xpay() { return g_fused_xpay.load(); }
 
 void Ctx::trim_pool() {
   std::lock_guard<std::mutex> g(pool_mu);
@@ -562,6 +565,7 @@ std::unique_ptr<Plan> plan_create(Ctx* ctx, const pdehip_problem& p, const pdehi
     if (!P->mixed) { z8(P->p); z8(P->q); }
     if (m == PDEHIP_MGPCG && !P->mixed) z8(P->z);
     if (P->mixed) { z8(P->xb); z4(P->r32); z4(P->x32); z4(P->p32); z4(P->q32); z4(P->z32); }
+    if (P->mixed && g.nz > 1) z4(P->p32b);
   } else if (m == PDEHIP_BICGSTAB && P->k32) {
     z8(P->r);
     const int nv = o.precond == PDEHIP_PC_JACOBI ? 8 : 6;
@@ -623,7 +627,7 @@ size_t plan_bytes(const Plan& P) {
   for (const Level& lv : P.L)
     for (const DBuf* b : {&lv.kap64, &lv.kap32, &lv.pw, &lv.u, &lv.b, &lv.r, &lv.alt, &lv.ainv, &lv.map})
       t += b->bytes();
-  for (const DBuf* b : {&P.r, &P.p, &P.q, &P.z, &P.w1, &P.w2, &P.w3, &P.w4, &P.xb, &P.r32, &P.x32, &P.p32,
+  for (const DBuf* b : {&P.r, &P.p, &P.q, &P.z, &P.w1, &P.w2, &P.w3, &P.w4, &P.xb, &P.r32, &P.x32, &P.p32, &P.p32b,
                         &P.q32, &P.z32, &P.V32, &P.V, &P.H, &P.cs, &P.sn, &P.gv, &P.yv, &P.hcb,
                         &P.part, &P.part2, &P.part3, &P.scal})
     t += b->bytes();
@@ -1011,6 +1015,10 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
   int npz = precondition();
   finalize(Li, si, part, npz, FIN_CG_INIT, 0, 0, nullptr);
   HIPCHK(hipMemcpyAsync(P.p32.get(), P.z32.get(), n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  // 3D: the direction update rides in the apply, p' into the other direction buffer
+  void* pcur = P.p32.get();
+  void* pnext = P.p32b.get();
+  const bool fuse_xpay = pnext && fused_xpay();
   const int every = check_every(P);
   const int max_iter = std::max(1, P.opt.max_iter);
   int launched = 0;
@@ -1020,14 +1028,21 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
     for (int c = 0; c < chunk; ++c) {
       // preconditioning and direction update of the previous iteration first: a chunk ends
       // at the replacement, the only place the solve can stop
+      bool applied = false;
       if (launched + c > 0) {
         npz = precondition();
         finalize(Li, si, part, npz, FIN_CG_BETA, 0, 0, nullptr);
-        cg_xpay(Li, P32, P.z32.get(), P.p32.get(), n);
+        if (fuse_xpay &&
+            apply_xpay_plane(Li, op32, P32, P.z32.get(), pcur, pnext, P.q32.get(), part)) {
+          std::swap(pcur, pnext);
+          applied = true;
+        } else {
+          cg_xpay(Li, P32, P.z32.get(), pcur, n);
+        }
       }
-      apply(Li, op32, P32, P.p32.get(), P.q32.get(), part);
+      if (!applied) apply(Li, op32, P32, pcur, P.q32.get(), part);
       finalize(Li, si, part, partial_blocks(op32), FIN_CG_ALPHA, 0, 0, nullptr);
-      cg_update(Li, P32, x32, r32, P.p32.get(), P.q32.get(), n, part, part2,
+      cg_update(Li, P32, x32, r32, pcur, P.q32.get(), n, part, part2,
                 fused ? &so->fresh : nullptr);
       finalize_ru(s, si, so, gate, part, vec_blocks(n), FIN_RU_CHECK, delta);
       fin_scale(s, part2, vec_blocks(n), si);

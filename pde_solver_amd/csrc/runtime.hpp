@@ -34,6 +34,9 @@ bool mixed_fp16();
 // mixed MG-PCG on 3D plane-march levels: residual replacement in one launch (else three)
 void set_fused_replace(bool on);
 bool fused_replace();
+// mixed MG-PCG on 3D FP32 plane-march levels: direction update p = z + beta p inside the apply
+void set_fused_xpay(bool on);
+bool fused_xpay();
 
 struct Ctx {
   int device = 0;
@@ -159,6 +162,7 @@ struct Plan {
   DBuf r, p, q, z, w1, w2, w3, w4;
   DBuf r32, x32, p32, q32, z32;
   DBuf xb;                         // mixed MG-PCG: the second FP64 solution buffer of the fused replacement
+  DBuf p32b;                       // ... 3D: the second direction buffer of the fused update + apply
   bool k32 = false;                // mixed BiCGStab/GMRES: FP32 Krylov vectors, FP64 x and residual
   DBuf v32[8], V32;                // their FP32 vectors (GMRES: basis in V32)
   DBuf V, H, cs, sn, gv, yv, hcb;   // GMRES (hcb: two staged Hessenberg columns)

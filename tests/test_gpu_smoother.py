@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 DEFAULTS = {"smooth.fused": 1, "smooth.rows": 2, "smooth.zchunk": 0, "smooth.ychunk": 64,
             "smooth.pair": 0, "smooth.ring": 1, "smooth.reg": 1, "stencil.ring": 1, "smooth.box": 65,
             "smooth.box2": 65, "smooth.box_z": 8, "smooth.box_rr": 65, "mixed.fp16": 1,
-            "mixed.replace": 1}
+            "mixed.replace": 1, "mixed.xpay": 1}
 
 
 @pytest.fixture(scope="module")
@@ -210,19 +210,21 @@ def test_plane_apply_3d_matches_oracle(ctx, kind, shape):
     assert err < 1e-13
 
 
+@pytest.mark.parametrize("key", ["mixed.replace", "mixed.xpay"])
 @pytest.mark.parametrize("kind,shape,max_iter", [
     ("flux", (33, 65, 129), 100), ("flux", (69, 61, 37), 100), ("const", (17, 129, 65), 100),
     # stopped by max_iter: after a replacement (the FP32 correction void) and between two
     ("flux", (33, 65, 129), 3), ("flux", (33, 65, 129), 4), ("flux", (69, 61, 37), 5)])
-def test_fused_replacement_bitwise(ctx, kind, shape, max_iter):
-    """Mixed MG-PCG residual replacement in one plane-march launch (the FP32 correction folded
-    into the other FP64 solution buffer, FP64 residual, scaled FP32 copy) against the three
-    launches it replaces: the same solution, iterations, replacements and final residual bit
-    for bit, converged or stopped by max_iter."""
-    print(f"fused replacement {kind} {shape} max_iter={max_iter}", flush=True)
+def test_fused_replacement_bitwise(ctx, kind, shape, max_iter, key):
+    """The mixed MG-PCG steps fused into plane marches against the launches they replace: the
+    residual replacement in one launch (the FP32 correction folded into the other FP64 solution
+    buffer, FP64 residual, scaled FP32 copy; key mixed.replace) and the direction update inside
+    the operator apply (key mixed.xpay).  The same solution, iterations, replacements and final
+    residual bit for bit, converged or stopped by max_iter."""
+    print(f"fused {key} {kind} {shape} max_iter={max_iter}", flush=True)
     s = (flux_problem if kind == "flux" else const_problem)(shape, seed=sum(shape) + 7)
-    u0, r0 = solve(ctx, s, capi.MIXED, 1, max_iter=max_iter, **{"mixed.replace": 0})
-    u1, r1 = solve(ctx, s, capi.MIXED, 1, max_iter=max_iter, **{"mixed.replace": 1})
+    u0, r0 = solve(ctx, s, capi.MIXED, 1, max_iter=max_iter, **{key: 0})
+    u1, r1 = solve(ctx, s, capi.MIXED, 1, max_iter=max_iter, **{key: 1})
     print(f"  iterations {r0.iterations}/{r1.iterations} refinements {r0.refinements}/"
           f"{r1.refinements} max diff {np.abs(u0 - u1).max():.3e}", flush=True)
     assert (r0.iterations, r0.refinements) == (r1.iterations, r1.refinements)
