@@ -455,14 +455,32 @@ __global__ void k_flush_f32(double* __restrict__ x, float* __restrict__ e, long 
 // r / snorm with snorm = |r0| fixed; a replacement is due once the recurrence residual has
 // fallen by `delta` below the largest one since the last replacement, or claims the outer
 // target.  Only a true (FP64) residual stops the solve.
+__device__ void fin_ru_step(KScalars* si, KScalars* so, KScalars* gate, double s, int op,
+                            double delta);
+
+// With maxp: afterwards, as k_fin_scale on si, the input normalisation from the max partials.
 __global__ void __launch_bounds__(1024) k_fin_ru(KScalars* si, KScalars* so, KScalars* gate,
                                                 const double* partials, int np, int op,
-                                                double delta) {
+                                                double delta, const double* maxp, int nm) {
   __shared__ double lds[1024 / kWave];
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partials[i];
   const double s = block_sum(acc, lds);
+  double mx = 0.0;
+  if (maxp) {
+    for (int i = threadIdx.x; i < nm; i += blockDim.x) mx = nmax(mx, maxp[i]);
+    mx = block_max(mx, lds);
+  }
   if (threadIdx.x != 0) return;
+  fin_ru_step(si, so, gate, s, op, delta);
+  if (maxp && !(si->done | si->breakdown)) {
+    si->inv_scale = mx > 0.0 ? mx : 1.0;
+    si->scale = mx > 0.0 ? 1.0 / mx : 1.0;
+  }
+}
+
+__device__ void fin_ru_step(KScalars* si, KScalars* so, KScalars* gate, double s, int op,
+                            double delta) {
   switch (op) {
     case FIN_RU_START:
       so->snorm = so->metric > 0.0 ? so->metric : 1.0;
@@ -539,8 +557,10 @@ void flush_f32(hipStream_t s, double* x64, float* x32, long long n, const double
 }
 
 void finalize_ru(hipStream_t s, KScalars* si, KScalars* so, KScalars* gate,
-                 const double* partials, int np, int op, double delta) {
-  hipLaunchKernelGGL(k_fin_ru, dim3(1), dim3(1024), 0, s, si, so, gate, partials, np, op, delta);
+                 const double* partials, int np, int op, double delta, const double* maxp,
+                 int nmax) {
+  hipLaunchKernelGGL(k_fin_ru, dim3(1), dim3(1024), 0, s, si, so, gate, partials, np, op, delta,
+                     maxp, nmax);
 }
 
 int maxabs_blocks(long long n) { return static_cast<int>(misc_blocks(n) > 1024 ? 1024 : misc_blocks(n)); }

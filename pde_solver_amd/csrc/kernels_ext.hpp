@@ -43,9 +43,11 @@ void to_f32_scaled_gated(hipStream_t s, const double* in, float* out, long long 
 void flush_f32(hipStream_t s, double* x64, float* x32, long long n, const double* scale,
                const KScalars* gate);
 // FIN_RU_START / FIN_RU_CHECK (tol = the replacement threshold delta) / FIN_RU_APPLY on the
-// inner (si) and outer (so) scalars; gate->done = !(replacement due).
+// inner (si) and outer (so) scalars; gate->done = !(replacement due).  With maxp, the same
+// launch then does fin_scale(maxp, nmax, si).
 void finalize_ru(hipStream_t s, KScalars* si, KScalars* so, KScalars* gate,
-                 const double* partials, int np, int op, double delta);
+                 const double* partials, int np, int op, double delta,
+                 const double* maxp = nullptr, int nmax = 0);
 int maxabs_blocks(long long n);
 void maxabs32(hipStream_t s, const float* a, long long n, double* partials);
 void fin_scale(hipStream_t s, const double* partials, int np, KScalars* sc);

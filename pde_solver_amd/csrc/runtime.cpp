@@ -1044,8 +1044,7 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
       finalize(Li, si, part, partial_blocks(op32), FIN_CG_ALPHA, 0, 0, nullptr);
       cg_update(Li, P32, x32, r32, pcur, P.q32.get(), n, part, part2,
                 fused ? &so->fresh : nullptr);
-      finalize_ru(s, si, so, gate, part, vec_blocks(n), FIN_RU_CHECK, delta);
-      fin_scale(s, part2, vec_blocks(n), si);
+      finalize_ru(s, si, so, gate, part, vec_blocks(n), FIN_RU_CHECK, delta, part2, vec_blocks(n));
       // replacement (only when due): x += snorm x32, r = b - A x in FP64, r32 = r / snorm
       if (fused) {
         replace_plane(Lg, op64, P.bvec.as<double>(), xs, x32, r64, r32, so, part);
