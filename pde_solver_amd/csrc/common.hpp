@@ -205,7 +205,7 @@ __device__ __forceinline__ T apply_row(const W7<T>& w, T xc, T xe, T xw, T xn, T
 // as a fixed FMA chain (E, W, N, S, F, B).  Every smoother kernel uses this, so the fused and
 // the in-place sweeps agree bit for bit whatever the compiler's contraction choices.
 template <int DIM, typename T>
-__device__ __forceinline__ T gs_value(const W7<T>& w, T b, T ue, T uw, T un, T us, T uf, T ub) {
+__device__ __forceinline__ T gs_sum(const W7<T>& w, T ue, T uw, T un, T us, T uf, T ub) {
   T s = w.E * ue;
   s = fma(w.W, uw, s);
   s = fma(w.N, un, s);
@@ -214,7 +214,11 @@ __device__ __forceinline__ T gs_value(const W7<T>& w, T b, T ue, T uw, T un, T u
     s = fma(w.F, uf, s);
     s = fma(w.B, ub, s);
   }
-  return (b - s) * recip<T>(w.C);
+  return s;
+}
+template <int DIM, typename T>
+__device__ __forceinline__ T gs_value(const W7<T>& w, T b, T ue, T uw, T un, T us, T uf, T ub) {
+  return (b - gs_sum<DIM, T>(w, ue, uw, un, us, uf, ub)) * recip<T>(w.C);
 }
 
 __device__ __forceinline__ bool is_boundary2(const DevOp& op, int i, int j) {

@@ -491,19 +491,20 @@ k_rbgs_ring3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin
 // planes and three front-face planes.
 constexpr int kRR = 4;
 
-__device__ __forceinline__ float dpp_west(float v) {   // lane l gets lane l-1 (lane 0: itself)
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v),
-                                                               __builtin_bit_cast(int, v), 0x138,
-                                                               0xf, 0xf, false));
+// Wave shifts with zero fill (bound_ctrl), so the compiler can fold the shift into the VOP2
+// instruction that consumes it.  Lanes 0 and 63 read 0: they are the tile's outer halo, whose
+// faces, centres and updates are formed but never selected or stored.
+__device__ __forceinline__ float dpp_west(float v) {   // lane l gets lane l-1 (lane 0: 0)
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138,
+                                                            0xf, 0xf, true));
 }
-__device__ __forceinline__ float dpp_east(float v) {   // lane l gets lane l+1 (lane 63: itself)
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v),
-                                                               __builtin_bit_cast(int, v), 0x130,
-                                                               0xf, 0xf, false));
+__device__ __forceinline__ float dpp_east(float v) {   // lane l gets lane l+1 (lane 63: 0)
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130,
+                                                            0xf, 0xf, true));
 }
 
 template <int KIND, Prec P, bool ZERO, bool DOT = false>
-__global__ void __launch_bounds__(kThreads, 2)   // two workgroups per CU: at most 128 VGPRs
+__global__ void __launch_bounds__(kThreads, 4)   // four waves per SIMD (two workgroups per CU)
 k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v,
             void* __restrict__ xout_v, int c1, int tiles_x, int tiles_y, int ntiles,
             int kchunk, const KScalars* sc, double* partials = nullptr) {
@@ -587,10 +588,11 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
   // Rings, with plane pstart + s at slot s mod L: U (6) x(p-2 .. p+3), B (3) b(p-1 .. p+1),
   // K (3) rho(p+1 .. p+3), ZF (3) zf(p-2 .. p) and FE / FN (2) faces of planes p-1 .. p during
   // the colour stages of step p; the faces of plane p+1 are formed after them, in the slots the
-  // second stage has just finished with.  The first step is the first colour's ring plane
+  // second stage has just finished with.  RC (2): recip of the folded centre, formed by a node's
+  // first stage and reused by its second (the same value gs_value would recompute).  The first step is the first colour's ring plane
   // k0 - 1; the faces and front faces it needs are formed before the loop.
   const int pstart = k0 - 1;
-  T U[R][6], B[R][3], K[R][3], ZF[R][3], FE[R][2], FN[R][2];
+  T U[R][6], B[R][3], K[R][3], ZF[R][3], FE[R][2], FN[R][2], RC[R][2];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     U[r][4] = ld_u(r, pstart - 2);
@@ -608,7 +610,7 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
 #pragma unroll
     for (int s = 0; s < 3; ++s) ZF[r][s] = T(0);
 #pragma unroll
-    for (int s = 0; s < 2; ++s) { FE[r][s] = T(0); FN[r][s] = T(0); }
+    for (int s = 0; s < 2; ++s) { FE[r][s] = T(0); FN[r][s] = T(0); RC[r][s] = T(0); }
   }
   if constexpr (HK) {
     // faces of plane k0 - 1 (slot 0) and front faces zf(k0 - 2) (slot 2), zf(k0 - 1) (slot 0)
@@ -663,11 +665,12 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
         }
         W7<T> w = weights_at<3, KIND, T>(op, 0, aw, ae, as, an, ZF[r][z_m1], ZF[r][z_0]);
         if (((m_fxy >> r) & 1u) || p == 1 || p == nz - 2) fold_center<3, T>(op, i, j, p, w);
+        RC[r][f_0] = recip<T>(w.C);
         const T xc = U[r][u_0];
         const T xn = r < R - 1 ? U[r + 1][u_0] : lxb[sx][wup][0][lane];
         const T xs = r > 0 ? U[r - 1][u_0] : lxb[sx][wdn][1][lane];
-        const T v = rounded<S, T>(gs_value<3, T>(w, B[r][b_0], dpp_east(xc), dpp_west(xc), xn, xs,
-                                                 U[r][u_1], U[r][u_m1]));
+        const T v = rounded<S, T>((B[r][b_0] - gs_sum<3, T>(w, dpp_east(xc), dpp_west(xc), xn, xs,
+                                                            U[r][u_1], U[r][u_m1])) * RC[r][f_0]);
         U[r][u_0] = upd ? v : xc;
       }
       lxb[sx][wv][0][lane] = U[0][u_0];
@@ -689,13 +692,14 @@ k_rbgs_reg3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_
             an = FN[r][f_m1];
             as = r > 0 ? FN[r - 1][f_m1] : lfn[g_m1][wdn][lane];
           }
-          W7<T> w = weights_at<3, KIND, T>(op, 0, aw, ae, as, an, ZF[r][z_m2], ZF[r][z_m1]);
-          if (((m_fxy >> r) & 1u) || q == 1 || q == nz - 2) fold_center<3, T>(op, i, j, q, w);
+          // neighbour weights only: the folded centre's reciprocal is the first stage's
+          const W7<T> w = weights_at<3, KIND, T>(op, 0, aw, ae, as, an, ZF[r][z_m2], ZF[r][z_m1]);
           const T xc = U[r][u_m1];
           const T xn = r < R - 1 ? U[r + 1][u_m1] : lxb[sxq][wup][0][lane];
           const T xs = r > 0 ? U[r - 1][u_m1] : lxb[sxq][wdn][1][lane];
-          const T v = rounded<S, T>(gs_value<3, T>(w, B[r][b_m1], dpp_east(xc), dpp_west(xc), xn,
-                                                   xs, U[r][u_0], U[r][u_m2]));
+          const T v = rounded<S, T>((B[r][b_m1] - gs_sum<3, T>(w, dpp_east(xc), dpp_west(xc), xn,
+                                                               xs, U[r][u_0], U[r][u_m2])) *
+                                    RC[r][f_m1]);
           U[r][u_m1] = upd ? v : xc;
         }
       }
@@ -1163,10 +1167,15 @@ int launch3_reg(const Launch& L, const DevOp& op, const void* b, const void* xin
   const int ntiles = tiles_x * tiles_y * chunks;
   const dim3 g(static_cast<unsigned>(8 * ((ntiles + 7) / 8)));
   if (partials) {
-    if (zero_in || static_cast<int>(g.x) > cap) return 0;
-    hipLaunchKernelGGL((k_rbgs_reg3<KIND, P, false, true>), g, dim3(kThreads), 0, L.stream, op, b,
-                       xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc, partials);
-    return static_cast<int>(g.x);
+    // the fused dot runs on FP32 levels only (an FP16 level's dot sweep is the ring kernel's)
+    if constexpr (P == P32) {
+      if (zero_in || static_cast<int>(g.x) > cap) return 0;
+      hipLaunchKernelGGL((k_rbgs_reg3<KIND, P, false, true>), g, dim3(kThreads), 0, L.stream, op,
+                         b, xin, xout, c1, tiles_x, tiles_y, ntiles, kchunk, L.sc, partials);
+      return static_cast<int>(g.x);
+    } else {
+      return 0;
+    }
   }
   if (zero_in)
     hipLaunchKernelGGL((k_rbgs_reg3<KIND, P, true>), g, dim3(kThreads), 0, L.stream, op, b, xin,
@@ -1260,13 +1269,13 @@ int rbgs_sweep_dot(const Launch& L, const DevOp& op, Prec p, const void* b, cons
   const bool ring = g_tune.ring != 0 && g_tune.rows == 2 && g_tune.pair == 0 && g_tune.dot != 0 &&
                     op.sxy < (1LL << 29) && op.nz > 1 && (p == P32 || p == P16);
   if (!ring) return 0;
-  if (g_tune.reg) {
+  // register rows for FP32 levels; an FP16 level (not a mixed plan's level 0) takes the ring
+  // sweep, whose FP16 dot variant fits the register budget without spilling
+  if (g_tune.reg && p == P32) {
     if (op.kind == PDEHIP_OP_CONST)
-      return p == P32 ? launch3_reg<PDEHIP_OP_CONST, P32>(L, op, b, xin, xout, first_color, false, partials, cap)
-                      : launch3_reg<PDEHIP_OP_CONST, P16>(L, op, b, xin, xout, first_color, false, partials, cap);
+      return launch3_reg<PDEHIP_OP_CONST, P32>(L, op, b, xin, xout, first_color, false, partials, cap);
     if (op.kind == PDEHIP_OP_FLUX)
-      return p == P32 ? launch3_reg<PDEHIP_OP_FLUX, P32>(L, op, b, xin, xout, first_color, false, partials, cap)
-                      : launch3_reg<PDEHIP_OP_FLUX, P16>(L, op, b, xin, xout, first_color, false, partials, cap);
+      return launch3_reg<PDEHIP_OP_FLUX, P32>(L, op, b, xin, xout, first_color, false, partials, cap);
     return 0;
   }
   if (op.kind == PDEHIP_OP_CONST)
