@@ -12,6 +12,7 @@
 // after it was read, and a thread can only be there once every thread has passed the barrier
 // in between.  The FP64 operator divides exactly (face_coef); the FP32/FP16 operators keep
 // rho = 1/kappa and form each face with one hardware reciprocal, within their own rounding.
+#include <algorithm>
 #include <type_traits>
 
 #include "bufio.hpp"
@@ -41,6 +42,15 @@ __device__ __forceinline__ T face_op(T kc, T kn, T f) {
   return face_coef<T>(kc, kn, f);
 }
 
+// A launch with fewer workgroups than the npart partials its finalize sums (4-byte levels run
+// taller tiles than FP64 ones, so fewer of them) zeroes the ones past its grid.
+__device__ __forceinline__ void pad_partials(double* partials, int npart) {
+  if (partials && blockIdx.x == 0)
+    for (int z = static_cast<int>(gridDim.x) + static_cast<int>(threadIdx.x); z < npart;
+         z += static_cast<int>(blockDim.x))
+      partials[z] = 0.0;
+}
+
 // Residual replacement of a mixed solve (MODE 2): the two FP64 solution buffers, the FP32
 // correction, the scaled FP32 residual out and the outer scalars (xsel, snorm).
 struct ReplArgs {
@@ -55,8 +65,8 @@ struct ReplArgs {
 template <int KIND, Prec P, int MODE, int R>
 __global__ void __launch_bounds__(kThreads)
 k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, void* __restrict__ yv,
-         double* partials, int tiles_x, int tiles_y, int ntiles, int kchunk, const KScalars* sc,
-         ReplArgs ra) {
+         double* partials, int npart, int tiles_x, int tiles_y, int ntiles, int kchunk,
+         const KScalars* sc, ReplArgs ra) {
   static_assert(MODE != 2 || P == P64, "fused replacement: FP64 operator only");
   using T = typename PT<P>::T;
   using S = typename PT<P>::S;
@@ -71,6 +81,7 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
   __shared__ T lfn[HK ? 2 : 1][KR][kLanes];
   __shared__ double lred[kWaves];
   if (skip(sc)) return;
+  pad_partials(partials, npart);
   const int per = static_cast<int>(gridDim.x >> 3);
   const int t = static_cast<int>(blockIdx.x & 7) * per + static_cast<int>(blockIdx.x >> 3);
   if (t >= ntiles) {
@@ -254,8 +265,8 @@ constexpr int kRingP = 6;
 template <int KIND, Prec P, int MODE, int R>
 __global__ void __launch_bounds__(kThreads)
 k_plane3r(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, void* __restrict__ yv,
-          double* partials, int tiles_x, int tiles_y, int ntiles, int kchunk, const KScalars* sc,
-          void* __restrict__ pov) {
+          double* partials, int npart, int tiles_x, int tiles_y, int ntiles, int kchunk,
+          const KScalars* sc, void* __restrict__ pov) {
   using T = typename PT<P>::T;
   using S = typename PT<P>::S;
   static_assert(sizeof(T) == 4, "ring plane march: 4-byte arithmetic types only");
@@ -273,6 +284,7 @@ k_plane3r(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, vo
   __shared__ T lfn[HK ? 2 : 1][KR + 2][PW];
   __shared__ double lred[kWaves];
   if (skip(sc)) return;
+  pad_partials(partials, npart);
   const int per = static_cast<int>(gridDim.x >> 3);
   const int t = static_cast<int>(blockIdx.x & 7) * per + static_cast<int>(blockIdx.x >> 3);
   if (t >= ntiles) {
@@ -460,17 +472,29 @@ k_plane3r(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, vo
 
 struct Shape { int tiles_x, tiles_y, chunks, ntiles, kchunk, blocks; };
 
-// Launch shape shared by every instantiation (the partials count must not depend on the
-// precision): occupancy of the FP64 FLUX residual, the heaviest of them.
-Shape shape_of(const DevOp& op) {
-  constexpr int R = 2;
+// Rows per wave of a plane march: 2 (tile 62 x 14), except the 4-byte residual: 4 (tile 62 x 30,
+// less halo per output node and one barrier per 4 rows; measured faster in a mixed solve at
+// 257^3, where the 4-row apply and fused direction update are slower than at 2 rows: at their
+// register count two workgroups fit a CU against three at 2 rows).  The FP64 tile's LDS planes
+// would not leave room for two workgroups per CU at 4 rows.  Ring and generic kernels of one
+// precision and mode share the shape, so their partials are the same, and so do the apply and
+// the fused direction update.
+template <Prec P, int MODE> constexpr int plane_rows() { return P != P64 && MODE == 1 ? 4 : 2; }
+
+// Launch shape of an R-row plane march.  Occupancy is that of the heaviest instantiation of
+// the row count (FP64 FLUX residual at 2 rows, FP32 FLUX ring residual at 4), so every
+// instantiation with the same R has the same shape.  The larger of the two shapes' workgroup
+// counts is the partials count of every plane march on the grid (plane_blocks); the smaller
+// launch zeroes the partials past its grid.
+Shape shape_of(const DevOp& op, int R) {
   const int ty = R * kWaves - 2;
   Shape s;
   s.tiles_x = (op.nx + kLanes - 3) / (kLanes - 2);
   s.tiles_y = (op.ny + ty - 1) / ty;
-  static const int occ = blocks_per_cu(k_plane3<PDEHIP_OP_FLUX, P64, 1, R>, kThreads);
+  static const int occ2 = blocks_per_cu(k_plane3<PDEHIP_OP_FLUX, P64, 1, 2>, kThreads);
+  static const int occ4 = blocks_per_cu(k_plane3r<PDEHIP_OP_FLUX, P32, 1, 4>, kThreads);
   const long long txy = (long long)s.tiles_x * s.tiles_y;
-  s.kchunk = plane_chunk(txy, op.nz, device_cus() * occ, 2, 4);
+  s.kchunk = plane_chunk(txy, op.nz, device_cus() * (R == 2 ? occ2 : occ4), 2, 4);
   // at most kMaxPartials workgroups (the reduction partials' capacity)
   while (s.kchunk < op.nz && txy * ((op.nz + s.kchunk - 1) / s.kchunk) > kMaxPartials - 8) s.kchunk *= 2;
   s.chunks = (op.nz + s.kchunk - 1) / s.kchunk;
@@ -484,18 +508,20 @@ bool g_plane_ring = true;
 template <int KIND, Prec P, int MODE>
 void launch(const Launch& L, const DevOp& op, const void* b, const void* x, void* y,
             double* partials) {
-  const Shape s = shape_of(op);
+  constexpr int R = plane_rows<P, MODE>();
+  const Shape s = shape_of(op, R);
+  const int npart = plane_blocks(op);
   // ring variant: 4-byte arithmetic, buffer offsets of a plane below 2^31 bytes
   if constexpr (sizeof(typename PT<P>::T) == 4 && KIND != PDEHIP_OP_POINTWISE) {
     if (g_plane_ring && op.sxy < (1LL << 29)) {
-      hipLaunchKernelGGL((k_plane3r<KIND, P, MODE, 2>), dim3(s.blocks), dim3(kThreads), 0,
-                         L.stream, op, b, x, y, partials, s.tiles_x, s.tiles_y, s.ntiles,
+      hipLaunchKernelGGL((k_plane3r<KIND, P, MODE, R>), dim3(s.blocks), dim3(kThreads), 0,
+                         L.stream, op, b, x, y, partials, npart, s.tiles_x, s.tiles_y, s.ntiles,
                          s.kchunk, L.sc, nullptr);
       return;
     }
   }
-  hipLaunchKernelGGL((k_plane3<KIND, P, MODE, 2>), dim3(s.blocks), dim3(kThreads), 0, L.stream,
-                     op, b, x, y, partials, s.tiles_x, s.tiles_y, s.ntiles, s.kchunk, L.sc,
+  hipLaunchKernelGGL((k_plane3<KIND, P, MODE, R>), dim3(s.blocks), dim3(kThreads), 0, L.stream,
+                     op, b, x, y, partials, npart, s.tiles_x, s.tiles_y, s.ntiles, s.kchunk, L.sc,
                      ReplArgs{});
 }
 
@@ -520,7 +546,7 @@ void launch_mode(const Launch& L, const DevOp& op, Prec p, const void* b, const 
 
 }  // namespace
 
-int plane_blocks(const DevOp& op) { return shape_of(op).blocks; }
+int plane_blocks(const DevOp& op) { return std::max(shape_of(op, 2).blocks, shape_of(op, 4).blocks); }
 
 void set_plane_ring(bool on) { g_plane_ring = on; }
 bool plane_ring() { return g_plane_ring; }
@@ -539,27 +565,30 @@ bool apply_xpay_plane(const Launch& L, const DevOp& op, Prec p, const void* z, c
   if (op.nz <= 1 || !plane_stencils() || !g_plane_ring || p != P32 || op.sxy >= (1LL << 29) ||
       op.kind == PDEHIP_OP_POINTWISE || p_in == p_out)
     return false;
-  const Shape s = shape_of(op);
+  constexpr int R = plane_rows<P32, 3>();
+  const Shape s = shape_of(op, R);
+  const int npart = plane_blocks(op);
   if (op.kind == PDEHIP_OP_CONST)
-    hipLaunchKernelGGL((k_plane3r<PDEHIP_OP_CONST, P32, 3, 2>), dim3(s.blocks), dim3(kThreads), 0,
-                       L.stream, op, z, p_in, y, partials, s.tiles_x, s.tiles_y, s.ntiles, s.kchunk,
-                       L.sc, p_out);
+    hipLaunchKernelGGL((k_plane3r<PDEHIP_OP_CONST, P32, 3, R>), dim3(s.blocks), dim3(kThreads), 0,
+                       L.stream, op, z, p_in, y, partials, npart, s.tiles_x, s.tiles_y, s.ntiles,
+                       s.kchunk, L.sc, p_out);
   else
-    hipLaunchKernelGGL((k_plane3r<PDEHIP_OP_FLUX, P32, 3, 2>), dim3(s.blocks), dim3(kThreads), 0,
-                       L.stream, op, z, p_in, y, partials, s.tiles_x, s.tiles_y, s.ntiles, s.kchunk,
-                       L.sc, p_out);
+    hipLaunchKernelGGL((k_plane3r<PDEHIP_OP_FLUX, P32, 3, R>), dim3(s.blocks), dim3(kThreads), 0,
+                       L.stream, op, z, p_in, y, partials, npart, s.tiles_x, s.tiles_y, s.ntiles,
+                       s.kchunk, L.sc, p_out);
   return true;
 }
 
 bool replace_plane(const Launch& Lgate, const DevOp& op64, const double* b, double* const x[2],
                    const float* e, double* r64, float* r32, const KScalars* so, double* partials) {
   if (op64.nz <= 1 || !plane_stencils()) return false;
-  const Shape s = shape_of(op64);
+  const Shape s = shape_of(op64, plane_rows<P64, 2>());
+  const int npart = plane_blocks(op64);
   const ReplArgs ra{{x[0], x[1]}, e, r32, so};
 #define PDEHIP_REPL_(K)                                                                         \
-  hipLaunchKernelGGL((k_plane3<K, P64, 2, 2>), dim3(s.blocks), dim3(kThreads), 0, Lgate.stream, \
-                     op64, b, nullptr, r64, partials, s.tiles_x, s.tiles_y, s.ntiles, s.kchunk,  \
-                     Lgate.sc, ra)
+  hipLaunchKernelGGL((k_plane3<K, P64, 2, plane_rows<P64, 2>()>), dim3(s.blocks), dim3(kThreads), \
+                     0, Lgate.stream, op64, b, nullptr, r64, partials, npart, s.tiles_x,         \
+                     s.tiles_y, s.ntiles, s.kchunk, Lgate.sc, ra)
   switch (op64.kind) {
     case PDEHIP_OP_CONST: PDEHIP_REPL_(PDEHIP_OP_CONST); break;
     case PDEHIP_OP_POINTWISE: PDEHIP_REPL_(PDEHIP_OP_POINTWISE); break;
