@@ -266,7 +266,9 @@ void pdehip_plan_destroy(pdehip_plan* plan);
  * launch (fold the FP32 correction into the FP64 solution, FP64 residual, scaled FP32 copy) or
  * in three; the same bits either way.  "mixed.xpay" (0/1, default 1): mixed MG-PCG on 3D grids
  * forms the new search direction inside the operator apply (one launch instead of two); the
- * same bits either way.
+ * same bits either way.  "restrict.march" (0/1, default 1): 3D multigrid restrictions march the
+ * coarse planes (each fine plane loaded once, weighted in the plane and carried to the next
+ * coarse plane) instead of gathering 27 fine values per coarse node; the same bits either way.
  * Not thread-safe against running solves. */
 int pdehip_tune_set(const char* key, int value);
 int pdehip_tune_get(const char* key, int* value);

@@ -276,6 +276,7 @@ int pdehip_tune_set(const char* key, int value) {
   else if (k == "mixed.fp16" && (value == 0 || value == 1)) { set_mixed_fp16(value == 1); return PDEHIP_OK; }
   else if (k == "mixed.replace" && (value == 0 || value == 1)) { set_fused_replace(value == 1); return PDEHIP_OK; }
   else if (k == "mixed.xpay" && (value == 0 || value == 1)) { set_fused_xpay(value == 1); return PDEHIP_OK; }
+  else if (k == "restrict.march" && (value == 0 || value == 1)) { set_restrict_march(value == 1); return PDEHIP_OK; }
   else if (k == "time.fused" && (value == 0 || value == 1)) { set_fused_steps(value == 1); return PDEHIP_OK; }
   else return PDEHIP_ERR_INVALID;
   set_smooth_tuning(t);
@@ -303,6 +304,7 @@ int pdehip_tune_get(const char* key, int* value) {
   else if (k == "mixed.fp16") *value = mixed_fp16() ? 1 : 0;
   else if (k == "mixed.replace") *value = fused_replace() ? 1 : 0;
   else if (k == "mixed.xpay") *value = fused_xpay() ? 1 : 0;
+  else if (k == "restrict.march") *value = restrict_march() ? 1 : 0;
   else if (k == "time.fused") *value = fused_steps() ? 1 : 0;
   else return PDEHIP_ERR_INVALID;
   return PDEHIP_OK;

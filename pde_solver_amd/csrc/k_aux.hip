@@ -14,6 +14,9 @@ template <> struct PT<P16> { using T = float; using S = __half; };
 __device__ __forceinline__ double ldd(const double* p, long long i) { return p[i]; }
 __device__ __forceinline__ double ldd(const float* p, long long i) { return p[i]; }
 __device__ __forceinline__ double ldd(const __half* p, long long i) { return __half2float(p[i]); }
+__device__ __forceinline__ double to_double(double v) { return v; }
+__device__ __forceinline__ double to_double(float v) { return v; }
+__device__ __forceinline__ double to_double(__half v) { return __half2float(v); }
 
 __device__ __forceinline__ bool skip(const KScalars* sc) { return sc && (sc->done | sc->breakdown); }
 
@@ -245,6 +248,217 @@ __global__ void __launch_bounds__(kTX * kTY) k_restrict3(DevOp f, const void* rf
     int ic, jc;
     if (!rim_node(c.nx, c.ny, r, &ic, &jc)) return;
     restrict3_node<SF, SC, true>(f, rf, c, bc, ic, jc, kc, s);
+  }
+}
+
+// restrict3_node<..., RIM = true> for a node next to an x or y face (ic or jc in {1, n-2}), with
+// the same operations in the same order, its fine values loaded first: each folded in-plane
+// weighting Q(k) reads the (up to) 2 x 2 fw_clip windows of its fold lists at once, clamped into
+// the grid, instead of one dependent load group per window and row.  The march's per-node
+// workgroups serve the rim with it.
+template <typename SF, typename SC>
+__device__ __forceinline__ void restrict_rim_node(const DevOp& f, const SF* __restrict__ rf,
+                                                  const DevOp& c, SC* __restrict__ bc, int ic,
+                                                  int jc, int kc, double s) {
+  const long long out = ((long long)kc * c.ny + jc) * c.nx + ic;
+  if (ic == 0 || jc == 0 || kc == 0 || ic == c.nx - 1 || jc == c.ny - 1 || kc == c.nz - 1) {
+    bc[out] = store_as<SC>(0.0);
+    return;
+  }
+  if (!(ic == 1 || jc == 1 || ic == c.nx - 2 || jc == c.ny - 2)) {
+    restrict3_node<SF, SC, true>(f, rf, c, bc, ic, jc, kc, s);
+    return;
+  }
+  const long long fsxy = (long long)f.nx * f.ny;
+  // fold lists (fold_list; at most two entries per axis, as the coarse sides are at least 5)
+  const int xi[2] = {ic, ic == 1 ? 0 : c.nx - 1}, yi[2] = {jc, jc == 1 ? 0 : c.ny - 1};
+  const double xw[2] = {1.0, ic == 1 ? c.theta[0] : c.theta[1]};
+  const double yw[2] = {1.0, jc == 1 ? c.theta[2] : c.theta[3]};
+  const int mx = ic == 1 || ic == c.nx - 2 ? 2 : 1, my = jc == 1 || jc == c.ny - 2 ? 2 : 1;
+  // restrict_value's Q: sum over the fold lists of w * fw_clip(2 xi, 2 yi, k)
+  auto Q = [&](int fk) -> double {
+    const SF* __restrict__ pl = rf + fk * fsxy;
+    double v[2][2][3][3];   // [y entry][x entry][row fj-1 .. fj+1][column fi-1 .. fi+1]
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int fi = 2 * xi[q], fj = 2 * yi[b];
+#pragma unroll
+        for (int dj = 0; dj < 3; ++dj) {
+          const long long row = (long long)min(max(fj + dj - 1, 0), f.ny - 1) * f.nx;
+#pragma unroll
+          for (int di = 0; di < 3; ++di)
+            v[b][q][dj][di] = ldd(pl, row + min(max(fi + di - 1, 0), f.nx - 1));
+        }
+      }
+    const double w1[3] = {0.25, 0.5, 0.25};
+    double acc = 0.0;
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (b >= my || q >= mx) continue;
+        const double w = yw[b] * xw[q];
+        if (w == 0.0) continue;
+        const int fi = 2 * xi[q], fj = 2 * yi[b];
+        double clip = 0.0;   // fw_clip(f, r, fi, fj, fk)
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj) {
+          const int j = fj + dj;
+          if (j < 0 || j >= f.ny) continue;
+          double a = 0.5 * v[b][q][dj + 1][1];
+          if (fi > 0) a = fma(0.25, v[b][q][dj + 1][0], a);
+          if (fi < f.nx - 1) a = fma(0.25, v[b][q][dj + 1][2], a);
+          clip = fma(w1[dj + 1], a, clip);
+        }
+        acc = fma(w, clip, acc);
+      }
+    return acc;
+  };
+  const int fk = 2 * kc;
+  const double ql = Q(fk - 1), qm = Q(fk), qh = Q(fk + 1);
+  double v = fma(0.25, qh, fma(0.5, qm, 0.25 * ql));
+  if (kc == 1 && c.theta[4] != 0.0) v = fma(c.theta[4], fma(0.25, ql, 0.5 * Q(0)), v);
+  if (kc == c.nz - 2 && c.theta[5] != 0.0) v = fma(c.theta[5], fma(0.5, Q(fk + 2), 0.25 * qh), v);
+  bc[out] = store_as<SC>(v * s);
+}
+
+// 3D restriction as a plane march over the interior box [2, nx-3] x [2, ny-3] of every interior
+// coarse plane; the rim (restrict_rim_node) and the boundary planes take the per-node path in
+// the workgroups past the march's.  A wave owns kRMRows coarse rows of
+// kRMCols = 63 coarse columns (lane l: coarse column ic, fine columns 2ic and 2ic+1; lane 0 is the
+// west halo column whose 2ic+1 value its east neighbour needs, passed by a DPP wave shift), a
+// workgroup kRMWaves such waves, and a chunk of coarse planes.  Each fine plane of the chunk is
+// loaded once (two values per lane per fine row, the 2 kRMRows + 1 fine rows under the wave's
+// coarse rows), weighted in the plane (fw_row3 per fine row, fw_col3 per coarse row: fw_in
+// operation for operation) and carried: coarse plane kc's low fine plane 2kc-1 is the previous
+// plane's high one.  The z combination and the z folds are restrict_value's, so every coarse
+// value equals the per-node kernel's bit for bit.
+constexpr int kRMLanes = 64;
+constexpr int kRMRows = 4;
+constexpr int kRMWaves = 4;
+constexpr int kRMCols = kRMLanes - 1;
+
+// lane l gets lane l-1's v (lane 0: 0, a halo lane whose result is never stored)
+__device__ __forceinline__ double dpp_west_d(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(u & 0xffffffffu), 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(u >> 32), 0x138, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+                                        static_cast<unsigned>(lo));
+}
+
+template <Prec PF, Prec PC>
+__global__ void __launch_bounds__(kRMLanes * kRMWaves)
+k_restrict3m(DevOp f, const void* rfv, DevOp c, void* bcv, double scale, const double* dscale,
+             const KScalars* sc, int ntx, int nty, int kchunk, int nmarch, long long nrim_items) {
+  using SF = typename PT<PF>::S;
+  using SC = typename PT<PC>::S;
+  constexpr int R = kRMRows;
+  constexpr int FR = 2 * R + 1;
+  if (skip(sc)) return;
+  const SF* __restrict__ rf = static_cast<const SF*>(rfv);
+  SC* __restrict__ bc = static_cast<SC*>(bcv);
+  const double s = dscale ? scale * (*dscale) : scale;
+  const long long fsxy = (long long)f.nx * f.ny;
+  const int nrb = static_cast<int>(gridDim.x) - nmarch;   // per-node workgroups, dispatched last
+  int b = static_cast<int>(blockIdx.x) - nmarch;
+  if (b >= 0) {
+    // boundary planes (every node, zero), then the rim of each interior plane
+    const long long plane = (long long)c.nx * c.ny;
+    const int bx = c.nx - 4, by = c.ny - 4;
+    const long long nrim = bx > 0 && by > 0 ? 4LL * c.nx + 4LL * by : plane;
+    for (long long it = (long long)b * blockDim.x + threadIdx.x; it < nrim_items;
+         it += (long long)nrb * blockDim.x) {
+      int ic, jc, kc;
+      if (it < 2 * plane) {
+        kc = it < plane ? 0 : c.nz - 1;
+        const long long q = it % plane;
+        jc = static_cast<int>(q / c.nx);
+        ic = static_cast<int>(q % c.nx);
+      } else {
+        const long long q = it - 2 * plane;
+        kc = 1 + static_cast<int>(q / nrim);
+        if (!rim_node(c.nx, c.ny, q % nrim, &ic, &jc)) continue;
+      }
+      restrict_rim_node<SF, SC>(f, rf, c, bc, ic, jc, kc, s);
+    }
+    return;
+  }
+  b += nmarch;
+  const int tx = b % ntx, rest = b / ntx, ty = rest % nty, cz = rest / nty;
+  const int lane = static_cast<int>(threadIdx.x) & (kRMLanes - 1);
+  const int wv = static_cast<int>(threadIdx.x) / kRMLanes;
+  const int ic = 1 + tx * kRMCols + lane;
+  const int icl = min(ic, c.nx - 2);          // lanes past the box load inside the fine row
+  const int jc0 = 2 + ty * (R * kRMWaves) + wv * R;
+  const int kc0 = 1 + cz * kchunk;
+  const int kc1 = min(kc0 + kchunk, c.nz - 1);
+  const bool col_out = lane >= 1 && ic <= c.nx - 3;
+  long long roff[FR];
+#pragma unroll
+  for (int m = 0; m < FR; ++m) roff[m] = (long long)min(2 * jc0 - 1 + m, f.ny - 1) * f.nx + 2 * icl;
+  // a fine plane's values under the wave (fine columns 2ic and 2ic+1 of each fine row), and
+  // their fw_in at the wave's R coarse rows
+  auto load_plane = [&](int fk, SF* w0, SF* w1) {
+    const SF* __restrict__ pl = rf + fk * fsxy;
+#pragma unroll
+    for (int m = 0; m < FR; ++m) { w0[m] = pl[roff[m]]; w1[m] = pl[roff[m] + 1]; }
+  };
+  auto weigh = [&](const SF* w0, const SF* w1, double* q) {
+    double a[FR];
+#pragma unroll
+    for (int m = 0; m < FR; ++m) {
+      const double v0 = to_double(w0[m]), v1 = to_double(w1[m]);
+      a[m] = fw_row3(dpp_west_d(v1), v0, v1);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) q[r] = fw_col3(a[2 * r], a[2 * r + 1], a[2 * r + 2]);
+  };
+  auto plane_q = [&](int fk, double* q) {
+    SF w0[FR], w1[FR];
+    load_plane(fk, w0, w1);
+    weigh(w0, w1, q);
+  };
+  const double th_lo = c.theta[4], th_hi = c.theta[5];
+  double ql[R], qm[R], qh[R], qx[R];
+  // the next step's two fine planes load while the current step weighs its own
+  SF cm0[FR], cm1[FR], ch0[FR], ch1[FR], nm0[FR], nm1[FR], nh0[FR], nh1[FR];
+  load_plane(2 * kc0, cm0, cm1);
+  load_plane(2 * kc0 + 1, ch0, ch1);
+  plane_q(2 * kc0 - 1, ql);
+  for (int kc = kc0; kc < kc1; ++kc) {
+    const bool more = kc + 1 < kc1;   // workgroup-uniform
+    if (more) {
+      load_plane(2 * kc + 2, nm0, nm1);
+      load_plane(2 * kc + 3, nh0, nh1);
+    }
+    weigh(cm0, cm1, qm);
+    weigh(ch0, ch1, qh);
+    double v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = fma(0.25, qh[r], fma(0.5, qm[r], 0.25 * ql[r]));
+    if (kc == 1 && th_lo != 0.0) {   // z folds (workgroup-uniform branches)
+      plane_q(0, qx);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = fma(th_lo, fma(0.25, ql[r], 0.5 * qx[r]), v[r]);
+    }
+    if (kc == c.nz - 2 && th_hi != 0.0) {
+      plane_q(2 * kc + 2, qx);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = fma(th_hi, fma(0.5, qx[r], 0.25 * qh[r]), v[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int jc = jc0 + r;
+      if (col_out && jc <= c.ny - 3) bc[((long long)kc * c.ny + jc) * c.nx + ic] = store_as<SC>(v[r] * s);
+      ql[r] = qh[r];
+    }
+    if (more) {
+#pragma unroll
+      for (int m = 0; m < FR; ++m) { cm0[m] = nm0[m]; cm1[m] = nm1[m]; ch0[m] = nh0[m]; ch1[m] = nh1[m]; }
+    }
   }
 }
 
@@ -619,9 +833,50 @@ template void closures_dirichlet<float>(const Launch&, const DevOp&, const FaceD
     default: hipLaunchKernelGGL(KER<P16>, __VA_ARGS__); break;                        \
   }
 
+namespace {
+bool g_restrict_march = true;
+
+// The plane-march restriction: chunks of at least 4 coarse planes, about two march workgroups
+// per CU; the per-node workgroups after them, at most 1024.
+template <Prec PF, Prec PC>
+void launch_restrict3m(const Launch& L, const DevOp& fine, const void* rf, const DevOp& coarse,
+                       void* bc, double scale, const double* dscale) {
+  const int bx = coarse.nx - 4, by = coarse.ny - 4;
+  const int ntx = (bx + kRMCols - 1) / kRMCols;
+  const int nty = (by + kRMRows * kRMWaves - 1) / (kRMRows * kRMWaves);
+  const int nzi = coarse.nz - 2;
+  const long long txy = (long long)ntx * nty;
+  const long long want = std::max(1LL, (2LL * device_cus() + txy - 1) / txy);
+  const int kchunk = std::max(4, static_cast<int>((nzi + want - 1) / want));
+  const int chunks = (nzi + kchunk - 1) / kchunk;
+  const int nmarch = static_cast<int>(txy * chunks);
+  const long long plane = (long long)coarse.nx * coarse.ny;
+  const long long nrim_items = 2 * plane + (long long)nzi * (4LL * coarse.nx + 4LL * by);
+  const int nrb = static_cast<int>(std::min<long long>((nrim_items + 255) / 256, 1024));
+  hipLaunchKernelGGL((k_restrict3m<PF, PC>), dim3(nmarch + nrb), dim3(kRMLanes * kRMWaves), 0,
+                     L.stream, fine, rf, coarse, bc, scale, dscale, L.sc, ntx, nty, kchunk, nmarch,
+                     nrim_items);
+}
+}  // namespace
+
+void set_restrict_march(bool on) { g_restrict_march = on; }
+bool restrict_march() { return g_restrict_march; }
+
 void restrict_fw(const Launch& L, const DevOp& fine, Prec pf, const void* rf, const DevOp& coarse,
                  Prec pc, void* bc, double scale, const double* dev_scale) {
   const int gx = (coarse.nx + kTX - 1) / kTX, gy = (coarse.ny + kTY - 1) / kTY;
+  if (g_restrict_march && coarse.nz >= 3 && coarse.nx >= 5 && coarse.ny >= 5 &&
+      2 * coarse.nx - 1 == fine.nx && 2 * coarse.ny - 1 == fine.ny && 2 * coarse.nz - 1 == fine.nz) {
+#define RM_(A, B) launch_restrict3m<A, B>(L, fine, rf, coarse, bc, scale, dev_scale)
+    if (pf == P64 && pc == P64) RM_(P64, P64);
+    else if (pf == P32 && pc == P32) RM_(P32, P32);
+    else if (pf == P32 && pc == P16) RM_(P32, P16);
+    else if (pf == P16 && pc == P16) RM_(P16, P16);
+    else if (pf == P64 && pc == P32) RM_(P64, P32);
+    else RM_(P16, P32);
+#undef RM_
+    return;
+  }
   if (coarse.nz > 1) {
     // interior box blocks, then rim blocks (rim_node numbering) stacked above them in y
     const int bx = coarse.nx - 4, by = coarse.ny - 4;

@@ -110,6 +110,10 @@ void set_plane_stencils(bool on);
 bool plane_stencils();
 // 3D FP32/FP16 plane march: the ring-buffered variant (k_plane3r) or k_plane3.
 void set_plane_ring(bool on);
+// 3D restrictions of standard coarsenings as a plane march (1, default) or one coarse node per
+// thread (0); the same bits either way.
+void set_restrict_march(bool on);
+bool restrict_march();
 bool plane_ring();
 
 // Reference-semantics relaxation (FP64): interior update (-(f + N(u)) - sum w u_nb)/C.

@@ -91,13 +91,18 @@ __device__ __forceinline__ double fw_row3(double w, double c, double e) {
   return fma(0.25, e, a);
 }
 
+// 0.25 a0 + 0.5 a1 + 0.25 a2 of three weighted fine rows (south to north)
+__device__ __forceinline__ double fw_col3(double a0, double a1, double a2) {
+  return fma(0.25, a2, fma(0.5, a1, 0.25 * a0));
+}
+
 // 3 x 3 in-plane weighting at (fi, fj) of fine plane fk, all nine nodes inside the grid
 template <typename R>
 __device__ __forceinline__ double fw_in(const R& r, int fi, int fj, int fk) {
   const double a0 = fw_row3(r(fi - 1, fj - 1, fk), r(fi, fj - 1, fk), r(fi + 1, fj - 1, fk));
   const double a1 = fw_row3(r(fi - 1, fj, fk), r(fi, fj, fk), r(fi + 1, fj, fk));
   const double a2 = fw_row3(r(fi - 1, fj + 1, fk), r(fi, fj + 1, fk), r(fi + 1, fj + 1, fk));
-  return fma(0.25, a2, fma(0.5, a1, 0.25 * a0));
+  return fw_col3(a0, a1, a2);
 }
 
 // the same, clipped at the plane's edges

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 DEFAULTS = {"smooth.fused": 1, "smooth.rows": 2, "smooth.zchunk": 0, "smooth.ychunk": 64,
             "smooth.pair": 0, "smooth.ring": 1, "smooth.reg": 1, "stencil.ring": 1, "smooth.box": 65,
             "smooth.box2": 65, "smooth.box_z": 8, "smooth.box_rr": 65, "mixed.fp16": 1,
-            "mixed.replace": 1, "mixed.xpay": 1}
+            "mixed.replace": 1, "mixed.xpay": 1, "restrict.march": 1}
 
 
 @pytest.fixture(scope="module")
@@ -95,6 +95,27 @@ def test_fused_sweep_bitwise_3d(ctx, kind, shape, tune, precision):
     s = (flux_problem if kind == "flux" else const_problem)(shape, seed=sum(shape))
     u0, r0 = solve(ctx, s, precision, 0, **tune)
     u1, r1 = solve(ctx, s, precision, 1, **tune)
+    print(f"  iterations {r0.iterations}/{r1.iterations}  max diff {np.abs(u0 - u1).max():.3e}", flush=True)
+    assert r0.iterations == r1.iterations
+    assert np.array_equal(u0, u1)
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+@pytest.mark.parametrize("kind,shape", [
+    ("flux", (33, 65, 129)),    # several march tiles in x and y, z chunks
+    ("flux", (65, 33, 17)),     # coarse levels whose box is one or two columns wide
+    ("const", (17, 129, 65)),
+    ("flux", (129, 65, 65)),    # long z: many chunks
+    ("flux", (69, 61, 37)),     # sides 4m+1, odd coarse levels
+])
+def test_restrict_march_bitwise(ctx, kind, shape, precision):
+    """The plane-march restriction (k_restrict3m) against the per-node one (k_restrict3): the
+    same coarse right-hand sides, so the same multigrid solve bit for bit, FP64 and mixed, with
+    the FP32/FP16 levels on plane marches (box residual-restriction off)."""
+    print(f"restrict march {kind} {shape} precision={precision}", flush=True)
+    s = (flux_problem if kind == "flux" else const_problem)(shape, seed=sum(shape) + 7)
+    u0, r0 = solve(ctx, s, precision, 1, **{"restrict.march": 0, "box_rr": 0})
+    u1, r1 = solve(ctx, s, precision, 1, **{"restrict.march": 1, "box_rr": 0})
     print(f"  iterations {r0.iterations}/{r1.iterations}  max diff {np.abs(u0 - u1).max():.3e}", flush=True)
     assert r0.iterations == r1.iterations
     assert np.array_equal(u0, u1)
