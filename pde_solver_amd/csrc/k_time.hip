@@ -441,14 +441,31 @@ __global__ void __launch_bounds__(1024) k_fin_scale(const double* partials, int 
 
 // x64 += s x32, x32 = 0, when a replacement is due (gate clear).
 __global__ void k_flush_f32(double* __restrict__ x, float* __restrict__ e, long long n,
-                            const double* scale, const KScalars* gate) {
+                            const double* scale, const KScalars* gate, int skip_fresh) {
   if (gate->done | gate->breakdown) return;
+  if (skip_fresh && gate->fresh) return;
   const double s = *scale;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     x[i] = fma(s, static_cast<double>(e[i]), x[i]);   // as the fused replacement forms it
     e[i] = 0.0f;
   }
+}
+
+// dst = src when the solve's current solution is the second buffer (so->xsel)
+__global__ void k_copy_if_xsel(double* __restrict__ dst, const double* __restrict__ src,
+                               long long n, const KScalars* so) {
+  if (!so->xsel) return;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// The final true residual is skipped when the solve converged at a replacement, which left
+// r = b - A x of the final x and its partials.
+__global__ void k_fresh_gate(const KScalars* so, const KScalars* si, KScalars* gate) {
+  gate->done = so->done && !so->breakdown && !si->breakdown;
+  gate->breakdown = 0;
 }
 
 // Residual-replacement bookkeeping of the mixed CG (one block): the inner recurrence runs on
@@ -551,9 +568,17 @@ void to_f32_scaled_gated(hipStream_t s, const double* in, float* out, long long 
 }
 
 void flush_f32(hipStream_t s, double* x64, float* x32, long long n, const double* scale,
-               const KScalars* gate) {
+               const KScalars* gate, bool skip_fresh) {
   hipLaunchKernelGGL(k_flush_f32, dim3(misc_blocks(n)), dim3(kBlock), 0, s, x64, x32, n, scale,
-                     gate);
+                     gate, skip_fresh ? 1 : 0);
+}
+
+void copy_if_xsel(hipStream_t s, double* dst, const double* src, long long n, const KScalars* so) {
+  hipLaunchKernelGGL(k_copy_if_xsel, dim3(misc_blocks(n)), dim3(kBlock), 0, s, dst, src, n, so);
+}
+
+void fresh_gate(hipStream_t s, const KScalars* so, const KScalars* si, KScalars* gate) {
+  hipLaunchKernelGGL(k_fresh_gate, dim3(1), dim3(1), 0, s, so, si, gate);
 }
 
 void finalize_ru(hipStream_t s, KScalars* si, KScalars* so, KScalars* gate,

@@ -39,9 +39,15 @@ void to_f32_scaled(hipStream_t s, const double* in, float* out, long long n, con
 void to_f32_scaled_gated(hipStream_t s, const double* in, float* out, long long n,
                          const double* norm, const KScalars* gate);
 // x64 += (*scale) x32, x32 = 0 (folds the FP32 correction accumulated since the last
-// replacement into the FP64 solution).
+// replacement into the FP64 solution).  skip_fresh: also skipped when gate->fresh is set (the
+// correction was voided by a fused replacement).
 void flush_f32(hipStream_t s, double* x64, float* x32, long long n, const double* scale,
-               const KScalars* gate);
+               const KScalars* gate, bool skip_fresh = false);
+// dst = src (n doubles) when so->xsel is set, else nothing.
+void copy_if_xsel(hipStream_t s, double* dst, const double* src, long long n, const KScalars* so);
+// gate->done = (so converged at a replacement: so->done, no breakdown in so or si),
+// gate->breakdown = 0: kernels launched with the gate skip exactly then.
+void fresh_gate(hipStream_t s, const KScalars* so, const KScalars* si, KScalars* gate);
 // FIN_RU_START / FIN_RU_CHECK (tol = the replacement threshold delta) / FIN_RU_APPLY on the
 // inner (si) and outer (so) scalars; gate->done = !(replacement due).  With maxp, the same
 // launch then does fin_scale(maxp, nmax, si).

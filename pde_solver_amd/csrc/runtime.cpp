@@ -1062,27 +1062,22 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
                        [](const KScalars* k) { return k[1].metric * k[0].snorm / k[0].target; }))
       break;
   }
-  // an unconverged solve keeps the correction accumulated since the last replacement
-  if (!fused) flush_f32(s, x64, x32, n, &so->snorm, so);
-  const KScalars ki = pull(P, 1);
-  const KScalars& ko = pull(P, 0);
-  if (fused) {
+  // Every decision after the loop is taken on the device, so the post-solve work queues behind
+  // the last chunk without a host round trip; the report reads the scalars when it syncs.
+  // An unconverged solve keeps the correction accumulated since the last replacement.
+  if (!fused) {
+    flush_f32(s, x64, x32, n, &so->snorm, so);
+  } else {
     // the current solution buffer back into P.x, then (unconverged, correction not void) the
     // correction accumulated since the last replacement
-    if (ko.xsel)
-      HIPCHK(hipMemcpyAsync(x64, xs[1], n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    if (!(ko.done | ko.breakdown) && !ko.fresh) flush_f32(s, x64, x32, n, &so->snorm, so);
+    copy_if_xsel(s, x64, xs[1], n, so);
+    flush_f32(s, x64, x32, n, &so->snorm, so, true);
   }
   // converged at a replacement: r64 and its partials are b - A x of the final x already
-  P.resid_fresh = ko.done && !ko.breakdown && !ki.breakdown;
-  rep->iterations = ki.iter;
-  rep->refinements = ko.steps;
-  rep->inner_iterations = ki.iter;
-  rep->final_metric = ko.metric;
-  rep->converged = ko.done && !ko.breakdown;
-  if (cancelled(P) && !ko.done) throw Error(PDEHIP_ERR_CANCELLED, "solve cancelled");
-  if (ko.breakdown || ki.breakdown) rep->status = PDEHIP_ERR_BREAKDOWN;
-  else if (!ko.done) rep->status = PDEHIP_ERR_NOT_CONVERGED;
+  fresh_gate(s, so, si, gate);
+  P.resid_gate = gate;
+  P.dev_report = true;
+  (void)rep;
 }
 
 // Iterated V-cycles (extension) until ||b - A u||_2 meets the target.
@@ -1621,6 +1616,8 @@ void plan_solve(Plan& P, pdehip_report* rep) {
   init_scalars_n(P, 2, eps2);
   HIPCHK(hipEventRecord(P.ev0, s));
   P.resid_fresh = false;
+  P.resid_gate = nullptr;
+  P.dev_report = false;
   switch (P.opt.method) {
     case PDEHIP_JACOBI:
     case PDEHIP_GAUSS_SEIDEL:
@@ -1658,7 +1655,10 @@ void plan_finish_enqueue(Plan& P) {
   double* part = P.part.as<double>();
   const int mth = P.opt.method;
   P.fin_resid = P.r && mth != PDEHIP_JACOBI && mth != PDEHIP_GAUSS_SEIDEL && mth != PDEHIP_SOR;
-  if (P.fin_resid && !P.resid_fresh) residual(L0, P.op(), P64, P.bvec.get(), P.x.get(), P.r.get(), part);
+  if (P.fin_resid && P.resid_gate)
+    residual(Launch{s, P.resid_gate}, P.op(), P64, P.bvec.get(), P.x.get(), P.r.get(), part);
+  else if (P.fin_resid && !P.resid_fresh)
+    residual(L0, P.op(), P64, P.bvec.get(), P.x.get(), P.r.get(), part);
   if (P.folded) {
     closures_nr(L0, P.op(), P.faces.fd, P.x.as<double>());
     closures_dirichlet(L0, P.op(), P.faces.fd, P.x.as<double>());
@@ -1676,7 +1676,8 @@ void plan_finish_enqueue(Plan& P) {
   if (P.fin_resid) HIPCHK(hipMemcpyAsync(h, part, np * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(h + np, sums, nb * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(h + np + nb, maxs, nb * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(&P.hsc[0], P.sc(0), sizeof(KScalars), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&P.hsc[0], P.sc(0), (P.dev_report ? 2 : 1) * sizeof(KScalars),
+                        hipMemcpyDeviceToHost, s));
 }
 
 // Report fields from the staged copies (after the stream has been synchronised).
@@ -1701,6 +1702,18 @@ void plan_finish_collect(Plan& P, pdehip_report* rep) {
   }
   rep->residual_l2 = std::sqrt(a);
   rep->residual_linf = m;
+  if (P.dev_report) {
+    const KScalars& ko = P.hsc[0];
+    const KScalars& ki = P.hsc[1];
+    rep->iterations = ki.iter;
+    rep->refinements = ko.steps;
+    rep->inner_iterations = ki.iter;
+    rep->final_metric = ko.metric;
+    rep->converged = ko.done && !ko.breakdown;
+    if (cancelled(P) && !ko.done) throw Error(PDEHIP_ERR_CANCELLED, "solve cancelled");
+    if (ko.breakdown || ki.breakdown) rep->status = PDEHIP_ERR_BREAKDOWN;
+    else if (!ko.done) rep->status = PDEHIP_ERR_NOT_CONVERGED;
+  }
 }
 
 void plan_download(Plan& P, double* u_out) {

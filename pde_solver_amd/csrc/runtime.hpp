@@ -173,6 +173,11 @@ struct Plan {
   double* hstage = nullptr;    // pinned staging of the report's reductions
   bool fin_resid = false;      // plan_finish_enqueue staged the true-residual partials
   bool resid_fresh = false;    // the solver left r = b - A x of the final x and its partials
+  // set by solvers that leave these decisions on the device: the final true residual runs
+  // gated by resid_gate (skipped when the solver left it), and the report's counts and status
+  // come from the scalar blocks staged with the report (outer block 0, inner block 1)
+  KScalars* resid_gate = nullptr;
+  bool dev_report = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evq[2] = {nullptr, nullptr};   // trailing convergence checks
 
