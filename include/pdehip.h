@@ -253,9 +253,9 @@ void pdehip_plan_destroy(pdehip_plan* plan);
  * default 1: the same as "smooth.ring" for the 3D FP32/FP16
  * apply and residual), "smooth.box" (0..1025, default 65: coarse 3D levels of mixed-precision
  * V-cycles whose sides are all at most this smooth on LDS boxes, the prolongation folded into
- * the first post-smoothing launch; 0: off), "smooth.box2" (0..1025, default 65: of those, the
+ * the first post-smoothing launch; 0: off), "smooth.box2" (0..1025, default 33: of those, the
  * levels whose sides are all at most this take two sweeps per launch, the others one),
- * "smooth.box_z" (4 or 8, default 8: interior planes per box), "smooth.box_rr" (0..4097,
+ * "smooth.box_z" (4 or 8, default 4: interior planes per box), "smooth.box_rr" (0..4097,
  * default 65: mixed 3D levels whose sides are all at most this form their residual and its
  * restriction in one launch; 0: off).  Results do not depend on any of these.
  * "time.fused" (0/1, default 1): RK4 time series take one launch per step (the four stages and

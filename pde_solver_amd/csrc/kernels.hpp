@@ -61,8 +61,10 @@ struct SmoothTuning {
   int dot = 1;       // the last level-0 sweep of a mixed V-cycle also leaves the r.z partials
   int box = 65;      // 3D FP32/FP16 coarse levels of mixed V-cycles with every side <= box
                      // smooth on boxes (k_box.hip); 0: plane-march sweeps on every level
-  int box2 = 65;     // ... two sweeps per box launch where every side is <= box2, else one
-  int box_z = 8;     // interior planes of a box (4 or 8)
+  int box2 = 33;     // ... two sweeps per box launch where every side is <= box2, else one
+  int box_z = 4;     // interior planes of a box (4 or 8); 4 with box2 = 33 measured 1.2%
+                     // faster than 8 with 65 on a mixed 257^3 solve (more, smaller boxes on
+                     // the latency-bound levels)
   int box_rr = 65;   // 3D FP32/FP16 levels of mixed V-cycles with every side <= box_rr form
                      // the residual and its restriction in one launch (k_box.hip); 0: off
 };

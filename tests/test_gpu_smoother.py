@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"smooth.fused": 1, "smooth.rows": 2, "smooth.zchunk": 0, "smooth.ychunk": 64,
             "smooth.pair": 0, "smooth.ring": 1, "smooth.reg": 1, "stencil.ring": 1, "smooth.box": 65,
-            "smooth.box2": 65, "smooth.box_z": 8, "smooth.box_rr": 65, "mixed.fp16": 1,
+            "smooth.box2": 33, "smooth.box_z": 4, "smooth.box_rr": 65, "mixed.fp16": 1,
             "mixed.replace": 1, "mixed.xpay": 1, "restrict.march": 1}
 
 
