@@ -257,7 +257,10 @@ void pdehip_plan_destroy(pdehip_plan* plan);
  * levels whose sides are all at most this take two sweeps per launch, the others one),
  * "smooth.box_z" (4 or 8, default 4: interior planes per box), "smooth.box_rr" (0..4097,
  * default 65: mixed 3D levels whose sides are all at most this form their residual and its
- * restriction in one launch; 0: off).  Results do not depend on any of these.
+ * restriction in one launch; 0: off), "smooth.box_small" (0..1025, default 33: box levels whose
+ * sides are all at most this use small boxes, 8 x "smooth.box_sxy" x 4 for the sweeps (with
+ * 4-plane boxes) and 4 x 4 x 4 coarse nodes for the residual-restriction; 0: off),
+ * "smooth.box_sxy" (4 or 8, default 4).  Results do not depend on any of these.
  * "time.fused" (0/1, default 1): RK4 time series take one launch per step (the four stages and
  * the closures, temporally blocked in LDS) instead of one per stage; the same bits either way.
  * "mixed.fp16" (0/1, default 1): mixed-precision plans created afterwards store their coarse

@@ -271,6 +271,8 @@ int pdehip_tune_set(const char* key, int value) {
   else if (k == "smooth.box2" && value >= 0 && value <= 1025) t.box2 = value;
   else if (k == "smooth.box_z" && (value == 4 || value == 8)) t.box_z = value;
   else if (k == "smooth.box_rr" && value >= 0 && value <= 4097) t.box_rr = value;
+  else if (k == "smooth.box_small" && value >= 0 && value <= 1025) t.box_small = value;
+  else if (k == "smooth.box_sxy" && (value == 4 || value == 8)) t.box_sxy = value;
   else if (k == "stencil.plane" && (value == 0 || value == 1)) { set_plane_stencils(value == 1); return PDEHIP_OK; }
   else if (k == "stencil.ring" && (value == 0 || value == 1)) { set_plane_ring(value == 1); return PDEHIP_OK; }
   else if (k == "mixed.fp16" && (value == 0 || value == 1)) { set_mixed_fp16(value == 1); return PDEHIP_OK; }
@@ -299,6 +301,8 @@ int pdehip_tune_get(const char* key, int* value) {
   else if (k == "smooth.box2") *value = t.box2;
   else if (k == "smooth.box_z") *value = t.box_z;
   else if (k == "smooth.box_rr") *value = t.box_rr;
+  else if (k == "smooth.box_small") *value = t.box_small;
+  else if (k == "smooth.box_sxy") *value = t.box_sxy;
   else if (k == "stencil.plane") *value = plane_stencils() ? 1 : 0;
   else if (k == "stencil.ring") *value = plane_ring() ? 1 : 0;
   else if (k == "mixed.fp16") *value = mixed_fp16() ? 1 : 0;

@@ -5,8 +5,9 @@
 // plane at a time with one barrier per plane, so a launch costs its prologue and a few plane
 // steps of latency whatever the grid's size.  On the coarse levels of a V-cycle (129^3 and
 // below) the level sits in L2 / Infinity Cache and that latency is the whole cost.  Here a
-// workgroup owns a kBX x kBY x BZ box of interior nodes and loads the box plus a ring of
-// H = 2 * sweeps nodes into LDS at once (every load of the box in flight together), then runs
+// workgroup owns a BX x BY x BZ box of interior nodes (16 x 8, or 8 x 8 / 8 x 4 on the small
+// levels, where more workgroups with less serial work each cut the latency) and loads the box
+// plus a ring of H = 2 * sweeps nodes into LDS at once (every load of the box in flight together), then runs
 // the colour stages of its sweeps in LDS with one barrier per stage:
 //   stage s (0 .. 2*sweeps-1) updates colour c1 (s even) or the other colour (s odd) on the box
 //   plus a ring of 2*sweeps-1-s nodes; the neighbours it reads sit one node further out and
@@ -38,7 +39,7 @@ template <> __device__ __forceinline__ float back<float>(float v) { return v; }
 template <> __device__ __forceinline__ float back<__half>(__half v) { return __half2float(v); }
 
 // IN: 0 x_in from memory, 1 x_in is zero, 2 x_in plus factor * prolong(uc) on interior nodes.
-template <int KIND, Prec P, int NS, int BZ, int IN>
+template <int KIND, Prec P, int NS, int BZ, int IN, int BX = kBX, int BY = kBY>
 __global__ void __launch_bounds__(kBoxThreads)
 k_box_rbgs(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v,
            void* __restrict__ xout_v, int c1, int nbx, int nby, int nboxes, DevOp cop,
@@ -49,7 +50,7 @@ k_box_rbgs(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v
   static_assert(sizeof(T) == 4, "box sweep: 4-byte arithmetic types only");
   constexpr bool HK = KIND == PDEHIP_OP_FLUX;
   constexpr int H = 2 * NS;
-  constexpr int EX = kBX + 2 * H, EY = kBY + 2 * H, EZ = BZ + 2 * H;
+  constexpr int EX = BX + 2 * H, EY = BY + 2 * H, EZ = BZ + 2 * H;
   constexpr int EXY = EX * EY, EN = EXY * EZ;
   // coarse nodes under the extended box (prolongation on load)
   constexpr int CX = EX / 2 + 2, CY = EY / 2 + 2, CZ = EZ / 2 + 2, CN = CX * CY * CZ;
@@ -63,7 +64,7 @@ k_box_rbgs(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v
   const int t = xcd_tile();
   if (t >= nboxes) return;
   const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
-  const int ox = 1 + bx * kBX, oy = 1 + by * kBY, oz = 1 + bz * BZ;   // first interior node
+  const int ox = 1 + bx * BX, oy = 1 + by * BY, oz = 1 + bz * BZ;   // first interior node
   const int gx0 = ox - H, gy0 = oy - H, gz0 = oz - H;                  // grid node of LDS (0,0,0)
   const int nx = op.nx, ny = op.ny, nz = op.nz;
   const S* __restrict__ b = static_cast<const S*>(bv);
@@ -162,7 +163,7 @@ k_box_rbgs(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v
   auto stage = [&](auto stc) {
     constexpr int st = decltype(stc)::value;
     constexpr int r = 2 * NS - 1 - st;                 // ring of this stage
-    constexpr int RX = kBX + 2 * r, RY = kBY + 2 * r, RZ = BZ + 2 * r, HX = RX / 2;
+    constexpr int RX = BX + 2 * r, RY = BY + 2 * r, RZ = BZ + 2 * r, HX = RX / 2;
     constexpr int NI = HX * RY * RZ;
     constexpr int o = H - r;                           // LDS offset of the stage's region
     constexpr int NQ = (NI + kBoxThreads - 1) / kBoxThreads;
@@ -216,10 +217,10 @@ k_box_rbgs(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v
 
   // out: the box's interior nodes, plus the boundary layer beside it on the grid faces it touches
   const int xl = ox == 1 ? 0 : ox, yl = oy == 1 ? 0 : oy, zl = oz == 1 ? 0 : oz;
-  const int xh = ox + kBX >= nx - 1 ? nx : ox + kBX;
-  const int yh = oy + kBY >= ny - 1 ? ny : oy + kBY;
+  const int xh = ox + BX >= nx - 1 ? nx : ox + BX;
+  const int yh = oy + BY >= ny - 1 ? ny : oy + BY;
   const int zh = oz + BZ >= nz - 1 ? nz : oz + BZ;
-  constexpr int OX = kBX + 2, OY = kBY + 2, OZ = BZ + 2, ON = OX * OY * OZ;
+  constexpr int OX = BX + 2, OY = BY + 2, OZ = BZ + 2, ON = OX * OY * OZ;
   for (int it = tid; it < ON; it += kBoxThreads) {
     const int gi = ox - 1 + it % OX, gj = oy - 1 + (it / OX) % OY, gk = oz - 1 + it / (OX * OY);
     if (gi < xl || gi >= xh || gj < yl || gj >= yh || gk < zl || gk >= zh) continue;
@@ -229,7 +230,8 @@ k_box_rbgs(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v
 }
 
 // ------------------------------------------------------------------ residual + restriction
-// One workgroup per box of kCX x kCY x kCZ coarse nodes: the fine residual r = b - A x (the
+// One workgroup per box of CX x CY x CZ coarse nodes (8 x 4 x 4; 4 x 4 x 4 on the small
+// levels): the fine residual r = b - A x (the
 // plane kernels' per-node arithmetic: faces by face_coef, apply_row, r rounded to the storage
 // type) on every fine node the box's full weighting reads, the face nodes its folded closure
 // terms read included, kept in LDS; then each coarse node's value (restrict_value,
@@ -238,7 +240,7 @@ k_box_rbgs(DevOp op, const void* __restrict__ bv, const void* __restrict__ xin_v
 constexpr int kCX = 8, kCY = 4, kCZ = 4;
 constexpr int kRRThreads = 256;
 
-template <int KIND, Prec PF, Prec PC>
+template <int KIND, Prec PF, Prec PC, int CX = kCX, int CY = kCY, int CZ = kCZ>
 __global__ void __launch_bounds__(kRRThreads)
 k_box_rr(DevOp f, const void* __restrict__ bv, const void* __restrict__ xv, DevOp c,
          void* __restrict__ bcv, double scale, const double* dscale, int ncx, int ncy,
@@ -248,8 +250,8 @@ k_box_rr(DevOp f, const void* __restrict__ bv, const void* __restrict__ xv, DevO
   using SC = typename PT<PC>::S;
   static_assert(sizeof(T) == 4, "box residual-restriction: 4-byte arithmetic types only");
   constexpr bool HK = KIND == PDEHIP_OP_FLUX;
-  // fine residual region [2*I0 - 2, 2*(I0 + kCX - 1) + 2] per axis, x region one node wider
-  constexpr int RX = 2 * kCX + 3, RY = 2 * kCY + 3, RZ = 2 * kCZ + 3, RN = RX * RY * RZ;
+  // fine residual region [2*I0 - 2, 2*(I0 + CX - 1) + 2] per axis, x region one node wider
+  constexpr int RX = 2 * CX + 3, RY = 2 * CY + 3, RZ = 2 * CZ + 3, RN = RX * RY * RZ;
   constexpr int XX = RX + 2, XY = RY + 2, XZ = RZ + 2, XXY = XX * XY, XN = XXY * XZ;
   __shared__ S xs[XN];
   __shared__ T rs[HK ? XN : 1];
@@ -258,7 +260,7 @@ k_box_rr(DevOp f, const void* __restrict__ bv, const void* __restrict__ xv, DevO
   if (skip(sc)) return;
   const int t = xcd_tile();
   if (t >= nboxes) return;
-  const int I0 = (t % ncx) * kCX, rest = t / ncx, J0 = (rest % ncy) * kCY, K0 = (rest / ncy) * kCZ;
+  const int I0 = (t % ncx) * CX, rest = t / ncx, J0 = (rest % ncy) * CY, K0 = (rest / ncy) * CZ;
   const int rx0 = 2 * I0 - 2, ry0 = 2 * J0 - 2, rz0 = 2 * K0 - 2;   // residual region origin
   const int nx = f.nx, ny = f.ny, nz = f.nz;
   const S* __restrict__ b = static_cast<const S*>(bv);
@@ -348,8 +350,8 @@ k_box_rr(DevOp f, const void* __restrict__ bv, const void* __restrict__ xv, DevO
   auto r = [&](int i, int j, int k) -> double {
     return static_cast<double>(back<S>(res[((k - rz0) * RY + (j - ry0)) * RX + (i - rx0)]));
   };
-  for (int q = tid; q < kCX * kCY * kCZ; q += kRRThreads) {
-    const int ic = I0 + q % kCX, jc = J0 + (q / kCX) % kCY, kc = K0 + q / (kCX * kCY);
+  for (int q = tid; q < CX * CY * CZ; q += kRRThreads) {
+    const int ic = I0 + q % CX, jc = J0 + (q / CX) % CY, kc = K0 + q / (CX * CY);
     if (ic >= c.nx || jc >= c.ny || kc >= c.nz) continue;
     const long long out = ((long long)kc * c.ny + jc) * c.nx + ic;
     if (ic == 0 || jc == 0 || kc == 0 || ic == c.nx - 1 || jc == c.ny - 1 || kc == c.nz - 1) {
@@ -360,57 +362,79 @@ k_box_rr(DevOp f, const void* __restrict__ bv, const void* __restrict__ xv, DevO
   }
 }
 
-template <int KIND, Prec PF, Prec PC>
+template <int KIND, Prec PF, Prec PC, int CX, int CY, int CZ>
 void launch_rr(const Launch& L, const DevOp& f, const void* b, const void* x, const DevOp& c,
                void* bc, double scale, const double* dscale) {
-  const int ncx = (c.nx + kCX - 1) / kCX, ncy = (c.ny + kCY - 1) / kCY, ncz = (c.nz + kCZ - 1) / kCZ;
+  const int ncx = (c.nx + CX - 1) / CX, ncy = (c.ny + CY - 1) / CY, ncz = (c.nz + CZ - 1) / CZ;
   const int nboxes = ncx * ncy * ncz;
   const dim3 g(static_cast<unsigned>(8 * ((nboxes + 7) / 8)));
-  hipLaunchKernelGGL((k_box_rr<KIND, PF, PC>), g, dim3(kRRThreads), 0, L.stream, f, b, x, c, bc,
-                     scale, dscale, ncx, ncy, nboxes, L.sc);
+  hipLaunchKernelGGL((k_box_rr<KIND, PF, PC, CX, CY, CZ>), g, dim3(kRRThreads), 0, L.stream, f, b,
+                     x, c, bc, scale, dscale, ncx, ncy, nboxes, L.sc);
+}
+
+// small: the level is within the small-box bound (coarse boxes of 4 x 4 x 4 nodes)
+template <int KIND, Prec PF, Prec PC>
+void rr_by_size(const Launch& L, const DevOp& f, const void* b, const void* x, const DevOp& c,
+                void* bc, double scale, const double* dscale, bool small) {
+  if (small) launch_rr<KIND, PF, PC, 4, 4, 4>(L, f, b, x, c, bc, scale, dscale);
+  else launch_rr<KIND, PF, PC, kCX, kCY, kCZ>(L, f, b, x, c, bc, scale, dscale);
 }
 
 template <int KIND>
 bool rr_by_prec(const Launch& L, const DevOp& f, Prec pf, const void* b, const void* x,
-                const DevOp& c, Prec pc, void* bc, double scale, const double* dscale) {
-  if (pf == P32 && pc == P32) launch_rr<KIND, P32, P32>(L, f, b, x, c, bc, scale, dscale);
-  else if (pf == P32 && pc == P16) launch_rr<KIND, P32, P16>(L, f, b, x, c, bc, scale, dscale);
-  else if (pf == P16 && pc == P16) launch_rr<KIND, P16, P16>(L, f, b, x, c, bc, scale, dscale);
+                const DevOp& c, Prec pc, void* bc, double scale, const double* dscale, bool small) {
+  if (pf == P32 && pc == P32) rr_by_size<KIND, P32, P32>(L, f, b, x, c, bc, scale, dscale, small);
+  else if (pf == P32 && pc == P16) rr_by_size<KIND, P32, P16>(L, f, b, x, c, bc, scale, dscale, small);
+  else if (pf == P16 && pc == P16) rr_by_size<KIND, P16, P16>(L, f, b, x, c, bc, scale, dscale, small);
   else return false;
   return true;
 }
 
-template <int KIND, Prec P, int NS, int BZ, int IN>
+template <int KIND, Prec P, int NS, int BZ, int IN, int BX, int BY>
 void launch(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout, int c1,
             const DevOp& cop, const void* uc, double factor, const double* dscale) {
-  const int nbx = (op.nx - 2 + kBX - 1) / kBX;
-  const int nby = (op.ny - 2 + kBY - 1) / kBY;
+  const int nbx = (op.nx - 2 + BX - 1) / BX;
+  const int nby = (op.ny - 2 + BY - 1) / BY;
   const int nbz = (op.nz - 2 + BZ - 1) / BZ;
   const int nboxes = nbx * nby * nbz;
   const dim3 g(static_cast<unsigned>(8 * ((nboxes + 7) / 8)));
-  hipLaunchKernelGGL((k_box_rbgs<KIND, P, NS, BZ, IN>), g, dim3(kBoxThreads), 0, L.stream, op, b,
-                     xin, xout, c1, nbx, nby, nboxes, cop, uc, factor, dscale, L.sc);
+  hipLaunchKernelGGL((k_box_rbgs<KIND, P, NS, BZ, IN, BX, BY>), g, dim3(kBoxThreads), 0, L.stream,
+                     op, b, xin, xout, c1, nbx, nby, nboxes, cop, uc, factor, dscale, L.sc);
 }
 
-template <int KIND, Prec P, int NS, int BZ>
+template <int KIND, Prec P, int NS, int BZ, int BX = kBX, int BY = kBY>
 void by_input(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout, int c1,
               int in_mode, const DevOp& cop, const void* uc, double factor, const double* dscale) {
-  if (in_mode == 1) launch<KIND, P, NS, BZ, 1>(L, op, b, xin, xout, c1, cop, uc, factor, dscale);
-  else if (in_mode == 2) launch<KIND, P, NS, BZ, 2>(L, op, b, xin, xout, c1, cop, uc, factor, dscale);
-  else launch<KIND, P, NS, BZ, 0>(L, op, b, xin, xout, c1, cop, uc, factor, dscale);
+  if (in_mode == 1) launch<KIND, P, NS, BZ, 1, BX, BY>(L, op, b, xin, xout, c1, cop, uc, factor, dscale);
+  else if (in_mode == 2) launch<KIND, P, NS, BZ, 2, BX, BY>(L, op, b, xin, xout, c1, cop, uc, factor, dscale);
+  else launch<KIND, P, NS, BZ, 0, BX, BY>(L, op, b, xin, xout, c1, cop, uc, factor, dscale);
 }
 
+// small: 0, or the x extent of the small boxes used on this level (8: 8 x 8 x 4, 4: 8 x 4 x 4;
+// 4-plane boxes only)
 template <int KIND, Prec P>
 void by_shape(const Launch& L, const DevOp& op, const void* b, const void* xin, void* xout, int c1,
               int sweeps, int bz, int in_mode, const DevOp& cop, const void* uc, double factor,
-              const double* dscale) {
-  if (sweeps == 2) {
+              const double* dscale, int small) {
+  if (bz == 4 && small == 8) {
+    if (sweeps == 2) by_input<KIND, P, 2, 4, 8, 8>(L, op, b, xin, xout, c1, in_mode, cop, uc, factor, dscale);
+    else by_input<KIND, P, 1, 4, 8, 8>(L, op, b, xin, xout, c1, in_mode, cop, uc, factor, dscale);
+  } else if (bz == 4 && small == 4) {
+    if (sweeps == 2) by_input<KIND, P, 2, 4, 8, 4>(L, op, b, xin, xout, c1, in_mode, cop, uc, factor, dscale);
+    else by_input<KIND, P, 1, 4, 8, 4>(L, op, b, xin, xout, c1, in_mode, cop, uc, factor, dscale);
+  } else if (sweeps == 2) {
     if (bz == 4) by_input<KIND, P, 2, 4>(L, op, b, xin, xout, c1, in_mode, cop, uc, factor, dscale);
     else by_input<KIND, P, 2, 8>(L, op, b, xin, xout, c1, in_mode, cop, uc, factor, dscale);
   } else {
     if (bz == 4) by_input<KIND, P, 1, 4>(L, op, b, xin, xout, c1, in_mode, cop, uc, factor, dscale);
     else by_input<KIND, P, 1, 8>(L, op, b, xin, xout, c1, in_mode, cop, uc, factor, dscale);
   }
+}
+
+// the level's sides are all within the small-box bound
+bool small_level(const DevOp& op) {
+  const int lim = smooth_tuning().box_small;
+  return lim > 0 && op.nx <= lim && op.ny <= lim && op.nz <= lim;
 }
 
 }  // namespace
@@ -420,10 +444,11 @@ bool box_resrestrict(const Launch& L, const DevOp& fine, Prec pf, const void* b,
   if (fine.nz < 3 || coarse.nz < 3 || fine.n >= (1LL << 31)) return false;
   if (2 * coarse.nx - 1 != fine.nx || 2 * coarse.ny - 1 != fine.ny || 2 * coarse.nz - 1 != fine.nz)
     return false;
+  const bool small = small_level(fine);
   if (fine.kind == PDEHIP_OP_CONST)
-    return rr_by_prec<PDEHIP_OP_CONST>(L, fine, pf, b, x, coarse, pc, bc, scale, dev_scale);
+    return rr_by_prec<PDEHIP_OP_CONST>(L, fine, pf, b, x, coarse, pc, bc, scale, dev_scale, small);
   if (fine.kind == PDEHIP_OP_FLUX)
-    return rr_by_prec<PDEHIP_OP_FLUX>(L, fine, pf, b, x, coarse, pc, bc, scale, dev_scale);
+    return rr_by_prec<PDEHIP_OP_FLUX>(L, fine, pf, b, x, coarse, pc, bc, scale, dev_scale, small);
   return false;
 }
 
@@ -437,12 +462,13 @@ bool box_sweeps(const Launch& L, const DevOp& op, Prec p, const void* b, const v
   if (in_mode != 1 && x_in == x_out) return false;   // other boxes read x_in's ring
   const DevOp cop = coarse ? *coarse : op;
   const int bz = smooth_tuning().box_z == 4 ? 4 : 8;
+  const int small = small_level(op) ? smooth_tuning().box_sxy : 0;
   if (op.kind == PDEHIP_OP_CONST) {
-    if (p == P32) by_shape<PDEHIP_OP_CONST, P32>(L, op, b, x_in, x_out, first_color, sweeps, bz, in_mode, cop, uc, factor, dev_scale);
-    else by_shape<PDEHIP_OP_CONST, P16>(L, op, b, x_in, x_out, first_color, sweeps, bz, in_mode, cop, uc, factor, dev_scale);
+    if (p == P32) by_shape<PDEHIP_OP_CONST, P32>(L, op, b, x_in, x_out, first_color, sweeps, bz, in_mode, cop, uc, factor, dev_scale, small);
+    else by_shape<PDEHIP_OP_CONST, P16>(L, op, b, x_in, x_out, first_color, sweeps, bz, in_mode, cop, uc, factor, dev_scale, small);
   } else {
-    if (p == P32) by_shape<PDEHIP_OP_FLUX, P32>(L, op, b, x_in, x_out, first_color, sweeps, bz, in_mode, cop, uc, factor, dev_scale);
-    else by_shape<PDEHIP_OP_FLUX, P16>(L, op, b, x_in, x_out, first_color, sweeps, bz, in_mode, cop, uc, factor, dev_scale);
+    if (p == P32) by_shape<PDEHIP_OP_FLUX, P32>(L, op, b, x_in, x_out, first_color, sweeps, bz, in_mode, cop, uc, factor, dev_scale, small);
+    else by_shape<PDEHIP_OP_FLUX, P16>(L, op, b, x_in, x_out, first_color, sweeps, bz, in_mode, cop, uc, factor, dev_scale, small);
   }
   return true;
 }

@@ -67,6 +67,11 @@ struct SmoothTuning {
                      // the latency-bound levels)
   int box_rr = 65;   // 3D FP32/FP16 levels of mixed V-cycles with every side <= box_rr form
                      // the residual and its restriction in one launch (k_box.hip); 0: off
+  int box_small = 33;  // box levels with every side <= box_small use small boxes: sweeps on
+                       // 8 x box_sxy x 4 boxes (with box_z = 4), residual-restriction on
+                       // 4 x 4 x 4 coarse boxes (more workgroups, less serial work each, on
+                       // the latency-bound levels); 0: off
+  int box_sxy = 4;     // y extent of the small sweep boxes (4 or 8)
 };
 void set_smooth_tuning(const SmoothTuning& t);
 SmoothTuning smooth_tuning();

@@ -77,12 +77,15 @@ __device__ __forceinline__ double prolong_fine(double fa, double s, bool half, d
 
 // Per-axis fold list of coarse coordinate c on an axis of n nodes: the node itself (weight 1)
 // plus the face node it absorbs (weight theta) when it is the first/last interior node.
+// Every slot is written at a fixed index (no per-thread array indexing, so the lists stay in
+// registers): slot 1 holds the low face when c == 1, else the high one; slot 2 the high face
+// (used only when both apply, n == 3).
 __device__ __forceinline__ int fold_list(int c, int n, double th_lo, double th_hi, int* idx, double* w) {
+  const bool lo = c == 1, hi = c == n - 2;
   idx[0] = c; w[0] = 1.0;
-  int m = 1;
-  if (c == 1) { idx[m] = 0; w[m] = th_lo; ++m; }
-  if (c == n - 2) { idx[m] = n - 1; w[m] = th_hi; ++m; }
-  return m;
+  idx[1] = lo ? 0 : n - 1; w[1] = lo ? th_lo : th_hi;
+  idx[2] = n - 1; w[2] = th_hi;
+  return 1 + (lo ? 1 : 0) + (hi ? 1 : 0);
 }
 
 // 0.5 c + 0.25 w + 0.25 e
@@ -145,11 +148,16 @@ __device__ __forceinline__ double restrict_value(const DevOp& f, const DevOp& c,
   }
   auto Q = [&](int k) -> double {
     double acc = 0.0;
-    for (int b = 0; b < my; ++b)
-      for (int q = 0; q < mx; ++q) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      if (b >= my) break;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        if (q >= mx) break;
         const double w = yw[b] * xw[q];
         if (w != 0.0) acc = fma(w, fw_clip(f, r, 2 * xi[q], 2 * yi[b], k), acc);
       }
+    }
     return acc;
   };
   if (fold_xy) { ql = Q(fk - 1); qm = Q(fk); qh = Q(fk + 1); }

@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"smooth.fused": 1, "smooth.rows": 2, "smooth.zchunk": 0, "smooth.ychunk": 64,
             "smooth.pair": 0, "smooth.ring": 1, "smooth.reg": 1, "stencil.ring": 1, "smooth.box": 65,
-            "smooth.box2": 33, "smooth.box_z": 4, "smooth.box_rr": 65, "mixed.fp16": 1,
+            "smooth.box2": 33, "smooth.box_z": 4, "smooth.box_rr": 65, "smooth.box_small": 33,
+            "smooth.box_sxy": 4, "mixed.fp16": 1,
             "mixed.replace": 1, "mixed.xpay": 1, "restrict.march": 1}
 
 
@@ -164,6 +165,33 @@ def test_box_smoothing_bitwise(ctx, kind, shape, bz, fp16, pre, post, box2):
     u1, r1 = solve(ctx, s, capi.MIXED, 1, pre, post,
                    **{"smooth.box": 129, "smooth.box2": box2, "smooth.box_z": bz,
                       "mixed.fp16": fp16})
+    print(f"  iterations {r0.iterations}/{r1.iterations}  max diff {np.abs(u0 - u1).max():.3e}", flush=True)
+    assert r0.iterations == r1.iterations
+    assert np.array_equal(u0, u1)
+
+
+SMALL_BOX_CASES = [
+    # (kind, (nz, ny, nx), FP16 coarse levels, y extent of the small boxes)
+    ("flux", (33, 65, 129), 1, 8),
+    ("flux", (65, 33, 17), 1, 4),
+    ("const", (65, 65, 65), 0, 4),
+    ("flux", (69, 61, 37), 1, 8),
+]
+
+
+@pytest.mark.parametrize("kind,shape,fp16,sxy", SMALL_BOX_CASES)
+def test_small_box_bitwise(ctx, kind, shape, fp16, sxy):
+    """Small boxes (8 x sxy x 4 sweep boxes, 4 x 4 x 4 coarse residual-restriction boxes) on
+    every coarse level against the plane-march sweeps, residual and restriction: the same
+    mixed solve bit for bit."""
+    print(f"small box {kind} {shape} fp16={fp16} sxy={sxy}", flush=True)
+    s = (flux_problem if kind == "flux" else const_problem)(shape, seed=sum(shape) + 11)
+    u0, r0 = solve(ctx, s, capi.MIXED, 1, **{"smooth.box": 0, "smooth.box_rr": 0,
+                                             "mixed.fp16": fp16})
+    u1, r1 = solve(ctx, s, capi.MIXED, 1, **{"smooth.box": 129, "smooth.box2": 129,
+                                             "smooth.box_z": 4, "smooth.box_rr": 129,
+                                             "smooth.box_small": 129, "smooth.box_sxy": sxy,
+                                             "mixed.fp16": fp16})
     print(f"  iterations {r0.iterations}/{r1.iterations}  max diff {np.abs(u0 - u1).max():.3e}", flush=True)
     assert r0.iterations == r1.iterations
     assert np.array_equal(u0, u1)
