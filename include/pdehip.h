@@ -213,7 +213,7 @@ int pdehip_residual_norms(pdehip_ctx* ctx, const pdehip_problem* prob, const dou
  * on device-resident inputs.  u_dev and every array member of *prob (f_field, kappa, face
  * samples) are DEVICE pointers; l2 and linf are host outputs.  The call allocates nothing: its
  * partial sums, flags and sampled boundary data live in a static scratch array of the library
- * (2.13 MB per device, held until the process ends).  Calls that overlap on one device share
+ * (2.16 MB per device, held until the process ends).  Calls that overlap on one device share
  * that array and race.  CONST and FLUX operators only (PDEHIP_ERR_UNSUPPORTED for POINTWISE);
  * a 2D grid side above 16385 nodes is rejected (PDEHIP_ERR_INVALID).  Results equal
  * pdehip_residual_norms on the same data. */

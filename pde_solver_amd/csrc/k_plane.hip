@@ -17,6 +17,7 @@
 
 #include "bufio.hpp"
 #include "kernels.hpp"
+#include "kernels_ext.hpp"
 
 namespace pdehip {
 namespace {
@@ -52,22 +53,31 @@ __device__ __forceinline__ void pad_partials(double* partials, int npart) {
 }
 
 // Residual replacement of a mixed solve (MODE 2): the two FP64 solution buffers, the FP32
-// correction, the scaled FP32 residual out and the outer scalars (xsel, snorm).
+// correction, the scaled FP32 residual out and the outer scalars (xsel, snorm).  Reference
+// residual norms (MODE 4): the source (field or constant), the max |r| partials and the
+// degenerate-row flag.
 struct ReplArgs {
   double* x[2];
   const float* e;
   float* r32;
   const KScalars* so;
+  const double* f;
+  double f_const;
+  double* maxs;
+  int* degenerate;
 };
 
 // MODE 0: y = A x, partials of x.y.  MODE 1: r = b - A x, partials of r.r.  MODE 2 (FP64 only):
-// MODE 1 on x = x[sel] + s e, which also goes to x[1 - sel], and r / s to r32.
+// MODE 1 on x = x[sel] + s e, which also goes to x[1 - sel], and r / s to r32.  MODE 4 (FP64
+// only): the reference residual L u + f of the interior rows with the unfolded weights
+// (k_resnorm3's rows: the same weights, the neighbour sum in its order), nothing stored,
+// partials of r.r and of max |r| (ra.maxs); a row whose |C| < 1e-12 sets *ra.degenerate.
 template <int KIND, Prec P, int MODE, int R>
 __global__ void __launch_bounds__(kThreads)
 k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, void* __restrict__ yv,
          double* partials, int npart, int tiles_x, int tiles_y, int ntiles, int kchunk,
          const KScalars* sc, ReplArgs ra) {
-  static_assert(MODE != 2 || P == P64, "fused replacement: FP64 operator only");
+  static_assert((MODE != 2 && MODE != 4) || P == P64, "replacement / norms: FP64 operator only");
   using T = typename PT<P>::T;
   using S = typename PT<P>::S;
   constexpr int TYE = R * kWaves;
@@ -82,10 +92,13 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
   __shared__ double lred[kWaves];
   if (skip(sc)) return;
   pad_partials(partials, npart);
+  if constexpr (MODE == 4) pad_partials(ra.maxs, npart);
   const int per = static_cast<int>(gridDim.x >> 3);
   const int t = static_cast<int>(blockIdx.x & 7) * per + static_cast<int>(blockIdx.x >> 3);
   if (t >= ntiles) {
     if (partials && threadIdx.x == 0) partials[blockIdx.x] = 0.0;
+    if constexpr (MODE == 4)
+      if (threadIdx.x == 0) ra.maxs[blockIdx.x] = 0.0;
     return;
   }
   const S* __restrict__ x = static_cast<const S*>(xv);
@@ -144,7 +157,8 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
     else return T(0);
   };
   auto ld_b = [&](int r, int k) -> T {
-    if constexpr (MODE >= 1) return T(ldS(b, kcl(k) + col[r]));
+    if constexpr (MODE == 4) return ra.f ? ra.f[kcl(k) + col[r]] : ra.f_const;
+    else if constexpr (MODE >= 1) return T(ldS(b, kcl(k) + col[r]));
     else return T(0);
   };
 
@@ -165,7 +179,7 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
     B[r][0] = T(0);
     B[r][1] = ld_b(r, pstart);
   }
-  double acc = 0.0;
+  double acc = 0.0, mxv = 0.0;
   for (int p = pstart; p < k1; ++p) {
     const int sl = p & 1;
 #pragma unroll
@@ -203,6 +217,7 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
         const T xc = X[r][1];
         T out;
         if (((m_bxy >> r) & 1u) || p == 0 || p == nz - 1) {
+          if constexpr (MODE == 4) continue;   // interior rows only
           out = MODE == 0 ? xc : T(B[r][0] - xc);
         } else {
           W7<T> w;
@@ -223,6 +238,16 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
             sum += w.B + w.F;
             w.C = T(op.e) - sum;
           }
+          if constexpr (MODE == 4) {
+            if (fabs(w.C) < 1e-12) { *ra.degenerate = 1; continue; }
+            double sn = w.E * lx[sl][lr][lane + 1] + w.W * lx[sl][lr][lane - 1] +
+                        w.N * lx[sl][lr + 1][lane] + w.S * lx[sl][lr - 1][lane];
+            sn = sn + w.F * X[r][2] + w.B * X[r][0];
+            const double rv = w.C * xc + sn + B[r][0];
+            acc += rv * rv;
+            mxv = nmax(mxv, fabs(rv));
+            continue;
+          }
           if (((m_fxy >> r) & 1u) || p == 1 || p == nz - 2) fold_center<3, T>(op, i, j, p, w);
           const T ax = apply_row<T>(w, xc, lx[sl][lr][lane + 1], lx[sl][lr][lane - 1],
                                     lx[sl][lr + 1][lane], lx[sl][lr - 1][lane], X[r][2], X[r][0]);
@@ -242,6 +267,11 @@ k_plane3(DevOp op, const void* __restrict__ bv, const void* __restrict__ xv, voi
   if (partials) {
     const double s = block_sum(acc, lred);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  }
+  if constexpr (MODE == 4) {
+    __syncthreads();   // lred reused
+    const double m = block_max(mxv, lred);
+    if (threadIdx.x == 0) ra.maxs[blockIdx.x] = m;
   }
 }
 
@@ -579,12 +609,40 @@ bool apply_xpay_plane(const Launch& L, const DevOp& op, Prec p, const void* z, c
   return true;
 }
 
+bool resnorm_plane(hipStream_t s, const DevOp& op, const double* u, const double* f, double f_const,
+                   double* sums, double* maxs, int* degenerate) {
+  if (op.nz < 3 || op.nx < 3 || op.ny < 3 || !plane_stencils()) return false;
+  const Shape sh = shape_of(op, plane_rows<P64, 4>());
+  const int npart = resnorm_blocks(op);
+  ReplArgs ra{};
+  ra.f = f;
+  ra.f_const = f_const;
+  ra.maxs = maxs;
+  ra.degenerate = degenerate;
+#define PDEHIP_RN_(K)                                                                           \
+  hipLaunchKernelGGL((k_plane3<K, P64, 4, plane_rows<P64, 4>()>), dim3(sh.blocks), dim3(kThreads), \
+                     0, s, op, nullptr, u, nullptr, sums, npart, sh.tiles_x, sh.tiles_y,         \
+                     sh.ntiles, sh.kchunk, nullptr, ra)
+  switch (op.kind) {
+    case PDEHIP_OP_CONST: PDEHIP_RN_(PDEHIP_OP_CONST); break;
+    case PDEHIP_OP_POINTWISE: PDEHIP_RN_(PDEHIP_OP_POINTWISE); break;
+    default: PDEHIP_RN_(PDEHIP_OP_FLUX); break;
+  }
+#undef PDEHIP_RN_
+  return true;
+}
+
 bool replace_plane(const Launch& Lgate, const DevOp& op64, const double* b, double* const x[2],
                    const float* e, double* r64, float* r32, const KScalars* so, double* partials) {
   if (op64.nz <= 1 || !plane_stencils()) return false;
   const Shape s = shape_of(op64, plane_rows<P64, 2>());
   const int npart = plane_blocks(op64);
-  const ReplArgs ra{{x[0], x[1]}, e, r32, so};
+  ReplArgs ra{};
+  ra.x[0] = x[0];
+  ra.x[1] = x[1];
+  ra.e = e;
+  ra.r32 = r32;
+  ra.so = so;
 #define PDEHIP_REPL_(K)                                                                         \
   hipLaunchKernelGGL((k_plane3<K, P64, 2, plane_rows<P64, 2>()>), dim3(s.blocks), dim3(kThreads), \
                      0, Lgate.stream, op64, b, nullptr, r64, partials, npart, s.tiles_x,         \

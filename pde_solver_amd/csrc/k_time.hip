@@ -1,5 +1,6 @@
 // k_time.hip — explicit time-step stage kernels, the reference residual-norm kernel and the
 // single-workgroup lexicographic Gauss-Seidel used by the reference-semantics V-cycle.
+#include <algorithm>
 #include <type_traits>
 
 #include "faces_dev.hpp"
@@ -668,9 +669,18 @@ void time_utt(hipStream_t s, const DevOp& op, const double* u, double* vel, doub
     hipLaunchKernelGGL((k_utt<PDEHIP_OP_CONST>), dim3(blocks), dim3(kBlock), 0, s, op, u, vel, next, f, f_const, nl, ut, utt, dt);
 }
 
+// 3D grids take the plane march (resnorm_plane) when plane stencils are on.  The partials
+// count of a 3D grid covers both kernels (the plane march zeroes the ones past its grid), so it
+// does not depend on the switch.
+static bool resnorm_3d_plane(const DevOp& op) {
+  return op.nz >= 3 && op.nx >= 3 && op.ny >= 3 && plane_stencils();
+}
+
 int resnorm_blocks(const DevOp& op) {
   long long blocks = (op.n + kBlock - 1) / kBlock;
-  return static_cast<int>(blocks > 2048 ? 2048 : blocks);
+  const int legacy = static_cast<int>(blocks > 2048 ? 2048 : blocks);
+  if (op.nz >= 3 && op.nx >= 3 && op.ny >= 3) return std::max(legacy, plane_blocks(op));
+  return legacy;
 }
 
 void residual_norms(hipStream_t s, const DevOp& op, const FaceData& fd, const double* u,
@@ -680,7 +690,9 @@ void residual_norms(hipStream_t s, const DevOp& op, const FaceData& fd, const do
   const bool d3 = op.nz > 1;
 #define RN_(D, K) hipLaunchKernelGGL((k_resnorm<D, K>), g, dim3(kBlock), 0, s, op, fd, u, f, f_const, nl, sums, maxs, degenerate)
 #define RN3_(K) hipLaunchKernelGGL((k_resnorm3<K>), g, dim3(kBlock), 0, s, op, u, f, f_const, sums, maxs, degenerate)
-  if (d3 && op.nx >= 3 && op.ny >= 3 && op.nz >= 3) {
+  if (resnorm_3d_plane(op) && resnorm_plane(s, op, u, f, f_const, sums, maxs, degenerate)) {
+    // launched
+  } else if (d3 && op.nx >= 3 && op.ny >= 3 && op.nz >= 3) {
     if (op.kind == PDEHIP_OP_POINTWISE) RN3_(PDEHIP_OP_POINTWISE);
     else if (op.kind == PDEHIP_OP_FLUX) RN3_(PDEHIP_OP_FLUX);
     else RN3_(PDEHIP_OP_CONST);

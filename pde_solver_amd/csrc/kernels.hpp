@@ -111,6 +111,12 @@ bool apply_xpay_plane(const Launch& L, const DevOp& op, Prec p, const void* z, c
 // the operator is not a 3D plane-march one.
 bool replace_plane(const Launch& Lgate, const DevOp& op64, const double* b, double* const x[2],
                    const float* e, double* r64, float* r32, const KScalars* so, double* partials);
+// Reference residual norms of a 3D FP64 level as a plane march (interior rows, L u + f with the
+// operator's unfolded weights): plane_blocks(op) partials of r.r into sums and of max |r| into
+// maxs, *degenerate set by a row whose |C| < 1e-12.  Returns false (nothing launched) when the
+// grid is not a 3D plane-march one.
+bool resnorm_plane(hipStream_t s, const DevOp& op, const double* u, const double* f, double f_const,
+                   double* sums, double* maxs, int* degenerate);
 // Partials capacity that covers either routing of this op.
 int partial_blocks_max(const DevOp& op);
 void set_plane_stencils(bool on);
