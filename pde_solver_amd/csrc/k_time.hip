@@ -415,6 +415,30 @@ __global__ void k_to_f32_scaled(const double* __restrict__ in, float* __restrict
     out[i] = static_cast<float>(in[i] * s);
 }
 
+// out32 = (float)(in64 / *norm), zero_out = 0, partials of out.out and max |out|
+__global__ void __launch_bounds__(kBlock) k_to_f32_scaled_stats(const double* __restrict__ in,
+                                                               float* __restrict__ out, long long n,
+                                                               const double* norm,
+                                                               float* __restrict__ zero_out,
+                                                               double* sums, double* maxs) {
+  __shared__ double lds[kBlock / kWave];
+  const double s = (*norm > 0.0) ? 1.0 / *norm : 1.0;
+  double acc = 0.0, m = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float v = static_cast<float>(in[i] * s);
+    out[i] = v;
+    if (zero_out) zero_out[i] = 0.0f;
+    acc += static_cast<double>(v) * static_cast<double>(v);
+    m = nmax(m, fabs(static_cast<double>(v)));
+  }
+  const double t = block_sum(acc, lds);
+  if (threadIdx.x == 0) sums[blockIdx.x] = t;
+  __syncthreads();   // lds reused
+  const double r = block_max(m, lds);
+  if (threadIdx.x == 0) maxs[blockIdx.x] = r;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_maxabs(const T* __restrict__ a, long long n,
                                                   double* partials) {
@@ -560,6 +584,12 @@ void add_f32(hipStream_t s, double* x, const float* e, long long n, const double
 void to_f32_scaled(hipStream_t s, const double* in, float* out, long long n, const double* norm) {
   hipLaunchKernelGGL(k_to_f32_scaled, dim3(misc_blocks(n)), dim3(kBlock), 0, s, in, out, n, norm,
                      static_cast<const KScalars*>(nullptr));
+}
+
+void to_f32_scaled_stats(hipStream_t s, const double* in, float* out, long long n,
+                         const double* norm, float* zero_out, double* sums, double* maxs) {
+  hipLaunchKernelGGL(k_to_f32_scaled_stats, dim3(vec_blocks(n)), dim3(kBlock), 0, s, in, out, n,
+                     norm, zero_out, sums, maxs);
 }
 
 void to_f32_scaled_gated(hipStream_t s, const double* in, float* out, long long n,

@@ -33,6 +33,11 @@ void build_rhs(hipStream_t s, const DevOp& op, const double* f, double f_const, 
                double* b);
 void add_f32(hipStream_t s, double* x, const float* e, long long n, const double* scale, double hs);
 void to_f32_scaled(hipStream_t s, const double* in, float* out, long long n, const double* norm);
+// to_f32_scaled that also clears zero_out (n floats) and leaves vec_blocks(n) partials of
+// out.out (the dot kernel's per-element products) in sums and of max |out| in maxs: the start of
+// a mixed solve in one pass.
+void to_f32_scaled_stats(hipStream_t s, const double* in, float* out, long long n,
+                         const double* norm, float* zero_out, double* sums, double* maxs);
 // Mixed CG with residual replacement (single problem).  `gate` is a scalars block whose done
 // flag is clear exactly when a replacement is due: kernels launched with it (residual, the two
 // below) run only then.

@@ -996,12 +996,10 @@ static void solve_mixed(Plan& P, pdehip_report* rep) {
   finalize(L0, so, part, partial_blocks(op64), FIN_R0, P.opt.tol, P.opt.tol_mode, nullptr);
   const double delta = P.opt.inner_tol > 0 ? P.opt.inner_tol : 1e-2;
   finalize_ru(s, si, so, gate, part, 0, FIN_RU_START, delta);
-  to_f32_scaled(s, r64, r32, n, &so->snorm);
-  fill(s, P32, x32, n, 0.0);
-  dot(Li, P32, r32, r32, n, part);
+  // r32 = r64 / snorm, x32 = 0, with the partials of r32.r32 and max |r32| in the same pass
+  to_f32_scaled_stats(s, r64, r32, n, &so->snorm, x32, part, part2);
   finalize(Li, si, part, vec_blocks(n), FIN_R0, 0.0, PDEHIP_TOL_ABS, nullptr);
-  maxabs32(s, r32, n, part2);
-  fin_scale(s, part2, maxabs_blocks(n), si);
+  fin_scale(s, part2, vec_blocks(n), si);
   // r . z comes from the V-cycle's last sweep where it can (else one dot launch)
   auto precondition = [&]() -> int {
     int np = 0;
