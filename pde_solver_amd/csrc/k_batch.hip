@@ -99,6 +99,30 @@ __device__ __forceinline__ W7<T> const_w(const DevOp& op) {
   return w;
 }
 
+// Neighbour lanes by DPP wave shifts with zero fill (one VALU op, or two for a double, instead
+// of an LDS permute): lane l gets lane l-1's value (from_west; lane 0: 0) or lane l+1's
+// (from_east; lane 63: 0).  Lanes 0 and 63 hold a strip's halo columns, which are never stored.
+__device__ __forceinline__ float from_west(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138,
+                                                            0xf, 0xf, true));
+}
+__device__ __forceinline__ float from_east(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130,
+                                                            0xf, 0xf, true));
+}
+__device__ __forceinline__ double dpp_d(double v, int ctrl_west) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = static_cast<int>(u & 0xffffffffu), hi = static_cast<int>(u >> 32);
+  const int rl = ctrl_west ? __builtin_amdgcn_mov_dpp(lo, 0x138, 0xf, 0xf, true)
+                           : __builtin_amdgcn_mov_dpp(lo, 0x130, 0xf, 0xf, true);
+  const int rh = ctrl_west ? __builtin_amdgcn_mov_dpp(hi, 0x138, 0xf, 0xf, true)
+                           : __builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, (static_cast<unsigned long long>(static_cast<unsigned>(rh)) << 32) |
+                                        static_cast<unsigned>(rl));
+}
+__device__ __forceinline__ double from_west(double v) { return dpp_d(v, 1); }
+__device__ __forceinline__ double from_east(double v) { return dpp_d(v, 0); }
+
 template <typename T, typename S> __device__ __forceinline__ T cvt(S v);
 template <> __device__ __forceinline__ double cvt<double, double>(double v) { return v; }
 template <> __device__ __forceinline__ float cvt<float, float>(float v) { return v; }
@@ -141,7 +165,7 @@ __device__ __forceinline__ void strip_march(const S* __restrict__ x, const SB* _
     ld_cols<T>(x, row(j + 1 + kAhead), c0, nx, X0[2 + kAhead], X1[2 + kAhead]);
     if constexpr (std::is_same<SB, void>::value) { B0[kAhead] = T(0); B1[kAhead] = T(0); }
     else ld_cols<T>(b, row(j + kAhead), c0, nx, B0[kAhead], B1[kAhead]);
-    const T w0 = __shfl_up(X1[1], 1, kLanes), e1 = __shfl_down(X0[1], 1, kLanes);
+    const T w0 = from_west(X1[1]), e1 = from_east(X0[1]);
     const long long id0 = (long long)j * nx + c0;
     if (out0) f(c0, j, id0, X0[1], w0, X1[1], X0[0], X0[2], B0[0]);
     if (out1) f(c0 + 1, j, id0 + 1, X1[1], X0[1], e1, X1[0], X1[2], B1[0]);
@@ -213,10 +237,10 @@ kb_sweep(const BEnt* __restrict__ tab, const BItem* __restrict__ items, int n, i
     ld_cols<T>(b, row(p + 1 + kAhead), c0, nx, B0[2 + kAhead], B1[2 + kAhead]);
     if (p >= 1 && p <= ny - 2) {
       if (((c1 + p) & 1) == 0) {   // column c0 has colour c1 on row p
-        const T uw = __shfl_up(U1[2], 1, kLanes);
+        const T uw = from_west(U1[2]);
         if (r1_0) U0[2] = upd(c0, p, B0[1], U1[2], uw, U0[3], U0[1]);
       } else {
-        const T ue = __shfl_down(U0[2], 1, kLanes);
+        const T ue = from_east(U0[2]);
         if (r1_1) U1[2] = upd(c0 + 1, p, B1[1], ue, U0[2], U1[3], U1[1]);
       }
     }
@@ -224,10 +248,10 @@ kb_sweep(const BEnt* __restrict__ tab, const BItem* __restrict__ items, int n, i
     if (q >= j0 && q < j1) {
       if (q >= 1 && q <= ny - 2) {
         if (((c2 + q) & 1) == 0) {
-          const T uw = __shfl_up(U1[1], 1, kLanes);
+          const T uw = from_west(U1[1]);
           if (r2_0) U0[1] = upd(c0, q, B0[0], U1[1], uw, U0[2], U0[0]);
         } else {
-          const T ue = __shfl_down(U0[1], 1, kLanes);
+          const T ue = from_east(U0[1]);
           if (r2_1) U1[1] = upd(c0 + 1, q, B1[0], ue, U0[1], U1[2], U1[0]);
         }
       }
