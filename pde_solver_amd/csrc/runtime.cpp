@@ -867,7 +867,9 @@ static void vcycle(Plan& P, const Launch& L, const void* b0, void* u0, double* d
   const DevOp& cop = P.mixed ? cl.op32 : cl.op64;
   void* cu = nl == 1 ? u0 : cl.u.get();
   const void* cb = nl == 1 ? b0 : cl.b.get();
-  fill(L.stream, cl.prec, cu, cl.g.n, 0.0);
+  // the dense coarsest solve writes every unknown of the level's own buffer, whose other nodes
+  // nothing else writes (zero since plan creation); the iterative one starts from zero
+  if (!(cl.m > 0 && nl > 1)) fill(L.stream, cl.prec, cu, cl.g.n, 0.0);
   if (cl.m > 0) {
     coarse_dense(L, cl.prec, cl.ainv.as<double>(), cl.map.as<int>(), cl.m, cb, cu);
   } else {
